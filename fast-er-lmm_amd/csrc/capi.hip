@@ -1,0 +1,185 @@
+// C ABI entry points: argument validation, error reporting and launch.  See include/fasterlmm_amd.h.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+
+#include "../../include/fasterlmm_amd.h"
+#include "kernels.h"
+
+
+
+using namespace flmm;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+static int check_launch(hipError_t e, const char* what) {
+  if (e != hipSuccess) return fail(FLMM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+  g_err.clear();
+  return FLMM_OK;
+}
+
+#define REQUIRE(cond, ...) \
+  do {                     \
+    if (!(cond)) return fail(FLMM_ERR_ARG, __VA_ARGS__); \
+  } while (0)
+
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+extern "C" {
+
+int flmm_abi_version(void) { return 1; }
+
+const char* flmm_last_error(void) { return g_err.c_str(); }
+
+int flmm_pack_planes(int prec, const double* src, int64_t n_rows, int64_t n_k, int64_t row_stride, int64_t k_stride,
+                     const int64_t* row_index, void* plane_hi, void* plane_lo, int64_t rows_pad, int64_t kpad,
+                     void* stream) {
+  REQUIRE(prec == FLMM_PREC_SPLIT_BF16 || prec == FLMM_PREC_FP32, "flmm_pack_planes: bad precision %d", prec);
+  REQUIRE(n_rows >= 0 && n_k >= 0 && n_rows <= rows_pad && n_k <= kpad, "flmm_pack_planes: bad sizes");
+  REQUIRE(rows_pad % TILE == 0 && kpad % KCHUNK == 0, "flmm_pack_planes: rows_pad %% 128 and kpad %% 32 required");
+  REQUIRE(plane_hi && (prec == FLMM_PREC_FP32 || plane_lo), "flmm_pack_planes: null plane");
+  REQUIRE(src || n_rows == 0 || n_k == 0, "flmm_pack_planes: null src");
+  if (rows_pad == 0 || kpad == 0) return FLMM_OK;
+  return check_launch(launch_pack_planes(prec, src, n_rows, n_k, row_stride, k_stride, row_index, plane_hi, plane_lo,
+                                         rows_pad, kpad, (hipStream_t)stream),
+                      "flmm_pack_planes");
+}
+
+int flmm_rotate(int prec, int mode, const void* a_hi, const void* a_lo, int64_t a_rows_pad, const void* b_hi,
+                const void* b_lo, int64_t b_rows_pad, int64_t n_pad, void* out0, void* out1, void* out2, void* out3,
+                void* stream) {
+  REQUIRE(prec == FLMM_PREC_SPLIT_BF16 || prec == FLMM_PREC_FP32, "flmm_rotate: bad precision %d", prec);
+  REQUIRE(mode == FLMM_ROTATE_UY || mode == FLMM_ROTATE_US, "flmm_rotate: bad mode %d", mode);
+  REQUIRE(a_rows_pad > 0 && b_rows_pad > 0 && n_pad > 0, "flmm_rotate: empty operand");
+  REQUIRE(a_rows_pad % TILE == 0 && b_rows_pad % TILE == 0 && n_pad % KCHUNK == 0,
+          "flmm_rotate: rows_pad %% 128 and n_pad %% 32 required");
+  const bool split = prec == FLMM_PREC_SPLIT_BF16;
+  REQUIRE(a_hi && b_hi && (!split || (a_lo && b_lo)), "flmm_rotate: null operand plane");
+  REQUIRE(aligned16(a_hi) && aligned16(b_hi) && (!split || (aligned16(a_lo) && aligned16(b_lo))),
+          "flmm_rotate: operand planes must be 16-byte aligned");
+  REQUIRE(out0 && (mode == FLMM_ROTATE_UY || (out1 && (!split || (out2 && out3)))), "flmm_rotate: null output");
+  hipError_t e;
+  if (split) {
+    TileArgs<PREC_SPLIT_BF16, 1> t{};
+    t.a[0][0] = a_hi; t.a[0][1] = a_lo; t.b[0][0] = b_hi; t.b[0][1] = b_lo;
+    t.a_rows_pad = a_rows_pad; t.b_rows_pad = b_rows_pad; t.nchunks = (int)(n_pad / KCHUNK);
+    e = launch_rotate<PREC_SPLIT_BF16>(mode, t, out0, out1, out2, out3, (hipStream_t)stream);
+  } else {
+    TileArgs<PREC_FP32, 1> t{};
+    t.a[0][0] = a_hi; t.b[0][0] = b_hi;
+    t.a_rows_pad = a_rows_pad; t.b_rows_pad = b_rows_pad; t.nchunks = (int)(n_pad / KCHUNK);
+    e = launch_rotate<PREC_FP32>(mode, t, out0, out1, out2, out3, (hipStream_t)stream);
+  }
+  return check_launch(e, "flmm_rotate");
+}
+
+int flmm_delta_weights(const double* s, int64_t neff, const double* s_lam, int64_t nlam, int64_t kpad, int n_grid,
+                       double lo, double hi, float* wplane, float* lam, void* stream) {
+  REQUIRE(n_grid >= 1 && n_grid <= TILE, "flmm_delta_weights: n_grid must be in [1, 128], got %d", n_grid);
+  REQUIRE(neff >= 1 && neff <= kpad && kpad % TILE == 0 && nlam >= 1, "flmm_delta_weights: bad neff / nlam / kpad");
+  REQUIRE(s && s_lam && wplane && lam, "flmm_delta_weights: null pointer");
+  return check_launch(
+      launch_delta_weights(s, neff, s_lam, nlam, kpad, n_grid, lo, hi, wplane, lam, (hipStream_t)stream),
+      "flmm_delta_weights");
+}
+
+int flmm_delta_grid(const float* wplane, const float* lam, int n_grid, double nco, const float* uy, int64_t P,
+                    int64_t p_pad, int64_t kpad, int32_t* idx, void* stream) {
+  REQUIRE(n_grid >= 1 && n_grid <= TILE, "flmm_delta_grid: n_grid must be in [1, 128]");
+  REQUIRE(P >= 0 && P <= p_pad && p_pad % TILE == 0 && kpad % TILE == 0 && kpad > 0, "flmm_delta_grid: bad sizes");
+  REQUIRE(nco > 0.0, "flmm_delta_grid: loss coefficient must be positive");
+  REQUIRE(wplane && lam && uy && idx, "flmm_delta_grid: null pointer");
+  REQUIRE(aligned16(wplane) && aligned16(uy), "flmm_delta_grid: planes must be 16-byte aligned");
+  if (p_pad == 0) return FLMM_OK;
+  return check_launch(launch_delta_grid(wplane, lam, n_grid, nco, uy, P, p_pad, kpad, idx, (hipStream_t)stream),
+                      "flmm_delta_grid");
+}
+
+int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_pad, int64_t kpad, const double* s,
+                      int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo, double hi,
+                      const int32_t* idx, double* log_delta, double* rWr, double* h2, void* q1_hi, void* q1_lo,
+                      void* q2_hi, void* q2_lo, void* stream) {
+  REQUIRE(prec == FLMM_PREC_SPLIT_BF16 || prec == FLMM_PREC_FP32, "flmm_delta_refine: bad precision %d", prec);
+  REQUIRE(mode == FLMM_DELTA_GOLDEN || mode == FLMM_DELTA_NEWTON || mode == FLMM_DELTA_FIXED,
+          "flmm_delta_refine: bad mode %d", mode);
+  REQUIRE(n_grid >= 1, "flmm_delta_refine: n_grid >= 1 required");
+  REQUIRE(P >= 0 && P <= p_pad && p_pad % TILE == 0 && kpad % TILE == 0 && neff >= 1 && neff <= kpad && nlam >= 1,
+          "flmm_delta_refine: bad sizes");
+  REQUIRE(nco > 0.0, "flmm_delta_refine: loss coefficient must be positive");
+  const bool split = prec == FLMM_PREC_SPLIT_BF16;
+  REQUIRE(uy && s && s_lam && (idx || mode == FLMM_DELTA_FIXED) && log_delta && rWr && h2 && q1_hi && q2_hi &&
+              (!split || (q1_lo && q2_lo)),
+          "flmm_delta_refine: null pointer");
+  if (p_pad == 0) return FLMM_OK;
+  return check_launch(launch_delta_refine(prec, mode, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco, n_grid, lo, hi,
+                                          idx, log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, (hipStream_t)stream),
+                      "flmm_delta_refine");
+}
+
+int flmm_wald_scan(int prec, const void* s1_hi, const void* s1_lo, const void* s2_hi, const void* s2_lo,
+                   int64_t mc_pad, const void* q1_hi, const void* q1_lo, const void* q2_hi, const void* q2_lo,
+                   int64_t p_pad, int64_t kpad, int64_t mc, int64_t P, int64_t n_real, const double* rWr,
+                   const double* h2, double nu, const int64_t* row_index, double* f, double* beta, double* se,
+                   double* sfve, double* nullh2, double* pmax, void* stream) {
+  REQUIRE(prec == FLMM_PREC_SPLIT_BF16 || prec == FLMM_PREC_FP32, "flmm_wald_scan: bad precision %d", prec);
+  REQUIRE(mc >= 0 && mc <= mc_pad && mc_pad % TILE == 0, "flmm_wald_scan: bad SNP extent");
+  REQUIRE(P >= 0 && P <= p_pad && p_pad % TILE == 0, "flmm_wald_scan: bad column extent");
+  REQUIRE(kpad > 0 && kpad % TILE == 0, "flmm_wald_scan: kpad must be a positive multiple of 128");
+  REQUIRE(n_real >= 0 && n_real <= P, "flmm_wald_scan: n_real must be in [0, P]");
+  REQUIRE(nu > 0.0, "flmm_wald_scan: dof must be positive");
+  const bool split = prec == FLMM_PREC_SPLIT_BF16;
+  REQUIRE(s1_hi && s2_hi && q1_hi && q2_hi && (!split || (s1_lo && s2_lo && q1_lo && q2_lo)),
+          "flmm_wald_scan: null operand plane");
+  REQUIRE(aligned16(s1_hi) && aligned16(s2_hi) && aligned16(q1_hi) && aligned16(q2_hi) &&
+              (!split || (aligned16(s1_lo) && aligned16(s2_lo) && aligned16(q1_lo) && aligned16(q2_lo))),
+          "flmm_wald_scan: operand planes must be 16-byte aligned");
+  REQUIRE(rWr && pmax, "flmm_wald_scan: null rWr / pmax");
+  REQUIRE(n_real == 0 || (f && beta && se && sfve && nullh2 && h2 && row_index), "flmm_wald_scan: null output");
+  if (mc == 0 || P == 0) return FLMM_OK;
+  ScanOut out{f, beta, se, sfve, nullh2, n_real};
+  hipError_t e;
+  if (split) {
+    TileArgs<PREC_SPLIT_BF16, 2> t{};
+    t.a[0][0] = s1_hi; t.a[0][1] = s1_lo; t.a[1][0] = s2_hi; t.a[1][1] = s2_lo;
+    t.b[0][0] = q1_hi; t.b[0][1] = q1_lo; t.b[1][0] = q2_hi; t.b[1][1] = q2_lo;
+    t.a_rows_pad = mc_pad; t.b_rows_pad = p_pad; t.nchunks = (int)(kpad / KCHUNK);
+    e = launch_wald_scan<PREC_SPLIT_BF16>(t, mc, P, n_real, rWr, h2, nu, row_index, out, pmax, (hipStream_t)stream);
+  } else {
+    TileArgs<PREC_FP32, 2> t{};
+    t.a[0][0] = s1_hi; t.a[1][0] = s2_hi; t.b[0][0] = q1_hi; t.b[1][0] = q2_hi;
+    t.a_rows_pad = mc_pad; t.b_rows_pad = p_pad; t.nchunks = (int)(kpad / KCHUNK);
+    e = launch_wald_scan<PREC_FP32>(t, mc, P, n_real, rWr, h2, nu, row_index, out, pmax, (hipStream_t)stream);
+  }
+  return check_launch(e, "flmm_wald_scan");
+}
+
+int flmm_standardise(int dtype, const void* src, int64_t n, int64_t m, int64_t ld_src, double* dst, int64_t ld_dst,
+                     double* mean, double* std, void* stream) {
+  REQUIRE(dtype == FLMM_DTYPE_F64 || dtype == FLMM_DTYPE_I8, "flmm_standardise: bad dtype %d", dtype);
+  REQUIRE(n >= 0 && m >= 0 && ld_src >= m && ld_dst >= m, "flmm_standardise: bad sizes");
+  REQUIRE((src && dst) || n == 0 || m == 0, "flmm_standardise: null pointer");
+  if (n == 0 || m == 0) return FLMM_OK;
+  return check_launch(launch_standardise(dtype, src, n, m, ld_src, dst, ld_dst, mean, std, (hipStream_t)stream),
+                      "flmm_standardise");
+}
+
+int flmm_max_reduce(const double* pmax, int64_t n_tiles, int64_t ld, int64_t P, double* max_F, void* stream) {
+  REQUIRE(n_tiles >= 0 && P >= 0 && ld >= P, "flmm_max_reduce: bad sizes");
+  REQUIRE((pmax || n_tiles == 0) && (max_F || P == 0), "flmm_max_reduce: null pointer");
+  if (P == 0 || n_tiles == 0) return FLMM_OK;
+  return check_launch(launch_max_reduce(pmax, n_tiles, ld, P, max_F, (hipStream_t)stream), "flmm_max_reduce");
+}
+
+}  // extern "C"

@@ -1,0 +1,310 @@
+// Per-column log-delta search for the projected-basis LMM (reference fasterlmm/core.py:525-595).
+//
+// Loss (-2 log-likelihood on the covariate-free projected basis, core.py:525-542):
+//     L(t) = n (log 2 pi + log(max(R(t), floor) / n) + 1) + Lambda(t),   delta = e^t
+//     R(t) = sum_k y_k^2 / (s_k + delta),   Lambda(t) = sum_j log(s_lam_j + delta)
+// The compat path has n = Neff and s_lam = s.  The GLS path's ML loss (core.py:148-187) is the same function with
+// n = N and s_lam = the full spectrum of K; its REML loss is the compat form up to a constant.
+//
+// Stage 1, grid: R at every grid point for every column is one GEMM, W_grid (G x K) times (UY o UY)^T, run on
+// FP32 MFMA (the squaring is fused into the LDS store of the tile engine).  Lambda at the grid points is
+// column independent and computed once.  The epilogue takes the per-column argmin (first index on ties, as
+// torch.argmin does, core.py:571).
+//
+// Stage 2, refinement, one wave per column, FP64:
+//   MODE_GOLDEN: the reference algorithm step for step -- bracket [g[i-1], g[i+1]], 50 golden-section
+//                iterations, (a+b)/2 clamped (core.py:577-595).
+//   MODE_FIXED:  no search; the caller's log delta is used as given.
+//   MODE_NEWTON: the same local minimiser in the same bracket, found by safeguarded Newton on
+//                L'(t) = delta (W1 - n R2 / R), W1 = sum 1/(s_lam+delta), R2 = sum y^2 w^2, w = 1/(s+delta):
+//                no logarithms per iteration and quadratic convergence; bisection keeps the iterate inside
+//                the bracket.  A bracket whose left slope is >= 0 (right slope <= 0) has its minimum at that
+//                edge, which is where golden section converges too.
+// The refinement then writes the scan's column-side operand planes Q1 = w o y and Q2 = w for the final delta,
+// the null residual rWr = max(sum w y^2, floor) (core.py:641-643) and nullh2 = 1 / (1 + delta).
+#include "kernels.h"
+
+namespace flmm {
+
+enum DeltaMode : int { MODE_GOLDEN = 0, MODE_NEWTON = 1, MODE_FIXED = 2 };
+
+__device__ inline double grid_value(int i, int G, double lo, double hi) {
+  if (G <= 1) return lo;
+  if (i == G - 1) return hi;
+  return (double)i * ((hi - lo) / (double)(G - 1)) + lo;
+}
+
+// One block per grid row g (128 blocks): W plane row g (fp32, zero beyond Neff and for g >= G) and
+// lam[g] = sum_k log(s_lam_k + delta_g) over the log-det spectrum.
+__global__ void delta_weights_kernel(const double* __restrict__ s, int64_t neff, const double* __restrict__ s_lam,
+                                     int64_t nlam, int64_t kpad, int G, double lo, double hi,
+                                     float* __restrict__ wplane, float* __restrict__ lam) {
+  __shared__ double red[256];
+  const int g = blockIdx.x;
+  const double delta = exp(grid_value(g, G, lo, hi));
+  double acc = 0.0;
+  for (int64_t k = threadIdx.x; k < kpad; k += blockDim.x) {
+    float w = 0.f;
+    if (g < G && k < neff) w = (float)(1.0 / (s[k] + delta));
+    wplane[((k >> 5) * TILE + g) * 32 + (k & 31)] = w;
+  }
+  if (g < G)
+    for (int64_t k = threadIdx.x; k < nlam; k += blockDim.x) acc += log(s_lam[k] + delta);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) lam[g] = (float)red[0];
+}
+
+// argmin over the grid rows of one 128-column tile
+__global__ __launch_bounds__(NTHREADS, 1) void delta_grid_kernel(TileArgs<PREC_FP32, 1> args, int64_t n_tiles,
+                                                                  const float* __restrict__ lam, int G, float nco,
+                                                                  int64_t P, int* __restrict__ idx_out) {
+  extern __shared__ char lds[];
+  const int64_t pt = xcd_remap(blockIdx.x, n_tiles);
+  const int64_t p0 = pt * TILE;
+  TileAcc<PREC_FP32, 1> acc;
+  tile_mainloop<PREC_FP32, 1, true>(args, 0, p0, acc, lds);
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  __syncthreads();
+  float* xl = (float*)lds;
+  int* xi = (int*)(xl + 128);
+#pragma unroll
+  for (int bj = 0; bj < 2; ++bj) {
+    float best = INFINITY;
+    int bidx = 0x7fffffff;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int g = acc_row(wm, bi, lane, v);
+        if (g < G) {
+          const float loss = nco * __logf(fmaxf(acc.c[0][bi][bj][v], 1e-30f)) + lam[g];
+          if (loss < best || (loss == best && g < bidx)) { best = loss; bidx = g; }
+        }
+      }
+    {
+      const float ob = __shfl_xor(best, 32, 64);
+      const int oi = __shfl_xor(bidx, 32, 64);
+      if (ob < best || (ob == best && oi < bidx)) { best = ob; bidx = oi; }
+    }
+    if (wm == 1 && lane < 32) { xl[(wn * 2 + bj) * 32 + lane] = best; xi[(wn * 2 + bj) * 32 + lane] = bidx; }
+    __syncthreads();
+    if (wm == 0 && lane < 32) {
+      const float ob = xl[(wn * 2 + bj) * 32 + lane];
+      const int oi = xi[(wn * 2 + bj) * 32 + lane];
+      if (ob < best || (ob == best && oi < bidx)) { best = ob; bidx = oi; }
+      const int64_t p = p0 + acc_col(wn, bj, lane);
+      if (p < P) idx_out[p] = (bidx == 0x7fffffff) ? 0 : bidx;
+    }
+    __syncthreads();
+  }
+}
+
+struct ColSums {
+  double R, R2, R3, W1, W2, LS;
+};
+
+// The likelihood of one column: y (neff values in the weight spectrum s) and the log-det spectrum s_lam (nlam
+// values; the same as s on the projected basis, the full spectrum for the ML loss of the GLS path).
+struct ColLik {
+  const float* uy;
+  int64_t p_pad;
+  const double* s;
+  int64_t neff;
+  const double* s_lam;
+  int64_t nlam;
+  double nco;  // coefficient of log R in the loss
+};
+
+// sums over k of one column at delta; with_log adds Lambda = sum log(s_lam + delta)
+__device__ inline ColSums col_sums(const ColLik& L, int64_t p, double delta, bool with_log, int lane) {
+  ColSums c{0, 0, 0, 0, 0, 0};
+  for (int64_t k = lane; k < L.neff; k += 64) {
+    const double y = (double)L.uy[((k >> 5) * L.p_pad + p) * 32 + (k & 31)];
+    const double w = 1.0 / (L.s[k] + delta);
+    const double yw = y * y * w;
+    c.R += yw;
+    c.R2 += yw * w;
+    c.R3 += yw * w * w;
+  }
+  for (int64_t k = lane; k < L.nlam; k += 64) {
+    const double sd = L.s_lam[k] + delta;
+    const double w = 1.0 / sd;
+    c.W1 += w;
+    c.W2 += w * w;
+    if (with_log) c.LS += log(sd);
+  }
+  c.R = wave_sum(c.R);
+  c.R2 = wave_sum(c.R2);
+  c.R3 = wave_sum(c.R3);
+  c.W1 = wave_sum(c.W1);
+  c.W2 = wave_sum(c.W2);
+  if (with_log) c.LS = wave_sum(c.LS);
+  return c;
+}
+
+constexpr double LOG2PI = 1.8378770664093453;
+
+__device__ inline double col_loss(const ColLik& L, int64_t p, double t, int lane) {
+  const ColSums c = col_sums(L, p, exp(t), true, lane);
+  return L.nco * (LOG2PI + log(fmax(c.R, FLOOR64) / L.nco) + 1.0) + c.LS;
+}
+
+// slope L'(t) and curvature L''(t)
+__device__ inline void col_slope(const ColLik& L, int64_t p, double t, int lane, double& g, double& gp) {
+  const double delta = exp(t);
+  const ColSums c = col_sums(L, p, delta, false, lane);
+  const double n = L.nco;
+  if (c.R <= FLOOR64) {  // all-zero column: R is floored, only Lambda moves
+    g = delta * c.W1;
+    gp = g - delta * delta * c.W2;
+    return;
+  }
+  const double h = c.W1 - n * c.R2 / c.R;
+  const double hp = -c.W2 + n * (2.0 * c.R3 * c.R - c.R2 * c.R2) / (c.R * c.R);
+  g = delta * h;
+  gp = delta * h + delta * delta * hp;
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void delta_refine_kernel(
+    int mode, const float* __restrict__ uy, int64_t P, int64_t p_pad, int64_t kpad, const double* __restrict__ s,
+    int64_t neff, const double* __restrict__ s_lam, int64_t nlam, double nco, int G, double lo, double hi,
+    const int* __restrict__ idx, double* __restrict__ log_delta,
+    double* __restrict__ rWr, double* __restrict__ h2, void* q1a, void* q1b, void* q2a, void* q2b) {
+  const int lane = threadIdx.x & 63;
+  const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (p >= p_pad) return;
+  if (p >= P) {  // padding columns: zero operand rows
+    for (int64_t k = lane; k < kpad; k += 64) {
+      const int64_t off = ((k >> 5) * p_pad + p) * 32 + (k & 31);
+      if constexpr (PREC == PREC_SPLIT_BF16) {
+        ((u16*)q1a)[off] = 0; ((u16*)q1b)[off] = 0; ((u16*)q2a)[off] = 0; ((u16*)q2b)[off] = 0;
+      } else {
+        ((float*)q1a)[off] = 0.f; ((float*)q2a)[off] = 0.f;
+      }
+    }
+    return;
+  }
+  const ColLik L{uy, p_pad, s, neff, s_lam, nlam, nco};
+  double t;
+  const int i = mode == MODE_FIXED ? 0 : idx[p];
+  const double a0 = grid_value(i > 0 ? i - 1 : 0, G, lo, hi);
+  const double b0 = grid_value(i + 1 < G ? i + 1 : G - 1, G, lo, hi);
+  if (mode == MODE_FIXED) {
+    t = log_delta[p];  // the caller's log delta, used as given (core.py:639)
+  } else if (mode == MODE_GOLDEN) {
+    const double inv_phi = 2.0 / (1.0 + sqrt(5.0));
+    const double inv_phi2 = inv_phi * inv_phi;
+    double a = a0, b = b0;
+    double c = a + inv_phi2 * (b - a), d = a + inv_phi * (b - a);
+    double fc = col_loss(L, p, c, lane), fd = col_loss(L, p, d, lane);
+    for (int it = 0; it < 50; ++it) {
+      if (fc < fd) b = d; else a = c;
+      c = a + inv_phi2 * (b - a);
+      d = a + inv_phi * (b - a);
+      fc = col_loss(L, p, c, lane);
+      fd = col_loss(L, p, d, lane);
+    }
+    t = 0.5 * (a + b);
+  } else {
+    double a = a0, b = b0, ga, gpa, gb, gpb;
+    col_slope(L, p, a, lane, ga, gpa);
+    col_slope(L, p, b, lane, gb, gpb);
+    if (!(b > a) || ga >= 0.0) {
+      t = a;
+    } else if (gb <= 0.0) {
+      t = b;
+    } else {
+      t = grid_value(i, G, lo, hi);
+      if (!(t > a && t < b)) t = 0.5 * (a + b);
+      for (int it = 0; it < 100; ++it) {
+        double g, gp;
+        col_slope(L, p, t, lane, g, gp);
+        if (g == 0.0) break;
+        if (g < 0.0) a = t; else b = t;
+        double tn = t - g / gp;
+        if (!(gp > 0.0) || !(tn > a && tn < b)) tn = 0.5 * (a + b);
+        const bool done = fabs(tn - t) <= 1e-13 * fmax(1.0, fabs(t)) || (b - a) <= 1e-13;
+        t = tn;
+        if (done) break;
+      }
+    }
+  }
+  if (mode != MODE_FIXED) t = fmin(fmax(t, lo), hi);
+  const double delta = exp(t);
+  double rw = 0.0;
+  for (int64_t k = lane; k < kpad; k += 64) {
+    double w = 0.0, q1 = 0.0;
+    if (k < neff) {
+      const double y = (double)uy[((k >> 5) * p_pad + p) * 32 + (k & 31)];
+      w = 1.0 / (s[k] + delta);
+      q1 = w * y;
+      rw += q1 * y;
+    }
+    const int64_t off = ((k >> 5) * p_pad + p) * 32 + (k & 31);
+    if constexpr (PREC == PREC_SPLIT_BF16) {
+      u16 hh, ll;
+      split_bf16(q1, hh, ll);
+      ((u16*)q1a)[off] = hh;
+      ((u16*)q1b)[off] = ll;
+      split_bf16(w, hh, ll);
+      ((u16*)q2a)[off] = hh;
+      ((u16*)q2b)[off] = ll;
+    } else {
+      ((float*)q1a)[off] = (float)q1;
+      ((float*)q2a)[off] = (float)w;
+    }
+  }
+  rw = wave_sum(rw);
+  if (lane == 0) {
+    log_delta[p] = t;
+    rWr[p] = fmax(rw, FLOOR64);
+    h2[p] = 1.0 / (1.0 + delta);
+  }
+}
+
+hipError_t launch_delta_weights(const double* s, int64_t neff, const double* s_lam, int64_t nlam, int64_t kpad, int G,
+                                double lo, double hi, float* wplane, float* lam, hipStream_t stream) {
+  hipLaunchKernelGGL(delta_weights_kernel, dim3(TILE), dim3(256), 0, stream, s, neff, s_lam, nlam, kpad, G, lo, hi,
+                     wplane, lam);
+  return hipGetLastError();
+}
+
+hipError_t launch_delta_grid(const float* wplane, const float* lam, int G, double nco, const float* uy, int64_t P,
+                             int64_t p_pad, int64_t kpad, int* idx, hipStream_t stream) {
+  TileArgs<PREC_FP32, 1> args{};
+  args.a[0][0] = wplane;
+  args.b[0][0] = uy;
+  args.a_rows_pad = TILE;
+  args.b_rows_pad = p_pad;
+  args.nchunks = (int)(kpad / KCHUNK);
+  const int64_t n_tiles = p_pad / TILE;
+  constexpr int lds = tile_lds_bytes<PREC_FP32, 1>();
+  hipLaunchKernelGGL(delta_grid_kernel, dim3((unsigned)n_tiles), dim3(NTHREADS), lds, stream, args, n_tiles, lam, G,
+                     (float)nco, P, idx);
+  return hipGetLastError();
+}
+
+hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_pad, int64_t kpad,
+                               const double* s, int64_t neff, const double* s_lam, int64_t nlam, double nco, int G,
+                               double lo, double hi, const int* idx,
+                               double* log_delta, double* rWr, double* h2, void* q1a, void* q1b, void* q2a,
+                               void* q2b, hipStream_t stream) {
+  const dim3 grid((unsigned)((p_pad + 3) / 4));
+  if (prec == PREC_SPLIT_BF16)
+    hipLaunchKernelGGL(delta_refine_kernel<PREC_SPLIT_BF16>, grid, dim3(256), 0, stream, mode, uy, P, p_pad, kpad, s,
+                       neff, s_lam, nlam, nco, G, lo, hi, idx, log_delta, rWr, h2, q1a, q1b, q2a, q2b);
+  else
+    hipLaunchKernelGGL(delta_refine_kernel<PREC_FP32>, grid, dim3(256), 0, stream, mode, uy, P, p_pad, kpad, s, neff,
+                       s_lam, nlam, nco, G, lo, hi, idx, log_delta, rWr, h2, q1a, q1b, q2a, q2b);
+  return hipGetLastError();
+}
+
+}  // namespace flmm

@@ -1,0 +1,71 @@
+// Shared device helpers for the gfx950 LMM-scan kernels.
+//
+// Operand planes.  Every GEMM operand that feeds an MFMA tile is stored as a "plane": a rows x K matrix laid
+// out in K-chunks of 32, chunk-major: element (row, k) lives at plane[((k / 32) * rows_pad + row) * 32 + k % 32].
+// One chunk of 128 rows is then a single contiguous 8 KiB (bf16) or 16 KiB (fp32) block, so a workgroup's tile
+// load is a straight coalesced copy, and rows_pad / K are padded with zeros (a zero contributes nothing to a
+// sum over K).  Split-BF16 operands carry two planes, hi = bf16(x) and lo = bf16(x - hi), so hi + lo holds ~16
+// significant bits of x; a product is recovered as hi*hi + hi*lo + lo*hi with FP32 accumulation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace flmm {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+constexpr int KCHUNK = 32;  // K extent of one plane chunk
+constexpr double FLOOR64 = 1e-300;  // positive clamp floor of the FP64 reference arithmetic (core.py:80-86)
+
+enum Prec : int { PREC_SPLIT_BF16 = 0, PREC_FP32 = 1 };
+
+__host__ __device__ inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// plane element offset (in elements) of (row, k)
+__device__ inline int64_t plane_off(int64_t row, int64_t k, int64_t rows_pad) {
+  return ((k >> 5) * rows_pad + row) * 32 + (k & 31);
+}
+
+// round-to-nearest-even float -> bf16 bits (inputs here are finite)
+__device__ inline u16 f32_to_bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (u16)(u >> 16);
+}
+__device__ inline float bf16_to_f32(u16 h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+// FP64 value -> (hi, lo) bf16 pair with x ~= hi + lo
+__device__ inline void split_bf16(double x, u16& hi, u16& lo) {
+  hi = f32_to_bf16_rne((float)x);
+  double r = x - (double)bf16_to_f32(hi);
+  lo = f32_to_bf16_rne((float)r);
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// XCD-aware block remap: the hardware hands consecutive workgroup ids to the 8 XCDs round-robin.  Renumber so
+// that XCD x runs a contiguous range of logical tiles, which keeps the tiles that share operands on one L2.
+__device__ inline int64_t xcd_remap(int64_t bid, int64_t nblocks) {
+  constexpr int NXCD = 8;
+  int64_t per = nblocks / NXCD;
+  int64_t rem = nblocks % NXCD;
+  int64_t xcd = bid % NXCD;
+  int64_t slot = bid / NXCD;
+  // the first `rem` XCDs own per+1 logical tiles, the others own per
+  int64_t start = xcd * per + (xcd < rem ? xcd : rem);
+  return start + slot;
+}
+
+}  // namespace flmm

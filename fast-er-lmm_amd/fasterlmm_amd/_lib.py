@@ -1,0 +1,91 @@
+"""ctypes binding of ``libfasterlmm_amd.so`` (declared in include/fasterlmm_amd.h).
+
+The library is the only compute path: if it is missing, or no ROCm GPU is visible, every entry point raises
+instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / "libfasterlmm_amd.so"
+
+PREC_SPLIT_BF16 = 0
+PREC_FP32 = 1
+ROTATE_UY = 0
+ROTATE_US = 1
+DELTA_GOLDEN = 0
+DELTA_NEWTON = 1
+DELTA_FIXED = 2
+
+_P = C.c_void_p
+_I64 = C.c_int64
+_I = C.c_int
+_D = C.c_double
+
+# name -> argtypes, in header order
+SIGNATURES: dict[str, list] = {
+    "flmm_abi_version": [],
+    "flmm_last_error": [],
+    "flmm_pack_planes": [_I, _P, _I64, _I64, _I64, _I64, _P, _P, _P, _I64, _I64, _P],
+    "flmm_rotate": [_I, _I, _P, _P, _I64, _P, _P, _I64, _I64, _P, _P, _P, _P, _P],
+    "flmm_delta_weights": [_P, _I64, _P, _I64, _I64, _I, _D, _D, _P, _P, _P],
+    "flmm_delta_grid": [_P, _P, _I, _D, _P, _I64, _I64, _I64, _P, _P],
+    "flmm_delta_refine": [_I, _I, _P, _I64, _I64, _I64, _P, _I64, _P, _I64, _D, _I, _D, _D, _P, _P, _P, _P, _P,
+                          _P, _P, _P, _P],
+    "flmm_wald_scan": [_I, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P, _P, _D, _P, _P,
+                       _P, _P, _P, _P, _P, _P],
+    "flmm_standardise": [_I, _P, _I64, _I64, _I64, _P, _I64, _P, _P, _P],
+    "flmm_max_reduce": [_P, _I64, _I64, _I64, _P, _P],
+}
+
+_lib: C.CDLL | None = None
+
+
+def load() -> C.CDLL:
+    """Open the shared library (once) and type its entry points."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python -m fasterlmm_amd.build` (hipcc, gfx950)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, argtypes in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = C.c_char_p if name == "flmm_last_error" else C.c_int
+    _lib = lib
+    return lib
+
+
+class FlmmError(RuntimeError):
+    pass
+
+
+def call(name: str, *args) -> None:
+    """Call an entry point; a non-zero status raises ValueError (bad argument) or FlmmError (HIP failure)."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        msg = lib.flmm_last_error().decode()
+        if rc == 1:
+            raise ValueError(msg)
+        raise FlmmError(msg)
+
+
+def device() -> torch.device:
+    """The GPU every kernel runs on; raises when there is none (no CPU fallback exists)."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("fasterlmm_amd needs a ROCm GPU (MI355X / gfx950); none is visible")
+    load()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()

@@ -1,0 +1,270 @@
+"""GPU pipeline of the LMM scan: operand planes, rotations, delta search and the fused Wald scan.
+
+Every array here is a torch tensor on the GPU used as device memory; all arithmetic of the hot path runs in the
+hand-written kernels of libfasterlmm_amd.so (called through ``_lib``).  torch is used for allocation, for the
+small FP64 setup products (kinship Gram, covariate projector, N x N projection) that rocBLAS runs, and for
+rocSOLVER's FP64 symmetric eigendecomposition.
+
+Data layout (HBM)
+  genotypes     Z (N, M) FP64 row-major, standardised (the reference's input layout)
+  planes        operand matrices in K-chunks of 32 (see include/fasterlmm_amd.h): Y_r columns (P_pad x N_pad),
+                U_eff^T (K_pad x N_pad), rotated columns UY (fp32, P_pad x K_pad), SNP planes S1 = US, S2 = US^2
+                (Mc_pad x K_pad) and column planes Q1 = w o UY, Q2 = w (P_pad x K_pad)
+  outputs       f, beta, se, sfve, nullh2 (M, n_real) FP64 row-major; max_F (P,) FP64
+with N_pad = roundup(N, 32), K_pad = roundup(Neff, 128), P_pad = roundup(P, 128), Mc_pad = roundup(Mc, 128).
+"""
+from __future__ import annotations
+
+import contextlib
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+
+TILE = 128
+KCHUNK = 32
+
+
+def rup(x: int, m: int) -> int:
+    return (int(x) + m - 1) // m * m
+
+
+@dataclass
+class Settings:
+    precision: str = "split_bf16"  # "split_bf16" (3-product split-BF16 MFMA) or "fp32" (FP32 MFMA)
+    delta: str = "newton"          # "newton" (exact local minimiser) or "golden" (the reference's 50 steps)
+
+
+settings = Settings()
+
+
+@contextlib.contextmanager
+def configure(**kw):
+    """Temporarily change ``settings`` (precision=..., delta=...)."""
+    old = {k: getattr(settings, k) for k in kw}
+    for k, v in kw.items():
+        if not hasattr(settings, k):
+            raise ValueError(f"unknown setting {k!r}")
+        setattr(settings, k, v)
+    try:
+        yield settings
+    finally:
+        for k, v in old.items():
+            setattr(settings, k, v)
+
+
+def prec_code(precision: str | None = None) -> int:
+    p = settings.precision if precision is None else precision
+    if p == "split_bf16":
+        return _lib.PREC_SPLIT_BF16
+    if p == "fp32":
+        return _lib.PREC_FP32
+    raise ValueError(f"unknown precision {p!r}")
+
+
+def delta_code(mode: str | None = None) -> int:
+    m = settings.delta if mode is None else mode
+    if m == "newton":
+        return _lib.DELTA_NEWTON
+    if m == "golden":
+        return _lib.DELTA_GOLDEN
+    raise ValueError(f"unknown delta mode {m!r}")
+
+
+class Planes:
+    """One operand matrix as MFMA planes: (hi, lo) bf16 pair or a single fp32 plane."""
+
+    def __init__(self, prec: int, rows_pad: int, kpad: int, dev, *, fp32_only: bool = False):
+        self.prec = _lib.PREC_FP32 if fp32_only else prec
+        self.rows_pad, self.kpad = rows_pad, kpad
+        n = rows_pad * kpad
+        if self.prec == _lib.PREC_SPLIT_BF16:
+            self.hi = torch.empty(n, dtype=torch.int16, device=dev)
+            self.lo = torch.empty(n, dtype=torch.int16, device=dev)
+        else:
+            self.hi = torch.empty(n, dtype=torch.float32, device=dev)
+            self.lo = None
+
+    @property
+    def ptrs(self):
+        return self.hi.data_ptr(), (self.lo.data_ptr() if self.lo is not None else None)
+
+    def to_dense(self, rows: int, k: int) -> torch.Tensor:
+        """(rows, k) FP64 view of the stored values (hi + lo for split planes)."""
+        def one(t, dt):
+            v = t.view(dt) if dt is not None else t
+            return v.reshape(self.kpad // KCHUNK, self.rows_pad, KCHUNK).permute(1, 0, 2).reshape(
+                self.rows_pad, self.kpad)[:rows, :k]
+        if self.prec == _lib.PREC_FP32:
+            return one(self.hi, None).double()
+        return one(self.hi, torch.bfloat16).double() + one(self.lo, torch.bfloat16).double()
+
+
+def fp32_plane_to_dense(plane: torch.Tensor, rows_pad: int, kpad: int, rows: int, k: int) -> torch.Tensor:
+    return plane.reshape(kpad // KCHUNK, rows_pad, KCHUNK).permute(1, 0, 2).reshape(rows_pad, kpad)[:rows, :k]
+
+
+def pack(prec: int, src: torch.Tensor, n_rows: int, n_k: int, row_stride: int, k_stride: int, rows_pad: int,
+         kpad: int, *, row_index: torch.Tensor | None = None, offset: int = 0) -> Planes:
+    """Planes of element (r, k) = src.flat[offset + src_row(r) * row_stride + k * k_stride]."""
+    assert src.dtype == torch.float64 and src.is_cuda
+    pl = Planes(prec, rows_pad, kpad, src.device)
+    base = src.data_ptr() + 8 * offset
+    hi, lo = pl.ptrs
+    _lib.call("flmm_pack_planes", pl.prec, base, n_rows, n_k, row_stride, k_stride, _lib.ptr(row_index), hi, lo,
+              rows_pad, kpad, _lib.stream())
+    return pl
+
+
+def rotate_uy(a: Planes, b: Planes) -> torch.Tensor:
+    """fp32 plane (a.rows_pad x b.rows_pad) of out[i][j] = sum_n a[i][n] b[j][n]."""
+    out = torch.empty(a.rows_pad * b.rows_pad, dtype=torch.float32, device=a.hi.device)
+    ah, al = a.ptrs
+    bh, bl = b.ptrs
+    _lib.call("flmm_rotate", a.prec, _lib.ROTATE_UY, ah, al, a.rows_pad, bh, bl, b.rows_pad, a.kpad, out.data_ptr(),
+              None, None, None, _lib.stream())
+    return out
+
+
+def rotate_us(a: Planes, b: Planes) -> tuple[Planes, Planes]:
+    """SNP-side scan planes S1 = x, S2 = x^2 of x[i][j] = sum_n a[i][n] b[j][n]."""
+    dev = a.hi.device
+    s1 = Planes(a.prec, a.rows_pad, b.rows_pad, dev)
+    s2 = Planes(a.prec, a.rows_pad, b.rows_pad, dev)
+    ah, al = a.ptrs
+    bh, bl = b.ptrs
+    if a.prec == _lib.PREC_SPLIT_BF16:
+        outs = (s1.hi.data_ptr(), s1.lo.data_ptr(), s2.hi.data_ptr(), s2.lo.data_ptr())
+    else:
+        outs = (s1.hi.data_ptr(), s2.hi.data_ptr(), None, None)
+    _lib.call("flmm_rotate", a.prec, _lib.ROTATE_US, ah, al, a.rows_pad, bh, bl, b.rows_pad, a.kpad, *outs,
+              _lib.stream())
+    return s1, s2
+
+
+def grid_values(n_grid: int, lo: float, hi: float) -> np.ndarray:
+    """The log-delta grid exactly as the kernels form it: i * step + lo, last point = hi."""
+    if n_grid <= 1:
+        return np.array([lo], dtype=np.float64)[:n_grid]
+    g = np.arange(n_grid, dtype=np.float64) * ((hi - lo) / (n_grid - 1)) + lo
+    g[-1] = hi
+    return g
+
+
+@dataclass
+class DeltaResult:
+    log_delta: torch.Tensor  # (P,) FP64
+    rWr: torch.Tensor        # (P,) FP64
+    h2: torch.Tensor         # (P,) FP64
+    q1: Planes
+    q2: Planes
+    idx: torch.Tensor | None = None
+    neff: int = 0
+
+
+def delta_grid_idx(uy: torch.Tensor, P: int, p_pad: int, kpad: int, s: torch.Tensor, s_lam: torch.Tensor,
+                   nco: float, n_grid: int, lo: float, hi: float) -> torch.Tensor:
+    """Per-column grid argmin (int32, (P,))."""
+    if n_grid > TILE:
+        raise ValueError(f"n_grid <= {TILE} is supported, got {n_grid}")
+    dev = uy.device
+    wplane = torch.empty(TILE * kpad, dtype=torch.float32, device=dev)
+    lam = torch.empty(TILE, dtype=torch.float32, device=dev)
+    idx = torch.empty(max(P, 1), dtype=torch.int32, device=dev)
+    st = _lib.stream()
+    _lib.call("flmm_delta_weights", s.data_ptr(), s.numel(), s_lam.data_ptr(), s_lam.numel(), kpad, n_grid, lo, hi,
+              wplane.data_ptr(), lam.data_ptr(), st)
+    _lib.call("flmm_delta_grid", wplane.data_ptr(), lam.data_ptr(), n_grid, float(nco), uy.data_ptr(), P, p_pad, kpad,
+              idx.data_ptr(), st)
+    return idx[:P]
+
+
+def delta_refine(prec: int, mode: int, uy: torch.Tensor, P: int, p_pad: int, kpad: int, s: torch.Tensor,
+                 s_lam: torch.Tensor, nco: float, n_grid: int, lo: float, hi: float, idx: torch.Tensor | None,
+                 log_delta_in: torch.Tensor | None = None) -> DeltaResult:
+    """Refine (or, for DELTA_FIXED, take log_delta_in) and build the column-side scan planes."""
+    dev = uy.device
+    log_delta = torch.empty(max(P, 1), dtype=torch.float64, device=dev)
+    if log_delta_in is not None:
+        log_delta[:P].copy_(log_delta_in.to(device=dev, dtype=torch.float64))
+    rWr = torch.empty(max(P, 1), dtype=torch.float64, device=dev)
+    h2 = torch.empty(max(P, 1), dtype=torch.float64, device=dev)
+    q1 = Planes(prec, p_pad, kpad, dev)
+    q2 = Planes(prec, p_pad, kpad, dev)
+    q1h, q1l = q1.ptrs
+    q2h, q2l = q2.ptrs
+    _lib.call("flmm_delta_refine", prec, mode, uy.data_ptr(), P, p_pad, kpad, s.data_ptr(), s.numel(),
+              s_lam.data_ptr(), s_lam.numel(), float(nco), n_grid, lo, hi, _lib.ptr(idx), log_delta.data_ptr(),
+              rWr.data_ptr(), h2.data_ptr(), q1h, q1l, q2h, q2l, _lib.stream())
+    return DeltaResult(log_delta[:P], rWr[:P], h2[:P], q1, q2, idx, int(s.numel()))
+
+
+class _ScanTimer:
+    """Optional HIP-event timing of every flmm_wald_scan launch (events on the launch stream)."""
+
+    def __init__(self):
+        self.enabled = False
+        self.launches: list = []
+
+    def reset(self, enabled: bool = True):
+        self.enabled = enabled
+        self.launches = []
+
+    def summary(self) -> dict:
+        """Launch count, total / mean kernel ms, Wald tests and algorithmic compat flops (4 Neff per test)."""
+        torch.cuda.synchronize()
+        ms = [a.elapsed_time(b) for a, b, *_ in self.launches]
+        tests = sum(t for _, _, t, _, _ in self.launches)
+        flops = sum(4.0 * neff * t for _, _, t, _, neff in self.launches)
+        return {"launches": len(ms), "total_ms": float(sum(ms)), "mean_ms": float(np.mean(ms)) if ms else 0.0,
+                "tests": int(tests), "alg_flops": float(flops)}
+
+
+scan_timer = _ScanTimer()
+
+
+@dataclass
+class ScanOutputs:
+    f: torch.Tensor
+    beta: torch.Tensor
+    se: torch.Tensor
+    sfve: torch.Tensor
+    nullh2: torch.Tensor
+    max_F: torch.Tensor
+
+    @staticmethod
+    def allocate(M: int, n_real: int, P: int, dev) -> "ScanOutputs":
+        z = lambda: torch.zeros(M, n_real, dtype=torch.float64, device=dev)  # noqa: E731
+        return ScanOutputs(z(), z(), z(), z(), z(), torch.full((P,), float("-inf"), dtype=torch.float64, device=dev))
+
+
+def wald_scan(s1: Planes, s2: Planes, d: DeltaResult, mc: int, P: int, n_real: int, nu: float,
+              row_index: torch.Tensor, out: ScanOutputs, pmax: torch.Tensor | None = None) -> None:
+    """Scan one SNP block against every column, update out (real-column rows and the running max_F)."""
+    if mc == 0 or P == 0:
+        return
+    mc_pad = s1.rows_pad
+    p_pad = d.q1.rows_pad
+    n_mt = (mc + TILE - 1) // TILE
+    if pmax is None or pmax.numel() < n_mt * p_pad:
+        pmax = torch.empty(n_mt * p_pad, dtype=torch.float64, device=s1.hi.device)
+    s1h, s1l = s1.ptrs
+    s2h, s2l = s2.ptrs
+    q1h, q1l = d.q1.ptrs
+    q2h, q2l = d.q2.ptrs
+    real = n_real > 0
+    o = (out.f, out.beta, out.se, out.sfve, out.nullh2)
+    optr = [t.data_ptr() if real else None for t in o]
+    st = _lib.stream()
+    ev = None
+    if scan_timer.enabled:
+        ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        ev[0].record()
+    _lib.call("flmm_wald_scan", s1.prec, s1h, s1l, s2h, s2l, mc_pad, q1h, q1l, q2h, q2l, p_pad, s1.kpad, mc, P, n_real,
+              d.rWr.data_ptr(), d.h2.data_ptr(), float(nu), row_index.data_ptr(), *optr, pmax.data_ptr(), st)
+    if ev is not None:
+        ev[1].record()
+        scan_timer.launches.append((ev[0], ev[1], mc * P, s1.kpad, d.neff))
+    _lib.call("flmm_max_reduce", pmax.data_ptr(), n_mt, p_pad, P, out.max_F.data_ptr(), st)

@@ -1,0 +1,90 @@
+"""Seeded synthetic GWAS panels for the scan's workloads.
+
+Genotypes follow the Balding-Nichols model: ancestral frequency p ~ U(maf_lo, maf_hi) per SNP, subpopulation
+frequency ~ Beta(p (1 - F) / F, (1 - p)(1 - F) / F) with F = Fst, diploid dosages ~ Binomial(2, p_k), stored as
+int8 0/1/2.  SNPs are split evenly over chromosomes 1..n_chrom (sorted).  Each phenotype has heritability
+h2 ~ U(h2_lo, h2_hi) from n_causal SNPs with N(0, 1) effects on standardised genotypes, plus Gaussian noise;
+optionally rank-inverse-normal transformed with Blom's c = 3/8 and average ties (the reference's RINT,
+fasterlmm/normalize.py:21-52).  Covariates are an intercept plus N(0, 1) columns.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+from scipy.stats import norm, rankdata
+
+
+@dataclass
+class Panel:
+    G: np.ndarray        # (N, M) int8 dosages
+    chrom: np.ndarray    # (M,) int chromosome labels, sorted
+    X: np.ndarray        # (N, C) covariates, intercept first
+    Y: np.ndarray        # (N, P) phenotypes
+    h2: np.ndarray       # (P,) simulated heritabilities
+    causal: np.ndarray   # (P, n_causal) causal SNP indices
+
+
+CONFIGS = {
+    # config 1: the reference's CPU-runnable example shape
+    1: dict(N=150, M=1500, n_chrom=16, P=20, C=3, maf=(0.05, 0.5), h2=(0.1, 0.8), rint=False, n_perm=100),
+    # config 2: the headline single-GPU scan
+    2: dict(N=1000, M=100_000, n_chrom=22, P=6484, C=4, maf=(0.01, 0.5), h2=(0.0, 0.9), rint=True, n_perm=100),
+}
+
+
+def rint_columns(Y: np.ndarray, c: float = 3.0 / 8.0) -> np.ndarray:
+    """Per-column Blom RINT: qnorm((rank - c) / (n - 2c + 1)), average ties, NaN kept (normalize.py:21-52)."""
+    Y = np.asarray(Y, dtype=np.float64)
+    out = np.full_like(Y, np.nan)
+    for j in range(Y.shape[1]):
+        m = ~np.isnan(Y[:, j])
+        n = int(m.sum())
+        if n:
+            out[m, j] = norm.ppf((rankdata(Y[m, j], method="average") - c) / (n - 2 * c + 1))
+    return out
+
+
+def balding_nichols(rng: np.random.Generator, N: int, M: int, *, n_pop: int = 4, fst: float = 0.1,
+                    maf=(0.05, 0.5), block: int = 8192) -> np.ndarray:
+    G = np.empty((N, M), dtype=np.int8)
+    pop = np.repeat(np.arange(n_pop), -(-N // n_pop))[:N]
+    for j0 in range(0, M, block):
+        j1 = min(M, j0 + block)
+        p = rng.uniform(maf[0], maf[1], size=j1 - j0)
+        a = p * (1 - fst) / fst
+        b = (1 - p) * (1 - fst) / fst
+        pk = rng.beta(a[None, :], b[None, :], size=(n_pop, j1 - j0))
+        G[:, j0:j1] = rng.binomial(2, pk[pop]).astype(np.int8)
+    # a column with no variation scans as zero; keep every SNP polymorphic
+    mono = np.flatnonzero(G.min(axis=0) == G.max(axis=0))
+    for j in mono:
+        G[rng.integers(0, N), j] = 1 if G[0, j] != 1 else 0
+    return G
+
+
+def make_panel(config: int | dict, seed: int = 19930909, *, n_causal: int = 10) -> Panel:
+    cfg = CONFIGS[config] if isinstance(config, int) else config
+    rng = np.random.default_rng(seed)
+    N, M, P, C = cfg["N"], cfg["M"], cfg["P"], cfg["C"]
+    G = balding_nichols(rng, N, M, maf=cfg["maf"])
+    per = -(-M // cfg["n_chrom"])
+    chrom = (np.arange(M) // per + 1).astype(np.int64)
+    X = np.concatenate([np.ones((N, 1)), rng.standard_normal((N, C - 1))], axis=1)
+    h2 = rng.uniform(cfg["h2"][0], cfg["h2"][1], size=P)
+    causal = np.stack([rng.choice(M, size=n_causal, replace=False) for _ in range(P)])
+    Y = np.empty((N, P))
+    for p0 in range(0, P, 1024):
+        p1 = min(P, p0 + 1024)
+        idx = causal[p0:p1]  # (b, n_causal)
+        Gc = G[:, idx.reshape(-1)].astype(np.float64).reshape(N, p1 - p0, n_causal)
+        mu = Gc.mean(axis=0, keepdims=True)
+        sd = Gc.std(axis=0, keepdims=True)
+        sd[sd == 0] = 1.0
+        g = (((Gc - mu) / sd) * rng.standard_normal((1, p1 - p0, n_causal))).sum(axis=2)
+        g = (g - g.mean(axis=0)) / np.maximum(g.std(axis=0), 1e-12)
+        hh = h2[p0:p1]
+        Y[:, p0:p1] = np.sqrt(hh) * g + np.sqrt(1 - hh) * rng.standard_normal((N, p1 - p0))
+    if cfg.get("rint"):
+        Y = rint_columns(Y)
+    return Panel(G=G, chrom=chrom, X=X, Y=Y, h2=h2, causal=causal)

@@ -1,0 +1,105 @@
+/* fasterlmm_amd.h -- C ABI of the MI355X (gfx950) LMM association scan.
+ *
+ * Plain pointers and sizes only: every array argument is a device pointer, every stream argument is a
+ * hipStream_t passed as void* (NULL = the default stream).  Calls are asynchronous on that stream, allocate
+ * nothing, and never synchronise the host.  Each returns FLMM_OK or an error code; flmm_last_error() gives the
+ * message of the last failure on the calling thread.
+ *
+ * Operand planes.  MFMA operands are "planes": a rows x K matrix stored in K-chunks of 32, chunk-major,
+ * element (row, k) at ((k / 32) * rows_pad + row) * 32 + k % 32.  rows_pad is a multiple of 128 and K a
+ * multiple of 128 (the padded extents are zero).  FLMM_PREC_SPLIT_BF16 planes come in (hi, lo) pairs of
+ * uint16 bf16 bits with x ~= hi + lo; FLMM_PREC_FP32 planes are single float planes (pass NULL for lo).
+ *
+ * The reference (gbrach/FaST-ER-LMM v1.6.2) has no native boundary: its hot path is a set of Python
+ * functions on torch tensors.  Each entry point below names the reference function whose arithmetic it
+ * replaces; the Python layer in fast-er-lmm_amd/fasterlmm_amd keeps the reference's function names and
+ * signatures on top of these.
+ */
+#ifndef FASTERLMM_AMD_H
+#define FASTERLMM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum flmm_status { FLMM_OK = 0, FLMM_ERR_ARG = 1, FLMM_ERR_HIP = 2 };
+enum flmm_precision { FLMM_PREC_SPLIT_BF16 = 0, FLMM_PREC_FP32 = 1 };
+enum flmm_rotate_mode { FLMM_ROTATE_UY = 0, FLMM_ROTATE_US = 1 };
+enum flmm_dtype { FLMM_DTYPE_F64 = 0, FLMM_DTYPE_I8 = 1 };
+enum flmm_delta_mode { FLMM_DELTA_GOLDEN = 0, FLMM_DELTA_NEWTON = 1, FLMM_DELTA_FIXED = 2 };
+
+/* ABI version of this header (bumped on any signature change). */
+int flmm_abi_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* flmm_last_error(void);
+
+/* Column standardisation (fasterlmm/io.py:223-254): for each of the m columns of the (n, m) row-major src
+ * (FP64 with NaN = missing, or int8 dosages with negative = missing), NaN-aware mean and population std,
+ * dst = (x - mean) / std, std == 0 -> column of zeros, missing -> 0 after standardising.  mean / std (m doubles
+ * each, may be NULL) receive the column statistics.  src == dst is allowed for FP64. */
+int flmm_standardise(int dtype, const void* src, int64_t n, int64_t m, int64_t ld_src, double* dst, int64_t ld_dst,
+                     double* mean, double* std, void* stream);
+
+/* Pack an FP64 matrix into operand planes.  Element (r, k) = src[src_row(r) * row_stride + k * k_stride],
+ * src_row(r) = row_index ? row_index[r] : r, for r < n_rows and k < n_k; zeros elsewhere up to
+ * (rows_pad, kpad).  Feeds the rotation products of fastlmm_compat_rotate / snp_wald_scan_compat
+ * (fasterlmm/core.py:517-518, 628-629): the columns of Y - X X^+ Y, the SNPs of one chromosome (gathered by
+ * row_index, core.py:709-716), and U_eff^T. */
+int flmm_pack_planes(int prec, const double* src, int64_t n_rows, int64_t n_k, int64_t row_stride, int64_t k_stride,
+                     const int64_t* row_index, void* plane_hi, void* plane_lo, int64_t rows_pad, int64_t kpad,
+                     void* stream);
+
+/* Rotation product out[i][j] = sum_n A[i][n] B[j][n] (A: a_rows_pad x n_pad planes, B: b_rows_pad x n_pad planes,
+ * B = U_eff^T).  Replaces U_eff.T @ Y_r (core.py:518) and U_eff.T @ S_r (core.py:629).
+ *   FLMM_ROTATE_UY: out0 = one fp32 plane (rows a_rows_pad, K = b_rows_pad) of the rotated columns.
+ *   FLMM_ROTATE_US: the scan's SNP-side planes S1 = x (out0, out1) and S2 = x^2 (out2, out3); for
+ *                   FLMM_PREC_FP32 out0 = x, out1 = x^2 (fp32 planes), out2/out3 unused. */
+int flmm_rotate(int prec, int mode, const void* a_hi, const void* a_lo, int64_t a_rows_pad, const void* b_hi,
+                const void* b_lo, int64_t b_rows_pad, int64_t n_pad, void* out0, void* out1, void* out2, void* out3,
+                void* stream);
+
+/* Grid weights for the delta search: fp32 plane wplane (128 rows x kpad) with w[g][k] = 1/(s_k + delta_g) for
+ * g < n_grid, k < neff (0 elsewhere), and lam[g] = sum_j log(s_lam_j + delta_g) over the nlam-long log-det
+ * spectrum (128 floats), on the grid linspace(lo, hi, n_grid) in log delta (core.py:238, 569).  n_grid <= 128.
+ * The compat loss uses s_lam = s (core.py:539); the GLS path's ML loss uses the full spectrum (core.py:177). */
+int flmm_delta_weights(const double* s, int64_t neff, const double* s_lam, int64_t nlam, int64_t kpad, int n_grid,
+                       double lo, double hi, float* wplane, float* lam, void* stream);
+
+/* Grid argmin of the profile loss nco * log(sum_k w_gk y_k^2) + lam[g] (core.py:525-542, 569-571; nco = Neff for
+ * the compat path) for columns 0..P-1 of the fp32 UY plane (rows p_pad, K = kpad); idx[p] = first grid index of
+ * the minimum. */
+int flmm_delta_grid(const float* wplane, const float* lam, int n_grid, double nco, const float* uy, int64_t P,
+                    int64_t p_pad, int64_t kpad, int32_t* idx, void* stream);
+
+/* Refine log delta in [grid[idx-1], grid[idx+1]] (core.py:575-595; FLMM_DELTA_GOLDEN replays the reference's 50
+ * golden-section steps, FLMM_DELTA_NEWTON converges to the same local minimiser by safeguarded Newton;
+ * FLMM_DELTA_FIXED reads log_delta[p] as given and skips the search, idx may be NULL), then
+ * write log_delta[p], rWr[p] = max(sum w y^2, 1e-300) (core.py:641-643), h2[p] = 1/(1+delta) (core.py:668-669)
+ * and the scan's column-side planes Q1 = w o y (q1_hi, q1_lo) and Q2 = w (q2_hi, q2_lo), rows p_pad.  The loss is
+ * nco * (log 2pi + log(R / nco) + 1) + sum_j log(s_lam_j + delta). */
+int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_pad, int64_t kpad, const double* s,
+                      int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo, double hi,
+                      const int32_t* idx, double* log_delta, double* rWr, double* h2, void* q1_hi, void* q1_lo,
+                      void* q2_hi, void* q2_lo, void* stream);
+
+/* Fused Wald scan of one SNP block (snp_wald_scan_compat's tile loop, core.py:636-667): sWy = S1.Q1, sWs = S2.Q2 on
+ * MFMA, FP64 epilogue with dof nu.  Real columns p < n_real write f, beta, se, sfve, nullh2 at
+ * [row_index[m] * n_real + p] of (M, n_real) row-major FP64 arrays (may be NULL when n_real == 0); every column
+ * writes its maximum F over the block's tile rows to pmax[tile][p] (pmax: ceil(mc/128) x p_pad doubles). */
+int flmm_wald_scan(int prec, const void* s1_hi, const void* s1_lo, const void* s2_hi, const void* s2_lo,
+                   int64_t mc_pad, const void* q1_hi, const void* q1_lo, const void* q2_hi, const void* q2_lo,
+                   int64_t p_pad, int64_t kpad, int64_t mc, int64_t P, int64_t n_real, const double* rWr,
+                   const double* h2, double nu, const int64_t* row_index, double* f, double* beta, double* se,
+                   double* sfve, double* nullh2, double* pmax, void* stream);
+
+/* max_F[p] = max(max_F[p], max_t pmax[t * ld + p]) for t < n_tiles (the genome-wide running max of
+ * core.py:657 and core.py:723). */
+int flmm_max_reduce(const double* pmax, int64_t n_tiles, int64_t ld, int64_t P, double* max_F, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,171 @@
+"""GPU parity of the compat (fastlmm) scan path against the FP64 oracle and the reference's golden vectors.
+
+Bars (the ones the scan promises): |delta log10 p| <= 1e-3 and |delta beta| <= 1e-5 * max(|beta|, se) against
+FP64, per test, for the split-BF16 and FP32 MFMA paths; delta search to 1e-5 in log delta.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+from scipy import stats
+
+from conftest import golden
+from oracle import lmm, perms as operms
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+LOG10P_TOL = 1e-3
+BETA_TOL = 1e-5
+
+
+def neglog10p(F, df):
+    return -stats.f.logsf(np.asarray(F, dtype=np.float64), 1, df) / np.log(10.0)
+
+
+def check_scan(res, ref, df, *, log10p_tol=LOG10P_TOL, beta_tol=BETA_TOL, what=""):
+    f = res["f"] if isinstance(res, dict) else res.f.cpu().numpy()
+    beta = res["beta"] if isinstance(res, dict) else res.beta.cpu().numpy()
+    se = res["se"] if isinstance(res, dict) else res.se.cpu().numpy()
+    dlp = np.abs(neglog10p(f, df) - neglog10p(ref["f"], df)).max()
+    scale = np.maximum(np.abs(ref["beta"]), ref["se"])
+    dbeta = (np.abs(beta - ref["beta"]) / scale).max()
+    dse = (np.abs(se - ref["se"]) / ref["se"]).max()
+    assert dlp <= log10p_tol, f"{what}: max |d log10 p| = {dlp:.3e}"
+    assert dbeta <= beta_tol, f"{what}: max |d beta| / max(|beta|, se) = {dbeta:.3e}"
+    assert dse <= beta_tol, f"{what}: max rel d se = {dse:.3e}"
+    return dlp, dbeta
+
+
+@pytest.fixture(scope="module")
+def core():
+    from fasterlmm_amd import core as c
+    return c
+
+
+@pytest.fixture(scope="module")
+def ex(example):
+    Zs = lmm.standardise_columns(example["Z_raw"].astype(np.float64))
+    return {**example, "Zs": Zs}
+
+
+def test_standardise_matches_oracle(example):
+    from fasterlmm_amd.io import standardise_columns
+    Z = example["Z_raw"].astype(np.float64).copy()
+    rng = np.random.default_rng(3)
+    Z[rng.integers(0, Z.shape[0], 40), rng.integers(0, Z.shape[1], 40)] = np.nan
+    Z[:, 7] = 1.0  # constant column
+    got = standardise_columns(torch.from_numpy(Z).to(DEV)).cpu().numpy()
+    want = lmm.standardise_columns(Z)
+    assert np.abs(got - want).max() <= 1e-13
+    assert np.all(got[:, 7] == 0.0)
+    # int8 input, negative = missing
+    G = example["Z_raw"].copy()
+    G[0, :10] = -1
+    got8 = standardise_columns(torch.from_numpy(G).to(DEV)).cpu().numpy()
+    Gf = G.astype(np.float64)
+    Gf[G < 0] = np.nan
+    assert np.abs(got8 - lmm.standardise_columns(Gf)).max() <= 1e-13
+
+
+@pytest.mark.parametrize("precision", ["split_bf16", "fp32"])
+@pytest.mark.parametrize("xkind", ["int", "cov2"])
+def test_compat_rotate_and_delta_vs_oracle(core, ex, precision, xkind):
+    from fasterlmm_amd import engine
+    Zs = ex["Zs"]
+    chrom = ex["chrom"]
+    X = np.ones((Zs.shape[0], 1)) if xkind == "int" else ex["X_cov2"]
+    Y = ex["Y"][:, :6]
+    K = lmm.grm(Zs[:, chrom != 1])
+    want = lmm.compat_rotate(K, X, Y)
+    with engine.configure(precision=precision):
+        spec = core.fastlmm_compat_rotate(torch.from_numpy(K).to(DEV), torch.from_numpy(X).to(DEV),
+                                          torch.from_numpy(Y).to(DEV))
+        assert np.abs(spec.s.cpu().numpy() - want["s"]).max() <= 1e-10
+        # rows of UY are defined up to the eigenvector sign
+        UY = spec.UY.cpu().numpy()
+        sign = np.sign((UY * want["UY"]).sum(axis=1, keepdims=True))
+        rel = np.abs(UY * sign - want["UY"]).max() / np.abs(want["UY"]).max()
+        assert rel <= (2e-5 if precision == "split_bf16" else 2e-6), rel
+        ld_grid = core.fit_delta_grid_compat(spec, refine=False).cpu().numpy()
+        ld = core.fit_delta_grid_compat(spec).cpu().numpy()
+    want_grid = lmm.fit_delta_grid_compat(want, refine=False)
+    want_ld = lmm.fit_delta_grid_compat(want)
+    assert np.mean(np.abs(ld_grid - want_grid) < 1e-12) == 1.0
+    assert np.abs(ld - want_ld).max() <= 1e-4
+
+
+@pytest.mark.parametrize("precision", ["split_bf16", "fp32"])
+def test_snp_wald_scan_compat_at_oracle_delta(core, ex, precision):
+    from fasterlmm_amd import engine
+    Zs, chrom = ex["Zs"], ex["chrom"]
+    X = ex["X_cov2"]
+    Y = ex["Y"][:, :5]
+    K = lmm.grm(Zs[:, chrom != 3])
+    want = lmm.compat_rotate(K, X, Y)
+    ld = lmm.fit_delta_grid_compat(want)
+    S = Zs[:, chrom == 3]
+    ref = lmm.snp_wald_scan_compat(want, ld, S, n_real=3)
+    with engine.configure(precision=precision):
+        spec = core.fastlmm_compat_rotate(torch.from_numpy(K).to(DEV), torch.from_numpy(X).to(DEV),
+                                          torch.from_numpy(Y).to(DEV))
+        res = core.snp_wald_scan_compat(spec, torch.from_numpy(ld).to(DEV), torch.from_numpy(S).to(DEV), n_real=3)
+    df = want["s"].shape[0] - 1
+    check_scan(res, ref, df, what=precision)
+    np.testing.assert_allclose(res.nullh2.cpu().numpy(), ref["nullh2"], rtol=1e-14)
+    mf = res.max_F.cpu().numpy()
+    np.testing.assert_allclose(mf, ref["max_F"], rtol=1e-4)
+    # max_F of the real columns is exactly the column max of their f
+    np.testing.assert_array_equal(mf[:3], res.f.cpu().numpy().max(axis=0))
+
+
+@pytest.mark.parametrize("tag", ["int", "cov2", "full"])
+def test_loco_scan_compat_vs_reference_golden(core, ex, tag):
+    """End-to-end LOCO scan on the reference's example panel against the reference implementation's outputs."""
+    ref = golden(f"compat_loco_{tag}.npz")
+    X = {"int": np.ones((ex["Zs"].shape[0], 1)), "cov2": ex["X_cov2"], "full": ex["X_full"]}[tag]
+    Y = ex["Y"][:, :3]
+    res = core.loco_scan_compat(torch.from_numpy(ex["Zs"]).to(DEV), torch.from_numpy(X).to(DEV),
+                                torch.from_numpy(Y).to(DEV), ex["chrom"])
+    df = ex["Zs"].shape[0] - X.shape[1] - 1
+    check_scan(res, ref, df, what=tag)
+    np.testing.assert_allclose(res.nullh2.cpu().numpy(), ref["nullh2"], rtol=1e-5)
+    np.testing.assert_allclose(res.max_F.cpu().numpy(), ref["max_F"], rtol=1e-4)
+
+
+def test_perm_threshold_vs_reference_golden(ex):
+    from fasterlmm_amd.io import align_inputs
+    from fasterlmm_amd.perms import perm_threshold
+    n = ex["Z_raw"].shape[0]
+    iid = [str(i) for i in range(n)]
+    data = align_inputs(iid, ex["Z_raw"].astype(np.float64), iid, ex["Y"], iid, ex["X_cov2"][:, 1:],
+                        chrom=ex["chrom"])
+    ref = golden("perm_loco_cov2.npz")
+    res, pmf = perm_threshold(data, [0, 1], n_perm=4, seed=19930909, loco=True)
+    df = n - 3 - 1
+    check_scan(res, ref, df, what="perm loco")
+    np.testing.assert_allclose(pmf.cpu().numpy(), ref["perm_max_F"], rtol=1e-4)
+    ref1 = golden("perm_single_cov2.npz")
+    res1, pmf1 = perm_threshold(data, [3], n_perm=3, seed=19930909, loco=False)
+    check_scan(res1, ref1, df, what="perm single")
+    np.testing.assert_allclose(pmf1.cpu().numpy(), ref1["perm_max_F"], rtol=1e-4)
+    # same permutation orders as the oracle restatement
+    o = operms.perm_orders(19930909, [0, 1], n, 4)
+    from fasterlmm_amd.perms import perm_orders
+    np.testing.assert_array_equal(perm_orders(19930909, [0, 1], n, 4), o)
+
+
+def test_golden_section_mode_replays_reference(core, ex):
+    """delta='golden' runs the reference's own 50 golden-section steps; it lands within the reference's own
+    loss-noise floor of the reference's refined log delta."""
+    from fasterlmm_amd import engine
+    ref = golden("compat_loco_cov2.npz")
+    Zs, chrom, X, Y = ex["Zs"], ex["chrom"], ex["X_cov2"], ex["Y"][:, :3]
+    for ci, c in enumerate(sorted(np.unique(chrom))[:4]):
+        K = lmm.grm(Zs[:, chrom != c])
+        with engine.configure(delta="golden", precision="fp32"):
+            spec = core.fastlmm_compat_rotate(torch.from_numpy(K).to(DEV), torch.from_numpy(X).to(DEV),
+                                              torch.from_numpy(Y).to(DEV))
+            ld = core.fit_delta_grid_compat(spec).cpu().numpy()
+        assert np.abs(ld - ref["log_delta"][ci]).max() <= 1e-4
