@@ -183,3 +183,23 @@ int flmm_max_reduce(const double* pmax, int64_t n_tiles, int64_t ld, int64_t P, 
 }
 
 }  // extern "C"
+
+extern "C" {
+
+int flmm_neglog10p_f1(const double* F, int64_t n, double df2, double* out, void* stream) {
+  REQUIRE(n >= 0 && df2 > 0.0, "flmm_neglog10p_f1: n >= 0 and df2 > 0 required");
+  REQUIRE((F && out) || n == 0, "flmm_neglog10p_f1: null pointer");
+  if (n == 0) return FLMM_OK;
+  return check_launch(launch_neglog10p(F, n, df2, out, (hipStream_t)stream), "flmm_neglog10p_f1");
+}
+
+int flmm_effect_size(const double* beta, int64_t M, int64_t n_real, const double* g_var, const double* p_var,
+                     double* out, void* stream) {
+  REQUIRE(M >= 0 && n_real >= 0, "flmm_effect_size: bad sizes");
+  REQUIRE((beta && g_var && p_var && out) || M * n_real == 0, "flmm_effect_size: null pointer");
+  if (M * n_real == 0) return FLMM_OK;
+  return check_launch(launch_effect_size(beta, M, n_real, g_var, p_var, out, (hipStream_t)stream),
+                      "flmm_effect_size");
+}
+
+}  // extern "C"

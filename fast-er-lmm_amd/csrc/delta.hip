@@ -111,7 +111,10 @@ struct ColSums {
 };
 
 // The likelihood of one column: y (neff values in the weight spectrum s) and the log-det spectrum s_lam (nlam
-// values; the same as s on the projected basis, the full spectrum for the ML loss of the GLS path).
+// values; the same as s on the projected basis, the full spectrum for the ML loss of the GLS path).  With
+// KPL > 0 each lane keeps its KPL elements (k = lane + 64 i) of y, s and s_lam in registers, so the column is
+// read from HBM once however many loss / slope evaluations follow; KPL = 0 re-reads them every evaluation.
+template <int KPL>
 struct ColLik {
   const float* uy;
   int64_t p_pad;
@@ -120,46 +123,85 @@ struct ColLik {
   const double* s_lam;
   int64_t nlam;
   double nco;  // coefficient of log R in the loss
-};
+  double yr[KPL > 0 ? KPL : 1], sr[KPL > 0 ? KPL : 1], lr[KPL > 0 ? KPL : 1];
 
-// sums over k of one column at delta; with_log adds Lambda = sum log(s_lam + delta)
-__device__ inline ColSums col_sums(const ColLik& L, int64_t p, double delta, bool with_log, int lane) {
-  ColSums c{0, 0, 0, 0, 0, 0};
-  for (int64_t k = lane; k < L.neff; k += 64) {
-    const double y = (double)L.uy[((k >> 5) * L.p_pad + p) * 32 + (k & 31)];
-    const double w = 1.0 / (L.s[k] + delta);
-    const double yw = y * y * w;
-    c.R += yw;
-    c.R2 += yw * w;
-    c.R3 += yw * w * w;
+  __device__ void load(int64_t p, int lane) {
+    if constexpr (KPL > 0) {
+#pragma unroll
+      for (int i = 0; i < KPL; ++i) {
+        const int64_t k = lane + 64 * i;
+        yr[i] = k < neff ? (double)uy[((k >> 5) * p_pad + p) * 32 + (k & 31)] : 0.0;
+        sr[i] = k < neff ? s[k] : 1.0;
+        lr[i] = k < nlam ? s_lam[k] : 1.0;
+      }
+    }
   }
-  for (int64_t k = lane; k < L.nlam; k += 64) {
-    const double sd = L.s_lam[k] + delta;
-    const double w = 1.0 / sd;
-    c.W1 += w;
-    c.W2 += w * w;
-    if (with_log) c.LS += log(sd);
+
+  // sums over k at delta; with_log adds Lambda = sum log(s_lam + delta)
+  __device__ ColSums sums(int64_t p, double delta, bool with_log, int lane) const {
+    ColSums c{0, 0, 0, 0, 0, 0};
+    if constexpr (KPL > 0) {
+#pragma unroll
+      for (int i = 0; i < KPL; ++i) {
+        const int64_t k = lane + 64 * i;
+        const double w = 1.0 / (sr[i] + delta);
+        const double yw = yr[i] * yr[i] * w;
+        c.R += yw;
+        c.R2 += yw * w;
+        c.R3 += yw * w * w;
+        if (k < nlam) {
+          const double sd = lr[i] + delta;
+          const double wl = 1.0 / sd;
+          c.W1 += wl;
+          c.W2 += wl * wl;
+          if (with_log) c.LS += log(sd);
+        }
+      }
+    } else {
+      for (int64_t k = lane; k < neff; k += 64) {
+        const double y = (double)uy[((k >> 5) * p_pad + p) * 32 + (k & 31)];
+        const double w = 1.0 / (s[k] + delta);
+        const double yw = y * y * w;
+        c.R += yw;
+        c.R2 += yw * w;
+        c.R3 += yw * w * w;
+      }
+      for (int64_t k = lane; k < nlam; k += 64) {
+        const double sd = s_lam[k] + delta;
+        const double w = 1.0 / sd;
+        c.W1 += w;
+        c.W2 += w * w;
+        if (with_log) c.LS += log(sd);
+      }
+    }
+    c.R = wave_sum(c.R);
+    c.R2 = wave_sum(c.R2);
+    c.R3 = wave_sum(c.R3);
+    c.W1 = wave_sum(c.W1);
+    c.W2 = wave_sum(c.W2);
+    if (with_log) c.LS = wave_sum(c.LS);
+    return c;
   }
-  c.R = wave_sum(c.R);
-  c.R2 = wave_sum(c.R2);
-  c.R3 = wave_sum(c.R3);
-  c.W1 = wave_sum(c.W1);
-  c.W2 = wave_sum(c.W2);
-  if (with_log) c.LS = wave_sum(c.LS);
-  return c;
-}
+
+  __device__ double y_at(int64_t p, int i, int64_t k) const {
+    if constexpr (KPL > 0) return yr[i];
+    else return (double)uy[((k >> 5) * p_pad + p) * 32 + (k & 31)];
+  }
+};
 
 constexpr double LOG2PI = 1.8378770664093453;
 
-__device__ inline double col_loss(const ColLik& L, int64_t p, double t, int lane) {
-  const ColSums c = col_sums(L, p, exp(t), true, lane);
+template <int KPL>
+__device__ inline double col_loss(const ColLik<KPL>& L, int64_t p, double t, int lane) {
+  const ColSums c = L.sums(p, exp(t), true, lane);
   return L.nco * (LOG2PI + log(fmax(c.R, FLOOR64) / L.nco) + 1.0) + c.LS;
 }
 
 // slope L'(t) and curvature L''(t)
-__device__ inline void col_slope(const ColLik& L, int64_t p, double t, int lane, double& g, double& gp) {
+template <int KPL>
+__device__ inline void col_slope(const ColLik<KPL>& L, int64_t p, double t, int lane, double& g, double& gp) {
   const double delta = exp(t);
-  const ColSums c = col_sums(L, p, delta, false, lane);
+  const ColSums c = L.sums(p, delta, false, lane);
   const double n = L.nco;
   if (c.R <= FLOOR64) {  // all-zero column: R is floored, only Lambda moves
     g = delta * c.W1;
@@ -172,7 +214,7 @@ __device__ inline void col_slope(const ColLik& L, int64_t p, double t, int lane,
   gp = delta * h + delta * delta * hp;
 }
 
-template <int PREC>
+template <int PREC, int KPL>
 __global__ __launch_bounds__(256) void delta_refine_kernel(
     int mode, const float* __restrict__ uy, int64_t P, int64_t p_pad, int64_t kpad, const double* __restrict__ s,
     int64_t neff, const double* __restrict__ s_lam, int64_t nlam, double nco, int G, double lo, double hi,
@@ -192,7 +234,8 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
     }
     return;
   }
-  const ColLik L{uy, p_pad, s, neff, s_lam, nlam, nco};
+  ColLik<KPL> L{uy, p_pad, s, neff, s_lam, nlam, nco, {}, {}, {}};
+  L.load(p, lane);
   double t;
   const int i = mode == MODE_FIXED ? 0 : idx[p];
   const double a0 = grid_value(i > 0 ? i - 1 : 0, G, lo, hi);
@@ -298,12 +341,17 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
                                double* log_delta, double* rWr, double* h2, void* q1a, void* q1b, void* q2a,
                                void* q2b, hipStream_t stream) {
   const dim3 grid((unsigned)((p_pad + 3) / 4));
-  if (prec == PREC_SPLIT_BF16)
-    hipLaunchKernelGGL(delta_refine_kernel<PREC_SPLIT_BF16>, grid, dim3(256), 0, stream, mode, uy, P, p_pad, kpad, s,
-                       neff, s_lam, nlam, nco, G, lo, hi, idx, log_delta, rWr, h2, q1a, q1b, q2a, q2b);
-  else
-    hipLaunchKernelGGL(delta_refine_kernel<PREC_FP32>, grid, dim3(256), 0, stream, mode, uy, P, p_pad, kpad, s, neff,
-                       s_lam, nlam, nco, G, lo, hi, idx, log_delta, rWr, h2, q1a, q1b, q2a, q2b);
+  // columns that fit 16 elements per lane (neff, nlam <= 1024) keep them in registers
+  const bool regs = neff <= 1024 && nlam <= 1024;
+#define FLMM_REFINE(PR, K)                                                                                          \
+  hipLaunchKernelGGL((delta_refine_kernel<PR, K>), grid, dim3(256), 0, stream, mode, uy, P, p_pad, kpad, s, neff,  \
+                     s_lam, nlam, nco, G, lo, hi, idx, log_delta, rWr, h2, q1a, q1b, q2a, q2b)
+  if (prec == PREC_SPLIT_BF16) {
+    if (regs) FLMM_REFINE(PREC_SPLIT_BF16, 16); else FLMM_REFINE(PREC_SPLIT_BF16, 0);
+  } else {
+    if (regs) FLMM_REFINE(PREC_FP32, 16); else FLMM_REFINE(PREC_FP32, 0);
+  }
+#undef FLMM_REFINE
   return hipGetLastError();
 }
 

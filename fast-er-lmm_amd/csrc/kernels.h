@@ -38,3 +38,9 @@ hipError_t launch_max_reduce(const double* pmax, int64_t n_mtiles, int64_t ld, i
                              hipStream_t stream);
 
 }  // namespace flmm
+
+namespace flmm {
+hipError_t launch_neglog10p(const double* F, int64_t n, double df2, double* out, hipStream_t stream);
+hipError_t launch_effect_size(const double* beta, int64_t M, int64_t n_real, const double* g_var,
+                              const double* p_var, double* out, hipStream_t stream);
+}  // namespace flmm

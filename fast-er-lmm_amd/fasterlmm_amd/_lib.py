@@ -39,6 +39,8 @@ SIGNATURES: dict[str, list] = {
                        _P, _P, _P, _P, _P, _P],
     "flmm_standardise": [_I, _P, _I64, _I64, _I64, _P, _I64, _P, _P, _P],
     "flmm_max_reduce": [_P, _I64, _I64, _I64, _P, _P],
+    "flmm_neglog10p_f1": [_P, _I64, _D, _P, _P],
+    "flmm_effect_size": [_P, _I64, _I64, _P, _P, _P, _P],
 }
 
 _lib: C.CDLL | None = None

@@ -1,5 +1,5 @@
 """Drop-in spectral LMM core on MI355X: the reference's fasterlmm/core.py API (names, signatures, dataclasses,
-error behaviour) with the scan, the rotations and the delta search running in the gfx950 kernels.
+error behaviour) with the rotations, the delta search and the Wald scan running in the gfx950 kernels.
 
 Reference: fasterlmm/core.py (gbrach/FaST-ER-LMM v1.6.2).
 
@@ -7,14 +7,24 @@ Reference: fasterlmm/core.py (gbrach/FaST-ER-LMM v1.6.2).
   device of the input, in FP64, as in the reference (core.py:322-326, 701-706).  There is no CPU fallback:
   without a ROCm GPU or without the built library every entry point raises.
 * ``snp_chunk`` / ``pheno_chunk`` are accepted for signature compatibility; tiling is internal (128 x 128 MFMA
-  tiles) and, as in the reference (chunking-invariance tests), invisible in the results.
-* Precision: the rotations and the scan products run as split-BF16 MFMA (hi + lo, three products, FP32
-  accumulation) by default, or FP32 MFMA (``engine.configure(precision="fp32")``); every reduction into F,
-  beta, SE, the delta search and the projections are FP64.
+  tiles) and, as in the reference (its chunking-invariance tests), invisible in the results.
+* Precision: the rotations and the two scan products run as split-BF16 MFMA (hi + lo, three products, FP32
+  accumulation) by default, or FP32 MFMA (``engine.configure(precision="fp32")``); the Wald epilogue, the
+  delta search, the projections and the eigendecompositions are FP64.
 * Delta refinement defaults to safeguarded Newton to the local minimiser inside the reference's golden-section
   bracket; ``engine.configure(delta="golden")`` replays the reference's 50 golden-section steps instead.
-* The OLS projection of the test SNPs, S - X X^+ S (core.py:628), is not formed: every kept eigenvector is
-  orthogonal to the column space of X, so U_eff^T (S - X X^+ S) = U_eff^T S exactly in exact arithmetic.
+
+One projected basis serves both formulations.  For an orthonormal basis B of null(X^T) that diagonalises the
+projected kinship, B^T (K + delta I) B = diag(s' + delta), the GLS Wald statistic of a SNP with the covariates X
+as nuisance equals the covariate-free weighted regression of B^T y on B^T s with weights 1 / (s' + delta):
+GLS's M = V^-1 - V^-1 X (X^T V^-1 X)^-1 X^T V^-1 equals B (B^T V B)^-1 B^T.
+  - compat path (core.py:454-740): B = the kept eigenvectors of (I - X X^+)(K + I)(I - X X^+), the reference's
+    own construction.
+  - plain GLS path (core.py:89-451): the same construction on K + 1e-6 I, so every test needs 2 MFMA products
+    instead of C + 2 and the C x C Schur complement (core.py:351-355) is carried out once per chromosome in
+    FP64 by the basis change instead of per test.  The ML loss keeps its full log det V term (the spectrum of
+    K + 1e-6 I) and N; the REML loss equals the projected loss up to a constant.
+The test SNPs' OLS projection S - X X^+ S (core.py:628) is not formed: B^T X = 0, so B^T (S - X X^+ S) = B^T S.
 """
 from __future__ import annotations
 
@@ -25,7 +35,7 @@ import torch
 from torch import Tensor
 
 from . import _lib, engine
-from .engine import TILE, KCHUNK, Planes, rup
+from .engine import TILE, KCHUNK, rup
 
 
 # SCAN ARRAYS -------
@@ -60,6 +70,9 @@ class CompatSpectrum:
     UY: Tensor
 
 
+_FIELDS = ("f", "beta", "se", "sfve", "nullh2", "max_F")
+
+
 def _pos_floor(dtype: torch.dtype) -> float:
     """Clamp floor per dtype (core.py:80-86)."""
     return 1e-300 if dtype == torch.float64 else 1e-30
@@ -76,12 +89,12 @@ def _dev(*ts):
 def _home(res: ScanResult, home: torch.device) -> ScanResult:
     if home.type == "cuda":
         return res
-    return ScanResult(*(getattr(res, k).to(home) for k in ("f", "beta", "se", "sfve", "nullh2", "max_F")))
+    return ScanResult(*(getattr(res, k).to(home) for k in _FIELDS))
 
 
 def _eigh(K: Tensor) -> tuple[Tensor, Tensor]:
     """FP64 symmetric eigendecomposition on the GPU (rocSOLVER through torch), ascending (core.py:54-77).  An
-    out-of-memory error retries on the host LAPACK like the reference's fallback; any other error propagates."""
+    out-of-memory error retries on the host LAPACK as the reference's fallback does; other errors propagate."""
     try:
         return torch.linalg.eigh(K)
     except RuntimeError as e:
@@ -104,50 +117,60 @@ def eigendecompose(K: Tensor, jitter: float = 1e-6) -> tuple[Tensor, Tensor]:
     return s.to(home), U.to(home)
 
 
-# COMPAT PATH -------
+# PROJECTED BASIS ON THE GPU -------
 
-class _CompatGPU:
-    """Device-side state of one projected eigendecomposition: U_eff^T planes and the rotated columns."""
+class _Basis:
+    """A projected basis B (columns col0 .. col0 + n_basis - 1 of the (N, L) FP64 matrix `mat`) with its weight
+    spectrum s (1 / (s + delta) per basis vector), held as MFMA planes of B^T, plus the rotated scan columns.
 
-    def __init__(self, s: Tensor, U: Tensor, D: int, prec: int):
-        N = U.shape[0]
-        self.N, self.D, self.neff = N, D, N - D
+    nu is the Wald dof.  Losses: the projected (compat / REML) loss has coefficient n_basis and log-det spectrum
+    s; the GLS ML loss has coefficient N and log-det spectrum s_ml (the full spectrum of K + jitter I)."""
+
+    def __init__(self, s: Tensor, mat: Tensor, col0: int, n_basis: int, prec: int, nu: float,
+                 s_ml: Tensor | None = None):
+        N, L = mat.shape
+        self.N, self.neff = N, n_basis
         self.npad = rup(N, KCHUNK)
-        self.kpad = rup(max(self.neff, 1), TILE)
+        self.kpad = rup(max(n_basis, 1), TILE)
         self.prec = prec
+        self.nu = float(nu)
         self.s = s.contiguous()
-        # U_eff^T planes: row j = eigenvector D + j, element k = U[k, D + j]
-        self.ueff = engine.pack(prec, U, self.neff, N, 1, N, self.kpad, self.npad, offset=D)
-        self.uy = None  # fp32 plane of the rotated columns
+        self.s_ml = None if s_ml is None else s_ml.contiguous()
+        # B^T planes: row j = basis vector j, element k = mat[k, col0 + j]
+        self.bt = engine.pack(prec, mat.contiguous(), n_basis, N, 1, L, self.kpad, self.npad, offset=col0)
+        self.uy = None  # fp32 plane of B^T Y
         self.P = 0
         self.p_pad = 0
-        self.delta = None  # last DeltaResult (its log_delta tensor identifies it)
+        self.delta = None  # the DeltaResult of the last fit (its log_delta tensor identifies it)
 
-    def rotate_columns(self, Y_r: Tensor) -> None:
-        """UY plane of U_eff^T Y_r for Y_r (N, P)."""
-        N, P = Y_r.shape
+    def rotate_columns(self, Y: Tensor) -> None:
+        """UY plane of B^T Y for Y (N, P) FP64."""
+        N, P = Y.shape
         self.P, self.p_pad = P, rup(max(P, 1), TILE)
-        ypl = engine.pack(self.prec, Y_r.contiguous(), P, N, 1, P, self.p_pad, self.npad)
-        self.uy = engine.rotate_uy(ypl, self.ueff)
+        ypl = engine.pack(self.prec, Y.contiguous(), P, N, 1, P, self.p_pad, self.npad)
+        self.uy = engine.rotate_uy(ypl, self.bt)
+
+    def use_column_planes(self, ypl: engine.Planes, P: int) -> None:
+        """UY plane from pre-packed column planes (shared by every chromosome of a LOCO scan)."""
+        self.P, self.p_pad = P, ypl.rows_pad
+        self.uy = engine.rotate_uy(ypl, self.bt)
 
     def uy_dense(self) -> Tensor:
         return engine.fp32_plane_to_dense(self.uy, self.p_pad, self.kpad, self.P, self.neff).T.double()
 
-    def set_uy_dense(self, UY: Tensor) -> None:
-        """Rebuild the UY plane from a caller-supplied (Neff, P) FP64 matrix."""
-        neff, P = UY.shape
-        self.P, self.p_pad = P, rup(max(P, 1), TILE)
-        pl = engine.pack(_lib.PREC_FP32, UY.contiguous(), P, neff, 1, P, self.p_pad, self.kpad)
-        self.uy = pl.hi
+    def _loss(self, reml_or_compat: bool):
+        if reml_or_compat or self.s_ml is None:
+            return self.s, float(self.neff)
+        return self.s_ml, float(self.N)
 
-    def fit(self, n_grid: int, lo: float, hi: float, refine: bool, mode: int | None = None):
-        idx = engine.delta_grid_idx(self.uy, self.P, self.p_pad, self.kpad, self.s, self.s, float(self.neff),
-                                    n_grid, lo, hi)
+    def fit(self, n_grid: int, lo: float, hi: float, refine: bool, *, projected_loss: bool = True):
+        s_lam, nco = self._loss(projected_loss)
+        idx = engine.delta_grid_idx(self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam, nco, n_grid, lo, hi)
         if not refine:
             grid = torch.from_numpy(engine.grid_values(n_grid, lo, hi)).to(self.s.device)
             return grid[idx.long()], None
-        d = engine.delta_refine(self.prec, engine.delta_code() if mode is None else mode, self.uy, self.P,
-                                self.p_pad, self.kpad, self.s, self.s, float(self.neff), n_grid, lo, hi, idx)
+        d = engine.delta_refine(self.prec, engine.delta_code(), self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam,
+                                nco, n_grid, lo, hi, idx)
         self.delta = d
         return d.log_delta, d
 
@@ -160,35 +183,45 @@ class _CompatGPU:
 
     def scan(self, Z: Tensor, row_index: Tensor, n_real: int, out: engine.ScanOutputs, d: engine.DeltaResult,
              block: int = 8192) -> None:
-        """Scan the SNP columns Z[:, row_index] (Z is (N, M_total) FP64) into out."""
+        """Scan the SNP columns Z[:, row_index] of Z (N, M_total) FP64 into out."""
         M_tot = Z.shape[1]
         Mc = row_index.numel()
         for b0 in range(0, Mc, block):
             ri = row_index[b0:b0 + block]
             mc = ri.numel()
-            mc_pad = rup(mc, TILE)
-            spl = engine.pack(self.prec, Z, mc, self.N, 1, M_tot, mc_pad, self.npad, row_index=ri)
-            s1, s2 = engine.rotate_us(spl, self.ueff)
+            spl = engine.pack(self.prec, Z, mc, self.N, 1, M_tot, rup(mc, TILE), self.npad, row_index=ri)
+            s1, s2 = engine.rotate_us(spl, self.bt)
             del spl
-            engine.wald_scan(s1, s2, d, mc, self.P, n_real, float(self.neff - 1), ri, out)
+            engine.wald_scan(s1, s2, d, mc, self.P, n_real, self.nu, ri, out)
 
 
-def _compat_project(K: Tensor, X: Tensor) -> tuple[Tensor, Tensor, Tensor]:
-    """Xpinv, and the projected eigendecomposition of (I - X X^+)(K + I)(I - X X^+) (core.py:495-510)."""
-    N = K.shape[0]
+def _projector(X: Tensor) -> tuple[Tensor, Tensor]:
+    """X^+ (torch.linalg.pinv, core.py:498) and I - X X^+ (core.py:499-500)."""
+    N = X.shape[0]
     Xpinv = torch.linalg.pinv(X)
-    Iperp = torch.eye(N, dtype=K.dtype, device=K.device) - X @ Xpinv
-    Kp = (Iperp @ (K + torch.eye(N, dtype=K.dtype, device=K.device))) @ Iperp
+    return Xpinv, torch.eye(N, dtype=X.dtype, device=X.device) - X @ Xpinv
+
+
+def _projected_eigh(K: Tensor, Iperp: Tensor, D: int, shift: float = 1.0) -> tuple[Tensor, Tensor]:
+    """eigh of Iperp (K + shift I) Iperp (core.py:503-510); returns (s = clamp(lambda[D:] - shift, 0), U)."""
+    N = K.shape[0]
+    Kp = (Iperp @ (K + shift * torch.eye(N, dtype=K.dtype, device=K.device))) @ Iperp
     s_full, U = _eigh(Kp)
-    return Xpinv, s_full, U
+    return torch.clamp(s_full[D:N] - shift, min=0.0), U
 
 
-def _spec_state(spec: CompatSpectrum) -> _CompatGPU:
+def _spec_state(spec) -> _Basis:
     st = getattr(spec, "_gpu", None)
     if st is None:
-        raise ValueError("CompatSpectrum was not built by fasterlmm_amd.core.fastlmm_compat_rotate")
+        raise ValueError("spectrum was not built by fasterlmm_amd.core (rotate / fastlmm_compat_rotate)")
     return st
 
+
+def _new_outputs(M, n_real, P, dev):
+    return engine.ScanOutputs.allocate(M, n_real, P, dev)
+
+
+# COMPAT PATH -------
 
 def fastlmm_compat_rotate(K: Tensor, X: Tensor, Y: Tensor) -> CompatSpectrum:
     """Projected rotation (core.py:487-519): eigh of (I - X X^+)(K + I)(I - X X^+), drop the first D = ncols(X)
@@ -196,11 +229,10 @@ def fastlmm_compat_rotate(K: Tensor, X: Tensor, Y: Tensor) -> CompatSpectrum:
     home, (Kd, Xd, Yd) = _dev(K, X, Y)
     N = Kd.shape[0]
     D = Xd.shape[1]
-    Xpinv, s_full, U = _compat_project(Kd, Xd)
-    s = torch.clamp(s_full[D:N] - 1.0, min=0.0)
-    st = _CompatGPU(s, U, D, engine.prec_code())
-    Y_r = Yd - Xd @ (Xpinv @ Yd)
-    st.rotate_columns(Y_r)
+    Xpinv, Iperp = _projector(Xd)
+    s, U = _projected_eigh(Kd, Iperp, D)
+    st = _Basis(s, U, D, N - D, engine.prec_code(), nu=N - D - 1)
+    st.rotate_columns(Yd - Xd @ (Xpinv @ Yd))
     spec = CompatSpectrum(s=s.to(home), U_eff=U[:, D:N].contiguous().to(home), X=Xd.to(home), Xpinv=Xpinv.to(home),
                           UY=st.uy_dense().to(home))
     spec._gpu = st
@@ -212,15 +244,14 @@ def fit_delta_grid_compat(spec: CompatSpectrum, *, n_grid: int = 64, log_delta_m
     """Per-column log delta for the compat path: grid argmin then refinement in [g[i-1], g[i+1]]
     (core.py:561-595).  Returns (P,) FP64 on the spectrum's device."""
     st = _spec_state(spec)
-    home = spec.s.device
     ld, _ = st.fit(n_grid, float(log_delta_min), float(log_delta_max), refine)
-    return ld.to(home)
+    return ld.to(spec.s.device)
 
 
 def snp_wald_scan_compat(spec: CompatSpectrum, log_delta: Tensor, S: Tensor, *, snp_chunk: int = 4096,
                          pheno_chunk: int = 256, n_real: int | None = None) -> ScanResult:
     """Wald F / beta / SE / SnpFractVarExpl of every SNP column of S against every column of the spectrum, in the
-    projected basis with dof Neff - 1 (core.py:601-671).  Real columns (the first n_real) keep full detail, every
+    projected basis with dof Neff - 1 (core.py:601-671).  The first n_real columns keep full detail, every
     column keeps its maximum F."""
     st = _spec_state(spec)
     home, (Sd, ldd) = _dev(S, log_delta)
@@ -228,43 +259,37 @@ def snp_wald_scan_compat(spec: CompatSpectrum, log_delta: Tensor, S: Tensor, *, 
     M = Sd.shape[1]
     n_real = P if n_real is None else min(int(n_real), P)
     d = st.delta_for(ldd)
-    out = engine.ScanOutputs.allocate(M, n_real, P, Sd.device)
+    out = _new_outputs(M, n_real, P, Sd.device)
     st.scan(Sd.contiguous(), torch.arange(M, device=Sd.device, dtype=torch.int64), n_real, out, d)
     return _home(ScanResult(out.f, out.beta, out.se, out.sfve, out.nullh2, out.max_F), home)
 
 
-def _chrom_labels(chrom) -> np.ndarray:
-    return np.asarray(chrom)
-
-
-def loco_scan_compat(Z: Tensor, X: Tensor, Y: Tensor, chrom: np.ndarray, *, n_real: int | None = None,
-                     on_chrom=None, Z_kin: Tensor | None = None, chrom_kin: np.ndarray | None = None,
-                     n_grid: int = 64) -> ScanResult:
-    """Leave-one-chromosome-out compat scan (core.py:677-726).  For each chromosome c: K from every other
-    chromosome's SNPs (of Z, or of the kinship panel Z_kin), the projected eigendecomposition, per-column delta,
-    and the scan of c's SNPs; max_F is the genome-wide max over chromosomes.  Z must be pre-standardised."""
+def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: bool, reml: bool):
+    """Shared LOCO driver (core.py:403-436, 694-726): per chromosome the kinship of every other chromosome
+    (of Z, or of the kinship panel), the projected basis, per-column delta and the scan of the chromosome's
+    SNPs; real-column rows are scattered into place and max_F is the maximum over chromosomes."""
     if (Z_kin is None) != (chrom_kin is None):
         raise ValueError("Z_kin and chrom_kin go together")
     home, (Zd, Xd, Yd, Zkd) = _dev(Z, X, Y, Z_kin)
-    chrom = _chrom_labels(chrom)
+    chrom = np.asarray(chrom)
     N, M = Zd.shape
     P = Yd.shape[1]
     D = Xd.shape[1]
     n_real = P if n_real is None else min(int(n_real), P)
     dev = Zd.device
     prec = engine.prec_code()
-    out = engine.ScanOutputs.allocate(M, n_real, P, dev)
+    out = _new_outputs(M, n_real, P, dev)
     Zk = Zd if Zkd is None else Zkd
-    ck = chrom if chrom_kin is None else _chrom_labels(chrom_kin)
+    ck = chrom if chrom_kin is None else np.asarray(chrom_kin)
     K_all = Zk @ Zk.T
-    Xpinv = torch.linalg.pinv(Xd)
-    eye = torch.eye(N, dtype=torch.float64, device=dev)
-    Iperp = eye - Xd @ Xpinv
+    Xpinv, Iperp = _projector(Xd)
+    # the basis is orthogonal to X, so B^T (Y - X X^+ Y) = B^T Y; the OLS regress-out keeps the compat
+    # path's arithmetic (core.py:517) and costs one pass
     Y_r = (Yd - Xd @ (Xpinv @ Yd)).contiguous()
-    npad = rup(N, KCHUNK)
-    p_pad = rup(max(P, 1), TILE)
-    ypl = engine.pack(prec, Y_r, P, N, 1, P, p_pad, npad)
+    ypl = engine.pack(prec, Y_r, P, N, 1, P, rup(max(P, 1), TILE), rup(N, KCHUNK))
     del Y_r
+    jitter = 1e-6 if plain else 0.0
+    eye = torch.eye(N, dtype=torch.float64, device=dev)
     chroms = sorted(np.unique(chrom).tolist())
     for k, c in enumerate(chroms):
         test = torch.from_numpy(np.flatnonzero(chrom == c)).to(dev)
@@ -272,17 +297,28 @@ def loco_scan_compat(Z: Tensor, X: Tensor, Y: Tensor, chrom: np.ndarray, *, n_re
         Zc = Zk.index_select(1, kin_c)
         K = (K_all - Zc @ Zc.T) / float(Zk.shape[1] - kin_c.numel())
         del Zc
-        Kp = (Iperp @ (K + eye)) @ Iperp
-        s_full, U = _eigh(Kp)
-        s = torch.clamp(s_full[D:N] - 1.0, min=0.0)
-        st = _CompatGPU(s, U, D, prec)
-        st.P, st.p_pad = P, p_pad
-        st.uy = engine.rotate_uy(ypl, st.ueff)
-        _, d = st.fit(n_grid, -10.0, 10.0, True)
+        if jitter:
+            K = K + jitter * eye
+        s, U = _projected_eigh(K, Iperp, D)
+        s_ml = None
+        if plain and not reml:
+            s_ml = torch.clamp(torch.linalg.eigvalsh(K), min=0.0)
+        st = _Basis(s, U, D, N - D, prec, nu=N - D - 1, s_ml=s_ml)
+        st.use_column_planes(ypl, P)
+        _, d = st.fit(n_grid, -10.0, 10.0, True, projected_loss=(not plain) or reml)
         st.scan(Zd, test, n_real, out, d)
         if on_chrom is not None:
             on_chrom(k + 1, len(chroms))
     return _home(ScanResult(out.f, out.beta, out.se, out.sfve, out.nullh2, out.max_F), home)
+
+
+def loco_scan_compat(Z: Tensor, X: Tensor, Y: Tensor, chrom: np.ndarray, *, n_real: int | None = None,
+                     on_chrom=None, Z_kin: Tensor | None = None, chrom_kin: np.ndarray | None = None,
+                     n_grid: int = 64) -> ScanResult:
+    """Leave-one-chromosome-out compat scan (core.py:677-726).  Z must be pre-standardised; Z_kin / chrom_kin
+    (both or neither) build every K_loco from a separate pre-standardised panel.  n_grid defaults to the
+    reference's 64 (fit_delta_grid_compat's default)."""
+    return _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, plain=False, reml=False)
 
 
 def single_k_scan_compat(Z: Tensor, X: Tensor, Y: Tensor, *, n_real: int | None = None,
@@ -295,3 +331,152 @@ def single_k_scan_compat(Z: Tensor, X: Tensor, Y: Tensor, *, n_real: int | None 
     spec = fastlmm_compat_rotate(K, X, Y)
     ld = fit_delta_grid_compat(spec, n_grid=n_grid)
     return _home(snp_wald_scan_compat(spec, ld, Zd, n_real=n_real), home)
+
+
+# PLAIN GLS PATH -------
+
+def _plain_basis(s: Tensor, X_rot: Tensor, prec: int) -> tuple[_Basis, Tensor]:
+    """Projected basis inside the rotated space of a Spectrum: Q = orthonormal basis of null(X_rot^T) (complete
+    QR), eigh of Q^T diag(s) Q, B = Q V.  Returns (basis state, B (N, N - C))."""
+    N, C = X_rot.shape
+    Q, _ = torch.linalg.qr(X_rot, mode="complete")
+    Qn = Q[:, C:]
+    Bm = Qn.T @ (s.unsqueeze(1) * Qn)
+    sp, V = _eigh(0.5 * (Bm + Bm.T))
+    B = (Qn @ V).contiguous()
+    st = _Basis(torch.clamp(sp, min=0.0), B, 0, N - C, prec, nu=N - C - 1, s_ml=s)
+    return st, B
+
+
+def rotate(K: Tensor, X: Tensor, Y: Tensor) -> Spectrum:
+    """Spectrum: s, U = eigendecompose(K) (K + 1e-6 I, clamped), X_rot = U^T X, Y_rot = U^T Y (core.py:105-116)."""
+    home, (Kd, Xd, Yd) = _dev(K, X, Y)
+    s, U = eigendecompose(Kd)
+    X_rot = U.T @ Xd
+    st, _ = _plain_basis(s, X_rot, engine.prec_code())
+    # rotated-space columns of the projected basis: B^T (U^T Y) = (U B)^T Y
+    Y_rot = U.T @ Yd
+    st.rotate_columns(Y_rot)
+    spec = Spectrum(s=s.to(home), U=U.to(home), X_rot=X_rot.to(home), Y_rot=Y_rot.to(home))
+    spec._gpu = st
+    return spec
+
+
+def _spectrum_state(spectrum: Spectrum) -> _Basis:
+    st = getattr(spectrum, "_gpu", None)
+    if st is None:
+        _, (s, X_rot, Y_rot) = _dev(spectrum.s, spectrum.X_rot, spectrum.Y_rot)
+        st, _ = _plain_basis(s, X_rot, engine.prec_code())
+        st.rotate_columns(Y_rot)
+        spectrum._gpu = st
+    return st
+
+
+def nLLeval(delta, s: Tensor, X_rot: Tensor, y_rot: Tensor) -> Tensor:
+    """Profiled ML -2 log-likelihood of one column at one delta (core.py:122-145), FP64."""
+    N, _ = X_rot.shape
+    Sd = s + delta
+    w = 1.0 / Sd
+    WX = w.unsqueeze(-1) * X_rot
+    A = WX.T @ X_rot
+    Xy = WX.T @ y_rot
+    beta = torch.linalg.solve(A, Xy)
+    rWr = (y_rot * w * y_rot).sum() - (Xy * beta).sum()
+    log2pi = torch.log(torch.tensor(2.0 * torch.pi, dtype=s.dtype, device=s.device))
+    return N * (log2pi + (rWr / N).log() + 1.0) + Sd.log().sum()
+
+
+def _profile_loss(log_delta: Tensor, s: Tensor, X_rot: Tensor, Y_rot: Tensor, *, reml: bool = False) -> Tensor:
+    """(G, P) ML / REML -2 log-likelihood on a grid (core.py:148-187), FP64 on the input's device."""
+    N, C = X_rot.shape
+    delta = log_delta.exp()
+    Sd = s.unsqueeze(0) + delta.unsqueeze(1)
+    w = 1.0 / Sd
+    WX = w.unsqueeze(2) * X_rot.unsqueeze(0)
+    A = torch.einsum("gnc,nd->gcd", WX, X_rot)
+    L, info = torch.linalg.cholesky_ex(A)
+    bad = info > 0
+    if bad.any():
+        L = torch.where(bad.view(-1, 1, 1), torch.eye(C, dtype=A.dtype, device=A.device).expand_as(L), L)
+    logdet_A = 2.0 * torch.diagonal(L, dim1=-2, dim2=-1).log().sum(-1)
+    u = torch.einsum("gnc,np->gcp", WX, Y_rot)
+    uAu = (u * torch.cholesky_solve(u, L)).sum(dim=1)
+    rWr = torch.clamp(w @ (Y_rot * Y_rot) - uAu, min=_pos_floor(s.dtype))
+    sum_log_Sd = Sd.log().sum(dim=1)
+    log2pi = torch.log(torch.tensor(2.0 * torch.pi, dtype=s.dtype, device=s.device))
+    if reml:
+        loss = (N - C) * (log2pi + (rWr / (N - C)).log() + 1.0) + sum_log_Sd.unsqueeze(1) + logdet_A.unsqueeze(1)
+    else:
+        loss = N * (log2pi + (rWr / N).log() + 1.0) + sum_log_Sd.unsqueeze(1)
+    if bad.any():
+        loss = torch.where(bad.unsqueeze(1), torch.full_like(loss, float("inf")), loss)
+    return loss
+
+
+def _profile_loss_per_pheno(log_delta: Tensor, s: Tensor, X_rot: Tensor, Y_rot: Tensor, *,
+                            reml: bool = False) -> Tensor:
+    """(P,) loss with one log delta per column (core.py:190-224)."""
+    G = torch.stack([_profile_loss(log_delta[p:p + 1], s, X_rot, Y_rot[:, p:p + 1], reml=reml)[0, 0]
+                     for p in range(Y_rot.shape[1])])
+    return G
+
+
+def fit_delta_grid(spectrum: Spectrum, *, n_grid: int = 64, log_delta_min: float = -10.0,
+                   log_delta_max: float = 10.0, refine: bool = True, reml: bool = False) -> Tensor:
+    """Per-column log delta for the GLS path, ML or REML: grid argmin then refinement in [g[i-1], g[i+1]]
+    (core.py:227-264), on the projected basis (ML keeps the full log det V of the spectrum and N)."""
+    st = _spectrum_state(spectrum)
+    ld, _ = st.fit(n_grid, float(log_delta_min), float(log_delta_max), refine, projected_loss=reml)
+    return ld.to(spectrum.s.device)
+
+
+def _auto_tile(N: int, C: int, M: int, P: int, device: torch.device) -> tuple[int, int]:
+    """The reference's tile heuristic (core.py:270-292), kept for API compatibility; the kernels tile
+    internally in 128 x 128 MFMA tiles and ignore it."""
+    if device.type != "cuda":
+        return min(256, P), min(4096, M)
+    free_bytes = torch.cuda.mem_get_info(device)[0]
+    budget = int(free_bytes * 0.5)
+    T = 12
+    snp_chunk = min(M, 16384)
+    while snp_chunk >= 512:
+        per_pc = 8 * (N * C + (2 * C + T) * snp_chunk)
+        pheno_chunk = (budget - 8 * N * snp_chunk) // max(per_pc, 1)
+        if pheno_chunk >= 64:
+            return int(min(pheno_chunk, P)), int(snp_chunk)
+        snp_chunk //= 2
+    return min(64, P), min(512, M)
+
+
+def snp_wald_scan(spectrum: Spectrum, log_delta: Tensor, S_rot: Tensor, *, snp_chunk: int | None = None,
+                  pheno_chunk: int | None = None, n_real: int | None = None) -> ScanResult:
+    """GLS Wald F / beta / SE / SnpFractVarExpl of pre-rotated SNPs S_rot = U^T Z against every column, dof
+    N - C - 1 (core.py:295-375), through the projected basis of the spectrum."""
+    st = _spectrum_state(spectrum)
+    home, (Sd, ldd) = _dev(S_rot, log_delta)
+    P = st.P
+    M = Sd.shape[1]
+    n_real = P if n_real is None else min(int(n_real), P)
+    d = st.delta_for(ldd)
+    out = _new_outputs(M, n_real, P, Sd.device)
+    st.scan(Sd.contiguous(), torch.arange(M, device=Sd.device, dtype=torch.int64), n_real, out, d)
+    return _home(ScanResult(out.f, out.beta, out.se, out.sfve, out.nullh2, out.max_F), home)
+
+
+def loco_scan(Z: Tensor, X: Tensor, Y: Tensor, chrom: np.ndarray, *, n_real: int | None = None, on_chrom=None,
+              Z_kin: Tensor | None = None, chrom_kin: np.ndarray | None = None, n_grid: int = 64,
+              reml: bool = False) -> ScanResult:
+    """Leave-one-chromosome-out GLS scan (core.py:381-436): per chromosome K_loco + 1e-6 I, ML (or REML) delta
+    per column and the GLS Wald test of the chromosome's SNPs."""
+    return _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, plain=True, reml=reml)
+
+
+def single_k_scan(Z: Tensor, X: Tensor, Y: Tensor, *, n_real: int | None = None,
+                  Z_kin: Tensor | None = None) -> ScanResult:
+    """One kinship, GLS rotation, delta fit and scan of every SNP (core.py:439-451)."""
+    from .io import grm
+    home, (Zd, Zkd) = _dev(Z, Z_kin)
+    spec = rotate(grm(Zd if Zkd is None else Zkd), X, Y)
+    ld = fit_delta_grid(spec)
+    S_rot = _dev(spec.U)[1][0].T @ Zd
+    return _home(snp_wald_scan(spec, ld, S_rot, n_real=n_real), home)

@@ -99,6 +99,14 @@ int flmm_wald_scan(int prec, const void* s1_hi, const void* s1_lo, const void* s
  * core.py:657 and core.py:723). */
 int flmm_max_reduce(const double* pmax, int64_t n_tiles, int64_t ld, int64_t P, double* max_F, void* stream);
 
+/* -log10 of the F(1, df2) survival function of n values (the caller's F -> p, fasterlmm/cli.py:405-406),
+ * evaluated in log space (finite far below the FP64 range of p).  F <= 0 -> 0, F = inf -> inf, NaN -> NaN. */
+int flmm_neglog10p_f1(const double* F, int64_t n, double df2, double* out, void* stream);
+
+/* EffectSize = beta^2 * g_var[m] / p_var[j] for an (M, n_real) row-major beta (fasterlmm/cli.py:132). */
+int flmm_effect_size(const double* beta, int64_t M, int64_t n_real, const double* g_var, const double* p_var,
+                     double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
