@@ -1,6 +1,6 @@
 """Throughput of the LOCO compat Wald scan on one MI355X: Wald tests/s (SNP x phenotype x permutation).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--no-fp32] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--precision P] [--no-compare] [--no-cpu]
 
 Workload (one "step"): the fastlmm-compat leave-one-chromosome-out scan that perm_threshold drives, over one
 synthetic panel resident in HBM -- per chromosome the LOCO kinship, the projected FP64 eigendecomposition, the
@@ -9,8 +9,9 @@ Wald scan, with full F/beta/SE/SFVE/nullh2 detail for the real phenotypes and th
 column.  Default workload: N=1,000 x M=100,000 (22 chromosomes) x 6,484 RINT phenotypes x (1 + 100
 permutations), C=4 -- 6.5488e10 Wald tests per step.
 
-The timed path is split-BF16 MFMA (3 products, FP32 accumulation, FP64 epilogue); the same step on the FP32
-MFMA path and the FP64 CPU oracle (on a bounded sample) are reported beside it.  Prints one JSON line.
+The timed path is split-FP16 MFMA (power-of-two-scaled fp16 hi + lo, 3 products, FP32 accumulation, FP64 epilogue);
+the same step on the split-BF16 and FP32 MFMA paths and the FP64 CPU oracle (on a bounded sample) are reported
+beside it.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -38,7 +39,8 @@ def parse():
     ap.add_argument("--n-perm", type=int, default=None)
     ap.add_argument("--pheno", type=int, default=None, help="override the number of real phenotypes")
     ap.add_argument("--snps", type=int, default=None, help="override M")
-    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--precision", default="split_fp16", choices=["split_fp16", "split_bf16", "fp32"])
+    ap.add_argument("--no-compare", action="store_true", help="skip the split-BF16 / FP32 comparison steps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     return ap.parse_args()
@@ -117,6 +119,22 @@ def cpu_baseline(panel, cfg, n_perm, budget_s):
                       f"(64 phenotypes x {1 + n_perm}), N={Zs.shape[0]}, {dt:.1f}s wall, numpy/BLAS on {threads} threads"}
 
 
+def timed_run(Z, X, Y_all, chrom, B, precision, steps):
+    """(results of the last step, seconds for `steps` steps, scan-kernel summary)."""
+    import torch
+    from fasterlmm_amd import engine
+    engine.scan_timer.reset(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = run_step(Z, X, Y_all, chrom, B, precision)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    scan = engine.scan_timer.summary()
+    engine.scan_timer.reset(False)
+    return res, dt, scan
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -125,66 +143,57 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
-    from fasterlmm_amd import engine
+    from scipy import stats
 
     cfg, panel, Z, X, Y_all, B, n_perm = build_workload(args.config, args.n_perm, args.pheno, args.snps)
     N, M = Z.shape
     P = Y_all.shape[1]
     tests_per_step = M * P
     chrom = panel.chrom
+    df = N - X.shape[1] - 1
+    prec = args.precision
 
     for w in range(args.warmup):
         t0 = time.perf_counter()
-        run_step(Z, X, Y_all, chrom, B, "split_bf16")
+        run_step(Z, X, Y_all, chrom, B, prec)
         torch.cuda.synchronize()
         log(f"[bench] warmup {w} {time.perf_counter() - t0:.3f}s")
 
-    engine.scan_timer.reset(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        res = run_step(Z, X, Y_all, chrom, B, "split_bf16")
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    scan = engine.scan_timer.summary()
-    engine.scan_timer.reset(False)
+    res, dt, scan = timed_run(Z, X, Y_all, chrom, B, prec, args.steps)
     ms_per_step = 1000.0 * dt / args.steps
     value = tests_per_step * args.steps / dt
-    log(f"[bench] split-bf16: {ms_per_step:.1f} ms/step, {value:.4e} tests/s, scan kernels {scan}")
+    log(f"[bench] {prec}: {ms_per_step:.1f} ms/step, {value:.4e} tests/s, scan kernels {scan}")
 
-    alg_tflops = scan["alg_flops"] / (scan["total_ms"] * 1e-3) / 1e12 if scan["total_ms"] else 0.0
-    peak_eff = MI355X_PEAK_BF16_TFLOPS / 3.0
-    roofline = {"bound": "mfma", "achieved": round(alg_tflops, 2), "peak": round(peak_eff, 1), "unit": "TFLOP/s",
-                "frac": round(alg_tflops / peak_eff, 4), "traffic": None,
-                "kernel": "wald_scan_kernel<split_bf16>", "mean_launch_ms": round(scan["mean_ms"], 4),
+    def tflops(sc):
+        return sc["alg_flops"] / (sc["total_ms"] * 1e-3) / 1e12 if sc["total_ms"] else 0.0
+
+    split = prec != "fp32"
+    alg = tflops(scan)
+    peak = MI355X_PEAK_BF16_TFLOPS / 3.0 if split else MI355X_PEAK_FP32_TFLOPS
+    roofline = {"bound": "mfma", "achieved": round(alg, 2), "peak": round(peak, 1), "unit": "TFLOP/s",
+                "frac": round(alg / peak, 4), "traffic": None,
+                "kernel": f"wald_scan_wide_kernel<{prec}>", "mean_launch_ms": round(scan["mean_ms"], 4),
                 "launches_per_step": scan["launches"] // args.steps,
-                "note": "achieved = 4*Neff algorithmic flop per Wald test / measured kernel time; peak = 2.5 PF dense "
-                        "bf16 / 3 MFMA products per split-BF16 product (issued MFMA rate = 3x achieved)"}
+                "note": "achieved = 4*Neff algorithmic flop per Wald test / HIP-event kernel time over the timed steps; "
+                        "peak = 2.5 PF dense fp16/bf16 MFMA / 3 products per split product (issued MFMA = 3x achieved)"}
 
     extra = {}
-    if not args.no_fp32:
-        engine.scan_timer.reset(True)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        res32 = run_step(Z, X, Y_all, chrom, B, "fp32")
-        torch.cuda.synchronize()
-        dt32 = time.perf_counter() - t1
-        s32 = engine.scan_timer.summary()
-        engine.scan_timer.reset(False)
-        fp32_rate = tests_per_step / dt32
-        a32 = s32["alg_flops"] / (s32["total_ms"] * 1e-3) / 1e12 if s32["total_ms"] else 0.0
-        # agreement of the two GPU paths on the real columns
-        from scipy import stats
-        df = N - X.shape[1] - 1
-        f16 = res.f.cpu().numpy()
-        f32 = res32.f.cpu().numpy()
-        dlp = float(np.nanmax(np.abs(stats.f.logsf(f16, 1, df) - stats.f.logsf(f32, 1, df))) / np.log(10))
-        extra["fp32_path"] = {"value": fp32_rate, "ms_per_step": 1000 * dt32, "scan_kernel_ms": s32["total_ms"],
-                              "scan_tflops": round(a32, 2), "frac_of_fp32_peak": round(a32 / MI355X_PEAK_FP32_TFLOPS, 4),
-                              "max_abs_dlog10p_vs_split": dlp}
-        extra["speedup_vs_fp32"] = value / fp32_rate
-        extra["scan_kernel_speedup_vs_fp32"] = s32["total_ms"] / scan["total_ms"] * args.steps if scan["total_ms"] else None
-        log(f"[bench] fp32: {1000 * dt32:.1f} ms/step, {fp32_rate:.4e} tests/s; split/fp32 = {value / fp32_rate:.2f}x")
+    f_head = res.f.cpu().numpy()
+    lp_head = stats.f.logsf(f_head, 1, df) / np.log(10)
+    for other in [] if args.no_compare else [p for p in ("split_bf16", "fp32") if p != prec]:
+        r2, dt2, s2 = timed_run(Z, X, Y_all, chrom, B, other, 1)
+        rate = tests_per_step / dt2
+        dlp = float(np.nanmax(np.abs(stats.f.logsf(r2.f.cpu().numpy(), 1, df) / np.log(10) - lp_head)))
+        a2 = tflops(s2)
+        pk = MI355X_PEAK_BF16_TFLOPS / 3.0 if other != "fp32" else MI355X_PEAK_FP32_TFLOPS
+        extra[f"{other}_path"] = {"value": rate, "ms_per_step": 1000 * dt2, "scan_kernel_ms": s2["total_ms"],
+                                  "scan_tflops": round(a2, 2), "scan_frac_of_peak": round(a2 / pk, 4),
+                                  "max_abs_dlog10p_vs_headline": dlp,
+                                  "headline_speedup": value / rate,
+                                  "headline_scan_kernel_speedup": (s2["total_ms"] / (scan["total_ms"] / args.steps))
+                                  if scan["total_ms"] else None}
+        log(f"[bench] {other}: {1000 * dt2:.1f} ms/step, {rate:.4e} tests/s; {prec}/{other} = {value / rate:.2f}x")
+        del r2
 
     cpu = None
     if not args.no_cpu:
@@ -192,14 +201,17 @@ def main():
         cpu = cpu_baseline(panel, cfg, n_perm, args.cpu_seconds)
         log(f"[bench] cpu baseline {cpu['value']:.4e} tests/s in {time.perf_counter() - t2:.1f}s")
 
+    desc = {"split_fp16": "split-FP16 (power-of-two-scaled fp16 hi+lo, 3 MFMA products, FP32 accumulate)",
+            "split_bf16": "split-BF16 (bf16 hi+lo, 3 MFMA products, FP32 accumulate)",
+            "fp32": "FP32 MFMA"}[prec]
     line = {
-        "metric": "Wald tests/s (SNP x pheno x perm), split-BF16 LOCO compat scan",
+        "metric": "Wald tests/s (SNP x pheno x perm), split-precision MFMA LOCO compat scan",
         "value": value, "unit": "Wald tests/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16", "data": "synthetic",
+        "dtype": {"split_fp16": "fp16", "split_bf16": "bf16", "fp32": "fp32"}[prec], "data": "synthetic",
         "config": {"workload": f"LOCO compat scan, N={N} M={M} ({len(np.unique(chrom))} chromosomes) "
                                f"P={B} phenotypes x (1+{n_perm} perms) = {P} columns, C={X.shape[1]}, 100-point grid",
-                   "precision": "split-BF16 (hi+lo, 3 MFMA products, FP32 accumulate), FP64 epilogue and delta search",
+                   "precision": desc + "; FP64 epilogue, delta search, projections and eigendecompositions",
                    "tests_per_step": tests_per_step, "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, **extra,
     }

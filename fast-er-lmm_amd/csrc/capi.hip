@@ -1,14 +1,14 @@
 // C ABI entry points: argument validation, error reporting and launch.  See include/fasterlmm_amd.h.
+#include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <string>
 
 #include "../../include/fasterlmm_amd.h"
 #include "kernels.h"
-
-
 
 using namespace flmm;
 
@@ -36,51 +36,76 @@ static int check_launch(hipError_t e, const char* what) {
   } while (0)
 
 static bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+static bool valid_prec(int p) { return p == FLMM_PREC_SPLIT_BF16 || p == FLMM_PREC_FP32 || p == FLMM_PREC_SPLIT_FP16; }
+static bool two_planes(int p) { return p != FLMM_PREC_FP32; }
 
 extern "C" {
 
-int flmm_abi_version(void) { return 1; }
+int flmm_abi_version(void) { return 2; }
 
 const char* flmm_last_error(void) { return g_err.c_str(); }
 
+int flmm_row_norms(const double* src, int64_t n_rows, int64_t n_k, int64_t row_stride, int64_t k_stride,
+                   const int64_t* row_index, double* norm, void* stream) {
+  REQUIRE(n_rows >= 0 && n_k >= 0, "flmm_row_norms: bad sizes");
+  REQUIRE((src && norm) || n_rows == 0, "flmm_row_norms: null pointer");
+  if (n_rows == 0) return FLMM_OK;
+  return check_launch(launch_row_norms(src, n_rows, n_k, row_stride, k_stride, row_index, norm, (hipStream_t)stream),
+                      "flmm_row_norms");
+}
+
 int flmm_pack_planes(int prec, const double* src, int64_t n_rows, int64_t n_k, int64_t row_stride, int64_t k_stride,
-                     const int64_t* row_index, void* plane_hi, void* plane_lo, int64_t rows_pad, int64_t kpad,
-                     void* stream) {
-  REQUIRE(prec == FLMM_PREC_SPLIT_BF16 || prec == FLMM_PREC_FP32, "flmm_pack_planes: bad precision %d", prec);
+                     const int64_t* row_index, const double* row_norm, void* plane_hi, void* plane_lo,
+                     float* row_scale, int64_t rows_pad, int64_t kpad, void* stream) {
+  REQUIRE(valid_prec(prec), "flmm_pack_planes: bad precision %d", prec);
   REQUIRE(n_rows >= 0 && n_k >= 0 && n_rows <= rows_pad && n_k <= kpad, "flmm_pack_planes: bad sizes");
   REQUIRE(rows_pad % TILE == 0 && kpad % KCHUNK == 0, "flmm_pack_planes: rows_pad %% 128 and kpad %% 32 required");
-  REQUIRE(plane_hi && (prec == FLMM_PREC_FP32 || plane_lo), "flmm_pack_planes: null plane");
+  REQUIRE(plane_hi && (!two_planes(prec) || plane_lo), "flmm_pack_planes: null plane");
+  REQUIRE(prec != FLMM_PREC_SPLIT_FP16 || (row_scale && (row_norm || n_rows == 0)),
+          "flmm_pack_planes: split-FP16 needs row_norm and row_scale");
   REQUIRE(src || n_rows == 0 || n_k == 0, "flmm_pack_planes: null src");
   if (rows_pad == 0 || kpad == 0) return FLMM_OK;
-  return check_launch(launch_pack_planes(prec, src, n_rows, n_k, row_stride, k_stride, row_index, plane_hi, plane_lo,
-                                         rows_pad, kpad, (hipStream_t)stream),
+  return check_launch(launch_pack_planes(prec, src, n_rows, n_k, row_stride, k_stride, row_index, row_norm, plane_hi,
+                                         plane_lo, row_scale, rows_pad, kpad, (hipStream_t)stream),
                       "flmm_pack_planes");
 }
 
-int flmm_rotate(int prec, int mode, const void* a_hi, const void* a_lo, int64_t a_rows_pad, const void* b_hi,
-                const void* b_lo, int64_t b_rows_pad, int64_t n_pad, void* out0, void* out1, void* out2, void* out3,
-                void* stream) {
-  REQUIRE(prec == FLMM_PREC_SPLIT_BF16 || prec == FLMM_PREC_FP32, "flmm_rotate: bad precision %d", prec);
+int flmm_rotate(int prec, int mode, const void* a_hi, const void* a_lo, const float* a_scale, int64_t a_rows_pad,
+                const void* b_hi, const void* b_lo, const float* b_scale, int64_t b_rows_pad, int64_t n_pad,
+                void* out0, void* out1, void* out2, void* out3, float* out_scale1, float* out_scale2, void* stream) {
+  REQUIRE(valid_prec(prec), "flmm_rotate: bad precision %d", prec);
   REQUIRE(mode == FLMM_ROTATE_UY || mode == FLMM_ROTATE_US, "flmm_rotate: bad mode %d", mode);
   REQUIRE(a_rows_pad > 0 && b_rows_pad > 0 && n_pad > 0, "flmm_rotate: empty operand");
   REQUIRE(a_rows_pad % TILE == 0 && b_rows_pad % TILE == 0 && n_pad % KCHUNK == 0,
           "flmm_rotate: rows_pad %% 128 and n_pad %% 32 required");
-  const bool split = prec == FLMM_PREC_SPLIT_BF16;
-  REQUIRE(a_hi && b_hi && (!split || (a_lo && b_lo)), "flmm_rotate: null operand plane");
-  REQUIRE(aligned16(a_hi) && aligned16(b_hi) && (!split || (aligned16(a_lo) && aligned16(b_lo))),
+  const bool two = two_planes(prec);
+  REQUIRE(a_hi && b_hi && (!two || (a_lo && b_lo)), "flmm_rotate: null operand plane");
+  REQUIRE(aligned16(a_hi) && aligned16(b_hi) && (!two || (aligned16(a_lo) && aligned16(b_lo))),
           "flmm_rotate: operand planes must be 16-byte aligned");
-  REQUIRE(out0 && (mode == FLMM_ROTATE_UY || (out1 && (!split || (out2 && out3)))), "flmm_rotate: null output");
+  REQUIRE(out0 && (mode == FLMM_ROTATE_UY || (out1 && (!two || (out2 && out3)))), "flmm_rotate: null output");
+  REQUIRE(prec != FLMM_PREC_SPLIT_FP16 ||
+              (a_scale && b_scale && (mode == FLMM_ROTATE_UY || (out_scale1 && out_scale2))),
+          "flmm_rotate: split-FP16 needs the operand and output row scales");
+  const int nch = (int)(n_pad / KCHUNK);
   hipError_t e;
-  if (split) {
+  if (prec == FLMM_PREC_SPLIT_BF16) {
     TileArgs<PREC_SPLIT_BF16, 1> t{};
     t.a[0][0] = a_hi; t.a[0][1] = a_lo; t.b[0][0] = b_hi; t.b[0][1] = b_lo;
-    t.a_rows_pad = a_rows_pad; t.b_rows_pad = b_rows_pad; t.nchunks = (int)(n_pad / KCHUNK);
-    e = launch_rotate<PREC_SPLIT_BF16>(mode, t, out0, out1, out2, out3, (hipStream_t)stream);
+    t.a_rows_pad = a_rows_pad; t.b_rows_pad = b_rows_pad; t.nchunks = nch;
+    e = launch_rotate<PREC_SPLIT_BF16>(mode, t, nullptr, nullptr, out0, out1, out2, out3, nullptr, nullptr,
+                                       (hipStream_t)stream);
+  } else if (prec == FLMM_PREC_SPLIT_FP16) {
+    TileArgs<PREC_SPLIT_FP16, 1> t{};
+    t.a[0][0] = a_hi; t.a[0][1] = a_lo; t.b[0][0] = b_hi; t.b[0][1] = b_lo;
+    t.a_rows_pad = a_rows_pad; t.b_rows_pad = b_rows_pad; t.nchunks = nch;
+    e = launch_rotate<PREC_SPLIT_FP16>(mode, t, a_scale, b_scale, out0, out1, out2, out3, out_scale1, out_scale2,
+                                       (hipStream_t)stream);
   } else {
     TileArgs<PREC_FP32, 1> t{};
     t.a[0][0] = a_hi; t.b[0][0] = b_hi;
-    t.a_rows_pad = a_rows_pad; t.b_rows_pad = b_rows_pad; t.nchunks = (int)(n_pad / KCHUNK);
-    e = launch_rotate<PREC_FP32>(mode, t, out0, out1, out2, out3, (hipStream_t)stream);
+    t.a_rows_pad = a_rows_pad; t.b_rows_pad = b_rows_pad; t.nchunks = nch;
+    e = launch_rotate<PREC_FP32>(mode, t, nullptr, nullptr, out0, out1, out2, out3, nullptr, nullptr,
+                                 (hipStream_t)stream);
   }
   return check_launch(e, "flmm_rotate");
 }
@@ -107,61 +132,145 @@ int flmm_delta_grid(const float* wplane, const float* lam, int n_grid, double nc
                       "flmm_delta_grid");
 }
 
+static int check_refine(const char* who, int prec, const float* uy, int64_t P, int64_t p_pad, int64_t kpad,
+                        const double* s, int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid,
+                        double* log_delta, double* rWr, double* h2, void* q1_hi, void* q1_lo, void* q2_hi,
+                        void* q2_lo, float* q1_scale, float* q2_scale) {
+  REQUIRE(valid_prec(prec), "%s: bad precision %d", who, prec);
+  REQUIRE(n_grid >= 1, "%s: n_grid >= 1 required", who);
+  REQUIRE(P >= 0 && P <= p_pad && p_pad % TILE == 0 && kpad % TILE == 0 && neff >= 1 && neff <= kpad && nlam >= 1,
+          "%s: bad sizes", who);
+  REQUIRE(nco > 0.0, "%s: loss coefficient must be positive", who);
+  const bool two = two_planes(prec);
+  REQUIRE(uy && s && s_lam && log_delta && rWr && h2 && q1_hi && q2_hi && (!two || (q1_lo && q2_lo)),
+          "%s: null pointer", who);
+  REQUIRE(prec != FLMM_PREC_SPLIT_FP16 || (q1_scale && q2_scale), "%s: split-FP16 needs q1_scale / q2_scale", who);
+  return FLMM_OK;
+}
+
 int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_pad, int64_t kpad, const double* s,
                       int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo, double hi,
                       const int32_t* idx, double* log_delta, double* rWr, double* h2, void* q1_hi, void* q1_lo,
-                      void* q2_hi, void* q2_lo, void* stream) {
-  REQUIRE(prec == FLMM_PREC_SPLIT_BF16 || prec == FLMM_PREC_FP32, "flmm_delta_refine: bad precision %d", prec);
-  REQUIRE(mode == FLMM_DELTA_GOLDEN || mode == FLMM_DELTA_NEWTON || mode == FLMM_DELTA_FIXED,
+                      void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale, void* stream) {
+  REQUIRE(mode == FLMM_DELTA_GOLDEN || mode == FLMM_DELTA_NEWTON || mode == FLMM_DELTA_FIXED ||
+              mode == FLMM_DELTA_GRID,
           "flmm_delta_refine: bad mode %d", mode);
-  REQUIRE(n_grid >= 1, "flmm_delta_refine: n_grid >= 1 required");
-  REQUIRE(P >= 0 && P <= p_pad && p_pad % TILE == 0 && kpad % TILE == 0 && neff >= 1 && neff <= kpad && nlam >= 1,
-          "flmm_delta_refine: bad sizes");
-  REQUIRE(nco > 0.0, "flmm_delta_refine: loss coefficient must be positive");
-  const bool split = prec == FLMM_PREC_SPLIT_BF16;
-  REQUIRE(uy && s && s_lam && (idx || mode == FLMM_DELTA_FIXED) && log_delta && rWr && h2 && q1_hi && q2_hi &&
-              (!split || (q1_lo && q2_lo)),
-          "flmm_delta_refine: null pointer");
+  REQUIRE(idx || mode == FLMM_DELTA_FIXED || mode == FLMM_DELTA_GRID, "flmm_delta_refine: null idx");
+  const int rc = check_refine("flmm_delta_refine", prec, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco, n_grid,
+                              log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, q1_scale, q2_scale);
+  if (rc) return rc;
   if (p_pad == 0) return FLMM_OK;
   return check_launch(launch_delta_refine(prec, mode, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco, n_grid, lo, hi,
-                                          idx, log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, (hipStream_t)stream),
+                                          idx, log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, q1_scale, q2_scale,
+                                          0.0, 0.0, nullptr, (hipStream_t)stream),
                       "flmm_delta_refine");
 }
 
+int flmm_delta_refine_learned(int prec, const float* uy, int64_t P, int64_t p_pad, int64_t kpad, const double* s,
+                              int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo,
+                              double hi, const int32_t* idx, double radius, double judge_tol, double* log_delta,
+                              int32_t* fallback, double* rWr, double* h2, void* q1_hi, void* q1_lo, void* q2_hi,
+                              void* q2_lo, float* q1_scale, float* q2_scale, void* stream) {
+  REQUIRE(radius > 0.0 && judge_tol >= 0.0 && idx, "flmm_delta_refine_learned: bad radius / tol / idx");
+  const int rc = check_refine("flmm_delta_refine_learned", prec, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco,
+                              n_grid, log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, q1_scale, q2_scale);
+  if (rc) return rc;
+  if (p_pad == 0) return FLMM_OK;
+  return check_launch(launch_delta_refine(prec, 3, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco, n_grid, lo, hi, idx,
+                                          log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, q1_scale, q2_scale, radius,
+                                          judge_tol, fallback, (hipStream_t)stream),
+                      "flmm_delta_refine_learned");
+}
+
+int flmm_mlp_features(const float* uy, int64_t P, int64_t p_pad, int64_t neff, const int32_t* bin_of_k, double n,
+                      double c, float* feat, void* stream) {
+  REQUIRE(P >= 0 && P <= p_pad && p_pad % TILE == 0 && neff >= 1, "flmm_mlp_features: bad sizes");
+  REQUIRE((uy && bin_of_k && feat) || P == 0, "flmm_mlp_features: null pointer");
+  if (P == 0) return FLMM_OK;
+  return check_launch(launch_mlp_features(uy, P, p_pad, neff, bin_of_k, (float)log(n), (float)c, feat,
+                                          (hipStream_t)stream),
+                      "flmm_mlp_features");
+}
+
+int flmm_mlp_dense_fwd(const float* in, int64_t B, int n_in, const float* W, const float* bias, int n_out, int act,
+                       float* out, void* stream) {
+  REQUIRE(B >= 0 && n_in > 0 && n_out > 0 && (act == 0 || act == 1), "flmm_mlp_dense_fwd: bad sizes / activation");
+  REQUIRE((in && W && bias && out) || B == 0, "flmm_mlp_dense_fwd: null pointer");
+  if (B == 0) return FLMM_OK;
+  return check_launch(launch_mlp_dense_fwd(in, B, n_in, W, bias, n_out, act, out, (hipStream_t)stream),
+                      "flmm_mlp_dense_fwd");
+}
+
+int flmm_mlp_dense_bwd(const float* dout, float* dz, const float* out, const float* in, int64_t B, int n_in,
+                       int n_out, int act, const float* W, float* din, float* dW, float* db, void* stream) {
+  REQUIRE(B > 0 && n_in > 0 && n_out > 0 && (act == 0 || act == 1), "flmm_mlp_dense_bwd: bad sizes / activation");
+  REQUIRE(dout && dz && out && in && W && dW && db, "flmm_mlp_dense_bwd: null pointer");
+  return check_launch(launch_mlp_dense_bwd(dout, dz, out, in, B, n_in, n_out, act, W, din, dW, db,
+                                           (hipStream_t)stream),
+                      "flmm_mlp_dense_bwd");
+}
+
+int flmm_mlp_mse_grad(const float* pred, const float* target, int64_t B, float* dpred, void* stream) {
+  REQUIRE(B > 0 && pred && target && dpred, "flmm_mlp_mse_grad: bad arguments");
+  return check_launch(launch_mlp_mse_grad(pred, target, B, dpred, (hipStream_t)stream), "flmm_mlp_mse_grad");
+}
+
+int flmm_adam(float* w, const float* g, float* m, float* v, int64_t n, double lr, double beta1, double beta2,
+              double eps, int step, void* stream) {
+  REQUIRE(n >= 0 && step >= 1 && lr > 0.0, "flmm_adam: bad arguments");
+  REQUIRE((w && g && m && v) || n == 0, "flmm_adam: null pointer");
+  if (n == 0) return FLMM_OK;
+  return check_launch(launch_adam(w, g, m, v, n, (float)lr, (float)beta1, (float)beta2, (float)eps, step,
+                                  (hipStream_t)stream),
+                      "flmm_adam");
+}
+
 int flmm_wald_scan(int prec, const void* s1_hi, const void* s1_lo, const void* s2_hi, const void* s2_lo,
-                   int64_t mc_pad, const void* q1_hi, const void* q1_lo, const void* q2_hi, const void* q2_lo,
-                   int64_t p_pad, int64_t kpad, int64_t mc, int64_t P, int64_t n_real, const double* rWr,
-                   const double* h2, double nu, const int64_t* row_index, double* f, double* beta, double* se,
-                   double* sfve, double* nullh2, double* pmax, void* stream) {
-  REQUIRE(prec == FLMM_PREC_SPLIT_BF16 || prec == FLMM_PREC_FP32, "flmm_wald_scan: bad precision %d", prec);
+                   const float* s1_scale, const float* s2_scale, int64_t mc_pad, const void* q1_hi,
+                   const void* q1_lo, const void* q2_hi, const void* q2_lo, const float* q1_scale,
+                   const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t mc, int64_t P, int64_t n_real,
+                   const double* rWr, const double* h2, double nu, const int64_t* row_index, double* f, double* beta,
+                   double* se, double* sfve, double* nullh2, double* pmax, void* stream) {
+  REQUIRE(valid_prec(prec), "flmm_wald_scan: bad precision %d", prec);
   REQUIRE(mc >= 0 && mc <= mc_pad && mc_pad % TILE == 0, "flmm_wald_scan: bad SNP extent");
   REQUIRE(P >= 0 && P <= p_pad && p_pad % TILE == 0, "flmm_wald_scan: bad column extent");
   REQUIRE(kpad > 0 && kpad % TILE == 0, "flmm_wald_scan: kpad must be a positive multiple of 128");
   REQUIRE(n_real >= 0 && n_real <= P, "flmm_wald_scan: n_real must be in [0, P]");
   REQUIRE(nu > 0.0, "flmm_wald_scan: dof must be positive");
-  const bool split = prec == FLMM_PREC_SPLIT_BF16;
-  REQUIRE(s1_hi && s2_hi && q1_hi && q2_hi && (!split || (s1_lo && s2_lo && q1_lo && q2_lo)),
+  const bool two = two_planes(prec);
+  REQUIRE(s1_hi && s2_hi && q1_hi && q2_hi && (!two || (s1_lo && s2_lo && q1_lo && q2_lo)),
           "flmm_wald_scan: null operand plane");
   REQUIRE(aligned16(s1_hi) && aligned16(s2_hi) && aligned16(q1_hi) && aligned16(q2_hi) &&
-              (!split || (aligned16(s1_lo) && aligned16(s2_lo) && aligned16(q1_lo) && aligned16(q2_lo))),
+              (!two || (aligned16(s1_lo) && aligned16(s2_lo) && aligned16(q1_lo) && aligned16(q2_lo))),
           "flmm_wald_scan: operand planes must be 16-byte aligned");
+  REQUIRE(prec != FLMM_PREC_SPLIT_FP16 || (s1_scale && s2_scale && q1_scale && q2_scale),
+          "flmm_wald_scan: split-FP16 needs the four row-scale arrays");
   REQUIRE(rWr && pmax, "flmm_wald_scan: null rWr / pmax");
   REQUIRE(n_real == 0 || (f && beta && se && sfve && nullh2 && h2 && row_index), "flmm_wald_scan: null output");
   if (mc == 0 || P == 0) return FLMM_OK;
   ScanOut out{f, beta, se, sfve, nullh2, n_real};
+  ScanScales sc{nullptr, nullptr, nullptr, nullptr};
+  if (prec == FLMM_PREC_SPLIT_FP16) sc = ScanScales{s1_scale, s2_scale, q1_scale, q2_scale};
+  // tile variant: "wide" (128 x 256, column operand streamed to registers) needs p_pad % 256 == 0;
+  // FLMM_SCAN_VARIANT=square forces the 128 x 128 LDS-staged variant
+  const char* var = getenv("FLMM_SCAN_VARIANT");
+  const bool wide = (p_pad % 256 == 0) && !(var && strcmp(var, "square") == 0);
+  const int nch = (int)(kpad / KCHUNK);
+  const hipStream_t st = (hipStream_t)stream;
   hipError_t e;
-  if (split) {
-    TileArgs<PREC_SPLIT_BF16, 2> t{};
-    t.a[0][0] = s1_hi; t.a[0][1] = s1_lo; t.a[1][0] = s2_hi; t.a[1][1] = s2_lo;
-    t.b[0][0] = q1_hi; t.b[0][1] = q1_lo; t.b[1][0] = q2_hi; t.b[1][1] = q2_lo;
-    t.a_rows_pad = mc_pad; t.b_rows_pad = p_pad; t.nchunks = (int)(kpad / KCHUNK);
-    e = launch_wald_scan<PREC_SPLIT_BF16>(t, mc, P, n_real, rWr, h2, nu, row_index, out, pmax, (hipStream_t)stream);
-  } else {
-    TileArgs<PREC_FP32, 2> t{};
-    t.a[0][0] = s1_hi; t.a[1][0] = s2_hi; t.b[0][0] = q1_hi; t.b[1][0] = q2_hi;
-    t.a_rows_pad = mc_pad; t.b_rows_pad = p_pad; t.nchunks = (int)(kpad / KCHUNK);
-    e = launch_wald_scan<PREC_FP32>(t, mc, P, n_real, rWr, h2, nu, row_index, out, pmax, (hipStream_t)stream);
-  }
+#define FLMM_SCAN(PR)                                                                                                \
+  do {                                                                                                               \
+    TileArgs<PR, 2> t{};                                                                                             \
+    t.a[0][0] = s1_hi; t.a[0][1] = s1_lo; t.a[1][0] = s2_hi; t.a[1][1] = s2_lo;                                      \
+    t.b[0][0] = q1_hi; t.b[0][1] = q1_lo; t.b[1][0] = q2_hi; t.b[1][1] = q2_lo;                                      \
+    t.a_rows_pad = mc_pad; t.b_rows_pad = p_pad; t.nchunks = nch;                                                    \
+    e = wide ? launch_wald_scan_wide<PR>(t, sc, mc, P, n_real, rWr, h2, nu, row_index, out, pmax, st)                \
+             : launch_wald_scan<PR>(t, sc, mc, P, n_real, rWr, h2, nu, row_index, out, pmax, st);                    \
+  } while (0)
+  if (prec == FLMM_PREC_SPLIT_BF16) FLMM_SCAN(PREC_SPLIT_BF16);
+  else if (prec == FLMM_PREC_SPLIT_FP16) FLMM_SCAN(PREC_SPLIT_FP16);
+  else FLMM_SCAN(PREC_FP32);
+#undef FLMM_SCAN
   return check_launch(e, "flmm_wald_scan");
 }
 
@@ -181,10 +290,6 @@ int flmm_max_reduce(const double* pmax, int64_t n_tiles, int64_t ld, int64_t P, 
   if (P == 0 || n_tiles == 0) return FLMM_OK;
   return check_launch(launch_max_reduce(pmax, n_tiles, ld, P, max_F, (hipStream_t)stream), "flmm_max_reduce");
 }
-
-}  // extern "C"
-
-extern "C" {
 
 int flmm_neglog10p_f1(const double* F, int64_t n, double df2, double* out, void* stream) {
   REQUIRE(n >= 0 && df2 > 0.0, "flmm_neglog10p_f1: n >= 0 and df2 > 0 required");

@@ -15,18 +15,23 @@
 //   MODE_GOLDEN: the reference algorithm step for step -- bracket [g[i-1], g[i+1]], 50 golden-section
 //                iterations, (a+b)/2 clamped (core.py:577-595).
 //   MODE_FIXED:  no search; the caller's log delta is used as given.
+//   MODE_GRID:   no refinement: the first minimiser of the FP64 loss over the grid nodes (refine=False).
+//   MODE_LEARNED: a proposal t0 per column (from the MLP) finished by Newton within t0 +- radius; accepted when it
+//                converges there and its loss is within judge_tol of the grid optimum, otherwise (flags[p] = 1)
+//                the column falls back to MODE_NEWTON's grid bracket.
 //   MODE_NEWTON: the same local minimiser in the same bracket, found by safeguarded Newton on
 //                L'(t) = delta (W1 - n R2 / R), W1 = sum 1/(s_lam+delta), R2 = sum y^2 w^2, w = 1/(s+delta):
 //                no logarithms per iteration and quadratic convergence; bisection keeps the iterate inside
 //                the bracket.  A bracket whose left slope is >= 0 (right slope <= 0) has its minimum at that
 //                edge, which is where golden section converges too.
-// The refinement then writes the scan's column-side operand planes Q1 = w o y and Q2 = w for the final delta,
+// The refinement then writes the scan's column-side operand planes Q1 = w o y and Q2 = w for the final delta
+// (split-FP16: each column scaled by the power of two of its maximum, recorded in q1s / q2s),
 // the null residual rWr = max(sum w y^2, floor) (core.py:641-643) and nullh2 = 1 / (1 + delta).
 #include "kernels.h"
 
 namespace flmm {
 
-enum DeltaMode : int { MODE_GOLDEN = 0, MODE_NEWTON = 1, MODE_FIXED = 2 };
+enum DeltaMode : int { MODE_GOLDEN = 0, MODE_NEWTON = 1, MODE_FIXED = 2, MODE_LEARNED = 3, MODE_GRID = 4 };
 
 __device__ inline double grid_value(int i, int G, double lo, double hi) {
   if (G <= 1) return lo;
@@ -214,90 +219,157 @@ __device__ inline void col_slope(const ColLik<KPL>& L, int64_t p, double t, int 
   gp = delta * h + delta * delta * hp;
 }
 
+template <int KPL>
+__device__ inline double golden_bracket(const ColLik<KPL>& L, int64_t p, int i, int G, double lo, double hi, int lane) {
+  const double inv_phi = 2.0 / (1.0 + sqrt(5.0));
+  const double inv_phi2 = inv_phi * inv_phi;
+  double a = grid_value(i > 0 ? i - 1 : 0, G, lo, hi);
+  double b = grid_value(i + 1 < G ? i + 1 : G - 1, G, lo, hi);
+  double c = a + inv_phi2 * (b - a), d = a + inv_phi * (b - a);
+  double fc = col_loss(L, p, c, lane), fd = col_loss(L, p, d, lane);
+  for (int it = 0; it < 50; ++it) {
+    if (fc < fd) b = d; else a = c;
+    c = a + inv_phi2 * (b - a);
+    d = a + inv_phi * (b - a);
+    fc = col_loss(L, p, c, lane);
+    fd = col_loss(L, p, d, lane);
+  }
+  return 0.5 * (a + b);
+}
+
+// safeguarded Newton on L'(t) in the grid bracket [g[i-1], g[i+1]]
+template <int KPL>
+__device__ inline double newton_bracket(const ColLik<KPL>& L, int64_t p, int i, int G, double lo, double hi, int lane) {
+  double a = grid_value(i > 0 ? i - 1 : 0, G, lo, hi);
+  double b = grid_value(i + 1 < G ? i + 1 : G - 1, G, lo, hi);
+  double ga, gpa, gb, gpb;
+  col_slope(L, p, a, lane, ga, gpa);
+  if (!(b > a) || ga >= 0.0) return a;
+  col_slope(L, p, b, lane, gb, gpb);
+  if (gb <= 0.0) return b;
+  double t = grid_value(i, G, lo, hi);
+  if (!(t > a && t < b)) t = 0.5 * (a + b);
+  for (int it = 0; it < 100; ++it) {
+    double g, gp;
+    col_slope(L, p, t, lane, g, gp);
+    if (g == 0.0) break;
+    if (g < 0.0) a = t; else b = t;
+    double tn = t - g / gp;
+    if (!(gp > 0.0) || !(tn > a && tn < b)) tn = 0.5 * (a + b);
+    const bool done = fabs(tn - t) <= 1e-13 * fmax(1.0, fabs(t)) || (b - a) <= 1e-13;
+    t = tn;
+    if (done) break;
+  }
+  return t;
+}
+
 template <int PREC, int KPL>
 __global__ __launch_bounds__(256) void delta_refine_kernel(
     int mode, const float* __restrict__ uy, int64_t P, int64_t p_pad, int64_t kpad, const double* __restrict__ s,
     int64_t neff, const double* __restrict__ s_lam, int64_t nlam, double nco, int G, double lo, double hi,
     const int* __restrict__ idx, double* __restrict__ log_delta,
-    double* __restrict__ rWr, double* __restrict__ h2, void* q1a, void* q1b, void* q2a, void* q2b) {
+    double* __restrict__ rWr, double* __restrict__ h2, void* q1a, void* q1b, void* q2a, void* q2b,
+    float* __restrict__ q1s, float* __restrict__ q2s, double radius, double judge_tol, int* __restrict__ flags) {
   const int lane = threadIdx.x & 63;
   const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (p >= p_pad) return;
   if (p >= P) {  // padding columns: zero operand rows
     for (int64_t k = lane; k < kpad; k += 64) {
       const int64_t off = ((k >> 5) * p_pad + p) * 32 + (k & 31);
-      if constexpr (PREC == PREC_SPLIT_BF16) {
+      if constexpr (is_split(PREC)) {
         ((u16*)q1a)[off] = 0; ((u16*)q1b)[off] = 0; ((u16*)q2a)[off] = 0; ((u16*)q2b)[off] = 0;
       } else {
         ((float*)q1a)[off] = 0.f; ((float*)q2a)[off] = 0.f;
       }
     }
+    if (PREC == PREC_SPLIT_FP16 && lane == 0) { q1s[p] = 1.f; q2s[p] = 1.f; }
     return;
   }
   ColLik<KPL> L{uy, p_pad, s, neff, s_lam, nlam, nco, {}, {}, {}};
   L.load(p, lane);
   double t;
-  const int i = mode == MODE_FIXED ? 0 : idx[p];
-  const double a0 = grid_value(i > 0 ? i - 1 : 0, G, lo, hi);
-  const double b0 = grid_value(i + 1 < G ? i + 1 : G - 1, G, lo, hi);
+  const int i = (mode == MODE_FIXED || mode == MODE_GRID) ? 0 : idx[p];
+  bool need_bracket = mode == MODE_NEWTON;
   if (mode == MODE_FIXED) {
     t = log_delta[p];  // the caller's log delta, used as given (core.py:639)
+  } else if (mode == MODE_GRID) {
+    // refine=False: the first argmin of the FP64 loss over the whole grid (core.py:570-573)
+    double best = INFINITY;
+    int bi = 0;
+    for (int g = 0; g < G; ++g) {
+      const double l = col_loss(L, p, grid_value(g, G, lo, hi), lane);
+      if (l < best) { best = l; bi = g; }
+    }
+    t = grid_value(bi, G, lo, hi);
   } else if (mode == MODE_GOLDEN) {
-    const double inv_phi = 2.0 / (1.0 + sqrt(5.0));
-    const double inv_phi2 = inv_phi * inv_phi;
-    double a = a0, b = b0;
-    double c = a + inv_phi2 * (b - a), d = a + inv_phi * (b - a);
-    double fc = col_loss(L, p, c, lane), fd = col_loss(L, p, d, lane);
-    for (int it = 0; it < 50; ++it) {
-      if (fc < fd) b = d; else a = c;
-      c = a + inv_phi2 * (b - a);
-      d = a + inv_phi * (b - a);
-      fc = col_loss(L, p, c, lane);
-      fd = col_loss(L, p, d, lane);
+    t = golden_bracket(L, p, i, G, lo, hi, lane);
+  } else if (mode == MODE_LEARNED) {
+    // proposal t0 (in log_delta), undamped Newton inside [t0 - radius, t0 + radius]; accepted only if it converges
+    // there to a point whose loss is not worse than the grid optimum by more than judge_tol
+    const double t0 = fmin(fmax(log_delta[p], lo), hi);
+    double tt = t0;
+    bool ok = false;
+    for (int it = 0; it < 8; ++it) {
+      double g, gp;
+      col_slope(L, p, tt, lane, g, gp);
+      if (!(gp > 0.0)) break;
+      double step = -g / gp;
+      step = fmin(fmax(step, -radius), radius);
+      const double tn = fmin(fmax(tt + step, lo), hi);
+      if (fabs(tn - t0) > radius) break;
+      const bool conv = fabs(tn - tt) <= 1e-12 * fmax(1.0, fabs(tt));
+      tt = tn;
+      if (conv) { ok = true; break; }
     }
-    t = 0.5 * (a + b);
-  } else {
-    double a = a0, b = b0, ga, gpa, gb, gpb;
-    col_slope(L, p, a, lane, ga, gpa);
-    col_slope(L, p, b, lane, gb, gpb);
-    if (!(b > a) || ga >= 0.0) {
-      t = a;
-    } else if (gb <= 0.0) {
-      t = b;
+    if (ok) {
+      const double lstar = col_loss(L, p, tt, lane);
+      const double lgrid = col_loss(L, p, grid_value(i, G, lo, hi), lane);
+      ok = lstar <= lgrid + judge_tol;
+    }
+    if (ok) {
+      t = tt;
     } else {
-      t = grid_value(i, G, lo, hi);
-      if (!(t > a && t < b)) t = 0.5 * (a + b);
-      for (int it = 0; it < 100; ++it) {
-        double g, gp;
-        col_slope(L, p, t, lane, g, gp);
-        if (g == 0.0) break;
-        if (g < 0.0) a = t; else b = t;
-        double tn = t - g / gp;
-        if (!(gp > 0.0) || !(tn > a && tn < b)) tn = 0.5 * (a + b);
-        const bool done = fabs(tn - t) <= 1e-13 * fmax(1.0, fabs(t)) || (b - a) <= 1e-13;
-        t = tn;
-        if (done) break;
-      }
+      need_bracket = true;
+      t = 0.0;
     }
+    if (flags && lane == 0) flags[p] = ok ? 0 : 1;
+  } else {
+    t = 0.0;
   }
+  if (need_bracket) t = newton_bracket(L, p, i, G, lo, hi, lane);
   if (mode != MODE_FIXED) t = fmin(fmax(t, lo), hi);
   const double delta = exp(t);
-  double rw = 0.0;
+  // pass 1: the null residual rWr and (split-FP16) the column maxima that set the planes' scales
+  double rw = 0.0, m1 = 0.0, m2 = 0.0;
+  for (int64_t k = lane; k < neff; k += 64) {
+    const double y = (double)uy[((k >> 5) * p_pad + p) * 32 + (k & 31)];
+    const double w = 1.0 / (s[k] + delta);
+    const double q1 = w * y;
+    rw += q1 * y;
+    m1 = fmax(m1, fabs(q1));
+    m2 = fmax(m2, w);
+  }
+  rw = wave_sum(rw);
+  double sc1 = 1.0, sc2 = 1.0;
+  if constexpr (PREC == PREC_SPLIT_FP16) {
+    sc1 = pow2_scale(wave_max(m1));
+    sc2 = pow2_scale(wave_max(m2));
+  }
+  // pass 2: the column-side scan planes Q1 = w o y and Q2 = w
   for (int64_t k = lane; k < kpad; k += 64) {
     double w = 0.0, q1 = 0.0;
     if (k < neff) {
       const double y = (double)uy[((k >> 5) * p_pad + p) * 32 + (k & 31)];
       w = 1.0 / (s[k] + delta);
       q1 = w * y;
-      rw += q1 * y;
     }
     const int64_t off = ((k >> 5) * p_pad + p) * 32 + (k & 31);
-    if constexpr (PREC == PREC_SPLIT_BF16) {
+    if constexpr (is_split(PREC)) {
       u16 hh, ll;
-      split_bf16(q1, hh, ll);
+      split2<PREC>(q1 * sc1, hh, ll);
       ((u16*)q1a)[off] = hh;
       ((u16*)q1b)[off] = ll;
-      split_bf16(w, hh, ll);
+      split2<PREC>(w * sc2, hh, ll);
       ((u16*)q2a)[off] = hh;
       ((u16*)q2b)[off] = ll;
     } else {
@@ -305,11 +377,14 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
       ((float*)q2a)[off] = (float)w;
     }
   }
-  rw = wave_sum(rw);
   if (lane == 0) {
     log_delta[p] = t;
     rWr[p] = fmax(rw, FLOOR64);
     h2[p] = 1.0 / (1.0 + delta);
+    if constexpr (PREC == PREC_SPLIT_FP16) {
+      q1s[p] = (float)sc1;
+      q2s[p] = (float)sc2;
+    }
   }
 }
 
@@ -339,15 +414,19 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
                                const double* s, int64_t neff, const double* s_lam, int64_t nlam, double nco, int G,
                                double lo, double hi, const int* idx,
                                double* log_delta, double* rWr, double* h2, void* q1a, void* q1b, void* q2a,
-                               void* q2b, hipStream_t stream) {
+                               void* q2b, float* q1s, float* q2s, double radius, double judge_tol, int* flags,
+                               hipStream_t stream) {
   const dim3 grid((unsigned)((p_pad + 3) / 4));
   // columns that fit 16 elements per lane (neff, nlam <= 1024) keep them in registers
   const bool regs = neff <= 1024 && nlam <= 1024;
 #define FLMM_REFINE(PR, K)                                                                                          \
   hipLaunchKernelGGL((delta_refine_kernel<PR, K>), grid, dim3(256), 0, stream, mode, uy, P, p_pad, kpad, s, neff,  \
-                     s_lam, nlam, nco, G, lo, hi, idx, log_delta, rWr, h2, q1a, q1b, q2a, q2b)
+                     s_lam, nlam, nco, G, lo, hi, idx, log_delta, rWr, h2, q1a, q1b, q2a, q2b, q1s, q2s, radius,     \
+                     judge_tol, flags)
   if (prec == PREC_SPLIT_BF16) {
     if (regs) FLMM_REFINE(PREC_SPLIT_BF16, 16); else FLMM_REFINE(PREC_SPLIT_BF16, 0);
+  } else if (prec == PREC_SPLIT_FP16) {
+    if (regs) FLMM_REFINE(PREC_SPLIT_FP16, 16); else FLMM_REFINE(PREC_SPLIT_FP16, 0);
   } else {
     if (regs) FLMM_REFINE(PREC_FP32, 16); else FLMM_REFINE(PREC_FP32, 0);
   }

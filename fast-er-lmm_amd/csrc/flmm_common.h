@@ -6,6 +6,9 @@
 // load is a straight coalesced copy, and rows_pad / K are padded with zeros (a zero contributes nothing to a
 // sum over K).  Split-BF16 operands carry two planes, hi = bf16(x) and lo = bf16(x - hi), so hi + lo holds ~16
 // significant bits of x; a product is recovered as hi*hi + hi*lo + lo*hi with FP32 accumulation.
+// Split-FP16 operands are the same construction in IEEE half precision (hi + lo ~ 22 significant bits) on a row
+// scaled by a power of two, row_scale = 2^(14 - ceil(log2(bound))) for a bound on the row's magnitudes, so every
+// entry stays inside FP16's exponent range; consumers divide the accumulated sums by the (exact) scales.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -13,6 +16,7 @@
 namespace flmm {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16;
@@ -20,7 +24,10 @@ typedef unsigned short u16;
 constexpr int KCHUNK = 32;  // K extent of one plane chunk
 constexpr double FLOOR64 = 1e-300;  // positive clamp floor of the FP64 reference arithmetic (core.py:80-86)
 
-enum Prec : int { PREC_SPLIT_BF16 = 0, PREC_FP32 = 1 };
+enum Prec : int { PREC_SPLIT_BF16 = 0, PREC_FP32 = 1, PREC_SPLIT_FP16 = 2 };
+
+// two-byte split precisions share one plane format and tile code path
+__host__ __device__ constexpr bool is_split(int prec) { return prec == PREC_SPLIT_BF16 || prec == PREC_SPLIT_FP16; }
 
 __host__ __device__ inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
@@ -44,9 +51,37 @@ __device__ inline void split_bf16(double x, u16& hi, u16& lo) {
   lo = f32_to_bf16_rne((float)r);
 }
 
+// FP64 value (already scaled) -> (hi, lo) fp16 bit pair with x ~= hi + lo
+__device__ inline void split_f16(double x, u16& hi, u16& lo) {
+  const _Float16 h = (_Float16)(float)x;
+  const _Float16 l = (_Float16)(float)(x - (double)(float)h);
+  hi = __builtin_bit_cast(u16, h);
+  lo = __builtin_bit_cast(u16, l);
+}
+
+// power-of-two scale that maps magnitudes up to `bound` to at most 2^14 (1 for a zero bound)
+__device__ inline double pow2_scale(double bound) {
+  if (!(bound > 0.0) || isinf(bound)) return 1.0;
+  int e;
+  frexp(bound, &e);  // bound = m 2^e, m in [0.5, 1)  ->  bound <= 2^e
+  return ldexp(1.0, 14 - e);
+}
+
+// split x into the plane pair of precision PREC (two-byte formats); x already scaled for FP16
+template <int PREC>
+__device__ inline void split2(double x, u16& hi, u16& lo) {
+  if constexpr (PREC == PREC_SPLIT_FP16) split_f16(x, hi, lo);
+  else split_bf16(x, hi, lo);
+}
+
 __device__ inline double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
   return v;
 }
 __device__ inline float wave_sum(float v) {

@@ -12,6 +12,8 @@
 // Precision modes
 //   PREC_SPLIT_BF16: operands are bf16 (hi, lo) plane pairs; acc += hi*hi + hi*lo + lo*hi through
 //                    v_mfma_f32_32x32x16_bf16 (FP32 accumulation).  The lo*lo term (~2^-16 relative) is dropped.
+//   PREC_SPLIT_FP16: the same three products on power-of-two-scaled fp16 (hi, lo) pairs through
+//                    v_mfma_f32_32x32x16_f16; ~22 significant bits per operand at the same MFMA rate.
 //   PREC_FP32:       operands are fp32 planes; acc += a*b through v_mfma_f32_32x32x2_f32.
 //
 // K-slot mapping.  MFMA sums over the k-slots of its A and B fragments pairwise (A lane l, element e with B
@@ -40,6 +42,8 @@ struct PlaneTraits<PREC_SPLIT_BF16> {
   static __device__ inline int swz(int row) { return (row >> 2) & 3; }
 };
 template <>
+struct PlaneTraits<PREC_SPLIT_FP16> : PlaneTraits<PREC_SPLIT_BF16> {};
+template <>
 struct PlaneTraits<PREC_FP32> {
   static constexpr int NSUB = 1;
   static constexpr int ESIZE = 4;
@@ -47,6 +51,16 @@ struct PlaneTraits<PREC_FP32> {
   static constexpr int SPR = 8;
   static __device__ inline int swz(int row) { return (row >> 1) & 7; }
 };
+
+// one split product term: v_mfma_f32_32x32x16_{bf16,f16} on the raw 16-byte fragments
+template <int PREC>
+__device__ inline f32x16 mfma_split(bf16x8 a, bf16x8 b, f32x16 c) {
+  if constexpr (PREC == PREC_SPLIT_FP16)
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0, 0,
+                                                   0);
+  else
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
 
 template <int PREC, int NG>
 struct TileArgs {
@@ -146,7 +160,7 @@ __device__ inline void tile_mainloop(const TileArgs<PREC, NG>& args, int64_t a_r
     if (kc + 1 < args.nchunks) gload(kc + 1);
 
     const int h = lane >> 5;
-    if constexpr (PREC == PREC_SPLIT_BF16) {
+    if constexpr (is_split(PREC)) {
 #pragma unroll
       for (int ks = 0; ks < 2; ++ks) {
         const int seg = h * 2 + ks;
@@ -176,9 +190,9 @@ __device__ inline void tile_mainloop(const TileArgs<PREC, NG>& args, int64_t a_r
 #pragma unroll
             for (int bj = 0; bj < 2; ++bj) {
               f32x16 c = acc.c[g][bi][bj];
-              c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[bi][0], bfr[bj][0], c, 0, 0, 0);
-              c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[bi][0], bfr[bj][1], c, 0, 0, 0);
-              c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[bi][1], bfr[bj][0], c, 0, 0, 0);
+              c = mfma_split<PREC>(af[bi][0], bfr[bj][0], c);
+              c = mfma_split<PREC>(af[bi][0], bfr[bj][1], c);
+              c = mfma_split<PREC>(af[bi][1], bfr[bj][0], c);
               acc.c[g][bi][bj] = c;
             }
         }

@@ -1,4 +1,5 @@
-// Host-side launch wrappers of the gfx950 kernels (defined in pack.hip, rotate.hip, delta.hip, scan.hip).
+// Host-side launch wrappers of the gfx950 kernels (pack.hip, rotate.hip, delta.hip, scan.hip, scan2.hip,
+// standardise.hip, pvalue.hip, mlp.hip).
 #pragma once
 #include "gemm_tile.h"
 
@@ -13,11 +14,22 @@ struct ScanOut {
   int64_t ld;  // = n_real
 };
 
+// per-row power-of-two scales of the four scan planes (split-FP16; all NULL otherwise)
+struct ScanScales {
+  const float* s1;
+  const float* s2;
+  const float* q1;
+  const float* q2;
+};
+
 hipError_t launch_pack_planes(int prec, const double* src, int64_t n_rows, int64_t n_k, int64_t row_stride,
-                              int64_t k_stride, const int64_t* row_index, void* out_a, void* out_b, int64_t rows_pad,
-                              int64_t kpad, hipStream_t stream);
+                              int64_t k_stride, const int64_t* row_index, const double* row_norm, void* out_a,
+                              void* out_b, float* row_scale, int64_t rows_pad, int64_t kpad, hipStream_t stream);
+hipError_t launch_row_norms(const double* src, int64_t n_rows, int64_t n_k, int64_t row_stride, int64_t k_stride,
+                            const int64_t* row_index, double* norm, hipStream_t stream);
 template <int PREC>
-hipError_t launch_rotate(int mode, const TileArgs<PREC, 1>& args, void* o0, void* o1, void* o2, void* o3,
+hipError_t launch_rotate(int mode, const TileArgs<PREC, 1>& args, const float* a_scale, const float* b_scale,
+                         void* o0, void* o1, void* o2, void* o3, float* out_scale1, float* out_scale2,
                          hipStream_t stream);
 hipError_t launch_delta_weights(const double* s, int64_t neff, const double* s_lam, int64_t nlam, int64_t kpad, int G,
                                 double lo, double hi, float* wplane, float* lam, hipStream_t stream);
@@ -25,22 +37,42 @@ hipError_t launch_delta_grid(const float* wplane, const float* lam, int G, doubl
                              int64_t p_pad, int64_t kpad, int* idx, hipStream_t stream);
 hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_pad, int64_t kpad,
                                const double* s, int64_t neff, const double* s_lam, int64_t nlam, double nco, int G,
-                               double lo, double hi, const int* idx,
-                               double* log_delta, double* rWr, double* h2, void* q1a, void* q1b, void* q2a,
-                               void* q2b, hipStream_t stream);
+                               double lo, double hi, const int* idx, double* log_delta, double* rWr, double* h2,
+                               void* q1a, void* q1b, void* q2a, void* q2b, float* q1s, float* q2s, double radius,
+                               double judge_tol, int* flags, hipStream_t stream);
 template <int PREC>
-hipError_t launch_wald_scan(const TileArgs<PREC, 2>& args, int64_t Mc, int64_t P, int64_t n_real,
-                            const double* rWr, const double* h2, double nu, const int64_t* row_index,
+hipError_t launch_wald_scan(const TileArgs<PREC, 2>& args, const ScanScales& sc, int64_t Mc, int64_t P,
+                            int64_t n_real, const double* rWr, const double* h2, double nu, const int64_t* row_index,
                             const ScanOut& out, double* pmax, hipStream_t stream);
-hipError_t launch_standardise(int dtype, const void* src, int64_t n, int64_t m, int64_t lds, double* dst, int64_t ldd,
-                              double* mean_out, double* std_out, hipStream_t stream);
+template <int PREC>
+hipError_t launch_wald_scan_wide(const TileArgs<PREC, 2>& args, const ScanScales& sc, int64_t Mc, int64_t P,
+                                 int64_t n_real, const double* rWr, const double* h2, double nu,
+                                 const int64_t* row_index, const ScanOut& out, double* pmax, hipStream_t stream);
 hipError_t launch_max_reduce(const double* pmax, int64_t n_mtiles, int64_t ld, int64_t P, double* max_F,
                              hipStream_t stream);
-
-}  // namespace flmm
-
-namespace flmm {
+hipError_t launch_standardise(int dtype, const void* src, int64_t n, int64_t m, int64_t lds, double* dst, int64_t ldd,
+                              double* mean_out, double* std_out, hipStream_t stream);
 hipError_t launch_neglog10p(const double* F, int64_t n, double df2, double* out, hipStream_t stream);
 hipError_t launch_effect_size(const double* beta, int64_t M, int64_t n_real, const double* g_var,
                               const double* p_var, double* out, hipStream_t stream);
+hipError_t launch_mlp_features(const float* uy, int64_t P, int64_t p_pad, int64_t neff, const int* bin_of_k,
+                               float logN, float C, float* feat, hipStream_t stream);
+hipError_t launch_mlp_dense_fwd(const float* in, int64_t B, int n_in, const float* W, const float* bias, int n_out,
+                                int act, float* out, hipStream_t stream);
+hipError_t launch_mlp_dense_bwd(const float* dout_in, float* dz, const float* out, const float* in, int64_t B,
+                                int n_in, int n_out, int act, const float* W, float* din, float* dW, float* db,
+                                hipStream_t stream);
+hipError_t launch_mlp_mse_grad(const float* pred, const float* target, int64_t B, float* dpred, hipStream_t stream);
+hipError_t launch_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float beta1, float beta2,
+                       float eps, int t, hipStream_t stream);
+
+// scan epilogue: unscaled (sWy, sWs) of accumulator pair (a, d) for SNP row m and column p
+__device__ inline void unscale_sums(const ScanScales& sc, int64_t m, double inv_q1, double inv_q2, double& a,
+                                    double& d) {
+  if (sc.s1) {
+    a *= inv_q1 / (double)sc.s1[m];
+    d *= inv_q2 / (double)sc.s2[m];
+  }
+}
+
 }  // namespace flmm

@@ -1,10 +1,11 @@
-// FP64 matrix -> MFMA operand planes (bf16 hi/lo pair or fp32), with optional row gather.
+// FP64 matrix -> MFMA operand planes (bf16 / fp16 hi-lo pair, or fp32), with optional row gather; row norms.
 //
 // Source element (row r, k) is src[src_row(r) * row_stride + k * k_stride], src_row(r) = row_index ? row_index[r]
 // : r, so the same kernel packs row-major SNP blocks, the transposed view of an (N, P) phenotype matrix and the
 // kept eigenvector columns of U.  Output rows [n_rows, rows_pad) and k in [n_k, kpad) are written as zeros.
 // A 64 x 64 tile goes through LDS so that both the read (along whichever source stride is 1) and the plane
-// write (32 consecutive k per row) are coalesced.
+// write (32 consecutive k per row) are coalesced.  Split-FP16 rows are multiplied by the power-of-two scale of
+// their L2 norm (row_norm, from flmm_row_norms), which bounds every entry, and the scale is recorded.
 #include "kernels.h"
 
 namespace flmm {
@@ -12,8 +13,10 @@ namespace flmm {
 template <int PREC>
 __global__ __launch_bounds__(256) void pack_planes_kernel(const double* __restrict__ src, int64_t n_rows, int64_t n_k,
                                                           int64_t row_stride, int64_t k_stride,
-                                                          const int64_t* __restrict__ row_index, void* out_a,
-                                                          void* out_b, int64_t rows_pad, int64_t kpad) {
+                                                          const int64_t* __restrict__ row_index,
+                                                          const double* __restrict__ row_norm, void* out_a,
+                                                          void* out_b, float* __restrict__ row_scale,
+                                                          int64_t rows_pad, int64_t kpad) {
   __shared__ double tile[64][65];
   const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
   const int tid = threadIdx.x;
@@ -36,15 +39,20 @@ __global__ __launch_bounds__(256) void pack_planes_kernel(const double* __restri
   const int kq = (tid & 3) * 16;
   const int64_t r = r0 + rr;
   if (r >= rows_pad) return;
+  double sc = 1.0;
+  if constexpr (PREC == PREC_SPLIT_FP16) {
+    sc = (r < n_rows) ? pow2_scale(row_norm[r]) : 1.0;
+    if (blockIdx.y == 0 && (tid & 3) == 0) row_scale[r] = (float)sc;
+  }
 #pragma unroll
   for (int e = 0; e < 16; ++e) {
     const int64_t k = k0 + kq + e;
     if (k >= kpad) break;
     const int64_t off = ((k >> 5) * rows_pad + r) * 32 + (k & 31);
     const double x = tile[rr][kq + e];
-    if constexpr (PREC == PREC_SPLIT_BF16) {
+    if constexpr (is_split(PREC)) {
       u16 h, l;
-      split_bf16(x, h, l);
+      split2<PREC>(x * sc, h, l);
       ((u16*)out_a)[off] = h;
       ((u16*)out_b)[off] = l;
     } else {
@@ -53,16 +61,56 @@ __global__ __launch_bounds__(256) void pack_planes_kernel(const double* __restri
   }
 }
 
+// norm[r] = sqrt(sum_k src(r, k)^2); thread per row when rows are the unit-stride direction, else wave per row
+__global__ __launch_bounds__(256) void row_norms_kernel(const double* __restrict__ src, int64_t n_rows, int64_t n_k,
+                                                        int64_t row_stride, int64_t k_stride,
+                                                        const int64_t* __restrict__ row_index,
+                                                        double* __restrict__ norm) {
+  if (row_stride == 1 && k_stride != 1) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    const int64_t sr = row_index ? row_index[r] : r;
+    double s = 0.0;
+    for (int64_t k = 0; k < n_k; ++k) {
+      const double x = src[sr * row_stride + k * k_stride];
+      s += x * x;
+    }
+    norm[r] = sqrt(s);
+  } else {
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n_rows) return;
+    const int64_t sr = row_index ? row_index[r] : r;
+    double s = 0.0;
+    for (int64_t k = lane; k < n_k; k += 64) {
+      const double x = src[sr * row_stride + k * k_stride];
+      s += x * x;
+    }
+    s = wave_sum(s);
+    if (lane == 0) norm[r] = sqrt(s);
+  }
+}
+
 hipError_t launch_pack_planes(int prec, const double* src, int64_t n_rows, int64_t n_k, int64_t row_stride,
-                              int64_t k_stride, const int64_t* row_index, void* out_a, void* out_b, int64_t rows_pad,
-                              int64_t kpad, hipStream_t stream) {
+                              int64_t k_stride, const int64_t* row_index, const double* row_norm, void* out_a,
+                              void* out_b, float* row_scale, int64_t rows_pad, int64_t kpad, hipStream_t stream) {
   const dim3 grid((unsigned)((rows_pad + 63) / 64), (unsigned)((kpad + 63) / 64));
-  if (prec == PREC_SPLIT_BF16)
-    hipLaunchKernelGGL(pack_planes_kernel<PREC_SPLIT_BF16>, grid, dim3(256), 0, stream, src, n_rows, n_k, row_stride,
-                       k_stride, row_index, out_a, out_b, rows_pad, kpad);
-  else
-    hipLaunchKernelGGL(pack_planes_kernel<PREC_FP32>, grid, dim3(256), 0, stream, src, n_rows, n_k, row_stride,
-                       k_stride, row_index, out_a, out_b, rows_pad, kpad);
+#define FLMM_PACK(PR)                                                                                              \
+  hipLaunchKernelGGL(pack_planes_kernel<PR>, grid, dim3(256), 0, stream, src, n_rows, n_k, row_stride, k_stride,    \
+                     row_index, row_norm, out_a, out_b, row_scale, rows_pad, kpad)
+  if (prec == PREC_SPLIT_BF16) FLMM_PACK(PREC_SPLIT_BF16);
+  else if (prec == PREC_SPLIT_FP16) FLMM_PACK(PREC_SPLIT_FP16);
+  else FLMM_PACK(PREC_FP32);
+#undef FLMM_PACK
+  return hipGetLastError();
+}
+
+hipError_t launch_row_norms(const double* src, int64_t n_rows, int64_t n_k, int64_t row_stride, int64_t k_stride,
+                            const int64_t* row_index, double* norm, hipStream_t stream) {
+  const bool thread_per_row = row_stride == 1 && k_stride != 1;
+  const dim3 grid((unsigned)(thread_per_row ? (n_rows + 255) / 256 : (n_rows + 3) / 4));
+  hipLaunchKernelGGL(row_norms_kernel, grid, dim3(256), 0, stream, src, n_rows, n_k, row_stride, k_stride, row_index,
+                     norm);
   return hipGetLastError();
 }
 

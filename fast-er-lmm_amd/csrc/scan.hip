@@ -26,7 +26,7 @@ __device__ inline double wald_f(double a, double d, double rwr, double nu) {
 
 template <int PREC>
 __global__ __launch_bounds__(NTHREADS, 1) void wald_scan_kernel(
-    TileArgs<PREC, 2> args, int64_t n_mtiles, int64_t n_tiles, int64_t Mc, int64_t P, int64_t n_real,
+    TileArgs<PREC, 2> args, ScanScales sc, int64_t n_mtiles, int64_t n_tiles, int64_t Mc, int64_t P, int64_t n_real,
     const double* __restrict__ rWr, const double* __restrict__ h2, double nu, const int64_t* __restrict__ row_index,
     ScanOut out, double* __restrict__ pmax, int64_t pmax_ld) {
   extern __shared__ char lds[];
@@ -53,6 +53,8 @@ __global__ __launch_bounds__(NTHREADS, 1) void wald_scan_kernel(
     const double rwr = pv ? rWr[p] : 1.0;
     const bool real = p < n_real;
     const double hh = real ? h2[p] : 0.0;
+    const double inv_q1 = (sc.q1 && pv) ? 1.0 / (double)sc.q1[p] : 1.0;
+    const double inv_q2 = (sc.q2 && pv) ? 1.0 / (double)sc.q2[p] : 1.0;
     double ba = 0.0, bd = 1.0;  // best (sWy, sWs) so far: t = ba^2 / bd
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi) {
@@ -60,8 +62,10 @@ __global__ __launch_bounds__(NTHREADS, 1) void wald_scan_kernel(
       for (int v = 0; v < 16; ++v) {
         const int64_t m = m0 + acc_row(wm, bi, lane, v);
         if (m < Mc) {
-          const double a = (double)acc.c[0][bi][bj][v];
-          const double d = fmax((double)acc.c[1][bi][bj][v], FLOOR64);
+          double a = (double)acc.c[0][bi][bj][v];
+          double d = (double)acc.c[1][bi][bj][v];
+          unscale_sums(sc, m, inv_q1, inv_q2, a, d);
+          d = fmax(d, FLOOR64);
           // t = a^2/d > ba^2/bd  <=>  a^2 bd > ba^2 d   (d, bd > 0)
           if (a * a * bd > ba * ba * d) { ba = a; bd = d; }
           if (real) {
@@ -108,14 +112,14 @@ __global__ void max_reduce_kernel(const double* __restrict__ pmax, int64_t n_mti
 }
 
 template <int PREC>
-hipError_t launch_wald_scan(const TileArgs<PREC, 2>& args, int64_t Mc, int64_t P, int64_t n_real,
+hipError_t launch_wald_scan(const TileArgs<PREC, 2>& args, const ScanScales& sc, int64_t Mc, int64_t P, int64_t n_real,
                             const double* rWr, const double* h2, double nu, const int64_t* row_index,
                             const ScanOut& out, double* pmax, hipStream_t stream) {
   const int64_t n_mtiles = (Mc + TILE - 1) / TILE;
   const int64_t n_ptiles = (P + TILE - 1) / TILE;
   const int64_t n_tiles = n_mtiles * n_ptiles;
   const int lds = tile_lds_bytes<PREC, 2>();
-  hipLaunchKernelGGL(wald_scan_kernel<PREC>, dim3((unsigned)n_tiles), dim3(NTHREADS), lds, stream, args, n_mtiles,
+  hipLaunchKernelGGL(wald_scan_kernel<PREC>, dim3((unsigned)n_tiles), dim3(NTHREADS), lds, stream, args, sc, n_mtiles,
                      n_tiles, Mc, P, n_real, rWr, h2, nu, row_index, out, pmax, args.b_rows_pad);
   return hipGetLastError();
 }
@@ -127,10 +131,13 @@ hipError_t launch_max_reduce(const double* pmax, int64_t n_mtiles, int64_t ld, i
   return hipGetLastError();
 }
 
-template hipError_t launch_wald_scan<PREC_SPLIT_BF16>(const TileArgs<PREC_SPLIT_BF16, 2>&, int64_t, int64_t,
+template hipError_t launch_wald_scan<PREC_SPLIT_FP16>(const TileArgs<PREC_SPLIT_FP16, 2>&, const ScanScales&, int64_t,
+                                                       int64_t, int64_t, const double*, const double*, double,
+                                                       const int64_t*, const ScanOut&, double*, hipStream_t);
+template hipError_t launch_wald_scan<PREC_SPLIT_BF16>(const TileArgs<PREC_SPLIT_BF16, 2>&, const ScanScales&, int64_t, int64_t,
                                                        int64_t, const double*, const double*, double,
                                                        const int64_t*, const ScanOut&, double*, hipStream_t);
-template hipError_t launch_wald_scan<PREC_FP32>(const TileArgs<PREC_FP32, 2>&, int64_t, int64_t, int64_t,
+template hipError_t launch_wald_scan<PREC_FP32>(const TileArgs<PREC_FP32, 2>&, const ScanScales&, int64_t, int64_t, int64_t,
                                                  const double*, const double*, double, const int64_t*,
                                                  const ScanOut&, double*, hipStream_t);
 

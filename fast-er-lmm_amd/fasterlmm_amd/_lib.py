@@ -14,11 +14,13 @@ LIB_PATH = Path(__file__).resolve().parent / "libfasterlmm_amd.so"
 
 PREC_SPLIT_BF16 = 0
 PREC_FP32 = 1
+PREC_SPLIT_FP16 = 2
 ROTATE_UY = 0
 ROTATE_US = 1
 DELTA_GOLDEN = 0
 DELTA_NEWTON = 1
 DELTA_FIXED = 2
+DELTA_GRID = 4
 
 _P = C.c_void_p
 _I64 = C.c_int64
@@ -29,15 +31,23 @@ _D = C.c_double
 SIGNATURES: dict[str, list] = {
     "flmm_abi_version": [],
     "flmm_last_error": [],
-    "flmm_pack_planes": [_I, _P, _I64, _I64, _I64, _I64, _P, _P, _P, _I64, _I64, _P],
-    "flmm_rotate": [_I, _I, _P, _P, _I64, _P, _P, _I64, _I64, _P, _P, _P, _P, _P],
+    "flmm_standardise": [_I, _P, _I64, _I64, _I64, _P, _I64, _P, _P, _P],
+    "flmm_row_norms": [_P, _I64, _I64, _I64, _I64, _P, _P, _P],
+    "flmm_pack_planes": [_I, _P, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _I64, _I64, _P],
+    "flmm_rotate": [_I, _I, _P, _P, _P, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P],
     "flmm_delta_weights": [_P, _I64, _P, _I64, _I64, _I, _D, _D, _P, _P, _P],
     "flmm_delta_grid": [_P, _P, _I, _D, _P, _I64, _I64, _I64, _P, _P],
     "flmm_delta_refine": [_I, _I, _P, _I64, _I64, _I64, _P, _I64, _P, _I64, _D, _I, _D, _D, _P, _P, _P, _P, _P,
-                          _P, _P, _P, _P],
-    "flmm_wald_scan": [_I, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P, _P, _D, _P, _P,
-                       _P, _P, _P, _P, _P, _P],
-    "flmm_standardise": [_I, _P, _I64, _I64, _I64, _P, _I64, _P, _P, _P],
+                          _P, _P, _P, _P, _P, _P],
+    "flmm_delta_refine_learned": [_I, _P, _I64, _I64, _I64, _P, _I64, _P, _I64, _D, _I, _D, _D, _P, _D, _D, _P, _P,
+                                  _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "flmm_mlp_features": [_P, _I64, _I64, _I64, _P, _D, _D, _P, _P],
+    "flmm_mlp_dense_fwd": [_P, _I64, _I, _P, _P, _I, _I, _P, _P],
+    "flmm_mlp_dense_bwd": [_P, _P, _P, _P, _I64, _I, _I, _I, _P, _P, _P, _P, _P],
+    "flmm_mlp_mse_grad": [_P, _P, _I64, _P, _P],
+    "flmm_adam": [_P, _P, _P, _P, _I64, _D, _D, _D, _D, _I, _P],
+    "flmm_wald_scan": [_I, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P,
+                       _P, _D, _P, _P, _P, _P, _P, _P, _P, _P],
     "flmm_max_reduce": [_P, _I64, _I64, _I64, _P, _P],
     "flmm_neglog10p_f1": [_P, _I64, _D, _P, _P],
     "flmm_effect_size": [_P, _I64, _I64, _P, _P, _P, _P],

@@ -35,7 +35,7 @@ import torch
 from torch import Tensor
 
 from . import _lib, engine
-from .engine import TILE, KCHUNK, rup
+from .engine import TILE, PTILE, KCHUNK, rup
 
 
 # SCAN ARRAYS -------
@@ -146,7 +146,7 @@ class _Basis:
     def rotate_columns(self, Y: Tensor) -> None:
         """UY plane of B^T Y for Y (N, P) FP64."""
         N, P = Y.shape
-        self.P, self.p_pad = P, rup(max(P, 1), TILE)
+        self.P, self.p_pad = P, rup(max(P, 1), PTILE)
         ypl = engine.pack(self.prec, Y.contiguous(), P, N, 1, P, self.p_pad, self.npad)
         self.uy = engine.rotate_uy(ypl, self.bt)
 
@@ -165,10 +165,18 @@ class _Basis:
 
     def fit(self, n_grid: int, lo: float, hi: float, refine: bool, *, projected_loss: bool = True):
         s_lam, nco = self._loss(projected_loss)
-        idx = engine.delta_grid_idx(self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam, nco, n_grid, lo, hi)
+        if refine and engine.settings.delta == "learned":
+            from . import learned
+            fit = learned.refine_learned(self, learned.model(), n_grid, lo, hi, n=self.N, c=self.N - self.neff,
+                                         projected_loss=projected_loss)
+            self.last_fallback = fit.fallback
+            return fit.log_delta, fit.delta
         if not refine:
-            grid = torch.from_numpy(engine.grid_values(n_grid, lo, hi)).to(self.s.device)
-            return grid[idx.long()], None
+            d = engine.delta_refine(self.prec, _lib.DELTA_GRID, self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam,
+                                    nco, n_grid, lo, hi, None)
+            self.delta = d
+            return d.log_delta, d
+        idx = engine.delta_grid_idx(self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam, nco, n_grid, lo, hi)
         d = engine.delta_refine(self.prec, engine.delta_code(), self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam,
                                 nco, n_grid, lo, hi, idx)
         self.delta = d
@@ -286,7 +294,7 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
     # the basis is orthogonal to X, so B^T (Y - X X^+ Y) = B^T Y; the OLS regress-out keeps the compat
     # path's arithmetic (core.py:517) and costs one pass
     Y_r = (Yd - Xd @ (Xpinv @ Yd)).contiguous()
-    ypl = engine.pack(prec, Y_r, P, N, 1, P, rup(max(P, 1), TILE), rup(N, KCHUNK))
+    ypl = engine.pack(prec, Y_r, P, N, 1, P, rup(max(P, 1), PTILE), rup(N, KCHUNK))
     del Y_r
     jitter = 1e-6 if plain else 0.0
     eye = torch.eye(N, dtype=torch.float64, device=dev)

@@ -24,6 +24,7 @@ import torch
 from . import _lib
 
 TILE = 128
+PTILE = 256  # column padding: the wide scan variant tiles columns by 256
 KCHUNK = 32
 
 
@@ -33,8 +34,10 @@ def rup(x: int, m: int) -> int:
 
 @dataclass
 class Settings:
-    precision: str = "split_bf16"  # "split_bf16" (3-product split-BF16 MFMA) or "fp32" (FP32 MFMA)
-    delta: str = "newton"          # "newton" (exact local minimiser) or "golden" (the reference's 50 steps)
+    precision: str = "split_fp16"  # "split_fp16" (scaled FP16 hi+lo, 3 MFMA products), "split_bf16" (BF16 hi+lo,
+                                   # 3 products) or "fp32" (FP32 MFMA)
+    delta: str = "newton"          # "newton" (exact local minimiser), "golden" (the reference's 50 steps) or
+                                   # "learned" (MLP proposal + judged Newton, grid as fallback and judge)
 
 
 settings = Settings()
@@ -57,6 +60,8 @@ def configure(**kw):
 
 def prec_code(precision: str | None = None) -> int:
     p = settings.precision if precision is None else precision
+    if p == "split_fp16":
+        return _lib.PREC_SPLIT_FP16
     if p == "split_bf16":
         return _lib.PREC_SPLIT_BF16
     if p == "fp32":
@@ -70,19 +75,25 @@ def delta_code(mode: str | None = None) -> int:
         return _lib.DELTA_NEWTON
     if m == "golden":
         return _lib.DELTA_GOLDEN
+    if m == "learned":  # the learned path's fallback / fixed refinements are Newton in the grid bracket
+        return _lib.DELTA_NEWTON
     raise ValueError(f"unknown delta mode {m!r}")
 
 
 class Planes:
-    """One operand matrix as MFMA planes: (hi, lo) bf16 pair or a single fp32 plane."""
+    """One operand matrix as MFMA planes: a (hi, lo) bf16 or fp16 pair (fp16 rows carry a power-of-two scale)
+    or a single fp32 plane."""
 
     def __init__(self, prec: int, rows_pad: int, kpad: int, dev, *, fp32_only: bool = False):
         self.prec = _lib.PREC_FP32 if fp32_only else prec
         self.rows_pad, self.kpad = rows_pad, kpad
         n = rows_pad * kpad
-        if self.prec == _lib.PREC_SPLIT_BF16:
+        self.scale = None
+        if self.prec in (_lib.PREC_SPLIT_BF16, _lib.PREC_SPLIT_FP16):
             self.hi = torch.empty(n, dtype=torch.int16, device=dev)
             self.lo = torch.empty(n, dtype=torch.int16, device=dev)
+            if self.prec == _lib.PREC_SPLIT_FP16:
+                self.scale = torch.empty(rows_pad, dtype=torch.float32, device=dev)
         else:
             self.hi = torch.empty(n, dtype=torch.float32, device=dev)
             self.lo = None
@@ -91,15 +102,23 @@ class Planes:
     def ptrs(self):
         return self.hi.data_ptr(), (self.lo.data_ptr() if self.lo is not None else None)
 
+    @property
+    def scale_ptr(self):
+        return None if self.scale is None else self.scale.data_ptr()
+
     def to_dense(self, rows: int, k: int) -> torch.Tensor:
-        """(rows, k) FP64 view of the stored values (hi + lo for split planes)."""
+        """(rows, k) FP64 view of the stored values (hi + lo, unscaled, for split planes)."""
         def one(t, dt):
             v = t.view(dt) if dt is not None else t
             return v.reshape(self.kpad // KCHUNK, self.rows_pad, KCHUNK).permute(1, 0, 2).reshape(
                 self.rows_pad, self.kpad)[:rows, :k]
         if self.prec == _lib.PREC_FP32:
             return one(self.hi, None).double()
-        return one(self.hi, torch.bfloat16).double() + one(self.lo, torch.bfloat16).double()
+        dt = torch.float16 if self.prec == _lib.PREC_SPLIT_FP16 else torch.bfloat16
+        x = one(self.hi, dt).double() + one(self.lo, dt).double()
+        if self.scale is not None:
+            x = x / self.scale[:rows].double().unsqueeze(1)
+        return x
 
 
 def fp32_plane_to_dense(plane: torch.Tensor, rows_pad: int, kpad: int, rows: int, k: int) -> torch.Tensor:
@@ -113,8 +132,13 @@ def pack(prec: int, src: torch.Tensor, n_rows: int, n_k: int, row_stride: int, k
     pl = Planes(prec, rows_pad, kpad, src.device)
     base = src.data_ptr() + 8 * offset
     hi, lo = pl.ptrs
-    _lib.call("flmm_pack_planes", pl.prec, base, n_rows, n_k, row_stride, k_stride, _lib.ptr(row_index), hi, lo,
-              rows_pad, kpad, _lib.stream())
+    norm = None
+    st = _lib.stream()
+    if pl.prec == _lib.PREC_SPLIT_FP16:
+        norm = torch.empty(max(n_rows, 1), dtype=torch.float64, device=src.device)
+        _lib.call("flmm_row_norms", base, n_rows, n_k, row_stride, k_stride, _lib.ptr(row_index), norm.data_ptr(), st)
+    _lib.call("flmm_pack_planes", pl.prec, base, n_rows, n_k, row_stride, k_stride, _lib.ptr(row_index),
+              _lib.ptr(norm), hi, lo, pl.scale_ptr, rows_pad, kpad, st)
     return pl
 
 
@@ -123,8 +147,8 @@ def rotate_uy(a: Planes, b: Planes) -> torch.Tensor:
     out = torch.empty(a.rows_pad * b.rows_pad, dtype=torch.float32, device=a.hi.device)
     ah, al = a.ptrs
     bh, bl = b.ptrs
-    _lib.call("flmm_rotate", a.prec, _lib.ROTATE_UY, ah, al, a.rows_pad, bh, bl, b.rows_pad, a.kpad, out.data_ptr(),
-              None, None, None, _lib.stream())
+    _lib.call("flmm_rotate", a.prec, _lib.ROTATE_UY, ah, al, a.scale_ptr, a.rows_pad, bh, bl, b.scale_ptr,
+              b.rows_pad, a.kpad, out.data_ptr(), None, None, None, None, None, _lib.stream())
     return out
 
 
@@ -135,12 +159,12 @@ def rotate_us(a: Planes, b: Planes) -> tuple[Planes, Planes]:
     s2 = Planes(a.prec, a.rows_pad, b.rows_pad, dev)
     ah, al = a.ptrs
     bh, bl = b.ptrs
-    if a.prec == _lib.PREC_SPLIT_BF16:
+    if a.prec != _lib.PREC_FP32:
         outs = (s1.hi.data_ptr(), s1.lo.data_ptr(), s2.hi.data_ptr(), s2.lo.data_ptr())
     else:
         outs = (s1.hi.data_ptr(), s2.hi.data_ptr(), None, None)
-    _lib.call("flmm_rotate", a.prec, _lib.ROTATE_US, ah, al, a.rows_pad, bh, bl, b.rows_pad, a.kpad, *outs,
-              _lib.stream())
+    _lib.call("flmm_rotate", a.prec, _lib.ROTATE_US, ah, al, a.scale_ptr, a.rows_pad, bh, bl, b.scale_ptr,
+              b.rows_pad, a.kpad, *outs, s1.scale_ptr, s2.scale_ptr, _lib.stream())
     return s1, s2
 
 
@@ -197,7 +221,7 @@ def delta_refine(prec: int, mode: int, uy: torch.Tensor, P: int, p_pad: int, kpa
     q2h, q2l = q2.ptrs
     _lib.call("flmm_delta_refine", prec, mode, uy.data_ptr(), P, p_pad, kpad, s.data_ptr(), s.numel(),
               s_lam.data_ptr(), s_lam.numel(), float(nco), n_grid, lo, hi, _lib.ptr(idx), log_delta.data_ptr(),
-              rWr.data_ptr(), h2.data_ptr(), q1h, q1l, q2h, q2l, _lib.stream())
+              rWr.data_ptr(), h2.data_ptr(), q1h, q1l, q2h, q2l, q1.scale_ptr, q2.scale_ptr, _lib.stream())
     return DeltaResult(log_delta[:P], rWr[:P], h2[:P], q1, q2, idx, int(s.numel()))
 
 
@@ -262,8 +286,9 @@ def wald_scan(s1: Planes, s2: Planes, d: DeltaResult, mc: int, P: int, n_real: i
     if scan_timer.enabled:
         ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
         ev[0].record()
-    _lib.call("flmm_wald_scan", s1.prec, s1h, s1l, s2h, s2l, mc_pad, q1h, q1l, q2h, q2l, p_pad, s1.kpad, mc, P, n_real,
-              d.rWr.data_ptr(), d.h2.data_ptr(), float(nu), row_index.data_ptr(), *optr, pmax.data_ptr(), st)
+    _lib.call("flmm_wald_scan", s1.prec, s1h, s1l, s2h, s2l, s1.scale_ptr, s2.scale_ptr, mc_pad, q1h, q1l, q2h, q2l,
+              d.q1.scale_ptr, d.q2.scale_ptr, p_pad, s1.kpad, mc, P, n_real, d.rWr.data_ptr(), d.h2.data_ptr(),
+              float(nu), row_index.data_ptr(), *optr, pmax.data_ptr(), st)
     if ev is not None:
         ev[1].record()
         scan_timer.launches.append((ev[0], ev[1], mc * P, s1.kpad, d.neff))

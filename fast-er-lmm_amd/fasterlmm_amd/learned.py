@@ -1,0 +1,323 @@
+"""Learned log-delta proposal: spectral features of a rotated column -> small FP32 MLP -> log delta, finished by
+a short Newton refinement and judged against the full grid (which stays as fallback and judge).
+
+Everything numeric runs in the gfx950 kernels of libfasterlmm_amd.so: features (flmm_mlp_features), the dense
+layers forward and backward (flmm_mlp_dense_fwd / _bwd), the MSE gradient and Adam (flmm_adam), and the judged
+refinement (flmm_delta_refine_learned).  Training data are simulated directly in a kinship eigenbasis: for
+eigenvalues s and heritability h2, the rotated phenotype has independent entries y_k ~ N(0, sg2 s_k + se2), which
+is exactly the LMM's distribution; labels are the grid + Newton optimum of the same likelihood.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import _lib, engine
+from .engine import TILE, PTILE, KCHUNK, rup
+
+N_IN, H = 34, 128
+NBIN = 32
+EDGES = np.logspace(-2.0, 3.0, NBIN + 1)  # absolute eigenvalue bins (a kinship Z Z^T / M has mean eigenvalue 1)
+SIZES = [(H, N_IN), (H,), (H, H), (H,), (1, H), (1,)]
+OFFS = np.cumsum([0] + [int(np.prod(s)) for s in SIZES])
+N_PARAMS = int(OFFS[-1])
+
+
+def bins_of(s: np.ndarray) -> np.ndarray:
+    return np.clip(np.searchsorted(EDGES, np.asarray(s), side="right") - 1, 0, NBIN - 1).astype(np.int32)
+
+
+def features(uy_plane: torch.Tensor, P: int, p_pad: int, s: torch.Tensor, n: int, c: int) -> torch.Tensor:
+    """(P, 34) fp32 features of the columns of a fp32 UY plane over the ascending spectrum s."""
+    dev = uy_plane.device
+    b = torch.from_numpy(bins_of(s.cpu().numpy())).to(dev)
+    feat = torch.empty(max(P, 1) * N_IN, dtype=torch.float32, device=dev)
+    _lib.call("flmm_mlp_features", uy_plane.data_ptr(), P, p_pad, s.numel(), b.data_ptr(), float(n), float(c),
+              feat.data_ptr(), _lib.stream())
+    return feat[:P * N_IN].view(P, N_IN)
+
+
+class DeltaMLP:
+    """34 -> 128 -> 128 -> 1 tanh MLP with parameters in one flat fp32 GPU buffer."""
+
+    def __init__(self, seed: int = 0, device=None):
+        dev = device or _lib.device()
+        rng = np.random.default_rng(seed)
+        theta = np.zeros(N_PARAMS, dtype=np.float32)
+        for (a, b), shape in zip(zip(OFFS[:-1], OFFS[1:]), SIZES):
+            if len(shape) == 2:
+                fan_in, fan_out = shape[1], shape[0]
+                lim = np.sqrt(6.0 / (fan_in + fan_out))
+                theta[a:b] = rng.uniform(-lim, lim, size=b - a)
+        self.theta = torch.from_numpy(theta).to(dev)
+        self.x_mean = torch.zeros(N_IN, dtype=torch.float32, device=dev)
+        self.x_std = torch.ones(N_IN, dtype=torch.float32, device=dev)
+        self.y_mean = 0.0
+        self.y_std = 1.0
+
+    def part(self, i: int, t: torch.Tensor | None = None) -> torch.Tensor:
+        return (self.theta if t is None else t)[int(OFFS[i]):int(OFFS[i + 1])]
+
+    # forward on normalised inputs; returns (pred, activations)
+    def _forward(self, x: torch.Tensor):
+        B = x.shape[0]
+        dev = x.device
+        h1 = torch.empty(B, H, dtype=torch.float32, device=dev)
+        h2 = torch.empty(B, H, dtype=torch.float32, device=dev)
+        out = torch.empty(B, 1, dtype=torch.float32, device=dev)
+        st = _lib.stream()
+        _lib.call("flmm_mlp_dense_fwd", x.data_ptr(), B, N_IN, self.part(0).data_ptr(), self.part(1).data_ptr(), H, 1,
+                  h1.data_ptr(), st)
+        _lib.call("flmm_mlp_dense_fwd", h1.data_ptr(), B, H, self.part(2).data_ptr(), self.part(3).data_ptr(), H, 1,
+                  h2.data_ptr(), st)
+        _lib.call("flmm_mlp_dense_fwd", h2.data_ptr(), B, H, self.part(4).data_ptr(), self.part(5).data_ptr(), 1, 0,
+                  out.data_ptr(), st)
+        return out[:, 0], (h1, h2)
+
+    def normalise(self, feat: torch.Tensor) -> torch.Tensor:
+        return ((feat - self.x_mean) / self.x_std).contiguous()
+
+    def predict(self, feat: torch.Tensor) -> torch.Tensor:
+        """Proposed log delta (FP64) for (P, 34) features."""
+        if feat.shape[0] == 0:
+            return torch.empty(0, dtype=torch.float64, device=feat.device)
+        pred, _ = self._forward(self.normalise(feat))
+        return pred.double() * self.y_std + self.y_mean
+
+    def grad(self, x: torch.Tensor, target: torch.Tensor) -> tuple[float, torch.Tensor]:
+        """MSE loss and its gradient (flat, like theta) on normalised inputs / targets, by the GPU kernels."""
+        B = x.shape[0]
+        dev = x.device
+        pred, (h1, h2) = self._forward(x)
+        g = torch.empty(N_PARAMS, dtype=torch.float32, device=dev)
+        dpred = torch.empty(B, 1, dtype=torch.float32, device=dev)
+        dh2 = torch.empty(B, H, dtype=torch.float32, device=dev)
+        dh1 = torch.empty(B, H, dtype=torch.float32, device=dev)
+        st = _lib.stream()
+        _lib.call("flmm_mlp_mse_grad", pred.data_ptr(), target.data_ptr(), B, dpred.data_ptr(), st)
+        out = pred.view(B, 1)
+        _lib.call("flmm_mlp_dense_bwd", dpred.data_ptr(), dpred.data_ptr(), out.data_ptr(), h2.data_ptr(), B, H, 1, 0,
+                  self.part(4).data_ptr(), dh2.data_ptr(), self.part(4, g).data_ptr(), self.part(5, g).data_ptr(), st)
+        _lib.call("flmm_mlp_dense_bwd", dh2.data_ptr(), dh2.data_ptr(), h2.data_ptr(), h1.data_ptr(), B, H, H, 1,
+                  self.part(2).data_ptr(), dh1.data_ptr(), self.part(2, g).data_ptr(), self.part(3, g).data_ptr(), st)
+        _lib.call("flmm_mlp_dense_bwd", dh1.data_ptr(), dh1.data_ptr(), h1.data_ptr(), x.data_ptr(), B, N_IN, H, 1,
+                  self.part(0).data_ptr(), None, self.part(0, g).data_ptr(), self.part(1, g).data_ptr(), st)
+        loss = float(((pred - target) ** 2).mean())
+        return loss, g
+
+    def fit(self, feat: torch.Tensor, target: torch.Tensor, *, epochs: int = 30, batch: int = 4096,
+            lr: float = 2e-3, seed: int = 0, verbose: bool = False) -> list[float]:
+        """Adam on the MSE of standardised targets; returns the per-epoch mean loss."""
+        dev = feat.device
+        self.x_mean = feat.mean(dim=0)
+        self.x_std = feat.std(dim=0).clamp_min(1e-3)
+        self.y_mean = float(target.double().mean())
+        self.y_std = float(target.double().std().clamp_min(1e-6))
+        x_all = self.normalise(feat)
+        y_all = ((target.double() - self.y_mean) / self.y_std).float().contiguous()
+        m = torch.zeros_like(self.theta)
+        v = torch.zeros_like(self.theta)
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(seed)
+        n = feat.shape[0]
+        step = 0
+        hist = []
+        for ep in range(epochs):
+            perm = torch.randperm(n, generator=gen, device=dev)
+            tot = 0.0
+            nb = 0
+            for b0 in range(0, n - batch + 1, batch):
+                idx = perm[b0:b0 + batch]
+                loss, g = self.grad(x_all[idx].contiguous(), y_all[idx].contiguous())
+                step += 1
+                lr_t = lr * (0.5 * (1 + np.cos(np.pi * ep / epochs)))
+                _lib.call("flmm_adam", self.theta.data_ptr(), g.data_ptr(), m.data_ptr(), v.data_ptr(), N_PARAMS,
+                          max(lr_t, 1e-5), 0.9, 0.999, 1e-8, step, _lib.stream())
+                tot += loss
+                nb += 1
+            hist.append(tot / max(nb, 1))
+            if verbose:
+                print(f"[mlp] epoch {ep} mse {hist[-1]:.4e}", flush=True)
+        return hist
+
+    def save(self, path: str | Path) -> None:
+        np.savez(path, theta=self.theta.cpu().numpy(), x_mean=self.x_mean.cpu().numpy(),
+                 x_std=self.x_std.cpu().numpy(), y_mean=self.y_mean, y_std=self.y_std)
+
+    @classmethod
+    def load(cls, path: str | Path) -> "DeltaMLP":
+        z = np.load(path)
+        net = cls()
+        dev = net.theta.device
+        net.theta = torch.from_numpy(z["theta"]).to(dev)
+        net.x_mean = torch.from_numpy(z["x_mean"]).to(dev)
+        net.x_std = torch.from_numpy(z["x_std"]).to(dev)
+        net.y_mean = float(z["y_mean"])
+        net.y_std = float(z["y_std"])
+        return net
+
+
+@dataclass
+class LearnedFit:
+    log_delta: torch.Tensor
+    fallback: torch.Tensor  # int32 per column, 1 = fell back to the grid bracket
+    delta: engine.DeltaResult
+
+
+def refine_learned(basis, net: DeltaMLP, n_grid: int, lo: float, hi: float, *, n: int, c: int,
+                   radius: float = 0.5, judge_tol: float = 1e-6, projected_loss: bool = True) -> LearnedFit:
+    """Learned delta for every column of a core._Basis: grid (judge) + MLP proposal + judged Newton."""
+    s_lam, nco = basis._loss(projected_loss)
+    idx = engine.delta_grid_idx(basis.uy, basis.P, basis.p_pad, basis.kpad, basis.s, s_lam, nco, n_grid, lo, hi)
+    feat = features(basis.uy, basis.P, basis.p_pad, basis.s, n, c)
+    t0 = net.predict(feat)
+    dev = basis.uy.device
+    P, p_pad, kpad = basis.P, basis.p_pad, basis.kpad
+    log_delta = torch.empty(max(P, 1), dtype=torch.float64, device=dev)
+    log_delta[:P].copy_(t0)
+    fb = torch.zeros(max(P, 1), dtype=torch.int32, device=dev)
+    rWr = torch.empty(max(P, 1), dtype=torch.float64, device=dev)
+    h2 = torch.empty(max(P, 1), dtype=torch.float64, device=dev)
+    q1 = engine.Planes(basis.prec, p_pad, kpad, dev)
+    q2 = engine.Planes(basis.prec, p_pad, kpad, dev)
+    q1h, q1l = q1.ptrs
+    q2h, q2l = q2.ptrs
+    _lib.call("flmm_delta_refine_learned", basis.prec, basis.uy.data_ptr(), P, p_pad, kpad, basis.s.data_ptr(),
+              basis.s.numel(), s_lam.data_ptr(), s_lam.numel(), float(nco), n_grid, float(lo), float(hi),
+              idx.data_ptr(), float(radius), float(judge_tol), log_delta.data_ptr(), fb.data_ptr(), rWr.data_ptr(),
+              h2.data_ptr(), q1h, q1l, q2h, q2l, q1.scale_ptr, q2.scale_ptr, _lib.stream())
+    d = engine.DeltaResult(log_delta[:P], rWr[:P], h2[:P], q1, q2, idx, int(basis.s.numel()))
+    basis.delta = d
+    return LearnedFit(log_delta[:P], fb[:P], d)
+
+
+# ---------------------------------------------------------------- training data
+
+def kinship_spectra(n: int, *, seeds=(0, 1, 2, 3), m_values=(5000, 20000), fst_values=(0.02, 0.1),
+                    device=None) -> list[torch.Tensor]:
+    """Eigenvalues (ascending, FP64, on the GPU) of intercept-projected Balding-Nichols kinships."""
+    from . import sim
+    from .core import _projected_eigh, _projector
+    from .io import standardise_columns
+    dev = device or _lib.device()
+    out = []
+    k = 0
+    for m in m_values:
+        for fst in fst_values:
+            rng = np.random.default_rng(seeds[k % len(seeds)] + 1000 * k)
+            k += 1
+            G = sim.balding_nichols(rng, n, m, fst=fst, maf=(0.05, 0.5))
+            Z = standardise_columns(torch.from_numpy(G).to(dev))
+            K = (Z @ Z.T) / m
+            X = torch.ones(n, 1, dtype=torch.float64, device=dev)
+            _, Iperp = _projector(X)
+            s, _ = _projected_eigh(K, Iperp, 1)
+            out.append(s.contiguous())
+    return out
+
+
+def simulate_columns(s: torch.Tensor, h2: torch.Tensor, gen: torch.Generator) -> torch.Tensor:
+    """(neff, P) rotated phenotypes y_k ~ N(0, sg2 s_k + se2), sg2 = h2 / mean(s), se2 = 1 - h2."""
+    sg2 = h2 / s.mean()
+    var = sg2.unsqueeze(0) * s.unsqueeze(1) + (1.0 - h2).unsqueeze(0)
+    return torch.randn(s.numel(), h2.numel(), generator=gen, device=s.device, dtype=torch.float64) * var.sqrt()
+
+
+def basis_from_columns(s: torch.Tensor, uy: torch.Tensor, prec: int):
+    """A core._Basis-like holder for columns already in the eigenbasis (no rotation needed)."""
+    class _B:
+        pass
+    b = _B()
+    neff, P = uy.shape
+    b.s = s.contiguous()
+    b.neff = neff
+    b.kpad = rup(neff, TILE)
+    b.P, b.p_pad = P, rup(max(P, 1), PTILE)
+    b.prec = prec
+    pl = engine.pack(_lib.PREC_FP32, uy.contiguous(), P, neff, 1, P, b.p_pad, b.kpad)
+    b.uy = pl.hi
+    b.delta = None
+    b._loss = lambda projected=True: (b.s, float(neff))
+    return b
+
+
+# ---------------------------------------------------------------- training and evaluation (the learned-delta workload)
+
+def make_training_set(spectra: list[torch.Tensor], n_pheno: int, *, n_grid: int = 100, seed: int = 0,
+                      h2_max: float = 0.95, prec: int | None = None):
+    """Simulated columns on the given spectra (equal shares), their features and grid + Newton labels."""
+    dev = spectra[0].device
+    prec = engine.prec_code() if prec is None else prec
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    feats, labels = [], []
+    per = -(-n_pheno // len(spectra))
+    for s in spectra:
+        h2 = torch.rand(per, generator=gen, device=dev, dtype=torch.float64) * h2_max
+        for c0 in range(0, per, 65536):
+            hh = h2[c0:c0 + 65536]
+            uy = simulate_columns(s, hh, gen)
+            b = basis_from_columns(s, uy, prec)
+            del uy
+            idx = engine.delta_grid_idx(b.uy, b.P, b.p_pad, b.kpad, s, s, float(b.neff), n_grid, -10.0, 10.0)
+            d = engine.delta_refine(prec, _lib.DELTA_NEWTON, b.uy, b.P, b.p_pad, b.kpad, s, s, float(b.neff), n_grid,
+                                    -10.0, 10.0, idx)
+            feats.append(features(b.uy, b.P, b.p_pad, s, b.neff + 1, 1))
+            labels.append(d.log_delta.float())
+    return torch.cat(feats)[:n_pheno], torch.cat(labels)[:n_pheno]
+
+
+def default_model_path() -> Path:
+    return Path(__file__).resolve().parent / "delta_mlp.npz"
+
+
+def train_default(*, n: int = 2000, n_pheno: int = 200_000, epochs: int = 30, seed: int = 0,
+                  verbose: bool = False) -> tuple[DeltaMLP, dict]:
+    """Train the proposal on n_pheno simulated phenotypes over N=n Balding-Nichols kinship spectra."""
+    import time
+    t0 = time.perf_counter()
+    spectra = kinship_spectra(n)
+    t1 = time.perf_counter()
+    feat, lab = make_training_set(spectra, n_pheno, seed=seed)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    net = DeltaMLP(seed=seed)
+    hist = net.fit(feat, lab, epochs=epochs, seed=seed, verbose=verbose)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    info = {"spectra": len(spectra), "n": n, "n_pheno": n_pheno, "epochs": epochs, "mse_first": hist[0],
+            "mse_last": hist[-1], "label_std": net.y_std, "s_spectra": t1 - t0, "s_labels": t2 - t1,
+            "s_train": t3 - t2}
+    return net, info
+
+
+_MODEL: DeltaMLP | None = None
+
+
+def model() -> DeltaMLP:
+    """The proposal network used by engine.settings.delta == "learned" (loaded once from delta_mlp.npz, written by
+    ``python -m fasterlmm_amd.learned`` on a GPU)."""
+    global _MODEL
+    if _MODEL is None:
+        p = default_model_path()
+        if not p.exists():
+            raise RuntimeError(f"{p} is missing; train it with `python -m fasterlmm_amd.learned` on the GPU")
+        _MODEL = DeltaMLP.load(p)
+    return _MODEL
+
+
+def set_model(net: DeltaMLP | None) -> None:
+    global _MODEL
+    _MODEL = net
+
+
+if __name__ == "__main__":
+    import json
+    import sys
+    net, info = train_default(verbose=True)
+    out = Path(sys.argv[1]) if len(sys.argv) > 1 else default_model_path()
+    net.save(out)
+    print(json.dumps(info))

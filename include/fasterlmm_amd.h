@@ -7,8 +7,12 @@
  *
  * Operand planes.  MFMA operands are "planes": a rows x K matrix stored in K-chunks of 32, chunk-major,
  * element (row, k) at ((k / 32) * rows_pad + row) * 32 + k % 32.  rows_pad is a multiple of 128 and K a
- * multiple of 128 (the padded extents are zero).  FLMM_PREC_SPLIT_BF16 planes come in (hi, lo) pairs of
- * uint16 bf16 bits with x ~= hi + lo; FLMM_PREC_FP32 planes are single float planes (pass NULL for lo).
+ * multiple of 128 (the padded extents are zero).  Precisions:
+ *   FLMM_PREC_SPLIT_FP16  (hi, lo) pairs of IEEE fp16 bits on rows scaled by a power of two (one float per row,
+ *                         "row scale"), x * scale ~= hi + lo to ~22 significant bits;
+ *   FLMM_PREC_SPLIT_BF16  (hi, lo) pairs of bf16 bits, x ~= hi + lo to ~16 significant bits, no scales;
+ *   FLMM_PREC_FP32        one float plane (pass NULL for lo and for the scales).
+ * A product of two split operands is hi*hi + hi*lo + lo*hi on MFMA with FP32 accumulation.
  *
  * The reference (gbrach/FaST-ER-LMM v1.6.2) has no native boundary: its hot path is a set of Python
  * functions on torch tensors.  Each entry point below names the reference function whose arithmetic it
@@ -25,10 +29,10 @@ extern "C" {
 #endif
 
 enum flmm_status { FLMM_OK = 0, FLMM_ERR_ARG = 1, FLMM_ERR_HIP = 2 };
-enum flmm_precision { FLMM_PREC_SPLIT_BF16 = 0, FLMM_PREC_FP32 = 1 };
+enum flmm_precision { FLMM_PREC_SPLIT_BF16 = 0, FLMM_PREC_FP32 = 1, FLMM_PREC_SPLIT_FP16 = 2 };
 enum flmm_rotate_mode { FLMM_ROTATE_UY = 0, FLMM_ROTATE_US = 1 };
 enum flmm_dtype { FLMM_DTYPE_F64 = 0, FLMM_DTYPE_I8 = 1 };
-enum flmm_delta_mode { FLMM_DELTA_GOLDEN = 0, FLMM_DELTA_NEWTON = 1, FLMM_DELTA_FIXED = 2 };
+enum flmm_delta_mode { FLMM_DELTA_GOLDEN = 0, FLMM_DELTA_NEWTON = 1, FLMM_DELTA_FIXED = 2, FLMM_DELTA_GRID = 4 };
 
 /* ABI version of this header (bumped on any signature change). */
 int flmm_abi_version(void);
@@ -43,23 +47,30 @@ const char* flmm_last_error(void);
 int flmm_standardise(int dtype, const void* src, int64_t n, int64_t m, int64_t ld_src, double* dst, int64_t ld_dst,
                      double* mean, double* std, void* stream);
 
+/* L2 norm of each row r < n_rows of the matrix addressed as in flmm_pack_planes (the magnitude bound that sets
+ * a split-FP16 row scale). */
+int flmm_row_norms(const double* src, int64_t n_rows, int64_t n_k, int64_t row_stride, int64_t k_stride,
+                   const int64_t* row_index, double* norm, void* stream);
+
 /* Pack an FP64 matrix into operand planes.  Element (r, k) = src[src_row(r) * row_stride + k * k_stride],
  * src_row(r) = row_index ? row_index[r] : r, for r < n_rows and k < n_k; zeros elsewhere up to
- * (rows_pad, kpad).  Feeds the rotation products of fastlmm_compat_rotate / snp_wald_scan_compat
- * (fasterlmm/core.py:517-518, 628-629): the columns of Y - X X^+ Y, the SNPs of one chromosome (gathered by
- * row_index, core.py:709-716), and U_eff^T. */
+ * (rows_pad, kpad).  Split-FP16 multiplies row r by 2^(14 - ceil(log2(row_norm[r]))) and writes that scale to
+ * row_scale[r] (rows_pad floats); other precisions ignore row_norm / row_scale (may be NULL).  Feeds the
+ * rotation products of fastlmm_compat_rotate / snp_wald_scan_compat (fasterlmm/core.py:517-518, 628-629): the
+ * columns of Y - X X^+ Y, the SNPs of one chromosome (gathered by row_index, core.py:709-716), and U_eff^T. */
 int flmm_pack_planes(int prec, const double* src, int64_t n_rows, int64_t n_k, int64_t row_stride, int64_t k_stride,
-                     const int64_t* row_index, void* plane_hi, void* plane_lo, int64_t rows_pad, int64_t kpad,
-                     void* stream);
+                     const int64_t* row_index, const double* row_norm, void* plane_hi, void* plane_lo,
+                     float* row_scale, int64_t rows_pad, int64_t kpad, void* stream);
 
-/* Rotation product out[i][j] = sum_n A[i][n] B[j][n] (A: a_rows_pad x n_pad planes, B: b_rows_pad x n_pad planes,
- * B = U_eff^T).  Replaces U_eff.T @ Y_r (core.py:518) and U_eff.T @ S_r (core.py:629).
- *   FLMM_ROTATE_UY: out0 = one fp32 plane (rows a_rows_pad, K = b_rows_pad) of the rotated columns.
- *   FLMM_ROTATE_US: the scan's SNP-side planes S1 = x (out0, out1) and S2 = x^2 (out2, out3); for
- *                   FLMM_PREC_FP32 out0 = x, out1 = x^2 (fp32 planes), out2/out3 unused. */
-int flmm_rotate(int prec, int mode, const void* a_hi, const void* a_lo, int64_t a_rows_pad, const void* b_hi,
-                const void* b_lo, int64_t b_rows_pad, int64_t n_pad, void* out0, void* out1, void* out2, void* out3,
-                void* stream);
+/* Rotation product x[i][j] = sum_n A[i][n] B[j][n] (A: a_rows_pad x n_pad planes, B: b_rows_pad x n_pad planes
+ * whose rows are orthonormal basis vectors).  Replaces U_eff.T @ Y_r (core.py:518) and U_eff.T @ S_r
+ * (core.py:629).  a_scale / b_scale are the operands' row scales (split-FP16 only).
+ *   FLMM_ROTATE_UY: out0 = one fp32 plane (rows a_rows_pad, K = b_rows_pad) of x.
+ *   FLMM_ROTATE_US: the scan's SNP-side planes S1 = x (out0, out1) and S2 = x^2 (out2, out3), with row scales
+ *                   out_scale1 / out_scale2 (split-FP16); for FLMM_PREC_FP32 out0 = x, out1 = x^2. */
+int flmm_rotate(int prec, int mode, const void* a_hi, const void* a_lo, const float* a_scale, int64_t a_rows_pad,
+                const void* b_hi, const void* b_lo, const float* b_scale, int64_t b_rows_pad, int64_t n_pad,
+                void* out0, void* out1, void* out2, void* out3, float* out_scale1, float* out_scale2, void* stream);
 
 /* Grid weights for the delta search: fp32 plane wplane (128 rows x kpad) with w[g][k] = 1/(s_k + delta_g) for
  * g < n_grid, k < neff (0 elsewhere), and lam[g] = sum_j log(s_lam_j + delta_g) over the nlam-long log-det
@@ -76,24 +87,62 @@ int flmm_delta_grid(const float* wplane, const float* lam, int n_grid, double nc
 
 /* Refine log delta in [grid[idx-1], grid[idx+1]] (core.py:575-595; FLMM_DELTA_GOLDEN replays the reference's 50
  * golden-section steps, FLMM_DELTA_NEWTON converges to the same local minimiser by safeguarded Newton;
- * FLMM_DELTA_FIXED reads log_delta[p] as given and skips the search, idx may be NULL), then
+ * FLMM_DELTA_FIXED reads log_delta[p] as given and skips the search; FLMM_DELTA_GRID returns the grid node of
+ * the first minimum of the FP64 loss, as refine=False does (core.py:570-573); idx may be NULL for both), then
  * write log_delta[p], rWr[p] = max(sum w y^2, 1e-300) (core.py:641-643), h2[p] = 1/(1+delta) (core.py:668-669)
- * and the scan's column-side planes Q1 = w o y (q1_hi, q1_lo) and Q2 = w (q2_hi, q2_lo), rows p_pad.  The loss is
- * nco * (log 2pi + log(R / nco) + 1) + sum_j log(s_lam_j + delta). */
+ * and the scan's column-side planes Q1 = w o y (q1_hi, q1_lo) and Q2 = w (q2_hi, q2_lo), rows p_pad, with row
+ * scales q1_scale / q2_scale (split-FP16).  The loss is nco * (log 2pi + log(R / nco) + 1) + sum_j
+ * log(s_lam_j + delta). */
 int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_pad, int64_t kpad, const double* s,
                       int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo, double hi,
                       const int32_t* idx, double* log_delta, double* rWr, double* h2, void* q1_hi, void* q1_lo,
-                      void* q2_hi, void* q2_lo, void* stream);
+                      void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale, void* stream);
+
+/* Learned-proposal refinement (no counterpart in the reference: the learned delta proposal).  log_delta[p]
+ * holds the proposal t0 on entry.  Newton on the loss slope inside [t0 - radius, t0 + radius]; the result is
+ * accepted when it converges there and its loss is at most judge_tol above the loss at the grid optimum
+ * grid[idx[p]]; otherwise fallback[p] = 1 and the column is refined in the grid bracket as FLMM_DELTA_NEWTON
+ * does.  Outputs as flmm_delta_refine; fallback may be NULL. */
+int flmm_delta_refine_learned(int prec, const float* uy, int64_t P, int64_t p_pad, int64_t kpad, const double* s,
+                              int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo,
+                              double hi, const int32_t* idx, double radius, double judge_tol, double* log_delta,
+                              int32_t* fallback, double* rWr, double* h2, void* q1_hi, void* q1_lo, void* q2_hi,
+                              void* q2_lo, float* q1_scale, float* q2_scale, void* stream);
+
+/* Spectral features of P rotated columns for the learned delta proposal (34 fp32 per column, row-major):
+ * feature b < 32 = log(mean y^2 in eigenvalue bin b / mean y^2), bin_of_k[k] in [0, 32) for k < neff;
+ * feature 32 = log(n) / 10, feature 33 = c / 10.  No counterpart in the reference. */
+int flmm_mlp_features(const float* uy, int64_t P, int64_t p_pad, int64_t neff, const int32_t* bin_of_k, double n,
+                      double c, float* feat, void* stream);
+
+/* Dense layer forward: out[b][j] = act(bias[j] + sum_i W[j][i] in[b][i]), act 1 = tanh, 0 = identity. */
+int flmm_mlp_dense_fwd(const float* in, int64_t B, int n_in, const float* W, const float* bias, int n_out, int act,
+                       float* out, void* stream);
+
+/* Dense layer backward: dz = dout * act'(out) (dz may alias dout), dW = dz^T in, db = sum_b dz,
+ * din = dz W (din may be NULL). */
+int flmm_mlp_dense_bwd(const float* dout, float* dz, const float* out, const float* in, int64_t B, int n_in,
+                       int n_out, int act, const float* W, float* din, float* dW, float* db, void* stream);
+
+/* Gradient of the mean squared error: dpred[b] = 2 (pred[b] - target[b]) / B. */
+int flmm_mlp_mse_grad(const float* pred, const float* target, int64_t B, float* dpred, void* stream);
+
+/* One Adam step (Kingma & Ba) on n parameters; step counts from 1. */
+int flmm_adam(float* w, const float* g, float* m, float* v, int64_t n, double lr, double beta1, double beta2,
+              double eps, int step, void* stream);
 
 /* Fused Wald scan of one SNP block (snp_wald_scan_compat's tile loop, core.py:636-667): sWy = S1.Q1, sWs = S2.Q2 on
- * MFMA, FP64 epilogue with dof nu.  Real columns p < n_real write f, beta, se, sfve, nullh2 at
- * [row_index[m] * n_real + p] of (M, n_real) row-major FP64 arrays (may be NULL when n_real == 0); every column
- * writes its maximum F over the block's tile rows to pmax[tile][p] (pmax: ceil(mc/128) x p_pad doubles). */
+ * MFMA, FP64 epilogue with dof nu; split-FP16 sums are divided by the planes' row scales.  Real columns
+ * p < n_real write f, beta, se, sfve, nullh2 at [row_index[m] * n_real + p] of (M, n_real) row-major FP64 arrays
+ * (may be NULL when n_real == 0); every column writes its maximum F over the block's tile rows to pmax[tile][p]
+ * (pmax: ceil(mc/128) x p_pad doubles).  With p_pad a multiple of 256 the 128 x 256 tile variant runs
+ * (environment FLMM_SCAN_VARIANT=square forces the 128 x 128 one); both give identical results. */
 int flmm_wald_scan(int prec, const void* s1_hi, const void* s1_lo, const void* s2_hi, const void* s2_lo,
-                   int64_t mc_pad, const void* q1_hi, const void* q1_lo, const void* q2_hi, const void* q2_lo,
-                   int64_t p_pad, int64_t kpad, int64_t mc, int64_t P, int64_t n_real, const double* rWr,
-                   const double* h2, double nu, const int64_t* row_index, double* f, double* beta, double* se,
-                   double* sfve, double* nullh2, double* pmax, void* stream);
+                   const float* s1_scale, const float* s2_scale, int64_t mc_pad, const void* q1_hi,
+                   const void* q1_lo, const void* q2_hi, const void* q2_lo, const float* q1_scale,
+                   const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t mc, int64_t P, int64_t n_real,
+                   const double* rWr, const double* h2, double nu, const int64_t* row_index, double* f, double* beta,
+                   double* se, double* sfve, double* nullh2, double* pmax, void* stream);
 
 /* max_F[p] = max(max_F[p], max_t pmax[t * ld + p]) for t < n_tiles (the genome-wide running max of
  * core.py:657 and core.py:723). */

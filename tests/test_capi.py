@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_error_channel():
     from fasterlmm_amd import _lib
     lib = _lib.load()
-    assert lib.flmm_abi_version() == 1
+    assert lib.flmm_abi_version() == 2
     # argument validation runs on the host and reports through flmm_last_error without touching a GPU
     rc = lib.flmm_max_reduce(None, 1, 0, 4, None, None)
     assert rc == 1
@@ -43,5 +43,7 @@ def test_bad_arguments_raise_value_error():
     with pytest.raises(ValueError, match="n_grid"):
         _lib.call("flmm_delta_weights", None, 10, None, 10, 128, 500, -10.0, 10.0, None, None, None)
     with pytest.raises(ValueError, match="precision"):
-        _lib.call("flmm_wald_scan", 7, *([None] * 4), 128, *([None] * 4), 128, 128, 1, 1, 0, None, None, 1.0,
+        _lib.call("flmm_wald_scan", 7, *([None] * 6), 128, *([None] * 6), 128, 128, 1, 1, 0, None, None, 1.0,
                   None, None, None, None, None, None, None, None)
+    with pytest.raises(ValueError, match="split-FP16"):
+        _lib.call("flmm_pack_planes", 2, 8, 4, 4, 1, 4, None, None, 16, 16, None, 128, 32, None)
