@@ -289,32 +289,48 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
     out = _new_outputs(M, n_real, P, dev)
     Zk = Zd if Zkd is None else Zkd
     ck = chrom if chrom_kin is None else np.asarray(chrom_kin)
-    K_all = Zk @ Zk.T
+    stage = engine.stage
+    with stage("kinship"):
+        K_all = Zk @ Zk.T
     Xpinv, Iperp = _projector(Xd)
     # the basis is orthogonal to X, so B^T (Y - X X^+ Y) = B^T Y; the OLS regress-out keeps the compat
     # path's arithmetic (core.py:517) and costs one pass
-    Y_r = (Yd - Xd @ (Xpinv @ Yd)).contiguous()
-    ypl = engine.pack(prec, Y_r, P, N, 1, P, rup(max(P, 1), PTILE), rup(N, KCHUNK))
-    del Y_r
+    with stage("pack_columns"):
+        Y_r = (Yd - Xd @ (Xpinv @ Yd)).contiguous()
+        ypl = engine.pack(prec, Y_r, P, N, 1, P, rup(max(P, 1), PTILE), rup(N, KCHUNK))
+        del Y_r
     jitter = 1e-6 if plain else 0.0
     eye = torch.eye(N, dtype=torch.float64, device=dev)
     chroms = sorted(np.unique(chrom).tolist())
+    # every chromosome's projected kinship first, then one batched FP64 eigendecomposition
+    Kps, Ks = [], []
+    with stage("kinship"):
+        for c in chroms:
+            kin_c = torch.from_numpy(np.flatnonzero(ck == c)).to(dev)
+            Zc = Zk.index_select(1, kin_c)
+            K = (K_all - Zc @ Zc.T) / float(Zk.shape[1] - kin_c.numel())
+            del Zc
+            if jitter:
+                K = K + jitter * eye
+            Kps.append((Iperp @ (K + eye)) @ Iperp)
+            if plain and not reml:
+                Ks.append(K)
+    with stage("eigh"):
+        ev, Ub = _eigh(torch.stack(Kps))
+        del Kps
+        s_mls = [torch.clamp(x, min=0.0) for x in torch.linalg.eigvalsh(torch.stack(Ks))] if Ks else None
+        del Ks
     for k, c in enumerate(chroms):
         test = torch.from_numpy(np.flatnonzero(chrom == c)).to(dev)
-        kin_c = torch.from_numpy(np.flatnonzero(ck == c)).to(dev)
-        Zc = Zk.index_select(1, kin_c)
-        K = (K_all - Zc @ Zc.T) / float(Zk.shape[1] - kin_c.numel())
-        del Zc
-        if jitter:
-            K = K + jitter * eye
-        s, U = _projected_eigh(K, Iperp, D)
-        s_ml = None
-        if plain and not reml:
-            s_ml = torch.clamp(torch.linalg.eigvalsh(K), min=0.0)
-        st = _Basis(s, U, D, N - D, prec, nu=N - D - 1, s_ml=s_ml)
-        st.use_column_planes(ypl, P)
-        _, d = st.fit(n_grid, -10.0, 10.0, True, projected_loss=(not plain) or reml)
-        st.scan(Zd, test, n_real, out, d)
+        s = torch.clamp(ev[k, D:N] - 1.0, min=0.0)
+        with stage("pack_basis"):
+            st = _Basis(s, Ub[k], D, N - D, prec, nu=N - D - 1, s_ml=None if s_mls is None else s_mls[k])
+        with stage("rotate_columns"):
+            st.use_column_planes(ypl, P)
+        with stage("delta"):
+            _, d = st.fit(n_grid, -10.0, 10.0, True, projected_loss=(not plain) or reml)
+        with stage("snps+scan"):
+            st.scan(Zd, test, n_real, out, d)
         if on_chrom is not None:
             on_chrom(k + 1, len(chroms))
     return _home(ScanResult(out.f, out.beta, out.se, out.sfve, out.nullh2, out.max_F), home)

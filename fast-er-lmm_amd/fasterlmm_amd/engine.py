@@ -249,6 +249,43 @@ class _ScanTimer:
 scan_timer = _ScanTimer()
 
 
+class _StageTimer:
+    """Optional HIP-event timing of pipeline stages (``with stage("eigh"): ...``), summed per stage name."""
+
+    def __init__(self):
+        self.enabled = False
+        self.events: list = []
+
+    def reset(self, enabled: bool = True):
+        self.enabled = enabled
+        self.events = []
+
+    def summary(self) -> dict:
+        torch.cuda.synchronize()
+        out: dict = {}
+        for name, a, b in self.events:
+            out[name] = out.get(name, 0.0) + a.elapsed_time(b)
+        return {k: round(v, 3) for k, v in out.items()}
+
+
+stage_timer = _StageTimer()
+
+
+@contextlib.contextmanager
+def stage(name: str):
+    if not stage_timer.enabled:
+        yield
+        return
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    try:
+        yield
+    finally:
+        b.record()
+        stage_timer.events.append((name, a, b))
+
+
 @dataclass
 class ScanOutputs:
     f: torch.Tensor
