@@ -19,6 +19,9 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // raw 16-byte fragment (a native vector, unlike HIP's
+                                                             // uint4 wrapper class, so register arrays of it stay in
+                                                             // registers)
 typedef unsigned short u16;
 
 constexpr int KCHUNK = 32;  // K extent of one plane chunk
@@ -72,6 +75,16 @@ template <int PREC>
 __device__ inline void split2(double x, u16& hi, u16& lo) {
   if constexpr (PREC == PREC_SPLIT_FP16) split_f16(x, hi, lo);
   else split_bf16(x, hi, lo);
+}
+
+// 1 / x for a positive normal x: the hardware reciprocal estimate refined by two Newton-Raphson steps
+// r <- r + r (1 - x r), each of which squares the relative error (far below FP64 rounding after two); a few
+// times cheaper than the IEEE division sequence, which also handles zeros, denormals and infinities
+__device__ inline double rcp_pos(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
 }
 
 __device__ inline double wave_sum(double v) {

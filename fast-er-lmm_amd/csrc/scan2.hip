@@ -1,26 +1,33 @@
 // Wald scan, variant "wide": 128 SNP rows x 256 columns per workgroup.  Same arithmetic and epilogue as
 // scan.hip (reference fasterlmm/core.py:649-667); different data movement:
 //   * the SNP-side planes (S1 = US, S2 = US^2) are shared by the four waves and staged through LDS, double
-//     buffered (one barrier per K chunk);
+//     buffered: loaded into registers at the start of a 32-k chunk, stored to the other buffer at its end, one
+//     barrier per chunk;
 //   * each wave owns 64 columns and streams its column-side fragments (Q1 = w o UY, Q2 = w) straight from global
-//     memory into registers, one K chunk ahead -- a 32-row block of one chunk is 2 KiB contiguous in the plane
-//     layout, so each fragment load is a fully coalesced 32-byte-per-lane read and needs no LDS at all;
+//     memory into registers one 16-k MFMA step ahead; a 32-row block of one chunk is 2 KiB contiguous in the
+//     plane layout, so the fragment loads are coalesced and need no LDS;
 //   * a wave covers all 128 rows of the tile, so the per-column maximum needs no cross-wave exchange.
-// LDS traffic per MFMA is ~40% of the square-tile variant's, which is what bounds it.
+// Register budget (one wave per SIMD): 256 accumulators (2 products x 4 x 2 blocks of 32 x 32) in AGPRs, four
+// column-fragment sets (32 VGPRs each for the split formats), 32 VGPRs of SNP-side staging, the A fragments of
+// one step and addressing.
 #include "kernels.h"
 
 namespace flmm {
 
 constexpr int W_TM = 128;   // SNP rows per workgroup
 constexpr int W_TP = 256;   // columns per workgroup (64 per wave)
+constexpr int EP_WAVE = 2 * 64 * 32 * 4;  // epilogue staging bytes per wave: [g][64 cols][32 rows] floats
+
+// float offset of (column col, 4-row group seg) in a wave's epilogue staging area
+__device__ inline int ep_off(int col, int seg) { return (col * 8 + (seg ^ ((col >> 1) & 7))) * 4; }
 
 template <int PREC>
 struct WideTraits;
 template <>
 struct WideTraits<PREC_SPLIT_BF16> {
-  static constexpr int NSUB = 2;
+  static constexpr int NSUB = 2;       // planes per operand (hi, lo)
   static constexpr int ROWB = 64;      // bytes per row per K chunk
-  static constexpr int BREGS = 2;      // uint4 per lane per (g, bj, sub) per chunk: 32 bytes = 16 bf16
+  static __device__ inline int swz(int row) { return (row >> 2) & 3; }
 };
 template <>
 struct WideTraits<PREC_SPLIT_FP16> : WideTraits<PREC_SPLIT_BF16> {};
@@ -28,17 +35,20 @@ template <>
 struct WideTraits<PREC_FP32> {
   static constexpr int NSUB = 1;
   static constexpr int ROWB = 128;
-  static constexpr int BREGS = 4;      // 64 bytes = 16 floats
+  static __device__ inline int swz(int row) { return (row >> 1) & 7; }
 };
 
-template <int PREC>
-__device__ inline int wide_swz(int row);
-template <>
-__device__ inline int wide_swz<PREC_SPLIT_BF16>(int row) { return (row >> 2) & 3; }
-template <>
-__device__ inline int wide_swz<PREC_SPLIT_FP16>(int row) { return (row >> 2) & 3; }
-template <>
-__device__ inline int wide_swz<PREC_FP32>(int row) { return (row >> 1) & 7; }
+// column-side fragments of one MFMA step: [g][bj][sub] -> v[(g * 2 + bj) * NSUB + sub], 16 bytes per lane each
+template <int NSUB>
+struct BFrag {
+  u32x4 v[2 * 2 * NSUB];
+};
+
+// SNP-side block of one chunk held in registers between its global load and its LDS store
+template <int N>
+struct AStage {
+  u32x4 v[N];
+};
 
 template <int PREC>
 __global__ __launch_bounds__(256, 1) void wald_scan_wide_kernel(
@@ -47,23 +57,73 @@ __global__ __launch_bounds__(256, 1) void wald_scan_wide_kernel(
     ScanOut out, double* __restrict__ pmax, int64_t pmax_ld) {
   using T = WideTraits<PREC>;
   constexpr int NSUB = T::NSUB;
-  constexpr int NPL = 2 * NSUB;                       // SNP-side planes
-  constexpr int PLANE_LDS = W_TM * T::ROWB;           // bytes per plane per stage
-  constexpr int STAGE = NPL * PLANE_LDS;              // bytes per stage (32 KiB)
-  constexpr int SPR = T::ROWB / 16;
-  constexpr int ROWS_PER_PASS = 256 / SPR;
-  constexpr int PASSES = W_TM / ROWS_PER_PASS;
-  constexpr int NA = NPL * PASSES;                    // uint4 per thread per chunk (A staging)
-  constexpr int NB = 2 * 2 * NSUB * T::BREGS;         // uint4 per lane per chunk (B fragments)
+  constexpr int NPL = 2 * NSUB;                    // SNP-side planes
+  constexpr int PLANE_LDS = W_TM * T::ROWB;        // bytes per plane per stage
+  constexpr int STAGE = NPL * PLANE_LDS;           // bytes per stage (32 KiB)
+  constexpr int BLK_PER_PLANE = PLANE_LDS / 1024;  // 1 KiB blocks (one 16-byte access per lane of a wave)
+  constexpr int NA = STAGE / 1024 / 4;             // blocks per wave per chunk
+  constexpr int ROWS_PER_BLK = 1024 / T::ROWB;
+  constexpr int SEGS = T::ROWB / 16;               // 16-byte segments per row
+  constexpr int NSTEP = T::ROWB / 32;              // 16-byte column-fragment steps per chunk
+  constexpr int NBR = 2 * 2 * NSUB;                // column-fragment loads per step
+  static_assert(BLK_PER_PLANE % NA == 0, "a wave's blocks stay inside one plane");
+  static_assert((2 * NSTEP) % 4 == 0 && 2 * NSTEP <= 8, "a chunk pair cycles the four column-fragment sets");
+  static_assert(2 * EP_WAVE <= STAGE, "epilogue staging of two waves fits one stage buffer");
 
-  extern __shared__ char lds[];
+  // two stage buffers; the chunk loop is unrolled by two so every access names a fixed buffer
+  __shared__ __attribute__((aligned(16))) char lds0[STAGE];
+  __shared__ __attribute__((aligned(16))) char lds1[STAGE];
   const int64_t tile = xcd_remap(blockIdx.x, n_tiles);
   const int64_t mt = tile % n_mtiles;
   const int64_t pt = tile / n_mtiles;
   const int64_t m0 = mt * W_TM, p0 = pt * W_TP;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int h = lane >> 5;
   const int nchunks = args.nchunks;
+
+  // ---- SNP-side staging: wave w copies 1 KiB blocks [w NA, (w + 1) NA) of the stage, all inside plane pl_w, through
+  // registers (loads issued at the start of a chunk, written to the other stage at its end).  Lane L of a block
+  // handles row blk * ROWS_PER_BLK + L / SEGS, logical segment L % SEGS, stored at physical segment
+  // (L % SEGS) ^ swz(row).
+  const int pl_w = (wave * NA) / BLK_PER_PLANE;
+  const int blk0 = (wave * NA) % BLK_PER_PLANE;
+  const char* a_src = (const char*)args.a[pl_w / NSUB][pl_w % NSUB] + m0 * T::ROWB + (blk0 * 1024 + lane * 16);
+  const int64_t a_kstride = args.a_rows_pad * T::ROWB;
+  const int a_row0 = blk0 * ROWS_PER_BLK + lane / SEGS;
+
+  // ---- column side: lane (row p0 + 64 wave + 32 bj + lane % 32, half h) reads bytes [h ROWB / 2, (h + 1) ROWB / 2)
+  // of its row's chunk, 16 bytes per step
+  const int64_t b_lane = (p0 + wave * 64 + (lane & 31)) * T::ROWB + h * (T::ROWB / 2);
+  const int64_t b_kstride = args.b_rows_pad * T::ROWB;
+
+  auto load_a = [&](int kc) {
+    AStage<NA> f;
+    const char* src = a_src + (int64_t)kc * a_kstride;
+#pragma unroll
+    for (int i = 0; i < NA; ++i) f.v[i] = *(const u32x4*)(src + i * 1024);
+    return f;
+  };
+  auto store_a = [&](const AStage<NA>& f, char* stage) {
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int r = a_row0 + i * ROWS_PER_BLK;
+      *(u32x4*)(stage + pl_w * PLANE_LDS + r * T::ROWB + (((lane % SEGS) ^ T::swz(r)) * 16)) = f.v[i];
+    }
+  };
+  auto load_b = [&](int kc, int step) {
+    BFrag<NSUB> f;
+    const int64_t off = (int64_t)kc * b_kstride + b_lane + step * 16;
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int sub = 0; sub < NSUB; ++sub)
+          f.v[(g * 2 + bj) * NSUB + sub] =
+              *(const u32x4*)((const char*)args.b[g][sub] + off + bj * 32 * T::ROWB);
+    return f;
+  };
 
   f32x16 acc[2][4][2];
 #pragma unroll
@@ -75,168 +135,170 @@ __global__ __launch_bounds__(256, 1) void wald_scan_wide_kernel(
 #pragma unroll
         for (int v = 0; v < 16; ++v) acc[g][i][j][v] = 0.f;
 
-  const int s_in = tid % SPR;
-  const int r_in0 = tid / SPR;
-  uint4 astage[NA];
-  uint4 bcur[NB], bnext[NB];
-
-  auto aload = [&](int kc) {
+  // one MFMA step (16 physical k per lane half) of chunk stage `st` with column fragments bf
+  auto mma_step = [&](const char* st, int step, const BFrag<NSUB>& bf) {
+    if constexpr (is_split(PREC)) {
+      const int seg = h * 2 + step;
 #pragma unroll
-    for (int pl = 0; pl < NPL; ++pl) {
-      const char* base = (const char*)args.a[pl / NSUB][pl % NSUB];
-      const char* chunk = base + ((int64_t)kc * args.a_rows_pad + m0) * T::ROWB;
+      for (int g = 0; g < 2; ++g) {
+        bf16x8 af[4][2];
 #pragma unroll
-      for (int ps = 0; ps < PASSES; ++ps) {
-        const int r = r_in0 + ps * ROWS_PER_PASS;
-        astage[pl * PASSES + ps] = *(const uint4*)(chunk + (int64_t)r * T::ROWB + s_in * 16);
+        for (int bi = 0; bi < 4; ++bi) {
+          const int r = bi * 32 + (lane & 31);
+#pragma unroll
+          for (int sub = 0; sub < 2; ++sub)
+            af[bi][sub] = *(const bf16x8*)(st + (g * 2 + sub) * PLANE_LDS + r * 64 + ((seg ^ T::swz(r)) * 16));
+        }
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+          const bf16x8 bh = __builtin_bit_cast(bf16x8, bf.v[(g * 2 + bj) * 2 + 0]);
+          const bf16x8 bl = __builtin_bit_cast(bf16x8, bf.v[(g * 2 + bj) * 2 + 1]);
+#pragma unroll
+          for (int bi = 0; bi < 4; ++bi) {
+            f32x16 c = acc[g][bi][bj];
+            c = mfma_split<PREC>(af[bi][0], bh, c);
+            c = mfma_split<PREC>(af[bi][0], bl, c);
+            c = mfma_split<PREC>(af[bi][1], bh, c);
+            acc[g][bi][bj] = c;
+          }
+        }
+      }
+    } else {
+      const int seg = h * 4 + step;
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        f32x4 af[4];
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi) {
+          const int r = bi * 32 + (lane & 31);
+          af[bi] = *(const f32x4*)(st + g * PLANE_LDS + r * 128 + ((seg ^ T::swz(r)) * 16));
+        }
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+          const f32x4 b4 = __builtin_bit_cast(f32x4, bf.v[g * 2 + bj]);
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int bi = 0; bi < 4; ++bi)
+              acc[g][bi][bj] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[bi][e], b4[e], acc[g][bi][bj], 0, 0, 0);
+        }
       }
     }
   };
-  auto astore = [&](int buf) {
-    char* st = lds + buf * STAGE;
-#pragma unroll
-    for (int pl = 0; pl < NPL; ++pl)
-#pragma unroll
-      for (int ps = 0; ps < PASSES; ++ps) {
-        const int r = r_in0 + ps * ROWS_PER_PASS;
-        *(uint4*)(st + pl * PLANE_LDS + r * T::ROWB + ((s_in ^ wide_swz<PREC>(r)) * 16)) = astage[pl * PASSES + ps];
-      }
+
+  // Software pipeline over chunk pairs (the two LDS buffers).  MFMA steps j = 0, 1, ... of the pair use
+  // column-fragment set b(j % 4), and step j issues the loads of step j + 2 first, so a fragment load has two
+  // MFMA steps (~96 MFMAs) to land.  The next chunk's SNP-side block is loaded into registers at the start of a
+  // chunk and written to the other buffer at its end.  sched_barrier pins each load batch ahead of the MFMA
+  // block it overlaps.  Past the last chunk the loads repeat the last one (valid addresses, values unused).
+  auto load_bs = [&](int kc, int st) {
+    int kk = kc + st / NSTEP;
+    if (kk >= nchunks) kk = nchunks - 1;
+    return load_b(kk, st % NSTEP);
   };
-  // this lane's fragment of column block (g, bj, sub) for chunk kc: row p0 + 64 wave + 32 bj + lane % 32,
-  // bytes [h * ROWB / 2, (h + 1) * ROWB / 2) of the row's chunk -- 16 consecutive physical k
-  auto bload = [&](int kc, uint4* dst) {
+  BFrag<NSUB> b0 = load_bs(0, 0), b1 = load_bs(0, 1), b2, b3;
+  AStage<NA> an;
+  store_a(load_a(0), lds0);
+  __syncthreads();
+  // step J of a chunk pair: chunk kc + J / NSTEP, MFMA step J % NSTEP, fragments BUSE; loads step J + 2 into BLOAD
+#define FLMM_WIDE_STEP(J, BUSE, BLOAD)                                                                               \
+  if constexpr ((J) < 2 * NSTEP) {                                                                                   \
+    constexpr int half = (J) / NSTEP, st = (J) % NSTEP;                                                              \
+    const int kcc = kc + half;                                                                                       \
+    if constexpr (st == 0) an = load_a(kcc + 1 < nchunks ? kcc + 1 : kcc);                                           \
+    BLOAD = load_bs(kcc, st + 2);                                                                                    \
+    __builtin_amdgcn_sched_barrier(0);                                                                               \
+    mma_step(half ? lds1 : lds0, st, BUSE);                                                                          \
+    __builtin_amdgcn_sched_barrier(0);                                                                               \
+    if constexpr (st == NSTEP - 1) {                                                                                 \
+      store_a(an, half ? lds0 : lds1); /* last read in the previous chunk, which ended with a barrier */            \
+      __syncthreads();                                                                                               \
+    }                                                                                                                \
+  }
+  // nchunks is even (K is padded to a multiple of 128)
+  for (int kc = 0; kc < nchunks; kc += 2) {
+    FLMM_WIDE_STEP(0, b0, b2)
+    FLMM_WIDE_STEP(1, b1, b3)
+    FLMM_WIDE_STEP(2, b2, b0)
+    FLMM_WIDE_STEP(3, b3, b1)
+    FLMM_WIDE_STEP(4, b0, b2)
+    FLMM_WIDE_STEP(5, b1, b3)
+    FLMM_WIDE_STEP(6, b2, b0)
+    FLMM_WIDE_STEP(7, b3, b1)
+  }
+#undef FLMM_WIDE_STEP
+
+  // epilogue: this wave owns columns p0 + 64 wave + [0, 64) and all 128 rows.  The LDS stages are free after the
+  // last barrier; each 32-row block of the wave's accumulators is staged there column-major ([g][col][row], the
+  // 16-byte row groups of a column XOR-swizzled so both the b128 writes and the b128 column reads are bank-conflict
+  // free), then lane
+  // c owns column p0 + 64 wave + c and walks the rows: per-row quantities are wave-uniform scalar loads and the
+  // FP64 tail is emitted once per loop body, not once per accumulator slot.
+  // reciprocal row scales of the tile's SNPs, once per tile (1 without FP16 scales)
+  __shared__ double row_inv[2][W_TM];
+  {
+    const int t = threadIdx.x & (W_TM - 1), which = threadIdx.x / W_TM;
+    const float* scl = which ? sc.s2 : sc.s1;
+    const int64_t m = m0 + t;
+    row_inv[which][t] = (scl && m < Mc) ? 1.0 / (double)scl[m] : 1.0;
+  }
+  __syncthreads();
+  float* ep = (float*)((wave < 2 ? lds0 : lds1) + (wave & 1) * EP_WAVE);
+  const int64_t pc = p0 + wave * 64 + lane;
+  const bool pv = pc < P;
+  const bool real = pc < n_real;
+  const double rwr = pv ? rWr[pc] : 1.0;
+  const double iq1 = (sc.q1 && pv) ? 1.0 / (double)sc.q1[pc] : 1.0;
+  const double iq2 = (sc.q2 && pv) ? 1.0 / (double)sc.q2[pc] : 1.0;
+  const double hh = real ? h2[pc] : 0.0;
+  const bool tile_real = p0 < n_real;
+  double ba = 0.0, bd = 1.0;
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi) {
 #pragma unroll
     for (int g = 0; g < 2; ++g)
 #pragma unroll
       for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-        for (int sub = 0; sub < NSUB; ++sub) {
-          const char* base = (const char*)args.b[g][sub];
-          const int64_t row = p0 + wave * 64 + bj * 32 + (lane & 31);
-          const char* src = base + ((int64_t)kc * args.b_rows_pad + row) * T::ROWB + h * (T::ROWB / 2);
-#pragma unroll
-          for (int q = 0; q < T::BREGS; ++q)
-            dst[((g * 2 + bj) * NSUB + sub) * T::BREGS + q] = *(const uint4*)(src + q * 16);
+        for (int q = 0; q < 4; ++q) {
+          const f32x16& c = acc[g][bi][bj];
+          const f32x4 w4 = {c[4 * q], c[4 * q + 1], c[4 * q + 2], c[4 * q + 3]};
+          *(f32x4*)(ep + ep_off(g * 64 + bj * 32 + (lane & 31), 2 * q + h)) = w4;
         }
-  };
-
-  aload(0);
-  bload(0, bcur);
-  astore(0);
-  __syncthreads();
-  for (int kc = 0; kc < nchunks; ++kc) {
-    const int buf = kc & 1;
-    const bool more = kc + 1 < nchunks;
-    if (more) {
-      aload(kc + 1);
-      bload(kc + 1, bnext);
-    }
-    const char* st = lds + buf * STAGE;
-    if constexpr (is_split(PREC)) {
+    __builtin_amdgcn_wave_barrier();
+    const int64_t mb = m0 + bi * 32;
+#pragma unroll 1
+    for (int t4 = 0; t4 < 8; ++t4) {
+      const f32x4 a4 = *(const f32x4*)(ep + ep_off(lane, t4));
+      const f32x4 d4 = *(const f32x4*)(ep + ep_off(64 + lane, t4));
 #pragma unroll
-      for (int ks = 0; ks < 2; ++ks) {
-        const int seg = h * 2 + ks;
-#pragma unroll
-        for (int g = 0; g < 2; ++g) {
-          bf16x8 af[4][2];
-#pragma unroll
-          for (int bi = 0; bi < 4; ++bi) {
-            const int r = bi * 32 + (lane & 31);
-#pragma unroll
-            for (int sub = 0; sub < 2; ++sub)
-              af[bi][sub] = *(const bf16x8*)(st + (g * 2 + sub) * PLANE_LDS + r * 64 + ((seg ^ wide_swz<PREC>(r)) * 16));
-          }
-#pragma unroll
-          for (int bj = 0; bj < 2; ++bj) {
-            const bf16x8 bh = *(const bf16x8*)&bcur[((g * 2 + bj) * 2 + 0) * 2 + ks];
-            const bf16x8 bl = *(const bf16x8*)&bcur[((g * 2 + bj) * 2 + 1) * 2 + ks];
-#pragma unroll
-            for (int bi = 0; bi < 4; ++bi) {
-              f32x16 c = acc[g][bi][bj];
-              c = mfma_split<PREC>(af[bi][0], bh, c);
-              c = mfma_split<PREC>(af[bi][0], bl, c);
-              c = mfma_split<PREC>(af[bi][1], bh, c);
-              acc[g][bi][bj] = c;
-            }
-          }
-        }
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int seg = h * 4 + q;
-#pragma unroll
-        for (int g = 0; g < 2; ++g) {
-          f32x4 af[4];
-#pragma unroll
-          for (int bi = 0; bi < 4; ++bi) {
-            const int r = bi * 32 + (lane & 31);
-            af[bi] = *(const f32x4*)(st + g * PLANE_LDS + r * 128 + ((seg ^ wide_swz<PREC>(r)) * 16));
-          }
-#pragma unroll
-          for (int bj = 0; bj < 2; ++bj) {
-            const f32x4 bf = *(const f32x4*)&bcur[(g * 2 + bj) * 4 + q];
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-#pragma unroll
-              for (int bi = 0; bi < 4; ++bi)
-                acc[g][bi][bj] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[bi][e], bf[e], acc[g][bi][bj], 0, 0, 0);
-          }
+      for (int e = 0; e < 4; ++e) {
+        const int64_t m = mb + 4 * t4 + e;
+        if (m >= Mc) break;
+        const int t = bi * 32 + 4 * t4 + e;
+        const double a = (double)a4[e] * iq1 * row_inv[0][t];
+        const double d = fmax((double)d4[e] * iq2 * row_inv[1][t], FLOOR64);
+        if (a * a * bd > ba * ba * d) { ba = a; bd = d; }
+        if (tile_real && real) {
+          const double beta = a / d;
+          const double rss = fmax(rwr - a * a / d, FLOOR64);
+          const double var = rss / nu / d;
+          const int64_t o = row_index[m] * out.ld + pc;
+          out.f[o] = beta * beta / var;
+          out.beta[o] = beta;
+          out.se[o] = sqrt(var);
+          out.sfve[o] = sqrt(fmax(a * a / (d * rwr), 0.0));
+          out.nullh2[o] = hh;
         }
       }
     }
-    if (more) {
-      astore(buf ^ 1);
-#pragma unroll
-      for (int q = 0; q < NB; ++q) bcur[q] = bnext[q];
-    }
-    __syncthreads();
+    __builtin_amdgcn_wave_barrier();
   }
-
-  // epilogue: this wave owns columns p0 + 64 wave + [0, 64) and all 128 rows
-#pragma unroll
-  for (int bj = 0; bj < 2; ++bj) {
-    const int64_t p = p0 + wave * 64 + bj * 32 + (lane & 31);
-    const bool pv = p < P;
-    const double rwr = pv ? rWr[p] : 1.0;
-    const bool real = p < n_real;
-    const double hh = real ? h2[p] : 0.0;
-    const double inv_q1 = (sc.q1 && pv) ? 1.0 / (double)sc.q1[p] : 1.0;
-    const double inv_q2 = (sc.q2 && pv) ? 1.0 / (double)sc.q2[p] : 1.0;
-    double ba = 0.0, bd = 1.0;
-#pragma unroll
-    for (int bi = 0; bi < 4; ++bi) {
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int64_t m = m0 + bi * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
-        if (m < Mc) {
-          double a = (double)acc[0][bi][bj][v];
-          double d = (double)acc[1][bi][bj][v];
-          unscale_sums(sc, m, inv_q1, inv_q2, a, d);
-          d = fmax(d, FLOOR64);
-          if (a * a * bd > ba * ba * d) { ba = a; bd = d; }
-          if (real) {
-            const double beta = a / d;
-            const double rss = fmax(rwr - a * a / d, FLOOR64);
-            const double var = rss / nu / d;
-            const int64_t o = row_index[m] * out.ld + p;
-            out.f[o] = beta * beta / var;
-            out.beta[o] = beta;
-            out.se[o] = sqrt(var);
-            out.sfve[o] = sqrt(fmax(a * a / (d * rwr), 0.0));
-            out.nullh2[o] = hh;
-          }
-        }
-      }
-    }
-    const double oa = __shfl_xor(ba, 32, 64), od = __shfl_xor(bd, 32, 64);
-    if (oa * oa * bd > ba * ba * od) { ba = oa; bd = od; }
-    if (h == 0 && pv) {
-      const double denom = fmax(bd, FLOOR64);
-      const double beta = ba / denom;
-      const double rss = fmax(rwr - ba * ba / denom, FLOOR64);
-      pmax[mt * pmax_ld + p] = beta * beta / (rss / nu / denom);
-    }
+  if (pv) {
+    const double denom = fmax(bd, FLOOR64);
+    const double beta = ba / denom;
+    const double rss = fmax(rwr - ba * ba / denom, FLOOR64);
+    pmax[mt * pmax_ld + pc] = beta * beta / (rss / nu / denom);
   }
 }
 
@@ -247,8 +309,7 @@ hipError_t launch_wald_scan_wide(const TileArgs<PREC, 2>& args, const ScanScales
   const int64_t n_mtiles = (Mc + W_TM - 1) / W_TM;
   const int64_t n_ptiles = (P + W_TP - 1) / W_TP;
   const int64_t n_tiles = n_mtiles * n_ptiles;
-  constexpr int lds = 2 * 2 * WideTraits<PREC>::NSUB * W_TM * WideTraits<PREC>::ROWB;
-  hipLaunchKernelGGL(wald_scan_wide_kernel<PREC>, dim3((unsigned)n_tiles), dim3(256), lds, stream, args, sc, n_mtiles,
+  hipLaunchKernelGGL(wald_scan_wide_kernel<PREC>, dim3((unsigned)n_tiles), dim3(256), 0, stream, args, sc, n_mtiles,
                      n_tiles, Mc, P, n_real, rWr, h2, nu, row_index, out, pmax, args.b_rows_pad);
   return hipGetLastError();
 }

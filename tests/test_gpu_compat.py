@@ -180,3 +180,44 @@ def test_golden_section_mode_replays_reference(core, ex):
                                               torch.from_numpy(Y).to(DEV))
             ld = core.fit_delta_grid_compat(spec).cpu().numpy()
         assert np.abs(ld - ref["log_delta"][ci]).max() <= 1e-4
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_scan_variants_multi_tile(core, ex, precision, monkeypatch):
+    """300 SNPs x 300 columns spans several SNP tiles (128 rows) and column tiles (256) with ragged edges, and
+    the real columns end inside a tile; the wide (128 x 256) and square (128 x 128) scan variants must both
+    match the oracle and agree with each other."""
+    from fasterlmm_amd import engine
+    Zs, chrom = ex["Zs"], ex["chrom"]
+    X = ex["X_cov2"]
+    rng = np.random.default_rng(11)
+    Yr = ex["Y"]
+    Y = np.concatenate([Yr] + [Yr[rng.permutation(Yr.shape[0])] for _ in range(14)], axis=1)  # 20 x 15 = 300
+    n_real = 20
+    K = lmm.grm(Zs[:, chrom > 4])
+    want = lmm.compat_rotate(K, X, Y)
+    ld = lmm.fit_delta_grid_compat(want)
+    S = Zs[:, chrom <= 4][:, :300]
+    assert S.shape[1] == 300
+    ref = lmm.snp_wald_scan_compat(want, ld, S, n_real=n_real)
+    df = want["s"].shape[0] - 1
+    got = {}
+    for variant in ("wide", "square"):
+        if variant == "square":
+            monkeypatch.setenv("FLMM_SCAN_VARIANT", "square")
+        else:
+            monkeypatch.delenv("FLMM_SCAN_VARIANT", raising=False)
+        with engine.configure(precision=precision):
+            spec = core.fastlmm_compat_rotate(torch.from_numpy(K).to(DEV), torch.from_numpy(X).to(DEV),
+                                              torch.from_numpy(Y).to(DEV))
+            res = core.snp_wald_scan_compat(spec, torch.from_numpy(ld).to(DEV), torch.from_numpy(S).to(DEV),
+                                            n_real=n_real)
+        check_scan(res, ref, df, beta_tol=BETA_BAR[precision], what=f"{precision}/{variant}")
+        np.testing.assert_allclose(res.max_F.cpu().numpy(), ref["max_F"], rtol=1e-4)
+        got[variant] = res
+    # same FP32 accumulation order in both variants; only the FP64 unscaling differs in rounding
+    for name in ("f", "beta", "se", "sfve", "max_F"):
+        a = getattr(got["wide"], name).cpu().numpy()
+        b = getattr(got["square"], name).cpu().numpy()
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=0, err_msg=name)
+    np.testing.assert_array_equal(got["wide"].nullh2.cpu().numpy(), got["square"].nullh2.cpu().numpy())
