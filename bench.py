@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-compare", action="store_true", help="skip the split-BF16 / FP32 comparison steps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one plain device copy of this many MiB first (reference kernel for HBM counters)")
     return ap.parse_args()
 
 
@@ -119,6 +121,20 @@ def cpu_baseline(panel, cfg, n_perm, budget_s):
                       f"(64 phenotypes x {1 + n_perm}), N={Zs.shape[0]}, {dt:.1f}s wall, numpy/BLAS on {threads} threads"}
 
 
+def traffic_per_launch(prec, scan):
+    """HBM bytes per scan launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload,
+    when one exists for this precision and launch count (else None)."""
+    f = ROOT / "profiles" / "wald_scan_wide_traffic.json"
+    if not f.exists():
+        return None, None
+    d = json.loads(f.read_text())
+    e = d.get(prec)
+    if not e or e.get("tests_per_launch") != round(scan["tests"] / max(scan["launches"], 1)):
+        return None, None
+    return e["bytes_per_launch"], {"fetch_bytes": e["fetch_bytes"], "write_bytes": e["write_bytes"],
+                                   "source": "profiles/wald_scan_wide_traffic.json"}
+
+
 def timed_run(Z, X, Y_all, chrom, B, precision, steps):
     """(results of the last step, seconds for `steps` steps, scan-kernel summary)."""
     import torch
@@ -147,6 +163,16 @@ def main():
     import numpy as np
     import torch
 
+    if args.copy_probe_mib > 0:
+        # a copy of known size: what FETCH_SIZE / WRITE_SIZE read for it calibrates those counters
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
+        del src, dst
+
     cfg, panel, Z, X, Y_all, B, n_perm = build_workload(args.config, args.n_perm, args.pheno, args.snps)
     N, M = Z.shape
     P = Y_all.shape[1]
@@ -171,13 +197,19 @@ def main():
 
     split = prec != "fp32"
     alg = tflops(scan)
-    peak = MI355X_PEAK_BF16_TFLOPS / 3.0 if split else MI355X_PEAK_FP32_TFLOPS
+    products = 3 if split else 1
+    traffic, traffic_detail = traffic_per_launch(prec, scan)
+    published = MI355X_PEAK_BF16_TFLOPS if split else MI355X_PEAK_FP32_TFLOPS
+    peak = published / products
     roofline = {"bound": "mfma", "achieved": round(alg, 2), "peak": round(peak, 1), "unit": "TFLOP/s",
-                "frac": round(alg / peak, 4), "traffic": None,
+                "frac": round(alg / peak, 4), "traffic": traffic, "traffic_detail": traffic_detail,
                 "kernel": f"wald_scan_wide_kernel<{prec}>", "mean_launch_ms": round(scan["mean_ms"], 4),
                 "launches_per_step": scan["launches"] // args.steps,
-                "note": "achieved = 4*Neff algorithmic flop per Wald test / HIP-event kernel time over the timed steps; "
-                        "peak = 2.5 PF dense fp16/bf16 MFMA / 3 products per split product (issued MFMA = 3x achieved)"}
+                "published_peak_tflops": published, "mfma_issued_tflops": round(products * alg, 2),
+                "mfma_utilisation": round(products * alg / published, 4),
+                "note": "achieved = 4*(N-C) algorithmic flop per Wald test x tests per launch / mean kernel time "
+                        "(HIP events on the launch stream); peak = published dense MFMA peak / products per "
+                        "split product (3 for split fp16/bf16), so frac = issued-MFMA utilisation"}
 
     extra = {}
     from fasterlmm_amd.pvalues import neglog10p

@@ -117,9 +117,12 @@ struct ColSums {
 
 // The likelihood of one column: y (neff values in the weight spectrum s) and the log-det spectrum s_lam (nlam
 // values; the same as s on the projected basis, the full spectrum for the ML loss of the GLS path).  With
-// KPL > 0 each lane keeps its KPL elements (k = lane + 64 i) of y, s and s_lam in registers, so the column is
-// read from HBM once however many loss / slope evaluations follow; KPL = 0 re-reads them every evaluation.
-template <int KPL>
+// KPL > 0 each lane keeps its KPL elements (k = lane + 64 i) of y in registers (as the FP32 values the rotation
+// produced), and the block holds both spectra once in LDS, so the column is read from HBM once however many
+// loss / slope evaluations follow; KPL = 0 re-reads everything from global memory every evaluation.
+// When s_lam is s (SAME, the projected compat basis), the log-det weights are the data weights: each element
+// costs one reciprocal.
+template <int KPL, bool SAME>
 struct ColLik {
   const float* uy;
   int64_t p_pad;
@@ -127,17 +130,19 @@ struct ColLik {
   int64_t neff;
   const double* s_lam;
   int64_t nlam;
-  double nco;  // coefficient of log R in the loss
-  double yr[KPL > 0 ? KPL : 1], sr[KPL > 0 ? KPL : 1], lr[KPL > 0 ? KPL : 1];
+  double nco;          // coefficient of log R in the loss
+  const double* ls;    // LDS copies (KPL > 0): s, s_lam (padded with 1), and their FP32 roundings
+  const double* ll;
+  const float* lsf;
+  const float* llf;
+  float yr[KPL > 0 ? KPL : 1];
 
   __device__ void load(int64_t p, int lane) {
     if constexpr (KPL > 0) {
 #pragma unroll
       for (int i = 0; i < KPL; ++i) {
         const int64_t k = lane + 64 * i;
-        yr[i] = k < neff ? (double)uy[((k >> 5) * p_pad + p) * 32 + (k & 31)] : 0.0;
-        sr[i] = k < neff ? s[k] : 1.0;
-        lr[i] = k < nlam ? s_lam[k] : 1.0;
+        yr[i] = k < neff ? uy[((k >> 5) * p_pad + p) * 32 + (k & 31)] : 0.f;
       }
     }
   }
@@ -148,15 +153,17 @@ struct ColLik {
     if constexpr (KPL > 0) {
 #pragma unroll
       for (int i = 0; i < KPL; ++i) {
-        const int64_t k = lane + 64 * i;
-        const double w = 1.0 / (sr[i] + delta);
-        const double yw = yr[i] * yr[i] * w;
+        const int k = lane + 64 * i;
+        const double y = (double)yr[i];
+        const double sk = ls[k];
+        const double w = rcp_pos(sk + delta);
+        const double yw = y * y * w;
         c.R += yw;
         c.R2 += yw * w;
         c.R3 += yw * w * w;
         if (k < nlam) {
-          const double sd = lr[i] + delta;
-          const double wl = 1.0 / sd;
+          const double sd = (SAME ? sk : ll[k]) + delta;
+          const double wl = SAME ? w : rcp_pos(sd);
           c.W1 += wl;
           c.W2 += wl * wl;
           if (with_log) c.LS += log(sd);
@@ -165,15 +172,20 @@ struct ColLik {
     } else {
       for (int64_t k = lane; k < neff; k += 64) {
         const double y = (double)uy[((k >> 5) * p_pad + p) * 32 + (k & 31)];
-        const double w = 1.0 / (s[k] + delta);
+        const double w = rcp_pos(s[k] + delta);
         const double yw = y * y * w;
         c.R += yw;
         c.R2 += yw * w;
         c.R3 += yw * w * w;
+        if constexpr (SAME) {
+          c.W1 += w;
+          c.W2 += w * w;
+          if (with_log) c.LS += log(s[k] + delta);
+        }
       }
-      for (int64_t k = lane; k < nlam; k += 64) {
+      if constexpr (!SAME) for (int64_t k = lane; k < nlam; k += 64) {
         const double sd = s_lam[k] + delta;
-        const double w = 1.0 / sd;
+        const double w = rcp_pos(sd);
         c.W1 += w;
         c.W2 += w * w;
         if (with_log) c.LS += log(sd);
@@ -188,23 +200,48 @@ struct ColLik {
     return c;
   }
 
-  __device__ double y_at(int64_t p, int i, int64_t k) const {
-    if constexpr (KPL > 0) return yr[i];
-    else return (double)uy[((k >> 5) * p_pad + p) * 32 + (k & 31)];
+  // FP32 slope L'(t) and curvature L''(t) (register path only): a cheap first approximation of the root
+  __device__ void slope32(float t, int lane, float& g, float& gp) const {
+    const float delta = __expf(t);
+    float R = 0.f, R2 = 0.f, R3 = 0.f, W1 = 0.f, W2 = 0.f;
+#pragma unroll
+    for (int i = 0; i < (KPL > 0 ? KPL : 1); ++i) {
+      const int k = lane + 64 * i;
+      const float w = __builtin_amdgcn_rcpf(lsf[k] + delta);
+      const float yw = yr[i] * yr[i] * w;
+      R += yw;
+      R2 += yw * w;
+      R3 += yw * w * w;
+      if (k < nlam) {
+        const float wl = SAME ? w : __builtin_amdgcn_rcpf(llf[k] + delta);
+        W1 += wl;
+        W2 += wl * wl;
+      }
+    }
+    R = wave_sum(R);
+    R2 = wave_sum(R2);
+    R3 = wave_sum(R3);
+    W1 = wave_sum(W1);
+    W2 = wave_sum(W2);
+    const float n = (float)nco;
+    const float h = W1 - n * R2 / R;
+    const float hp = -W2 + n * (2.f * R3 * R - R2 * R2) / (R * R);
+    g = delta * h;
+    gp = delta * h + delta * delta * hp;
   }
 };
 
 constexpr double LOG2PI = 1.8378770664093453;
 
-template <int KPL>
-__device__ inline double col_loss(const ColLik<KPL>& L, int64_t p, double t, int lane) {
+template <class LK>
+__device__ inline double col_loss(const LK& L, int64_t p, double t, int lane) {
   const ColSums c = L.sums(p, exp(t), true, lane);
   return L.nco * (LOG2PI + log(fmax(c.R, FLOOR64) / L.nco) + 1.0) + c.LS;
 }
 
 // slope L'(t) and curvature L''(t)
-template <int KPL>
-__device__ inline void col_slope(const ColLik<KPL>& L, int64_t p, double t, int lane, double& g, double& gp) {
+template <class LK>
+__device__ inline void col_slope(const LK& L, int64_t p, double t, int lane, double& g, double& gp) {
   const double delta = exp(t);
   const ColSums c = L.sums(p, delta, false, lane);
   const double n = L.nco;
@@ -219,8 +256,8 @@ __device__ inline void col_slope(const ColLik<KPL>& L, int64_t p, double t, int 
   gp = delta * h + delta * delta * hp;
 }
 
-template <int KPL>
-__device__ inline double golden_bracket(const ColLik<KPL>& L, int64_t p, int i, int G, double lo, double hi, int lane) {
+template <class LK>
+__device__ inline double golden_bracket(const LK& L, int64_t p, int i, int G, double lo, double hi, int lane) {
   const double inv_phi = 2.0 / (1.0 + sqrt(5.0));
   const double inv_phi2 = inv_phi * inv_phi;
   double a = grid_value(i > 0 ? i - 1 : 0, G, lo, hi);
@@ -237,25 +274,54 @@ __device__ inline double golden_bracket(const ColLik<KPL>& L, int64_t p, int i, 
   return 0.5 * (a + b);
 }
 
-// safeguarded Newton on L'(t) in the grid bracket [g[i-1], g[i+1]]
-template <int KPL>
-__device__ inline double newton_bracket(const ColLik<KPL>& L, int64_t p, int i, int G, double lo, double hi, int lane) {
-  double a = grid_value(i > 0 ? i - 1 : 0, G, lo, hi);
-  double b = grid_value(i + 1 < G ? i + 1 : G - 1, G, lo, hi);
-  double ga, gpa, gb, gpb;
-  col_slope(L, p, a, lane, ga, gpa);
-  if (!(b > a) || ga >= 0.0) return a;
-  col_slope(L, p, b, lane, gb, gpb);
-  if (gb <= 0.0) return b;
+// safeguarded Newton on L'(t) in the grid bracket [g[i-1], g[i+1]], started at the grid optimum g[i].  The slopes
+// at the bracket ends are only needed for the edge cases (loss rising at the left end -> the left end; falling at
+// the right end -> the right end), so they are evaluated the first time an iterate tries to leave the bracket;
+// an interior minimum that Newton reaches directly costs no end evaluations.
+//
+// On the register path an FP32 Newton phase runs first (cheap passes, root to ~1e-5); its result, when it stays
+// inside the bracket and converges, is only the starting point of the FP64 iteration, which decides the answer.
+template <class LK, bool FP32_START>
+__device__ inline double newton_bracket(const LK& L, int64_t p, int i, int G, double lo, double hi, int lane) {
+  const double a0 = grid_value(i > 0 ? i - 1 : 0, G, lo, hi);
+  const double b0 = grid_value(i + 1 < G ? i + 1 : G - 1, G, lo, hi);
+  if (!(b0 > a0)) return a0;
+  double a = a0, b = b0;
   double t = grid_value(i, G, lo, hi);
   if (!(t > a && t < b)) t = 0.5 * (a + b);
+  if constexpr (FP32_START) {
+    float tf = (float)t;
+    for (int it = 0; it < 12; ++it) {
+      float g, gp;
+      L.slope32(tf, lane, g, gp);
+      const float tn = tf - g / gp;
+      if (!(gp > 0.f) || !((double)tn > a0 && (double)tn < b0)) break;
+      const bool conv = fabsf(tn - tf) <= 1e-5f * fmaxf(1.f, fabsf(tf));
+      tf = tn;
+      if (conv) {
+        t = (double)tf;
+        break;
+      }
+    }
+  }
+  bool ends_checked = false;
   for (int it = 0; it < 100; ++it) {
     double g, gp;
     col_slope(L, p, t, lane, g, gp);
     if (g == 0.0) break;
     if (g < 0.0) a = t; else b = t;
     double tn = t - g / gp;
-    if (!(gp > 0.0) || !(tn > a && tn < b)) tn = 0.5 * (a + b);
+    if (!(gp > 0.0) || !(tn > a && tn < b)) {
+      if (!ends_checked) {
+        ends_checked = true;
+        double ge, gpe;
+        col_slope(L, p, a0, lane, ge, gpe);
+        if (ge >= 0.0) return a0;
+        col_slope(L, p, b0, lane, ge, gpe);
+        if (ge <= 0.0) return b0;
+      }
+      tn = 0.5 * (a + b);
+    }
     const bool done = fabs(tn - t) <= 1e-13 * fmax(1.0, fabs(t)) || (b - a) <= 1e-13;
     t = tn;
     if (done) break;
@@ -263,7 +329,7 @@ __device__ inline double newton_bracket(const ColLik<KPL>& L, int64_t p, int i, 
   return t;
 }
 
-template <int PREC, int KPL>
+template <int PREC, int KPL, bool SAME>
 __global__ __launch_bounds__(256) void delta_refine_kernel(
     int mode, const float* __restrict__ uy, int64_t P, int64_t p_pad, int64_t kpad, const double* __restrict__ s,
     int64_t neff, const double* __restrict__ s_lam, int64_t nlam, double nco, int G, double lo, double hi,
@@ -272,6 +338,23 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
     float* __restrict__ q1s, float* __restrict__ q2s, double radius, double judge_tol, int* __restrict__ flags) {
   const int lane = threadIdx.x & 63;
   const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  // both spectra once per block in LDS (register path), padded with 1 beyond their lengths
+  constexpr int NS = KPL > 0 ? 64 * KPL : 1;
+  __shared__ double sh_s[NS], sh_l[SAME ? 1 : NS];
+  __shared__ float sh_sf[NS], sh_lf[SAME ? 1 : NS];
+  if constexpr (KPL > 0) {
+    for (int k = threadIdx.x; k < NS; k += blockDim.x) {
+      const double v = k < neff ? s[k] : 1.0;
+      sh_s[k] = v;
+      sh_sf[k] = (float)v;
+      if constexpr (!SAME) {
+        const double u = k < nlam ? s_lam[k] : 1.0;
+        sh_l[k] = u;
+        sh_lf[k] = (float)u;
+      }
+    }
+    __syncthreads();
+  }
   if (p >= p_pad) return;
   if (p >= P) {  // padding columns: zero operand rows
     for (int64_t k = lane; k < kpad; k += 64) {
@@ -285,7 +368,7 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
     if (PREC == PREC_SPLIT_FP16 && lane == 0) { q1s[p] = 1.f; q2s[p] = 1.f; }
     return;
   }
-  ColLik<KPL> L{uy, p_pad, s, neff, s_lam, nlam, nco, {}, {}, {}};
+  ColLik<KPL, SAME> L{uy, p_pad, s, neff, s_lam, nlam, nco, sh_s, SAME ? sh_s : sh_l, sh_sf, SAME ? sh_sf : sh_lf, {}};
   L.load(p, lane);
   double t;
   const int i = (mode == MODE_FIXED || mode == MODE_GRID) ? 0 : idx[p];
@@ -336,18 +419,32 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
   } else {
     t = 0.0;
   }
-  if (need_bracket) t = newton_bracket(L, p, i, G, lo, hi, lane);
+  if (need_bracket) t = newton_bracket<ColLik<KPL, SAME>, (KPL > 0)>(L, p, i, G, lo, hi, lane);
   if (mode != MODE_FIXED) t = fmin(fmax(t, lo), hi);
   const double delta = exp(t);
   // pass 1: the null residual rWr and (split-FP16) the column maxima that set the planes' scales
   double rw = 0.0, m1 = 0.0, m2 = 0.0;
-  for (int64_t k = lane; k < neff; k += 64) {
-    const double y = (double)uy[((k >> 5) * p_pad + p) * 32 + (k & 31)];
-    const double w = 1.0 / (s[k] + delta);
-    const double q1 = w * y;
+  auto elem = [&](int i, int64_t k, double& w, double& q1) {  // weight and w * y of element k (k < neff)
+    double y, sk;
+    if constexpr (KPL > 0) { y = (double)L.yr[i]; sk = sh_s[k]; }
+    else { y = (double)uy[((k >> 5) * p_pad + p) * 32 + (k & 31)]; sk = s[k]; }
+    w = rcp_pos(sk + delta);
+    q1 = w * y;
+  };
+  auto pass1 = [&](int i, int64_t k) {
+    if (k >= neff) return;
+    double w, q1;
+    elem(i, k, w, q1);
+    const double y = KPL > 0 ? (double)L.yr[i] : (double)uy[((k >> 5) * p_pad + p) * 32 + (k & 31)];
     rw += q1 * y;
     m1 = fmax(m1, fabs(q1));
     m2 = fmax(m2, w);
+  };
+  if constexpr (KPL > 0) {
+#pragma unroll
+    for (int i = 0; i < KPL; ++i) pass1(i, lane + 64 * i);
+  } else {
+    for (int64_t k = lane; k < neff; k += 64) pass1(0, k);
   }
   rw = wave_sum(rw);
   double sc1 = 1.0, sc2 = 1.0;
@@ -355,13 +452,12 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
     sc1 = pow2_scale(wave_max(m1));
     sc2 = pow2_scale(wave_max(m2));
   }
-  // pass 2: the column-side scan planes Q1 = w o y and Q2 = w
-  for (int64_t k = lane; k < kpad; k += 64) {
+  // pass 2: the column-side scan planes Q1 = w o y and Q2 = w (zero on the padding k in [neff, kpad))
+  auto pass2 = [&](int64_t k) {  // re-reads y from HBM: keeping pass 2 rolled holds the register count down
     double w = 0.0, q1 = 0.0;
     if (k < neff) {
-      const double y = (double)uy[((k >> 5) * p_pad + p) * 32 + (k & 31)];
-      w = 1.0 / (s[k] + delta);
-      q1 = w * y;
+      w = rcp_pos(s[k] + delta);
+      q1 = w * (double)uy[((k >> 5) * p_pad + p) * 32 + (k & 31)];
     }
     const int64_t off = ((k >> 5) * p_pad + p) * 32 + (k & 31);
     if constexpr (is_split(PREC)) {
@@ -376,7 +472,8 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
       ((float*)q1a)[off] = (float)q1;
       ((float*)q2a)[off] = (float)w;
     }
-  }
+  };
+  for (int64_t k = lane; k < kpad; k += 64) pass2(k);
   if (lane == 0) {
     log_delta[p] = t;
     rWr[p] = fmax(rw, FLOOR64);
@@ -417,12 +514,17 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
                                void* q2b, float* q1s, float* q2s, double radius, double judge_tol, int* flags,
                                hipStream_t stream) {
   const dim3 grid((unsigned)((p_pad + 3) / 4));
-  // columns that fit 16 elements per lane (neff, nlam <= 1024) keep them in registers
-  const bool regs = neff <= 1024 && nlam <= 1024;
+  // columns that fit 16 elements per lane (neff, nlam, kpad <= 1024) keep them in registers
+  const bool regs = neff <= 1024 && nlam <= 1024 && kpad <= 1024;
+  const bool same = s_lam == s && nlam == neff;
+#define FLMM_REFINE1(PR, K, SM)                                                                                     \
+  hipLaunchKernelGGL((delta_refine_kernel<PR, K, SM>), grid, dim3(256), 0, stream, mode, uy, P, p_pad, kpad, s,    \
+                     neff, s_lam, nlam, nco, G, lo, hi, idx, log_delta, rWr, h2, q1a, q1b, q2a, q2b, q1s, q2s,       \
+                     radius, judge_tol, flags)
 #define FLMM_REFINE(PR, K)                                                                                          \
-  hipLaunchKernelGGL((delta_refine_kernel<PR, K>), grid, dim3(256), 0, stream, mode, uy, P, p_pad, kpad, s, neff,  \
-                     s_lam, nlam, nco, G, lo, hi, idx, log_delta, rWr, h2, q1a, q1b, q2a, q2b, q1s, q2s, radius,     \
-                     judge_tol, flags)
+  do {                                                                                                              \
+    if (same) FLMM_REFINE1(PR, K, true); else FLMM_REFINE1(PR, K, false);                                           \
+  } while (0)
   if (prec == PREC_SPLIT_BF16) {
     if (regs) FLMM_REFINE(PREC_SPLIT_BF16, 16); else FLMM_REFINE(PREC_SPLIT_BF16, 0);
   } else if (prec == PREC_SPLIT_FP16) {
@@ -431,6 +533,7 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
     if (regs) FLMM_REFINE(PREC_FP32, 16); else FLMM_REFINE(PREC_FP32, 0);
   }
 #undef FLMM_REFINE
+#undef FLMM_REFINE1
   return hipGetLastError();
 }
 

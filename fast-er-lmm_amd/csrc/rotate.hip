@@ -33,14 +33,21 @@ __global__ __launch_bounds__(NTHREADS, 1) void rotate_kernel(TileArgs<PREC, 1> a
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int64_t rows_pad = args.a_rows_pad;
+  // split-FP16 scales are powers of two, so multiplying by their reciprocals divides them out exactly
+  double inv_b[2] = {1.0, 1.0};
+  if constexpr (PREC == PREC_SPLIT_FP16) {
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) inv_b[bj] = 1.0 / (double)b_scale[j0 + acc_col(wn, bj, lane)];
+  }
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
       const int64_t i = i0 + acc_row(wm, bi, lane, v);
-      double sa = 1.0, s1 = 1.0, s2 = 1.0;
+      double inv_a = 1.0, s1 = 1.0, s2 = 1.0;
       if constexpr (PREC == PREC_SPLIT_FP16) {
-        sa = (double)a_scale[i];
+        const double sa = (double)a_scale[i];
+        inv_a = 1.0 / sa;
         s1 = sa;
         s2 = sa * sa * (1.0 / 16384.0);
         if (MODE == MODE_US && jt == 0 && wn == 0 && (lane & 31) == 0) {
@@ -53,7 +60,7 @@ __global__ __launch_bounds__(NTHREADS, 1) void rotate_kernel(TileArgs<PREC, 1> a
         const int64_t j = j0 + acc_col(wn, bj, lane);
         const int64_t off = ((j >> 5) * rows_pad + i) * 32 + (j & 31);
         double x = (double)acc.c[0][bi][bj][v];
-        if constexpr (PREC == PREC_SPLIT_FP16) x /= sa * (double)b_scale[j];
+        if constexpr (PREC == PREC_SPLIT_FP16) x *= inv_a * inv_b[bj];
         if constexpr (MODE == MODE_UY) {
           ((float*)o0)[off] = (float)x;
         } else if constexpr (is_split(PREC)) {

@@ -4,18 +4,19 @@
 //     buffered: loaded into registers at the start of a 32-k chunk, stored to the other buffer at its end, one
 //     barrier per chunk;
 //   * each wave owns 64 columns and streams its column-side fragments (Q1 = w o UY, Q2 = w) straight from global
-//     memory into registers one 16-k MFMA step ahead; a 32-row block of one chunk is 2 KiB contiguous in the
-//     plane layout, so the fragment loads are coalesced and need no LDS;
+//     memory into registers one K chunk ahead; a 32-row block of one chunk is 2 KiB contiguous in the plane
+//     layout, so the fragment loads are coalesced and need no LDS;
 //   * a wave covers all 128 rows of the tile, so the per-column maximum needs no cross-wave exchange.
-// Register budget (one wave per SIMD): 256 accumulators (2 products x 4 x 2 blocks of 32 x 32) in AGPRs, four
-// column-fragment sets (32 VGPRs each for the split formats), 32 VGPRs of SNP-side staging, the A fragments of
-// one step and addressing.
+// Register budget (one wave per SIMD): 256 accumulators (2 products x 4 x 2 blocks of 32 x 32) in AGPRs, two
+// 64-VGPR column-fragment sets (one K chunk each), 32 VGPRs of SNP-side staging, the A fragments of one step and
+// addressing.
 #include "kernels.h"
 
 namespace flmm {
 
 constexpr int W_TM = 128;   // SNP rows per workgroup
 constexpr int W_TP = 256;   // columns per workgroup (64 per wave)
+constexpr int64_t W_GM = 8; // SNP tiles per scheduling group
 constexpr int EP_WAVE = 2 * 64 * 32 * 4;  // epilogue staging bytes per wave: [g][64 cols][32 rows] floats
 
 // float offset of (column col, 4-row group seg) in a wave's epilogue staging area
@@ -38,10 +39,11 @@ struct WideTraits<PREC_FP32> {
   static __device__ inline int swz(int row) { return (row >> 1) & 7; }
 };
 
-// column-side fragments of one MFMA step: [g][bj][sub] -> v[(g * 2 + bj) * NSUB + sub], 16 bytes per lane each
-template <int NSUB>
-struct BFrag {
-  u32x4 v[2 * 2 * NSUB];
+// column-side fragments of one K chunk: [g][bj][sub] -> v[(g * 2 + bj) * NSUB + sub], NSTEP x 16 bytes per lane
+// (the lane's contiguous half-row of the chunk), piece q feeding MFMA step q
+template <int NSUB, int NSTEP>
+struct BChunk {
+  u32x4 v[2 * 2 * NSUB][NSTEP];
 };
 
 // SNP-side block of one chunk held in registers between its global load and its LDS store
@@ -52,7 +54,8 @@ struct AStage {
 
 template <int PREC>
 __global__ __launch_bounds__(256, 1) void wald_scan_wide_kernel(
-    TileArgs<PREC, 2> args, ScanScales sc, int64_t n_mtiles, int64_t n_tiles, int64_t Mc, int64_t P, int64_t n_real,
+    TileArgs<PREC, 2> args, ScanScales sc, int64_t n_mtiles, int64_t n_ptiles, int64_t n_tiles, int64_t Mc, int64_t P,
+    int64_t n_real,
     const double* __restrict__ rWr, const double* __restrict__ h2, double nu, const int64_t* __restrict__ row_index,
     ScanOut out, double* __restrict__ pmax, int64_t pmax_ld) {
   using T = WideTraits<PREC>;
@@ -65,17 +68,22 @@ __global__ __launch_bounds__(256, 1) void wald_scan_wide_kernel(
   constexpr int ROWS_PER_BLK = 1024 / T::ROWB;
   constexpr int SEGS = T::ROWB / 16;               // 16-byte segments per row
   constexpr int NSTEP = T::ROWB / 32;              // 16-byte column-fragment steps per chunk
-  constexpr int NBR = 2 * 2 * NSUB;                // column-fragment loads per step
   static_assert(BLK_PER_PLANE % NA == 0, "a wave's blocks stay inside one plane");
-  static_assert((2 * NSTEP) % 4 == 0 && 2 * NSTEP <= 8, "a chunk pair cycles the four column-fragment sets");
   static_assert(2 * EP_WAVE <= STAGE, "epilogue staging of two waves fits one stage buffer");
 
   // two stage buffers; the chunk loop is unrolled by two so every access names a fixed buffer
   __shared__ __attribute__((aligned(16))) char lds0[STAGE];
   __shared__ __attribute__((aligned(16))) char lds1[STAGE];
+  // Grouped order: each XCD walks a contiguous range of logical tiles (xcd_remap), and within a group of W_GM SNP
+  // tiles the SNP tile varies fastest, so the ~32 workgroups an XCD runs at once cover ~8 SNP tiles x 4 column
+  // tiles: each SNP-side chunk is shared by ~4 of them and each column-side chunk by ~8 while both are in L2.
   const int64_t tile = xcd_remap(blockIdx.x, n_tiles);
-  const int64_t mt = tile % n_mtiles;
-  const int64_t pt = tile / n_mtiles;
+  const int64_t group = tile / (W_GM * n_ptiles);
+  const int64_t first_m = group * W_GM;
+  const int64_t gm = n_mtiles - first_m < W_GM ? n_mtiles - first_m : W_GM;
+  const int64_t r_in = tile - group * (W_GM * n_ptiles);
+  const int64_t mt = first_m + r_in % gm;
+  const int64_t pt = r_in / gm;
   const int64_t m0 = mt * W_TM, p0 = pt * W_TP;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -111,17 +119,21 @@ __global__ __launch_bounds__(256, 1) void wald_scan_wide_kernel(
       *(u32x4*)(stage + pl_w * PLANE_LDS + r * T::ROWB + (((lane % SEGS) ^ T::swz(r)) * 16)) = f.v[i];
     }
   };
-  auto load_b = [&](int kc, int step) {
-    BFrag<NSUB> f;
-    const int64_t off = (int64_t)kc * b_kstride + b_lane + step * 16;
+  // the lane's whole half-row of chunk kc for every column block: both 16-byte pieces of a 32-byte span are
+  // fetched back to back, so each 64-byte row chunk crosses L2 -> L1 once
+  auto load_b = [&](int kc) {
+    BChunk<NSUB, NSTEP> f;
+    const int64_t off = (int64_t)kc * b_kstride + b_lane;
 #pragma unroll
     for (int g = 0; g < 2; ++g)
 #pragma unroll
       for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
         for (int sub = 0; sub < NSUB; ++sub)
-          f.v[(g * 2 + bj) * NSUB + sub] =
-              *(const u32x4*)((const char*)args.b[g][sub] + off + bj * 32 * T::ROWB);
+#pragma unroll
+          for (int q = 0; q < NSTEP; ++q)
+            f.v[(g * 2 + bj) * NSUB + sub][q] =
+                *(const u32x4*)((const char*)args.b[g][sub] + off + bj * 32 * T::ROWB + q * 16);
     return f;
   };
 
@@ -136,7 +148,7 @@ __global__ __launch_bounds__(256, 1) void wald_scan_wide_kernel(
         for (int v = 0; v < 16; ++v) acc[g][i][j][v] = 0.f;
 
   // one MFMA step (16 physical k per lane half) of chunk stage `st` with column fragments bf
-  auto mma_step = [&](const char* st, int step, const BFrag<NSUB>& bf) {
+  auto mma_step = [&](const char* st, int step, const BChunk<NSUB, NSTEP>& bf) {
     if constexpr (is_split(PREC)) {
       const int seg = h * 2 + step;
 #pragma unroll
@@ -151,8 +163,8 @@ __global__ __launch_bounds__(256, 1) void wald_scan_wide_kernel(
         }
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj) {
-          const bf16x8 bh = __builtin_bit_cast(bf16x8, bf.v[(g * 2 + bj) * 2 + 0]);
-          const bf16x8 bl = __builtin_bit_cast(bf16x8, bf.v[(g * 2 + bj) * 2 + 1]);
+          const bf16x8 bh = __builtin_bit_cast(bf16x8, bf.v[(g * 2 + bj) * 2 + 0][step]);
+          const bf16x8 bl = __builtin_bit_cast(bf16x8, bf.v[(g * 2 + bj) * 2 + 1][step]);
 #pragma unroll
           for (int bi = 0; bi < 4; ++bi) {
             f32x16 c = acc[g][bi][bj];
@@ -175,7 +187,7 @@ __global__ __launch_bounds__(256, 1) void wald_scan_wide_kernel(
         }
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj) {
-          const f32x4 b4 = __builtin_bit_cast(f32x4, bf.v[g * 2 + bj]);
+          const f32x4 b4 = __builtin_bit_cast(f32x4, bf.v[g * 2 + bj][step]);
 #pragma unroll
           for (int e = 0; e < 4; ++e)
 #pragma unroll
@@ -186,47 +198,30 @@ __global__ __launch_bounds__(256, 1) void wald_scan_wide_kernel(
     }
   };
 
-  // Software pipeline over chunk pairs (the two LDS buffers).  MFMA steps j = 0, 1, ... of the pair use
-  // column-fragment set b(j % 4), and step j issues the loads of step j + 2 first, so a fragment load has two
-  // MFMA steps (~96 MFMAs) to land.  The next chunk's SNP-side block is loaded into registers at the start of a
-  // chunk and written to the other buffer at its end.  sched_barrier pins each load batch ahead of the MFMA
-  // block it overlaps.  Past the last chunk the loads repeat the last one (valid addresses, values unused).
-  auto load_bs = [&](int kc, int st) {
-    int kk = kc + st / NSTEP;
-    if (kk >= nchunks) kk = nchunks - 1;
-    return load_b(kk, st % NSTEP);
+  // Software pipeline over chunk pairs (the two LDS buffers, two column-fragment sets): at the start of chunk kc
+  // the SNP-side block and the column fragments of chunk kc + 1 are loaded into registers, so both have a whole
+  // chunk of MFMAs (96 per wave) to land; the SNP-side block is written to the other LDS buffer at the end of the
+  // chunk.  sched_barrier keeps each load batch ahead of the MFMAs it overlaps.  Past the last chunk the loads
+  // repeat the last chunk (valid addresses, values unused).
+  auto chunk = [&](int kc, const char* cur, char* nxt, const BChunk<NSUB, NSTEP>& bc, BChunk<NSUB, NSTEP>& bn) {
+    const int kn = kc + 1 < nchunks ? kc + 1 : kc;
+    const AStage<NA> an = load_a(kn);
+    bn = load_b(kn);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int step = 0; step < NSTEP; ++step) mma_step(cur, step, bc);
+    __builtin_amdgcn_sched_barrier(0);
+    store_a(an, nxt);  // nxt was last read in the previous chunk, which ended with a barrier
+    __syncthreads();
   };
-  BFrag<NSUB> b0 = load_bs(0, 0), b1 = load_bs(0, 1), b2, b3;
-  AStage<NA> an;
+  BChunk<NSUB, NSTEP> be = load_b(0), bo;
   store_a(load_a(0), lds0);
   __syncthreads();
-  // step J of a chunk pair: chunk kc + J / NSTEP, MFMA step J % NSTEP, fragments BUSE; loads step J + 2 into BLOAD
-#define FLMM_WIDE_STEP(J, BUSE, BLOAD)                                                                               \
-  if constexpr ((J) < 2 * NSTEP) {                                                                                   \
-    constexpr int half = (J) / NSTEP, st = (J) % NSTEP;                                                              \
-    const int kcc = kc + half;                                                                                       \
-    if constexpr (st == 0) an = load_a(kcc + 1 < nchunks ? kcc + 1 : kcc);                                           \
-    BLOAD = load_bs(kcc, st + 2);                                                                                    \
-    __builtin_amdgcn_sched_barrier(0);                                                                               \
-    mma_step(half ? lds1 : lds0, st, BUSE);                                                                          \
-    __builtin_amdgcn_sched_barrier(0);                                                                               \
-    if constexpr (st == NSTEP - 1) {                                                                                 \
-      store_a(an, half ? lds0 : lds1); /* last read in the previous chunk, which ended with a barrier */            \
-      __syncthreads();                                                                                               \
-    }                                                                                                                \
-  }
   // nchunks is even (K is padded to a multiple of 128)
   for (int kc = 0; kc < nchunks; kc += 2) {
-    FLMM_WIDE_STEP(0, b0, b2)
-    FLMM_WIDE_STEP(1, b1, b3)
-    FLMM_WIDE_STEP(2, b2, b0)
-    FLMM_WIDE_STEP(3, b3, b1)
-    FLMM_WIDE_STEP(4, b0, b2)
-    FLMM_WIDE_STEP(5, b1, b3)
-    FLMM_WIDE_STEP(6, b2, b0)
-    FLMM_WIDE_STEP(7, b3, b1)
+    chunk(kc, lds0, lds1, be, bo);
+    chunk(kc + 1, lds1, lds0, bo, be);
   }
-#undef FLMM_WIDE_STEP
 
   // epilogue: this wave owns columns p0 + 64 wave + [0, 64) and all 128 rows.  The LDS stages are free after the
   // last barrier; each 32-row block of the wave's accumulators is staged there column-major ([g][col][row], the
@@ -310,7 +305,7 @@ hipError_t launch_wald_scan_wide(const TileArgs<PREC, 2>& args, const ScanScales
   const int64_t n_ptiles = (P + W_TP - 1) / W_TP;
   const int64_t n_tiles = n_mtiles * n_ptiles;
   hipLaunchKernelGGL(wald_scan_wide_kernel<PREC>, dim3((unsigned)n_tiles), dim3(256), 0, stream, args, sc, n_mtiles,
-                     n_tiles, Mc, P, n_real, rWr, h2, nu, row_index, out, pmax, args.b_rows_pad);
+                     n_ptiles, n_tiles, Mc, P, n_real, rWr, h2, nu, row_index, out, pmax, args.b_rows_pad);
   return hipGetLastError();
 }
 
