@@ -201,15 +201,37 @@ __global__ __launch_bounds__(256, 1) void wald_scan_wide_kernel(
   // Software pipeline over chunk pairs (the two LDS buffers, two column-fragment sets): at the start of chunk kc
   // the SNP-side block and the column fragments of chunk kc + 1 are loaded into registers, so both have a whole
   // chunk of MFMAs (96 per wave) to land; the SNP-side block is written to the other LDS buffer at the end of the
-  // chunk.  sched_barrier keeps each load batch ahead of the MFMAs it overlaps.  Past the last chunk the loads
-  // repeat the last chunk (valid addresses, values unused).
+  // chunk.  Past the last chunk the loads repeat the last chunk (valid addresses, values unused).
   auto chunk = [&](int kc, const char* cur, char* nxt, const BChunk<NSUB, NSTEP>& bc, BChunk<NSUB, NSTEP>& bn) {
     const int kn = kc + 1 < nchunks ? kc + 1 : kc;
     const AStage<NA> an = load_a(kn);
     bn = load_b(kn);
-    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (!is_split(PREC)) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int step = 0; step < NSTEP; ++step) mma_step(cur, step, bc);
+    if constexpr (is_split(PREC)) {
+      // Interleave per MFMA step (16 LDS fragment reads, 48 MFMAs, and 12 of the chunk's 24 global loads): the
+      // first product's 8 fragment reads up front, the second product's 8 reads and 4 global loads spread over the
+      // first product's 24 MFMAs, the other 8 global loads over the second product's 24 MFMAs.  (Masks: 0x008
+      // MFMA, 0x020 VMEM read, 0x100 DS read.)
+#pragma unroll
+      for (int step = 0; step < NSTEP; ++step) {
+        __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+          __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+          __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
+          __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+        }
+      }
+    }
     __builtin_amdgcn_sched_barrier(0);
     store_a(an, nxt);  // nxt was last read in the previous chunk, which ended with a barrier
     __syncthreads();
@@ -223,21 +245,32 @@ __global__ __launch_bounds__(256, 1) void wald_scan_wide_kernel(
     chunk(kc + 1, lds1, lds0, bo, be);
   }
 
-  // epilogue: this wave owns columns p0 + 64 wave + [0, 64) and all 128 rows.  The LDS stages are free after the
-  // last barrier; each 32-row block of the wave's accumulators is staged there column-major ([g][col][row], the
-  // 16-byte row groups of a column XOR-swizzled so both the b128 writes and the b128 column reads are bank-conflict
-  // free), then lane
-  // c owns column p0 + 64 wave + c and walks the rows: per-row quantities are wave-uniform scalar loads and the
-  // FP64 tail is emitted once per loop body, not once per accumulator slot.
-  // reciprocal row scales of the tile's SNPs, once per tile (1 without FP16 scales)
+  // ---- epilogue: this wave owns columns p0 + 64 wave + [0, 64) and all 128 rows.
+  // Reciprocal row scales of the tile's SNPs (exact powers of two for split-FP16, 1 otherwise), once per tile.
   __shared__ double row_inv[2][W_TM];
+  __shared__ __attribute__((aligned(16))) float row_invf[2][W_TM];
   {
     const int t = threadIdx.x & (W_TM - 1), which = threadIdx.x / W_TM;
     const float* scl = which ? sc.s2 : sc.s1;
     const int64_t m = m0 + t;
-    row_inv[which][t] = (scl && m < Mc) ? 1.0 / (double)scl[m] : 1.0;
+    const double r = (scl && m < Mc) ? 1.0 / (double)scl[m] : 1.0;
+    row_inv[which][t] = r;
+    row_invf[which][t] = (float)r;
   }
   __syncthreads();
+  const bool tile_real = p0 < n_real;
+
+  // Each 32-row block of the wave's accumulators is staged through the (now free) LDS stages column-major
+  // ([g][col][row], the 16-byte row groups of a column XOR-swizzled so both the b128 writes and the b128 column
+  // reads are bank-conflict free); lane c then owns column p0 + 64 wave + c and walks the rows, with per-row
+  // quantities wave-uniform.
+  //   * Tiles without real columns (the permutation columns, ~99% of tiles) only need each column's maximum F.
+  //     F is increasing in a^2 / d for a fixed column, so rows are compared in FP32 on a' = A / s1[m],
+  //     d' = D / s2[m] (the column scales are common to all rows and cancel), and F of the chosen row is then
+  //     evaluated in FP64 exactly as the per-test outputs are.
+  //   * Tiles holding real columns evaluate every test in FP64, write the per-test outputs of the real columns,
+  //     and take each column's maximum of those FP64 F values (a real column's maximum is the column max of f).
+  // Padding rows (m >= Mc) are skipped.
   float* ep = (float*)((wave < 2 ? lds0 : lds1) + (wave & 1) * EP_WAVE);
   const int64_t pc = p0 + wave * 64 + lane;
   const bool pv = pc < P;
@@ -246,8 +279,9 @@ __global__ __launch_bounds__(256, 1) void wald_scan_wide_kernel(
   const double iq1 = (sc.q1 && pv) ? 1.0 / (double)sc.q1[pc] : 1.0;
   const double iq2 = (sc.q2 && pv) ? 1.0 / (double)sc.q2[pc] : 1.0;
   const double hh = real ? h2[pc] : 0.0;
-  const bool tile_real = p0 < n_real;
-  double ba = 0.0, bd = 1.0;
+  double fmx = -INFINITY;                       // real tiles: FP64 running maximum
+  float bn = 0.f, bdd = 1.f, bA = 0.f, bD = 1.f;  // other tiles: best a'^2, its d', and its raw sums
+  int bt = -1;
 #pragma unroll
   for (int bi = 0; bi < 4; ++bi) {
 #pragma unroll
@@ -262,39 +296,68 @@ __global__ __launch_bounds__(256, 1) void wald_scan_wide_kernel(
         }
     __builtin_amdgcn_wave_barrier();
     const int64_t mb = m0 + bi * 32;
-#pragma unroll 1
-    for (int t4 = 0; t4 < 8; ++t4) {
-      const f32x4 a4 = *(const f32x4*)(ep + ep_off(lane, t4));
-      const f32x4 d4 = *(const f32x4*)(ep + ep_off(64 + lane, t4));
+    const int nrow = Mc - mb < 32 ? (int)(Mc - mb) : 32;  // >= 1 for the first block; may be <= 0 after
+    if (!tile_real) {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int64_t m = mb + 4 * t4 + e;
-        if (m >= Mc) break;
-        const int t = bi * 32 + 4 * t4 + e;
-        const double a = (double)a4[e] * iq1 * row_inv[0][t];
-        const double d = fmax((double)d4[e] * iq2 * row_inv[1][t], FLOOR64);
-        if (a * a * bd > ba * ba * d) { ba = a; bd = d; }
-        if (tile_real && real) {
+      for (int t4 = 0; t4 < 8; ++t4) {
+        const f32x4 a4 = *(const f32x4*)(ep + ep_off(lane, t4));
+        const f32x4 d4 = *(const f32x4*)(ep + ep_off(64 + lane, t4));
+        const f32x4 r1 = *(const f32x4*)&row_invf[0][bi * 32 + 4 * t4];
+        const f32x4 r2 = *(const f32x4*)&row_invf[1][bi * 32 + 4 * t4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float a1 = a4[e] * r1[e], d1 = d4[e] * r2[e];
+          const float n1 = a1 * a1;
+          const bool better = (4 * t4 + e < nrow) && (n1 * bdd > bn * d1);
+          bn = better ? n1 : bn;
+          bdd = better ? d1 : bdd;
+          bA = better ? a4[e] : bA;
+          bD = better ? d4[e] : bD;
+          bt = better ? bi * 32 + 4 * t4 + e : bt;
+        }
+      }
+    } else {
+#pragma unroll 1
+      for (int t4 = 0; t4 < 8; ++t4) {
+        const f32x4 a4 = *(const f32x4*)(ep + ep_off(lane, t4));
+        const f32x4 d4 = *(const f32x4*)(ep + ep_off(64 + lane, t4));
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          if (4 * t4 + e >= nrow) break;
+          const int64_t m = mb + 4 * t4 + e;
+          const int t = bi * 32 + 4 * t4 + e;
+          const double a = (double)a4[e] * iq1 * row_inv[0][t];
+          const double d = fmax((double)d4[e] * iq2 * row_inv[1][t], FLOOR64);
           const double beta = a / d;
           const double rss = fmax(rwr - a * a / d, FLOOR64);
           const double var = rss / nu / d;
-          const int64_t o = row_index[m] * out.ld + pc;
-          out.f[o] = beta * beta / var;
-          out.beta[o] = beta;
-          out.se[o] = sqrt(var);
-          out.sfve[o] = sqrt(fmax(a * a / (d * rwr), 0.0));
-          out.nullh2[o] = hh;
+          const double F = beta * beta / var;
+          fmx = fmax(fmx, F);
+          if (real) {
+            const int64_t o = row_index[m] * out.ld + pc;
+            out.f[o] = F;
+            out.beta[o] = beta;
+            out.se[o] = sqrt(var);
+            out.sfve[o] = sqrt(fmax(a * a / (d * rwr), 0.0));
+            out.nullh2[o] = hh;
+          }
         }
       }
     }
     __builtin_amdgcn_wave_barrier();
   }
-  if (pv) {
-    const double denom = fmax(bd, FLOOR64);
-    const double beta = ba / denom;
-    const double rss = fmax(rwr - ba * ba / denom, FLOOR64);
-    pmax[mt * pmax_ld + pc] = beta * beta / (rss / nu / denom);
+  if (!pv) return;
+  if (!tile_real) {
+    fmx = 0.0;
+    if (bt >= 0) {
+      const double a = (double)bA * iq1 * row_inv[0][bt];
+      const double d = fmax((double)bD * iq2 * row_inv[1][bt], FLOOR64);
+      const double beta = a / d;
+      const double rss = fmax(rwr - a * a / d, FLOOR64);
+      fmx = beta * beta / (rss / nu / d);
+    }
   }
+  pmax[mt * pmax_ld + pc] = fmx;
 }
 
 template <int PREC>
