@@ -41,9 +41,23 @@ static bool two_planes(int p) { return p != FLMM_PREC_FP32; }
 
 extern "C" {
 
-int flmm_abi_version(void) { return 2; }
+int flmm_abi_version(void) { return 3; }
 
 const char* flmm_last_error(void) { return g_err.c_str(); }
+
+int flmm_bed_decode(const uint8_t* bed, int64_t rec_bytes, const int64_t* row_idx, int64_t n,
+                    const int64_t* col_idx, int64_t m, int count_a1, int out_kind, void* out, int64_t ld,
+                    double* mean, double* std, void* stream) {
+  REQUIRE(out_kind == FLMM_BED_I8 || out_kind == FLMM_BED_F64 || out_kind == FLMM_BED_F32,
+          "flmm_bed_decode: bad output kind %d", out_kind);
+  REQUIRE(rec_bytes > 0 && n >= 0 && m >= 0 && ld >= m, "flmm_bed_decode: bad sizes");
+  REQUIRE(row_idx || n <= 4 * rec_bytes, "flmm_bed_decode: n exceeds the individuals of a record");
+  REQUIRE((bed && out) || n == 0 || m == 0, "flmm_bed_decode: null pointer");
+  if (n == 0 || m == 0) return FLMM_OK;
+  return check_launch(launch_bed_decode(out_kind, bed, rec_bytes, row_idx, n, col_idx, m, count_a1 != 0, out, ld,
+                                        mean, std, (hipStream_t)stream),
+                      "flmm_bed_decode");
+}
 
 int flmm_row_norms(const double* src, int64_t n_rows, int64_t n_k, int64_t row_stride, int64_t k_stride,
                    const int64_t* row_index, double* norm, void* stream) {

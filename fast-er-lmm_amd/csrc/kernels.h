@@ -75,4 +75,10 @@ __device__ inline void unscale_sums(const ScanScales& sc, int64_t m, double inv_
   }
 }
 
+// .bed decoding (bed.hip)
+enum BedOut : int { BED_I8 = 0, BED_F64 = 1, BED_F32 = 2 };
+hipError_t launch_bed_decode(int out_kind, const uint8_t* bed, int64_t rec_bytes, const int64_t* row_idx, int64_t n,
+                             const int64_t* col_idx, int64_t m, bool count_a1, void* out, int64_t ld,
+                             double* mean_out, double* std_out, hipStream_t stream);
+
 }  // namespace flmm

@@ -21,6 +21,9 @@ DELTA_GOLDEN = 0
 DELTA_NEWTON = 1
 DELTA_FIXED = 2
 DELTA_GRID = 4
+BED_I8 = 0
+BED_F64 = 1
+BED_F32 = 2
 
 _P = C.c_void_p
 _I64 = C.c_int64
@@ -32,6 +35,7 @@ SIGNATURES: dict[str, list] = {
     "flmm_abi_version": [],
     "flmm_last_error": [],
     "flmm_standardise": [_I, _P, _I64, _I64, _I64, _P, _I64, _P, _P, _P],
+    "flmm_bed_decode": [_P, _I64, _P, _I64, _P, _I64, _I, _I, _P, _I64, _P, _P, _P],
     "flmm_row_norms": [_P, _I64, _I64, _I64, _I64, _P, _P, _P],
     "flmm_pack_planes": [_I, _P, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _I64, _I64, _P],
     "flmm_rotate": [_I, _I, _P, _P, _P, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P],

@@ -33,6 +33,7 @@ enum flmm_precision { FLMM_PREC_SPLIT_BF16 = 0, FLMM_PREC_FP32 = 1, FLMM_PREC_SP
 enum flmm_rotate_mode { FLMM_ROTATE_UY = 0, FLMM_ROTATE_US = 1 };
 enum flmm_dtype { FLMM_DTYPE_F64 = 0, FLMM_DTYPE_I8 = 1 };
 enum flmm_delta_mode { FLMM_DELTA_GOLDEN = 0, FLMM_DELTA_NEWTON = 1, FLMM_DELTA_FIXED = 2, FLMM_DELTA_GRID = 4 };
+enum flmm_bed_out { FLMM_BED_I8 = 0, FLMM_BED_F64 = 1, FLMM_BED_F32 = 2 };
 
 /* ABI version of this header (bumped on any signature change). */
 int flmm_abi_version(void);
@@ -46,6 +47,20 @@ const char* flmm_last_error(void);
  * each, may be NULL) receive the column statistics.  src == dst is allowed for FP64. */
 int flmm_standardise(int dtype, const void* src, int64_t n, int64_t m, int64_t ld_src, double* dst, int64_t ld_dst,
                      double* mean, double* std, void* stream);
+
+/* Decode a PLINK 1 SNP-major .bed on the device, replacing the pysnptools reads of fasterlmm/io.py:54-67
+ * (read_plink) and fasterlmm/io_stream.py:60-71 (read_block, which standardises the block it reads).  bed = the
+ * file's bytes after its 3 magic bytes (0x6C 0x1B 0x01): one record of rec_bytes = ceil(n_file / 4) bytes per
+ * variant, individual i in byte i / 4 at bit 2 (i % 4).  Output row r < n is individual row_idx[r] (row_idx
+ * NULL: r), output column j < m is variant col_idx[j] (col_idx NULL: j); the caller guarantees
+ * row_idx[r] < 4 rec_bytes and col_idx[j] < the number of records.  out is (n, m) row-major, leading dimension
+ * ld.  count_a1 != 0: dosage = count of the first (.bim A1) allele, codes 00 -> 2, 10 -> 1, 11 -> 0,
+ * 01 -> missing; count_a1 == 0: 2 minus that.  out_kind FLMM_BED_I8: int8 dosages, -1 = missing;
+ * FLMM_BED_F64 / FLMM_BED_F32: the unit standardisation of flmm_standardise over the selected rows (io.py:247-254),
+ * with mean / std (m doubles each, may be NULL) receiving the statistics. */
+int flmm_bed_decode(const uint8_t* bed, int64_t rec_bytes, const int64_t* row_idx, int64_t n,
+                    const int64_t* col_idx, int64_t m, int count_a1, int out_kind, void* out, int64_t ld,
+                    double* mean, double* std, void* stream);
 
 /* L2 norm of each row r < n_rows of the matrix addressed as in flmm_pack_planes (the magnitude bound that sets
  * a split-FP16 row scale). */

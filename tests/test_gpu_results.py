@@ -1,0 +1,47 @@
+"""End to end on the GPU: permutation LOCO scan of the reference's example panel -> result files, checked
+against the reference's golden outputs through scipy's F(1, df2) tail."""
+from __future__ import annotations
+
+import numpy as np
+import pyarrow.parquet as pq
+import pytest
+from scipy import stats
+
+from conftest import golden
+from fasterlmm_amd import results
+
+pytestmark = pytest.mark.gpu
+
+
+def test_write_batch_matches_reference_pvalues(example, tmp_path):
+    from fasterlmm_amd.io import align_inputs
+    from fasterlmm_amd.perms import perm_threshold
+    n = example["Z_raw"].shape[0]
+    iid = [str(i) for i in range(n)]
+    data = align_inputs(iid, example["Z_raw"].astype(np.float64), iid, example["Y"], iid, example["X_cov2"][:, 1:],
+                        chrom=example["chrom"])
+    ref = golden("perm_loco_cov2.npz")
+    res, pmf = perm_threshold(data, [0, 1], n_perm=4, seed=19930909, loco=True)
+    N = data.Z.shape[0]
+    df2 = N - data.X.shape[1] - 1
+    Zr = example["Z_raw"].astype(np.float64)
+    ctx = results.writer_context([f"v{j}" for j in range(Zr.shape[1])], data.chrom, np.arange(Zr.shape[1]),
+                                 np.nanvar(Zr, axis=0), n_perm=4)
+    bw = results.BundleWriter(tmp_path / results.BUNDLE_FILENAME)
+    results.write_batch(ctx, tmp_path, ["p0", "p1"], data.Y[:, [0, 1]], res, pmf, df2, bundle_writer=bw)
+    tab = pq.read_table(bw.close()).to_pandas()
+    f_gpu, pmf_gpu = res.f.cpu().numpy(), pmf.cpu().numpy()
+    for j, name in enumerate(["p0", "p1"]):
+        t = tab[tab.Pheno == name].sort_values("sid_index")
+        # the writer lays down exactly F(1, df2).sf of the scan's own F (GPU tail vs scipy)
+        np.testing.assert_allclose(t.PValue.to_numpy(), stats.f.sf(f_gpu[:, j], 1, df2), rtol=1e-12)
+        thr_gpu = np.quantile(stats.f.sf(pmf_gpu[j], 1, df2), 0.05)
+        np.testing.assert_allclose(t.threshold.to_numpy()[0], thr_gpu, rtol=1e-12)
+        # and the scan behind it meets its bars against the reference's outputs
+        dlp = np.abs(np.log10(t.PValue.to_numpy()) - np.log10(stats.f.sf(ref["f"][:, j], 1, df2))).max()
+        assert dlp <= 1e-3, dlp
+        scale = np.maximum(np.abs(ref["beta"][:, j]), ref["se"][:, j])
+        assert (np.abs(t.SnpWeight.to_numpy() - ref["beta"][:, j]) / scale).max() <= 1e-5
+        thr_ref = np.quantile(stats.f.sf(ref["perm_max_F"][j], 1, df2), 0.05)
+        assert abs(np.log10(t.threshold.to_numpy()[0]) - np.log10(thr_ref)) <= 1e-3
+        assert (tmp_path / name / "gwas.tsv").exists()
