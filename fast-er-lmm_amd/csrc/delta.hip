@@ -387,12 +387,15 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
   } else if (mode == MODE_GOLDEN) {
     t = golden_bracket(L, p, i, G, lo, hi, lane);
   } else if (mode == MODE_LEARNED) {
-    // proposal t0 (in log_delta), undamped Newton inside [t0 - radius, t0 + radius]; accepted only if it converges
-    // there to a point whose loss is not worse than the grid optimum by more than judge_tol
+    // proposal t0 (in log_delta), undamped Newton inside [t0 - radius, t0 + radius].  Accepted only if it converges
+    // there, to a point inside the grid optimum's bracket [g[i-1], g[i+1]] (the basin the grid search refines),
+    // whose loss is not worse than the grid optimum's by more than judge_tol; otherwise the grid bracket is refined
     const double t0 = fmin(fmax(log_delta[p], lo), hi);
+    const double a_br = grid_value(i > 0 ? i - 1 : 0, G, lo, hi);
+    const double b_br = grid_value(i + 1 < G ? i + 1 : G - 1, G, lo, hi);
     double tt = t0;
     bool ok = false;
-    for (int it = 0; it < 8; ++it) {
+    for (int it = 0; it < 10; ++it) {
       double g, gp;
       col_slope(L, p, tt, lane, g, gp);
       if (!(gp > 0.0)) break;
@@ -404,6 +407,7 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
       tt = tn;
       if (conv) { ok = true; break; }
     }
+    ok = ok && tt >= a_br && tt <= b_br;
     if (ok) {
       const double lstar = col_loss(L, p, tt, lane);
       const double lgrid = col_loss(L, p, grid_value(i, G, lo, hi), lane);

@@ -40,10 +40,31 @@ __global__ __launch_bounds__(256) void mlp_features_kernel(const float* __restri
   }
   tot = wave_sum(tot);
   __syncthreads();
+  // per-bin log(mean y^2 / overall mean); empty bins interpolated linearly (in bin index) between the nearest
+  // occupied bins, held flat beyond the occupied range
+  __shared__ float val[4][NBIN];
   if (p < P && lane < NBIN) {
     const float mean_all = fmaxf(tot / (float)neff, 1e-30f);
     const int c = cnt[w][lane];
-    feat[p * 34 + lane] = c > 0 ? __logf(fmaxf(acc[w][lane] / (float)c, 1e-30f) / mean_all) : 0.f;
+    val[w][lane] = c > 0 ? __logf(fmaxf(acc[w][lane] / (float)c, 1e-30f) / mean_all) : 0.f;
+  }
+  __syncthreads();
+  if (p < P && lane < NBIN) {
+    // occupancy depends only on the spectrum, so every wave sees the same pattern
+    int lo = -1, hi = -1;
+    for (int b = lane; b >= 0; --b)
+      if (cnt[w][b] > 0) { lo = b; break; }
+    for (int b = lane; b < NBIN; ++b)
+      if (cnt[w][b] > 0) { hi = b; break; }
+    float v;
+    if (lo == lane) v = val[w][lane];
+    else if (lo >= 0 && hi >= 0) {
+      const float f = (float)(lane - lo) / (float)(hi - lo);
+      v = (1.f - f) * val[w][lo] + f * val[w][hi];
+    } else if (lo >= 0) v = val[w][lo];
+    else if (hi >= 0) v = val[w][hi];
+    else v = 0.f;
+    feat[p * 34 + lane] = v;
   }
   if (p < P && lane == 0) {
     feat[p * 34 + 32] = logN / 10.f;

@@ -115,9 +115,10 @@ int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_
 
 /* Learned-proposal refinement (no counterpart in the reference: the learned delta proposal).  log_delta[p]
  * holds the proposal t0 on entry.  Newton on the loss slope inside [t0 - radius, t0 + radius]; the result is
- * accepted when it converges there and its loss is at most judge_tol above the loss at the grid optimum
- * grid[idx[p]]; otherwise fallback[p] = 1 and the column is refined in the grid bracket as FLMM_DELTA_NEWTON
- * does.  Outputs as flmm_delta_refine; fallback may be NULL. */
+ * accepted when it converges there, lies in the grid optimum's bracket [grid[idx[p] - 1], grid[idx[p] + 1]],
+ * and its loss is at most judge_tol above the loss at the grid optimum grid[idx[p]]; otherwise fallback[p] = 1
+ * and the column is refined in the grid bracket as FLMM_DELTA_NEWTON does.  Outputs as flmm_delta_refine;
+ * fallback may be NULL. */
 int flmm_delta_refine_learned(int prec, const float* uy, int64_t P, int64_t p_pad, int64_t kpad, const double* s,
                               int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo,
                               double hi, const int32_t* idx, double radius, double judge_tol, double* log_delta,

@@ -19,14 +19,27 @@ def bins_of(s: np.ndarray, edges: np.ndarray) -> np.ndarray:
 
 
 def features(uy: np.ndarray, bin_of_k: np.ndarray, n: float, c: float) -> np.ndarray:
-    """(P, 34) features of the (neff, P) rotated columns uy."""
+    """(P, 34) features of the (neff, P) rotated columns uy: per eigenvalue bin, log(mean y^2 in the bin / mean
+    y^2); a bin holding no eigenvalue takes the linear interpolation (in bin index) of its nearest occupied
+    neighbours, and bins beyond the occupied range repeat the end value (all zero when no bin is occupied)."""
     y2 = uy * uy
     mean_all = np.maximum(y2.mean(axis=0), 1e-30)
     out = np.zeros((uy.shape[1], N_IN))
-    for b in range(NBIN):
-        m = bin_of_k == b
-        if m.any():
-            out[:, b] = np.log(np.maximum(y2[m].mean(axis=0), 1e-30) / mean_all)
+    occ = [b for b in range(NBIN) if np.any(bin_of_k == b)]
+    for b in occ:
+        out[:, b] = np.log(np.maximum(y2[bin_of_k == b].mean(axis=0), 1e-30) / mean_all)
+    if occ:
+        for b in range(NBIN):
+            if b in occ:
+                continue
+            left = [o for o in occ if o < b]
+            right = [o for o in occ if o > b]
+            if left and right:
+                lo, hi = left[-1], right[0]
+                f = (b - lo) / (hi - lo)
+                out[:, b] = (1 - f) * out[:, lo] + f * out[:, hi]
+            else:
+                out[:, b] = out[:, left[-1]] if left else out[:, right[0]]
     out[:, 32] = np.log(n) / 10
     out[:, 33] = c / 10
     return out
