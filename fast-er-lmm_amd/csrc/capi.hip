@@ -41,7 +41,7 @@ static bool two_planes(int p) { return p != FLMM_PREC_FP32; }
 
 extern "C" {
 
-int flmm_abi_version(void) { return 3; }
+int flmm_abi_version(void) { return 4; }
 
 const char* flmm_last_error(void) { return g_err.c_str(); }
 
@@ -286,6 +286,71 @@ int flmm_wald_scan(int prec, const void* s1_hi, const void* s1_lo, const void* s
   else FLMM_SCAN(PREC_FP32);
 #undef FLMM_SCAN
   return check_launch(e, "flmm_wald_scan");
+}
+
+int64_t flmm_scan_table_bytes(int n_groups) {
+  if (n_groups < 0) return -1;
+  return (int64_t)(4 + n_groups + 1) * 8 + (int64_t)n_groups * (int64_t)sizeof(flmm_scan_group);
+}
+
+int flmm_scan_table_fill(int prec, int variant, const flmm_scan_group* groups, int n_groups, void* host_table) {
+  REQUIRE(valid_prec(prec), "flmm_scan_table_fill: bad precision %d", prec);
+  REQUIRE(variant == FLMM_SCAN_SQUARE || variant == FLMM_SCAN_WIDE, "flmm_scan_table_fill: bad variant %d", variant);
+  REQUIRE(n_groups >= 0 && n_groups <= 65535, "flmm_scan_table_fill: n_groups must be in [0, 65535]");
+  REQUIRE(host_table && (groups || n_groups == 0), "flmm_scan_table_fill: null pointer");
+  const bool two = two_planes(prec);
+  const int64_t tp = variant == FLMM_SCAN_WIDE ? 256 : TILE;
+  int64_t* hdr = (int64_t*)host_table;
+  int64_t total = 0;
+  for (int g = 0; g < n_groups; ++g) {
+    const flmm_scan_group& j = groups[g];
+    REQUIRE(j.mc >= 0 && j.mc <= j.mc_pad && j.mc_pad % TILE == 0, "flmm_scan_table_fill: group %d: bad SNP extent", g);
+    REQUIRE(j.P >= 0 && j.P <= j.p_pad && j.p_pad % tp == 0,
+            "flmm_scan_table_fill: group %d: bad column extent (p_pad %% %lld required)", g, (long long)tp);
+    REQUIRE(j.kpad > 0 && j.kpad % TILE == 0, "flmm_scan_table_fill: group %d: kpad must be a positive multiple of 128",
+            g);
+    REQUIRE(j.n_real >= 0 && j.n_real <= j.P, "flmm_scan_table_fill: group %d: n_real must be in [0, P]", g);
+    REQUIRE(j.nu > 0.0, "flmm_scan_table_fill: group %d: dof must be positive", g);
+    REQUIRE(j.s1_hi && j.s2_hi && j.q1_hi && j.q2_hi && (!two || (j.s1_lo && j.s2_lo && j.q1_lo && j.q2_lo)),
+            "flmm_scan_table_fill: group %d: null operand plane", g);
+    REQUIRE(aligned16(j.s1_hi) && aligned16(j.s2_hi) && aligned16(j.q1_hi) && aligned16(j.q2_hi) &&
+                (!two || (aligned16(j.s1_lo) && aligned16(j.s2_lo) && aligned16(j.q1_lo) && aligned16(j.q2_lo))),
+            "flmm_scan_table_fill: group %d: operand planes must be 16-byte aligned", g);
+    REQUIRE(prec != FLMM_PREC_SPLIT_FP16 || (j.s1_scale && j.s2_scale && j.q1_scale && j.q2_scale),
+            "flmm_scan_table_fill: group %d: split-FP16 needs the four row-scale arrays", g);
+    REQUIRE(j.rWr && j.pmax && (j.max_F || j.P == 0), "flmm_scan_table_fill: group %d: null rWr / pmax / max_F", g);
+    REQUIRE(j.n_real == 0 || (j.f && j.beta && j.se && j.sfve && j.nullh2 && j.h2 && j.row_index),
+            "flmm_scan_table_fill: group %d: null output", g);
+    hdr[4 + g] = total;
+    total += ((j.mc + TILE - 1) / TILE) * ((j.P + tp - 1) / tp);
+  }
+  hdr[0] = n_groups;
+  hdr[1] = total;
+  hdr[2] = variant;
+  hdr[3] = 0;
+  hdr[4 + n_groups] = total;
+  if (n_groups > 0) memcpy(hdr + 4 + n_groups + 1, groups, (size_t)n_groups * sizeof(flmm_scan_group));
+  g_err.clear();
+  return FLMM_OK;
+}
+
+int flmm_wald_scan_grouped(int prec, int variant, const void* table, int blocks, void* stream) {
+  REQUIRE(valid_prec(prec), "flmm_wald_scan_grouped: bad precision %d", prec);
+  REQUIRE(variant == FLMM_SCAN_SQUARE || variant == FLMM_SCAN_WIDE, "flmm_wald_scan_grouped: bad variant %d",
+          variant);
+  REQUIRE(table && aligned16(table), "flmm_wald_scan_grouped: table must be a 16-byte aligned device pointer");
+  REQUIRE(blocks > 0 && blocks % 8 == 0, "flmm_wald_scan_grouped: blocks must be a positive multiple of 8, got %d",
+          blocks);
+  return check_launch(launch_wald_scan_grouped(prec, variant, (const int64_t*)table, blocks, (hipStream_t)stream),
+                      "flmm_wald_scan_grouped");
+}
+
+int flmm_max_reduce_grouped(const void* table, int n_groups, int64_t max_P, void* stream) {
+  REQUIRE(table, "flmm_max_reduce_grouped: null table");
+  REQUIRE(n_groups >= 0 && n_groups <= 65535 && max_P >= 0, "flmm_max_reduce_grouped: bad sizes");
+  if (n_groups == 0 || max_P == 0) return FLMM_OK;
+  return check_launch(launch_max_reduce_grouped((const int64_t*)table, n_groups, max_P, (hipStream_t)stream),
+                      "flmm_max_reduce_grouped");
 }
 
 int flmm_standardise(int dtype, const void* src, int64_t n, int64_t m, int64_t ld_src, double* dst, int64_t ld_dst,

@@ -75,6 +75,10 @@ __device__ inline void unscale_sums(const ScanScales& sc, int64_t m, double inv_
   }
 }
 
+// grouped scans (scan_grouped.hip); the table layout is described in include/fasterlmm_amd.h
+hipError_t launch_wald_scan_grouped(int prec, int variant, const int64_t* table, int blocks, hipStream_t stream);
+hipError_t launch_max_reduce_grouped(const int64_t* table, int n_groups, int64_t max_P, hipStream_t stream);
+
 // .bed decoding (bed.hip)
 enum BedOut : int { BED_I8 = 0, BED_F64 = 1, BED_F32 = 2 };
 hipError_t launch_bed_decode(int out_kind, const uint8_t* bed, int64_t rec_bytes, const int64_t* row_idx, int64_t n,

@@ -1,363 +1,16 @@
-// Wald scan, variant "wide": 128 SNP rows x 256 columns per workgroup.  Same arithmetic and epilogue as
-// scan.hip (reference fasterlmm/core.py:649-667); different data movement:
-//   * the SNP-side planes (S1 = US, S2 = US^2) are shared by the four waves and staged through LDS, double
-//     buffered: loaded into registers at the start of a 32-k chunk, stored to the other buffer at its end, one
-//     barrier per chunk;
-//   * each wave owns 64 columns and streams its column-side fragments (Q1 = w o UY, Q2 = w) straight from global
-//     memory into registers one K chunk ahead; a 32-row block of one chunk is 2 KiB contiguous in the plane
-//     layout, so the fragment loads are coalesced and need no LDS;
-//   * a wave covers all 128 rows of the tile, so the per-column maximum needs no cross-wave exchange.
-// Register budget (one wave per SIMD): 256 accumulators (2 products x 4 x 2 blocks of 32 x 32) in AGPRs, two
-// 64-VGPR column-fragment sets (one K chunk each), 32 VGPRs of SNP-side staging, the A fragments of one step and
-// addressing.
-#include "kernels.h"
+// Wald scan, variant "wide" (128 SNP rows x 256 columns per workgroup): one launch over one SNP block, the
+// tile body in scan_wide.h.
+#include "scan_wide.h"
 
 namespace flmm {
-
-constexpr int W_TM = 128;   // SNP rows per workgroup
-constexpr int W_TP = 256;   // columns per workgroup (64 per wave)
-constexpr int64_t W_GM = 8; // SNP tiles per scheduling group
-constexpr int EP_WAVE = 2 * 64 * 32 * 4;  // epilogue staging bytes per wave: [g][64 cols][32 rows] floats
-
-// float offset of (column col, 4-row group seg) in a wave's epilogue staging area
-__device__ inline int ep_off(int col, int seg) { return (col * 8 + (seg ^ ((col >> 1) & 7))) * 4; }
-
-template <int PREC>
-struct WideTraits;
-template <>
-struct WideTraits<PREC_SPLIT_BF16> {
-  static constexpr int NSUB = 2;       // planes per operand (hi, lo)
-  static constexpr int ROWB = 64;      // bytes per row per K chunk
-  static __device__ inline int swz(int row) { return (row >> 2) & 3; }
-};
-template <>
-struct WideTraits<PREC_SPLIT_FP16> : WideTraits<PREC_SPLIT_BF16> {};
-template <>
-struct WideTraits<PREC_FP32> {
-  static constexpr int NSUB = 1;
-  static constexpr int ROWB = 128;
-  static __device__ inline int swz(int row) { return (row >> 1) & 7; }
-};
-
-// column-side fragments of one K chunk: [g][bj][sub] -> v[(g * 2 + bj) * NSUB + sub], NSTEP x 16 bytes per lane
-// (the lane's contiguous half-row of the chunk), piece q feeding MFMA step q
-template <int NSUB, int NSTEP>
-struct BChunk {
-  u32x4 v[2 * 2 * NSUB][NSTEP];
-};
-
-// SNP-side block of one chunk held in registers between its global load and its LDS store
-template <int N>
-struct AStage {
-  u32x4 v[N];
-};
 
 template <int PREC>
 __global__ __launch_bounds__(256, 1) void wald_scan_wide_kernel(
     TileArgs<PREC, 2> args, ScanScales sc, int64_t n_mtiles, int64_t n_ptiles, int64_t n_tiles, int64_t Mc, int64_t P,
-    int64_t n_real,
-    const double* __restrict__ rWr, const double* __restrict__ h2, double nu, const int64_t* __restrict__ row_index,
-    ScanOut out, double* __restrict__ pmax, int64_t pmax_ld) {
-  using T = WideTraits<PREC>;
-  constexpr int NSUB = T::NSUB;
-  constexpr int NPL = 2 * NSUB;                    // SNP-side planes
-  constexpr int PLANE_LDS = W_TM * T::ROWB;        // bytes per plane per stage
-  constexpr int STAGE = NPL * PLANE_LDS;           // bytes per stage (32 KiB)
-  constexpr int BLK_PER_PLANE = PLANE_LDS / 1024;  // 1 KiB blocks (one 16-byte access per lane of a wave)
-  constexpr int NA = STAGE / 1024 / 4;             // blocks per wave per chunk
-  constexpr int ROWS_PER_BLK = 1024 / T::ROWB;
-  constexpr int SEGS = T::ROWB / 16;               // 16-byte segments per row
-  constexpr int NSTEP = T::ROWB / 32;              // 16-byte column-fragment steps per chunk
-  static_assert(BLK_PER_PLANE % NA == 0, "a wave's blocks stay inside one plane");
-  static_assert(2 * EP_WAVE <= STAGE, "epilogue staging of two waves fits one stage buffer");
-
-  // two stage buffers; the chunk loop is unrolled by two so every access names a fixed buffer
-  __shared__ __attribute__((aligned(16))) char lds0[STAGE];
-  __shared__ __attribute__((aligned(16))) char lds1[STAGE];
-  // Grouped order: each XCD walks a contiguous range of logical tiles (xcd_remap), and within a group of W_GM SNP
-  // tiles the SNP tile varies fastest, so the ~32 workgroups an XCD runs at once cover ~8 SNP tiles x 4 column
-  // tiles: each SNP-side chunk is shared by ~4 of them and each column-side chunk by ~8 while both are in L2.
-  const int64_t tile = xcd_remap(blockIdx.x, n_tiles);
-  const int64_t group = tile / (W_GM * n_ptiles);
-  const int64_t first_m = group * W_GM;
-  const int64_t gm = n_mtiles - first_m < W_GM ? n_mtiles - first_m : W_GM;
-  const int64_t r_in = tile - group * (W_GM * n_ptiles);
-  const int64_t mt = first_m + r_in % gm;
-  const int64_t pt = r_in / gm;
-  const int64_t m0 = mt * W_TM, p0 = pt * W_TP;
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int h = lane >> 5;
-  const int nchunks = args.nchunks;
-
-  // ---- SNP-side staging: wave w copies 1 KiB blocks [w NA, (w + 1) NA) of the stage, all inside plane pl_w, through
-  // registers (loads issued at the start of a chunk, written to the other stage at its end).  Lane L of a block
-  // handles row blk * ROWS_PER_BLK + L / SEGS, logical segment L % SEGS, stored at physical segment
-  // (L % SEGS) ^ swz(row).
-  const int pl_w = (wave * NA) / BLK_PER_PLANE;
-  const int blk0 = (wave * NA) % BLK_PER_PLANE;
-  const char* a_src = (const char*)args.a[pl_w / NSUB][pl_w % NSUB] + m0 * T::ROWB + (blk0 * 1024 + lane * 16);
-  const int64_t a_kstride = args.a_rows_pad * T::ROWB;
-  const int a_row0 = blk0 * ROWS_PER_BLK + lane / SEGS;
-
-  // ---- column side: lane (row p0 + 64 wave + 32 bj + lane % 32, half h) reads bytes [h ROWB / 2, (h + 1) ROWB / 2)
-  // of its row's chunk, 16 bytes per step
-  const int64_t b_lane = (p0 + wave * 64 + (lane & 31)) * T::ROWB + h * (T::ROWB / 2);
-  const int64_t b_kstride = args.b_rows_pad * T::ROWB;
-
-  auto load_a = [&](int kc) {
-    AStage<NA> f;
-    const char* src = a_src + (int64_t)kc * a_kstride;
-#pragma unroll
-    for (int i = 0; i < NA; ++i) f.v[i] = *(const u32x4*)(src + i * 1024);
-    return f;
-  };
-  auto store_a = [&](const AStage<NA>& f, char* stage) {
-#pragma unroll
-    for (int i = 0; i < NA; ++i) {
-      const int r = a_row0 + i * ROWS_PER_BLK;
-      *(u32x4*)(stage + pl_w * PLANE_LDS + r * T::ROWB + (((lane % SEGS) ^ T::swz(r)) * 16)) = f.v[i];
-    }
-  };
-  // the lane's whole half-row of chunk kc for every column block: both 16-byte pieces of a 32-byte span are
-  // fetched back to back, so each 64-byte row chunk crosses L2 -> L1 once
-  auto load_b = [&](int kc) {
-    BChunk<NSUB, NSTEP> f;
-    const int64_t off = (int64_t)kc * b_kstride + b_lane;
-#pragma unroll
-    for (int g = 0; g < 2; ++g)
-#pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-        for (int sub = 0; sub < NSUB; ++sub)
-#pragma unroll
-          for (int q = 0; q < NSTEP; ++q)
-            f.v[(g * 2 + bj) * NSUB + sub][q] =
-                *(const u32x4*)((const char*)args.b[g][sub] + off + bj * 32 * T::ROWB + q * 16);
-    return f;
-  };
-
-  f32x16 acc[2][4][2];
-#pragma unroll
-  for (int g = 0; g < 2; ++g)
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int v = 0; v < 16; ++v) acc[g][i][j][v] = 0.f;
-
-  // one MFMA step (16 physical k per lane half) of chunk stage `st` with column fragments bf
-  auto mma_step = [&](const char* st, int step, const BChunk<NSUB, NSTEP>& bf) {
-    if constexpr (is_split(PREC)) {
-      const int seg = h * 2 + step;
-#pragma unroll
-      for (int g = 0; g < 2; ++g) {
-        bf16x8 af[4][2];
-#pragma unroll
-        for (int bi = 0; bi < 4; ++bi) {
-          const int r = bi * 32 + (lane & 31);
-#pragma unroll
-          for (int sub = 0; sub < 2; ++sub)
-            af[bi][sub] = *(const bf16x8*)(st + (g * 2 + sub) * PLANE_LDS + r * 64 + ((seg ^ T::swz(r)) * 16));
-        }
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj) {
-          const bf16x8 bh = __builtin_bit_cast(bf16x8, bf.v[(g * 2 + bj) * 2 + 0][step]);
-          const bf16x8 bl = __builtin_bit_cast(bf16x8, bf.v[(g * 2 + bj) * 2 + 1][step]);
-#pragma unroll
-          for (int bi = 0; bi < 4; ++bi) {
-            f32x16 c = acc[g][bi][bj];
-            c = mfma_split<PREC>(af[bi][0], bh, c);
-            c = mfma_split<PREC>(af[bi][0], bl, c);
-            c = mfma_split<PREC>(af[bi][1], bh, c);
-            acc[g][bi][bj] = c;
-          }
-        }
-      }
-    } else {
-      const int seg = h * 4 + step;
-#pragma unroll
-      for (int g = 0; g < 2; ++g) {
-        f32x4 af[4];
-#pragma unroll
-        for (int bi = 0; bi < 4; ++bi) {
-          const int r = bi * 32 + (lane & 31);
-          af[bi] = *(const f32x4*)(st + g * PLANE_LDS + r * 128 + ((seg ^ T::swz(r)) * 16));
-        }
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj) {
-          const f32x4 b4 = __builtin_bit_cast(f32x4, bf.v[g * 2 + bj][step]);
-#pragma unroll
-          for (int e = 0; e < 4; ++e)
-#pragma unroll
-            for (int bi = 0; bi < 4; ++bi)
-              acc[g][bi][bj] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[bi][e], b4[e], acc[g][bi][bj], 0, 0, 0);
-        }
-      }
-    }
-  };
-
-  // Software pipeline over chunk pairs (the two LDS buffers, two column-fragment sets): at the start of chunk kc
-  // the SNP-side block and the column fragments of chunk kc + 1 are loaded into registers, so both have a whole
-  // chunk of MFMAs (96 per wave) to land; the SNP-side block is written to the other LDS buffer at the end of the
-  // chunk.  Past the last chunk the loads repeat the last chunk (valid addresses, values unused).
-  auto chunk = [&](int kc, const char* cur, char* nxt, const BChunk<NSUB, NSTEP>& bc, BChunk<NSUB, NSTEP>& bn) {
-    const int kn = kc + 1 < nchunks ? kc + 1 : kc;
-    const AStage<NA> an = load_a(kn);
-    bn = load_b(kn);
-    if constexpr (!is_split(PREC)) __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int step = 0; step < NSTEP; ++step) mma_step(cur, step, bc);
-    if constexpr (is_split(PREC)) {
-      // Interleave per MFMA step (16 LDS fragment reads, 48 MFMAs, and 12 of the chunk's 24 global loads): the
-      // first product's 8 fragment reads up front, the second product's 8 reads and 4 global loads spread over the
-      // first product's 24 MFMAs, the other 8 global loads over the second product's 24 MFMAs.  (Masks: 0x008
-      // MFMA, 0x020 VMEM read, 0x100 DS read.)
-#pragma unroll
-      for (int step = 0; step < NSTEP; ++step) {
-        __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
-          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-          __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
-          __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
-          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-        }
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-          __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
-          __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
-        }
-      }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    store_a(an, nxt);  // nxt was last read in the previous chunk, which ended with a barrier
-    __syncthreads();
-  };
-  BChunk<NSUB, NSTEP> be = load_b(0), bo;
-  store_a(load_a(0), lds0);
-  __syncthreads();
-  // nchunks is even (K is padded to a multiple of 128)
-  for (int kc = 0; kc < nchunks; kc += 2) {
-    chunk(kc, lds0, lds1, be, bo);
-    chunk(kc + 1, lds1, lds0, bo, be);
-  }
-
-  // ---- epilogue: this wave owns columns p0 + 64 wave + [0, 64) and all 128 rows.
-  // Reciprocal row scales of the tile's SNPs (exact powers of two for split-FP16, 1 otherwise), once per tile.
-  __shared__ double row_inv[2][W_TM];
-  __shared__ __attribute__((aligned(16))) float row_invf[2][W_TM];
-  {
-    const int t = threadIdx.x & (W_TM - 1), which = threadIdx.x / W_TM;
-    const float* scl = which ? sc.s2 : sc.s1;
-    const int64_t m = m0 + t;
-    const double r = (scl && m < Mc) ? 1.0 / (double)scl[m] : 1.0;
-    row_inv[which][t] = r;
-    row_invf[which][t] = (float)r;
-  }
-  __syncthreads();
-  const bool tile_real = p0 < n_real;
-
-  // Each 32-row block of the wave's accumulators is staged through the (now free) LDS stages column-major
-  // ([g][col][row], the 16-byte row groups of a column XOR-swizzled so both the b128 writes and the b128 column
-  // reads are bank-conflict free); lane c then owns column p0 + 64 wave + c and walks the rows, with per-row
-  // quantities wave-uniform.
-  //   * Tiles without real columns (the permutation columns, ~99% of tiles) only need each column's maximum F.
-  //     F is increasing in a^2 / d for a fixed column, so rows are compared in FP32 on a' = A / s1[m],
-  //     d' = D / s2[m] (the column scales are common to all rows and cancel), and F of the chosen row is then
-  //     evaluated in FP64 exactly as the per-test outputs are.
-  //   * Tiles holding real columns evaluate every test in FP64, write the per-test outputs of the real columns,
-  //     and take each column's maximum of those FP64 F values (a real column's maximum is the column max of f).
-  // Padding rows (m >= Mc) are skipped.
-  float* ep = (float*)((wave < 2 ? lds0 : lds1) + (wave & 1) * EP_WAVE);
-  const int64_t pc = p0 + wave * 64 + lane;
-  const bool pv = pc < P;
-  const bool real = pc < n_real;
-  const double rwr = pv ? rWr[pc] : 1.0;
-  const double iq1 = (sc.q1 && pv) ? 1.0 / (double)sc.q1[pc] : 1.0;
-  const double iq2 = (sc.q2 && pv) ? 1.0 / (double)sc.q2[pc] : 1.0;
-  const double hh = real ? h2[pc] : 0.0;
-  double fmx = -INFINITY;                       // real tiles: FP64 running maximum
-  float bn = 0.f, bdd = 1.f, bA = 0.f, bD = 1.f;  // other tiles: best a'^2, its d', and its raw sums
-  int bt = -1;
-#pragma unroll
-  for (int bi = 0; bi < 4; ++bi) {
-#pragma unroll
-    for (int g = 0; g < 2; ++g)
-#pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const f32x16& c = acc[g][bi][bj];
-          const f32x4 w4 = {c[4 * q], c[4 * q + 1], c[4 * q + 2], c[4 * q + 3]};
-          *(f32x4*)(ep + ep_off(g * 64 + bj * 32 + (lane & 31), 2 * q + h)) = w4;
-        }
-    __builtin_amdgcn_wave_barrier();
-    const int64_t mb = m0 + bi * 32;
-    const int nrow = Mc - mb < 32 ? (int)(Mc - mb) : 32;  // >= 1 for the first block; may be <= 0 after
-    if (!tile_real) {
-#pragma unroll
-      for (int t4 = 0; t4 < 8; ++t4) {
-        const f32x4 a4 = *(const f32x4*)(ep + ep_off(lane, t4));
-        const f32x4 d4 = *(const f32x4*)(ep + ep_off(64 + lane, t4));
-        const f32x4 r1 = *(const f32x4*)&row_invf[0][bi * 32 + 4 * t4];
-        const f32x4 r2 = *(const f32x4*)&row_invf[1][bi * 32 + 4 * t4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const float a1 = a4[e] * r1[e], d1 = d4[e] * r2[e];
-          const float n1 = a1 * a1;
-          const bool better = (4 * t4 + e < nrow) && (n1 * bdd > bn * d1);
-          bn = better ? n1 : bn;
-          bdd = better ? d1 : bdd;
-          bA = better ? a4[e] : bA;
-          bD = better ? d4[e] : bD;
-          bt = better ? bi * 32 + 4 * t4 + e : bt;
-        }
-      }
-    } else {
-#pragma unroll 1
-      for (int t4 = 0; t4 < 8; ++t4) {
-        const f32x4 a4 = *(const f32x4*)(ep + ep_off(lane, t4));
-        const f32x4 d4 = *(const f32x4*)(ep + ep_off(64 + lane, t4));
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          if (4 * t4 + e >= nrow) break;
-          const int64_t m = mb + 4 * t4 + e;
-          const int t = bi * 32 + 4 * t4 + e;
-          const double a = (double)a4[e] * iq1 * row_inv[0][t];
-          const double d = fmax((double)d4[e] * iq2 * row_inv[1][t], FLOOR64);
-          const double beta = a / d;
-          const double rss = fmax(rwr - a * a / d, FLOOR64);
-          const double var = rss / nu / d;
-          const double F = beta * beta / var;
-          fmx = fmax(fmx, F);
-          if (real) {
-            const int64_t o = row_index[m] * out.ld + pc;
-            out.f[o] = F;
-            out.beta[o] = beta;
-            out.se[o] = sqrt(var);
-            out.sfve[o] = sqrt(fmax(a * a / (d * rwr), 0.0));
-            out.nullh2[o] = hh;
-          }
-        }
-      }
-    }
-    __builtin_amdgcn_wave_barrier();
-  }
-  if (!pv) return;
-  if (!tile_real) {
-    fmx = 0.0;
-    if (bt >= 0) {
-      const double a = (double)bA * iq1 * row_inv[0][bt];
-      const double d = fmax((double)bD * iq2 * row_inv[1][bt], FLOOR64);
-      const double beta = a / d;
-      const double rss = fmax(rwr - a * a / d, FLOOR64);
-      fmx = beta * beta / (rss / nu / d);
-    }
-  }
-  pmax[mt * pmax_ld + pc] = fmx;
+    int64_t n_real, const double* __restrict__ rWr, const double* __restrict__ h2, double nu,
+    const int64_t* __restrict__ row_index, ScanOut out, double* __restrict__ pmax, int64_t pmax_ld) {
+  wide_tile<PREC>(args, sc, n_mtiles, n_ptiles, xcd_remap(blockIdx.x, n_tiles), Mc, P, n_real, rWr, h2, nu, row_index,
+                  out, pmax, pmax_ld);
 }
 
 template <int PREC>

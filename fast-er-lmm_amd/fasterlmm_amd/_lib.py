@@ -24,6 +24,8 @@ DELTA_GRID = 4
 BED_I8 = 0
 BED_F64 = 1
 BED_F32 = 2
+SCAN_SQUARE = 0
+SCAN_WIDE = 1
 
 _P = C.c_void_p
 _I64 = C.c_int64
@@ -34,6 +36,10 @@ _D = C.c_double
 SIGNATURES: dict[str, list] = {
     "flmm_abi_version": [],
     "flmm_last_error": [],
+    "flmm_scan_table_bytes": [_I],
+    "flmm_scan_table_fill": [_I, _I, _P, _I, _P],
+    "flmm_wald_scan_grouped": [_I, _I, _P, _I, _P],
+    "flmm_max_reduce_grouped": [_P, _I, _I64, _P],
     "flmm_standardise": [_I, _P, _I64, _I64, _I64, _P, _I64, _P, _P, _P],
     "flmm_bed_decode": [_P, _I64, _P, _I64, _P, _I64, _I, _I, _P, _I64, _P, _P, _P],
     "flmm_row_norms": [_P, _I64, _I64, _I64, _I64, _P, _P, _P],
@@ -57,6 +63,22 @@ SIGNATURES: dict[str, list] = {
     "flmm_effect_size": [_P, _I64, _I64, _P, _P, _P, _P],
 }
 
+# entry points that do not return an int status
+RESTYPES = {"flmm_last_error": C.c_char_p, "flmm_scan_table_bytes": C.c_int64}
+
+
+class ScanGroup(C.Structure):
+    """Mirror of ``flmm_scan_group``: one SNP block x column set of a grouped scan launch (device pointers)."""
+    _fields_ = [
+        ("s1_hi", _P), ("s1_lo", _P), ("s2_hi", _P), ("s2_lo", _P), ("s1_scale", _P), ("s2_scale", _P),
+        ("mc_pad", _I64),
+        ("q1_hi", _P), ("q1_lo", _P), ("q2_hi", _P), ("q2_lo", _P), ("q1_scale", _P), ("q2_scale", _P),
+        ("p_pad", _I64), ("kpad", _I64), ("mc", _I64), ("P", _I64), ("n_real", _I64),
+        ("rWr", _P), ("h2", _P), ("nu", _D), ("row_index", _P),
+        ("f", _P), ("beta", _P), ("se", _P), ("sfve", _P), ("nullh2", _P), ("pmax", _P), ("max_F", _P),
+    ]
+
+
 _lib: C.CDLL | None = None
 
 
@@ -71,7 +93,7 @@ def load() -> C.CDLL:
     for name, argtypes in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.argtypes = argtypes
-        fn.restype = C.c_char_p if name == "flmm_last_error" else C.c_int
+        fn.restype = RESTYPES.get(name, C.c_int)
     _lib = lib
     return lib
 

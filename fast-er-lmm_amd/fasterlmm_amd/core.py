@@ -192,15 +192,17 @@ class _Basis:
     def scan(self, Z: Tensor, row_index: Tensor, n_real: int, out: engine.ScanOutputs, d: engine.DeltaResult,
              block: int = 8192) -> None:
         """Scan the SNP columns Z[:, row_index] of Z (N, M_total) FP64 into out."""
-        M_tot = Z.shape[1]
         Mc = row_index.numel()
         for b0 in range(0, Mc, block):
             ri = row_index[b0:b0 + block]
-            mc = ri.numel()
-            spl = engine.pack(self.prec, Z, mc, self.N, 1, M_tot, rup(mc, TILE), self.npad, row_index=ri)
-            s1, s2 = engine.rotate_us(spl, self.bt)
-            del spl
-            engine.wald_scan(s1, s2, d, mc, self.P, n_real, self.nu, ri, out)
+            s1, s2 = self.snp_planes(Z, ri)
+            engine.wald_scan(s1, s2, d, ri.numel(), self.P, n_real, self.nu, ri, out)
+
+    def snp_planes(self, Z: Tensor, row_index: Tensor) -> tuple[engine.Planes, engine.Planes]:
+        """Scan planes S1 = B^T s, S2 = (B^T s)^2 of the SNP columns Z[:, row_index] (core.py:628-631)."""
+        mc = row_index.numel()
+        spl = engine.pack(self.prec, Z, mc, self.N, 1, Z.shape[1], rup(mc, TILE), self.npad, row_index=row_index)
+        return engine.rotate_us(spl, self.bt)
 
 
 def _projector(X: Tensor) -> tuple[Tensor, Tensor]:

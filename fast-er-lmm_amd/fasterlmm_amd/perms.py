@@ -59,3 +59,42 @@ def perm_threshold(data: AlignedDataset, pheno_idx: list[int], *, n_perm: int = 
         res = ScanResult(*(getattr(res, k).to(home) for k in ("f", "beta", "se", "sfve", "nullh2", "max_F")))
         perm_max_F = perm_max_F.to(home)
     return res, perm_max_F
+
+
+def perm_threshold_groups(data: AlignedDataset, pheno_idx: list[int], *, n_perm: int = 100, seed: int = 19930909,
+                          mode: str = "grouped", runner=None, on_chrom=None) -> list:
+    """The reference's NA-grouped permutation run (cli.py:337-375 with perms.py:21-68 per group) with the scans of
+    every group fused: phenotypes are grouped by NA pattern (io.group_by_na), each group keeps its observed
+    strains and is scanned with its reals and their shuffles (seeded by the full-table column, as the reference's
+    ``seed_idx``), all groups through one grouped launch per SNP block (mode="grouped") or one launch pair per
+    group (mode="loop").  Groups with fewer than C + 2 strains are skipped, as in the reference.
+
+    Returns [(keep_rows, cols, ScanResult of the group's reals, perm_max_F (B_g, n_perm))] per scanned group, equal
+    to ``perm_threshold(subset_dataset(data, keep, cols), range(len(cols)), seed_idx=cols)`` per group."""
+    from .grouped import loco_scan_compat_groups
+    from .io import group_by_na
+    dev = _lib.device()
+    C = data.X.shape[1]
+    groups = [(keep, cols) for keep, cols in group_by_na(data.Y, pheno_idx) if len(keep) - C - 1 >= 1]
+    if not groups:
+        return []
+    Ys, keeps, n_reals = [], [], []
+    for keep, cols in groups:
+        r = torch.as_tensor(keep, dtype=torch.long, device=data.Y.device)
+        y_real = data.Y[r][:, cols].to(device=dev, dtype=torch.float64)
+        orders = perm_orders(seed, cols, len(keep), n_perm)
+        Ys.append(pack_perm_columns(y_real, orders))
+        keeps.append(keep)
+        n_reals.append(len(cols))
+    res = loco_scan_compat_groups(data.Z, data.X, Ys, keeps, data.chrom, n_reals=n_reals, Z_kin=data.Z_kin,
+                                  chrom_kin=data.chrom_kin, mode=mode, runner=runner, on_chrom=on_chrom)
+    out = []
+    home = data.Y.device
+    for (keep, cols), r in zip(groups, res):
+        B = len(cols)
+        pmf = r.max_F[B:].reshape(B, n_perm)
+        if home.type != "cuda":
+            r = ScanResult(*(getattr(r, k).to(home) for k in ("f", "beta", "se", "sfve", "nullh2", "max_F")))
+            pmf = pmf.to(home)
+        out.append((keep, cols, r, pmf))
+    return out

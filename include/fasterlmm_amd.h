@@ -34,12 +34,63 @@ enum flmm_rotate_mode { FLMM_ROTATE_UY = 0, FLMM_ROTATE_US = 1 };
 enum flmm_dtype { FLMM_DTYPE_F64 = 0, FLMM_DTYPE_I8 = 1 };
 enum flmm_delta_mode { FLMM_DELTA_GOLDEN = 0, FLMM_DELTA_NEWTON = 1, FLMM_DELTA_FIXED = 2, FLMM_DELTA_GRID = 4 };
 enum flmm_bed_out { FLMM_BED_I8 = 0, FLMM_BED_F64 = 1, FLMM_BED_F32 = 2 };
+enum flmm_scan_variant { FLMM_SCAN_SQUARE = 0, FLMM_SCAN_WIDE = 1 };
+
+/* One scan of a grouped launch (flmm_wald_scan_grouped): the arguments of flmm_wald_scan for one SNP block
+ * against one column set, plus that column set's per-tile maxima (ceil(mc / 128) x p_pad doubles, scratch of the
+ * launch) and its running genome-wide maximum max_F (P doubles, updated by flmm_max_reduce_grouped).  All
+ * pointers are device pointers; the layout is fixed (29 eight-byte fields). */
+typedef struct flmm_scan_group {
+  const void* s1_hi;
+  const void* s1_lo;
+  const void* s2_hi;
+  const void* s2_lo;
+  const float* s1_scale;
+  const float* s2_scale;
+  int64_t mc_pad;
+  const void* q1_hi;
+  const void* q1_lo;
+  const void* q2_hi;
+  const void* q2_lo;
+  const float* q1_scale;
+  const float* q2_scale;
+  int64_t p_pad;
+  int64_t kpad;
+  int64_t mc;
+  int64_t P;
+  int64_t n_real;
+  const double* rWr;
+  const double* h2;
+  double nu;
+  const int64_t* row_index;
+  double* f;
+  double* beta;
+  double* se;
+  double* sfve;
+  double* nullh2;
+  double* pmax;
+  double* max_F;
+} flmm_scan_group;
 
 /* ABI version of this header (bumped on any signature change). */
 int flmm_abi_version(void);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* flmm_last_error(void);
+
+/* Grouped scans in one persistent launch (no counterpart in the reference, which loops over missingness groups
+ * with one loco_scan_compat each: fasterlmm/cli.py:337-375, fasterlmm/perms.py:62).  The launch reads a device
+ * table: flmm_scan_table_bytes(n) bytes, filled on the host by flmm_scan_table_fill (which validates every
+ * group as flmm_wald_scan does, for the tile variant FLMM_SCAN_SQUARE (128 x 128) or FLMM_SCAN_WIDE
+ * (128 x 256, needs p_pad % 256 == 0)) and copied to the device by the caller.  flmm_wald_scan_grouped runs every
+ * tile of every group on a persistent grid of `blocks` workgroups (a positive multiple of 8); it takes nothing
+ * but the table, so a captured graph of it replays after the table is rewritten with the same variant and
+ * precision.  flmm_max_reduce_grouped then folds each group's per-tile maxima into its max_F (grid sized by
+ * n_groups and the largest P). */
+int64_t flmm_scan_table_bytes(int n_groups);
+int flmm_scan_table_fill(int prec, int variant, const flmm_scan_group* groups, int n_groups, void* host_table);
+int flmm_wald_scan_grouped(int prec, int variant, const void* table, int blocks, void* stream);
+int flmm_max_reduce_grouped(const void* table, int n_groups, int64_t max_P, void* stream);
 
 /* Column standardisation (fasterlmm/io.py:223-254): for each of the m columns of the (n, m) row-major src
  * (FP64 with NaN = missing, or int8 dosages with negative = missing), NaN-aware mean and population std,

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_error_channel():
     from fasterlmm_amd import _lib
     lib = _lib.load()
-    assert lib.flmm_abi_version() == 3
+    assert lib.flmm_abi_version() == 4
     # argument validation runs on the host and reports through flmm_last_error without touching a GPU
     rc = lib.flmm_max_reduce(None, 1, 0, 4, None, None)
     assert rc == 1
@@ -47,3 +47,44 @@ def test_bad_arguments_raise_value_error():
                   None, None, None, None, None, None, None, None)
     with pytest.raises(ValueError, match="split-FP16"):
         _lib.call("flmm_pack_planes", 2, 8, 4, 4, 1, 4, None, None, 16, 16, None, 128, 32, None)
+
+
+def _fake_group(_lib, mc=300, P=200, p_pad=256, n_real=3, base=0x10000):
+    """A scan group whose pointers are distinct, 16-byte aligned, never dereferenced addresses."""
+    g = _lib.ScanGroup()
+    for i, (name, typ) in enumerate(g._fields_):
+        if typ is ctypes.c_void_p:
+            setattr(g, name, base + 256 * i)
+    g.mc_pad = (mc + 127) // 128 * 128
+    g.p_pad, g.kpad, g.mc, g.P, g.n_real, g.nu = p_pad, 1024, mc, P, n_real, 990.0
+    return g
+
+
+def test_scan_table_layout_and_validation():
+    """flmm_scan_table_fill is host-only: header, tile prefix sums and the packed group records."""
+    import numpy as np
+    import pytest
+    from fasterlmm_amd import _lib
+    lib = _lib.load()
+    assert ctypes.sizeof(_lib.ScanGroup) == 29 * 8
+    groups = (_lib.ScanGroup * 3)(_fake_group(_lib, 300, 200), _fake_group(_lib, 128, 256, base=0x20000),
+                                  _fake_group(_lib, 0, 256, base=0x30000))
+    nbytes = lib.flmm_scan_table_bytes(3)
+    assert nbytes == (4 + 4) * 8 + 3 * 29 * 8
+    for variant, tp in ((_lib.SCAN_SQUARE, 128), (_lib.SCAN_WIDE, 256)):
+        buf = np.zeros(nbytes // 8, np.int64)
+        _lib.call("flmm_scan_table_fill", _lib.PREC_SPLIT_FP16, variant, ctypes.addressof(groups), 3,
+                  buf.ctypes.data)
+        tiles = [3 * -(-200 // tp), 1 * -(-256 // tp), 0]
+        assert list(buf[:4]) == [3, sum(tiles), variant, 0]
+        assert list(buf[4:8]) == [0, tiles[0], tiles[0] + tiles[1], sum(tiles)]
+        assert buf[8:].tobytes() == bytes(groups)
+    bad = (_lib.ScanGroup * 1)(_fake_group(_lib, p_pad=384))
+    with pytest.raises(ValueError, match="p_pad % 256"):
+        _lib.call("flmm_scan_table_fill", 0, _lib.SCAN_WIDE, ctypes.addressof(bad), 1, buf.ctypes.data)
+    _lib.call("flmm_scan_table_fill", 0, _lib.SCAN_SQUARE, ctypes.addressof(bad), 1, buf.ctypes.data)
+    bad[0].q2_lo += 8
+    with pytest.raises(ValueError, match="group 0: operand planes must be 16-byte aligned"):
+        _lib.call("flmm_scan_table_fill", 0, _lib.SCAN_SQUARE, ctypes.addressof(bad), 1, buf.ctypes.data)
+    with pytest.raises(ValueError, match="blocks must be a positive multiple of 8"):
+        _lib.call("flmm_wald_scan_grouped", 0, 1, buf.ctypes.data, 100, None)
