@@ -1,6 +1,7 @@
 """Throughput of the LOCO compat Wald scan on one MI355X: Wald tests/s (SNP x phenotype x permutation).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--precision P] [--no-compare] [--no-cpu]
+    python bench.py --config 3      (uneven NA groups: grouped launch vs per-group loop, see run_grouped)
 
 Workload (one "step"): the fastlmm-compat leave-one-chromosome-out scan that perm_threshold drives, over one
 synthetic panel resident in HBM -- per chromosome the LOCO kinship, the projected FP64 eigendecomposition, the
@@ -154,12 +155,129 @@ def timed_run(Z, X, Y_all, chrom, B, precision, steps):
     return res, dt, scan
 
 
+def run_grouped(args):
+    """Uneven batching (N=5,000 x M=200,000 x 2,000 RINT phenotypes in 64 seeded missing-at-random NA groups with
+    retained N ~ U(3,000, 5,000), C=6, 100 permutations each): every group's full kinship, then the first
+    chromosome's per-group LOCO bases, delta fits and SNP planes, and the scans of all 64 groups for that
+    chromosome timed as ONE grouped launch (autotuned, replayed from a captured HIP graph) against the per-group
+    launch loop on the same planes.  A step is one grouped scan of the chromosome's SNP block."""
+    import numpy as np
+    import torch
+    from fasterlmm_amd import engine, grouped, sim
+    from fasterlmm_amd.io import group_by_na
+    cfg = dict(sim.CONFIGS[3])
+    if args.pheno:
+        cfg["P"] = args.pheno
+    if args.snps:
+        cfg["M"] = args.snps
+    n_perm = cfg["n_perm"] if args.n_perm is None else args.n_perm
+    t0 = time.perf_counter()
+    panel = sim.make_panel(cfg, seed=19930909)
+    log(f"[bench3] panel N={cfg['N']} M={cfg['M']} P={cfg['P']} C={cfg['C']} in {time.perf_counter() - t0:.1f}s")
+    dev = torch.device("cuda")
+    groups = group_by_na(torch.from_numpy(panel.Y), list(range(cfg["P"])))
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(19930909)
+    Ys, keeps, n_reals = [], [], []
+    for keep, cols in groups:
+        y = torch.from_numpy(panel.Y[np.asarray(keep)][:, cols]).to(dev)
+        n, b = y.shape
+        orders = torch.rand(b, n_perm, n, generator=gen, device=dev).argsort(dim=2)
+        yp = torch.gather(y.T.unsqueeze(1).expand(b, n_perm, n), 2, orders).reshape(b * n_perm, n).T
+        Ys.append(torch.cat([y, yp], dim=1))
+        keeps.append(keep)
+        n_reals.append(b)
+    Ns = [len(k) for k in keeps]
+    log(f"[bench3] {len(groups)} NA groups, retained N {min(Ns)}..{max(Ns)}, columns per group "
+        f"{min(y.shape[1] for y in Ys)}..{max(y.shape[1] for y in Ys)}")
+    Z = torch.from_numpy(panel.G).to(dev).double()
+    X = torch.from_numpy(panel.X).to(dev)
+    prec = engine.prec_code(args.precision)
+    engine.stage_timer.reset(True)
+    with engine.configure(precision=args.precision):
+        t0 = time.perf_counter()
+        gl = grouped.GroupedLoco(Z, X, Ys, keeps, panel.chrom, n_reals=n_reals, n_grid=100,
+                                 progress=lambda c, g: log(f"[bench3] chromosome {c} group {g} basis ready")
+                                 if g % 8 == 7 else None)
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t0
+        log(f"[bench3] per-group kinship and column planes in {t_setup:.1f}s")
+        c = gl.chroms[0]
+        t0 = time.perf_counter()
+        jobs = next(gl.chromosome_jobs(c))
+        torch.cuda.synchronize()
+        t_prep = time.perf_counter() - t0
+    stages = engine.stage_timer.summary()
+    engine.stage_timer.reset(False)
+    log(f"[bench3] chromosome {c}: bases, delta fits and SNP planes of {len(jobs)} groups in {t_prep:.1f}s {stages}")
+    tests = sum(j.mc * j.P for j in jobs)
+    alg_flops = sum(4.0 * j.d.neff * j.mc * j.P for j in jobs)
+    runner = grouped.GroupedScan(prec, len(jobs), max(j.P for j in jobs))
+    t0 = time.perf_counter()
+    runner.run(jobs)  # autotune + graph capture
+    torch.cuda.synchronize()
+    t_tune = time.perf_counter() - t0
+    key = runner.tuner.key(prec, jobs)
+    log(f"[bench3] autotune {t_tune:.2f}s: choice {runner.last_choice}, trials {runner.tuner.trials.get(key)}")
+    want = [{k: getattr(j.out, k).clone() for k in ("f", "beta", "max_F")} for j in jobs]
+
+    def timed(fn, reps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - a) / reps
+
+    t_grouped = timed(lambda: runner.run(jobs), args.steps)
+    same = all(torch.equal(getattr(j.out, k), w[k]) for j, w in zip(jobs, want) for k in w)
+    t_loop = timed(lambda: grouped.per_group_loop(jobs), args.steps)
+    runner_eager = grouped.GroupedScan(prec, len(jobs), max(j.P for j in jobs), graph=False, tuner=runner.tuner)
+    t_eager = timed(lambda: runner_eager.run(jobs), args.steps)
+    value = tests / t_grouped
+    split = args.precision != "fp32"
+    products = 3 if split else 1
+    published = MI355X_PEAK_BF16_TFLOPS if split else MI355X_PEAK_FP32_TFLOPS
+    alg = alg_flops / t_grouped / 1e12
+    log(f"[bench3] grouped {1000 * t_grouped:.2f} ms ({value:.4e} tests/s), eager grouped {1000 * t_eager:.2f} ms, "
+        f"per-group loop {1000 * t_loop:.2f} ms -> {t_loop / t_grouped:.3f}x")
+    variant, blocks = runner.last_choice
+    line = {
+        "metric": "Wald tests/s, grouped scan of uneven NA groups (one chromosome's SNP block, all groups)",
+        "value": value, "unit": "Wald tests/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1000 * t_grouped, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"split_fp16": "fp16", "split_bf16": "bf16", "fp32": "fp32"}[args.precision], "data": "synthetic",
+        "config": {"workload": f"uneven NA groups: N={cfg['N']} M={cfg['M']} ({cfg['n_chrom']} chromosomes) "
+                               f"P={cfg['P']} phenotypes in {len(jobs)} groups (retained N {min(Ns)}..{max(Ns)}) "
+                               f"x (1+{n_perm} perms), C={cfg['C']}; step = chromosome {c} ({jobs[0].mc} SNPs) "
+                               f"against every group's columns", "tests_per_step": tests,
+                   "parallelism": "single GPU"},
+        "grouped": {"variant": "wide 128x256" if variant == 1 else "square 128x128", "blocks": blocks,
+                    "graph_replay_ms": 1000 * t_grouped, "eager_ms": 1000 * t_eager,
+                    "per_group_loop_ms": 1000 * t_loop, "speedup_vs_loop": t_loop / t_grouped,
+                    "identical_to_first_run": same, "autotune_s": t_tune,
+                    "autotune_trials_ms": [(round(ms, 3), v, b) for ms, v, b in runner.tuner.trials.get(key, [])]},
+        "roofline": {"bound": "mfma", "achieved": round(alg, 2), "peak": round(published / products, 1),
+                     "unit": "TFLOP/s", "frac": round(alg * products / published, 4), "traffic": None,
+                     "kernel": "wald_scan_grouped_kernel", "mfma_issued_tflops": round(products * alg, 2)},
+        "pipeline_s": {"kinship_and_columns_all_groups": t_setup, "chromosome_bases_deltas_snp_planes": t_prep,
+                       "stages_ms": stages},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; use --gpus 1"}))
         log("bench.py: multi-GPU runs are not part of this build (one MI355X); run with --gpus 1")
         sys.exit(2)
+    if args.config == 3:
+        run_grouped(args)
+        return
     import numpy as np
     import torch
 

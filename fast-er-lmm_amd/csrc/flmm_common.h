@@ -24,6 +24,14 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // raw 16-byte frag
                                                              // registers)
 typedef unsigned short u16;
 
+// Global-memory view of a pointer.  Accesses through a generic pointer that the compiler cannot trace to a kernel
+// argument (e.g. one read from a device table) compile to flat_* instructions, which count against the LDS wait
+// counter as well and so stall every LDS wait behind in-flight HBM loads; through this view they are global_*.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 constexpr int KCHUNK = 32;  // K extent of one plane chunk
 constexpr double FLOOR64 = 1e-300;  // positive clamp floor of the FP64 reference arithmetic (core.py:80-86)
 

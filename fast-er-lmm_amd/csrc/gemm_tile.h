@@ -127,7 +127,7 @@ __device__ inline void tile_mainloop(const TileArgs<PREC, NG>& args, int64_t a_r
         for (int ps = 0; ps < PASSES; ++ps) {
           const int r = r_in0 + ps * ROWS_PER_PASS;
           pre[(side * NPL + pl) * PASSES + ps] =
-              *(const uint4*)(chunk + (int64_t)r * T::ROWB + s_in * 16);
+              __builtin_bit_cast(uint4, *gmem((const u32x4*)(chunk + (int64_t)r * T::ROWB + s_in * 16)));
         }
       }
   };

@@ -70,8 +70,8 @@ hipError_t launch_adam(float* w, const float* g, float* m, float* v, int64_t n, 
 __device__ inline void unscale_sums(const ScanScales& sc, int64_t m, double inv_q1, double inv_q2, double& a,
                                     double& d) {
   if (sc.s1) {
-    a *= inv_q1 / (double)sc.s1[m];
-    d *= inv_q2 / (double)sc.s2[m];
+    a *= inv_q1 / (double)gmem(sc.s1)[m];
+    d *= inv_q2 / (double)gmem(sc.s2)[m];
   }
 }
 

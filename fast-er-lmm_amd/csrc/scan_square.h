@@ -54,11 +54,11 @@ __device__ inline void square_tile(
     const int cl = acc_col(wn, bj, lane);
     const int64_t p = p0 + cl;
     const bool pv = p < P;
-    const double rwr = pv ? rWr[p] : 1.0;
+    const double rwr = pv ? gmem(rWr)[p] : 1.0;
     const bool real = p < n_real;
-    const double hh = real ? h2[p] : 0.0;
-    const double inv_q1 = (sc.q1 && pv) ? 1.0 / (double)sc.q1[p] : 1.0;
-    const double inv_q2 = (sc.q2 && pv) ? 1.0 / (double)sc.q2[p] : 1.0;
+    const double hh = real ? gmem(h2)[p] : 0.0;
+    const double inv_q1 = (sc.q1 && pv) ? 1.0 / (double)gmem(sc.q1)[p] : 1.0;
+    const double inv_q2 = (sc.q2 && pv) ? 1.0 / (double)gmem(sc.q2)[p] : 1.0;
     double ba = 0.0, bd = 1.0;  // best (sWy, sWs) so far: t = ba^2 / bd
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi) {
@@ -76,12 +76,12 @@ __device__ inline void square_tile(
             const double beta = a / d;
             const double rss = fmax(rwr - a * a / d, FLOOR64);
             const double var = rss / nu / d;
-            const int64_t o = row_index[m] * out.ld + p;
-            out.f[o] = beta * beta / var;
-            out.beta[o] = beta;
-            out.se[o] = sqrt(var);
-            out.sfve[o] = sqrt(fmax(a * a / (d * rwr), 0.0));
-            out.nullh2[o] = hh;
+            const int64_t o = gmem(row_index)[m] * out.ld + p;
+            gmem(out.f)[o] = beta * beta / var;
+            gmem(out.beta)[o] = beta;
+            gmem(out.se)[o] = sqrt(var);
+            gmem(out.sfve)[o] = sqrt(fmax(a * a / (d * rwr), 0.0));
+            gmem(out.nullh2)[o] = hh;
           }
         }
       }
@@ -99,7 +99,7 @@ __device__ inline void square_tile(
     if (wm == 0 && lane < 32) {
       const double oa = xa[(wn * 2 + bj) * 32 + lane], od = xd[(wn * 2 + bj) * 32 + lane];
       if (oa * oa * bd > ba * ba * od) { ba = oa; bd = od; }
-      if (pv) pmax[mt * pmax_ld + p] = wald_f(ba, bd, rwr, nu);
+      if (pv) gmem(pmax)[mt * pmax_ld + p] = wald_f(ba, bd, rwr, nu);
     }
     __syncthreads();
   }

@@ -114,7 +114,7 @@ __device__ inline void wide_tile(
     for (int q = 0; q < NPL; ++q) {
       const char* src = (const char*)args.a[q / NSUB][q % NSUB] + a_lane + (int64_t)kc * a_kstride;
 #pragma unroll
-      for (int i = 0; i < NB; ++i) f.v[q * NB + i] = *(const u32x4*)(src + i * 1024);
+      for (int i = 0; i < NB; ++i) f.v[q * NB + i] = *gmem((const u32x4*)(src + i * 1024));
     }
     return f;
   };
@@ -142,7 +142,7 @@ __device__ inline void wide_tile(
 #pragma unroll
           for (int q = 0; q < NSTEP; ++q)
             f.v[(g * 2 + bj) * NSUB + sub][q] =
-                *(const u32x4*)((const char*)args.b[g][sub] + off + bj * 32 * T::ROWB + q * 16);
+                *gmem((const u32x4*)((const char*)args.b[g][sub] + off + bj * 32 * T::ROWB + q * 16));
     return f;
   };
 
@@ -262,7 +262,7 @@ __device__ inline void wide_tile(
     const int t = threadIdx.x & (W_TM - 1), which = threadIdx.x / W_TM;
     const float* scl = which ? sc.s2 : sc.s1;
     const int64_t m = m0 + t;
-    const double r = (scl && m < Mc) ? 1.0 / (double)scl[m] : 1.0;
+    const double r = (scl && m < Mc) ? 1.0 / (double)gmem(scl)[m] : 1.0;
     row_inv[which][t] = r;
     row_invf[which][t] = (float)r;
   }
@@ -284,10 +284,10 @@ __device__ inline void wide_tile(
   const int64_t pc = p0 + wave * 64 + lane;
   const bool pv = pc < P;
   const bool real = pc < n_real;
-  const double rwr = pv ? rWr[pc] : 1.0;
-  const double iq1 = (sc.q1 && pv) ? 1.0 / (double)sc.q1[pc] : 1.0;
-  const double iq2 = (sc.q2 && pv) ? 1.0 / (double)sc.q2[pc] : 1.0;
-  const double hh = real ? h2[pc] : 0.0;
+  const double rwr = pv ? gmem(rWr)[pc] : 1.0;
+  const double iq1 = (sc.q1 && pv) ? 1.0 / (double)gmem(sc.q1)[pc] : 1.0;
+  const double iq2 = (sc.q2 && pv) ? 1.0 / (double)gmem(sc.q2)[pc] : 1.0;
+  const double hh = real ? gmem(h2)[pc] : 0.0;
   double fmx = -INFINITY;                       // real tiles: FP64 running maximum
   float bn = 0.f, bdd = 1.f, bA = 0.f, bD = 1.f;  // other tiles: best a'^2, its d', and its raw sums
   int bt = -1;
@@ -343,12 +343,12 @@ __device__ inline void wide_tile(
           const double F = beta * beta / var;
           fmx = fmax(fmx, F);
           if (real) {
-            const int64_t o = row_index[m] * out.ld + pc;
-            out.f[o] = F;
-            out.beta[o] = beta;
-            out.se[o] = sqrt(var);
-            out.sfve[o] = sqrt(fmax(a * a / (d * rwr), 0.0));
-            out.nullh2[o] = hh;
+            const int64_t o = gmem(row_index)[m] * out.ld + pc;
+            gmem(out.f)[o] = F;
+            gmem(out.beta)[o] = beta;
+            gmem(out.se)[o] = sqrt(var);
+            gmem(out.sfve)[o] = sqrt(fmax(a * a / (d * rwr), 0.0));
+            gmem(out.nullh2)[o] = hh;
           }
         }
       }
@@ -366,7 +366,7 @@ __device__ inline void wide_tile(
       fmx = beta * beta / (rss / nu / d);
     }
   }
-  pmax[mt * pmax_ld + pc] = fmx;
+  gmem(pmax)[mt * pmax_ld + pc] = fmx;
 }
 
 }  // namespace flmm

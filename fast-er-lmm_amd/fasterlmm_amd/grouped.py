@@ -246,12 +246,9 @@ def per_group_loop(jobs: list[ScanJob]) -> None:
 
 # GROUPED LOCO DRIVER -------
 
-def loco_scan_compat_groups(Z: Tensor, X: Tensor, Ys: list[Tensor], keeps: list, chrom, *,
-                            n_reals: list[int] | None = None, Z_kin: Tensor | None = None, chrom_kin=None,
-                            n_grid: int = 64, mode: str = "grouped", runner: GroupedScan | None = None,
-                            snp_block: int = 16384, on_chrom=None) -> list:
-    """Leave-one-chromosome-out compat scans of several strain subsets (the NA groups of fasterlmm/cli.py:337-375)
-    with the scans of every group run together.
+class GroupedLoco:
+    """Leave-one-chromosome-out compat scans of several strain subsets (the NA groups of fasterlmm/cli.py:337-375),
+    prepared so that the scans of every group can run together.
 
     Group g keeps the rows ``keeps[g]`` of the raw genotypes Z (N, M; NaN = missing) and covariates X, and scans
     the columns of Ys[g] (N_g, P_g) -- reals first, n_reals[g] of them.  Per group the result equals
@@ -260,112 +257,145 @@ def loco_scan_compat_groups(Z: Tensor, X: Tensor, Ys: list[Tensor], keeps: list,
     over the group's rows, K_loco comes from the group's standardised panel (or from Z_kin's rows), and every
     group has its own eigendecomposition, delta fit and dof N_g - C - 1.
 
-    Chromosome by chromosome, every group's basis, delta fit and SNP planes are built, then the scans of all
-    groups for each block of up to ``snp_block`` SNPs run as one grouped launch (mode="grouped", through
-    ``runner``: a GroupedScan, by default graph-captured and autotuned) or as one launch pair per group
-    (mode="loop").  Genotypes are standardised per chromosome, so no group holds a full standardised copy of Z.
-    Returns one ScanResult per group, on the device of Z."""
-    from . import core
-    from .io import standardise_columns
-    if mode not in ("grouped", "loop"):
-        raise ValueError(f"mode must be 'grouped' or 'loop', got {mode!r}")
-    if (Z_kin is None) != (chrom_kin is None):
-        raise ValueError("Z_kin and chrom_kin go together")
-    if len(Ys) != len(keeps):
-        raise ValueError("one column matrix per group is needed")
-    home = Z.device if isinstance(Z, Tensor) else torch.device("cpu")
-    dev = _lib.device()
-    Zd = torch.as_tensor(Z).to(device=dev, dtype=torch.float64)
-    Zkd = None if Z_kin is None else torch.as_tensor(Z_kin).to(device=dev, dtype=torch.float64)
-    Xd = torch.as_tensor(X).to(device=dev, dtype=torch.float64)
-    chrom = np.asarray(chrom)
-    ck = chrom if chrom_kin is None else np.asarray(chrom_kin)
-    Zk = Zd if Zkd is None else Zkd
-    Mk = Zk.shape[1]
-    D = Xd.shape[1]
-    prec = engine.prec_code()
-    G = len(Ys)
-    rows = [torch.as_tensor(np.asarray(k, dtype=np.int64)).to(dev) for k in keeps]
-    Ns = [int(r.numel()) for r in rows]
-    for g, n in enumerate(Ns):
-        if n - D - 1 < 1:
-            raise ValueError(f"group {g} keeps {n} rows, too few for {D} covariates")
-    Yg = [torch.as_tensor(y).to(device=dev, dtype=torch.float64) for y in Ys]
-    Ps = [int(y.shape[1]) for y in Yg]
-    nr = [P if n_reals is None else min(int(n_reals[g]), P) for g, P in enumerate(Ps)]
-    outs = [core._new_outputs(Zd.shape[1], nr[g], Ps[g], dev) for g in range(G)]
-    if mode == "grouped" and runner is None:
-        runner = GroupedScan(prec, G, max(Ps))
-    chroms = sorted(np.unique(chrom).tolist())
-    kin_cols = {c: torch.from_numpy(np.flatnonzero(ck == c)).to(dev) for c in sorted(np.unique(ck).tolist())}
-    test_cols = {c: torch.from_numpy(np.flatnonzero(chrom == c)).to(dev) for c in chroms}
+    Construction computes each group's full kinship (one chromosome's columns at a time, so no group holds a
+    standardised copy of Z), covariate projector and packed scan columns.  ``chromosome_jobs(c)`` then builds every
+    group's basis, delta fit and SNP planes for chromosome c and yields one list of ScanJobs per block of up to
+    ``snp_block`` SNPs; the caller runs each list (GroupedScan.run or per_group_loop) before asking for the next."""
 
-    def std_rows(mat: Tensor, cols: Tensor, g: int) -> Tensor:
-        return standardise_columns(mat.index_select(1, cols).index_select(0, rows[g]))
-
-    # per group: K_all of its standardised panel (one chromosome's columns at a time), projector, column planes
-    K_all = [torch.zeros(n, n, dtype=torch.float64, device=dev) for n in Ns]
-    with engine.stage("kinship"):
-        for c, kc in kin_cols.items():
-            for g in range(G):
-                zs = std_rows(Zk, kc, g)
-                K_all[g].addmm_(zs, zs.T)
-                del zs
-    Iperps, ypls = [], []
-    with engine.stage("pack_columns"):
-        for g in range(G):
-            Xg = Xd.index_select(0, rows[g])
-            Xpinv, Iperp = core._projector(Xg)
-            Iperps.append(Iperp)
-            Y_r = (Yg[g] - Xg @ (Xpinv @ Yg[g])).contiguous()
-            ypls.append(engine.pack(prec, Y_r, Ps[g], Ns[g], 1, Ps[g], engine.rup(max(Ps[g], 1), engine.PTILE),
-                                    engine.rup(Ns[g], engine.KCHUNK)))
-            del Y_r
-    for ci, c in enumerate(chroms):
-        tc = test_cols[c]
-        kc = kin_cols.get(c)
-        bases, ds = [], []
-        for g in range(G):
-            N = Ns[g]
-            eye = torch.eye(N, dtype=torch.float64, device=dev)
-            with engine.stage("kinship"):
-                if kc is not None and kc.numel():
-                    zs = std_rows(Zk, kc, g)
-                    K = (K_all[g] - zs @ zs.T) / float(Mk - kc.numel())
+    def __init__(self, Z: Tensor, X: Tensor, Ys: list, keeps: list, chrom, *, n_reals: list[int] | None = None,
+                 Z_kin: Tensor | None = None, chrom_kin=None, n_grid: int = 64, snp_block: int = 16384,
+                 progress=None):
+        from . import core
+        self.progress = progress  # progress(chromosome, group) after each group's basis, if given
+        if (Z_kin is None) != (chrom_kin is None):
+            raise ValueError("Z_kin and chrom_kin go together")
+        if len(Ys) != len(keeps):
+            raise ValueError("one column matrix per group is needed")
+        self.home = Z.device if isinstance(Z, Tensor) else torch.device("cpu")
+        dev = self.dev = _lib.device()
+        self.Z = torch.as_tensor(Z).to(device=dev, dtype=torch.float64)
+        Zkd = None if Z_kin is None else torch.as_tensor(Z_kin).to(device=dev, dtype=torch.float64)
+        X = torch.as_tensor(X).to(device=dev, dtype=torch.float64)
+        self.chrom = np.asarray(chrom)
+        ck = self.chrom if chrom_kin is None else np.asarray(chrom_kin)
+        self.Zk = self.Z if Zkd is None else Zkd
+        self.Mk = self.Zk.shape[1]
+        self.D = D = X.shape[1]
+        self.prec = prec = engine.prec_code()
+        self.n_grid, self.snp_block = int(n_grid), int(snp_block)
+        self.G = G = len(Ys)
+        self.rows = [torch.as_tensor(np.asarray(k, dtype=np.int64)).to(dev) for k in keeps]
+        self.Ns = [int(r.numel()) for r in self.rows]
+        for g, n in enumerate(self.Ns):
+            if n - D - 1 < 1:
+                raise ValueError(f"group {g} keeps {n} rows, too few for {D} covariates")
+        Yg = [torch.as_tensor(y).to(device=dev, dtype=torch.float64) for y in Ys]
+        self.Ps = [int(y.shape[1]) for y in Yg]
+        self.n_reals = [P if n_reals is None else min(int(n_reals[g]), P) for g, P in enumerate(self.Ps)]
+        self.outs = [core._new_outputs(self.Z.shape[1], self.n_reals[g], self.Ps[g], dev) for g in range(G)]
+        self.chroms = sorted(np.unique(self.chrom).tolist())
+        self.kin_cols = {c: torch.from_numpy(np.flatnonzero(ck == c)).to(dev) for c in sorted(np.unique(ck).tolist())}
+        # per group: K_all of its standardised panel, projector, packed columns
+        self.K_all = [torch.zeros(n, n, dtype=torch.float64, device=dev) for n in self.Ns]
+        with engine.stage("kinship"):
+            for c, kc in self.kin_cols.items():
+                for g in range(G):
+                    zs = self._std_rows(self.Zk, kc, g)
+                    self.K_all[g].addmm_(zs, zs.T)
                     del zs
-                else:
-                    K = K_all[g] / float(Mk)
-                Kp = (Iperps[g] @ (K + eye)) @ Iperps[g]
-                del K
-            with engine.stage("eigh"):
-                ev, U = core._eigh(Kp)
-                del Kp
-            s = torch.clamp(ev[D:N] - 1.0, min=0.0)
-            with engine.stage("pack_basis"):
-                st = core._Basis(s, U, D, N - D, prec, nu=N - D - 1)
-                del U
-            with engine.stage("rotate_columns"):
-                st.use_column_planes(ypls[g], Ps[g])
-            with engine.stage("delta"):
-                _, d = st.fit(n_grid, -10.0, 10.0, True, projected_loss=True)
-            bases.append(st)
-            ds.append(d)
-        for b0 in range(0, tc.numel(), snp_block):
-            ri = tc[b0:b0 + snp_block]
+        self.Iperps, self.ypls = [], []
+        with engine.stage("pack_columns"):
+            for g in range(G):
+                Xg = X.index_select(0, self.rows[g])
+                Xpinv, Iperp = core._projector(Xg)
+                self.Iperps.append(Iperp)
+                Y_r = (Yg[g] - Xg @ (Xpinv @ Yg[g])).contiguous()
+                self.ypls.append(engine.pack(prec, Y_r, self.Ps[g], self.Ns[g], 1, self.Ps[g],
+                                             engine.rup(max(self.Ps[g], 1), engine.PTILE),
+                                             engine.rup(self.Ns[g], engine.KCHUNK)))
+                del Y_r
+
+    def _std_rows(self, mat: Tensor, cols: Tensor, g: int) -> Tensor:
+        from .io import standardise_columns
+        return standardise_columns(mat.index_select(1, cols).index_select(0, self.rows[g]))
+
+    def _basis(self, c, g: int):
+        """Group g's projected basis for chromosome c (its K_loco), with the rotated columns and delta fit."""
+        from . import core
+        N, D, dev = self.Ns[g], self.D, self.dev
+        kc = self.kin_cols.get(c)
+        eye = torch.eye(N, dtype=torch.float64, device=dev)
+        with engine.stage("kinship"):
+            if kc is not None and kc.numel():
+                zs = self._std_rows(self.Zk, kc, g)
+                K = (self.K_all[g] - zs @ zs.T) / float(self.Mk - kc.numel())
+                del zs
+            else:
+                K = self.K_all[g] / float(self.Mk)
+            Kp = (self.Iperps[g] @ (K + eye)) @ self.Iperps[g]
+            del K
+        with engine.stage("eigh"):
+            ev, U = core._eigh(Kp)
+            del Kp
+        s = torch.clamp(ev[D:N] - 1.0, min=0.0)
+        with engine.stage("pack_basis"):
+            st = core._Basis(s, U, D, N - D, self.prec, nu=N - D - 1)
+            del U
+        with engine.stage("rotate_columns"):
+            st.use_column_planes(self.ypls[g], self.Ps[g])
+        with engine.stage("delta"):
+            _, d = st.fit(self.n_grid, -10.0, 10.0, True, projected_loss=True)
+        return st, d
+
+    def chromosome_jobs(self, c):
+        tc = torch.from_numpy(np.flatnonzero(self.chrom == c)).to(self.dev)
+        bases = []
+        for g in range(self.G):
+            bases.append(self._basis(c, g))
+            if self.progress is not None:
+                self.progress(c, g)
+        for b0 in range(0, tc.numel(), self.snp_block):
+            ri = tc[b0:b0 + self.snp_block]
+            local = torch.arange(ri.numel(), device=self.dev)
             jobs = []
             with engine.stage("snps"):
-                for g in range(G):
-                    zs = std_rows(Zd, ri, g)
-                    s1, s2 = bases[g].snp_planes(zs, torch.arange(ri.numel(), device=dev))
+                for g, (st, d) in enumerate(bases):
+                    zs = self._std_rows(self.Z, ri, g)
+                    s1, s2 = st.snp_planes(zs, local)
                     del zs
-                    jobs.append(ScanJob(s1, s2, ds[g], ri.numel(), Ps[g], nr[g], bases[g].nu, ri, outs[g]))
+                    jobs.append(ScanJob(s1, s2, d, ri.numel(), self.Ps[g], self.n_reals[g], st.nu, ri, self.outs[g]))
+            yield jobs
+            del jobs
+
+    def results(self) -> list:
+        from . import core
+        res = [core.ScanResult(o.f, o.beta, o.se, o.sfve, o.nullh2, o.max_F) for o in self.outs]
+        return [core._home(r, self.home) for r in res]
+
+
+def loco_scan_compat_groups(Z: Tensor, X: Tensor, Ys: list, keeps: list, chrom, *, n_reals: list[int] | None = None,
+                            Z_kin: Tensor | None = None, chrom_kin=None, n_grid: int = 64, mode: str = "grouped",
+                            runner: GroupedScan | None = None, snp_block: int = 16384, test_chroms=None,
+                            on_chrom=None) -> list:
+    """LOCO compat scans of several strain subsets (see GroupedLoco for the per-group contract), chromosome by
+    chromosome, with the scans of all groups for each SNP block run as one grouped launch (mode="grouped",
+    through ``runner``: a GroupedScan, by default graph-captured and autotuned) or as one launch pair per group
+    (mode="loop").  ``test_chroms`` limits the scanned chromosomes (each still leaves itself out of K); rows of
+    other chromosomes keep zeros and do not enter max_F.  Returns one ScanResult per group, on the device of Z."""
+    if mode not in ("grouped", "loop"):
+        raise ValueError(f"mode must be 'grouped' or 'loop', got {mode!r}")
+    gl = GroupedLoco(Z, X, Ys, keeps, chrom, n_reals=n_reals, Z_kin=Z_kin, chrom_kin=chrom_kin, n_grid=n_grid,
+                     snp_block=snp_block)
+    if mode == "grouped" and runner is None:
+        runner = GroupedScan(gl.prec, gl.G, max(gl.Ps))
+    chroms = gl.chroms if test_chroms is None else [c for c in gl.chroms if c in set(test_chroms)]
+    for ci, c in enumerate(chroms):
+        for jobs in gl.chromosome_jobs(c):
             with engine.stage("scan"):
                 if mode == "grouped":
                     runner.run(jobs)
                 else:
                     per_group_loop(jobs)
-            del jobs
         if on_chrom is not None:
             on_chrom(ci + 1, len(chroms))
-    res = [core.ScanResult(o.f, o.beta, o.se, o.sfve, o.nullh2, o.max_F) for o in outs]
-    return [core._home(r, home) for r in res]
+    return gl.results()

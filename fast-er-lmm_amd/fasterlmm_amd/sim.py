@@ -4,7 +4,7 @@ Genotypes follow the Balding-Nichols model: ancestral frequency p ~ U(maf_lo, ma
 frequency ~ Beta(p (1 - F) / F, (1 - p)(1 - F) / F) with F = Fst, diploid dosages ~ Binomial(2, p_k), stored as
 int8 0/1/2.  SNPs are split evenly over chromosomes 1..n_chrom (sorted).  Each phenotype has heritability
 h2 ~ U(h2_lo, h2_hi) from n_causal SNPs with N(0, 1) effects on standardised genotypes, plus Gaussian noise;
-optionally rank-inverse-normal transformed with Blom's c = 3/8 and average ties (the reference's RINT,
+optionally masked by seeded missing-at-random NA groups (na_mask) and rank-inverse-normal transformed with Blom's c = 3/8 and average ties (the reference's RINT,
 fasterlmm/normalize.py:21-52).  Covariates are an intercept plus N(0, 1) columns.
 """
 from __future__ import annotations
@@ -30,6 +30,9 @@ CONFIGS = {
     1: dict(N=150, M=1500, n_chrom=16, P=20, C=3, maf=(0.05, 0.5), h2=(0.1, 0.8), rint=False, n_perm=100),
     # config 2: the headline single-GPU scan
     2: dict(N=1000, M=100_000, n_chrom=22, P=6484, C=4, maf=(0.01, 0.5), h2=(0.0, 0.9), rint=True, n_perm=100),
+    # config 3: uneven batching -- 64 missingness groups, retained N ~ U(3000, 5000)
+    3: dict(N=5000, M=200_000, n_chrom=22, P=2000, C=6, maf=(0.01, 0.5), h2=(0.0, 0.9), rint=True, n_perm=100,
+            na_groups=64, n_keep=(3000, 5000), sampler="bernoulli2"),
 }
 
 
@@ -46,7 +49,9 @@ def rint_columns(Y: np.ndarray, c: float = 3.0 / 8.0) -> np.ndarray:
 
 
 def balding_nichols(rng: np.random.Generator, N: int, M: int, *, n_pop: int = 4, fst: float = 0.1,
-                    maf=(0.05, 0.5), block: int = 8192) -> np.ndarray:
+                    maf=(0.05, 0.5), block: int = 8192, sampler: str = "binomial") -> np.ndarray:
+    """sampler "binomial" draws each dosage with rng.binomial(2, p); "bernoulli2" draws the same Binomial(2, p)
+    law as the sum of two float32-uniform Bernoulli(p) trials, several times faster for large panels."""
     G = np.empty((N, M), dtype=np.int8)
     pop = np.repeat(np.arange(n_pop), -(-N // n_pop))[:N]
     for j0 in range(0, M, block):
@@ -55,7 +60,12 @@ def balding_nichols(rng: np.random.Generator, N: int, M: int, *, n_pop: int = 4,
         a = p * (1 - fst) / fst
         b = (1 - p) * (1 - fst) / fst
         pk = rng.beta(a[None, :], b[None, :], size=(n_pop, j1 - j0))
-        G[:, j0:j1] = rng.binomial(2, pk[pop]).astype(np.int8)
+        if sampler == "bernoulli2":
+            pf = pk[pop].astype(np.float32)
+            G[:, j0:j1] = (rng.random((N, j1 - j0), dtype=np.float32) < pf).astype(np.int8)
+            G[:, j0:j1] += (rng.random((N, j1 - j0), dtype=np.float32) < pf).astype(np.int8)
+        else:
+            G[:, j0:j1] = rng.binomial(2, pk[pop]).astype(np.int8)
     # a column with no variation scans as zero; keep every SNP polymorphic
     mono = np.flatnonzero(G.min(axis=0) == G.max(axis=0))
     for j in mono:
@@ -63,11 +73,23 @@ def balding_nichols(rng: np.random.Generator, N: int, M: int, *, n_pop: int = 4,
     return G
 
 
+def na_mask(rng: np.random.Generator, N: int, P: int, n_groups: int, n_keep) -> np.ndarray:
+    """(N, P) missing-at-random mask of n_groups NA patterns: group g holds the columns j with j % n_groups == g
+    and keeps a uniformly drawn set of n_g ~ U{n_keep[0], ..., n_keep[1]} strains (n_g <= N)."""
+    mask = np.ones((N, P), dtype=bool)
+    for g in range(n_groups):
+        n_g = min(N, int(rng.integers(n_keep[0], n_keep[1] + 1)))
+        keep = rng.choice(N, size=n_g, replace=False)
+        cols = np.arange(g, P, n_groups)
+        mask[np.ix_(keep, cols)] = False
+    return mask
+
+
 def make_panel(config: int | dict, seed: int = 19930909, *, n_causal: int = 10) -> Panel:
     cfg = CONFIGS[config] if isinstance(config, int) else config
     rng = np.random.default_rng(seed)
     N, M, P, C = cfg["N"], cfg["M"], cfg["P"], cfg["C"]
-    G = balding_nichols(rng, N, M, maf=cfg["maf"])
+    G = balding_nichols(rng, N, M, maf=cfg["maf"], sampler=cfg.get("sampler", "binomial"))
     per = -(-M // cfg["n_chrom"])
     chrom = (np.arange(M) // per + 1).astype(np.int64)
     X = np.concatenate([np.ones((N, 1)), rng.standard_normal((N, C - 1))], axis=1)
@@ -85,6 +107,8 @@ def make_panel(config: int | dict, seed: int = 19930909, *, n_causal: int = 10) 
         g = (g - g.mean(axis=0)) / np.maximum(g.std(axis=0), 1e-12)
         hh = h2[p0:p1]
         Y[:, p0:p1] = np.sqrt(hh) * g + np.sqrt(1 - hh) * rng.standard_normal((N, p1 - p0))
+    if cfg.get("na_groups"):
+        Y[na_mask(rng, N, P, cfg["na_groups"], cfg["n_keep"])] = np.nan
     if cfg.get("rint"):
         Y = rint_columns(Y)
     return Panel(G=G, chrom=chrom, X=X, Y=Y, h2=h2, causal=causal)
