@@ -41,7 +41,7 @@ static bool two_planes(int p) { return p != FLMM_PREC_FP32; }
 
 extern "C" {
 
-int flmm_abi_version(void) { return 4; }
+int flmm_abi_version(void) { return 5; }
 
 const char* flmm_last_error(void) { return g_err.c_str(); }
 
@@ -286,6 +286,53 @@ int flmm_wald_scan(int prec, const void* s1_hi, const void* s1_lo, const void* s
   else FLMM_SCAN(PREC_FP32);
 #undef FLMM_SCAN
   return check_launch(e, "flmm_wald_scan");
+}
+
+int flmm_wald_scan_lowrank(int prec, const void* s1_hi, const void* s1_lo, const void* s2_hi, const void* s2_lo,
+                           const float* s1_scale, const float* s2_scale, int64_t mc_pad, const void* q1_hi,
+                           const void* q1_lo, const void* q2_hi, const void* q2_lo, const float* q1_scale,
+                           const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t kpad2, int64_t mc, int64_t P,
+                           int64_t n_real, const double* rWr, const double* h2, double nu, const int64_t* row_index,
+                           double* f, double* beta, double* se, double* sfve, double* nullh2, double* pmax,
+                           void* stream) {
+  REQUIRE(valid_prec(prec), "flmm_wald_scan_lowrank: bad precision %d", prec);
+  REQUIRE(mc >= 0 && mc <= mc_pad && mc_pad % TILE == 0, "flmm_wald_scan_lowrank: bad SNP extent");
+  REQUIRE(P >= 0 && P <= p_pad && p_pad % 256 == 0, "flmm_wald_scan_lowrank: bad column extent (p_pad %% 256)");
+  REQUIRE(kpad2 > 0 && kpad2 % TILE == 0 && kpad % TILE == 0 && kpad2 < kpad,
+          "flmm_wald_scan_lowrank: need 0 < kpad2 < kpad, both multiples of 128");
+  REQUIRE(n_real >= 0 && n_real <= P, "flmm_wald_scan_lowrank: n_real must be in [0, P]");
+  REQUIRE(nu > 0.0, "flmm_wald_scan_lowrank: dof must be positive");
+  const bool two = two_planes(prec);
+  REQUIRE(s1_hi && s2_hi && q1_hi && q2_hi && (!two || (s1_lo && s2_lo && q1_lo && q2_lo)),
+          "flmm_wald_scan_lowrank: null operand plane");
+  REQUIRE(aligned16(s1_hi) && aligned16(s2_hi) && aligned16(q1_hi) && aligned16(q2_hi) &&
+              (!two || (aligned16(s1_lo) && aligned16(s2_lo) && aligned16(q1_lo) && aligned16(q2_lo))),
+          "flmm_wald_scan_lowrank: operand planes must be 16-byte aligned");
+  REQUIRE(prec != FLMM_PREC_SPLIT_FP16 || (s1_scale && s2_scale && q1_scale && q2_scale),
+          "flmm_wald_scan_lowrank: split-FP16 needs the four row-scale arrays");
+  REQUIRE(rWr && pmax, "flmm_wald_scan_lowrank: null rWr / pmax");
+  REQUIRE(n_real == 0 || (f && beta && se && sfve && nullh2 && h2 && row_index),
+          "flmm_wald_scan_lowrank: null output");
+  if (mc == 0 || P == 0) return FLMM_OK;
+  ScanOut out{f, beta, se, sfve, nullh2, n_real};
+  ScanScales sc{nullptr, nullptr, nullptr, nullptr};
+  if (prec == FLMM_PREC_SPLIT_FP16) sc = ScanScales{s1_scale, s2_scale, q1_scale, q2_scale};
+  const hipStream_t st = (hipStream_t)stream;
+  hipError_t e;
+#define FLMM_SCAN(PR)                                                                                                \
+  do {                                                                                                               \
+    TileArgs<PR, 2> t{};                                                                                             \
+    t.a[0][0] = s1_hi; t.a[0][1] = s1_lo; t.a[1][0] = s2_hi; t.a[1][1] = s2_lo;                                      \
+    t.b[0][0] = q1_hi; t.b[0][1] = q1_lo; t.b[1][0] = q2_hi; t.b[1][1] = q2_lo;                                      \
+    t.a_rows_pad = mc_pad; t.b_rows_pad = p_pad; t.nchunks = (int)(kpad / KCHUNK);                                  \
+    t.nchunks2 = (int)(kpad2 / KCHUNK);                                                                              \
+    e = launch_wald_scan_lowrank<PR>(t, sc, mc, P, n_real, rWr, h2, nu, row_index, out, pmax, st);                  \
+  } while (0)
+  if (prec == FLMM_PREC_SPLIT_BF16) FLMM_SCAN(PREC_SPLIT_BF16);
+  else if (prec == FLMM_PREC_SPLIT_FP16) FLMM_SCAN(PREC_SPLIT_FP16);
+  else FLMM_SCAN(PREC_FP32);
+#undef FLMM_SCAN
+  return check_launch(e, "flmm_wald_scan_lowrank");
 }
 
 int64_t flmm_scan_table_bytes(int n_groups) {

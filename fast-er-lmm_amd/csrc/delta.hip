@@ -256,6 +256,32 @@ __device__ inline void col_slope(const LK& L, int64_t p, double t, int lane, dou
   gp = delta * h + delta * delta * hp;
 }
 
+// The grid argmin handed in comes from the FP32 grid kernel; where neighbouring grid losses differ by less than
+// FP32 resolves (flat likelihoods, h2 near 0) it can sit a node away from the FP64 argmin the reference brackets
+// (core.py:570-575).  Walk from it to the FP64 local minimum of the grid: left while the left neighbour is not
+// higher (ties go to the first index, as argmin's do), right while the right neighbour is lower.
+template <class LK>
+__device__ inline int fp64_grid_argmin(const LK& L, int64_t p, int i, int G, double lo, double hi, int lane) {
+  double li = col_loss(L, p, grid_value(i, G, lo, hi), lane);
+  bool moved_left = false;
+  for (int it = 0; it < 8 && i > 0; ++it) {
+    const double l = col_loss(L, p, grid_value(i - 1, G, lo, hi), lane);
+    if (!(l <= li)) break;
+    --i;
+    li = l;
+    moved_left = true;
+  }
+  if (!moved_left) {
+    for (int it = 0; it < 8 && i + 1 < G; ++it) {
+      const double l = col_loss(L, p, grid_value(i + 1, G, lo, hi), lane);
+      if (!(l < li)) break;
+      ++i;
+      li = l;
+    }
+  }
+  return i;
+}
+
 template <class LK>
 __device__ inline double golden_bracket(const LK& L, int64_t p, int i, int G, double lo, double hi, int lane) {
   const double inv_phi = 2.0 / (1.0 + sqrt(5.0));
@@ -371,7 +397,10 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
   ColLik<KPL, SAME> L{uy, p_pad, s, neff, s_lam, nlam, nco, sh_s, SAME ? sh_s : sh_l, sh_sf, SAME ? sh_sf : sh_lf, {}};
   L.load(p, lane);
   double t;
-  const int i = (mode == MODE_FIXED || mode == MODE_GRID) ? 0 : idx[p];
+  int i = (mode == MODE_FIXED || mode == MODE_GRID) ? 0 : idx[p];
+  // the reference replay (golden section) brackets the FP64 argmin exactly; Newton keeps the FP32 grid's bracket
+  // (the two differ only where neighbouring grid losses tie to FP32 precision, i.e. on flat likelihoods)
+  if (mode == MODE_GOLDEN) i = fp64_grid_argmin(L, p, i, G, lo, hi, lane);
   bool need_bracket = mode == MODE_NEWTON;
   if (mode == MODE_FIXED) {
     t = log_delta[p];  // the caller's log delta, used as given (core.py:639)

@@ -69,6 +69,7 @@ struct TileArgs {
   int64_t a_rows_pad;    // rows_pad of every A plane
   int64_t b_rows_pad;
   int nchunks;           // K / 32
+  int nchunks2;          // K / 32 of the second product where it is shorter (low-rank scan only)
 };
 
 // LDS bytes needed by one workgroup

@@ -48,6 +48,10 @@ template <int PREC>
 hipError_t launch_wald_scan_wide(const TileArgs<PREC, 2>& args, const ScanScales& sc, int64_t Mc, int64_t P,
                                  int64_t n_real, const double* rWr, const double* h2, double nu,
                                  const int64_t* row_index, const ScanOut& out, double* pmax, hipStream_t stream);
+template <int PREC>
+hipError_t launch_wald_scan_lowrank(const TileArgs<PREC, 2>& args, const ScanScales& sc, int64_t Mc, int64_t P,
+                                    int64_t n_real, const double* rWr, const double* h2, double nu,
+                                    const int64_t* row_index, const ScanOut& out, double* pmax, hipStream_t stream);
 hipError_t launch_max_reduce(const double* pmax, int64_t n_mtiles, int64_t ld, int64_t P, double* max_F,
                              hipStream_t stream);
 hipError_t launch_standardise(int dtype, const void* src, int64_t n, int64_t m, int64_t lds, double* dst, int64_t ldd,

@@ -25,6 +25,37 @@ hipError_t launch_wald_scan_wide(const TileArgs<PREC, 2>& args, const ScanScales
   return hipGetLastError();
 }
 
+// Low-rank scan: the wide tile with the second product limited to the first args.nchunks2 K-chunks.
+template <int PREC>
+__global__ __launch_bounds__(256, 1) void wald_scan_lowrank_kernel(
+    TileArgs<PREC, 2> args, ScanScales sc, int64_t n_mtiles, int64_t n_ptiles, int64_t n_tiles, int64_t Mc, int64_t P,
+    int64_t n_real, const double* __restrict__ rWr, const double* __restrict__ h2, double nu,
+    const int64_t* __restrict__ row_index, ScanOut out, double* __restrict__ pmax, int64_t pmax_ld) {
+  wide_tile<PREC, true>(args, sc, n_mtiles, n_ptiles, xcd_remap(blockIdx.x, n_tiles), Mc, P, n_real, rWr, h2, nu,
+                        row_index, out, pmax, pmax_ld);
+}
+
+template <int PREC>
+hipError_t launch_wald_scan_lowrank(const TileArgs<PREC, 2>& args, const ScanScales& sc, int64_t Mc, int64_t P,
+                                    int64_t n_real, const double* rWr, const double* h2, double nu,
+                                    const int64_t* row_index, const ScanOut& out, double* pmax, hipStream_t stream) {
+  const int64_t n_mtiles = (Mc + W_TM - 1) / W_TM;
+  const int64_t n_ptiles = (P + W_TP - 1) / W_TP;
+  const int64_t n_tiles = n_mtiles * n_ptiles;
+  hipLaunchKernelGGL(wald_scan_lowrank_kernel<PREC>, dim3((unsigned)n_tiles), dim3(256), 0, stream, args, sc,
+                     n_mtiles, n_ptiles, n_tiles, Mc, P, n_real, rWr, h2, nu, row_index, out, pmax, args.b_rows_pad);
+  return hipGetLastError();
+}
+
+#define FLMM_LR_INST(PR)                                                                                            \
+  template hipError_t launch_wald_scan_lowrank<PR>(const TileArgs<PR, 2>&, const ScanScales&, int64_t, int64_t,     \
+                                                   int64_t, const double*, const double*, double, const int64_t*,   \
+                                                   const ScanOut&, double*, hipStream_t);
+FLMM_LR_INST(PREC_SPLIT_FP16)
+FLMM_LR_INST(PREC_SPLIT_BF16)
+FLMM_LR_INST(PREC_FP32)
+#undef FLMM_LR_INST
+
 template hipError_t launch_wald_scan_wide<PREC_SPLIT_FP16>(const TileArgs<PREC_SPLIT_FP16, 2>&, const ScanScales&,
                                                             int64_t, int64_t, int64_t, const double*, const double*,
                                                             double, const int64_t*, const ScanOut&, double*,

@@ -11,6 +11,8 @@
 // 64-VGPR column-fragment sets (one K chunk each), 32 VGPRs of SNP-side staging, the A fragments of one step and
 // addressing.
 #pragma once
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace flmm {
@@ -56,7 +58,10 @@ struct AStage {
 
 // One 128 x 256 tile (logical index `tile` of the n_mtiles x n_ptiles grid, in the grouped order below) of the
 // wide Wald scan, run by a 256-thread workgroup.
-template <int PREC>
+// LOWRANK: the first product (sWy) runs over all args.nchunks K-chunks and the second (sWs) over the first
+// args.nchunks2 only (the low-rank scan's off-rank cross term lives in the first product alone); chunks past
+// nchunks2 load, stage and multiply the first product's planes only.
+template <int PREC, bool LOWRANK = false>
 __device__ inline void wide_tile(
     const TileArgs<PREC, 2>& args, const ScanScales& sc, int64_t n_mtiles, int64_t n_ptiles, int64_t tile, int64_t Mc,
     int64_t P, int64_t n_real,
@@ -108,19 +113,22 @@ __device__ inline void wide_tile(
   const int64_t b_lane = (p0 + wave * 64 + (lane & 31)) * T::ROWB + h * (T::ROWB / 2);
   const int64_t b_kstride = args.b_rows_pad * T::ROWB;
 
-  auto load_a = [&](int kc) {
+  // `two` (std::true_type / std::false_type): whether the chunk carries the second product's planes
+  auto load_a = [&](auto two, int kc) {
+    constexpr int NQ = decltype(two)::value ? NPL : NSUB;
     AStage<NA> f;
 #pragma unroll
-    for (int q = 0; q < NPL; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       const char* src = (const char*)args.a[q / NSUB][q % NSUB] + a_lane + (int64_t)kc * a_kstride;
 #pragma unroll
       for (int i = 0; i < NB; ++i) f.v[q * NB + i] = *gmem((const u32x4*)(src + i * 1024));
     }
     return f;
   };
-  auto store_a = [&](const AStage<NA>& f, char* stage) {
+  auto store_a = [&](auto two, const AStage<NA>& f, char* stage) {
+    constexpr int NQ = decltype(two)::value ? NPL : NSUB;
 #pragma unroll
-    for (int q = 0; q < NPL; ++q) {
+    for (int q = 0; q < NQ; ++q) {
 #pragma unroll
       for (int i = 0; i < NB; ++i) {
         const int r = a_row0 + i * ROWS_PER_BLK;
@@ -130,11 +138,12 @@ __device__ inline void wide_tile(
   };
   // the lane's whole half-row of chunk kc for every column block: both 16-byte pieces of a 32-byte span are
   // fetched back to back, so each 64-byte row chunk crosses L2 -> L1 once
-  auto load_b = [&](int kc) {
+  auto load_b = [&](auto two, int kc) {
+    constexpr int NG = decltype(two)::value ? 2 : 1;
     BChunk<NSUB, NSTEP> f;
     const int64_t off = (int64_t)kc * b_kstride + b_lane;
 #pragma unroll
-    for (int g = 0; g < 2; ++g)
+    for (int g = 0; g < NG; ++g)
 #pragma unroll
       for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
@@ -157,11 +166,12 @@ __device__ inline void wide_tile(
         for (int v = 0; v < 16; ++v) acc[g][i][j][v] = 0.f;
 
   // one MFMA step (16 physical k per lane half) of chunk stage `st` with column fragments bf
-  auto mma_step = [&](const char* st, int step, const BChunk<NSUB, NSTEP>& bf) {
+  auto mma_step = [&](auto two, const char* st, int step, const BChunk<NSUB, NSTEP>& bf) {
+    constexpr int NG = decltype(two)::value ? 2 : 1;
     if constexpr (is_split(PREC)) {
       const int seg = h * 2 + step;
 #pragma unroll
-      for (int g = 0; g < 2; ++g) {
+      for (int g = 0; g < NG; ++g) {
         bf16x8 af[4][2];
 #pragma unroll
         for (int bi = 0; bi < 4; ++bi) {
@@ -187,7 +197,7 @@ __device__ inline void wide_tile(
     } else {
       const int seg = h * 4 + step;
 #pragma unroll
-      for (int g = 0; g < 2; ++g) {
+      for (int g = 0; g < NG; ++g) {
         f32x4 af[4];
 #pragma unroll
         for (int bi = 0; bi < 4; ++bi) {
@@ -211,14 +221,16 @@ __device__ inline void wide_tile(
   // the SNP-side block and the column fragments of chunk kc + 1 are loaded into registers, so both have a whole
   // chunk of MFMAs (96 per wave) to land; the SNP-side block is written to the other LDS buffer at the end of the
   // chunk.  Past the last chunk the loads repeat the last chunk (valid addresses, values unused).
-  auto chunk = [&](int kc, const char* cur, char* nxt, const BChunk<NSUB, NSTEP>& bc, BChunk<NSUB, NSTEP>& bn) {
+  auto chunk = [&](auto cur2, auto next2, int kc, const char* cur, char* nxt, const BChunk<NSUB, NSTEP>& bc,
+                   BChunk<NSUB, NSTEP>& bn) {
+    constexpr bool C2 = decltype(cur2)::value, N2 = decltype(next2)::value;
     const int kn = kc + 1 < nchunks ? kc + 1 : kc;
-    const AStage<NA> an = load_a(kn);
-    bn = load_b(kn);
+    const AStage<NA> an = load_a(next2, kn);
+    bn = load_b(next2, kn);
     if constexpr (!is_split(PREC)) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int step = 0; step < NSTEP; ++step) mma_step(cur, step, bc);
-    if constexpr (is_split(PREC)) {
+    for (int step = 0; step < NSTEP; ++step) mma_step(cur2, cur, step, bc);
+    if constexpr (is_split(PREC) && C2 && N2) {
       // Interleave per MFMA step (16 LDS fragment reads, 48 MFMAs, and 12 of the chunk's 24 global loads): the
       // first product's 8 fragment reads up front, the second product's 8 reads and 4 global loads spread over the
       // first product's 24 MFMAs, the other 8 global loads over the second product's 24 MFMAs.  (Masks: 0x008
@@ -240,18 +252,52 @@ __device__ inline void wide_tile(
           __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
         }
       }
+    } else if constexpr (is_split(PREC) && !C2 && !N2) {
+      // one product: the same shape over the chunk's two MFMA steps (8 fragment reads of step 0 up front, step
+      // 1's 8 reads and 4 of the 12 global loads over step 0's 24 MFMAs, the other 8 loads over step 1's)
+      __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
+        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+      }
     }
     __builtin_amdgcn_sched_barrier(0);
-    store_a(an, nxt);  // nxt was last read in the previous chunk, which ended with a barrier
+    store_a(next2, an, nxt);  // nxt was last read in the previous chunk, which ended with a barrier
     __syncthreads();
   };
-  BChunk<NSUB, NSTEP> be = load_b(0), bo;
-  store_a(load_a(0), lds0);
+  constexpr std::true_type two{};
+  constexpr std::false_type one{};
+  BChunk<NSUB, NSTEP> be = load_b(two, 0), bo;
+  store_a(two, load_a(two, 0), lds0);
   __syncthreads();
-  // nchunks is even (K is padded to a multiple of 128)
-  for (int kc = 0; kc < nchunks; kc += 2) {
-    chunk(kc, lds0, lds1, be, bo);
-    chunk(kc + 1, lds1, lds0, bo, be);
+  if constexpr (!LOWRANK) {
+    // nchunks is even (K is padded to a multiple of 128)
+    for (int kc = 0; kc < nchunks; kc += 2) {
+      chunk(two, two, kc, lds0, lds1, be, bo);
+      chunk(two, two, kc + 1, lds1, lds0, bo, be);
+    }
+  } else {
+    // nchunks2 and nchunks are multiples of 4 with 0 < nchunks2 < nchunks
+    const int n2 = args.nchunks2;
+    for (int kc = 0; kc < n2 - 2; kc += 2) {
+      chunk(two, two, kc, lds0, lds1, be, bo);
+      chunk(two, two, kc + 1, lds1, lds0, bo, be);
+    }
+    chunk(two, two, n2 - 2, lds0, lds1, be, bo);
+    chunk(two, one, n2 - 1, lds1, lds0, bo, be);
+    for (int kc = n2; kc < nchunks; kc += 2) {
+      chunk(one, one, kc, lds0, lds1, be, bo);
+      chunk(one, one, kc + 1, lds1, lds0, bo, be);
+    }
   }
 
   // ---- epilogue: this wave owns columns p0 + 64 wave + [0, 64) and all 128 rows.

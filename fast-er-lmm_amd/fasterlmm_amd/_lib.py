@@ -58,6 +58,8 @@ SIGNATURES: dict[str, list] = {
     "flmm_adam": [_P, _P, _P, _P, _I64, _D, _D, _D, _D, _I, _P],
     "flmm_wald_scan": [_I, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P,
                        _P, _D, _P, _P, _P, _P, _P, _P, _P, _P],
+    "flmm_wald_scan_lowrank": [_I, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64,
+                               _I64, _I64, _P, _P, _D, _P, _P, _P, _P, _P, _P, _P, _P],
     "flmm_max_reduce": [_P, _I64, _I64, _I64, _P, _P],
     "flmm_neglog10p_f1": [_P, _I64, _D, _P, _P],
     "flmm_effect_size": [_P, _I64, _I64, _P, _P, _P, _P],

@@ -1,0 +1,43 @@
+"""Randomised top-r eigendecomposition (the reference's fasterlmm/extreme.py:23-45) on the GPU.
+
+Halko, Martinsson & Tropp (SIAM Review 2011), algorithm 4.4 (randomised subspace iteration) + 5.3 (direct
+eigenvalue decomposition): Y = K Omega for a Gaussian probe Omega (N, r + oversample), n_iter subspace iterations
+Y <- K Y, Q = qr(Y), eigh of the small Q^T K Q, U = Q U_small; the top r pairs are kept.  K is only ever applied to
+a block of vectors -- K = Z Z^T / M as Z (Z^T Y) / M -- so the N x N kernel is never formed.  FP64, rocBLAS GEMMs
+and rocSOLVER's small eigh through torch.
+"""
+from __future__ import annotations
+
+import torch
+from torch import Tensor
+
+
+def randomized_range(apply_k, N: int, rank: int, *, n_oversample: int = 20, n_iter: int = 2, seed: int = 19930909,
+                     dtype=torch.float64, device=None) -> tuple[Tensor, Tensor]:
+    """Top-`rank` eigenpairs (descending) of the symmetric PSD operator apply_k (an (N, b) -> (N, b) product)."""
+    target = min(rank + n_oversample, N)
+    gen = torch.Generator(device=device).manual_seed(seed)
+    Omega = torch.randn(N, target, generator=gen, device=device, dtype=dtype)
+    Y = apply_k(Omega)
+    for _ in range(n_iter):
+        # re-orthonormalise between iterations (keeps the subspace well conditioned; same span)
+        Y = apply_k(torch.linalg.qr(Y)[0])
+    Q, _ = torch.linalg.qr(Y)
+    B = Q.T @ apply_k(Q)
+    B = 0.5 * (B + B.T)
+    s_small, U_small = torch.linalg.eigh(B)
+    s_small = torch.clamp(s_small, min=0.0)
+    idx = torch.argsort(s_small, descending=True)[:rank]
+    return s_small[idx], Q @ U_small[:, idx]
+
+
+def randomized_eigh(Z: Tensor, rank: int = 200, *, n_oversample: int = 20, n_iter: int = 2,
+                    seed: int = 19930909) -> tuple[Tensor, Tensor]:
+    """Top-r eigendecomposition of K = Z Z^T / M (extreme.py:23-45): returns (s_top (r,) descending, U_top (N, r)),
+    on the device and dtype of Z's GPU copy (FP64)."""
+    from .core import _dev
+    home, (Zd,) = _dev(Z)
+    N, M = Zd.shape
+    s, U = randomized_range(lambda Y: Zd @ (Zd.T @ Y) / M, N, int(rank), n_oversample=n_oversample,
+                            n_iter=n_iter, seed=seed, dtype=Zd.dtype, device=Zd.device)
+    return s.to(home), U.to(home)

@@ -1,0 +1,334 @@
+"""Exact low-rank spectral LMM on MI355X: the reference's fasterlmm/lowrank.py API (names, dataclasses, signatures)
+with the delta search and the Wald scan in the gfx950 kernels.
+
+Reference: fasterlmm/lowrank.py (gbrach/FaST-ER-LMM v1.6.2).
+
+The kinship is a factor G (N, k_in) of pruned markers, K = G G^T / k_in, never formed.  The basis U (N, k) spans
+the X-regressed factor; the N - D - k directions outside it see only delta, carried exactly by the off-rank
+residuals UUY = Y_r - U U^T Y_r and UUS (lowrank.py:1-10).  ``lowrank_basis(..., rank=r)`` truncates the factor to
+its leading r singular directions with the randomised range finder (fasterlmm/extreme.py:23-45) -- the rank-r
+kinship of the large-cohort path -- and the exact off-rank algebra then holds for that truncated kinship.
+
+How the kernels carry the off-rank terms (one column p, one SNP s, w = 1/(s_j + delta)):
+  * delta search: the compat loss kernels run on the extended spectrum [s, 0] with rotated column
+    [U^T y_r ; |UUY|] and log-det spectrum [s, 0 x (Neff - k)], which is the low-rank loss (lowrank.py:128-150)
+    term for term; they also produce rWr (on + off rank) and the Q2 planes [w ; 1/delta].
+  * scan (flmm_wald_scan_lowrank): sWy = S1.Q1 over K = kpad2 + N with S1 = [U^T s_r ; s_r] and
+    Q1 = [w o U^T y_r ; UUY / delta], because UUS^T UUY = s_r^T UUY (UUY is orthogonal to U and X);
+    sWs = S2.Q2 over kpad2 only, with S2 = [(U^T s_r)^2 ; |UUS|^2 at entry k].  So the off-rank cross term is
+    one K = N contraction on MFMA and the off-rank SNP sum of squares rides as one extra spectral entry.
+The basis, rotations and off-rank residuals are FP64 (rocBLAS / rocSOLVER through torch); operand planes are
+split-FP16 / split-BF16 / FP32 as ``engine.settings.precision`` says, the epilogue FP64.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+from torch import Tensor
+
+from . import _lib, engine
+from .core import ScanResult, _dev, _eigh, _home
+from .engine import TILE, PTILE, KCHUNK, rup
+
+
+@dataclass
+class LowRankBasis:
+    """Genotype-only part (lowrank.py:25-38): s (k,) ascending kept eigenvalues of the X-regressed kinship, U_eff
+    (N, k) orthonormal, X (N, D), Xpinv (D, N), Neff = N - D."""
+    s: Tensor
+    U_eff: Tensor
+    X: Tensor
+    Xpinv: Tensor
+    Neff: int
+
+
+@dataclass
+class LowRankSpectrum:
+    """A LowRankBasis with one column set rotated onto it (lowrank.py:41-56): UY (k, P), UUY (N, P)."""
+    s: Tensor
+    U_eff: Tensor
+    X: Tensor
+    Xpinv: Tensor
+    Neff: int
+    UY: Tensor
+    UUY: Tensor
+
+
+def _regress(X: Tensor, Xpinv: Tensor, A: Tensor) -> Tensor:
+    return A - X @ (Xpinv @ A)
+
+
+def lowrank_basis(G: Tensor, X: Tensor, *, sv_floor: float = 1e-10, rank: int | None = None,
+                  n_oversample: int = 20, n_iter: int = 2, seed: int = 19930909) -> LowRankBasis:
+    """fastlmm's low-rank decomposition (lowrank.py:59-96): G_ = G - X X^+ G; for k_in < N the eigh of the
+    k_in x k_in Gram G_^T G_ (U = G_ V / sv, s = sv^2 / k_in, singular values <= sv_floor dropped), else eigh of
+    G_ G_^T / k_in.  ``rank`` keeps only the leading `rank` directions, found by the randomised range finder on
+    G_ (extreme.py:23-45; Y = G_ G_^T Omega without forming the kernel), the truncated kinship of the
+    large-cohort path.  Everything on the GPU in FP64; s comes back ascending."""
+    home, (Gd, Xd) = _dev(G, X)
+    N, k_in = Gd.shape
+    D = Xd.shape[1]
+    Xpinv = torch.linalg.pinv(Xd)
+    if rank is not None and rank < min(N, k_in):
+        s, U = _randomized_factor(Gd, Xd, Xpinv, int(rank), n_oversample, n_iter, seed)
+        keep = s.sqrt() * np.sqrt(k_in) > sv_floor
+        s, U = s[keep], U[:, keep]
+        order = torch.argsort(s)
+        s, U = s[order], U[:, order].contiguous()
+    else:
+        G_ = _regress(Xd, Xpinv, Gd)
+        if k_in < N:
+            evals, V = _eigh(G_.T @ G_)
+            sv = torch.clamp(evals, min=0.0).sqrt()
+            keep = sv > sv_floor
+            sv, V = sv[keep], V[:, keep]
+            U = G_ @ (V / sv)
+            s = sv * sv / k_in
+        else:
+            evals, Uf = _eigh((G_ @ G_.T) / k_in)
+            keep = torch.clamp(evals, min=0.0) > sv_floor * sv_floor / k_in
+            s = torch.clamp(evals[keep], min=0.0)
+            U = Uf[:, keep].contiguous()
+    return LowRankBasis(s=s.to(home), U_eff=U.to(home), X=Xd.to(home), Xpinv=Xpinv.to(home), Neff=N - D)
+
+
+def _randomized_factor(G: Tensor, X: Tensor, Xpinv: Tensor, rank: int, n_oversample: int, n_iter: int, seed: int):
+    """Top-`rank` eigenpairs of K_ = G_ G_^T / k_in, G_ = (I - X X^+) G, by the randomised range finder with
+    n_iter subspace iterations (Halko, Martinsson & Tropp 2011, alg. 4.4 + 5.3; extreme.py:23-45)."""
+    from .extreme import randomized_range
+    N, k_in = G.shape
+    proj = lambda A: _regress(X, Xpinv, A)  # noqa: E731
+    apply_k = lambda Y: proj(G @ (G.T @ proj(Y))) / k_in  # noqa: E731
+    return randomized_range(apply_k, N, rank, n_oversample=n_oversample, n_iter=n_iter, seed=seed,
+                            dtype=G.dtype, device=G.device)
+
+
+def rotate_phenos(basis: LowRankBasis, Y: Tensor) -> tuple[Tensor, Tensor]:
+    """lowrank.py:99-111: Y_r = Y - X X^+ Y, UY = U^T Y_r (k, P), UUY = Y_r - U UY (N, P), FP64 on the GPU."""
+    home, (Yd, Xd, Xp, U) = _dev(Y, basis.X, basis.Xpinv, basis.U_eff)
+    Y_r = _regress(Xd, Xp, Yd)
+    UY = U.T @ Y_r
+    return UY.to(home), (Y_r - U @ UY).to(home)
+
+
+def lowrank_rotate(G: Tensor, X: Tensor, Y: Tensor, *, sv_floor: float = 1e-10, rank: int | None = None,
+                   **kw) -> LowRankSpectrum:
+    """lowrank.py:114-122: basis + one rotation."""
+    b = lowrank_basis(G, X, sv_floor=sv_floor, rank=rank, **kw)
+    UY, UUY = rotate_phenos(b, Y)
+    return LowRankSpectrum(s=b.s, U_eff=b.U_eff, X=b.X, Xpinv=b.Xpinv, Neff=b.Neff, UY=UY, UUY=UUY)
+
+
+# DELTA SEARCH -------
+
+class _LowRankState:
+    """GPU state of one spectrum: the extended spectrum and rotated-column plane that make the compat delta
+    kernels evaluate the low-rank loss, and the last fit."""
+
+    def __init__(self, spec: LowRankSpectrum, prec: int):
+        _, (s, UY, UUY) = _dev(spec.s, spec.UY, spec.UUY)
+        k, P = UY.shape
+        self.k, self.P, self.Neff, self.N = k, P, int(spec.Neff), UUY.shape[0]
+        self.prec = prec
+        dev = s.device
+        self.s = torch.cat([s, torch.zeros(1, dtype=torch.float64, device=dev)]).contiguous()  # weight spectrum
+        self.s_lam = torch.cat([s, torch.zeros(max(self.Neff - k, 0), dtype=torch.float64, device=dev)]).contiguous()
+        self.neff = k + 1
+        self.kpad = rup(k + 1, TILE)
+        self.p_pad = rup(max(P, 1), PTILE)
+        yy_off = (UUY * UUY).sum(dim=0)
+        uy_ext = torch.cat([UY, yy_off.sqrt().unsqueeze(0)], dim=0).contiguous()  # (k + 1, P)
+        self.uy = engine.pack(_lib.PREC_FP32, uy_ext, P, k + 1, 1, P, self.p_pad, self.kpad).hi
+        self.delta = None
+
+    def _loss(self, projected: bool = True):
+        return self.s_lam, float(self.Neff)
+
+    def fit(self, n_grid: int, lo: float, hi: float, refine: bool) -> engine.DeltaResult:
+        s_lam, nco = self._loss()
+        if not refine:
+            d = engine.delta_refine(self.prec, _lib.DELTA_GRID, self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam,
+                                    nco, n_grid, lo, hi, None)
+        elif engine.settings.delta == "learned":
+            from . import learned
+            fit = learned.refine_learned(self, learned.model(), n_grid, lo, hi, n=self.N, c=self.N - self.Neff)
+            self.last_fallback = fit.fallback
+            d = fit.delta
+        else:
+            idx = engine.delta_grid_idx(self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam, nco, n_grid, lo, hi)
+            d = engine.delta_refine(self.prec, engine.delta_code(), self.uy, self.P, self.p_pad, self.kpad, self.s,
+                                    s_lam, nco, n_grid, lo, hi, idx)
+        self.delta = d
+        return d
+
+    def delta_for(self, log_delta: Tensor) -> engine.DeltaResult:
+        d = self.delta
+        if d is not None and d.log_delta.data_ptr() == log_delta.data_ptr() and d.q1.prec == self.prec:
+            return d
+        return engine.delta_refine(self.prec, _lib.DELTA_FIXED, self.uy, self.P, self.p_pad, self.kpad, self.s,
+                                   self.s_lam, float(self.Neff), 1, 0.0, 0.0, None, log_delta_in=log_delta)
+
+
+def _state(spec: LowRankSpectrum) -> _LowRankState:
+    st = getattr(spec, "_gpu", None)
+    prec = engine.prec_code()
+    if st is None or st.prec != prec:
+        st = _LowRankState(spec, prec)
+        spec._gpu = st
+    return st
+
+
+def fit_delta_grid_lowrank(spec: LowRankSpectrum, *, n_grid: int = 64, log_delta_min: float = -10.0,
+                           log_delta_max: float = 10.0, refine: bool = True) -> Tensor:
+    """lowrank.py:172-208: per-column log delta on the low-rank (ML) loss, grid then refinement in
+    [g[i-1], g[i+1]] (Newton to the bracket's minimiser by default, the reference's golden section with
+    engine.configure(delta="golden")).  Returns (P,) FP64 on the spectrum's device."""
+    st = _state(spec)
+    d = st.fit(n_grid, float(log_delta_min), float(log_delta_max), refine)
+    return d.log_delta.to(spec.s.device)
+
+
+# SCAN -------
+
+def _norms(parts: list[Tensor]) -> Tensor:
+    """Row norms of the concatenation of several (n_k_i, n_rows) column blocks (each column = one row)."""
+    return torch.sqrt(sum((p * p).sum(dim=0) for p in parts))
+
+
+def pack_concat(prec: int, parts: list[Tensor], rows_pad: int, kpads: list[int]) -> engine.Planes:
+    """Operand planes whose K axis is the concatenation of the row-major (n_k_i, n_rows) FP64 blocks `parts`,
+    block i occupying kpads[i] (a multiple of 32) entries; split-FP16 rows share one scale (from the norm of the
+    whole row), so each block is packed with the combined norm at its K offset."""
+    n_rows = parts[0].shape[1]
+    dev = parts[0].device
+    pl = engine.Planes(prec, rows_pad, sum(kpads), dev)
+    norm = _norms(parts).contiguous() if pl.prec == _lib.PREC_SPLIT_FP16 else None
+    esize = pl.hi.element_size()
+    k0 = 0
+    st = _lib.stream()
+    for part, kp in zip(parts, kpads):
+        part = part.contiguous()
+        off = k0 * rows_pad * esize
+        hi = pl.hi.data_ptr() + off
+        lo = pl.lo.data_ptr() + off if pl.lo is not None else None
+        _lib.call("flmm_pack_planes", pl.prec, part.data_ptr(), n_rows, part.shape[0], 1, n_rows, None,
+                  _lib.ptr(norm), hi, lo, pl.scale_ptr, rows_pad, kp, st)
+        k0 += kp
+    return pl
+
+
+class _ColumnPlanes:
+    """Q1 = [w o UY ; UUY / delta] for one delta fit (the column side of the low-rank scan)."""
+
+    def __init__(self, st: _LowRankState, spec: LowRankSpectrum, d: engine.DeltaResult):
+        _, (s, UY, UUY) = _dev(spec.s, spec.UY, spec.UUY)
+        delta = d.log_delta.exp()
+        w = 1.0 / (s.unsqueeze(1) + delta.unsqueeze(0))  # (k, P)
+        self.npad = rup(st.N, TILE)
+        self.kpad2 = st.kpad
+        self.q1 = pack_concat(st.prec, [w * UY, UUY / delta.unsqueeze(0)], st.p_pad, [st.kpad, self.npad])
+        self.d = d
+
+
+def snp_wald_scan_lowrank(spec: LowRankSpectrum, log_delta: Tensor, S: Tensor, *, snp_chunk: int | None = None,
+                          pheno_chunk: int | None = None, n_real: int | None = None) -> ScanResult:
+    """lowrank.py:239-316: Wald F / beta / SE / SFVE / nullh2 of every SNP column of S (N, M; standardised)
+    against every column of the spectrum, on- and off-rank terms exact, dof Neff - 1; the first n_real columns
+    keep full detail and every column its maximum F.  pheno_chunk is accepted for signature compatibility (all
+    columns scan together); SNPs go in blocks of snp_chunk (default 16,384)."""
+    st = _state(spec)
+    home, (Sd, ldd) = _dev(S, log_delta)
+    M = Sd.shape[1]
+    n_real = st.P if n_real is None else min(int(n_real), st.P)
+    d = st.delta_for(ldd)
+    cp = _ColumnPlanes(st, spec, d)
+    out = engine.ScanOutputs.allocate(M, n_real, st.P, Sd.device)
+    _scan_blocks(st, spec, cp, Sd, torch.arange(M, device=Sd.device), n_real, out, snp_chunk or 16384)
+    return _home(ScanResult(out.f, out.beta, out.se, out.sfve, out.nullh2, out.max_F), home)
+
+
+def _scan_blocks(st: _LowRankState, spec: LowRankSpectrum, cp: _ColumnPlanes, S: Tensor, row_index: Tensor,
+                 n_real: int, out: engine.ScanOutputs, block: int) -> None:
+    """Scan the SNP columns S[:, j] (j < S.shape[1]) into output rows row_index[j]."""
+    _, (X, Xp, U) = _dev(spec.X, spec.Xpinv, spec.U_eff)
+    for b0 in range(0, S.shape[1], block):
+        with engine.stage("snps"):
+            S_r = _regress(X, Xp, S[:, b0:b0 + block])
+            ri = row_index[b0:b0 + block]
+            s1, s2, mc = snp_planes_lowrank(st, U, S_r)
+            del S_r
+        with engine.stage("scan"):
+            wald_scan_lowrank(s1, s2, cp, mc, st, n_real, ri, out)
+
+
+def snp_planes_lowrank(st: _LowRankState, U: Tensor, S_r: Tensor):
+    """S1 = [U^T s_r ; s_r] (K = kpad2 + N_pad) and S2 = [(U^T s_r)^2 ; |s_r|^2 - |U^T s_r|^2 at entry k]
+    (K = kpad2) for the regressed-out SNP block S_r (N, mc)."""
+    mc = S_r.shape[1]
+    rows_pad = rup(mc, TILE)
+    US = U.T @ S_r  # (k, mc)
+    us2 = US * US
+    ss_off = torch.clamp((S_r * S_r).sum(dim=0) - us2.sum(dim=0), min=0.0)
+    s1 = pack_concat(st.prec, [US, S_r], rows_pad, [st.kpad, rup(st.N, TILE)])
+    s2 = pack_concat(st.prec, [torch.cat([us2, ss_off.unsqueeze(0)], dim=0)], rows_pad, [st.kpad])
+    return s1, s2, mc
+
+
+def wald_scan_lowrank(s1: engine.Planes, s2: engine.Planes, cp: _ColumnPlanes, mc: int, st: _LowRankState,
+                      n_real: int, row_index: Tensor, out: engine.ScanOutputs) -> None:
+    if mc == 0 or st.P == 0:
+        return
+    d = cp.d
+    n_mt = -(-mc // TILE)
+    pmax = torch.empty(n_mt * st.p_pad, dtype=torch.float64, device=s1.hi.device)
+    o = (out.f, out.beta, out.se, out.sfve, out.nullh2)
+    optr = [t.data_ptr() if n_real > 0 else None for t in o]
+    (s1h, s1l), (s2h, s2l) = s1.ptrs, s2.ptrs
+    (q1h, q1l), (q2h, q2l) = cp.q1.ptrs, d.q2.ptrs
+    stream = _lib.stream()
+    ev = None
+    if engine.scan_timer.enabled:
+        ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        ev[0].record()
+    _lib.call("flmm_wald_scan_lowrank", s1.prec, s1h, s1l, s2h, s2l, s1.scale_ptr, s2.scale_ptr, s1.rows_pad,
+              q1h, q1l, q2h, q2l, cp.q1.scale_ptr, d.q2.scale_ptr, st.p_pad, s1.kpad, s2.kpad, mc, st.P, n_real,
+              d.rWr.data_ptr(), d.h2.data_ptr(), float(st.Neff - 1), row_index.data_ptr(), *optr, pmax.data_ptr(),
+              stream)
+    if ev is not None:
+        ev[1].record()
+        # algorithmic flops: on-rank 4 k per test, off-rank cross term 2 N per test
+        engine.scan_timer.launches.append((ev[0], ev[1], mc * st.P, s1.kpad, st.k + st.N // 2))
+    _lib.call("flmm_max_reduce", pmax.data_ptr(), n_mt, st.p_pad, st.P, out.max_F.data_ptr(), stream)
+
+
+# DRIVERS -------
+
+def loco_scan_lowrank(G: Tensor, g_chrom: np.ndarray, Z: Tensor, z_chrom: np.ndarray, X: Tensor, Y: Tensor, *,
+                      n_real: int | None = None, on_chrom=None, rank: int | None = None,
+                      snp_chunk: int = 16384) -> ScanResult:
+    """lowrank.py:322-360: per chromosome c (sorted labels of z_chrom) the kinship factor drops c's markers, the
+    basis, per-column delta and the scan of c's SNPs; max_F is the maximum over chromosomes.  G and Z are
+    column-standardised.  ``rank`` truncates each chromosome's kinship (randomised range finder)."""
+    from .extreme_scan import _loco_scan_core
+    home, (Zd,) = _dev(Z)
+    z_chrom = np.asarray(z_chrom)
+    chroms = sorted(np.unique(z_chrom).tolist())
+
+    def block_source(c):
+        cols = np.flatnonzero(z_chrom == c)
+        for s in range(0, len(cols), snp_chunk):
+            cidx = cols[s:s + snp_chunk]
+            yield Zd.index_select(1, torch.from_numpy(cidx).to(Zd.device)), cidx
+
+    res = _loco_scan_core(chroms, block_source, Zd.shape[1], G, np.asarray(g_chrom), X, Y, n_real=n_real,
+                          block_size=snp_chunk, on_chrom=on_chrom, rank=rank)
+    return _home(res, home)
+
+
+def single_k_scan_lowrank(G: Tensor, Z: Tensor, X: Tensor, Y: Tensor, *, n_real: int | None = None,
+                          rank: int | None = None) -> ScanResult:
+    """lowrank.py:363-373: one kinship factor, one rotation + delta fit, one scan over all of Z."""
+    spec = lowrank_rotate(G, X, Y, rank=rank)
+    return snp_wald_scan_lowrank(spec, fit_delta_grid_lowrank(spec), Z, n_real=n_real)

@@ -198,6 +198,23 @@ int flmm_mlp_mse_grad(const float* pred, const float* target, int64_t B, float* 
 int flmm_adam(float* w, const float* g, float* m, float* v, int64_t n, double lr, double beta1, double beta2,
               double eps, int step, void* stream);
 
+/* Fused Wald scan of one SNP block in the low-rank basis (snp_wald_scan_lowrank, fasterlmm/lowrank.py:239-316):
+ * the arguments of flmm_wald_scan, with the first product sWy = S1.Q1 over kpad and the second sWs = S2.Q2 over
+ * the first kpad2 < kpad entries only (S2 / Q2 planes hold kpad2).  The caller lays out
+ *   S1 = [U^T s_r ; s_r]  (K = kpad2 on-rank entries, then the N entries of the regressed-out SNP s_r)
+ *   Q1 = [w o U^T y_r ; (y_r - U U^T y_r) / delta]   (same split)
+ *   S2 = [(U^T s_r)^2 ; |s_r - U U^T s_r|^2 at entry k]   Q2 = [w ; 1/delta at entry k]   (k = rank)
+ * so sWy and sWs carry both the on-rank and the exact off-rank terms (lowrank.py:282-289; the cross term uses
+ * (s_r - U U^T s_r)^T (y_r - U U^T y_r) = s_r^T (y_r - U U^T y_r)), with one row / column scale per plane.
+ * Needs p_pad % 256 == 0 and kpad2 % 128 == 0. */
+int flmm_wald_scan_lowrank(int prec, const void* s1_hi, const void* s1_lo, const void* s2_hi, const void* s2_lo,
+                           const float* s1_scale, const float* s2_scale, int64_t mc_pad, const void* q1_hi,
+                           const void* q1_lo, const void* q2_hi, const void* q2_lo, const float* q1_scale,
+                           const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t kpad2, int64_t mc, int64_t P,
+                           int64_t n_real, const double* rWr, const double* h2, double nu, const int64_t* row_index,
+                           double* f, double* beta, double* se, double* sfve, double* nullh2, double* pmax,
+                           void* stream);
+
 /* Fused Wald scan of one SNP block (snp_wald_scan_compat's tile loop, core.py:636-667): sWy = S1.Q1, sWs = S2.Q2 on
  * MFMA, FP64 epilogue with dof nu; split-FP16 sums are divided by the planes' row scales.  Real columns
  * p < n_real write f, beta, se, sfve, nullh2 at [row_index[m] * n_real + p] of (M, n_real) row-major FP64 arrays
