@@ -2,6 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--precision P] [--no-compare] [--no-cpu]
     python bench.py --config 3      (uneven NA groups: grouped launch vs per-group loop, see run_grouped)
+    python bench.py --config 5      (large-cohort low-rank LOCO, see run_lowrank)
 
 Workload (one "step"): the fastlmm-compat leave-one-chromosome-out scan that perm_threshold drives, over one
 synthetic panel resident in HBM -- per chromosome the LOCO kinship, the projected FP64 eigendecomposition, the
@@ -40,6 +41,8 @@ def parse():
     ap.add_argument("--n-perm", type=int, default=None)
     ap.add_argument("--pheno", type=int, default=None, help="override the number of real phenotypes")
     ap.add_argument("--snps", type=int, default=None, help="override M")
+    ap.add_argument("--n-samples", type=int, default=None, help="override N (config 5)")
+    ap.add_argument("--rank", type=int, default=None, help="kinship rank (config 5)")
     ap.add_argument("--precision", default="split_fp16", choices=["split_fp16", "split_bf16", "fp32"])
     ap.add_argument("--no-compare", action="store_true", help="skip the split-BF16 / FP32 comparison steps")
     ap.add_argument("--no-cpu", action="store_true")
@@ -269,6 +272,154 @@ def run_grouped(args):
     print(json.dumps(line), flush=True)
 
 
+def run_lowrank(args):
+    """Large-cohort low-rank path: N=50,000 x M=500,000 int8 dosages resident in HBM, scanned in 16,384-SNP blocks
+    against a rank-2,000 kinship of 50,000 evenly strided markers (randomised range finder per LOCO kinship),
+    P=1,000 phenotypes x (1 + 20 permutations), C=10, LOCO over 22 chromosomes.  One step = the whole LOCO scan.
+    The CPU leg runs the FP64 low-rank oracle on a 1% SNP subsample of the first and last chromosomes (real
+    phenotypes, with the GPU's deterministic basis and deltas) and reports its rate and the GPU's error against it."""
+    import numpy as np
+    import torch
+    from fasterlmm_amd import engine, extreme_scan, sim
+    from fasterlmm_amd.io import standardise_columns
+    cfg = dict(sim.CONFIGS[5])
+    for k, v in (("P", args.pheno), ("M", args.snps), ("N", args.n_samples)):
+        if v:
+            cfg[k] = v
+    if args.snps:
+        cfg["n_kin"] = min(cfg["n_kin"], cfg["M"] // 10)
+    n_perm = cfg["n_perm"] if args.n_perm is None else args.n_perm
+    rank = cfg["rank"] if args.rank is None else args.rank
+    t0 = time.perf_counter()
+    panel = sim.make_panel_gpu(cfg, seed=19930909, block=cfg["block"])
+    dev = panel.G.device
+    B = cfg["P"]
+    N = cfg["N"]
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(7)
+    Y_all = torch.empty(N, B * (1 + n_perm), dtype=torch.float64, device=dev)
+    Y_all[:, :B] = panel.Y
+    for b0 in range(0, B, 64):
+        b1 = min(B, b0 + 64)
+        orders = torch.rand(b1 - b0, n_perm, N, generator=gen, device=dev).argsort(dim=2)
+        yp = torch.gather(panel.Y[:, b0:b1].T.unsqueeze(1).expand(b1 - b0, n_perm, N), 2, orders)
+        Y_all[:, B + b0 * n_perm:B + b1 * n_perm] = yp.reshape((b1 - b0) * n_perm, N).T
+        del orders, yp
+    kin = torch.from_numpy(panel.kin_idx).to(dev)
+    Gk = standardise_columns(panel.G.index_select(1, kin))
+    g_chrom = panel.chrom[panel.kin_idx]
+    torch.cuda.synchronize()
+    log(f"[bench5] panel N={N} M={cfg['M']} P={B}x(1+{n_perm}) C={cfg['C']} kinship markers {Gk.shape[1]} "
+        f"rank {rank} in {time.perf_counter() - t0:.1f}s")
+    prec = args.precision
+    M = cfg["M"]
+    P = Y_all.shape[1]
+    tests = M * P
+
+    def step():
+        with engine.configure(precision=prec):
+            return extreme_scan.loco_scan_resident(
+                panel.G, panel.chrom, Gk, g_chrom, panel.X, Y_all, n_real=B, block_size=cfg["block"], rank=rank,
+                on_chrom=lambda k, n: log(f"[bench5] chromosome {k}/{n} done") if k % 4 == 0 or k == n else None)
+
+    for w in range(args.warmup):
+        a = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        log(f"[bench5] warmup {w} {time.perf_counter() - a:.1f}s")
+    engine.scan_timer.reset(True)
+    engine.stage_timer.reset(True)
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    for _ in range(args.steps):
+        res = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - a) / args.steps
+    scan = engine.scan_timer.summary()
+    stages = {k: round(v / args.steps, 1) for k, v in engine.stage_timer.summary().items()}
+    engine.scan_timer.reset(False)
+    engine.stage_timer.reset(False)
+    value = tests / dt
+    split = prec != "fp32"
+    products = 3 if split else 1
+    published = MI355X_PEAK_BF16_TFLOPS if split else MI355X_PEAK_FP32_TFLOPS
+    alg = scan["alg_flops"] / (scan["total_ms"] * 1e-3) / 1e12 if scan["total_ms"] else 0.0
+    log(f"[bench5] {prec}: {dt:.2f} s/step, {value:.4e} tests/s, scan kernels {scan['launches']} launches "
+        f"{scan['total_ms'] / args.steps:.0f} ms/step ({alg:.1f} TF/s alg), stages {stages}")
+    cpu = None
+    if not args.no_cpu:
+        cpu = lowrank_cpu_check(panel, Gk, g_chrom, Y_all, B, res, rank, prec)
+        log(f"[bench5] cpu leg: {cpu}")
+    line = {
+        "metric": "Wald tests/s (SNP x pheno x perm), low-rank LOCO scan, int8 genotypes resident in HBM",
+        "value": value, "unit": "Wald tests/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1000 * dt, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"split_fp16": "fp16", "split_bf16": "bf16", "fp32": "fp32"}[prec], "data": "synthetic",
+        "config": {"workload": f"low-rank LOCO: N={N} M={M} int8 in {cfg['block']}-SNP blocks "
+                               f"({cfg['n_chrom']} chromosomes), rank-{rank} kinship from {Gk.shape[1]} strided "
+                               f"markers, P={B} x (1+{n_perm} perms) = {P} columns, C={cfg['C']}",
+                   "tests_per_step": tests, "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": round(alg, 2), "peak": round(published / products, 1),
+                     "unit": "TFLOP/s", "frac": round(alg * products / published, 4), "traffic": None,
+                     "kernel": "wald_scan_lowrank_kernel", "mean_launch_ms": round(scan["mean_ms"], 3),
+                     "launches_per_step": scan["launches"] // args.steps,
+                     "note": "algorithmic flops per test = 4 k (on-rank products) + 2 N (off-rank cross term)"},
+        "stages_ms_per_step": stages, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def lowrank_cpu_check(panel, Gk, g_chrom, Y_all, B, res, rank, prec):
+    """The FP64 low-rank oracle (numpy, host cores) on a 1% SNP subsample of the first and last chromosomes and the
+    real phenotypes, given the GPU's basis and deltas for those chromosomes (recomputed; the GPU path is
+    deterministic); returns the rate and the GPU's error against it."""
+    import numpy as np
+    import torch
+    from scipy import stats
+    from fasterlmm_amd import engine, lowrank
+    from oracle import lmm, lowrank as olr
+    threads = len(os.sched_getaffinity(0))
+    chroms = sorted(np.unique(panel.chrom).tolist())
+    rng = np.random.default_rng(5)
+    t_cpu, n_tests, dlp, dbeta, drelf, n_inf = 0.0, 0, 0.0, 0.0, 0.0, 0
+    X = panel.X.cpu().numpy()
+    Nn = X.shape[0]
+    df = Nn - X.shape[1] - 1
+    for c in (chroms[0], chroms[-1]):
+        keep = torch.from_numpy(np.flatnonzero(g_chrom != c)).to(Gk.device)
+        with engine.configure(precision=prec):
+            b = lowrank.lowrank_basis(Gk.index_select(1, keep), panel.X, rank=rank)
+            UY, UUY = lowrank.rotate_phenos(b, Y_all)
+            spec = lowrank.LowRankSpectrum(s=b.s, U_eff=b.U_eff, X=b.X, Xpinv=b.Xpinv, Neff=b.Neff, UY=UY, UUY=UUY)
+            ld = lowrank.fit_delta_grid_lowrank(spec)
+        cols = np.flatnonzero(panel.chrom == c)
+        sub = np.sort(rng.choice(cols, size=max(1, len(cols) // 100), replace=False))
+        S = lmm.standardise_columns(panel.G[:, torch.from_numpy(sub).to(Gk.device)].cpu().numpy().astype(np.float64))
+        ospec = {"s": b.s.cpu().numpy(), "U_eff": b.U_eff.cpu().numpy(), "X": X, "Xpinv": b.Xpinv.cpu().numpy(),
+                 "Neff": b.Neff, "UY": UY[:, :B].cpu().numpy(), "UUY": UUY[:, :B].cpu().numpy()}
+        ldr = ld[:B].cpu().numpy()
+        del b, UY, UUY, spec
+        a = time.perf_counter()
+        ref = olr.snp_wald_scan_lowrank(ospec, ldr, S, n_real=B)
+        t_cpu += time.perf_counter() - a
+        n_tests += S.shape[1] * B
+        f = res.f[torch.from_numpy(sub).to(res.f.device)].cpu().numpy()
+        beta = res.beta[torch.from_numpy(sub).to(res.f.device)].cpu().numpy()
+        lp = lambda F: -stats.f.logsf(F, 1, df) / np.log(10.0)  # noqa: E731
+        la, lb = lp(f), lp(ref["f"])
+        fin = np.isfinite(la) & np.isfinite(lb)  # scipy's tail underflows to -inf for the very largest F
+        dlp = max(dlp, float(np.abs(la[fin] - lb[fin]).max()) if fin.any() else 0.0)
+        n_inf += int((~fin).sum())
+        drelf = max(drelf, float((np.abs(f - ref["f"]) / np.maximum(ref["f"], 1.0)).max()))
+        dbeta = max(dbeta, float((np.abs(beta - ref["beta"]) / np.maximum(np.abs(ref["beta"]), ref["se"])).max()))
+    return {"value": n_tests / t_cpu, "unit": "Wald tests/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle low-rank Wald scan (numpy/BLAS, {threads} threads) of a 1% SNP subsample of "
+                      f"chromosomes {chroms[0]} and {chroms[-1]} x {B} real phenotypes, given the GPU basis and "
+                      f"deltas of those chromosomes: {n_tests} tests in {t_cpu:.1f}s",
+            "gpu_vs_oracle": {"max_abs_dlog10p": dlp, "max_dbeta_over_max_beta_se": dbeta, "max_dF_over_max_F_1": drelf,
+                              "tests_with_p_below_double_range": n_inf}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -277,6 +428,9 @@ def main():
         sys.exit(2)
     if args.config == 3:
         run_grouped(args)
+        return
+    if args.config == 5:
+        run_lowrank(args)
         return
     import numpy as np
     import torch

@@ -59,8 +59,10 @@ struct AStage {
 // One 128 x 256 tile (logical index `tile` of the n_mtiles x n_ptiles grid, in the grouped order below) of the
 // wide Wald scan, run by a 256-thread workgroup.
 // LOWRANK: the first product (sWy) runs over all args.nchunks K-chunks and the second (sWs) over the first
-// args.nchunks2 only (the low-rank scan's off-rank cross term lives in the first product alone); chunks past
-// nchunks2 load, stage and multiply the first product's planes only.
+// args.nchunks2 only (the low-rank scan's off-rank cross term lives in the first product alone).  The chunks past
+// nchunks2 (K = N, the long part) run first and load, stage and multiply the first product's planes only; their
+// sums go to block partials of FLUSH chunks in the second accumulator set (idle until then), folded into the
+// first, so no FP32 accumulation chain runs over more than FLUSH chunks plus the folds.
 template <int PREC, bool LOWRANK = false>
 __device__ inline void wide_tile(
     const TileArgs<PREC, 2>& args, const ScanScales& sc, int64_t n_mtiles, int64_t n_ptiles, int64_t tile, int64_t Mc,
@@ -166,8 +168,10 @@ __device__ inline void wide_tile(
         for (int v = 0; v < 16; ++v) acc[g][i][j][v] = 0.f;
 
   // one MFMA step (16 physical k per lane half) of chunk stage `st` with column fragments bf
-  auto mma_step = [&](auto two, const char* st, int step, const BChunk<NSUB, NSTEP>& bf) {
+  // `tgt`: accumulator set of the single product when `two` is false
+  auto mma_step = [&](auto two, auto tgt, const char* st, int step, const BChunk<NSUB, NSTEP>& bf) {
     constexpr int NG = decltype(two)::value ? 2 : 1;
+    constexpr int T1 = decltype(tgt)::value;
     if constexpr (is_split(PREC)) {
       const int seg = h * 2 + step;
 #pragma unroll
@@ -186,11 +190,11 @@ __device__ inline void wide_tile(
           const bf16x8 bl = __builtin_bit_cast(bf16x8, bf.v[(g * 2 + bj) * 2 + 1][step]);
 #pragma unroll
           for (int bi = 0; bi < 4; ++bi) {
-            f32x16 c = acc[g][bi][bj];
+            f32x16 c = acc[NG == 2 ? g : T1][bi][bj];
             c = mfma_split<PREC>(af[bi][0], bh, c);
             c = mfma_split<PREC>(af[bi][0], bl, c);
             c = mfma_split<PREC>(af[bi][1], bh, c);
-            acc[g][bi][bj] = c;
+            acc[NG == 2 ? g : T1][bi][bj] = c;
           }
         }
       }
@@ -211,7 +215,8 @@ __device__ inline void wide_tile(
           for (int e = 0; e < 4; ++e)
 #pragma unroll
             for (int bi = 0; bi < 4; ++bi)
-              acc[g][bi][bj] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[bi][e], b4[e], acc[g][bi][bj], 0, 0, 0);
+              acc[NG == 2 ? g : T1][bi][bj] =
+                  __builtin_amdgcn_mfma_f32_32x32x2f32(af[bi][e], b4[e], acc[NG == 2 ? g : T1][bi][bj], 0, 0, 0);
         }
       }
     }
@@ -221,15 +226,15 @@ __device__ inline void wide_tile(
   // the SNP-side block and the column fragments of chunk kc + 1 are loaded into registers, so both have a whole
   // chunk of MFMAs (96 per wave) to land; the SNP-side block is written to the other LDS buffer at the end of the
   // chunk.  Past the last chunk the loads repeat the last chunk (valid addresses, values unused).
-  auto chunk = [&](auto cur2, auto next2, int kc, const char* cur, char* nxt, const BChunk<NSUB, NSTEP>& bc,
+  // kn: the (physical) chunk whose operands this chunk prefetches
+  auto chunk = [&](auto cur2, auto next2, auto tgt, int kn, const char* cur, char* nxt, const BChunk<NSUB, NSTEP>& bc,
                    BChunk<NSUB, NSTEP>& bn) {
     constexpr bool C2 = decltype(cur2)::value, N2 = decltype(next2)::value;
-    const int kn = kc + 1 < nchunks ? kc + 1 : kc;
     const AStage<NA> an = load_a(next2, kn);
     bn = load_b(next2, kn);
     if constexpr (!is_split(PREC)) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int step = 0; step < NSTEP; ++step) mma_step(cur2, cur, step, bc);
+    for (int step = 0; step < NSTEP; ++step) mma_step(cur2, tgt, cur, step, bc);
     if constexpr (is_split(PREC) && C2 && N2) {
       // Interleave per MFMA step (16 LDS fragment reads, 48 MFMAs, and 12 of the chunk's 24 global loads): the
       // first product's 8 fragment reads up front, the second product's 8 reads and 4 global loads spread over the
@@ -276,27 +281,47 @@ __device__ inline void wide_tile(
   };
   constexpr std::true_type two{};
   constexpr std::false_type one{};
-  BChunk<NSUB, NSTEP> be = load_b(two, 0), bo;
-  store_a(two, load_a(two, 0), lds0);
-  __syncthreads();
+  constexpr std::integral_constant<int, 0> t0{};
+  constexpr std::integral_constant<int, 1> t1{};
   if constexpr (!LOWRANK) {
-    // nchunks is even (K is padded to a multiple of 128)
+    BChunk<NSUB, NSTEP> be = load_b(two, 0), bo;
+    store_a(two, load_a(two, 0), lds0);
+    __syncthreads();
+    // nchunks is even (K is padded to a multiple of 128); past the last chunk the prefetch repeats it
     for (int kc = 0; kc < nchunks; kc += 2) {
-      chunk(two, two, kc, lds0, lds1, be, bo);
-      chunk(two, two, kc + 1, lds1, lds0, bo, be);
+      chunk(two, two, t0, kc + 1 < nchunks ? kc + 1 : kc, lds0, lds1, be, bo);
+      chunk(two, two, t0, kc + 2 < nchunks ? kc + 2 : kc + 1, lds1, lds0, bo, be);
     }
   } else {
-    // nchunks2 and nchunks are multiples of 4 with 0 < nchunks2 < nchunks
+    // nchunks2 and nchunks are multiples of 4 with 0 < nchunks2 < nchunks.  Phase A: chunks n2 .. nchunks - 1
+    // (first product only, block partials in acc[1]); phase B: chunks 0 .. n2 - 1 (both products).
+    constexpr int FLUSH = 32;
     const int n2 = args.nchunks2;
-    for (int kc = 0; kc < n2 - 2; kc += 2) {
-      chunk(two, two, kc, lds0, lds1, be, bo);
-      chunk(two, two, kc + 1, lds1, lds0, bo, be);
+    const int nA = nchunks - n2;
+    auto fold = [&]() {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc[0][i][j] += acc[1][i][j];
+#pragma unroll
+          for (int v = 0; v < 16; ++v) acc[1][i][j][v] = 0.f;
+        }
+    };
+    BChunk<NSUB, NSTEP> be = load_b(one, n2), bo;
+    store_a(one, load_a(one, n2), lds0);
+    __syncthreads();
+    for (int l = 0; l < nA - 2; l += 2) {
+      chunk(one, one, t1, n2 + l + 1, lds0, lds1, be, bo);
+      chunk(one, one, t1, n2 + l + 2, lds1, lds0, bo, be);
+      if ((l + 2) % FLUSH == 0) fold();
     }
-    chunk(two, two, n2 - 2, lds0, lds1, be, bo);
-    chunk(two, one, n2 - 1, lds1, lds0, bo, be);
-    for (int kc = n2; kc < nchunks; kc += 2) {
-      chunk(one, one, kc, lds0, lds1, be, bo);
-      chunk(one, one, kc + 1, lds1, lds0, bo, be);
+    chunk(one, one, t1, nchunks - 1, lds0, lds1, be, bo);
+    chunk(one, two, t1, 0, lds1, lds0, bo, be);
+    fold();
+    for (int kc = 0; kc < n2; kc += 2) {
+      chunk(two, two, t0, kc + 1, lds0, lds1, be, bo);
+      chunk(two, two, t0, kc + 2 < n2 ? kc + 2 : kc + 1, lds1, lds0, bo, be);
     }
   }
 

@@ -112,3 +112,63 @@ def make_panel(config: int | dict, seed: int = 19930909, *, n_causal: int = 10) 
     if cfg.get("rint"):
         Y = rint_columns(Y)
     return Panel(G=G, chrom=chrom, X=X, Y=Y, h2=h2, causal=causal)
+
+
+# config 5: the large-cohort low-rank workload, generated on the GPU
+CONFIGS[5] = dict(N=50_000, M=500_000, n_chrom=22, P=1000, C=10, maf=(0.01, 0.5), h2=(0.0, 0.9), rint=False,
+                  n_perm=20, n_kin=50_000, rank=2000, block=16384)
+
+
+@dataclass
+class GpuPanel:
+    G: "torch.Tensor"      # (N, M) int8 dosages on the GPU
+    chrom: np.ndarray      # (M,) chromosome labels, sorted
+    kin_idx: np.ndarray    # kinship marker columns (an even stride across the genome)
+    X: "torch.Tensor"      # (N, C) FP64 covariates, intercept first
+    Y: "torch.Tensor"      # (N, P) FP64 phenotypes
+
+
+def make_panel_gpu(cfg: dict, seed: int = 19930909, *, n_causal: int = 10, block: int = 16384,
+                   device=None) -> GpuPanel:
+    """The same model as make_panel (Balding-Nichols genotypes over 4 subpopulations, Fst = 0.1, phenotypes with
+    n_causal standardised causal SNPs at heritability h2, intercept + N(0, 1) covariates) drawn with torch's GPU
+    generator, for panels too large to draw on the host: dosages are sums of two Bernoulli(p_k) draws."""
+    import torch
+    dev = torch.device("cuda") if device is None else torch.device(device)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    N, M, P, C = cfg["N"], cfg["M"], cfg["P"], cfg["C"]
+    fst, n_pop = 0.1, 4
+    pop = torch.arange(N, device=dev) * n_pop // N
+    G = torch.empty((N, M), dtype=torch.int8, device=dev)
+    for j0 in range(0, M, block):
+        j1 = min(M, j0 + block)
+        p = torch.rand(j1 - j0, generator=gen, device=dev, dtype=torch.float64) * (cfg["maf"][1] - cfg["maf"][0]) \
+            + cfg["maf"][0]
+        a = p * (1 - fst) / fst
+        b = (1 - p) * (1 - fst) / fst
+        ga = torch._standard_gamma(a.unsqueeze(0).expand(n_pop, -1).contiguous(), generator=gen)
+        gb = torch._standard_gamma(b.unsqueeze(0).expand(n_pop, -1).contiguous(), generator=gen)
+        pk = (ga / (ga + gb)).float()[pop]  # (N, b) subpopulation frequency of each strain
+        g = (torch.rand(pk.shape, generator=gen, device=dev) < pk).to(torch.int8)
+        g += (torch.rand(pk.shape, generator=gen, device=dev) < pk).to(torch.int8)
+        G[:, j0:j1] = g
+        del pk, g
+    per = -(-M // cfg["n_chrom"])
+    chrom = (np.arange(M) // per + 1).astype(np.int64)
+    n_kin = min(cfg.get("n_kin", M), M)
+    kin_idx = np.unique(np.linspace(0, M - 1, n_kin).round().astype(np.int64))
+    X = torch.cat([torch.ones(N, 1, device=dev, dtype=torch.float64),
+                   torch.randn(N, C - 1, generator=gen, device=dev, dtype=torch.float64)], dim=1)
+    h2 = torch.rand(P, generator=gen, device=dev, dtype=torch.float64) * (cfg["h2"][1] - cfg["h2"][0]) + cfg["h2"][0]
+    causal = torch.randint(0, M, (P, n_causal), generator=gen, device=dev)
+    Y = torch.empty((N, P), dtype=torch.float64, device=dev)
+    for p0 in range(0, P, 256):
+        p1 = min(P, p0 + 256)
+        Gc = G[:, causal[p0:p1].reshape(-1)].double().reshape(N, p1 - p0, n_causal)
+        Gc = (Gc - Gc.mean(0, keepdim=True)) / Gc.std(0, unbiased=False, keepdim=True).clamp(min=1e-12)
+        g = (Gc * torch.randn(1, p1 - p0, n_causal, generator=gen, device=dev, dtype=torch.float64)).sum(2)
+        g = (g - g.mean(0)) / g.std(0, unbiased=False).clamp(min=1e-12)
+        e = torch.randn(N, p1 - p0, generator=gen, device=dev, dtype=torch.float64)
+        Y[:, p0:p1] = h2[p0:p1].sqrt() * g + (1 - h2[p0:p1]).sqrt() * e
+    return GpuPanel(G=G, chrom=chrom, kin_idx=kin_idx, X=X, Y=Y)
