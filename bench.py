@@ -385,10 +385,11 @@ def lowrank_cpu_check(panel, Gk, g_chrom, Y_all, B, res, rank, prec):
     X = panel.X.cpu().numpy()
     Nn = X.shape[0]
     df = Nn - X.shape[1] - 1
+    G32 = Gk.float()
     for c in (chroms[0], chroms[-1]):
-        keep = torch.from_numpy(np.flatnonzero(g_chrom != c)).to(Gk.device)
+        keep = torch.from_numpy((g_chrom != c).astype(np.float64)).to(Gk.device)
         with engine.configure(precision=prec):
-            b = lowrank.lowrank_basis(Gk.index_select(1, keep), panel.X, rank=rank)
+            b = lowrank._basis(Gk, panel.X, keep, rank=rank, G32=G32)  # the LOCO engine's call, same bits
             UY, UUY = lowrank.rotate_phenos(b, Y_all)
             spec = lowrank.LowRankSpectrum(s=b.s, U_eff=b.U_eff, X=b.X, Xpinv=b.Xpinv, Neff=b.Neff, UY=UY, UUY=UUY)
             ld = lowrank.fit_delta_grid_lowrank(spec)

@@ -3,13 +3,28 @@
 Halko, Martinsson & Tropp (SIAM Review 2011), algorithm 4.4 (randomised subspace iteration) + 5.3 (direct
 eigenvalue decomposition): Y = K Omega for a Gaussian probe Omega (N, r + oversample), n_iter subspace iterations
 Y <- K Y, Q = qr(Y), eigh of the small Q^T K Q, U = Q U_small; the top r pairs are kept.  K is only ever applied to
-a block of vectors -- K = Z Z^T / M as Z (Z^T Y) / M -- so the N x N kernel is never formed.  FP64, rocBLAS GEMMs
-and rocSOLVER's small eigh through torch.
+a block of vectors -- K = Z Z^T / M as Z (Z^T Y) / M -- so the N x N kernel is never formed, and the tall
+orthonormalisations are Gram-based (orthonormal_basis): GEMMs and small eighs, no Householder QR of the N x b
+block (rocSOLVER's takes ~1.5 s at 50,000 x 2,020, the GEMMs ~0.1 s).  FP64 unless stated.
 """
 from __future__ import annotations
 
 import torch
 from torch import Tensor
+
+
+def orthonormal_basis(Y: Tensor, *, passes: int = 2, rtol: float = 1e-13) -> Tensor:
+    """An orthonormal basis (FP64) of span(Y) for a tall Y (N, b): Q = Y V diag(lam)^(-1/2) from the eigh of the
+    small Gram Y^T Y, repeated `passes` times (the first pass leaves an orthogonality error ~ eps cond(Y)^2, the
+    second removes it -- the Gram form of CholeskyQR2, built from GEMMs and one b x b eigh instead of a
+    Householder QR of the tall matrix); directions with Gram eigenvalue <= rtol x the largest are dropped."""
+    Q = Y.double()
+    for _ in range(passes):
+        G = Q.T @ Q
+        lam, V = torch.linalg.eigh(0.5 * (G + G.T))
+        keep = lam > rtol * lam[-1].clamp(min=0.0)
+        Q = Q @ (V[:, keep] / lam[keep].sqrt())
+    return Q
 
 
 def randomized_range(apply_k, N: int, rank: int, *, n_oversample: int = 20, n_iter: int = 2, seed: int = 19930909,
@@ -21,8 +36,8 @@ def randomized_range(apply_k, N: int, rank: int, *, n_oversample: int = 20, n_it
     Y = apply_k(Omega)
     for _ in range(n_iter):
         # re-orthonormalise between iterations (keeps the subspace well conditioned; same span)
-        Y = apply_k(torch.linalg.qr(Y)[0])
-    Q, _ = torch.linalg.qr(Y)
+        Y = apply_k(orthonormal_basis(Y).to(dtype))
+    Q = orthonormal_basis(Y).to(dtype)
     B = Q.T @ apply_k(Q)
     B = 0.5 * (B + B.T)
     s_small, U_small = torch.linalg.eigh(B)

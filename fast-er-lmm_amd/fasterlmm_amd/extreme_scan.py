@@ -16,8 +16,8 @@ from torch import Tensor
 
 from . import engine
 from .core import ScanResult, _dev, _home
-from .lowrank import (LowRankSpectrum, _ColumnPlanes, _scan_blocks, _state, fit_delta_grid_lowrank, lowrank_basis,
-                      rotate_phenos)
+from .lowrank import (LowRankSpectrum, _basis, _ColumnPlanes, _scan_blocks, _state, fit_delta_grid_lowrank,
+                      refresh_real_max, rotate_phenos)
 
 
 def _loco_scan_core(chroms: list, block_source, M: int, G: Tensor, g_chrom: np.ndarray, X: Tensor, Y: Tensor, *,
@@ -30,10 +30,11 @@ def _loco_scan_core(chroms: list, block_source, M: int, G: Tensor, g_chrom: np.n
     n_real = P if n_real is None else min(int(n_real), P)
     dev = Gd.device
     out = engine.ScanOutputs.allocate(M, n_real, P, dev)
+    G32 = Gd.float() if rank is not None else None  # the range finder's iterations run on an FP32 copy
     for k_idx, c in enumerate(chroms):
         with engine.stage("basis"):
-            keep = torch.from_numpy(np.flatnonzero(g_chrom != c)).to(dev)
-            basis = lowrank_basis(Gd.index_select(1, keep), Xd, rank=rank)
+            keep = torch.from_numpy((g_chrom != c).astype(np.float64)).to(dev)
+            basis = _basis(Gd, Xd, keep, rank=rank, G32=G32)
             del keep
         with engine.stage("rotate_columns"):
             UY, UUY = rotate_phenos(basis, Yd)
@@ -50,6 +51,7 @@ def _loco_scan_core(chroms: list, block_source, M: int, G: Tensor, g_chrom: np.n
         del spec, st, cp, UY, UUY, basis
         if on_chrom is not None:
             on_chrom(k_idx + 1, len(chroms))
+    refresh_real_max(out, n_real)
     return ScanResult(out.f, out.beta, out.se, out.sfve, out.nullh2, out.max_F)
 
 

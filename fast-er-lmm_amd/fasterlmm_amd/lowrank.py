@@ -65,44 +65,83 @@ def lowrank_basis(G: Tensor, X: Tensor, *, sv_floor: float = 1e-10, rank: int | 
     """fastlmm's low-rank decomposition (lowrank.py:59-96): G_ = G - X X^+ G; for k_in < N the eigh of the
     k_in x k_in Gram G_^T G_ (U = G_ V / sv, s = sv^2 / k_in, singular values <= sv_floor dropped), else eigh of
     G_ G_^T / k_in.  ``rank`` keeps only the leading `rank` directions, found by the randomised range finder on
-    G_ (extreme.py:23-45; Y = G_ G_^T Omega without forming the kernel), the truncated kinship of the
-    large-cohort path.  Everything on the GPU in FP64; s comes back ascending."""
+    G_ (extreme.py:23-45; K applied as G_ (G_^T .), never formed), the truncated kinship of the large-cohort
+    path.  On the GPU; s comes back ascending."""
     home, (Gd, Xd) = _dev(G, X)
-    N, k_in = Gd.shape
-    D = Xd.shape[1]
-    Xpinv = torch.linalg.pinv(Xd)
+    b = _basis(Gd, Xd, None, sv_floor=sv_floor, rank=rank, n_oversample=n_oversample, n_iter=n_iter, seed=seed)
+    return LowRankBasis(s=b.s.to(home), U_eff=b.U_eff.to(home), X=b.X.to(home), Xpinv=b.Xpinv.to(home),
+                        Neff=b.Neff)
+
+
+def _basis(G: Tensor, X: Tensor, keep: Tensor | None, *, sv_floor: float = 1e-10, rank: int | None = None,
+           n_oversample: int = 20, n_iter: int = 2, seed: int = 19930909, G32: Tensor | None = None) -> LowRankBasis:
+    """lowrank_basis of the columns `keep` (a 0/1 FP64 mask over G's columns, None = all) of the FP64 GPU factor G.
+    With a rank the masked factor is never copied (the mask multiplies G^T Y) and the range finder's subspace
+    iterations run on an FP32 copy of G (G32, made here if not given); the last orthonormalisation and the
+    Rayleigh-Ritz projection are FP64, so U is orthonormal and orthogonal to X to FP64 precision."""
+    N, k_all = G.shape
+    D = X.shape[1]
+    Xpinv = torch.linalg.pinv(X)
+    k_in = k_all if keep is None else int(keep.sum().item())
     if rank is not None and rank < min(N, k_in):
-        s, U = _randomized_factor(Gd, Xd, Xpinv, int(rank), n_oversample, n_iter, seed)
-        keep = s.sqrt() * np.sqrt(k_in) > sv_floor
-        s, U = s[keep], U[:, keep]
+        s, U = _randomized_factor(G, X, Xpinv, keep, k_in, int(rank), n_oversample, n_iter, seed, G32)
+        ok = s.sqrt() * np.sqrt(k_in) > sv_floor
+        s, U = s[ok], U[:, ok]
         order = torch.argsort(s)
-        s, U = s[order], U[:, order].contiguous()
+        return LowRankBasis(s=s[order], U_eff=U[:, order].contiguous(), X=X, Xpinv=Xpinv, Neff=N - D)
+    Gk = G if keep is None else G.index_select(1, torch.nonzero(keep).flatten())
+    G_ = _regress(X, Xpinv, Gk)
+    del Gk
+    if k_in < N:
+        evals, V = _eigh(G_.T @ G_)
+        sv = torch.clamp(evals, min=0.0).sqrt()
+        ok = sv > sv_floor
+        sv, V = sv[ok], V[:, ok]
+        U = G_ @ (V / sv)
+        s = sv * sv / k_in
     else:
-        G_ = _regress(Xd, Xpinv, Gd)
-        if k_in < N:
-            evals, V = _eigh(G_.T @ G_)
-            sv = torch.clamp(evals, min=0.0).sqrt()
-            keep = sv > sv_floor
-            sv, V = sv[keep], V[:, keep]
-            U = G_ @ (V / sv)
-            s = sv * sv / k_in
-        else:
-            evals, Uf = _eigh((G_ @ G_.T) / k_in)
-            keep = torch.clamp(evals, min=0.0) > sv_floor * sv_floor / k_in
-            s = torch.clamp(evals[keep], min=0.0)
-            U = Uf[:, keep].contiguous()
-    return LowRankBasis(s=s.to(home), U_eff=U.to(home), X=Xd.to(home), Xpinv=Xpinv.to(home), Neff=N - D)
+        evals, Uf = _eigh((G_ @ G_.T) / k_in)
+        ok = torch.clamp(evals, min=0.0) > sv_floor * sv_floor / k_in
+        s = torch.clamp(evals[ok], min=0.0)
+        U = Uf[:, ok].contiguous()
+    return LowRankBasis(s=s, U_eff=U, X=X, Xpinv=Xpinv, Neff=N - D)
 
 
-def _randomized_factor(G: Tensor, X: Tensor, Xpinv: Tensor, rank: int, n_oversample: int, n_iter: int, seed: int):
-    """Top-`rank` eigenpairs of K_ = G_ G_^T / k_in, G_ = (I - X X^+) G, by the randomised range finder with
-    n_iter subspace iterations (Halko, Martinsson & Tropp 2011, alg. 4.4 + 5.3; extreme.py:23-45)."""
-    from .extreme import randomized_range
-    N, k_in = G.shape
-    proj = lambda A: _regress(X, Xpinv, A)  # noqa: E731
-    apply_k = lambda Y: proj(G @ (G.T @ proj(Y))) / k_in  # noqa: E731
-    return randomized_range(apply_k, N, rank, n_oversample=n_oversample, n_iter=n_iter, seed=seed,
-                            dtype=G.dtype, device=G.device)
+def _randomized_factor(G: Tensor, X: Tensor, Xpinv: Tensor, keep: Tensor | None, k_in: int, rank: int,
+                       n_oversample: int, n_iter: int, seed: int, G32: Tensor | None):
+    """Top-`rank` eigenpairs of K_ = G_ G_^T / k_in, G_ = (I - X X^+) G[:, keep], by the randomised range finder
+    (Halko, Martinsson & Tropp 2011, alg. 4.4 + 5.3; extreme.py:23-45): Y = K_ Omega and n_iter subspace
+    iterations (re-orthonormalised) in FP32, then an orthonormal basis Q of (I - X X^+) Y and the eigh of
+    Q^T K_ Q in FP64."""
+    N = G.shape[0]
+    target = min(rank + n_oversample, N)
+    G32 = G.float() if G32 is None else G32
+    X32, Xp32 = X.float(), Xpinv.float()
+    m32 = None if keep is None else keep.float().unsqueeze(1)
+
+    def apply32(Y):
+        Z = G32.T @ (Y - X32 @ (Xp32 @ Y))
+        if m32 is not None:
+            Z = Z * m32
+        W = G32 @ Z
+        return (W - X32 @ (Xp32 @ W)) / k_in
+
+    from .extreme import orthonormal_basis
+    gen = torch.Generator(device=G.device).manual_seed(seed)
+    Y = apply32(torch.randn(N, target, generator=gen, device=G.device, dtype=torch.float32))
+    for _ in range(n_iter):
+        Y = apply32(orthonormal_basis(Y).float())
+    Q = orthonormal_basis(_regress(X, Xpinv, Y.double()))
+    del Y
+    Zq = G.T @ Q  # Q is orthogonal to X, so G_^T Q = G^T Q
+    if keep is not None:
+        Zq = Zq * keep.unsqueeze(1)
+    B = (Zq.T @ Zq) / k_in
+    del Zq
+    s_small, U_small = torch.linalg.eigh(0.5 * (B + B.T))
+    s_small = torch.clamp(s_small, min=0.0)
+    idx = torch.argsort(s_small, descending=True)[:rank]
+    return s_small[idx], Q @ U_small[:, idx]
 
 
 def rotate_phenos(basis: LowRankBasis, Y: Tensor) -> tuple[Tensor, Tensor]:
@@ -220,7 +259,8 @@ def pack_concat(prec: int, parts: list[Tensor], rows_pad: int, kpads: list[int])
 
 
 class _ColumnPlanes:
-    """Q1 = [w o UY ; UUY / delta] for one delta fit (the column side of the low-rank scan)."""
+    """Q1 = [w o UY ; UUY / delta] for one delta fit (the column side of the low-rank scan), plus the FP64 pieces
+    the re-scoring of large-F tests needs (w, delta, rWr from the FP64 rotated columns)."""
 
     def __init__(self, st: _LowRankState, spec: LowRankSpectrum, d: engine.DeltaResult):
         _, (s, UY, UUY) = _dev(spec.s, spec.UY, spec.UUY)
@@ -230,6 +270,8 @@ class _ColumnPlanes:
         self.kpad2 = st.kpad
         self.q1 = pack_concat(st.prec, [w * UY, UUY / delta.unsqueeze(0)], st.p_pad, [st.kpad, self.npad])
         self.d = d
+        self.w, self.delta, self.UY, self.UUY = w, delta, UY, UUY
+        self.rWr = torch.clamp((w * UY * UY).sum(dim=0) + (UUY * UUY).sum(dim=0) / delta, min=1e-300)
 
 
 def snp_wald_scan_lowrank(spec: LowRankSpectrum, log_delta: Tensor, S: Tensor, *, snp_chunk: int | None = None,
@@ -246,6 +288,7 @@ def snp_wald_scan_lowrank(spec: LowRankSpectrum, log_delta: Tensor, S: Tensor, *
     cp = _ColumnPlanes(st, spec, d)
     out = engine.ScanOutputs.allocate(M, n_real, st.P, Sd.device)
     _scan_blocks(st, spec, cp, Sd, torch.arange(M, device=Sd.device), n_real, out, snp_chunk or 16384)
+    refresh_real_max(out, n_real)
     return _home(ScanResult(out.f, out.beta, out.se, out.sfve, out.nullh2, out.max_F), home)
 
 
@@ -257,10 +300,53 @@ def _scan_blocks(st: _LowRankState, spec: LowRankSpectrum, cp: _ColumnPlanes, S:
         with engine.stage("snps"):
             S_r = _regress(X, Xp, S[:, b0:b0 + block])
             ri = row_index[b0:b0 + block]
-            s1, s2, mc = snp_planes_lowrank(st, U, S_r)
-            del S_r
+            s1, s2, mc, US, ss_off = snp_planes_lowrank(st, U, S_r)
         with engine.stage("scan"):
             wald_scan_lowrank(s1, s2, cp, mc, st, n_real, ri, out)
+        del s1, s2
+        with engine.stage("rescore"):
+            rescore_lowrank(st, cp, S_r, US, ss_off, ri, n_real, out)
+        del S_r, US, ss_off
+
+
+# Real-column tests whose F exceeds this are recomputed in FP64 after the scan: |d log10 p| grows like
+# F x (relative F error) / (2 ln 10), so the few very large F (causal hits at large N) would otherwise be the
+# first to approach the 1e-3 bar.
+RESCORE_F = 30.0
+
+
+def rescore_lowrank(st: _LowRankState, cp: _ColumnPlanes, S_r: Tensor, US: Tensor, ss_off: Tensor,
+                    row_index: Tensor, n_real: int, out: engine.ScanOutputs, chunk: int = 4096) -> None:
+    """FP64 re-evaluation of the real-column tests of one SNP block with F > RESCORE_F, from the FP64 operands
+    (on-rank US, w, UY; off-rank s_r, UUY, |UUS|^2; rWr): the same Wald arithmetic as the kernel's epilogue
+    (lowrank.py:282-301) on exact FP64 sums."""
+    if n_real == 0 or S_r.shape[1] == 0:
+        return
+    F = out.f.index_select(0, row_index)
+    mi, pi = torch.nonzero(F > RESCORE_F, as_tuple=True)
+    nu = float(st.Neff - 1)
+    fl = 1e-300
+    for c0 in range(0, mi.numel(), chunk):
+        m, p = mi[c0:c0 + chunk], pi[c0:c0 + chunk]
+        delta = cp.delta[p]
+        w = cp.w[:, p]
+        us = US[:, m]
+        a = (us * w * cp.UY[:, p]).sum(dim=0) + (S_r[:, m] * cp.UUY[:, p]).sum(dim=0) / delta
+        d = torch.clamp((us * us * w).sum(dim=0) + ss_off[m] / delta, min=fl)
+        rWr = cp.rWr[p]
+        beta = a / d
+        var = torch.clamp(rWr - a * a / d, min=fl) / nu / d
+        rows = row_index[m]
+        out.f[rows, p] = beta * beta / var
+        out.beta[rows, p] = beta
+        out.se[rows, p] = var.sqrt()
+        out.sfve[rows, p] = torch.clamp(a * a / (d * rWr), min=0.0).sqrt()
+
+
+def refresh_real_max(out: engine.ScanOutputs, n_real: int) -> None:
+    """max_F of the real columns from their (re-scored) per-SNP F."""
+    if n_real > 0 and out.f.shape[0] > 0:
+        out.max_F[:n_real] = out.f.max(dim=0).values
 
 
 def snp_planes_lowrank(st: _LowRankState, U: Tensor, S_r: Tensor):
@@ -273,7 +359,7 @@ def snp_planes_lowrank(st: _LowRankState, U: Tensor, S_r: Tensor):
     ss_off = torch.clamp((S_r * S_r).sum(dim=0) - us2.sum(dim=0), min=0.0)
     s1 = pack_concat(st.prec, [US, S_r], rows_pad, [st.kpad, rup(st.N, TILE)])
     s2 = pack_concat(st.prec, [torch.cat([us2, ss_off.unsqueeze(0)], dim=0)], rows_pad, [st.kpad])
-    return s1, s2, mc
+    return s1, s2, mc, US, ss_off
 
 
 def wald_scan_lowrank(s1: engine.Planes, s2: engine.Planes, cp: _ColumnPlanes, mc: int, st: _LowRankState,
