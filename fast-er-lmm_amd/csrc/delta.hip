@@ -382,15 +382,7 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
     __syncthreads();
   }
   if (p >= p_pad) return;
-  if (p >= P) {  // padding columns: zero operand rows
-    for (int64_t k = lane; k < kpad; k += 64) {
-      const int64_t off = ((k >> 5) * p_pad + p) * 32 + (k & 31);
-      if constexpr (is_split(PREC)) {
-        ((u16*)q1a)[off] = 0; ((u16*)q1b)[off] = 0; ((u16*)q2a)[off] = 0; ((u16*)q2b)[off] = 0;
-      } else {
-        ((float*)q1a)[off] = 0.f; ((float*)q2a)[off] = 0.f;
-      }
-    }
+  if (p >= P) {  // padding columns (their plane rows are zeroed by delta_planes_kernel)
     if (PREC == PREC_SPLIT_FP16 && lane == 0) { q1s[p] = 1.f; q2s[p] = 1.f; }
     return;
   }
@@ -485,14 +477,49 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
     sc1 = pow2_scale(wave_max(m1));
     sc2 = pow2_scale(wave_max(m2));
   }
-  // pass 2: the column-side scan planes Q1 = w o y and Q2 = w (zero on the padding k in [neff, kpad))
-  auto pass2 = [&](int64_t k) {  // re-reads y from HBM: keeping pass 2 rolled holds the register count down
-    double w = 0.0, q1 = 0.0;
-    if (k < neff) {
-      w = rcp_pos(s[k] + delta);
-      q1 = w * (double)uy[((k >> 5) * p_pad + p) * 32 + (k & 31)];
+  // the column-side scan planes Q1 = w o y and Q2 = w are written by delta_planes_kernel from log_delta and the
+  // scales below (the same arithmetic, with every block's reads and writes contiguous)
+  if (lane == 0) {
+    log_delta[p] = t;
+    rWr[p] = fmax(rw, FLOOR64);
+    h2[p] = 1.0 / (1.0 + delta);
+    if constexpr (PREC == PREC_SPLIT_FP16) {
+      q1s[p] = (float)sc1;
+      q2s[p] = (float)sc2;
     }
-    const int64_t off = ((k >> 5) * p_pad + p) * 32 + (k & 31);
+  }
+}
+
+// Q1 = w o y and Q2 = w, w = 1 / (s_k + exp(log_delta[p])), written as the scan's column-side planes (zero on the
+// padding k in [neff, kpad) and on the padding columns p in [P, p_pad)); split-FP16 rows are multiplied by the
+// scales the refinement chose.  A block owns 8 adjacent columns and walks the K chunks: for one chunk its 256
+// elements are contiguous in every plane (element (p, k) at ((k / 32) p_pad + p) 32 + k % 32), so each chunk is
+// one 1 KiB read of UY and four contiguous 512 B (fp32: two 1 KiB) writes.
+template <int PREC>
+__global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restrict__ uy, int64_t P, int64_t p_pad,
+                                                           int64_t kpad, const double* __restrict__ s, int64_t neff,
+                                                           const double* __restrict__ log_delta,
+                                                           const float* __restrict__ q1s,
+                                                           const float* __restrict__ q2s, void* q1a, void* q1b,
+                                                           void* q2a, void* q2b) {
+  const int64_t p = (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);  // p < p_pad (p_pad is a multiple of 8)
+  const int j = threadIdx.x & 31;
+  const bool live = p < P;
+  const double delta = live ? exp(log_delta[p]) : 1.0;
+  double sc1 = 1.0, sc2 = 1.0;
+  if constexpr (PREC == PREC_SPLIT_FP16) {
+    if (live) { sc1 = (double)q1s[p]; sc2 = (double)q2s[p]; }
+  }
+  const int64_t nch = kpad / 32;  // a multiple of 4 (kpad % 128 == 0): four chunks' loads in flight per thread
+#pragma unroll 4
+  for (int64_t c = 0; c < nch; ++c) {
+    const int64_t k = c * 32 + j;
+    const int64_t off = (c * p_pad + p) * 32 + j;
+    double w = 0.0, q1 = 0.0;
+    if (live && k < neff) {
+      w = rcp_pos(s[k] + delta);
+      q1 = w * (double)uy[off];
+    }
     if constexpr (is_split(PREC)) {
       u16 hh, ll;
       split2<PREC>(q1 * sc1, hh, ll);
@@ -504,16 +531,6 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
     } else {
       ((float*)q1a)[off] = (float)q1;
       ((float*)q2a)[off] = (float)w;
-    }
-  };
-  for (int64_t k = lane; k < kpad; k += 64) pass2(k);
-  if (lane == 0) {
-    log_delta[p] = t;
-    rWr[p] = fmax(rw, FLOOR64);
-    h2[p] = 1.0 / (1.0 + delta);
-    if constexpr (PREC == PREC_SPLIT_FP16) {
-      q1s[p] = (float)sc1;
-      q2s[p] = (float)sc2;
     }
   }
 }
@@ -567,6 +584,16 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
   }
 #undef FLMM_REFINE
 #undef FLMM_REFINE1
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const dim3 pgrid((unsigned)(p_pad / 8));
+#define FLMM_PLANES(PR)                                                                                              \
+  hipLaunchKernelGGL((delta_planes_kernel<PR>), pgrid, dim3(256), 0, stream, uy, P, p_pad, kpad, s, neff, log_delta, \
+                     q1s, q2s, q1a, q1b, q2a, q2b)
+  if (prec == PREC_SPLIT_BF16) FLMM_PLANES(PREC_SPLIT_BF16);
+  else if (prec == PREC_SPLIT_FP16) FLMM_PLANES(PREC_SPLIT_FP16);
+  else FLMM_PLANES(PREC_FP32);
+#undef FLMM_PLANES
   return hipGetLastError();
 }
 
