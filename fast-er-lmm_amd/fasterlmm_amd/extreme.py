@@ -14,16 +14,29 @@ from torch import Tensor
 
 
 def orthonormal_basis(Y: Tensor, *, passes: int = 2, rtol: float = 1e-13) -> Tensor:
-    """An orthonormal basis (FP64) of span(Y) for a tall Y (N, b): Q = Y V diag(lam)^(-1/2) from the eigh of the
-    small Gram Y^T Y, repeated `passes` times (the first pass leaves an orthogonality error ~ eps cond(Y)^2, the
-    second removes it -- the Gram form of CholeskyQR2, built from GEMMs and one b x b eigh instead of a
-    Householder QR of the tall matrix); directions with Gram eigenvalue <= rtol x the largest are dropped."""
+    """An orthonormal basis (FP64) of span(Y) for a tall Y (N, b), by CholeskyQR2: Q = Y L^-T with L the Cholesky
+    factor of the small Gram Y^T Y (inverted explicitly, applied as a GEMM), twice (the first pass leaves an orthogonality error ~ eps cond(Y)^2, the second
+    removes it) -- GEMMs, one b x b Cholesky and one triangular solve per pass instead of a Householder QR of the
+    tall matrix.  Where a Gram is numerically singular (Y rank-deficient) that pass takes its eigendecomposition
+    instead and drops the directions with eigenvalue <= rtol x the largest."""
     Q = Y.double()
     for _ in range(passes):
         G = Q.T @ Q
-        lam, V = torch.linalg.eigh(0.5 * (G + G.T))
-        keep = lam > rtol * lam[-1].clamp(min=0.0)
-        Q = Q @ (V[:, keep] / lam[keep].sqrt())
+        G = 0.5 * (G + G.T)
+        L, info = torch.linalg.cholesky_ex(G)
+        lam_min = None
+        if int(info.item()) == 0:
+            d = torch.diagonal(L)
+            lam_min = float((d.min() / d.max()).item()) ** 2  # a cheap lower estimate of the Gram's conditioning
+        if lam_min is not None and lam_min > 1e3 * rtol:
+            # invert the small factor and apply it with one GEMM (a triangular solve against the tall block would
+            # need a workspace of its size)
+            Li = torch.linalg.solve_triangular(L, torch.eye(L.shape[0], dtype=L.dtype, device=L.device), upper=False)
+            Q = Q @ Li.T
+        else:
+            lam, V = torch.linalg.eigh(G)
+            keep = lam > rtol * lam[-1].clamp(min=0.0)
+            Q = Q @ (V[:, keep] / lam[keep].sqrt())
     return Q
 
 
