@@ -264,8 +264,9 @@ class GroupedLoco:
 
     def __init__(self, Z: Tensor, X: Tensor, Ys: list, keeps: list, chrom, *, n_reals: list[int] | None = None,
                  Z_kin: Tensor | None = None, chrom_kin=None, n_grid: int = 64, snp_block: int = 16384,
-                 progress=None, eigh_batch: int = 8):
+                 progress=None, eigh_batch: int = 8, loco: bool = True):
         from . import core
+        self.loco = bool(loco)  # False: one kinship of the whole panel per group, every SNP scanned against it
         self.progress = progress  # progress(chromosome, k) after the k-th basis of a chromosome, if given
         self.eigh_batch = max(1, int(eigh_batch))
         if (Z_kin is None) != (chrom_kin is None):
@@ -323,7 +324,7 @@ class GroupedLoco:
     def _projected_kernel(self, c, g: int) -> Tensor:
         """Group g's projected kinship (I - X X^+)(K_loco + I)(I - X X^+) for chromosome c (core.py:503-510)."""
         N, dev = self.Ns[g], self.dev
-        kc = self.kin_cols.get(c)
+        kc = self.kin_cols.get(c) if self.loco else None
         eye = torch.eye(N, dtype=torch.float64, device=dev)
         if kc is not None and kc.numel():
             zs = self._std_rows(self.Zk, kc, g)
@@ -373,7 +374,9 @@ class GroupedLoco:
         return out
 
     def chromosome_jobs(self, c):
-        tc = torch.from_numpy(np.flatnonzero(self.chrom == c)).to(self.dev)
+        """Jobs for the SNPs of chromosome c (LOCO), or for every SNP when the scan is not LOCO (c is ignored)."""
+        cols = np.flatnonzero(self.chrom == c) if self.loco else np.arange(self.chrom.shape[0])
+        tc = torch.from_numpy(cols).to(self.dev)
         bases = self._bases(c)
         for b0 in range(0, tc.numel(), self.snp_block):
             ri = tc[b0:b0 + self.snp_block]
@@ -397,7 +400,7 @@ class GroupedLoco:
 def loco_scan_compat_groups(Z: Tensor, X: Tensor, Ys: list, keeps: list, chrom, *, n_reals: list[int] | None = None,
                             Z_kin: Tensor | None = None, chrom_kin=None, n_grid: int = 64, mode: str = "grouped",
                             runner: GroupedScan | None = None, snp_block: int = 16384, test_chroms=None,
-                            on_chrom=None) -> list:
+                            on_chrom=None, loco: bool = True) -> list:
     """LOCO compat scans of several strain subsets (see GroupedLoco for the per-group contract), chromosome by
     chromosome, with the scans of all groups for each SNP block run as one grouped launch (mode="grouped",
     through ``runner``: a GroupedScan, by default graph-captured and autotuned) or as one launch pair per group
@@ -406,10 +409,12 @@ def loco_scan_compat_groups(Z: Tensor, X: Tensor, Ys: list, keeps: list, chrom, 
     if mode not in ("grouped", "loop"):
         raise ValueError(f"mode must be 'grouped' or 'loop', got {mode!r}")
     gl = GroupedLoco(Z, X, Ys, keeps, chrom, n_reals=n_reals, Z_kin=Z_kin, chrom_kin=chrom_kin, n_grid=n_grid,
-                     snp_block=snp_block)
+                     snp_block=snp_block, loco=loco)
     if mode == "grouped" and runner is None:
         runner = GroupedScan(gl.prec, gl.G, max(gl.Ps))
     chroms = gl.chroms if test_chroms is None else [c for c in gl.chroms if c in set(test_chroms)]
+    if not loco:
+        chroms = chroms[:1]  # one pass: the single kinship against every SNP
     for ci, c in enumerate(chroms):
         for jobs in gl.chromosome_jobs(c):
             with engine.stage("scan"):

@@ -62,7 +62,8 @@ def perm_threshold(data: AlignedDataset, pheno_idx: list[int], *, n_perm: int = 
 
 
 def perm_threshold_groups(data: AlignedDataset, pheno_idx: list[int], *, n_perm: int = 100, seed: int = 19930909,
-                          mode: str = "grouped", runner=None, on_chrom=None, test_chroms=None) -> list:
+                          mode: str = "grouped", runner=None, on_chrom=None, test_chroms=None,
+                          loco: bool = True) -> list:
     """The reference's NA-grouped permutation run (cli.py:337-375 with perms.py:21-68 per group) with the scans of
     every group fused: phenotypes are grouped by NA pattern (io.group_by_na), each group keeps its observed
     strains and is scanned with its reals and their shuffles (seeded by the full-table column, as the reference's
@@ -70,7 +71,8 @@ def perm_threshold_groups(data: AlignedDataset, pheno_idx: list[int], *, n_perm:
     group (mode="loop").  Groups with fewer than C + 2 strains are skipped, as in the reference.
 
     Returns [(keep_rows, cols, ScanResult of the group's reals, perm_max_F (B_g, n_perm))] per scanned group, equal
-    to ``perm_threshold(subset_dataset(data, keep, cols), range(len(cols)), seed_idx=cols)`` per group."""
+    to ``perm_threshold(subset_dataset(data, keep, cols), range(len(cols)), seed_idx=cols, loco=loco)`` per group
+    (loco=False: one kinship of all SNPs per group)."""
     from .grouped import loco_scan_compat_groups
     from .io import group_by_na
     dev = _lib.device()
@@ -88,7 +90,7 @@ def perm_threshold_groups(data: AlignedDataset, pheno_idx: list[int], *, n_perm:
         n_reals.append(len(cols))
     res = loco_scan_compat_groups(data.Z, data.X, Ys, keeps, data.chrom, n_reals=n_reals, Z_kin=data.Z_kin,
                                   chrom_kin=data.chrom_kin, mode=mode, runner=runner, on_chrom=on_chrom,
-                                  test_chroms=test_chroms)
+                                  test_chroms=test_chroms, loco=loco)
     out = []
     home = data.Y.device
     for (keep, cols), r in zip(groups, res):

@@ -175,19 +175,20 @@ def _na_dataset(example):
     return align_inputs(iid, Z, iid, Y, iid, example["X_cov2"][:, 1:], chrom=example["chrom"]), Z, Y
 
 
-@pytest.mark.parametrize("mode", ["grouped", "loop"])
-def test_perm_threshold_groups_vs_oracle(example, mode):
+@pytest.mark.parametrize("mode,loco", [("grouped", True), ("loop", True), ("grouped", False)])
+def test_perm_threshold_groups_vs_oracle(example, mode, loco):
     from fasterlmm_amd.perms import perm_threshold_groups
     data, Z, Y = _na_dataset(example)
     pheno = list(range(18))
-    res = perm_threshold_groups(data, pheno, n_perm=3, seed=19930909, mode=mode)
+    res = perm_threshold_groups(data, pheno, n_perm=3, seed=19930909, mode=mode, loco=loco)
     groups = lmm.group_by_na(Y, pheno)
     assert [(k, c) for k, c, _, _ in res] == [(list(k), list(c)) for k, c in groups]
     assert len(res) == 4
     X = data.X.cpu().numpy()
     for keep, cols, r, pmf in res:
         ref, ref_pmf = operms.perm_threshold(Z[keep], X[keep], Y[keep][:, cols], example["chrom"],
-                                             list(range(len(cols))), n_perm=3, seed=19930909, seed_idx=cols)
+                                             list(range(len(cols))), n_perm=3, seed=19930909, seed_idx=cols,
+                                             loco=loco)
         df = len(keep) - X.shape[1] - 1
         check_scan(r, ref, df, what=f"group of {len(keep)} strains")
         np.testing.assert_allclose(pmf.cpu().numpy(), ref_pmf, rtol=1e-4)
