@@ -96,6 +96,50 @@ def run_step(Z, X, Y_all, chrom, B, precision):
         return core.loco_scan_compat(Z, X, Y_all, chrom, n_real=B, n_grid=100)
 
 
+def compat_cpu_check(panel, res, B, n_top=50, frac=0.01):
+    """Full-size parity of the GPU result against the FP64 oracle (numpy, host cores): chromosome 1's LOCO
+    kinship, projected eigh, rotation and per-column delta (100-point grid + 50 golden-section steps) for the B
+    real phenotypes, then the Wald tests of a 1% SNP sample of chromosome 1 plus its n_top SNPs of largest GPU F,
+    against every real phenotype.  Reports max |d log10 p| (tests whose p is below the double range in scipy's
+    tail are compared by F), max |d beta| / max(|beta|, se), and the GPU's log delta against the oracle's."""
+    import numpy as np
+    import torch
+    from scipy import stats
+    from oracle import lmm
+    t0 = time.perf_counter()
+    Zs = lmm.standardise_columns(panel.G.astype(np.float64))
+    chrom = panel.chrom
+    c = sorted(np.unique(chrom))[0]
+    test = np.flatnonzero(chrom == c)
+    K = lmm.grm(Zs[:, chrom != c])
+    spec = lmm.compat_rotate(K, panel.X, panel.Y[:, :B])
+    ld = lmm.fit_delta_grid_compat(spec, n_grid=100)
+    f_gpu = res.f.index_select(0, torch.from_numpy(test).to(res.f.device)).cpu().numpy()
+    rng = np.random.default_rng(7)
+    top = np.argsort(-f_gpu.max(axis=1))[:n_top]
+    pick = np.unique(np.concatenate([rng.choice(test.size, max(1, int(frac * test.size)), replace=False), top]))
+    rows = test[pick]
+    ref = lmm.snp_wald_scan_compat(spec, ld, Zs[:, rows], n_real=B)
+    df = spec["s"].shape[0] - 1
+    ri = torch.from_numpy(rows).to(res.f.device)
+    got = {k: getattr(res, k).index_select(0, ri).cpu().numpy() for k in ("f", "beta", "se", "nullh2")}
+    lp = lambda F: -stats.f.logsf(F, 1, df) / np.log(10.0)  # noqa: E731
+    a, b = lp(got["f"]), lp(ref["f"])
+    fin = np.isfinite(a) & np.isfinite(b)
+    dlp = float(np.abs(a - b)[fin].max())
+    dbeta = float((np.abs(got["beta"] - ref["beta"]) / np.maximum(np.abs(ref["beta"]), ref["se"])).max())
+    ld_gpu = np.log(1.0 / got["nullh2"][0] - 1.0)
+    dld = np.abs(ld_gpu - ld)
+    worst = int(np.argmax(np.abs(a - b).max(axis=0) * fin.all(axis=0)))
+    return {"sample": f"chromosome {c}: {rows.size} SNPs ({frac:.0%} random + {n_top} of largest GPU F) x {B} real "
+                      f"phenotypes = {rows.size * B} tests; FP64 oracle LOCO pipeline, {time.perf_counter() - t0:.1f}s",
+            "max_abs_dlog10p": dlp, "max_dbeta_rel": dbeta, "max_F_oracle": float(ref["f"].max()),
+            "max_abs_dlog_delta": float(dld.max()), "columns_dlog_delta_gt_1e-3": int((dld > 1e-3).sum()),
+            "worst_column": {"col": worst, "log_delta_gpu": float(ld_gpu[worst]), "log_delta_oracle": float(ld[worst]),
+                             "dlog10p": float(np.abs(a - b)[:, worst].max())},
+            "not_finite_in_scipy": int((~fin).sum())}
+
+
 def cpu_baseline(panel, cfg, n_perm, budget_s):
     """FP64 oracle (numpy, all host cores) on a bounded sample of the same workload: the first chromosome's
     SNPs against a slice of the scan columns, full compat pipeline (LOCO kinship, projected eigh, rotation,
@@ -129,7 +173,8 @@ def cpu_baseline(panel, cfg, n_perm, budget_s):
 def traffic_per_launch(prec, scan):
     """HBM bytes per scan launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload,
     when one exists for this precision and launch count (else None)."""
-    f = ROOT / "profiles" / "wald_scan_wide_traffic.json"
+    name = "wald_scan_factored_traffic.json" if scan["factored"] else "wald_scan_wide_traffic.json"
+    f = ROOT / "profiles" / name
     if not f.exists():
         return None, None
     d = json.loads(f.read_text())
@@ -137,7 +182,7 @@ def traffic_per_launch(prec, scan):
     if not e or e.get("tests_per_launch") != round(scan["tests"] / max(scan["launches"], 1)):
         return None, None
     return e["bytes_per_launch"], {"fetch_bytes": e["fetch_bytes"], "write_bytes": e["write_bytes"],
-                                   "source": "profiles/wald_scan_wide_traffic.json"}
+                                   "source": f"profiles/{name}"}
 
 
 def timed_run(Z, X, Y_all, chrom, B, precision, steps):
@@ -215,7 +260,9 @@ def run_grouped(args):
     engine.stage_timer.reset(False)
     log(f"[bench3] chromosome {c}: bases, delta fits and SNP planes of {len(jobs)} groups in {t_prep:.1f}s {stages}")
     tests = sum(j.mc * j.P for j in jobs)
-    alg_flops = sum(4.0 * j.d.neff * j.mc * j.P for j in jobs)
+    # the launched algorithm: sWy over Neff, sWs over Neff (exact weights) or the R-term exponential sum (factored)
+    alg_flops = sum(2.0 * (j.d.neff + (j.d.fac.R if j.factored else j.d.neff)) * j.mc * j.P for j in jobs)
+    ref_flops = sum(4.0 * j.d.neff * j.mc * j.P for j in jobs)
     runner = grouped.GroupedScan(prec, len(jobs), max(j.P for j in jobs))
     t0 = time.perf_counter()
     runner.run(jobs)  # autotune + graph capture
@@ -258,14 +305,16 @@ def run_grouped(args):
                                f"x (1+{n_perm} perms), C={cfg['C']}; step = chromosome {c} ({jobs[0].mc} SNPs) "
                                f"against every group's columns", "tests_per_step": tests,
                    "parallelism": "single GPU"},
-        "grouped": {"variant": "wide 128x256" if variant == 1 else "square 128x128", "blocks": blocks,
+        "grouped": {"variant": {0: "square 128x128", 1: "wide 128x256",
+                                2: "wide 128x256, factored weights"}[variant], "blocks": blocks,
                     "graph_replay_ms": 1000 * t_grouped, "eager_ms": 1000 * t_eager,
                     "per_group_loop_ms": 1000 * t_loop, "speedup_vs_loop": t_loop / t_grouped,
                     "identical_to_first_run": same, "autotune_s": t_tune,
                     "autotune_trials_ms": [(round(ms, 3), v, b) for ms, v, b in runner.tuner.trials.get(key, [])]},
         "roofline": {"bound": "mfma", "achieved": round(alg, 2), "peak": round(published / products, 1),
                      "unit": "TFLOP/s", "frac": round(alg * products / published, 4), "traffic": None,
-                     "kernel": "wald_scan_grouped_kernel", "mfma_issued_tflops": round(products * alg, 2)},
+                     "kernel": "wald_scan_grouped_kernel", "mfma_issued_tflops": round(products * alg, 2),
+                     "reference_algorithm_tflops": round(ref_flops / t_grouped / 1e12, 2)},
         "pipeline_s": {"kinship_and_columns_all_groups": t_setup, "chromosome_bases_deltas_snp_planes": t_prep,
                        "stages_ms": stages},
         "cpu_baseline": None,
@@ -466,8 +515,8 @@ def main():
     value = tests_per_step * args.steps / dt
     log(f"[bench] {prec}: {ms_per_step:.1f} ms/step, {value:.4e} tests/s, scan kernels {scan}")
 
-    def tflops(sc):
-        return sc["alg_flops"] / (sc["total_ms"] * 1e-3) / 1e12 if sc["total_ms"] else 0.0
+    def tflops(sc, key="alg_flops"):
+        return sc[key] / (sc["total_ms"] * 1e-3) / 1e12 if sc["total_ms"] else 0.0
 
     split = prec != "fp32"
     alg = tflops(scan)
@@ -475,15 +524,22 @@ def main():
     traffic, traffic_detail = traffic_per_launch(prec, scan)
     published = MI355X_PEAK_BF16_TFLOPS if split else MI355X_PEAK_FP32_TFLOPS
     peak = published / products
+    kernel = (f"wald_scan_lowrank_kernel<{prec}> (factored weights)" if scan["factored"]
+              else f"wald_scan_wide_kernel<{prec}>")
     roofline = {"bound": "mfma", "achieved": round(alg, 2), "peak": round(peak, 1), "unit": "TFLOP/s",
                 "frac": round(alg / peak, 4), "traffic": traffic, "traffic_detail": traffic_detail,
-                "kernel": f"wald_scan_wide_kernel<{prec}>", "mean_launch_ms": round(scan["mean_ms"], 4),
+                "kernel": kernel, "mean_launch_ms": round(scan["mean_ms"], 4),
                 "launches_per_step": scan["launches"] // args.steps,
-                "published_peak_tflops": published, "mfma_issued_tflops": round(products * alg, 2),
-                "mfma_utilisation": round(products * alg / published, 4),
-                "note": "achieved = 4*(N-C) algorithmic flop per Wald test x tests per launch / mean kernel time "
-                        "(HIP events on the launch stream); peak = published dense MFMA peak / products per "
-                        "split product (3 for split fp16/bf16), so frac = issued-MFMA utilisation"}
+                "published_peak_tflops": published,
+                "mfma_issued_tflops": round(products * tflops(scan, "kernel_flops"), 2),
+                "mfma_utilisation": round(products * tflops(scan, "kernel_flops") / published, 4),
+                "reference_algorithm_tflops": round(tflops(scan, "ref_flops"), 2),
+                "factored_terms": scan["R"],
+                "note": "achieved = the launched algorithm's flops per Wald test (sWy over Neff = N-C: 2 Neff; sWs "
+                        "over the R-term exponential sum of the factored weights: 2 R; exact weights: 2 Neff) x "
+                        "tests per launch / mean kernel time (HIP events on the launch stream); peak = published "
+                        "dense MFMA peak / 3 products per split product, so frac = MFMA utilisation; "
+                        "reference_algorithm_tflops counts the reference's 4 Neff per test over the same time"}
 
     extra = {}
     from fasterlmm_amd.pvalues import neglog10p
@@ -494,6 +550,7 @@ def main():
         dlp = float((neglog10p(r2.f, df) - lp_head).abs().max())
         a2 = tflops(s2)
         pk = MI355X_PEAK_BF16_TFLOPS / 3.0 if other != "fp32" else MI355X_PEAK_FP32_TFLOPS
+        # factored or exact as the headline's weights setting chooses for that precision (fp32: exact)
         extra[f"{other}_path"] = {"value": rate, "ms_per_step": 1000 * dt2, "scan_kernel_ms": s2["total_ms"],
                                   "scan_tflops": round(a2, 2), "scan_frac_of_peak": round(a2 / pk, 4),
                                   "max_abs_dlog10p_vs_headline": dlp,
@@ -508,6 +565,9 @@ def main():
         t2 = time.perf_counter()
         cpu = cpu_baseline(panel, cfg, n_perm, args.cpu_seconds)
         log(f"[bench] cpu baseline {cpu['value']:.4e} tests/s in {time.perf_counter() - t2:.1f}s")
+        if args.config == 2:
+            cpu["gpu_vs_oracle"] = compat_cpu_check(panel, res, B)
+            log(f"[bench] full-size parity vs the FP64 oracle: {cpu['gpu_vs_oracle']}")
 
     desc = {"split_fp16": "split-FP16 (power-of-two-scaled fp16 hi+lo, 3 MFMA products, FP32 accumulate)",
             "split_bf16": "split-BF16 (bf16 hi+lo, 3 MFMA products, FP32 accumulate)",

@@ -41,7 +41,7 @@ static bool two_planes(int p) { return p != FLMM_PREC_FP32; }
 
 extern "C" {
 
-int flmm_abi_version(void) { return 5; }
+int flmm_abi_version(void) { return 6; }
 
 const char* flmm_last_error(void) { return g_err.c_str(); }
 
@@ -88,7 +88,8 @@ int flmm_rotate(int prec, int mode, const void* a_hi, const void* a_lo, const fl
                 const void* b_hi, const void* b_lo, const float* b_scale, int64_t b_rows_pad, int64_t n_pad,
                 void* out0, void* out1, void* out2, void* out3, float* out_scale1, float* out_scale2, void* stream) {
   REQUIRE(valid_prec(prec), "flmm_rotate: bad precision %d", prec);
-  REQUIRE(mode == FLMM_ROTATE_UY || mode == FLMM_ROTATE_US, "flmm_rotate: bad mode %d", mode);
+  REQUIRE(mode == FLMM_ROTATE_UY || mode == FLMM_ROTATE_US || mode == FLMM_ROTATE_US_X2, "flmm_rotate: bad mode %d",
+          mode);
   REQUIRE(a_rows_pad > 0 && b_rows_pad > 0 && n_pad > 0, "flmm_rotate: empty operand");
   REQUIRE(a_rows_pad % TILE == 0 && b_rows_pad % TILE == 0 && n_pad % KCHUNK == 0,
           "flmm_rotate: rows_pad %% 128 and n_pad %% 32 required");
@@ -96,9 +97,13 @@ int flmm_rotate(int prec, int mode, const void* a_hi, const void* a_lo, const fl
   REQUIRE(a_hi && b_hi && (!two || (a_lo && b_lo)), "flmm_rotate: null operand plane");
   REQUIRE(aligned16(a_hi) && aligned16(b_hi) && (!two || (aligned16(a_lo) && aligned16(b_lo))),
           "flmm_rotate: operand planes must be 16-byte aligned");
-  REQUIRE(out0 && (mode == FLMM_ROTATE_UY || (out1 && (!two || (out2 && out3)))), "flmm_rotate: null output");
+  if (mode == FLMM_ROTATE_US_X2)
+    REQUIRE(out0 && out2 && (!two || out1) && aligned16(out2), "flmm_rotate: null or misaligned output");
+  else
+    REQUIRE(out0 && (mode == FLMM_ROTATE_UY || (out1 && (!two || (out2 && out3)))), "flmm_rotate: null output");
   REQUIRE(prec != FLMM_PREC_SPLIT_FP16 ||
-              (a_scale && b_scale && (mode == FLMM_ROTATE_UY || (out_scale1 && out_scale2))),
+              (a_scale && b_scale &&
+               (mode == FLMM_ROTATE_UY || (out_scale1 && (mode == FLMM_ROTATE_US_X2 || out_scale2)))),
           "flmm_rotate: split-FP16 needs the operand and output row scales");
   const int nch = (int)(n_pad / KCHUNK);
   hipError_t e;
@@ -125,12 +130,12 @@ int flmm_rotate(int prec, int mode, const void* a_hi, const void* a_lo, const fl
 }
 
 int flmm_delta_weights(const double* s, int64_t neff, const double* s_lam, int64_t nlam, int64_t kpad, int n_grid,
-                       double lo, double hi, float* wplane, float* lam, void* stream) {
+                       double lo, double hi, float* wplane, float* lam, double* lam64, void* stream) {
   REQUIRE(n_grid >= 1 && n_grid <= TILE, "flmm_delta_weights: n_grid must be in [1, 128], got %d", n_grid);
   REQUIRE(neff >= 1 && neff <= kpad && kpad % TILE == 0 && nlam >= 1, "flmm_delta_weights: bad neff / nlam / kpad");
   REQUIRE(s && s_lam && wplane && lam, "flmm_delta_weights: null pointer");
   return check_launch(
-      launch_delta_weights(s, neff, s_lam, nlam, kpad, n_grid, lo, hi, wplane, lam, (hipStream_t)stream),
+      launch_delta_weights(s, neff, s_lam, nlam, kpad, n_grid, lo, hi, wplane, lam, lam64, (hipStream_t)stream),
       "flmm_delta_weights");
 }
 
@@ -156,16 +161,19 @@ static int check_refine(const char* who, int prec, const float* uy, int64_t P, i
           "%s: bad sizes", who);
   REQUIRE(nco > 0.0, "%s: loss coefficient must be positive", who);
   const bool two = two_planes(prec);
-  REQUIRE(uy && s && s_lam && log_delta && rWr && h2 && q1_hi && q2_hi && (!two || (q1_lo && q2_lo)),
+  // q2_hi == NULL: the Q2 planes are not written (factored weights: flmm_factored_weight_planes writes them)
+  REQUIRE(uy && s && s_lam && log_delta && rWr && h2 && q1_hi && (!two || q1_lo) && (!q2_hi || !two || q2_lo),
           "%s: null pointer", who);
-  REQUIRE(prec != FLMM_PREC_SPLIT_FP16 || (q1_scale && q2_scale), "%s: split-FP16 needs q1_scale / q2_scale", who);
+  REQUIRE(prec != FLMM_PREC_SPLIT_FP16 || (q1_scale && (!q2_hi || q2_scale)),
+          "%s: split-FP16 needs q1_scale / q2_scale", who);
   return FLMM_OK;
 }
 
 int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_pad, int64_t kpad, const double* s,
                       int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo, double hi,
-                      const int32_t* idx, double* log_delta, double* rWr, double* h2, void* q1_hi, void* q1_lo,
-                      void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale, void* stream) {
+                      const int32_t* idx, const double* lam64, double* log_delta, double* rWr, double* h2,
+                      void* q1_hi, void* q1_lo, void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale,
+                      void* stream) {
   REQUIRE(mode == FLMM_DELTA_GOLDEN || mode == FLMM_DELTA_NEWTON || mode == FLMM_DELTA_FIXED ||
               mode == FLMM_DELTA_GRID,
           "flmm_delta_refine: bad mode %d", mode);
@@ -175,24 +183,62 @@ int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_
   if (rc) return rc;
   if (p_pad == 0) return FLMM_OK;
   return check_launch(launch_delta_refine(prec, mode, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco, n_grid, lo, hi,
-                                          idx, log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, q1_scale, q2_scale,
-                                          0.0, 0.0, nullptr, (hipStream_t)stream),
+                                          idx, lam64, log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, q1_scale,
+                                          q2_scale, 0.0, 0.0, nullptr, (hipStream_t)stream),
                       "flmm_delta_refine");
+}
+
+int flmm_gemm_f64(const double* a, int64_t rows, int64_t k, const double* b, int64_t cols, double* c, void* stream) {
+  REQUIRE(rows >= 0 && rows % 64 == 0 && k > 0 && k % KCHUNK == 0 && cols > 0 && cols % TILE == 0,
+          "flmm_gemm_f64: need rows %% 64, k %% 32 and cols %% 128 == 0");
+  REQUIRE(a && b && c, "flmm_gemm_f64: null pointer");
+  if (rows == 0) return FLMM_OK;
+  return check_launch(launch_gemm_f64(a, rows, k, b, cols, c, (hipStream_t)stream), "flmm_gemm_f64");
+}
+
+int flmm_rescore_f64(const double* US, const double* UY, int64_t ld, const double* s, int64_t neff,
+                     const double* log_delta, const int64_t* us_row, const int64_t* col, const int64_t* out_row,
+                     int64_t n, double nu, double* f, double* beta, double* se, double* sfve, int64_t out_ld,
+                     void* stream) {
+  REQUIRE(n >= 0 && neff >= 1 && neff <= ld && out_ld >= 1 && nu > 0.0, "flmm_rescore_f64: bad sizes");
+  REQUIRE(n == 0 || (US && UY && s && log_delta && us_row && col && out_row && f && beta && se && sfve),
+          "flmm_rescore_f64: null pointer");
+  if (n == 0) return FLMM_OK;
+  return check_launch(launch_rescore(US, UY, ld, s, neff, log_delta, us_row, col, out_row, n, nu, f, beta, se, sfve,
+                                     out_ld, (hipStream_t)stream),
+                      "flmm_rescore_f64");
+}
+
+int flmm_factored_weight_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
+                                int64_t R, const double* log_delta, void* q2_hi, void* q2_lo, float* q2_scale,
+                                void* stream) {
+  REQUIRE(valid_prec(prec), "flmm_factored_weight_planes: bad precision %d", prec);
+  REQUIRE(P >= 0 && P <= p_pad && p_pad % TILE == 0, "flmm_factored_weight_planes: bad column extent");
+  REQUIRE(rpad > 0 && rpad % TILE == 0 && R >= 1 && R <= rpad,
+          "flmm_factored_weight_planes: need 1 <= R <= rpad, rpad a multiple of 128");
+  const bool two = two_planes(prec);
+  REQUIRE(t && lam && log_delta && q2_hi && (!two || q2_lo), "flmm_factored_weight_planes: null pointer");
+  REQUIRE(aligned16(q2_hi) && (!two || aligned16(q2_lo)), "flmm_factored_weight_planes: planes must be 16-byte aligned");
+  REQUIRE(prec != FLMM_PREC_SPLIT_FP16 || q2_scale, "flmm_factored_weight_planes: split-FP16 needs q2_scale");
+  if (p_pad == 0) return FLMM_OK;
+  return check_launch(launch_factored_planes(prec, P, p_pad, rpad, t, lam, R, log_delta, q2_hi, q2_lo, q2_scale,
+                                             (hipStream_t)stream),
+                      "flmm_factored_weight_planes");
 }
 
 int flmm_delta_refine_learned(int prec, const float* uy, int64_t P, int64_t p_pad, int64_t kpad, const double* s,
                               int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo,
-                              double hi, const int32_t* idx, double radius, double judge_tol, double* log_delta,
-                              int32_t* fallback, double* rWr, double* h2, void* q1_hi, void* q1_lo, void* q2_hi,
-                              void* q2_lo, float* q1_scale, float* q2_scale, void* stream) {
+                              double hi, const int32_t* idx, const double* lam64, double radius, double judge_tol,
+                              double* log_delta, int32_t* fallback, double* rWr, double* h2, void* q1_hi,
+                              void* q1_lo, void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale, void* stream) {
   REQUIRE(radius > 0.0 && judge_tol >= 0.0 && idx, "flmm_delta_refine_learned: bad radius / tol / idx");
   const int rc = check_refine("flmm_delta_refine_learned", prec, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco,
                               n_grid, log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, q1_scale, q2_scale);
   if (rc) return rc;
   if (p_pad == 0) return FLMM_OK;
   return check_launch(launch_delta_refine(prec, 3, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco, n_grid, lo, hi, idx,
-                                          log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, q1_scale, q2_scale, radius,
-                                          judge_tol, fallback, (hipStream_t)stream),
+                                          lam64, log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, q1_scale, q2_scale,
+                                          radius, judge_tol, fallback, (hipStream_t)stream),
                       "flmm_delta_refine_learned");
 }
 
@@ -342,11 +388,12 @@ int64_t flmm_scan_table_bytes(int n_groups) {
 
 int flmm_scan_table_fill(int prec, int variant, const flmm_scan_group* groups, int n_groups, void* host_table) {
   REQUIRE(valid_prec(prec), "flmm_scan_table_fill: bad precision %d", prec);
-  REQUIRE(variant == FLMM_SCAN_SQUARE || variant == FLMM_SCAN_WIDE, "flmm_scan_table_fill: bad variant %d", variant);
+  REQUIRE(variant == FLMM_SCAN_SQUARE || variant == FLMM_SCAN_WIDE || variant == FLMM_SCAN_WIDE_FACTORED,
+          "flmm_scan_table_fill: bad variant %d", variant);
   REQUIRE(n_groups >= 0 && n_groups <= 65535, "flmm_scan_table_fill: n_groups must be in [0, 65535]");
   REQUIRE(host_table && (groups || n_groups == 0), "flmm_scan_table_fill: null pointer");
   const bool two = two_planes(prec);
-  const int64_t tp = variant == FLMM_SCAN_WIDE ? 256 : TILE;
+  const int64_t tp = variant == FLMM_SCAN_SQUARE ? TILE : 256;
   int64_t* hdr = (int64_t*)host_table;
   int64_t total = 0;
   for (int g = 0; g < n_groups; ++g) {
@@ -356,6 +403,11 @@ int flmm_scan_table_fill(int prec, int variant, const flmm_scan_group* groups, i
             "flmm_scan_table_fill: group %d: bad column extent (p_pad %% %lld required)", g, (long long)tp);
     REQUIRE(j.kpad > 0 && j.kpad % TILE == 0, "flmm_scan_table_fill: group %d: kpad must be a positive multiple of 128",
             g);
+    if (variant == FLMM_SCAN_WIDE_FACTORED)
+      REQUIRE(j.kpad2 > 0 && j.kpad2 % TILE == 0 && j.kpad2 < j.kpad,
+              "flmm_scan_table_fill: group %d: the factored variant needs kpad2 a multiple of 128 in (0, kpad)", g);
+    else
+      REQUIRE(j.kpad2 == 0, "flmm_scan_table_fill: group %d: kpad2 > 0 needs the factored variant", g);
     REQUIRE(j.n_real >= 0 && j.n_real <= j.P, "flmm_scan_table_fill: group %d: n_real must be in [0, P]", g);
     REQUIRE(j.nu > 0.0, "flmm_scan_table_fill: group %d: dof must be positive", g);
     REQUIRE(j.s1_hi && j.s2_hi && j.q1_hi && j.q2_hi && (!two || (j.s1_lo && j.s2_lo && j.q1_lo && j.q2_lo)),
@@ -383,8 +435,8 @@ int flmm_scan_table_fill(int prec, int variant, const flmm_scan_group* groups, i
 
 int flmm_wald_scan_grouped(int prec, int variant, const void* table, int blocks, void* stream) {
   REQUIRE(valid_prec(prec), "flmm_wald_scan_grouped: bad precision %d", prec);
-  REQUIRE(variant == FLMM_SCAN_SQUARE || variant == FLMM_SCAN_WIDE, "flmm_wald_scan_grouped: bad variant %d",
-          variant);
+  REQUIRE(variant == FLMM_SCAN_SQUARE || variant == FLMM_SCAN_WIDE || variant == FLMM_SCAN_WIDE_FACTORED,
+          "flmm_wald_scan_grouped: bad variant %d", variant);
   REQUIRE(table && aligned16(table), "flmm_wald_scan_grouped: table must be a 16-byte aligned device pointer");
   REQUIRE(blocks > 0 && blocks % 8 == 0, "flmm_wald_scan_grouped: blocks must be a positive multiple of 8, got %d",
           blocks);

@@ -11,7 +11,8 @@
 // column independent and computed once.  The epilogue takes the per-column argmin (first index on ties, as
 // torch.argmin does, core.py:571).
 //
-// Stage 2, refinement, one wave per column, FP64:
+// Stage 2, refinement, one wave per column, FP64.  Every search mode first walks from the FP32 grid argmin to the
+// FP64 grid minimum (fp64_grid_argmin), so it brackets what the reference's FP64 argmin brackets.
 //   MODE_GOLDEN: the reference algorithm step for step -- bracket [g[i-1], g[i+1]], 50 golden-section
 //                iterations, (a+b)/2 clamped (core.py:577-595).
 //   MODE_FIXED:  no search; the caller's log delta is used as given.
@@ -25,7 +26,8 @@
 //                the bracket.  A bracket whose left slope is >= 0 (right slope <= 0) has its minimum at that
 //                edge, which is where golden section converges too.
 // The refinement then writes the scan's column-side operand planes Q1 = w o y and Q2 = w for the final delta
-// (split-FP16: each column scaled by the power of two of its maximum, recorded in q1s / q2s),
+// (split-FP16: each column scaled by the power of two of its maximum, recorded in q1s / q2s; Q2 skipped for
+// factored weights, whose planes factored_planes_kernel writes),
 // the null residual rWr = max(sum w y^2, floor) (core.py:641-643) and nullh2 = 1 / (1 + delta).
 #include "kernels.h"
 
@@ -43,7 +45,8 @@ __device__ inline double grid_value(int i, int G, double lo, double hi) {
 // lam[g] = sum_k log(s_lam_k + delta_g) over the log-det spectrum.
 __global__ void delta_weights_kernel(const double* __restrict__ s, int64_t neff, const double* __restrict__ s_lam,
                                      int64_t nlam, int64_t kpad, int G, double lo, double hi,
-                                     float* __restrict__ wplane, float* __restrict__ lam) {
+                                     float* __restrict__ wplane, float* __restrict__ lam,
+                                     double* __restrict__ lam64) {
   __shared__ double red[256];
   const int g = blockIdx.x;
   const double delta = exp(grid_value(g, G, lo, hi));
@@ -61,7 +64,10 @@ __global__ void delta_weights_kernel(const double* __restrict__ s, int64_t neff,
     if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
     __syncthreads();
   }
-  if (threadIdx.x == 0) lam[g] = (float)red[0];
+  if (threadIdx.x == 0) {
+    lam[g] = (float)red[0];
+    if (lam64) lam64[g] = red[0];
+  }
 }
 
 // argmin over the grid rows of one 128-column tile
@@ -200,6 +206,24 @@ struct ColLik {
     return c;
   }
 
+  // R = sum_k y_k^2 / (s_k + delta) alone
+  __device__ double R_only(int64_t p, double delta, int lane) const {
+    double r = 0.0;
+    if constexpr (KPL > 0) {
+#pragma unroll
+      for (int i = 0; i < KPL; ++i) {
+        const double y = (double)yr[i];
+        r += y * y * rcp_pos(ls[lane + 64 * i] + delta);
+      }
+    } else {
+      for (int64_t k = lane; k < neff; k += 64) {
+        const double y = (double)uy[((k >> 5) * p_pad + p) * 32 + (k & 31)];
+        r += y * y * rcp_pos(s[k] + delta);
+      }
+    }
+    return wave_sum(r);
+  }
+
   // FP32 slope L'(t) and curvature L''(t) (register path only): a cheap first approximation of the root
   __device__ void slope32(float t, int lane, float& g, float& gp) const {
     const float delta = __expf(t);
@@ -257,23 +281,36 @@ __device__ inline void col_slope(const LK& L, int64_t p, double t, int lane, dou
 }
 
 // The grid argmin handed in comes from the FP32 grid kernel; where neighbouring grid losses differ by less than
-// FP32 resolves (flat likelihoods, h2 near 0) it can sit a node away from the FP64 argmin the reference brackets
-// (core.py:570-575).  Walk from it to the FP64 local minimum of the grid: left while the left neighbour is not
-// higher (ties go to the first index, as argmin's do), right while the right neighbour is lower.
+// FP32 resolves (flat likelihoods, h2 near 0) it can sit one or many nodes away from the FP64 argmin the reference
+// brackets (core.py:570-575) -- on a profile that falls monotonically to the grid's edge by less than FP32 resolves,
+// the FP32 argmin is the first of a run of equal values.  Walk from it to the FP64 local minimum of the grid: left
+// while the left neighbour is not higher (ties go to the first index, as argmin's do), right while the right
+// neighbour is lower; one FP64 loss per step, and only flat columns take more than one or two.
+// lam64: the column-independent log-det term at every grid node in FP64 (from the weights kernel), so a step costs
+// one R sum and one log; NULL: each node's log-det is summed per column.
 template <class LK>
-__device__ inline int fp64_grid_argmin(const LK& L, int64_t p, int i, int G, double lo, double hi, int lane) {
-  double li = col_loss(L, p, grid_value(i, G, lo, hi), lane);
+__device__ inline double node_loss(const LK& L, int64_t p, int g, int G, double lo, double hi,
+                                   const double* __restrict__ lam64, int lane) {
+  const double t = grid_value(g, G, lo, hi);
+  if (!lam64) return col_loss(L, p, t, lane);
+  return L.nco * (LOG2PI + log(fmax(L.R_only(p, exp(t), lane), FLOOR64) / L.nco) + 1.0) + lam64[g];
+}
+
+template <class LK>
+__device__ inline int fp64_grid_argmin(const LK& L, int64_t p, int i, int G, double lo, double hi,
+                                       const double* __restrict__ lam64, int lane) {
+  double li = node_loss(L, p, i, G, lo, hi, lam64, lane);
   bool moved_left = false;
-  for (int it = 0; it < 8 && i > 0; ++it) {
-    const double l = col_loss(L, p, grid_value(i - 1, G, lo, hi), lane);
+  for (int it = 0; it < G && i > 0; ++it) {
+    const double l = node_loss(L, p, i - 1, G, lo, hi, lam64, lane);
     if (!(l <= li)) break;
     --i;
     li = l;
     moved_left = true;
   }
   if (!moved_left) {
-    for (int it = 0; it < 8 && i + 1 < G; ++it) {
-      const double l = col_loss(L, p, grid_value(i + 1, G, lo, hi), lane);
+    for (int it = 0; it < G && i + 1 < G; ++it) {
+      const double l = node_loss(L, p, i + 1, G, lo, hi, lam64, lane);
       if (!(l < li)) break;
       ++i;
       li = l;
@@ -359,7 +396,7 @@ template <int PREC, int KPL, bool SAME>
 __global__ __launch_bounds__(256) void delta_refine_kernel(
     int mode, const float* __restrict__ uy, int64_t P, int64_t p_pad, int64_t kpad, const double* __restrict__ s,
     int64_t neff, const double* __restrict__ s_lam, int64_t nlam, double nco, int G, double lo, double hi,
-    const int* __restrict__ idx, double* __restrict__ log_delta,
+    const int* __restrict__ idx, const double* __restrict__ lam64, double* __restrict__ log_delta,
     double* __restrict__ rWr, double* __restrict__ h2, void* q1a, void* q1b, void* q2a, void* q2b,
     float* __restrict__ q1s, float* __restrict__ q2s, double radius, double judge_tol, int* __restrict__ flags) {
   const int lane = threadIdx.x & 63;
@@ -383,16 +420,19 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
   }
   if (p >= p_pad) return;
   if (p >= P) {  // padding columns (their plane rows are zeroed by delta_planes_kernel)
-    if (PREC == PREC_SPLIT_FP16 && lane == 0) { q1s[p] = 1.f; q2s[p] = 1.f; }
+    if (PREC == PREC_SPLIT_FP16 && lane == 0) {
+      q1s[p] = 1.f;
+      if (q2s) q2s[p] = 1.f;
+    }
     return;
   }
   ColLik<KPL, SAME> L{uy, p_pad, s, neff, s_lam, nlam, nco, sh_s, SAME ? sh_s : sh_l, sh_sf, SAME ? sh_sf : sh_lf, {}};
   L.load(p, lane);
   double t;
   int i = (mode == MODE_FIXED || mode == MODE_GRID) ? 0 : idx[p];
-  // the reference replay (golden section) brackets the FP64 argmin exactly; Newton keeps the FP32 grid's bracket
-  // (the two differ only where neighbouring grid losses tie to FP32 precision, i.e. on flat likelihoods)
-  if (mode == MODE_GOLDEN) i = fp64_grid_argmin(L, p, i, G, lo, hi, lane);
+  // every search brackets the FP64 grid argmin, as the reference does (the FP32 grid's pick differs only where
+  // neighbouring grid losses tie to FP32 precision, i.e. on flat likelihoods)
+  if (mode != MODE_FIXED && mode != MODE_GRID) i = fp64_grid_argmin(L, p, i, G, lo, hi, lam64, lane);
   bool need_bracket = mode == MODE_NEWTON;
   if (mode == MODE_FIXED) {
     t = log_delta[p];  // the caller's log delta, used as given (core.py:639)
@@ -485,14 +525,14 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
     h2[p] = 1.0 / (1.0 + delta);
     if constexpr (PREC == PREC_SPLIT_FP16) {
       q1s[p] = (float)sc1;
-      q2s[p] = (float)sc2;
+      if (q2s) q2s[p] = (float)sc2;
     }
   }
 }
 
 // Q1 = w o y and Q2 = w, w = 1 / (s_k + exp(log_delta[p])), written as the scan's column-side planes (zero on the
 // padding k in [neff, kpad) and on the padding columns p in [P, p_pad)); split-FP16 rows are multiplied by the
-// scales the refinement chose.  A block owns 8 adjacent columns and walks the K chunks: for one chunk its 256
+// scales the refinement chose.  q2a == NULL: Q2 is not written (factored weights, factored_planes_kernel).  A block owns 8 adjacent columns and walks the K chunks: for one chunk its 256
 // elements are contiguous in every plane (element (p, k) at ((k / 32) p_pad + p) 32 + k % 32), so each chunk is
 // one 1 KiB read of UY and four contiguous 512 B (fp32: two 1 KiB) writes.
 template <int PREC>
@@ -508,8 +548,9 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
   const double delta = live ? exp(log_delta[p]) : 1.0;
   double sc1 = 1.0, sc2 = 1.0;
   if constexpr (PREC == PREC_SPLIT_FP16) {
-    if (live) { sc1 = (double)q1s[p]; sc2 = (double)q2s[p]; }
+    if (live) { sc1 = (double)q1s[p]; sc2 = q2s ? (double)q2s[p] : 1.0; }
   }
+  const bool wq2 = q2a != nullptr;
   const int64_t nch = kpad / 32;  // a multiple of 4 (kpad % 128 == 0): four chunks' loads in flight per thread
 #pragma unroll 4
   for (int64_t c = 0; c < nch; ++c) {
@@ -525,20 +566,84 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
       split2<PREC>(q1 * sc1, hh, ll);
       ((u16*)q1a)[off] = hh;
       ((u16*)q1b)[off] = ll;
-      split2<PREC>(w * sc2, hh, ll);
-      ((u16*)q2a)[off] = hh;
-      ((u16*)q2b)[off] = ll;
+      if (wq2) {
+        split2<PREC>(w * sc2, hh, ll);
+        ((u16*)q2a)[off] = hh;
+        ((u16*)q2b)[off] = ll;
+      }
     } else {
       ((float*)q1a)[off] = (float)q1;
-      ((float*)q2a)[off] = (float)w;
+      if (wq2) ((float*)q2a)[off] = (float)w;
     }
   }
 }
 
+// Factored weights.  1 / z = sum_r lam_r^2 exp(-t_r z) to a stated relative accuracy on the z range of a scan (a
+// positive exponential sum, nodes t_r and lam_r from the caller), so with z = s_k + delta_p
+//   sWs = sum_k US_k^2 / (s_k + delta_p) = sum_r C[m][r] B[p][r],  C = (US^2) A,  A[k][r] = lam_r exp(-t_r s_k),
+//   B[p][r] = lam_r exp(-t_r delta_p),
+// and the scan's second product runs over R ~ 100 terms instead of Neff.  This kernel writes B as the Q2 planes
+// (rows p_pad, K = rpad; zero for r >= R and on the padding columns p >= P); split-FP16 rows are multiplied by the
+// power of two of their L2 norm (q2_scale[p]), which bounds every entry by 2^14.  A block owns 8 columns; thread
+// (column tid / 32, j = tid % 32) handles r = 32 c + j of every chunk c, so each chunk's 256 elements are
+// contiguous in the planes.  The norm is a fixed-order sum over the column's 32 lanes.
+template <int PREC>
+__global__ __launch_bounds__(256) void factored_planes_kernel(int64_t P, int64_t p_pad, int64_t rpad,
+                                                              const double* __restrict__ t,
+                                                              const double* __restrict__ lam, int64_t R,
+                                                              const double* __restrict__ log_delta, void* qa,
+                                                              void* qb, float* __restrict__ qs) {
+  const int64_t p = (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);  // p < p_pad (p_pad is a multiple of 8)
+  const int j = threadIdx.x & 31;
+  const bool live = p < P;
+  const double delta = live ? exp(log_delta[p]) : 0.0;
+  auto val = [&](int64_t r) { return (live && r < R) ? lam[r] * exp(-t[r] * delta) : 0.0; };
+  double sc = 1.0;
+  if constexpr (PREC == PREC_SPLIT_FP16) {
+    double ss = 0.0;
+    for (int64_t r = j; r < R; r += 32) {
+      const double v = val(r);
+      ss += v * v;
+    }
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 32);
+    sc = live ? pow2_scale(sqrt(ss)) : 1.0;
+    if (j == 0) qs[p] = (float)sc;
+  }
+  const int64_t nch = rpad / 32;
+  for (int64_t c = 0; c < nch; ++c) {
+    const int64_t r = c * 32 + j;
+    const int64_t off = (c * p_pad + p) * 32 + j;
+    const double v = val(r);
+    if constexpr (is_split(PREC)) {
+      u16 hh, ll;
+      split2<PREC>(v * sc, hh, ll);
+      ((u16*)qa)[off] = hh;
+      ((u16*)qb)[off] = ll;
+    } else {
+      ((float*)qa)[off] = (float)v;
+    }
+  }
+}
+
+hipError_t launch_factored_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
+                                  int64_t R, const double* log_delta, void* qa, void* qb, float* qs,
+                                  hipStream_t stream) {
+  const dim3 grid((unsigned)(p_pad / 8));
+#define FLMM_FPL(PR)                                                                                                 \
+  hipLaunchKernelGGL((factored_planes_kernel<PR>), grid, dim3(256), 0, stream, P, p_pad, rpad, t, lam, R, log_delta, \
+                     qa, qb, qs)
+  if (prec == PREC_SPLIT_BF16) FLMM_FPL(PREC_SPLIT_BF16);
+  else if (prec == PREC_SPLIT_FP16) FLMM_FPL(PREC_SPLIT_FP16);
+  else FLMM_FPL(PREC_FP32);
+#undef FLMM_FPL
+  return hipGetLastError();
+}
+
 hipError_t launch_delta_weights(const double* s, int64_t neff, const double* s_lam, int64_t nlam, int64_t kpad, int G,
-                                double lo, double hi, float* wplane, float* lam, hipStream_t stream) {
+                                double lo, double hi, float* wplane, float* lam, double* lam64, hipStream_t stream) {
   hipLaunchKernelGGL(delta_weights_kernel, dim3(TILE), dim3(256), 0, stream, s, neff, s_lam, nlam, kpad, G, lo, hi,
-                     wplane, lam);
+                     wplane, lam, lam64);
   return hipGetLastError();
 }
 
@@ -559,7 +664,7 @@ hipError_t launch_delta_grid(const float* wplane, const float* lam, int G, doubl
 
 hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_pad, int64_t kpad,
                                const double* s, int64_t neff, const double* s_lam, int64_t nlam, double nco, int G,
-                               double lo, double hi, const int* idx,
+                               double lo, double hi, const int* idx, const double* lam64,
                                double* log_delta, double* rWr, double* h2, void* q1a, void* q1b, void* q2a,
                                void* q2b, float* q1s, float* q2s, double radius, double judge_tol, int* flags,
                                hipStream_t stream) {
@@ -569,8 +674,8 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
   const bool same = s_lam == s && nlam == neff;
 #define FLMM_REFINE1(PR, K, SM)                                                                                     \
   hipLaunchKernelGGL((delta_refine_kernel<PR, K, SM>), grid, dim3(256), 0, stream, mode, uy, P, p_pad, kpad, s,    \
-                     neff, s_lam, nlam, nco, G, lo, hi, idx, log_delta, rWr, h2, q1a, q1b, q2a, q2b, q1s, q2s,       \
-                     radius, judge_tol, flags)
+                     neff, s_lam, nlam, nco, G, lo, hi, idx, lam64, log_delta, rWr, h2, q1a, q1b, q2a, q2b, q1s,     \
+                     q2s, radius, judge_tol, flags)
 #define FLMM_REFINE(PR, K)                                                                                          \
   do {                                                                                                              \
     if (same) FLMM_REFINE1(PR, K, true); else FLMM_REFINE1(PR, K, false);                                           \

@@ -32,14 +32,24 @@ hipError_t launch_rotate(int mode, const TileArgs<PREC, 1>& args, const float* a
                          void* o0, void* o1, void* o2, void* o3, float* out_scale1, float* out_scale2,
                          hipStream_t stream);
 hipError_t launch_delta_weights(const double* s, int64_t neff, const double* s_lam, int64_t nlam, int64_t kpad, int G,
-                                double lo, double hi, float* wplane, float* lam, hipStream_t stream);
+                                double lo, double hi, float* wplane, float* lam, double* lam64, hipStream_t stream);
 hipError_t launch_delta_grid(const float* wplane, const float* lam, int G, double nco, const float* uy, int64_t P,
                              int64_t p_pad, int64_t kpad, int* idx, hipStream_t stream);
 hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_pad, int64_t kpad,
                                const double* s, int64_t neff, const double* s_lam, int64_t nlam, double nco, int G,
-                               double lo, double hi, const int* idx, double* log_delta, double* rWr, double* h2,
+                               double lo, double hi, const int* idx, const double* lam64, double* log_delta,
+                               double* rWr, double* h2,
                                void* q1a, void* q1b, void* q2a, void* q2b, float* q1s, float* q2s, double radius,
                                double judge_tol, int* flags, hipStream_t stream);
+hipError_t launch_gemm_f64(const double* x2, int64_t rows, int64_t kpad, const double* A, int64_t rpad, double* C,
+                           hipStream_t stream);
+hipError_t launch_rescore(const double* US, const double* UY, int64_t ld, const double* s, int64_t neff,
+                          const double* log_delta, const int64_t* us_row, const int64_t* col, const int64_t* out_row,
+                          int64_t n, double nu, double* f, double* beta, double* se, double* sfve, int64_t out_ld,
+                          hipStream_t stream);
+hipError_t launch_factored_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
+                                  int64_t R, const double* log_delta, void* qa, void* qb, float* qs,
+                                  hipStream_t stream);
 template <int PREC>
 hipError_t launch_wald_scan(const TileArgs<PREC, 2>& args, const ScanScales& sc, int64_t Mc, int64_t P,
                             int64_t n_real, const double* rWr, const double* h2, double nu, const int64_t* row_index,

@@ -6,6 +6,8 @@
 // Epilogues write the next stage's operands directly (rows i, K index j):
 //   MODE_UY:  one fp32 plane of the rotated columns, read by the delta search and the grid GEMM
 //   MODE_US:  the scan's SNP-side planes S1 = x and S2 = x^2 (split pairs, or fp32)
+//   MODE_US_X2: the S1 planes as MODE_US, and x^2 in FP64, row-major (a_rows_pad x b_rows_pad), for the factored
+//             weights' S2 = x^2 A (a small FP64 GEMM by the caller)
 // Split-FP16 operands arrive with per-row power-of-two scales (a_scale, b_scale) that the epilogue divides out
 // exactly.  Because B's rows are unit vectors, |x[i][j]| <= ||A_i|| < 2^14 / a_scale[i], so the S1 / S2 planes are
 // written with scales a_scale[i] and a_scale[i]^2 / 2^14 (out_scale1 / out_scale2).
@@ -13,7 +15,7 @@
 
 namespace flmm {
 
-enum RotMode : int { MODE_UY = 0, MODE_US = 1 };
+enum RotMode : int { MODE_UY = 0, MODE_US = 1, MODE_US_X2 = 2 };
 
 template <int PREC, int MODE>
 __global__ __launch_bounds__(NTHREADS, 1) void rotate_kernel(TileArgs<PREC, 1> args, int64_t n_jtiles,
@@ -50,9 +52,9 @@ __global__ __launch_bounds__(NTHREADS, 1) void rotate_kernel(TileArgs<PREC, 1> a
         inv_a = 1.0 / sa;
         s1 = sa;
         s2 = sa * sa * (1.0 / 16384.0);
-        if (MODE == MODE_US && jt == 0 && wn == 0 && (lane & 31) == 0) {
+        if (MODE != MODE_UY && jt == 0 && wn == 0 && (lane & 31) == 0) {
           out_scale1[i] = (float)s1;
-          out_scale2[i] = (float)s2;
+          if (MODE == MODE_US) out_scale2[i] = (float)s2;
         }
       }
 #pragma unroll
@@ -63,6 +65,17 @@ __global__ __launch_bounds__(NTHREADS, 1) void rotate_kernel(TileArgs<PREC, 1> a
         if constexpr (PREC == PREC_SPLIT_FP16) x *= inv_a * inv_b[bj];
         if constexpr (MODE == MODE_UY) {
           ((float*)o0)[off] = (float)x;
+        } else if constexpr (MODE == MODE_US_X2) {
+          // lanes 0..31 of one store write 32 consecutive j of row i (256 contiguous bytes)
+          ((double*)o2)[i * (n_jtiles * TILE) + j] = x * x;
+          if constexpr (is_split(PREC)) {
+            u16 h, l;
+            split2<PREC>(x * s1, h, l);
+            ((u16*)o0)[off] = h;
+            ((u16*)o1)[off] = l;
+          } else {
+            ((float*)o0)[off] = (float)x;
+          }
         } else if constexpr (is_split(PREC)) {
           u16 h, l;
           split2<PREC>(x * s1, h, l);
@@ -90,9 +103,117 @@ hipError_t launch_rotate(int mode, const TileArgs<PREC, 1>& args, const float* a
   if (mode == MODE_UY)
     hipLaunchKernelGGL((rotate_kernel<PREC, MODE_UY>), dim3((unsigned)n_tiles), dim3(NTHREADS), lds, stream, args,
                        n_jtiles, n_tiles, a_scale, b_scale, o0, o1, o2, o3, out_scale1, out_scale2);
+  else if (mode == MODE_US_X2)
+    hipLaunchKernelGGL((rotate_kernel<PREC, MODE_US_X2>), dim3((unsigned)n_tiles), dim3(NTHREADS), lds, stream, args,
+                       n_jtiles, n_tiles, a_scale, b_scale, o0, o1, o2, o3, out_scale1, out_scale2);
   else
     hipLaunchKernelGGL((rotate_kernel<PREC, MODE_US>), dim3((unsigned)n_tiles), dim3(NTHREADS), lds, stream, args,
                        n_jtiles, n_tiles, a_scale, b_scale, o0, o1, o2, o3, out_scale1, out_scale2);
+  return hipGetLastError();
+}
+
+// Row GEMM in FP64: C[i][r] = sum_{k < kpad} X2[i][k] A[k][r] (X2: rows x kpad, A: kpad x rpad, C: rows x rpad,
+// all row-major; rows % 64 == 0, kpad % 32 == 0, rpad % 128 == 0) -- the factored weights' SNP side (X2 = US^2)
+// and the FP64 rotations of the re-scoring (X2 = S^T or Y^T, A = U_eff).  Each element sums k in order, so a row's
+// result does not depend on how many rows a launch holds.  A 256-thread block owns 64 rows x 128 r: thread
+// (g = tid / 32, c = tid % 32) accumulates rows 8 g .. 8 g + 7 at r = c + 32 q (q < 4), 32 FP64 sums, from K
+// chunks of 32 staged in LDS (A chunk 32 x 128, X2 chunk 64 x 32 padded against bank conflicts).
+__global__ __launch_bounds__(256) void gemm_f64_kernel(const double* __restrict__ x2, int64_t kpad,
+                                                       const double* __restrict__ A, int64_t rpad,
+                                                       double* __restrict__ C) {
+  __shared__ double as[32][128];
+  __shared__ double xs[64][33];
+  const int tid = threadIdx.x, g = tid >> 5, c = tid & 31;
+  const int64_t i0 = (int64_t)blockIdx.x * 64, r0 = (int64_t)blockIdx.y * 128;
+  double acc[8][4];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[i][q] = 0.0;
+  for (int64_t k0 = 0; k0 < kpad; k0 += 32) {
+#pragma unroll
+    for (int e = tid; e < 32 * 128; e += 256) as[e >> 7][e & 127] = A[(k0 + (e >> 7)) * rpad + r0 + (e & 127)];
+#pragma unroll
+    for (int e = tid; e < 64 * 32; e += 256) xs[e >> 5][e & 31] = x2[(i0 + (e >> 5)) * kpad + k0 + (e & 31)];
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < 32; ++k) {
+      double a[4], x[8];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a[q] = as[k][c + 32 * q];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) x[i] = xs[8 * g + i][k];
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[i][q] = fma(x[i], a[q], acc[i][q]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) C[(i0 + 8 * g + i) * rpad + r0 + c + 32 * q] = acc[i][q];
+}
+
+hipError_t launch_gemm_f64(const double* x2, int64_t rows, int64_t kpad, const double* A, int64_t rpad, double* C,
+                           hipStream_t stream) {
+  const dim3 grid((unsigned)(rows / 64), (unsigned)(rpad / 128));
+  hipLaunchKernelGGL(gemm_f64_kernel, grid, dim3(256), 0, stream, x2, kpad, A, rpad, C);
+  return hipGetLastError();
+}
+
+// FP64 re-evaluation of selected tests i < n, (SNP row us_row[i] of US, column col[i]): with w_k = 1 / (s_k + delta),
+// delta = exp(log_delta[col]), the exact sums a = sum_k US w UY, d = sum_k US^2 w and rWr = sum_k w UY^2 over the
+// FP64 operands (US: SNP rows, UY: column rows, both with leading dimension ld), then the scan epilogue's Wald
+// arithmetic (core.py:649-667) in FP64, written at [out_row[i] * out_ld + col[i]].  One wave per test, lanes
+// striding k, fixed-order wave sums.
+__global__ __launch_bounds__(256) void rescore_kernel(const double* __restrict__ US, const double* __restrict__ UY,
+                                                      int64_t ld, const double* __restrict__ s, int64_t neff,
+                                                      const double* __restrict__ log_delta,
+                                                      const int64_t* __restrict__ us_row,
+                                                      const int64_t* __restrict__ col,
+                                                      const int64_t* __restrict__ out_row, int64_t n, double nu,
+                                                      double* __restrict__ f, double* __restrict__ beta,
+                                                      double* __restrict__ se, double* __restrict__ sfve,
+                                                      int64_t out_ld) {
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;  // wave-uniform
+  const int64_t m = us_row[i], p = col[i];
+  const double delta = exp(log_delta[p]);
+  const double* u = US + m * ld;
+  const double* y = UY + p * ld;
+  double a = 0.0, d = 0.0, r = 0.0;
+  for (int64_t k = lane; k < neff; k += 64) {
+    const double w = 1.0 / (s[k] + delta);
+    const double uk = u[k], yk = y[k];
+    a += uk * w * yk;
+    d += uk * uk * w;
+    r += w * yk * yk;
+  }
+  a = wave_sum(a);
+  d = fmax(wave_sum(d), FLOOR64);
+  r = fmax(wave_sum(r), FLOOR64);
+  if (lane == 0) {
+    const double b = a / d;
+    const double rss = fmax(r - a * a / d, FLOOR64);
+    const double var = rss / nu / d;
+    const int64_t o = out_row[i] * out_ld + p;
+    f[o] = b * b / var;
+    beta[o] = b;
+    se[o] = sqrt(var);
+    sfve[o] = sqrt(fmax(a * a / (d * r), 0.0));
+  }
+}
+
+hipError_t launch_rescore(const double* US, const double* UY, int64_t ld, const double* s, int64_t neff,
+                          const double* log_delta, const int64_t* us_row, const int64_t* col, const int64_t* out_row,
+                          int64_t n, double nu, double* f, double* beta, double* se, double* sfve, int64_t out_ld,
+                          hipStream_t stream) {
+  const dim3 grid((unsigned)((n + 3) / 4));
+  hipLaunchKernelGGL(rescore_kernel, grid, dim3(256), 0, stream, US, UY, ld, s, neff, log_delta, us_row, col, out_row,
+                     n, nu, f, beta, se, sfve, out_ld);
   return hipGetLastError();
 }
 

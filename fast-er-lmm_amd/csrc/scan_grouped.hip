@@ -8,7 +8,8 @@
 // groups (or a new chromosome) after the table is rewritten in place.  The grid is persistent (a multiple of 8
 // workgroups); round r hands XCD x (blocks b with b % 8 == x) the contiguous tiles
 // r G + x G / 8 + [0, G / 8), so the workgroups an XCD runs together share L2, and each tile is found by binary
-// search over the prefix sums and run by the square or wide tile body.
+// search over the prefix sums and run by the square or wide tile body, or (FLMM_SCAN_WIDE_FACTORED: factored
+// weights, every group with kpad2 > 0) the wide two-phase body with its second product over kpad2.
 #include "../../include/fasterlmm_amd.h"
 #include "scan_square.h"
 #include "scan_wide.h"
@@ -28,6 +29,7 @@ __device__ inline void job_args(const flmm_scan_group& j, TileArgs<PREC, 2>& t, 
   t.a_rows_pad = j.mc_pad;
   t.b_rows_pad = j.p_pad;
   t.nchunks = (int)(j.kpad / KCHUNK);
+  t.nchunks2 = (int)(j.kpad2 / KCHUNK);
   sc = PREC == PREC_SPLIT_FP16 ? ScanScales{j.s1_scale, j.s2_scale, j.q1_scale, j.q2_scale}
                                : ScanScales{nullptr, nullptr, nullptr, nullptr};
   out = ScanOut{j.f, j.beta, j.se, j.sfve, j.nullh2, j.n_real};
@@ -57,6 +59,9 @@ __global__ __launch_bounds__(256, 1) void wald_scan_grouped_kernel(const int64_t
       if constexpr (VARIANT == FLMM_SCAN_SQUARE)
         square_tile<PREC>(args, sc, n_mtiles, t - start[lo], j.mc, j.P, j.n_real, j.rWr, j.h2, j.nu, j.row_index,
                           out, j.pmax, j.p_pad);
+      else if constexpr (VARIANT == FLMM_SCAN_WIDE_FACTORED)  // second product over kpad2 (the low-rank tile)
+        wide_tile<PREC, true>(args, sc, n_mtiles, (j.P + W_TP - 1) / W_TP, t - start[lo], j.mc, j.P, j.n_real, j.rWr,
+                              j.h2, j.nu, j.row_index, out, j.pmax, j.p_pad);
       else
         wide_tile<PREC>(args, sc, n_mtiles, (j.P + W_TP - 1) / W_TP, t - start[lo], j.mc, j.P, j.n_real, j.rWr, j.h2, j.nu,
                         j.row_index, out, j.pmax, j.p_pad);
@@ -85,8 +90,11 @@ hipError_t launch_wald_scan_grouped(int prec, int variant, const int64_t* table,
       constexpr int lds = tile_lds_bytes<PR, 2>();                                                                   \
       hipLaunchKernelGGL((wald_scan_grouped_kernel<PR, FLMM_SCAN_SQUARE>), dim3(blocks), dim3(256), lds, stream,     \
                          table);                                                                                     \
-    } else {                                                                                                         \
+    } else if (variant == FLMM_SCAN_WIDE) {                                                                          \
       hipLaunchKernelGGL((wald_scan_grouped_kernel<PR, FLMM_SCAN_WIDE>), dim3(blocks), dim3(256), 0, stream, table); \
+    } else {                                                                                                         \
+      hipLaunchKernelGGL((wald_scan_grouped_kernel<PR, FLMM_SCAN_WIDE_FACTORED>), dim3(blocks), dim3(256), 0, stream, \
+                         table);                                                                                     \
     }                                                                                                                \
   } while (0)
   if (prec == PREC_SPLIT_BF16) FLMM_GRP(PREC_SPLIT_BF16);

@@ -17,6 +17,7 @@ PREC_FP32 = 1
 PREC_SPLIT_FP16 = 2
 ROTATE_UY = 0
 ROTATE_US = 1
+ROTATE_US_X2 = 2
 DELTA_GOLDEN = 0
 DELTA_NEWTON = 1
 DELTA_FIXED = 2
@@ -26,6 +27,7 @@ BED_F64 = 1
 BED_F32 = 2
 SCAN_SQUARE = 0
 SCAN_WIDE = 1
+SCAN_WIDE_FACTORED = 2
 
 _P = C.c_void_p
 _I64 = C.c_int64
@@ -45,12 +47,15 @@ SIGNATURES: dict[str, list] = {
     "flmm_row_norms": [_P, _I64, _I64, _I64, _I64, _P, _P, _P],
     "flmm_pack_planes": [_I, _P, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _I64, _I64, _P],
     "flmm_rotate": [_I, _I, _P, _P, _P, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P],
-    "flmm_delta_weights": [_P, _I64, _P, _I64, _I64, _I, _D, _D, _P, _P, _P],
+    "flmm_delta_weights": [_P, _I64, _P, _I64, _I64, _I, _D, _D, _P, _P, _P, _P],
     "flmm_delta_grid": [_P, _P, _I, _D, _P, _I64, _I64, _I64, _P, _P],
     "flmm_delta_refine": [_I, _I, _P, _I64, _I64, _I64, _P, _I64, _P, _I64, _D, _I, _D, _D, _P, _P, _P, _P, _P,
-                          _P, _P, _P, _P, _P, _P],
-    "flmm_delta_refine_learned": [_I, _P, _I64, _I64, _I64, _P, _I64, _P, _I64, _D, _I, _D, _D, _P, _D, _D, _P, _P,
-                                  _P, _P, _P, _P, _P, _P, _P, _P, _P],
+                          _P, _P, _P, _P, _P, _P, _P],
+    "flmm_gemm_f64": [_P, _I64, _I64, _P, _I64, _P, _P],
+    "flmm_rescore_f64": [_P, _P, _I64, _P, _I64, _P, _P, _P, _P, _I64, _D, _P, _P, _P, _P, _I64, _P],
+    "flmm_factored_weight_planes": [_I, _I64, _I64, _I64, _P, _P, _I64, _P, _P, _P, _P, _P],
+    "flmm_delta_refine_learned": [_I, _P, _I64, _I64, _I64, _P, _I64, _P, _I64, _D, _I, _D, _D, _P, _P, _D, _D, _P,
+                                  _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "flmm_mlp_features": [_P, _I64, _I64, _I64, _P, _D, _D, _P, _P],
     "flmm_mlp_dense_fwd": [_P, _I64, _I, _P, _P, _I, _I, _P, _P],
     "flmm_mlp_dense_bwd": [_P, _P, _P, _P, _I64, _I, _I, _I, _P, _P, _P, _P, _P],
@@ -75,7 +80,7 @@ class ScanGroup(C.Structure):
         ("s1_hi", _P), ("s1_lo", _P), ("s2_hi", _P), ("s2_lo", _P), ("s1_scale", _P), ("s2_scale", _P),
         ("mc_pad", _I64),
         ("q1_hi", _P), ("q1_lo", _P), ("q2_hi", _P), ("q2_lo", _P), ("q1_scale", _P), ("q2_scale", _P),
-        ("p_pad", _I64), ("kpad", _I64), ("mc", _I64), ("P", _I64), ("n_real", _I64),
+        ("p_pad", _I64), ("kpad", _I64), ("kpad2", _I64), ("mc", _I64), ("P", _I64), ("n_real", _I64),
         ("rWr", _P), ("h2", _P), ("nu", _D), ("row_index", _P),
         ("f", _P), ("beta", _P), ("se", _P), ("sfve", _P), ("nullh2", _P), ("pmax", _P), ("max_F", _P),
     ]

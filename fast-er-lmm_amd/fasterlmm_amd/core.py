@@ -28,6 +28,7 @@ The test SNPs' OLS projection S - X X^+ S (core.py:628) is not formed: B^T X = 0
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass
 
 import numpy as np
@@ -119,6 +120,12 @@ def eigendecompose(K: Tensor, jitter: float = 1e-6) -> tuple[Tensor, Tensor]:
 
 # PROJECTED BASIS ON THE GPU -------
 
+# Real-column tests with F above this are re-evaluated in FP64 (_Basis.rescore).  The scan's FP32-accumulated sums
+# carry ~1e-7 relative error, and rss = rWr - sWy^2 / sWs amplifies it by ~1 + F / nu, while |d log10 p| grows like
+# F x (relative F error) / (2 ln 10): below F = 30 the error stays near 1e-6, far inside the 1e-3 bar.
+RESCORE_F = 30.0
+
+
 class _Basis:
     """A projected basis B (columns col0 .. col0 + n_basis - 1 of the (N, L) FP64 matrix `mat`) with its weight
     spectrum s (1 / (s + delta) per basis vector), held as MFMA planes of B^T, plus the rotated scan columns.
@@ -135,9 +142,14 @@ class _Basis:
         self.prec = prec
         self.nu = float(nu)
         self.s = s.contiguous()
+        self.s_range = tuple(torch.stack(torch.aminmax(self.s)).tolist()) if n_basis > 0 else (0.0, 0.0)
         self.s_ml = None if s_ml is None else s_ml.contiguous()
         # B^T planes: row j = basis vector j, element k = mat[k, col0 + j]
         self.bt = engine.pack(prec, mat.contiguous(), n_basis, N, 1, L, self.kpad, self.npad, offset=col0)
+        self.mat, self.col0 = mat, col0  # the FP64 basis, for the FP64 re-scoring
+        self.y_cols = None  # FP64 scan columns (at least the real ones), for the FP64 re-scoring
+        self._u_pad = None
+        self._uy64 = {}
         self.uy = None  # fp32 plane of B^T Y
         self.P = 0
         self.p_pad = 0
@@ -149,11 +161,53 @@ class _Basis:
         self.P, self.p_pad = P, rup(max(P, 1), PTILE)
         ypl = engine.pack(self.prec, Y.contiguous(), P, N, 1, P, self.p_pad, self.npad)
         self.uy = engine.rotate_uy(ypl, self.bt)
+        self.y_cols, self._uy64 = Y, {}
 
-    def use_column_planes(self, ypl: engine.Planes, P: int) -> None:
-        """UY plane from pre-packed column planes (shared by every chromosome of a LOCO scan)."""
+    def use_column_planes(self, ypl: engine.Planes, P: int, y_real: Tensor | None = None) -> None:
+        """UY plane from pre-packed column planes (shared by every chromosome of a LOCO scan); y_real: the FP64
+        real columns (N, n_real) for the re-scoring."""
         self.P, self.p_pad = P, ypl.rows_pad
         self.uy = engine.rotate_uy(ypl, self.bt)
+        self.y_cols, self._uy64 = y_real, {}
+
+    def _rotate_f64(self, cols: Tensor) -> Tensor:
+        """FP64 B^T c for the columns c of cols (N, n): rows (rup(n, 64), kpad), row i = column i rotated."""
+        if self._u_pad is None:
+            self._u_pad = torch.zeros(self.npad, self.kpad, dtype=torch.float64, device=cols.device)
+            self._u_pad[: self.N, : self.neff] = self.mat[:, self.col0:self.col0 + self.neff]
+        n = cols.shape[1]
+        rows = rup(max(n, 1), 64)
+        a = torch.zeros(rows, self.npad, dtype=torch.float64, device=cols.device)
+        a[:n, : self.N] = cols.T
+        c = torch.empty(rows, self.kpad, dtype=torch.float64, device=cols.device)
+        _lib.call("flmm_gemm_f64", a.data_ptr(), rows, self.npad, self._u_pad.data_ptr(), self.kpad, c.data_ptr(),
+                  _lib.stream())
+        return c
+
+    def rescore(self, columns, local: Tensor, row_index: Tensor, n_real: int, out: engine.ScanOutputs,
+                d: engine.DeltaResult) -> int:
+        """FP64 re-evaluation (flmm_rescore_f64) of one scanned SNP block's real-column tests with F > RESCORE_F.
+        Block position i holds SNP column local[i] of columns(idx) -> (N, len(idx)) FP64 and output row
+        row_index[i].  Returns the number of re-scored tests."""
+        if n_real == 0 or self.y_cols is None or row_index.numel() == 0:
+            return 0
+        F = out.f.index_select(0, row_index)
+        nz = torch.nonzero(F > RESCORE_F)  # (n, 2): block position, column
+        n = int(nz.shape[0])
+        if n == 0:
+            return 0
+        mi, pi = nz[:, 0].contiguous(), nz[:, 1].contiguous()  # the kernel reads plain int64 arrays
+        um, inv = torch.unique(mi, return_inverse=True)
+        inv = inv.contiguous()
+        US = self._rotate_f64(columns(local.index_select(0, um)))
+        UY = self._uy64.get(n_real)
+        if UY is None:
+            UY = self._uy64[n_real] = self._rotate_f64(self.y_cols[:, :n_real])
+        _lib.call("flmm_rescore_f64", US.data_ptr(), UY.data_ptr(), self.kpad, self.s.data_ptr(), self.neff,
+                  d.log_delta.data_ptr(), inv.data_ptr(), pi.data_ptr(), row_index.index_select(0, mi).data_ptr(), n,
+                  float(self.nu), out.f.data_ptr(), out.beta.data_ptr(), out.se.data_ptr(), out.sfve.data_ptr(),
+                  out.f.shape[1], _lib.stream())
+        return n
 
     def uy_dense(self) -> Tensor:
         return engine.fp32_plane_to_dense(self.uy, self.p_pad, self.kpad, self.P, self.neff).T.double()
@@ -163,22 +217,37 @@ class _Basis:
             return self.s, float(self.neff)
         return self.s_ml, float(self.N)
 
+    def factored(self, lo: float, hi: float) -> engine.Factored | None:
+        """Factored sWs weights for log delta in [lo, hi] (engine.settings.weights: "auto" = for the split
+        precisions), or None for the exact weights."""
+        w = engine.settings.weights
+        if w not in ("auto", "factored", "exact"):
+            raise ValueError(f"unknown weights setting {w!r}")
+        if w == "exact" or (w == "auto" and self.prec == _lib.PREC_FP32) or self.neff == 0:
+            return None
+        if not (math.isfinite(lo) and math.isfinite(hi)):
+            return None
+        return engine.factored_weights(self.s, self.s_range, self.kpad, lo, hi)
+
     def fit(self, n_grid: int, lo: float, hi: float, refine: bool, *, projected_loss: bool = True):
         s_lam, nco = self._loss(projected_loss)
+        fac = self.factored(lo, hi)  # every refined or grid log delta lies in [lo, hi]
         if refine and engine.settings.delta == "learned":
             from . import learned
             fit = learned.refine_learned(self, learned.model(), n_grid, lo, hi, n=self.N, c=self.N - self.neff,
                                          projected_loss=projected_loss)
+            if fac is not None:
+                engine.attach_factored(self.prec, fit.delta, fac)
             self.last_fallback = fit.fallback
             return fit.log_delta, fit.delta
         if not refine:
             d = engine.delta_refine(self.prec, _lib.DELTA_GRID, self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam,
-                                    nco, n_grid, lo, hi, None)
+                                    nco, n_grid, lo, hi, None, fac=fac)
             self.delta = d
             return d.log_delta, d
-        idx = engine.delta_grid_idx(self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam, nco, n_grid, lo, hi)
+        idx, lam64 = engine.delta_grid(self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam, nco, n_grid, lo, hi)
         d = engine.delta_refine(self.prec, engine.delta_code(), self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam,
-                                nco, n_grid, lo, hi, idx)
+                                nco, n_grid, lo, hi, idx, fac=fac, lam64=lam64)
         self.delta = d
         return d.log_delta, d
 
@@ -186,8 +255,12 @@ class _Basis:
         d = self.delta
         if d is not None and d.log_delta.data_ptr() == log_delta.data_ptr() and d.q1.prec == self.prec:
             return d
+        fac = None
+        if log_delta.numel() > 0:
+            lo, hi = (float(v) for v in torch.aminmax(log_delta.double()))
+            fac = self.factored(lo, hi)
         return engine.delta_refine(self.prec, _lib.DELTA_FIXED, self.uy, self.P, self.p_pad, self.kpad, self.s,
-                                   self.s, float(self.neff), 1, 0.0, 0.0, None, log_delta_in=log_delta)
+                                   self.s, float(self.neff), 1, 0.0, 0.0, None, log_delta_in=log_delta, fac=fac)
 
     def scan(self, Z: Tensor, row_index: Tensor, n_real: int, out: engine.ScanOutputs, d: engine.DeltaResult,
              block: int = 8192) -> None:
@@ -195,13 +268,20 @@ class _Basis:
         Mc = row_index.numel()
         for b0 in range(0, Mc, block):
             ri = row_index[b0:b0 + block]
-            s1, s2 = self.snp_planes(Z, ri)
+            s1, s2 = self.snp_planes(Z, ri, d)
             engine.wald_scan(s1, s2, d, ri.numel(), self.P, n_real, self.nu, ri, out)
+            del s1, s2
+            self.rescore(lambda idx: Z.index_select(1, idx), ri, ri, n_real, out, d)
 
-    def snp_planes(self, Z: Tensor, row_index: Tensor) -> tuple[engine.Planes, engine.Planes]:
-        """Scan planes S1 = B^T s, S2 = (B^T s)^2 of the SNP columns Z[:, row_index] (core.py:628-631)."""
+    def snp_planes(self, Z: Tensor, row_index: Tensor,
+                   d: engine.DeltaResult | None = None) -> tuple[engine.Planes, engine.Planes]:
+        """Scan planes S1 = B^T s, S2 = (B^T s)^2 of the SNP columns Z[:, row_index] (core.py:628-631), or
+        S2 = (B^T s)^2 A for the factored weights of d (default: the last fit)."""
+        d = self.delta if d is None else d
         mc = row_index.numel()
         spl = engine.pack(self.prec, Z, mc, self.N, 1, Z.shape[1], rup(mc, TILE), self.npad, row_index=row_index)
+        if d is not None and d.fac is not None:
+            return engine.factored_snp_planes(spl, self.bt, mc, d.fac)
         return engine.rotate_us(spl, self.bt)
 
 
@@ -271,6 +351,7 @@ def snp_wald_scan_compat(spec: CompatSpectrum, log_delta: Tensor, S: Tensor, *, 
     d = st.delta_for(ldd)
     out = _new_outputs(M, n_real, P, Sd.device)
     st.scan(Sd.contiguous(), torch.arange(M, device=Sd.device, dtype=torch.int64), n_real, out, d)
+    engine.refresh_real_max(out, n_real)
     return _home(ScanResult(out.f, out.beta, out.se, out.sfve, out.nullh2, out.max_F), home)
 
 
@@ -300,6 +381,7 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
     with stage("pack_columns"):
         Y_r = (Yd - Xd @ (Xpinv @ Yd)).contiguous()
         ypl = engine.pack(prec, Y_r, P, N, 1, P, rup(max(P, 1), PTILE), rup(N, KCHUNK))
+        y_real = Y_r[:, :n_real].contiguous()  # kept for the FP64 re-scoring of large-F real tests
         del Y_r
     jitter = 1e-6 if plain else 0.0
     eye = torch.eye(N, dtype=torch.float64, device=dev)
@@ -328,13 +410,14 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
         with stage("pack_basis"):
             st = _Basis(s, Ub[k], D, N - D, prec, nu=N - D - 1, s_ml=None if s_mls is None else s_mls[k])
         with stage("rotate_columns"):
-            st.use_column_planes(ypl, P)
+            st.use_column_planes(ypl, P, y_real)
         with stage("delta"):
             _, d = st.fit(n_grid, -10.0, 10.0, True, projected_loss=(not plain) or reml)
         with stage("snps+scan"):
             st.scan(Zd, test, n_real, out, d)
         if on_chrom is not None:
             on_chrom(k + 1, len(chroms))
+    engine.refresh_real_max(out, n_real)
     return _home(ScanResult(out.f, out.beta, out.se, out.sfve, out.nullh2, out.max_F), home)
 
 
@@ -486,6 +569,7 @@ def snp_wald_scan(spectrum: Spectrum, log_delta: Tensor, S_rot: Tensor, *, snp_c
     d = st.delta_for(ldd)
     out = _new_outputs(M, n_real, P, Sd.device)
     st.scan(Sd.contiguous(), torch.arange(M, device=Sd.device, dtype=torch.int64), n_real, out, d)
+    engine.refresh_real_max(out, n_real)
     return _home(ScanResult(out.f, out.beta, out.se, out.sfve, out.nullh2, out.max_F), home)
 
 

@@ -9,7 +9,8 @@ Data layout (HBM)
   genotypes     Z (N, M) FP64 row-major, standardised (the reference's input layout)
   planes        operand matrices in K-chunks of 32 (see include/fasterlmm_amd.h): Y_r columns (P_pad x N_pad),
                 U_eff^T (K_pad x N_pad), rotated columns UY (fp32, P_pad x K_pad), SNP planes S1 = US, S2 = US^2
-                (Mc_pad x K_pad) and column planes Q1 = w o UY, Q2 = w (P_pad x K_pad)
+                (Mc_pad x K_pad) and column planes Q1 = w o UY, Q2 = w (P_pad x K_pad); with factored weights
+                (expsum.py) S2 = US^2 A (Mc_pad x R_pad) and Q2 = B (P_pad x R_pad), R_pad = 128 typically
   outputs       f, beta, se, sfve, nullh2 (M, n_real) FP64 row-major; max_F (P,) FP64
 with N_pad = roundup(N, 32), K_pad = roundup(Neff, 128), P_pad = roundup(P, 128), Mc_pad = roundup(Mc, 128).
 """
@@ -21,7 +22,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, expsum
 
 TILE = 128
 PTILE = 256  # column padding: the wide scan variant tiles columns by 256
@@ -38,6 +39,8 @@ class Settings:
                                    # 3 products) or "fp32" (FP32 MFMA)
     delta: str = "newton"          # "newton" (exact local minimiser), "golden" (the reference's 50 steps) or
                                    # "learned" (MLP proposal + judged Newton, grid as fallback and judge)
+    weights: str = "auto"          # sWs weights 1 / (s + delta): "factored" (exponential sum, second product over
+                                   # ~128 terms), "exact" (over Neff), "auto" = factored for the split precisions
 
 
 settings = Settings()
@@ -142,6 +145,19 @@ def pack(prec: int, src: torch.Tensor, n_rows: int, n_k: int, row_stride: int, k
     return pl
 
 
+def rotate_us_x2(a: Planes, b: Planes) -> tuple[Planes, torch.Tensor]:
+    """S1 = x planes and x^2 as FP64 (a.rows_pad, b.rows_pad) row-major, x[i][j] = sum_n a[i][n] b[j][n]."""
+    dev = a.hi.device
+    s1 = Planes(a.prec, a.rows_pad, b.rows_pad, dev)
+    x2 = torch.empty(a.rows_pad, b.rows_pad, dtype=torch.float64, device=dev)
+    ah, al = a.ptrs
+    bh, bl = b.ptrs
+    o1 = s1.lo.data_ptr() if s1.lo is not None else None
+    _lib.call("flmm_rotate", a.prec, _lib.ROTATE_US_X2, ah, al, a.scale_ptr, a.rows_pad, bh, bl, b.scale_ptr,
+              b.rows_pad, a.kpad, s1.hi.data_ptr(), o1, x2.data_ptr(), None, s1.scale_ptr, None, _lib.stream())
+    return s1, x2
+
+
 def rotate_uy(a: Planes, b: Planes) -> torch.Tensor:
     """fp32 plane (a.rows_pad x b.rows_pad) of out[i][j] = sum_n a[i][n] b[j][n]."""
     out = torch.empty(a.rows_pad * b.rows_pad, dtype=torch.float32, device=a.hi.device)
@@ -178,37 +194,97 @@ def grid_values(n_grid: int, lo: float, hi: float) -> np.ndarray:
 
 
 @dataclass
+class Factored:
+    """Factored weights of one basis and log-delta range (expsum.py): R nodes, padded to rpad, and the SNP-side
+    map A (kpad, rpad) FP64, A[k, r] = lam_r exp(-t_r s_k) (zero for k >= neff or r >= R)."""
+    R: int
+    rpad: int
+    t: torch.Tensor    # (R,) FP64
+    lam: torch.Tensor  # (R,) FP64
+    A: torch.Tensor
+
+
+def factored_weights(s: torch.Tensor, s_range: tuple[float, float], kpad: int, log_delta_lo: float,
+                     log_delta_hi: float) -> Factored | None:
+    """Nodes for z = s_k + delta in the basis' eigenvalue range and [exp(lo), exp(hi)]; None when the sum would
+    not be shorter than the exact weights (rpad >= kpad)."""
+    t, lam = expsum.weight_nodes(*expsum.z_range(s_range[0], s_range[1], log_delta_lo, log_delta_hi))
+    R = t.size
+    rpad = rup(R, TILE)
+    if rpad >= kpad:
+        return None
+    dev = s.device
+    tt = torch.from_numpy(t).to(dev)
+    ll = torch.from_numpy(lam).to(dev)
+    A = torch.zeros(kpad, rpad, dtype=torch.float64, device=dev)
+    A[: s.numel(), :R] = ll.unsqueeze(0) * torch.exp(-s.unsqueeze(1) * tt.unsqueeze(0))
+    return Factored(R, rpad, tt, ll, A)
+
+
+@dataclass
 class DeltaResult:
     log_delta: torch.Tensor  # (P,) FP64
     rWr: torch.Tensor        # (P,) FP64
     h2: torch.Tensor         # (P,) FP64
     q1: Planes
-    q2: Planes
+    q2: Planes | None        # exact Q2 = w (K = kpad), or with fac the factored Q2 = B (K = fac.rpad)
     idx: torch.Tensor | None = None
     neff: int = 0
+    fac: Factored | None = None
 
 
-def delta_grid_idx(uy: torch.Tensor, P: int, p_pad: int, kpad: int, s: torch.Tensor, s_lam: torch.Tensor,
-                   nco: float, n_grid: int, lo: float, hi: float) -> torch.Tensor:
-    """Per-column grid argmin (int32, (P,))."""
+def attach_factored(prec: int, d: DeltaResult, fac: Factored) -> None:
+    """Write the factored Q2 planes B[p][r] = lam_r exp(-t_r delta_p) for d's columns and switch d to them."""
+    P = d.log_delta.numel()
+    p_pad = d.q1.rows_pad
+    q2 = Planes(prec, p_pad, fac.rpad, d.q1.hi.device)
+    q2h, q2l = q2.ptrs
+    _lib.call("flmm_factored_weight_planes", prec, P, p_pad, fac.rpad, fac.t.data_ptr(), fac.lam.data_ptr(), fac.R,
+              d.log_delta.data_ptr(), q2h, q2l, q2.scale_ptr, _lib.stream())
+    d.q2, d.fac = q2, fac
+
+
+def factored_snp_planes(a: Planes, bt: Planes, mc: int, fac: Factored) -> tuple[Planes, Planes]:
+    """SNP-side planes for factored weights: S1 = x and S2 = (x o x) A, x = the rotated SNPs (rows of a)."""
+    s1, x2 = rotate_us_x2(a, bt)
+    c = torch.empty(a.rows_pad, fac.rpad, dtype=torch.float64, device=x2.device)
+    _lib.call("flmm_gemm_f64", x2.data_ptr(), a.rows_pad, bt.rows_pad, fac.A.data_ptr(), fac.rpad, c.data_ptr(),
+              _lib.stream())
+    s2 = pack(a.prec, c, mc, fac.rpad, fac.rpad, 1, a.rows_pad, fac.rpad)
+    return s1, s2
+
+
+def delta_grid(uy: torch.Tensor, P: int, p_pad: int, kpad: int, s: torch.Tensor, s_lam: torch.Tensor,
+               nco: float, n_grid: int, lo: float, hi: float) -> tuple[torch.Tensor, torch.Tensor]:
+    """Per-column FP32 grid argmin (int32, (P,)) and the FP64 log-det term at every grid node (the refinement's
+    FP64 walk to the grid minimum uses it)."""
     if n_grid > TILE:
         raise ValueError(f"n_grid <= {TILE} is supported, got {n_grid}")
     dev = uy.device
     wplane = torch.empty(TILE * kpad, dtype=torch.float32, device=dev)
     lam = torch.empty(TILE, dtype=torch.float32, device=dev)
+    lam64 = torch.empty(TILE, dtype=torch.float64, device=dev)
     idx = torch.empty(max(P, 1), dtype=torch.int32, device=dev)
     st = _lib.stream()
     _lib.call("flmm_delta_weights", s.data_ptr(), s.numel(), s_lam.data_ptr(), s_lam.numel(), kpad, n_grid, lo, hi,
-              wplane.data_ptr(), lam.data_ptr(), st)
+              wplane.data_ptr(), lam.data_ptr(), lam64.data_ptr(), st)
     _lib.call("flmm_delta_grid", wplane.data_ptr(), lam.data_ptr(), n_grid, float(nco), uy.data_ptr(), P, p_pad, kpad,
               idx.data_ptr(), st)
-    return idx[:P]
+    return idx[:P], lam64
+
+
+def delta_grid_idx(uy: torch.Tensor, P: int, p_pad: int, kpad: int, s: torch.Tensor, s_lam: torch.Tensor,
+                   nco: float, n_grid: int, lo: float, hi: float) -> torch.Tensor:
+    """Per-column FP32 grid argmin (int32, (P,))."""
+    return delta_grid(uy, P, p_pad, kpad, s, s_lam, nco, n_grid, lo, hi)[0]
 
 
 def delta_refine(prec: int, mode: int, uy: torch.Tensor, P: int, p_pad: int, kpad: int, s: torch.Tensor,
                  s_lam: torch.Tensor, nco: float, n_grid: int, lo: float, hi: float, idx: torch.Tensor | None,
-                 log_delta_in: torch.Tensor | None = None) -> DeltaResult:
-    """Refine (or, for DELTA_FIXED, take log_delta_in) and build the column-side scan planes."""
+                 log_delta_in: torch.Tensor | None = None, fac: Factored | None = None,
+                 lam64: torch.Tensor | None = None) -> DeltaResult:
+    """Refine (or, for DELTA_FIXED, take log_delta_in) and build the column-side scan planes (with fac, the
+    factored Q2 instead of the exact one); lam64: delta_grid's FP64 log-det table for the same grid."""
     dev = uy.device
     log_delta = torch.empty(max(P, 1), dtype=torch.float64, device=dev)
     if log_delta_in is not None:
@@ -216,13 +292,18 @@ def delta_refine(prec: int, mode: int, uy: torch.Tensor, P: int, p_pad: int, kpa
     rWr = torch.empty(max(P, 1), dtype=torch.float64, device=dev)
     h2 = torch.empty(max(P, 1), dtype=torch.float64, device=dev)
     q1 = Planes(prec, p_pad, kpad, dev)
-    q2 = Planes(prec, p_pad, kpad, dev)
+    q2 = Planes(prec, p_pad, kpad, dev) if fac is None else None
     q1h, q1l = q1.ptrs
-    q2h, q2l = q2.ptrs
+    q2h, q2l = q2.ptrs if q2 is not None else (None, None)
     _lib.call("flmm_delta_refine", prec, mode, uy.data_ptr(), P, p_pad, kpad, s.data_ptr(), s.numel(),
-              s_lam.data_ptr(), s_lam.numel(), float(nco), n_grid, lo, hi, _lib.ptr(idx), log_delta.data_ptr(),
-              rWr.data_ptr(), h2.data_ptr(), q1h, q1l, q2h, q2l, q1.scale_ptr, q2.scale_ptr, _lib.stream())
-    return DeltaResult(log_delta[:P], rWr[:P], h2[:P], q1, q2, idx, int(s.numel()))
+              s_lam.data_ptr(), s_lam.numel(), float(nco), n_grid, lo, hi, _lib.ptr(idx), _lib.ptr(lam64),
+              log_delta.data_ptr(),
+              rWr.data_ptr(), h2.data_ptr(), q1h, q1l, q2h, q2l, q1.scale_ptr, q2.scale_ptr if q2 else None,
+              _lib.stream())
+    d = DeltaResult(log_delta[:P], rWr[:P], h2[:P], q1, q2, idx, int(s.numel()))
+    if fac is not None:
+        attach_factored(prec, d, fac)
+    return d
 
 
 class _ScanTimer:
@@ -236,14 +317,23 @@ class _ScanTimer:
         self.enabled = enabled
         self.launches = []
 
+    def record(self, ev, tests: int, ref_pt: float, alg_pt: float, kernel_pt: float, R: int = 0) -> None:
+        """One launch: its events, Wald tests, and flops per test of the reference algorithm (ref_pt), of the
+        launched algorithm (alg_pt) and over the padded K the kernel multiplies (kernel_pt); R > 0 for factored
+        weights (the exponential sum's term count)."""
+        self.launches.append((ev[0], ev[1], tests, ref_pt, alg_pt, kernel_pt, R))
+
     def summary(self) -> dict:
-        """Launch count, total / mean kernel ms, Wald tests and algorithmic compat flops (4 Neff per test)."""
+        """Launch count, total / mean kernel ms, Wald tests and the three flop totals (ref_flops: the reference
+        algorithm, 4 Neff per compat test; alg_flops: what the launches compute, 2 Neff + 2 R per test with
+        factored weights; kernel_flops: the same over the padded K)."""
         torch.cuda.synchronize()
         ms = [a.elapsed_time(b) for a, b, *_ in self.launches]
-        tests = sum(t for _, _, t, _, _ in self.launches)
-        flops = sum(4.0 * neff * t for _, _, t, _, neff in self.launches)
+        tot = lambda i: float(sum(r[2] * r[i] for r in self.launches))  # noqa: E731
         return {"launches": len(ms), "total_ms": float(sum(ms)), "mean_ms": float(np.mean(ms)) if ms else 0.0,
-                "tests": int(tests), "alg_flops": float(flops)}
+                "tests": int(sum(r[2] for r in self.launches)), "ref_flops": tot(3), "alg_flops": tot(4),
+                "kernel_flops": tot(5), "factored": any(r[6] for r in self.launches),
+                "R": max((r[6] for r in self.launches), default=0)}
 
 
 scan_timer = _ScanTimer()
@@ -301,6 +391,12 @@ class ScanOutputs:
         return ScanOutputs(z(), z(), z(), z(), z(), torch.full((P,), float("-inf"), dtype=torch.float64, device=dev))
 
 
+def refresh_real_max(out: ScanOutputs, n_real: int) -> None:
+    """max_F of the real columns as the column maxima of their (re-scored) per-SNP F."""
+    if n_real > 0 and out.f.shape[0] > 0:
+        out.max_F[:n_real] = out.f.max(dim=0).values
+
+
 def wald_scan(s1: Planes, s2: Planes, d: DeltaResult, mc: int, P: int, n_real: int, nu: float,
               row_index: torch.Tensor, out: ScanOutputs, pmax: torch.Tensor | None = None) -> None:
     """Scan one SNP block against every column, update out (real-column rows and the running max_F)."""
@@ -323,10 +419,18 @@ def wald_scan(s1: Planes, s2: Planes, d: DeltaResult, mc: int, P: int, n_real: i
     if scan_timer.enabled:
         ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
         ev[0].record()
-    _lib.call("flmm_wald_scan", s1.prec, s1h, s1l, s2h, s2l, s1.scale_ptr, s2.scale_ptr, mc_pad, q1h, q1l, q2h, q2l,
-              d.q1.scale_ptr, d.q2.scale_ptr, p_pad, s1.kpad, mc, P, n_real, d.rWr.data_ptr(), d.h2.data_ptr(),
-              float(nu), row_index.data_ptr(), *optr, pmax.data_ptr(), st)
+    if d.fac is not None:  # factored weights: sWs over the first fac.rpad entries (S2 / Q2 hold K = rpad)
+        if s2.kpad != d.fac.rpad or d.q2.kpad != d.fac.rpad:
+            raise ValueError("factored scan: the SNP planes were not built for these weights")
+        _lib.call("flmm_wald_scan_lowrank", s1.prec, s1h, s1l, s2h, s2l, s1.scale_ptr, s2.scale_ptr, mc_pad, q1h, q1l,
+                  q2h, q2l, d.q1.scale_ptr, d.q2.scale_ptr, p_pad, s1.kpad, d.fac.rpad, mc, P, n_real,
+                  d.rWr.data_ptr(), d.h2.data_ptr(), float(nu), row_index.data_ptr(), *optr, pmax.data_ptr(), st)
+    else:
+        _lib.call("flmm_wald_scan", s1.prec, s1h, s1l, s2h, s2l, s1.scale_ptr, s2.scale_ptr, mc_pad, q1h, q1l, q2h,
+                  q2l, d.q1.scale_ptr, d.q2.scale_ptr, p_pad, s1.kpad, mc, P, n_real, d.rWr.data_ptr(),
+                  d.h2.data_ptr(), float(nu), row_index.data_ptr(), *optr, pmax.data_ptr(), st)
     if ev is not None:
         ev[1].record()
-        scan_timer.launches.append((ev[0], ev[1], mc * P, s1.kpad, d.neff))
+        k2, r2 = (d.fac.rpad, d.fac.R) if d.fac else (s1.kpad, d.neff)
+        scan_timer.record(ev, mc * P, 4.0 * d.neff, 2.0 * (d.neff + r2), 2.0 * (s1.kpad + k2), d.fac.R if d.fac else 0)
     _lib.call("flmm_max_reduce", pmax.data_ptr(), n_mt, p_pad, P, out.max_F.data_ptr(), st)

@@ -15,9 +15,11 @@ The tile variant (128 x 128 square tiles or 128 x 256 wide tiles) and the persis
 from __future__ import annotations
 
 import ctypes as C
+import functools
 import json
 import os
 from dataclasses import dataclass
+from typing import Callable
 
 import numpy as np
 import torch
@@ -42,10 +44,23 @@ class ScanJob:
     row_index: Tensor  # (mc,) int64: output row of each SNP
     out: ScanOutputs
     pmax: Tensor | None = None  # per-tile maxima scratch, allocated by GroupedScan
+    finish: Callable[[], None] | None = None  # run after the scan (the FP64 re-scoring of large-F real tests)
+
+    @property
+    def factored(self) -> bool:
+        return self.d.fac is not None
 
     def tiles(self, variant: int) -> int:
-        tp = W_TP if variant == _lib.SCAN_WIDE else TILE
+        tp = TILE if variant == _lib.SCAN_SQUARE else W_TP
         return -(-self.mc // TILE) * -(-self.P // tp)
+
+
+def _factored(jobs: list[ScanJob]) -> bool:
+    """Whether the jobs use factored weights (all of them) or exact ones (none); a mix cannot share a launch."""
+    f = {j.factored for j in jobs}
+    if len(f) > 1:
+        raise ValueError("a grouped launch needs every job with factored weights or none")
+    return f == {True}
 
 
 def _descriptor(j: ScanJob) -> _lib.ScanGroup:
@@ -58,6 +73,7 @@ def _descriptor(j: ScanJob) -> _lib.ScanGroup:
     g.q2_hi, g.q2_lo = j.d.q2.ptrs
     g.q1_scale, g.q2_scale = j.d.q1.scale_ptr, j.d.q2.scale_ptr
     g.p_pad, g.kpad, g.mc, g.P, g.n_real = j.d.q1.rows_pad, j.s1.kpad, j.mc, j.P, j.n_real
+    g.kpad2 = j.d.fac.rpad if j.factored else 0
     g.rWr, g.h2, g.nu = j.d.rWr.data_ptr(), j.d.h2.data_ptr(), float(j.nu)
     g.row_index = j.row_index.data_ptr()
     if j.n_real > 0:
@@ -71,9 +87,10 @@ def _descriptor(j: ScanJob) -> _lib.ScanGroup:
 class ScanTuner:
     """Project-written autotuner of the grouped scan: (tile variant, persistent grid size) per table shape.
 
-    The shape key is (precision, group count, the tile counts of both variants, the largest K).  Candidates are
-    the square and wide variants on 1, 2 or 4 workgroups per CU (256 CUs) and one workgroup per tile; each is
-    run once to warm up and ``reps`` times under HIP events on the launch stream, and the fastest mean wins."""
+    The shape key is (precision, group count, the tile counts of both variants, the largest K, factored or exact
+    weights).  Candidates are the square and wide variants (factored weights: the factored wide variant only) on
+    1, 2 or 4 workgroups per CU (256 CUs) and one workgroup per tile; each is run once to warm up and ``reps``
+    times under HIP events on the launch stream, and the fastest mean wins."""
 
     CUS = 256
 
@@ -90,11 +107,12 @@ class ScanTuner:
     def key(prec: int, jobs: list[ScanJob]) -> str:
         kmax = max(j.s1.kpad for j in jobs)
         return f"p{prec}-g{len(jobs)}-sq{sum(j.tiles(_lib.SCAN_SQUARE) for j in jobs)}-" \
-               f"w{sum(j.tiles(_lib.SCAN_WIDE) for j in jobs)}-k{kmax}"
+               f"w{sum(j.tiles(_lib.SCAN_WIDE) for j in jobs)}-k{kmax}-f{int(_factored(jobs))}"
 
     def candidates(self, jobs: list[ScanJob]) -> list[tuple[int, int]]:
         out = []
-        for variant in (_lib.SCAN_WIDE, _lib.SCAN_SQUARE):
+        variants = (_lib.SCAN_WIDE_FACTORED,) if _factored(jobs) else (_lib.SCAN_WIDE, _lib.SCAN_SQUARE)
+        for variant in variants:
             total = sum(j.tiles(variant) for j in jobs)
             per_tile = engine.rup(max(total, 8), 8)
             for blocks in sorted({min(self.CUS * f, per_tile) for f in (1, 2, 4)} | {per_tile}):
@@ -142,8 +160,10 @@ class GroupedScan:
     ``run(jobs)`` writes the table (pinned host buffer -> device buffer of fixed address), asks the tuner for
     the variant and grid size, and launches the persistent scan and the per-group maximum fold -- by replaying a
     captured HIP graph when ``graph`` is set (captured on first use for each variant / grid size, with room for
-    ``max_groups`` groups of at most ``max_P`` columns) or eagerly otherwise.  Each job's pmax scratch is
-    allocated here.  The outputs of a job are complete once the launch stream reaches the end of ``run``."""
+    ``max_groups`` groups of at most ``max_P`` columns) or eagerly otherwise.  Jobs with factored weights and
+    jobs with exact ones (a group whose Neff <= 128 keeps the exact weights) run as two such launches, each with
+    its own table buffer.  Each job's pmax scratch is allocated here.  The outputs of a job are complete once the
+    launch stream reaches the end of ``run``."""
 
     def __init__(self, prec: int, max_groups: int, max_P: int, *, graph: bool = True, tuner: ScanTuner | None = None,
                  variant: int | None = None, blocks: int | None = None):
@@ -158,26 +178,36 @@ class GroupedScan:
         self.fixed = (variant, blocks) if variant is not None else None
         if self.fixed is not None and (blocks is None or blocks <= 0 or blocks % 8):
             raise ValueError("a fixed grid size must be a positive multiple of 8")
-        nbytes = int(_lib.load().flmm_scan_table_bytes(self.max_groups))
-        self._host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
-        self._table = torch.empty(nbytes, dtype=torch.uint8, device=self.dev)
-        self._copied = None  # event after the last host -> device table copy
+        self._nbytes = int(_lib.load().flmm_scan_table_bytes(self.max_groups))
+        self._bufs: dict[bool, list] = {}  # factored? -> [pinned host table, device table, copy event]
         self._graphs: dict[tuple[int, int], _Graph] = {}
         self.last_choice: tuple[int, int] | None = None
+        self.last_choices: list[tuple[int, int]] = []
 
     # table ---------------------------------------------------------------------------------------------------
+    def _buf(self, variant: int) -> list:
+        kind = variant == _lib.SCAN_WIDE_FACTORED
+        b = self._bufs.get(kind)
+        if b is None:
+            b = [torch.empty(self._nbytes, dtype=torch.uint8, pin_memory=True),
+                 torch.empty(self._nbytes, dtype=torch.uint8, device=self.dev), None]
+            self._bufs[kind] = b
+        return b
+
     def _write_table(self, variant: int, jobs: list[ScanJob]) -> Tensor:
         if len(jobs) > self.max_groups:
             raise ValueError(f"{len(jobs)} groups exceed the runner's max_groups={self.max_groups}")
-        if self._copied is not None:
-            self._copied.synchronize()  # the previous copy has read the pinned buffer
+        b = self._buf(variant)
+        host, table, copied = b
+        if copied is not None:
+            copied.synchronize()  # the previous copy has read the pinned buffer
         arr = (_lib.ScanGroup * len(jobs))(*(_descriptor(j) for j in jobs))
         _lib.call("flmm_scan_table_fill", self.prec, variant, C.addressof(arr) if jobs else None, len(jobs),
-                  self._host.data_ptr())
-        self._table.copy_(self._host, non_blocking=True)
-        self._copied = torch.cuda.Event()
-        self._copied.record()
-        return self._table
+                  host.data_ptr())
+        table.copy_(host, non_blocking=True)
+        b[2] = torch.cuda.Event()
+        b[2].record()
+        return table
 
     def _launch_scan(self, variant: int, blocks: int, table: Tensor) -> None:
         _lib.call("flmm_wald_scan_grouped", self.prec, variant, table.data_ptr(), int(blocks), _lib.stream())
@@ -189,17 +219,18 @@ class GroupedScan:
         key = (variant, blocks)
         g = self._graphs.get(key)
         if g is None:
+            table = self._buf(variant)[1]
             # eager launches first (code-object load happens outside the capture); both are idempotent on the
             # table already written: the scan rewrites the same outputs and the fold is a maximum
-            self._launch_scan(variant, blocks, self._table)
-            self._launch_reduce(self._table, self.max_groups, self.max_P)
+            self._launch_scan(variant, blocks, table)
+            self._launch_reduce(table, self.max_groups, self.max_P)
             graph = torch.cuda.CUDAGraph()
             side = torch.cuda.Stream()
             side.wait_stream(torch.cuda.current_stream())
             with torch.cuda.stream(side):
                 with torch.cuda.graph(graph, stream=side):
-                    self._launch_scan(variant, blocks, self._table)
-                    self._launch_reduce(self._table, self.max_groups, self.max_P)
+                    self._launch_scan(variant, blocks, table)
+                    self._launch_reduce(table, self.max_groups, self.max_P)
             torch.cuda.current_stream().wait_stream(side)
             g = _Graph(graph, variant, blocks, self.max_groups, self.max_P)
             self._graphs[key] = g
@@ -219,7 +250,12 @@ class GroupedScan:
 
     def choose(self, jobs: list[ScanJob]) -> tuple[int, int]:
         if self.fixed is not None:
-            return self.fixed
+            variant, blocks = self.fixed
+            if _factored(jobs):  # the wide tile with its second product over kpad2
+                if variant == _lib.SCAN_SQUARE:
+                    raise ValueError("factored weights run on the wide tile only")
+                variant = _lib.SCAN_WIDE_FACTORED
+            return variant, blocks
         return self.tuner.choose(self, jobs)
 
     def run(self, jobs: list[ScanJob]) -> None:
@@ -227,14 +263,19 @@ class GroupedScan:
         if not jobs:
             return
         self.prepare(jobs)
-        variant, blocks = self.choose(jobs)
-        self.last_choice = (variant, blocks)
-        table = self._write_table(variant, jobs)
-        if self.use_graph:
-            self._graph_for(variant, blocks).graph.replay()
-        else:
-            self._launch_scan(variant, blocks, table)
-            self._launch_reduce(table, len(jobs), max(j.P for j in jobs))
+        self.last_choices = []
+        for part in ([j for j in jobs if j.factored], [j for j in jobs if not j.factored]):
+            if not part:
+                continue
+            variant, blocks = self.choose(part)
+            self.last_choices.append((variant, blocks))
+            table = self._write_table(variant, part)
+            if self.use_graph:
+                self._graph_for(variant, blocks).graph.replay()
+            else:
+                self._launch_scan(variant, blocks, table)
+                self._launch_reduce(table, len(part), max(j.P for j in part))
+        self.last_choice = self.last_choices[0]
 
 
 def per_group_loop(jobs: list[ScanJob]) -> None:
@@ -305,13 +346,14 @@ class GroupedLoco:
                     zs = self._std_rows(self.Zk, kc, g)
                     self.K_all[g].addmm_(zs, zs.T)
                     del zs
-        self.Iperps, self.ypls = [], []
+        self.Iperps, self.ypls, self.y_reals = [], [], []
         with engine.stage("pack_columns"):
             for g in range(G):
                 Xg = X.index_select(0, self.rows[g])
                 Xpinv, Iperp = core._projector(Xg)
                 self.Iperps.append(Iperp)
                 Y_r = (Yg[g] - Xg @ (Xpinv @ Yg[g])).contiguous()
+                self.y_reals.append(Y_r[:, : self.n_reals[g]].contiguous())
                 self.ypls.append(engine.pack(prec, Y_r, self.Ps[g], self.Ns[g], 1, self.Ps[g],
                                              engine.rup(max(self.Ps[g], 1), engine.PTILE),
                                              engine.rup(self.Ns[g], engine.KCHUNK)))
@@ -364,7 +406,7 @@ class GroupedLoco:
                 with engine.stage("pack_basis"):
                     st = core._Basis(s, U[i, :N, :], lo, N - D, self.prec, nu=N - D - 1)
                 with engine.stage("rotate_columns"):
-                    st.use_column_planes(self.ypls[g], self.Ps[g])
+                    st.use_column_planes(self.ypls[g], self.Ps[g], self.y_reals[g])
                 with engine.stage("delta"):
                     _, d = st.fit(self.n_grid, -10.0, 10.0, True, projected_loss=True)
                 out[g] = (st, d)
@@ -385,14 +427,23 @@ class GroupedLoco:
             with engine.stage("snps"):
                 for g, (st, d) in enumerate(bases):
                     zs = self._std_rows(self.Z, ri, g)
-                    s1, s2 = st.snp_planes(zs, local)
+                    s1, s2 = st.snp_planes(zs, local, d)
                     del zs
-                    jobs.append(ScanJob(s1, s2, d, ri.numel(), self.Ps[g], self.n_reals[g], st.nu, ri, self.outs[g]))
+                    jobs.append(ScanJob(s1, s2, d, ri.numel(), self.Ps[g], self.n_reals[g], st.nu, ri, self.outs[g],
+                                        finish=functools.partial(self._rescore, st, d, ri, local, g)))
             yield jobs
             del jobs
 
+    def _rescore(self, st, d, ri: Tensor, local: Tensor, g: int) -> None:
+        """FP64 re-scoring of the block's large-F real tests of group g (core._Basis.rescore); only the selected
+        SNPs are standardised again."""
+        st.rescore(lambda idx: self._std_rows(self.Z, ri.index_select(0, idx), g), local, ri, self.n_reals[g],
+                   self.outs[g], d)
+
     def results(self) -> list:
         from . import core
+        for o, n in zip(self.outs, self.n_reals):
+            engine.refresh_real_max(o, n)
         res = [core.ScanResult(o.f, o.beta, o.se, o.sfve, o.nullh2, o.max_F) for o in self.outs]
         return [core._home(r, self.home) for r in res]
 
@@ -422,6 +473,10 @@ def loco_scan_compat_groups(Z: Tensor, X: Tensor, Ys: list, keeps: list, chrom, 
                     runner.run(jobs)
                 else:
                     per_group_loop(jobs)
+            with engine.stage("rescore"):
+                for j in jobs:
+                    if j.finish is not None:
+                        j.finish()
         if on_chrom is not None:
             on_chrom(ci + 1, len(chroms))
     return gl.results()

@@ -171,7 +171,7 @@ def refine_learned(basis, net: DeltaMLP, n_grid: int, lo: float, hi: float, *, n
                    radius: float = 1.0, judge_tol: float = 1e-6, projected_loss: bool = True) -> LearnedFit:
     """Learned delta for every column of a core._Basis: grid (judge) + MLP proposal + judged Newton."""
     s_lam, nco = basis._loss(projected_loss)
-    idx = engine.delta_grid_idx(basis.uy, basis.P, basis.p_pad, basis.kpad, basis.s, s_lam, nco, n_grid, lo, hi)
+    idx, lam64 = engine.delta_grid(basis.uy, basis.P, basis.p_pad, basis.kpad, basis.s, s_lam, nco, n_grid, lo, hi)
     feat = features(basis.uy, basis.P, basis.p_pad, basis.s, n, c)
     t0 = net.predict(feat)
     dev = basis.uy.device
@@ -187,7 +187,8 @@ def refine_learned(basis, net: DeltaMLP, n_grid: int, lo: float, hi: float, *, n
     q2h, q2l = q2.ptrs
     _lib.call("flmm_delta_refine_learned", basis.prec, basis.uy.data_ptr(), P, p_pad, kpad, basis.s.data_ptr(),
               basis.s.numel(), s_lam.data_ptr(), s_lam.numel(), float(nco), n_grid, float(lo), float(hi),
-              idx.data_ptr(), float(radius), float(judge_tol), log_delta.data_ptr(), fb.data_ptr(), rWr.data_ptr(),
+              idx.data_ptr(), lam64.data_ptr(), float(radius), float(judge_tol), log_delta.data_ptr(), fb.data_ptr(),
+              rWr.data_ptr(),
               h2.data_ptr(), q1h, q1l, q2h, q2l, q1.scale_ptr, q2.scale_ptr, _lib.stream())
     d = engine.DeltaResult(log_delta[:P], rWr[:P], h2[:P], q1, q2, idx, int(basis.s.numel()))
     basis.delta = d
@@ -265,9 +266,9 @@ def make_training_set(spectra: list[torch.Tensor], n_pheno: int, *, n_grid: int 
             uy = simulate_columns(s, hh, gen)
             b = basis_from_columns(s, uy, prec)
             del uy
-            idx = engine.delta_grid_idx(b.uy, b.P, b.p_pad, b.kpad, s, s, float(b.neff), n_grid, -10.0, 10.0)
+            idx, lam64 = engine.delta_grid(b.uy, b.P, b.p_pad, b.kpad, s, s, float(b.neff), n_grid, -10.0, 10.0)
             d = engine.delta_refine(prec, _lib.DELTA_NEWTON, b.uy, b.P, b.p_pad, b.kpad, s, s, float(b.neff), n_grid,
-                                    -10.0, 10.0, idx)
+                                    -10.0, 10.0, idx, lam64=lam64)
             feats.append(features(b.uy, b.P, b.p_pad, s, b.neff + 1, 1))
             labels.append(d.log_delta.float())
     return torch.cat(feats)[:n_pheno], torch.cat(labels)[:n_pheno]

@@ -196,9 +196,10 @@ class _LowRankState:
             self.last_fallback = fit.fallback
             d = fit.delta
         else:
-            idx = engine.delta_grid_idx(self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam, nco, n_grid, lo, hi)
+            idx, lam64 = engine.delta_grid(self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam, nco, n_grid, lo,
+                                           hi)
             d = engine.delta_refine(self.prec, engine.delta_code(), self.uy, self.P, self.p_pad, self.kpad, self.s,
-                                    s_lam, nco, n_grid, lo, hi, idx)
+                                    s_lam, nco, n_grid, lo, hi, idx, lam64=lam64)
         self.delta = d
         return d
 
@@ -343,10 +344,7 @@ def rescore_lowrank(st: _LowRankState, cp: _ColumnPlanes, S_r: Tensor, US: Tenso
         out.sfve[rows, p] = torch.clamp(a * a / (d * rWr), min=0.0).sqrt()
 
 
-def refresh_real_max(out: engine.ScanOutputs, n_real: int) -> None:
-    """max_F of the real columns from their (re-scored) per-SNP F."""
-    if n_real > 0 and out.f.shape[0] > 0:
-        out.max_F[:n_real] = out.f.max(dim=0).values
+refresh_real_max = engine.refresh_real_max  # max_F of the real columns from their (re-scored) per-SNP F
 
 
 def snp_planes_lowrank(st: _LowRankState, U: Tensor, S_r: Tensor):
@@ -385,7 +383,8 @@ def wald_scan_lowrank(s1: engine.Planes, s2: engine.Planes, cp: _ColumnPlanes, m
     if ev is not None:
         ev[1].record()
         # algorithmic flops: on-rank 4 k per test, off-rank cross term 2 N per test
-        engine.scan_timer.launches.append((ev[0], ev[1], mc * st.P, s1.kpad, st.k + st.N // 2))
+        alg = 4.0 * st.k + 2.0 * st.N
+        engine.scan_timer.record(ev, mc * st.P, alg, alg, 2.0 * (s1.kpad + s2.kpad))
     _lib.call("flmm_max_reduce", pmax.data_ptr(), n_mt, st.p_pad, st.P, out.max_F.data_ptr(), stream)
 
 

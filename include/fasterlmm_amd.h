@@ -30,16 +30,16 @@ extern "C" {
 
 enum flmm_status { FLMM_OK = 0, FLMM_ERR_ARG = 1, FLMM_ERR_HIP = 2 };
 enum flmm_precision { FLMM_PREC_SPLIT_BF16 = 0, FLMM_PREC_FP32 = 1, FLMM_PREC_SPLIT_FP16 = 2 };
-enum flmm_rotate_mode { FLMM_ROTATE_UY = 0, FLMM_ROTATE_US = 1 };
+enum flmm_rotate_mode { FLMM_ROTATE_UY = 0, FLMM_ROTATE_US = 1, FLMM_ROTATE_US_X2 = 2 };
 enum flmm_dtype { FLMM_DTYPE_F64 = 0, FLMM_DTYPE_I8 = 1 };
 enum flmm_delta_mode { FLMM_DELTA_GOLDEN = 0, FLMM_DELTA_NEWTON = 1, FLMM_DELTA_FIXED = 2, FLMM_DELTA_GRID = 4 };
 enum flmm_bed_out { FLMM_BED_I8 = 0, FLMM_BED_F64 = 1, FLMM_BED_F32 = 2 };
-enum flmm_scan_variant { FLMM_SCAN_SQUARE = 0, FLMM_SCAN_WIDE = 1 };
+enum flmm_scan_variant { FLMM_SCAN_SQUARE = 0, FLMM_SCAN_WIDE = 1, FLMM_SCAN_WIDE_FACTORED = 2 };
 
 /* One scan of a grouped launch (flmm_wald_scan_grouped): the arguments of flmm_wald_scan for one SNP block
  * against one column set, plus that column set's per-tile maxima (ceil(mc / 128) x p_pad doubles, scratch of the
  * launch) and its running genome-wide maximum max_F (P doubles, updated by flmm_max_reduce_grouped).  All
- * pointers are device pointers; the layout is fixed (29 eight-byte fields). */
+ * pointers are device pointers; the layout is fixed (30 eight-byte fields). */
 typedef struct flmm_scan_group {
   const void* s1_hi;
   const void* s1_lo;
@@ -56,6 +56,7 @@ typedef struct flmm_scan_group {
   const float* q2_scale;
   int64_t p_pad;
   int64_t kpad;
+  int64_t kpad2; /* 0; FLMM_SCAN_WIDE_FACTORED: the second product runs over the first kpad2 < kpad */
   int64_t mc;
   int64_t P;
   int64_t n_real;
@@ -82,7 +83,8 @@ const char* flmm_last_error(void);
  * with one loco_scan_compat each: fasterlmm/cli.py:337-375, fasterlmm/perms.py:62).  The launch reads a device
  * table: flmm_scan_table_bytes(n) bytes, filled on the host by flmm_scan_table_fill (which validates every
  * group as flmm_wald_scan does, for the tile variant FLMM_SCAN_SQUARE (128 x 128) or FLMM_SCAN_WIDE
- * (128 x 256, needs p_pad % 256 == 0)) and copied to the device by the caller.  flmm_wald_scan_grouped runs every
+ * (128 x 256, needs p_pad % 256 == 0), or as flmm_wald_scan_lowrank does for FLMM_SCAN_WIDE_FACTORED (every group
+ * with 0 < kpad2 < kpad: factored weights)) and copied to the device by the caller.  flmm_wald_scan_grouped runs every
  * tile of every group on a persistent grid of `blocks` workgroups (a positive multiple of 8); it takes nothing
  * but the table, so a captured graph of it replays after the table is rewritten with the same variant and
  * precision.  flmm_max_reduce_grouped then folds each group's per-tile maxima into its max_F (grid sized by
@@ -133,17 +135,21 @@ int flmm_pack_planes(int prec, const double* src, int64_t n_rows, int64_t n_k, i
  * (core.py:629).  a_scale / b_scale are the operands' row scales (split-FP16 only).
  *   FLMM_ROTATE_UY: out0 = one fp32 plane (rows a_rows_pad, K = b_rows_pad) of x.
  *   FLMM_ROTATE_US: the scan's SNP-side planes S1 = x (out0, out1) and S2 = x^2 (out2, out3), with row scales
- *                   out_scale1 / out_scale2 (split-FP16); for FLMM_PREC_FP32 out0 = x, out1 = x^2. */
+ *                   out_scale1 / out_scale2 (split-FP16); for FLMM_PREC_FP32 out0 = x, out1 = x^2.
+ *   FLMM_ROTATE_US_X2: S1 = x as FLMM_ROTATE_US (out0, out1; out_scale1) and out2 = x^2 as FP64, row-major
+ *                   (a_rows_pad x b_rows_pad), from which the caller forms the factored-weight S2 = x^2 A
+ *                   (flmm_factored_weight_planes). */
 int flmm_rotate(int prec, int mode, const void* a_hi, const void* a_lo, const float* a_scale, int64_t a_rows_pad,
                 const void* b_hi, const void* b_lo, const float* b_scale, int64_t b_rows_pad, int64_t n_pad,
                 void* out0, void* out1, void* out2, void* out3, float* out_scale1, float* out_scale2, void* stream);
 
 /* Grid weights for the delta search: fp32 plane wplane (128 rows x kpad) with w[g][k] = 1/(s_k + delta_g) for
  * g < n_grid, k < neff (0 elsewhere), and lam[g] = sum_j log(s_lam_j + delta_g) over the nlam-long log-det
- * spectrum (128 floats), on the grid linspace(lo, hi, n_grid) in log delta (core.py:238, 569).  n_grid <= 128.
+ * spectrum (128 floats; lam64, if not NULL, the same in FP64), on the grid linspace(lo, hi, n_grid) in log delta
+ * (core.py:238, 569).  n_grid <= 128.
  * The compat loss uses s_lam = s (core.py:539); the GLS path's ML loss uses the full spectrum (core.py:177). */
 int flmm_delta_weights(const double* s, int64_t neff, const double* s_lam, int64_t nlam, int64_t kpad, int n_grid,
-                       double lo, double hi, float* wplane, float* lam, void* stream);
+                       double lo, double hi, float* wplane, float* lam, double* lam64, void* stream);
 
 /* Grid argmin of the profile loss nco * log(sum_k w_gk y_k^2) + lam[g] (core.py:525-542, 569-571; nco = Neff for
  * the compat path) for columns 0..P-1 of the fp32 UY plane (rows p_pad, K = kpad); idx[p] = first grid index of
@@ -151,18 +157,53 @@ int flmm_delta_weights(const double* s, int64_t neff, const double* s_lam, int64
 int flmm_delta_grid(const float* wplane, const float* lam, int n_grid, double nco, const float* uy, int64_t P,
                     int64_t p_pad, int64_t kpad, int32_t* idx, void* stream);
 
-/* Refine log delta in [grid[idx-1], grid[idx+1]] (core.py:575-595; FLMM_DELTA_GOLDEN replays the reference's 50
+/* Refine log delta in [grid[i-1], grid[i+1]] (core.py:575-595), i = the FP64 grid minimum reached from the FP32 grid
+ * argmin idx (the reference brackets the FP64 argmin; FP32 ties on flat likelihoods are resolved in FP64, each
+ * step using lam64 -- the FP64 log-det table of flmm_delta_weights -- or, if NULL, per-column sums;
+ * FLMM_DELTA_GOLDEN replays the reference's 50
  * golden-section steps, FLMM_DELTA_NEWTON converges to the same local minimiser by safeguarded Newton;
  * FLMM_DELTA_FIXED reads log_delta[p] as given and skips the search; FLMM_DELTA_GRID returns the grid node of
  * the first minimum of the FP64 loss, as refine=False does (core.py:570-573); idx may be NULL for both), then
  * write log_delta[p], rWr[p] = max(sum w y^2, 1e-300) (core.py:641-643), h2[p] = 1/(1+delta) (core.py:668-669)
  * and the scan's column-side planes Q1 = w o y (q1_hi, q1_lo) and Q2 = w (q2_hi, q2_lo), rows p_pad, with row
- * scales q1_scale / q2_scale (split-FP16).  The loss is nco * (log 2pi + log(R / nco) + 1) + sum_j
+ * scales q1_scale / q2_scale (split-FP16); q2_hi == NULL skips Q2 (factored weights).  The loss is nco * (log 2pi + log(R / nco) + 1) + sum_j
  * log(s_lam_j + delta). */
 int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_pad, int64_t kpad, const double* s,
                       int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo, double hi,
-                      const int32_t* idx, double* log_delta, double* rWr, double* h2, void* q1_hi, void* q1_lo,
-                      void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale, void* stream);
+                      const int32_t* idx, const double* lam64, double* log_delta, double* rWr, double* h2,
+                      void* q1_hi, void* q1_lo, void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale,
+                      void* stream);
+
+/* Factored weights (no counterpart in the reference, whose sWs = w^T (US o US) runs over all Neff eigenvalues,
+ * core.py:640-651).  With a positive exponential sum 1/z = sum_{r<R} lam_r^2 exp(-t_r z) (relative accuracy
+ * stated by the caller on the scan's range of z = s_k + delta_p),
+ *   sWs[m][p] = sum_k US[m][k]^2 / (s_k + delta_p) = sum_r C[m][r] B[p][r],
+ *   C = (US o US) A, A[k][r] = lam_r exp(-t_r s_k),   B[p][r] = lam_r exp(-t_r delta_p),
+ * so the scan's second product runs over rpad ~ 128 instead of Neff (flmm_wald_scan_lowrank with kpad2 = rpad;
+ * S2 = planes of C, e.g. from FLMM_ROTATE_US_X2 and flmm_pack_planes).  This call writes B (delta_p =
+ * exp(log_delta[p]), p < P) as the Q2 planes (rows p_pad, K = rpad, zero for r >= R and p >= P), split-FP16 rows
+ * scaled by the power of two of their L2 norm (q2_scale). */
+/* FP64 row GEMM C = A B (A: rows x k, B: k x cols, C: rows x cols, all row-major; rows % 64 == 0, k % 32 == 0,
+ * cols % 128 == 0); every element sums k in order, so results do not depend on the number of rows per call.  Used
+ * for the factored weights' SNP side C = (US o US) A (with FLMM_ROTATE_US_X2) and for the FP64 operands of
+ * flmm_rescore_f64 (S^T U_eff, Y^T U_eff: U_eff.T @ S_r and U_eff.T @ Y_r of core.py:518, 629). */
+int flmm_gemm_f64(const double* a, int64_t rows, int64_t k, const double* b, int64_t cols, double* c, void* stream);
+
+/* FP64 re-evaluation of n selected tests (the scan epilogue, core.py:640-667, on exact FP64 sums): test i pairs
+ * row us_row[i] of US (rotated SNPs, leading dimension ld) with column col[i] (row col[i] of UY, leading
+ * dimension ld; delta = exp(log_delta[col[i]])): a = sum_k US w UY, d = sum_k US^2 w, rWr = sum_k w UY^2,
+ * w_k = 1 / (s_k + delta), k < neff; then f, beta, se, sfve as the scan writes them, at
+ * [out_row[i] * out_ld + col[i]].  The split-precision scan's FP32 sums carry ~1e-7 relative error, which
+ * rss = rWr - sWy^2 / sWs amplifies by ~F / nu: re-scoring the large-F tests of the real columns keeps
+ * |d log10 p| at FP64 level where p is smallest. */
+int flmm_rescore_f64(const double* US, const double* UY, int64_t ld, const double* s, int64_t neff,
+                     const double* log_delta, const int64_t* us_row, const int64_t* col, const int64_t* out_row,
+                     int64_t n, double nu, double* f, double* beta, double* se, double* sfve, int64_t out_ld,
+                     void* stream);
+
+int flmm_factored_weight_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
+                                int64_t R, const double* log_delta, void* q2_hi, void* q2_lo, float* q2_scale,
+                                void* stream);
 
 /* Learned-proposal refinement (no counterpart in the reference: the learned delta proposal).  log_delta[p]
  * holds the proposal t0 on entry.  Newton on the loss slope inside [t0 - radius, t0 + radius]; the result is
@@ -172,9 +213,9 @@ int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_
  * fallback may be NULL. */
 int flmm_delta_refine_learned(int prec, const float* uy, int64_t P, int64_t p_pad, int64_t kpad, const double* s,
                               int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo,
-                              double hi, const int32_t* idx, double radius, double judge_tol, double* log_delta,
-                              int32_t* fallback, double* rWr, double* h2, void* q1_hi, void* q1_lo, void* q2_hi,
-                              void* q2_lo, float* q1_scale, float* q2_scale, void* stream);
+                              double hi, const int32_t* idx, const double* lam64, double radius, double judge_tol,
+                              double* log_delta, int32_t* fallback, double* rWr, double* h2, void* q1_hi,
+                              void* q1_lo, void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale, void* stream);
 
 /* Spectral features of P rotated columns for the learned delta proposal (34 fp32 per column, row-major):
  * feature b < 32 = log(mean y^2 in eigenvalue bin b / mean y^2), bin_of_k[k] in [0, 32) for k < neff;
@@ -206,7 +247,8 @@ int flmm_adam(float* w, const float* g, float* m, float* v, int64_t n, double lr
  *   S2 = [(U^T s_r)^2 ; |s_r - U U^T s_r|^2 at entry k]   Q2 = [w ; 1/delta at entry k]   (k = rank)
  * so sWy and sWs carry both the on-rank and the exact off-rank terms (lowrank.py:282-289; the cross term uses
  * (s_r - U U^T s_r)^T (y_r - U U^T y_r) = s_r^T (y_r - U U^T y_r)), with one row / column scale per plane.
- * Needs p_pad % 256 == 0 and kpad2 % 128 == 0. */
+ * The same call runs the factored-weight compat scan (flmm_factored_weight_planes): S1 = US, Q1 = w o UY over
+ * kpad, S2 = C, Q2 = B over kpad2.  Needs p_pad % 256 == 0 and kpad2 % 128 == 0. */
 int flmm_wald_scan_lowrank(int prec, const void* s1_hi, const void* s1_lo, const void* s2_hi, const void* s2_lo,
                            const float* s1_scale, const float* s2_scale, int64_t mc_pad, const void* q1_hi,
                            const void* q1_lo, const void* q2_hi, const void* q2_lo, const float* q1_scale,

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_error_channel():
     from fasterlmm_amd import _lib
     lib = _lib.load()
-    assert lib.flmm_abi_version() == 5
+    assert lib.flmm_abi_version() == 6
     # argument validation runs on the host and reports through flmm_last_error without touching a GPU
     rc = lib.flmm_max_reduce(None, 1, 0, 4, None, None)
     assert rc == 1
@@ -41,12 +41,16 @@ def test_bad_arguments_raise_value_error():
     import pytest
     from fasterlmm_amd import _lib
     with pytest.raises(ValueError, match="n_grid"):
-        _lib.call("flmm_delta_weights", None, 10, None, 10, 128, 500, -10.0, 10.0, None, None, None)
+        _lib.call("flmm_delta_weights", None, 10, None, 10, 128, 500, -10.0, 10.0, None, None, None, None)
     with pytest.raises(ValueError, match="precision"):
         _lib.call("flmm_wald_scan", 7, *([None] * 6), 128, *([None] * 6), 128, 128, 1, 1, 0, None, None, 1.0,
                   None, None, None, None, None, None, None, None)
     with pytest.raises(ValueError, match="split-FP16"):
         _lib.call("flmm_pack_planes", 2, 8, 4, 4, 1, 4, None, None, 16, 16, None, 128, 32, None)
+    with pytest.raises(ValueError, match="R <= rpad"):
+        _lib.call("flmm_factored_weight_planes", 2, 10, 256, 128, 16, 16, 129, 16, 16, 16, 16, None)
+    with pytest.raises(ValueError, match="q2_scale"):
+        _lib.call("flmm_factored_weight_planes", 2, 10, 256, 128, 16, 16, 100, 16, 16, 16, None, None)
 
 
 def _fake_group(_lib, mc=300, P=200, p_pad=256, n_real=3, base=0x10000):
@@ -66,11 +70,11 @@ def test_scan_table_layout_and_validation():
     import pytest
     from fasterlmm_amd import _lib
     lib = _lib.load()
-    assert ctypes.sizeof(_lib.ScanGroup) == 29 * 8
+    assert ctypes.sizeof(_lib.ScanGroup) == 30 * 8
     groups = (_lib.ScanGroup * 3)(_fake_group(_lib, 300, 200), _fake_group(_lib, 128, 256, base=0x20000),
                                   _fake_group(_lib, 0, 256, base=0x30000))
     nbytes = lib.flmm_scan_table_bytes(3)
-    assert nbytes == (4 + 4) * 8 + 3 * 29 * 8
+    assert nbytes == (4 + 4) * 8 + 3 * 30 * 8
     for variant, tp in ((_lib.SCAN_SQUARE, 128), (_lib.SCAN_WIDE, 256)):
         buf = np.zeros(nbytes // 8, np.int64)
         _lib.call("flmm_scan_table_fill", _lib.PREC_SPLIT_FP16, variant, ctypes.addressof(groups), 3,
@@ -86,5 +90,18 @@ def test_scan_table_layout_and_validation():
     bad[0].q2_lo += 8
     with pytest.raises(ValueError, match="group 0: operand planes must be 16-byte aligned"):
         _lib.call("flmm_scan_table_fill", 0, _lib.SCAN_SQUARE, ctypes.addressof(bad), 1, buf.ctypes.data)
+    # factored weights (kpad2 > 0) run on the wide variant only, with kpad2 a multiple of 128 below kpad
+    fac = (_lib.ScanGroup * 1)(_fake_group(_lib))
+    with pytest.raises(ValueError, match="factored variant needs kpad2"):
+        _lib.call("flmm_scan_table_fill", 0, _lib.SCAN_WIDE_FACTORED, ctypes.addressof(fac), 1, buf.ctypes.data)
+    fac[0].kpad2 = 128
+    _lib.call("flmm_scan_table_fill", 0, _lib.SCAN_WIDE_FACTORED, ctypes.addressof(fac), 1, buf.ctypes.data)
+    assert list(buf[:3]) == [1, 3 * 1, _lib.SCAN_WIDE_FACTORED]
+    for v in (_lib.SCAN_WIDE, _lib.SCAN_SQUARE):
+        with pytest.raises(ValueError, match="kpad2 > 0 needs the factored variant"):
+            _lib.call("flmm_scan_table_fill", 0, v, ctypes.addressof(fac), 1, buf.ctypes.data)
+    fac[0].kpad2 = 1024
+    with pytest.raises(ValueError, match="factored variant needs kpad2"):
+        _lib.call("flmm_scan_table_fill", 0, _lib.SCAN_WIDE_FACTORED, ctypes.addressof(fac), 1, buf.ctypes.data)
     with pytest.raises(ValueError, match="blocks must be a positive multiple of 8"):
         _lib.call("flmm_wald_scan_grouped", 0, 1, buf.ctypes.data, 100, None)

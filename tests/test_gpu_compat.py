@@ -185,8 +185,9 @@ def test_golden_section_mode_replays_reference(core, ex):
 @pytest.mark.parametrize("precision", PRECISIONS)
 def test_scan_variants_multi_tile(core, ex, precision, monkeypatch):
     """300 SNPs x 300 columns spans several SNP tiles (128 rows) and column tiles (256) with ragged edges, and
-    the real columns end inside a tile; the wide (128 x 256) and square (128 x 128) scan variants must both
-    match the oracle and agree with each other."""
+    the real columns end inside a tile; the wide (128 x 256) and square (128 x 128) scan variants with exact
+    weights must both match the oracle and agree with each other, and the factored weights (the second product
+    over an exponential sum of ~128 terms, the wide two-phase tile) must match the oracle to the same bars."""
     from fasterlmm_amd import engine
     Zs, chrom = ex["Zs"], ex["chrom"]
     X = ex["X_cov2"]
@@ -202,16 +203,19 @@ def test_scan_variants_multi_tile(core, ex, precision, monkeypatch):
     ref = lmm.snp_wald_scan_compat(want, ld, S, n_real=n_real)
     df = want["s"].shape[0] - 1
     got = {}
-    for variant in ("wide", "square"):
+    for variant in ("wide", "square", "factored"):
         if variant == "square":
             monkeypatch.setenv("FLMM_SCAN_VARIANT", "square")
         else:
             monkeypatch.delenv("FLMM_SCAN_VARIANT", raising=False)
-        with engine.configure(precision=precision):
+        engine.scan_timer.reset(True)
+        with engine.configure(precision=precision, weights="factored" if variant == "factored" else "exact"):
             spec = core.fastlmm_compat_rotate(torch.from_numpy(K).to(DEV), torch.from_numpy(X).to(DEV),
                                               torch.from_numpy(Y).to(DEV))
             res = core.snp_wald_scan_compat(spec, torch.from_numpy(ld).to(DEV), torch.from_numpy(S).to(DEV),
                                             n_real=n_real)
+            assert engine.scan_timer.summary()["factored"] == (variant == "factored")
+            engine.scan_timer.reset(False)
         check_scan(res, ref, df, beta_tol=BETA_BAR[precision], what=f"{precision}/{variant}")
         np.testing.assert_allclose(res.max_F.cpu().numpy(), ref["max_F"], rtol=1e-4)
         got[variant] = res
@@ -221,3 +225,36 @@ def test_scan_variants_multi_tile(core, ex, precision, monkeypatch):
         b = getattr(got["square"], name).cpu().numpy()
         np.testing.assert_allclose(a, b, rtol=1e-12, atol=0, err_msg=name)
     np.testing.assert_array_equal(got["wide"].nullh2.cpu().numpy(), got["square"].nullh2.cpu().numpy())
+
+
+@pytest.mark.parametrize("precision", ["split_fp16", "fp32"])
+def test_large_f_tests_are_rescored_in_fp64(core, ex, precision):
+    """Real-column tests with F above the re-scoring threshold are re-evaluated from FP64 operands, so they match
+    the oracle to FP64 level where the cancellation in rss = rWr - sWy^2 / sWs would amplify the MFMA sums'
+    rounding by ~F / nu; max_F of those columns follows."""
+    from fasterlmm_amd import engine
+    Zs, chrom, X = ex["Zs"], ex["chrom"], ex["X_cov2"]
+    S = Zs[:, chrom == 2]
+    Y = ex["Y"].copy()
+    rng = np.random.default_rng(5)
+    # max F ~1.8e4, 8e3, 2.6e3, 5.8e2 (oracle); the first two columns' delta sits at the grid's lower edge
+    for j, noise in enumerate([0.002, 0.005, 0.02, 0.1]):
+        Y[:, j] = S[:, 5 + 7 * j] + noise * rng.standard_normal(Y.shape[0])
+    K = lmm.grm(Zs[:, chrom != 2])
+    want = lmm.compat_rotate(K, X, Y)
+    ld = lmm.fit_delta_grid_compat(want)
+    ref = lmm.snp_wald_scan_compat(want, ld, S, n_real=8)
+    df = want["s"].shape[0] - 1
+    with engine.configure(precision=precision):
+        spec = core.fastlmm_compat_rotate(torch.from_numpy(K).to(DEV), torch.from_numpy(X).to(DEV),
+                                          torch.from_numpy(Y).to(DEV))
+        res = core.snp_wald_scan_compat(spec, torch.from_numpy(ld).to(DEV), torch.from_numpy(S).to(DEV), n_real=8)
+    big = ref["f"] > 2 * core.RESCORE_F
+    assert big.sum() >= 4 and ref["f"].max() > 1e4
+    for k in ("f", "beta", "se"):
+        np.testing.assert_allclose(getattr(res, k).cpu().numpy()[big], ref[k][big], rtol=1e-8, err_msg=k)
+    check_scan(res, ref, df, what=f"{precision} with planted hits")
+    hit = ref["max_F"][:8] > 2 * core.RESCORE_F
+    assert hit.sum() >= 4
+    np.testing.assert_allclose(res.max_F.cpu().numpy()[:8][hit], ref["max_F"][:8][hit], rtol=1e-8)
+    np.testing.assert_allclose(res.max_F.cpu().numpy(), ref["max_F"], rtol=1e-4)

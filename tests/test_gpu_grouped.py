@@ -88,7 +88,13 @@ def test_grouped_equals_per_group_launches(mods, example, precision, monkeypatch
     _lib, core, engine, grouped = mods
     jobs = _jobs(mods, _groups(example), precision)
     prec = engine.prec_code(precision)
-    for variant, env in ((_lib.SCAN_WIDE, None), (_lib.SCAN_SQUARE, "square")):
+    # the default weights: factored for the split precisions where the sum is shorter than Neff (the 150- and
+    # 125-strain groups, K = 256), exact for the 100-strain group (K = 128) and for fp32: a mixed job list runs as
+    # one launch per kind
+    kinds = [j.factored for j in jobs]
+    assert kinds == ([True, False, False] if precision != "fp32" else [False] * 3), kinds
+    variants = ((_lib.SCAN_WIDE, None),) if any(kinds) else ((_lib.SCAN_WIDE, None), (_lib.SCAN_SQUARE, "square"))
+    for variant, env in variants:
         if env:
             monkeypatch.setenv("FLMM_SCAN_VARIANT", env)
         else:
@@ -129,7 +135,9 @@ def test_graph_replays_after_table_rewrites(mods, example):
         torch.cuda.synchronize()
         assert torch.isfinite(y)
         _assert_same(_snapshot(jobs), want[name], f"replay {rep} ({name})")
-    assert len(runner._graphs) == 1
+    # one graph per weight kind present (factored and exact jobs run as separate launches), reused by every replay
+    kinds = {j.factored for j in jobs_a + jobs_b}
+    assert len(runner._graphs) == len(kinds)
 
 
 def test_autotuner_choice(mods, example, tmp_path):
@@ -144,19 +152,21 @@ def test_autotuner_choice(mods, example, tmp_path):
     _reset(jobs)
     runner.run(jobs)
     torch.cuda.synchronize()
-    key = tuner.key(_lib.PREC_SPLIT_FP16, jobs)
-    assert runner.last_choice in tuner.candidates(jobs)
-    assert len(tuner.trials[key]) == len(tuner.candidates(jobs))
+    parts = [p for p in ([j for j in jobs if j.factored], [j for j in jobs if not j.factored]) if p]
+    assert len(runner.last_choices) == len(parts)
     got = _snapshot(jobs)
-    if runner.last_choice[0] == _lib.SCAN_WIDE:
+    tuner2 = grouped.ScanTuner(path=path)
+    for part, choice in zip(parts, runner.last_choices):
+        key = tuner.key(_lib.PREC_SPLIT_FP16, part)
+        assert choice in tuner.candidates(part)
+        assert len(tuner.trials[key]) == len(tuner.candidates(part))
+        assert tuner2.cache[key] == choice  # the choice persists and is reused without new trials
+    if all(c[0] in (_lib.SCAN_WIDE, _lib.SCAN_WIDE_FACTORED) for c in runner.last_choices):  # the loop's tile bodies
         _assert_same(got, want, "tuned")
     else:  # the square variant rounds the FP64 unscaling differently
         for a, b in zip(got, want):
             for k in FIELDS:
                 torch.testing.assert_close(a[k], b[k], rtol=1e-12, atol=0)
-    # the choice persists and is reused without new trials
-    tuner2 = grouped.ScanTuner(path=path)
-    assert tuner2.cache[key] == runner.last_choice
 
 
 def _na_dataset(example):
