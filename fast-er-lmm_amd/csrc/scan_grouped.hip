@@ -60,8 +60,8 @@ __global__ __launch_bounds__(256, 1) void wald_scan_grouped_kernel(const int64_t
         square_tile<PREC>(args, sc, n_mtiles, t - start[lo], j.mc, j.P, j.n_real, j.rWr, j.h2, j.nu, j.row_index,
                           out, j.pmax, j.p_pad);
       else if constexpr (VARIANT == FLMM_SCAN_WIDE_FACTORED)  // second product over kpad2 (the low-rank tile)
-        wide_tile<PREC, true>(args, sc, n_mtiles, (j.P + W_TP - 1) / W_TP, t - start[lo], j.mc, j.P, j.n_real, j.rWr,
-                              j.h2, j.nu, j.row_index, out, j.pmax, j.p_pad);
+        wide_tile<PREC, K_PAIRED>(args, sc, n_mtiles, (j.P + W_TP - 1) / W_TP, t - start[lo], j.mc, j.P, j.n_real,
+                                  j.rWr, j.h2, j.nu, j.row_index, out, j.pmax, j.p_pad);
       else
         wide_tile<PREC>(args, sc, n_mtiles, (j.P + W_TP - 1) / W_TP, t - start[lo], j.mc, j.P, j.n_real, j.rWr, j.h2, j.nu,
                         j.row_index, out, j.pmax, j.p_pad);

@@ -56,14 +56,23 @@ struct AStage {
   u32x4 v[N];
 };
 
+// K ranges of the two products
+constexpr int K_EXACT = 0;    // both products over args.nchunks
+// The first product (sWy) over all args.nchunks K-chunks, the second (sWs) over the first args.nchunks2 only: the
+// compat scan with factored weights (sWs over the exponential sum) and the low-rank scan (the off-rank cross term
+// lives in the first product alone).  The chunks past nchunks2 run first (phase A), then both products over the
+// first nchunks2 (phase B).  Phase A uses the second product's idle accumulators and plane slots in one of two ways:
+constexpr int K_PAIRED = 1;   // chunk pairs: each stage holds two consecutive first-product chunks, the odd one in
+                              // the second product's slots and accumulators, so every barrier and prefetch carries a
+                              // two-product chunk's MFMAs; the sets are added before phase B (chains of up to
+                              // nchunks / 2 chunks: for the compat scan's K = Neff)
+constexpr int K_FLUSHED = 2;  // one first-product chunk per stage, summed in block partials of FLUSH chunks in the
+                              // second accumulator set and folded into the first, so no FP32 chain runs over more
+                              // than FLUSH chunks plus the folds (the low-rank scan's K = N, up to ~50,000)
+
 // One 128 x 256 tile (logical index `tile` of the n_mtiles x n_ptiles grid, in the grouped order below) of the
 // wide Wald scan, run by a 256-thread workgroup.
-// LOWRANK: the first product (sWy) runs over all args.nchunks K-chunks and the second (sWs) over the first
-// args.nchunks2 only (the low-rank scan's off-rank cross term lives in the first product alone).  The chunks past
-// nchunks2 (K = N, the long part) run first and load, stage and multiply the first product's planes only; their
-// sums go to block partials of FLUSH chunks in the second accumulator set (idle until then), folded into the
-// first, so no FP32 accumulation chain runs over more than FLUSH chunks plus the folds.
-template <int PREC, bool LOWRANK = false>
+template <int PREC, int KMODE = K_EXACT>
 __device__ inline void wide_tile(
     const TileArgs<PREC, 2>& args, const ScanScales& sc, int64_t n_mtiles, int64_t n_ptiles, int64_t tile, int64_t Mc,
     int64_t P, int64_t n_real,
@@ -115,13 +124,17 @@ __device__ inline void wide_tile(
   const int64_t b_lane = (p0 + wave * 64 + (lane & 31)) * T::ROWB + h * (T::ROWB / 2);
   const int64_t b_kstride = args.b_rows_pad * T::ROWB;
 
+  // plane pointers of the two operand slots (K_PAIRED points the second slot at the first product, one chunk on)
+  const void* pa[2][2] = {{args.a[0][0], args.a[0][1]}, {args.a[1][0], args.a[1][1]}};
+  const void* pb[2][2] = {{args.b[0][0], args.b[0][1]}, {args.b[1][0], args.b[1][1]}};
+
   // `two` (std::true_type / std::false_type): whether the chunk carries the second product's planes
   auto load_a = [&](auto two, int kc) {
     constexpr int NQ = decltype(two)::value ? NPL : NSUB;
     AStage<NA> f;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const char* src = (const char*)args.a[q / NSUB][q % NSUB] + a_lane + (int64_t)kc * a_kstride;
+      const char* src = (const char*)pa[q / NSUB][q % NSUB] + a_lane + (int64_t)kc * a_kstride;
 #pragma unroll
       for (int i = 0; i < NB; ++i) f.v[q * NB + i] = *gmem((const u32x4*)(src + i * 1024));
     }
@@ -153,7 +166,7 @@ __device__ inline void wide_tile(
 #pragma unroll
           for (int q = 0; q < NSTEP; ++q)
             f.v[(g * 2 + bj) * NSUB + sub][q] =
-                *gmem((const u32x4*)((const char*)args.b[g][sub] + off + bj * 32 * T::ROWB + q * 16));
+                *gmem((const u32x4*)((const char*)pb[g][sub] + off + bj * 32 * T::ROWB + q * 16));
     return f;
   };
 
@@ -283,7 +296,17 @@ __device__ inline void wide_tile(
   constexpr std::false_type one{};
   constexpr std::integral_constant<int, 0> t0{};
   constexpr std::integral_constant<int, 1> t1{};
-  if constexpr (!LOWRANK) {
+  auto fold = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        acc[0][i][j] += acc[1][i][j];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[1][i][j][v] = 0.f;
+      }
+  };
+  if constexpr (KMODE == K_EXACT) {
     BChunk<NSUB, NSTEP> be = load_b(two, 0), bo;
     store_a(two, load_a(two, 0), lds0);
     __syncthreads();
@@ -292,22 +315,42 @@ __device__ inline void wide_tile(
       chunk(two, two, t0, kc + 1 < nchunks ? kc + 1 : kc, lds0, lds1, be, bo);
       chunk(two, two, t0, kc + 2 < nchunks ? kc + 2 : kc + 1, lds1, lds0, bo, be);
     }
+  } else if constexpr (KMODE == K_PAIRED) {
+    // nchunks2 and nchunks are multiples of 4 with 0 < nchunks2 < nchunks.  Phase A: the nP = (nchunks - n2) / 2
+    // (even) chunk pairs (n2 + 2 j, n2 + 2 j + 1) as two-product chunks; phase B: chunks 0 .. n2 - 1.
+    const int n2 = args.nchunks2;
+    const int nP = (nchunks - n2) / 2;
+#pragma unroll
+    for (int u = 0; u < NSUB; ++u) {
+      pa[1][u] = (const char*)pa[0][u] + a_kstride;
+      pb[1][u] = (const char*)pb[0][u] + b_kstride;
+    }
+    BChunk<NSUB, NSTEP> be = load_b(two, n2), bo;
+    store_a(two, load_a(two, n2), lds0);
+    __syncthreads();
+    for (int j = 0; j < nP - 2; j += 2) {
+      chunk(two, two, t0, n2 + 2 * (j + 1), lds0, lds1, be, bo);
+      chunk(two, two, t0, n2 + 2 * (j + 2), lds1, lds0, bo, be);
+    }
+    chunk(two, two, t0, n2 + 2 * (nP - 1), lds0, lds1, be, bo);
+    // the last pair prefetches phase B's chunk 0 with the second product's own planes
+#pragma unroll
+    for (int u = 0; u < NSUB; ++u) {
+      pa[1][u] = args.a[1][u];
+      pb[1][u] = args.b[1][u];
+    }
+    chunk(two, two, t0, 0, lds1, lds0, bo, be);
+    fold();
+    for (int kc = 0; kc < n2; kc += 2) {
+      chunk(two, two, t0, kc + 1, lds0, lds1, be, bo);
+      chunk(two, two, t0, kc + 2 < n2 ? kc + 2 : kc + 1, lds1, lds0, bo, be);
+    }
   } else {
     // nchunks2 and nchunks are multiples of 4 with 0 < nchunks2 < nchunks.  Phase A: chunks n2 .. nchunks - 1
     // (first product only, block partials in acc[1]); phase B: chunks 0 .. n2 - 1 (both products).
     constexpr int FLUSH = 32;
     const int n2 = args.nchunks2;
     const int nA = nchunks - n2;
-    auto fold = [&]() {
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          acc[0][i][j] += acc[1][i][j];
-#pragma unroll
-          for (int v = 0; v < 16; ++v) acc[1][i][j][v] = 0.f;
-        }
-    };
     BChunk<NSUB, NSTEP> be = load_b(one, n2), bo;
     store_a(one, load_a(one, n2), lds0);
     __syncthreads();
