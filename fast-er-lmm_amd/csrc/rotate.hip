@@ -113,52 +113,50 @@ hipError_t launch_rotate(int mode, const TileArgs<PREC, 1>& args, const float* a
 }
 
 // Row GEMM in FP64: C[i][r] = sum_{k < kpad} X2[i][k] A[k][r] (X2: rows x kpad, A: kpad x rpad, C: rows x rpad,
-// all row-major; rows % 64 == 0, kpad % 32 == 0, rpad % 128 == 0) -- the factored weights' SNP side (X2 = US^2)
+// all row-major; rows % 32 == 0, kpad % 32 == 0, rpad % 64 == 0) -- the factored weights' SNP side (X2 = US^2)
 // and the FP64 rotations of the re-scoring (X2 = S^T or Y^T, A = U_eff).  Each element sums k in order, so a row's
-// result does not depend on how many rows a launch holds.  A 256-thread block owns 64 rows x 128 r: thread
-// (g = tid / 32, c = tid % 32) accumulates rows 8 g .. 8 g + 7 at r = c + 32 q (q < 4), 32 FP64 sums, from K
-// chunks of 32 staged in LDS (A chunk 32 x 128, X2 chunk 64 x 32 padded against bank conflicts).
+// result does not depend on how many rows a launch holds.  A 256-thread block owns 32 rows x 64 r (a 4,608-row
+// SNP block against 128 terms is 288 blocks): thread (g = tid / 32, c = tid % 32) accumulates rows 4 g .. 4 g + 3
+// at r = c and c + 32, 8 FP64 sums, from K chunks of 32 staged in LDS (A chunk 32 x 64, X2 chunk 32 x 32 padded
+// against bank conflicts).
 __global__ __launch_bounds__(256) void gemm_f64_kernel(const double* __restrict__ x2, int64_t kpad,
                                                        const double* __restrict__ A, int64_t rpad,
                                                        double* __restrict__ C) {
-  __shared__ double as[32][128];
-  __shared__ double xs[64][33];
+  __shared__ double as[32][64];
+  __shared__ double xs[32][33];
   const int tid = threadIdx.x, g = tid >> 5, c = tid & 31;
-  const int64_t i0 = (int64_t)blockIdx.x * 64, r0 = (int64_t)blockIdx.y * 128;
-  double acc[8][4];
+  const int64_t i0 = (int64_t)blockIdx.x * 32, r0 = (int64_t)blockIdx.y * 64;
+  double acc[4][2];
 #pragma unroll
-  for (int i = 0; i < 8; ++i)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) acc[i][q] = 0.0;
+  for (int i = 0; i < 4; ++i) acc[i][0] = acc[i][1] = 0.0;
   for (int64_t k0 = 0; k0 < kpad; k0 += 32) {
 #pragma unroll
-    for (int e = tid; e < 32 * 128; e += 256) as[e >> 7][e & 127] = A[(k0 + (e >> 7)) * rpad + r0 + (e & 127)];
+    for (int e = tid; e < 32 * 64; e += 256) as[e >> 6][e & 63] = A[(k0 + (e >> 6)) * rpad + r0 + (e & 63)];
 #pragma unroll
-    for (int e = tid; e < 64 * 32; e += 256) xs[e >> 5][e & 31] = x2[(i0 + (e >> 5)) * kpad + k0 + (e & 31)];
+    for (int e = tid; e < 32 * 32; e += 256) xs[e >> 5][e & 31] = x2[(i0 + (e >> 5)) * kpad + k0 + (e & 31)];
     __syncthreads();
-#pragma unroll 4
+#pragma unroll 8
     for (int k = 0; k < 32; ++k) {
-      double a[4], x[8];
+      const double a0 = as[k][c], a1 = as[k][c + 32];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) a[q] = as[k][c + 32 * q];
-#pragma unroll
-      for (int i = 0; i < 8; ++i) x[i] = xs[8 * g + i][k];
-#pragma unroll
-      for (int i = 0; i < 8; ++i)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[i][q] = fma(x[i], a[q], acc[i][q]);
+      for (int i = 0; i < 4; ++i) {
+        const double x = xs[4 * g + i][k];
+        acc[i][0] = fma(x, a0, acc[i][0]);
+        acc[i][1] = fma(x, a1, acc[i][1]);
+      }
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 8; ++i)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) C[(i0 + 8 * g + i) * rpad + r0 + c + 32 * q] = acc[i][q];
+  for (int i = 0; i < 4; ++i) {
+    C[(i0 + 4 * g + i) * rpad + r0 + c] = acc[i][0];
+    C[(i0 + 4 * g + i) * rpad + r0 + c + 32] = acc[i][1];
+  }
 }
 
 hipError_t launch_gemm_f64(const double* x2, int64_t rows, int64_t kpad, const double* A, int64_t rpad, double* C,
                            hipStream_t stream) {
-  const dim3 grid((unsigned)(rows / 64), (unsigned)(rpad / 128));
+  const dim3 grid((unsigned)(rows / 32), (unsigned)(rpad / 64));
   hipLaunchKernelGGL(gemm_f64_kernel, grid, dim3(256), 0, stream, x2, kpad, A, rpad, C);
   return hipGetLastError();
 }

@@ -108,6 +108,37 @@ __device__ inline void wide_tile(
   const int h = lane >> 5;
   const int nchunks = args.nchunks;
 
+  // ---- the epilogue's operands: the tile rows' reciprocal scales (exact powers of two for split-FP16, 1
+  // otherwise) and the wave's columns' rWr, reciprocal column scales and h2.  Loaded here, with the first chunk,
+  // and parked in LDS by `stash` before the prologue's barrier, so their latency overlaps the first chunk's loads.
+  __shared__ double row_inv[2][W_TM];
+  __shared__ __attribute__((aligned(16))) float row_invf[2][W_TM];
+  __shared__ double colv[4][4][64];  // [wave][rWr, 1 / q1 scale, 1 / q2 scale, h2][lane]
+  float e_rs;
+  double e_rwr, e_q1, e_q2, e_h2;
+  {
+    const int64_t pc = p0 + wave * 64 + lane;
+    const bool pv = pc < P, real = pc < n_real;
+    const int t = threadIdx.x & (W_TM - 1), which = threadIdx.x / W_TM;
+    const float* scl = which ? sc.s2 : sc.s1;
+    const int64_t m = m0 + t;
+    e_rs = (scl && m < Mc) ? gmem(scl)[m] : 1.f;
+    e_rwr = pv ? gmem(rWr)[pc] : 1.0;
+    e_q1 = (sc.q1 && pv) ? (double)gmem(sc.q1)[pc] : 1.0;
+    e_q2 = (sc.q2 && pv) ? (double)gmem(sc.q2)[pc] : 1.0;
+    e_h2 = real ? gmem(h2)[pc] : 0.0;
+  }
+  auto stash = [&]() {
+    const int t = threadIdx.x & (W_TM - 1), which = threadIdx.x / W_TM;
+    const double r = 1.0 / (double)e_rs;
+    row_inv[which][t] = r;
+    row_invf[which][t] = (float)r;
+    colv[wave][0][lane] = e_rwr;
+    colv[wave][1][lane] = 1.0 / e_q1;
+    colv[wave][2][lane] = 1.0 / e_q2;
+    colv[wave][3][lane] = e_h2;
+  };
+
   // ---- SNP-side staging: wave w copies 1 KiB blocks [w NB, (w + 1) NB) of every plane of the stage through
   // registers (loads issued at the start of a chunk, written to the other stage at its end).  Lane L of a block
   // handles row blk * ROWS_PER_BLK + L / SEGS, logical segment L % SEGS, stored at physical segment
@@ -309,6 +340,7 @@ __device__ inline void wide_tile(
   if constexpr (KMODE == K_EXACT) {
     BChunk<NSUB, NSTEP> be = load_b(two, 0), bo;
     store_a(two, load_a(two, 0), lds0);
+    stash();
     __syncthreads();
     // nchunks is even (K is padded to a multiple of 128); past the last chunk the prefetch repeats it
     for (int kc = 0; kc < nchunks; kc += 2) {
@@ -327,6 +359,7 @@ __device__ inline void wide_tile(
     }
     BChunk<NSUB, NSTEP> be = load_b(two, n2), bo;
     store_a(two, load_a(two, n2), lds0);
+    stash();
     __syncthreads();
     for (int j = 0; j < nP - 2; j += 2) {
       chunk(two, two, t0, n2 + 2 * (j + 1), lds0, lds1, be, bo);
@@ -353,6 +386,7 @@ __device__ inline void wide_tile(
     const int nA = nchunks - n2;
     BChunk<NSUB, NSTEP> be = load_b(one, n2), bo;
     store_a(one, load_a(one, n2), lds0);
+    stash();
     __syncthreads();
     for (int l = 0; l < nA - 2; l += 2) {
       chunk(one, one, t1, n2 + l + 1, lds0, lds1, be, bo);
@@ -368,19 +402,8 @@ __device__ inline void wide_tile(
     }
   }
 
-  // ---- epilogue: this wave owns columns p0 + 64 wave + [0, 64) and all 128 rows.
-  // Reciprocal row scales of the tile's SNPs (exact powers of two for split-FP16, 1 otherwise), once per tile.
-  __shared__ double row_inv[2][W_TM];
-  __shared__ __attribute__((aligned(16))) float row_invf[2][W_TM];
-  {
-    const int t = threadIdx.x & (W_TM - 1), which = threadIdx.x / W_TM;
-    const float* scl = which ? sc.s2 : sc.s1;
-    const int64_t m = m0 + t;
-    const double r = (scl && m < Mc) ? 1.0 / (double)gmem(scl)[m] : 1.0;
-    row_inv[which][t] = r;
-    row_invf[which][t] = (float)r;
-  }
-  __syncthreads();
+  // ---- epilogue: this wave owns columns p0 + 64 wave + [0, 64) and all 128 rows (row scales and column values
+  // stashed in LDS at the start; the last chunk ended with a barrier).
   const bool tile_real = p0 < n_real;
 
   // Each 32-row block of the wave's accumulators is staged through the (now free) LDS stages column-major
@@ -398,10 +421,10 @@ __device__ inline void wide_tile(
   const int64_t pc = p0 + wave * 64 + lane;
   const bool pv = pc < P;
   const bool real = pc < n_real;
-  const double rwr = pv ? gmem(rWr)[pc] : 1.0;
-  const double iq1 = (sc.q1 && pv) ? 1.0 / (double)gmem(sc.q1)[pc] : 1.0;
-  const double iq2 = (sc.q2 && pv) ? 1.0 / (double)gmem(sc.q2)[pc] : 1.0;
-  const double hh = real ? gmem(h2)[pc] : 0.0;
+  const double rwr = colv[wave][0][lane];
+  const double iq1 = colv[wave][1][lane];
+  const double iq2 = colv[wave][2][lane];
+  const double hh = colv[wave][3][lane];
   double fmx = -INFINITY;                       // real tiles: FP64 running maximum
   float bn = 0.f, bdd = 1.f, bA = 0.f, bD = 1.f;  // other tiles: best a'^2, its d', and its raw sums
   int bt = -1;
