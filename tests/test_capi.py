@@ -47,6 +47,14 @@ def test_bad_arguments_raise_value_error():
                   None, None, None, None, None, None, None, None)
     with pytest.raises(ValueError, match="split-FP16"):
         _lib.call("flmm_pack_planes", 2, 8, 4, 4, 1, 4, None, None, 16, 16, None, 128, 32, None)
+    with pytest.raises(ValueError, match="rows % 64"):
+        _lib.call("flmm_gemm_f64", 16, 100, 32, 16, 128, 16, None)
+    with pytest.raises(ValueError, match="null pointer"):
+        _lib.call("flmm_gemm_f64", None, 64, 32, 16, 128, 16, None)
+    with pytest.raises(ValueError, match="bad sizes"):
+        _lib.call("flmm_rescore_f64", 16, 16, 128, 16, 200, 16, 16, 16, 16, 5, 10.0, 16, 16, 16, 16, 4, None)
+    with pytest.raises(ValueError, match="null pointer"):
+        _lib.call("flmm_rescore_f64", 16, None, 128, 16, 100, 16, 16, 16, 16, 5, 10.0, 16, 16, 16, 16, 4, None)
     with pytest.raises(ValueError, match="R <= rpad"):
         _lib.call("flmm_factored_weight_planes", 2, 10, 256, 128, 16, 16, 129, 16, 16, 16, 16, None)
     with pytest.raises(ValueError, match="q2_scale"):

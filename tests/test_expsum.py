@@ -49,3 +49,25 @@ def test_bad_ranges_raise():
     for a, b in [(0.0, 1.0), (2.0, 1.0), (1.0, math.inf)]:
         with pytest.raises(ValueError):
             expsum.weight_nodes(a, b)
+
+
+def test_engine_factored_weights_map():
+    """engine.factored_weights (the SNP-side map A and the nodes) on CPU tensors: A is zero beyond Neff and R, the
+    factored sWs matches the exact one, and no factoring is offered when the sum would not be shorter than K."""
+    import torch
+    from fasterlmm_amd import engine
+    rng = np.random.default_rng(11)
+    s = torch.from_numpy(np.sort(rng.exponential(1.5, 300)))
+    s[:7] = 0.0
+    rng_s = (float(s.min()), float(s.max()))
+    fac = engine.factored_weights(s, rng_s, 384, -10.0, 10.0)
+    assert fac is not None and fac.rpad == 128 and 0 < fac.R <= 128
+    A = fac.A.numpy()
+    assert A.shape == (384, 128) and not A[300:].any() and not A[:, fac.R:].any()
+    log_delta = np.linspace(-10.0, 10.0, 41)
+    B = fac.lam.numpy()[None, :] * np.exp(-np.outer(np.exp(log_delta), fac.t.numpy()))
+    US2 = rng.standard_normal((25, 300)) ** 2
+    exact = US2 @ (1.0 / (s.numpy()[:, None] + np.exp(log_delta)[None, :]))
+    fact = (US2 @ A[:300, : fac.R]) @ B.T
+    assert np.abs(fact / exact - 1).max() <= 1.5 * expsum.EPS
+    assert engine.factored_weights(s, rng_s, 128, -10.0, 10.0) is None
