@@ -374,6 +374,7 @@ __device__ inline double newton_bracket(const LK& L, int64_t p, int i, int G, do
     if (g == 0.0) break;
     if (g < 0.0) a = t; else b = t;
     double tn = t - g / gp;
+    bool newton = true;
     if (!(gp > 0.0) || !(tn > a && tn < b)) {
       if (!ends_checked) {
         ends_checked = true;
@@ -384,8 +385,12 @@ __device__ inline double newton_bracket(const LK& L, int64_t p, int i, int G, do
         if (ge <= 0.0) return b0;
       }
       tn = 0.5 * (a + b);
+      newton = false;
     }
-    const bool done = fabs(tn - t) <= 1e-13 * fmax(1.0, fabs(t)) || (b - a) <= 1e-13;
+    // a Newton step below 1e-7 leaves the new iterate within ~1e-14 (quadratic convergence): accept it without
+    // another evaluation; bisection steps stop on the 1e-13 criterion
+    const double step = fabs(tn - t), scale = fmax(1.0, fabs(t));
+    const bool done = step <= 1e-13 * scale || (b - a) <= 1e-13 || (newton && step <= 1e-7 * scale);
     t = tn;
     if (done) break;
   }
