@@ -188,6 +188,32 @@ int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_
                       "flmm_delta_refine");
 }
 
+int flmm_gram(int prec, const void* z_hi, const void* z_lo, const float* z_scale, int64_t rows_pad, int64_t kpad,
+              int64_t n, int n_split, double* partial, double* out, int64_t ld, void* stream) {
+  REQUIRE(valid_prec(prec), "flmm_gram: bad precision %d", prec);
+  REQUIRE(rows_pad > 0 && rows_pad % TILE == 0 && kpad > 0 && kpad % KCHUNK == 0 && n > 0 && n <= rows_pad &&
+              n_split >= 1 && ld >= n,
+          "flmm_gram: need rows_pad %% 128, kpad %% 32, 0 < n <= rows_pad, n_split >= 1, ld >= n");
+  const bool two = two_planes(prec);
+  REQUIRE(z_hi && (!two || z_lo) && partial && out, "flmm_gram: null pointer");
+  REQUIRE(aligned16(z_hi) && (!two || aligned16(z_lo)), "flmm_gram: planes must be 16-byte aligned");
+  REQUIRE(prec != FLMM_PREC_SPLIT_FP16 || z_scale, "flmm_gram: split-FP16 needs the row scales");
+  const hipStream_t st = (hipStream_t)stream;
+  hipError_t e;
+#define FLMM_GRAM(PR)                                                                                                \
+  do {                                                                                                               \
+    TileArgs<PR, 1> t{};                                                                                             \
+    t.a[0][0] = z_hi; t.a[0][1] = z_lo; t.b[0][0] = z_hi; t.b[0][1] = z_lo;                                          \
+    t.a_rows_pad = rows_pad; t.b_rows_pad = rows_pad; t.nchunks = (int)(kpad / KCHUNK);                              \
+    e = launch_gram<PR>(t, prec == FLMM_PREC_SPLIT_FP16 ? z_scale : nullptr, n_split, partial, n, out, ld, st);      \
+  } while (0)
+  if (prec == FLMM_PREC_SPLIT_BF16) FLMM_GRAM(PREC_SPLIT_BF16);
+  else if (prec == FLMM_PREC_SPLIT_FP16) FLMM_GRAM(PREC_SPLIT_FP16);
+  else FLMM_GRAM(PREC_FP32);
+#undef FLMM_GRAM
+  return check_launch(e, "flmm_gram");
+}
+
 int flmm_gemm_f64(const double* a, int64_t rows, int64_t k, const double* b, int64_t cols, double* c, void* stream) {
   REQUIRE(rows >= 0 && rows % 64 == 0 && k > 0 && k % KCHUNK == 0 && cols > 0 && cols % TILE == 0,
           "flmm_gemm_f64: need rows %% 64, k %% 32 and cols %% 128 == 0");

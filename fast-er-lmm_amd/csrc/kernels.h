@@ -47,6 +47,10 @@ hipError_t launch_rescore(const double* US, const double* UY, int64_t ld, const 
                           const double* log_delta, const int64_t* us_row, const int64_t* col, const int64_t* out_row,
                           int64_t n, double nu, double* f, double* beta, double* se, double* sfve, int64_t out_ld,
                           hipStream_t stream);
+template <int PREC>
+hipError_t launch_gram(const TileArgs<PREC, 1>& args, const float* row_scale, int n_split, double* partial, int64_t n,
+                       double* out, int64_t ld, hipStream_t stream);
+int gram_tile_pairs(int64_t rows_pad);
 hipError_t launch_factored_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
                                   int64_t R, const double* log_delta, void* qa, void* qb, float* qs,
                                   hipStream_t stream);

@@ -307,6 +307,12 @@ def _spec_state(spec) -> _Basis:
     return st
 
 
+def _kinship(Z: Tensor) -> Tensor:
+    """K = Z Z^T / M of a standardised panel (io.grm's product, io.py:257-265), the Gram on MFMA in the split
+    precisions (engine.gram)."""
+    return engine.gram(Z) / float(Z.shape[1])
+
+
 def _new_outputs(M, n_real, P, dev):
     return engine.ScanOutputs.allocate(M, n_real, P, dev)
 
@@ -374,7 +380,7 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
     ck = chrom if chrom_kin is None else np.asarray(chrom_kin)
     stage = engine.stage
     with stage("kinship"):
-        K_all = Zk @ Zk.T
+        K_all = engine.gram(Zk, prec)
     Xpinv, Iperp = _projector(Xd)
     # the basis is orthogonal to X, so B^T (Y - X X^+ Y) = B^T Y; the OLS regress-out keeps the compat
     # path's arithmetic (core.py:517) and costs one pass
@@ -392,7 +398,7 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
         for c in chroms:
             kin_c = torch.from_numpy(np.flatnonzero(ck == c)).to(dev)
             Zc = Zk.index_select(1, kin_c)
-            K = (K_all - Zc @ Zc.T) / float(Zk.shape[1] - kin_c.numel())
+            K = (K_all - engine.gram(Zc, prec)) / float(Zk.shape[1] - kin_c.numel())
             del Zc
             if jitter:
                 K = K + jitter * eye
@@ -434,9 +440,8 @@ def single_k_scan_compat(Z: Tensor, X: Tensor, Y: Tensor, *, n_real: int | None 
                          Z_kin: Tensor | None = None, n_grid: int = 64) -> ScanResult:
     """One kinship over all SNPs (or the Z_kin panel), compat rotation, delta fit, scan of every SNP
     (core.py:729-740)."""
-    from .io import grm
     home, (Zd, Zkd) = _dev(Z, Z_kin)
-    K = grm(Zd if Zkd is None else Zkd)
+    K = _kinship(Zd if Zkd is None else Zkd)
     spec = fastlmm_compat_rotate(K, X, Y)
     ld = fit_delta_grid_compat(spec, n_grid=n_grid)
     return _home(snp_wald_scan_compat(spec, ld, Zd, n_real=n_real), home)
@@ -584,9 +589,8 @@ def loco_scan(Z: Tensor, X: Tensor, Y: Tensor, chrom: np.ndarray, *, n_real: int
 def single_k_scan(Z: Tensor, X: Tensor, Y: Tensor, *, n_real: int | None = None,
                   Z_kin: Tensor | None = None) -> ScanResult:
     """One kinship, GLS rotation, delta fit and scan of every SNP (core.py:439-451)."""
-    from .io import grm
     home, (Zd, Zkd) = _dev(Z, Z_kin)
-    spec = rotate(grm(Zd if Zkd is None else Zkd), X, Y)
+    spec = rotate(_kinship(Zd if Zkd is None else Zkd), X, Y)
     ld = fit_delta_grid(spec)
     S_rot = _dev(spec.U)[1][0].T @ Zd
     return _home(snp_wald_scan(spec, ld, S_rot, n_real=n_real), home)

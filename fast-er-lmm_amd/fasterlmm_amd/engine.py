@@ -41,6 +41,8 @@ class Settings:
                                    # "learned" (MLP proposal + judged Newton, grid as fallback and judge)
     weights: str = "auto"          # sWs weights 1 / (s + delta): "factored" (exponential sum, second product over
                                    # ~128 terms), "exact" (over Neff), "auto" = factored for the split precisions
+    kinship: str = "auto"          # kinship Gram Z Z^T: "split" (MFMA in the split precision, FP64 folds every
+                                   # 1,024 k), "fp64" (rocBLAS), "auto" = split for the split precisions
 
 
 settings = Settings()
@@ -143,6 +145,33 @@ def pack(prec: int, src: torch.Tensor, n_rows: int, n_k: int, row_stride: int, k
     _lib.call("flmm_pack_planes", pl.prec, base, n_rows, n_k, row_stride, k_stride, _lib.ptr(row_index),
               _lib.ptr(norm), hi, lo, pl.scale_ptr, rows_pad, kpad, st)
     return pl
+
+
+def gram(Z: torch.Tensor, prec: int | None = None) -> torch.Tensor:
+    """G = Z Z^T (n x n FP64) of Z (n, m) FP64 on the GPU: flmm_gram on MFMA in the split precisions (the kinship
+    setting "split", default for them), rocBLAS FP64 otherwise.  Deterministic: the same Z gives the same bits."""
+    prec = prec_code() if prec is None else prec
+    k = settings.kinship
+    if k not in ("auto", "split", "fp64"):
+        raise ValueError(f"unknown kinship setting {k!r}")
+    split = prec in (_lib.PREC_SPLIT_FP16, _lib.PREC_SPLIT_BF16)
+    if k == "fp64" or (k == "auto" and not split):
+        return Z @ Z.T
+    n, m = Z.shape
+    if m == 0:
+        return torch.zeros(n, n, dtype=torch.float64, device=Z.device)
+    Z = Z.contiguous()
+    rows_pad, kpad = rup(n, TILE), rup(m, KCHUNK)
+    pl = pack(prec, Z, n, m, m, 1, rows_pad, kpad)
+    t = rows_pad // TILE
+    pairs = t * (t + 1) // 2
+    n_split = max(1, min(-(-1024 // pairs), -(-(kpad // KCHUNK) // 32)))  # >= 1,024 workgroups where K allows
+    partial = torch.empty(n_split, rows_pad, rows_pad, dtype=torch.float64, device=Z.device)
+    out = torch.empty(n, n, dtype=torch.float64, device=Z.device)
+    hi, lo = pl.ptrs
+    _lib.call("flmm_gram", pl.prec, hi, lo, pl.scale_ptr, rows_pad, kpad, n, n_split, partial.data_ptr(),
+              out.data_ptr(), n, _lib.stream())
+    return out
 
 
 def rotate_us_x2(a: Planes, b: Planes) -> tuple[Planes, torch.Tensor]:

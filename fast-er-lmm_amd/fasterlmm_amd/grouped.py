@@ -344,7 +344,7 @@ class GroupedLoco:
             for c, kc in self.kin_cols.items():
                 for g in range(G):
                     zs = self._std_rows(self.Zk, kc, g)
-                    self.K_all[g].addmm_(zs, zs.T)
+                    self.K_all[g] += engine.gram(zs, prec)
                     del zs
         self.Iperps, self.ypls, self.y_reals = [], [], []
         with engine.stage("pack_columns"):
@@ -370,7 +370,7 @@ class GroupedLoco:
         eye = torch.eye(N, dtype=torch.float64, device=dev)
         if kc is not None and kc.numel():
             zs = self._std_rows(self.Zk, kc, g)
-            K = (self.K_all[g] - zs @ zs.T) / float(self.Mk - kc.numel())
+            K = (self.K_all[g] - engine.gram(zs, self.prec)) / float(self.Mk - kc.numel())
             del zs
         else:
             K = self.K_all[g] / float(self.Mk)

@@ -183,6 +183,16 @@ int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_
  * S2 = planes of C, e.g. from FLMM_ROTATE_US_X2 and flmm_pack_planes).  This call writes B (delta_p =
  * exp(log_delta[p]), p < P) as the Q2 planes (rows p_pad, K = rpad, zero for r >= R and p >= P), split-FP16 rows
  * scaled by the power of two of their L2 norm (q2_scale). */
+/* Kinship Gram G = Z Z^T (n x n FP64, leading dimension ld) for Z (n x m) packed as operand planes with the strains
+ * as rows (rows_pad x kpad, flmm_pack_planes; K = the SNPs): the product of io.grm's K = Z Z^T / M and of the LOCO
+ * kernels (fasterlmm/io.py:257-265, core.py:708-716) on MFMA in the library's split precisions.  Upper-triangular
+ * 128 x 128 tile pairs x n_split K ranges run as workgroups; each folds its FP32 sums into FP64 every 1,024 k and
+ * writes its FP64 tile to partial (n_split x rows_pad x rows_pad doubles, caller scratch); the splits are summed
+ * in a fixed order into out, mirrored, so G is deterministic and exactly symmetric.  No counterpart in the
+ * reference's arithmetic, which forms K in FP64. */
+int flmm_gram(int prec, const void* z_hi, const void* z_lo, const float* z_scale, int64_t rows_pad, int64_t kpad,
+              int64_t n, int n_split, double* partial, double* out, int64_t ld, void* stream);
+
 /* FP64 row GEMM C = A B (A: rows x k, B: k x cols, C: rows x cols, all row-major; rows % 64 == 0, k % 32 == 0,
  * cols % 128 == 0); every element sums k in order, so results do not depend on the number of rows per call.  Used
  * for the factored weights' SNP side C = (US o US) A (with FLMM_ROTATE_US_X2) and for the FP64 operands of

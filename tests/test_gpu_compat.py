@@ -258,3 +258,24 @@ def test_large_f_tests_are_rescored_in_fp64(core, ex, precision):
     assert hit.sum() >= 4
     np.testing.assert_allclose(res.max_F.cpu().numpy()[:8][hit], ref["max_F"][:8][hit], rtol=1e-8)
     np.testing.assert_allclose(res.max_F.cpu().numpy(), ref["max_F"], rtol=1e-4)
+
+
+@pytest.mark.parametrize("precision", ["split_fp16", "split_bf16"])
+def test_split_gram_matches_fp64(precision):
+    """The kinship Gram on MFMA (engine.gram / flmm_gram): within the split format's rounding of the FP64 product
+    (relative to the largest diagonal entry), exactly symmetric, the same bits on a repeat, for ragged shapes
+    (one strain, 33 SNPs, several K splits)."""
+    from fasterlmm_amd import engine
+    rng = np.random.default_rng(4)
+    tol = 1e-6 if precision == "split_fp16" else 2e-5
+    for n, m in ((150, 1500), (300, 70000), (1, 40), (129, 33), (1000, 5000)):
+        Z = rng.standard_normal((n, m))
+        Zd = torch.from_numpy(Z).to(DEV)
+        with engine.configure(precision=precision):
+            G = engine.gram(Zd)
+            G2 = engine.gram(Zd)
+        assert torch.equal(G, G2), (n, m)
+        assert torch.equal(G, G.T), (n, m)
+        want = Z @ Z.T
+        err = float(np.abs(G.cpu().numpy() - want).max() / np.abs(np.diag(want)).max())
+        assert err <= tol, (n, m, err)
