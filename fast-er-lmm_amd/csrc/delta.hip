@@ -537,9 +537,11 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
 
 // Q1 = w o y and Q2 = w, w = 1 / (s_k + exp(log_delta[p])), written as the scan's column-side planes (zero on the
 // padding k in [neff, kpad) and on the padding columns p in [P, p_pad)); split-FP16 rows are multiplied by the
-// scales the refinement chose.  q2a == NULL: Q2 is not written (factored weights, factored_planes_kernel).  A block owns 8 adjacent columns and walks the K chunks: for one chunk its 256
-// elements are contiguous in every plane (element (p, k) at ((k / 32) p_pad + p) 32 + k % 32), so each chunk is
-// one 1 KiB read of UY and four contiguous 512 B (fp32: two 1 KiB) writes.
+// scales the refinement chose.  q2a == NULL: Q2 is not written (factored weights, factored_planes_kernel).
+// A block owns 32 adjacent columns and walks the K chunks; thread (column tid / 8, quad tid % 8) handles 4
+// consecutive k of each chunk, so for one chunk the block's UY read is one contiguous 4 KiB span (a 16-byte load
+// per lane) and its writes are contiguous 8-byte stores per plane (element (p, k) at
+// ((k / 32) p_pad + p) 32 + k % 32).
 template <int PREC>
 __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restrict__ uy, int64_t P, int64_t p_pad,
                                                            int64_t kpad, const double* __restrict__ s, int64_t neff,
@@ -547,8 +549,8 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
                                                            const float* __restrict__ q1s,
                                                            const float* __restrict__ q2s, void* q1a, void* q1b,
                                                            void* q2a, void* q2b) {
-  const int64_t p = (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);  // p < p_pad (p_pad is a multiple of 8)
-  const int j = threadIdx.x & 31;
+  const int64_t p = (int64_t)blockIdx.x * 32 + (threadIdx.x >> 3);  // p < p_pad (p_pad is a multiple of 32)
+  const int j4 = (threadIdx.x & 7) * 4;
   const bool live = p < P;
   const double delta = live ? exp(log_delta[p]) : 1.0;
   double sc1 = 1.0, sc2 = 1.0;
@@ -556,29 +558,37 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
     if (live) { sc1 = (double)q1s[p]; sc2 = q2s ? (double)q2s[p] : 1.0; }
   }
   const bool wq2 = q2a != nullptr;
-  const int64_t nch = kpad / 32;  // a multiple of 4 (kpad % 128 == 0): four chunks' loads in flight per thread
-#pragma unroll 4
+  const int64_t nch = kpad / 32;  // a multiple of 4 (kpad % 128 == 0)
+#pragma unroll 2
   for (int64_t c = 0; c < nch; ++c) {
-    const int64_t k = c * 32 + j;
-    const int64_t off = (c * p_pad + p) * 32 + j;
-    double w = 0.0, q1 = 0.0;
-    if (live && k < neff) {
-      w = rcp_pos(s[k] + delta);
-      q1 = w * (double)uy[off];
+    const int64_t k0 = c * 32 + j4;
+    const int64_t off = (c * p_pad + p) * 32 + j4;
+    const f32x4 y4 = *(const f32x4*)(uy + off);
+    double w[4], q1[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      w[e] = 0.0;
+      q1[e] = 0.0;
+      if (live && k0 + e < neff) {
+        w[e] = rcp_pos(s[k0 + e] + delta);
+        q1[e] = w[e] * (double)y4[e];
+      }
     }
     if constexpr (is_split(PREC)) {
-      u16 hh, ll;
-      split2<PREC>(q1 * sc1, hh, ll);
-      ((u16*)q1a)[off] = hh;
-      ((u16*)q1b)[off] = ll;
+      u16 h[4], l[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) split2<PREC>(q1[e] * sc1, h[e], l[e]);
+      *(uint2*)((u16*)q1a + off) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
+      *(uint2*)((u16*)q1b + off) = make_uint2(l[0] | ((uint32_t)l[1] << 16), l[2] | ((uint32_t)l[3] << 16));
       if (wq2) {
-        split2<PREC>(w * sc2, hh, ll);
-        ((u16*)q2a)[off] = hh;
-        ((u16*)q2b)[off] = ll;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) split2<PREC>(w[e] * sc2, h[e], l[e]);
+        *(uint2*)((u16*)q2a + off) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
+        *(uint2*)((u16*)q2b + off) = make_uint2(l[0] | ((uint32_t)l[1] << 16), l[2] | ((uint32_t)l[3] << 16));
       }
     } else {
-      ((float*)q1a)[off] = (float)q1;
-      if (wq2) ((float*)q2a)[off] = (float)w;
+      *(f32x4*)((float*)q1a + off) = f32x4{(float)q1[0], (float)q1[1], (float)q1[2], (float)q1[3]};
+      if (wq2) *(f32x4*)((float*)q2a + off) = f32x4{(float)w[0], (float)w[1], (float)w[2], (float)w[3]};
     }
   }
 }
@@ -696,7 +706,7 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
 #undef FLMM_REFINE1
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const dim3 pgrid((unsigned)(p_pad / 8));
+  const dim3 pgrid((unsigned)(p_pad / 32));
 #define FLMM_PLANES(PR)                                                                                              \
   hipLaunchKernelGGL((delta_planes_kernel<PR>), pgrid, dim3(256), 0, stream, uy, P, p_pad, kpad, s, neff, log_delta, \
                      q1s, q2s, q1a, q1b, q2a, q2b)

@@ -201,8 +201,10 @@ class _Basis:
         inv = inv.contiguous()
         US = self._rotate_f64(columns(local.index_select(0, um)))
         UY = self._uy64.get(n_real)
-        if UY is None:
-            UY = self._uy64[n_real] = self._rotate_f64(self.y_cols[:, :n_real])
+        if UY is None:  # does not depend on the SNP block, so a library FP64 GEMM keeps results block-invariant
+            UY = torch.zeros(n_real, self.kpad, dtype=torch.float64, device=self.mat.device)
+            UY[:, : self.neff] = self.y_cols[:, :n_real].T @ self.mat[:, self.col0:self.col0 + self.neff]
+            self._uy64[n_real] = UY
         _lib.call("flmm_rescore_f64", US.data_ptr(), UY.data_ptr(), self.kpad, self.s.data_ptr(), self.neff,
                   d.log_delta.data_ptr(), inv.data_ptr(), pi.data_ptr(), row_index.index_select(0, mi).data_ptr(), n,
                   float(self.nu), out.f.data_ptr(), out.beta.data_ptr(), out.se.data_ptr(), out.sfve.data_ptr(),
