@@ -41,7 +41,7 @@ static bool two_planes(int p) { return p != FLMM_PREC_FP32; }
 
 extern "C" {
 
-int flmm_abi_version(void) { return 6; }
+int flmm_abi_version(void) { return 7; }
 
 const char* flmm_last_error(void) { return g_err.c_str(); }
 
@@ -212,6 +212,26 @@ int flmm_gram(int prec, const void* z_hi, const void* z_lo, const float* z_scale
   else FLMM_GRAM(PREC_FP32);
 #undef FLMM_GRAM
   return check_launch(e, "flmm_gram");
+}
+
+int flmm_gram_f64(const double* z, int64_t ldz, int64_t n, int64_t m, int n_split, double* partial, double* out,
+                  int64_t ld, void* stream) {
+  REQUIRE(n > 0 && m > 0 && ldz >= m && n_split >= 1 && ld >= n,
+          "flmm_gram_f64: need n > 0, m > 0, ldz >= m, n_split >= 1, ld >= n");
+  REQUIRE(z && partial && out, "flmm_gram_f64: null pointer");
+  return check_launch(launch_gram_f64(z, ldz, n, m, n_split, partial, out, ld, (hipStream_t)stream), "flmm_gram_f64");
+}
+
+int flmm_rotate_f64(const double* z, int64_t ldz, const int64_t* col_index, int64_t mc, int64_t n, const double* u,
+                    int64_t ldu, int64_t neff, double* out, int64_t rows_pad, int64_t kpad, void* stream) {
+  REQUIRE(mc >= 0 && n >= 1 && neff >= 1 && neff <= kpad && kpad % TILE == 0 && rows_pad % TILE == 0 &&
+              rows_pad >= mc && ldu >= neff && (col_index || ldz >= mc) && ldz >= 1,
+          "flmm_rotate_f64: need 0 <= mc <= rows_pad, 1 <= neff <= kpad, rows_pad and kpad %% 128, ldu >= neff, "
+          "ldz >= mc without col_index");
+  REQUIRE(z && u && out, "flmm_rotate_f64: null pointer");
+  if (rows_pad == 0) return FLMM_OK;
+  return check_launch(launch_rotate_f64(z, ldz, col_index, mc, n, u, ldu, neff, out, rows_pad, kpad, (hipStream_t)stream),
+                      "flmm_rotate_f64");
 }
 
 int flmm_gemm_f64(const double* a, int64_t rows, int64_t k, const double* b, int64_t cols, double* c, void* stream) {

@@ -95,6 +95,93 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(const double* __restri
   out[(e / n) * ld + e % n] = acc;
 }
 
+// FP64 Gram on the FP64 matrix cores (v_mfma_f64_16x16x4f64), reading Z (n x m, row stride ldz) directly.  One
+// 256-thread workgroup per (upper tile pair, K range), as above; the four waves split the 128 x 128 tile 2 x 2 and
+// each holds 4 x 4 blocks of 16 x 16 FP64 accumulators.  K is staged through LDS 16 k at a time (rows padded to 18
+// doubles: a half-wave's 8-byte reads of one k over 16 rows and of the next k cover the 64 banks once); the next
+// stage's values are fetched into registers while the current one is multiplied.  Every product and sum is FP64,
+// in a fixed order, so the result is deterministic and agrees with any FP64 evaluation of Z Z^T to FP64 rounding.
+constexpr int G64_KT = 16;              // k per LDS stage
+constexpr int G64_LD = G64_KT + 2;      // LDS row stride (doubles)
+constexpr int G64_ROWS_PER_PASS = NTHREADS / G64_KT;  // rows one load instruction covers (16)
+constexpr int G64_LOADS = TILE / G64_ROWS_PER_PASS;    // loads per thread per operand and stage (8)
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(NTHREADS) void gram_f64_kernel(const double* __restrict__ Z, int64_t ldz, int64_t n,
+                                                            int64_t m, int n_tiles, int64_t per,
+                                                            double* __restrict__ partial, int64_t n_pad) {
+  __shared__ double sa[TILE * G64_LD], sb[TILE * G64_LD];
+  int64_t pair = blockIdx.x, ti = 0;
+  while (pair >= n_tiles - ti) {
+    pair -= n_tiles - ti;
+    ++ti;
+  }
+  const int64_t tj = ti + pair;
+  const bool diag = ti == tj;  // one operand tile serves both sides
+  const int s = blockIdx.y;
+  const int64_t k0 = (int64_t)s * per, k1 = k0 + per < m ? k0 + per : m;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int kk = tid % G64_KT, r0 = tid / G64_KT;
+
+  f64x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+  double ra[G64_LOADS], rb[G64_LOADS];
+  auto fetch = [&](int64_t kb) {
+    const int64_t k = kb + kk;
+#pragma unroll
+    for (int j = 0; j < G64_LOADS; ++j) {
+      const int64_t ia = ti * TILE + r0 + j * G64_ROWS_PER_PASS, ib = tj * TILE + r0 + j * G64_ROWS_PER_PASS;
+      ra[j] = (k < k1 && ia < n) ? Z[ia * ldz + k] : 0.0;
+      if (!diag) rb[j] = (k < k1 && ib < n) ? Z[ib * ldz + k] : 0.0;
+    }
+  };
+  const double* sbr = diag ? sa : sb;
+  if (k0 < k1) fetch(k0);
+  for (int64_t kb = k0; kb < k1; kb += G64_KT) {
+#pragma unroll
+    for (int j = 0; j < G64_LOADS; ++j) {
+      sa[(r0 + j * G64_ROWS_PER_PASS) * G64_LD + kk] = ra[j];
+      if (!diag) sb[(r0 + j * G64_ROWS_PER_PASS) * G64_LD + kk] = rb[j];
+    }
+    __syncthreads();
+    if (kb + G64_KT < k1) fetch(kb + G64_KT);  // in flight during this stage's products
+#pragma unroll
+    for (int ks = 0; ks < G64_KT / 4; ++ks) {
+      // A[i][k] of a 16 x 4 block at lane i + 16 k; B[k][j] = Z[j][k] at lane j + 16 k
+      const int col = ks * 4 + lane / 16;
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = sa[(wm * 64 + i * 16 + lane % 16) * G64_LD + col];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = sbr[(wn * 64 + j * 16 + lane % 16) * G64_LD + col];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();  // the next stage overwrites the LDS
+  }
+
+  // accumulator item v of lane l holds row l / 16 + 4 v, column l % 16 of its 16 x 16 block
+  double* out = partial + (int64_t)s * n_pad * n_pad;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t r = ti * TILE + wm * 64 + i * 16 + lane / 16 + 4 * v;
+        const int64_t c = tj * TILE + wn * 64 + j * 16 + lane % 16;
+        out[r * n_pad + c] = acc[i][j][v];
+      }
+}
+
 int gram_tile_pairs(int64_t rows_pad) {
   const int64_t t = rows_pad / TILE;
   return (int)(t * (t + 1) / 2);
@@ -112,6 +199,21 @@ hipError_t launch_gram(const TileArgs<PREC, 1>& args, const float* row_scale, in
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(gram_reduce_kernel, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, stream, partial, n_split,
                      args.a_rows_pad, n, out, ld);
+  return hipGetLastError();
+}
+
+hipError_t launch_gram_f64(const double* Z, int64_t ldz, int64_t n, int64_t m, int n_split, double* partial,
+                           double* out, int64_t ld, hipStream_t stream) {
+  const int64_t n_pad = (n + TILE - 1) / TILE * TILE;
+  const int64_t stages = (m + G64_KT - 1) / G64_KT;
+  const int64_t per = (stages + n_split - 1) / n_split * G64_KT;  // k per split, whole stages
+  const dim3 grid((unsigned)gram_tile_pairs(n_pad), (unsigned)n_split);
+  hipLaunchKernelGGL(gram_f64_kernel, grid, dim3(NTHREADS), 0, stream, Z, ldz, n, m, (int)(n_pad / TILE), per,
+                     partial, n_pad);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(gram_reduce_kernel, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, stream, partial, n_split,
+                     n_pad, n, out, ld);
   return hipGetLastError();
 }
 

@@ -51,6 +51,11 @@ template <int PREC>
 hipError_t launch_gram(const TileArgs<PREC, 1>& args, const float* row_scale, int n_split, double* partial, int64_t n,
                        double* out, int64_t ld, hipStream_t stream);
 int gram_tile_pairs(int64_t rows_pad);
+hipError_t launch_rotate_f64(const double* z, int64_t ldz, const int64_t* col_index, int64_t mc, int64_t n,
+                             const double* u, int64_t ldu, int64_t neff, double* out, int64_t rows_pad, int64_t kpad,
+                             hipStream_t stream);
+hipError_t launch_gram_f64(const double* Z, int64_t ldz, int64_t n, int64_t m, int n_split, double* partial,
+                           double* out, int64_t ld, hipStream_t stream);
 hipError_t launch_factored_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
                                   int64_t R, const double* log_delta, void* qa, void* qb, float* qs,
                                   hipStream_t stream);

@@ -215,6 +215,95 @@ hipError_t launch_rescore(const double* US, const double* UY, int64_t ld, const 
   return hipGetLastError();
 }
 
+// FP64 rotation of SNP columns on the FP64 matrix cores (v_mfma_f64_16x16x4f64): out[i][j] = sum_n z[n][c(i)]
+// u[n][j], c(i) = col_index[i] (or i), for i < mc, j < neff, zeros elsewhere in the rows_pad x kpad output -- U_eff^T
+// S_r of core.py:628-629 evaluated in FP64, whose products enter the scan weighted by 1 / (s_k + delta), large where
+// both are small.  One 256-thread workgroup per 128 x 128 output tile, the four waves 2 x 2, each with 4 x 4 blocks
+// of 16 x 16 FP64 accumulators; both operands are strain-major (k = strain), staged 16 strains at a time in
+// k-major LDS rows 144 doubles apart (a half-wave's 8-byte reads of 16 consecutive entries of two k rows cover the
+// 64 banks once); the next stage is fetched into registers while the current one is multiplied.
+constexpr int R64_KT = 16;
+constexpr int R64_LD = TILE + 16;
+constexpr int R64_LOADS = R64_KT / (NTHREADS / TILE);  // k rows per thread and stage (8)
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(NTHREADS) void rotate_f64_kernel(const double* __restrict__ z, int64_t ldz,
+                                                              const int64_t* __restrict__ col_index, int64_t mc,
+                                                              int64_t n, const double* __restrict__ u, int64_t ldu,
+                                                              int64_t neff, double* __restrict__ out, int64_t ldo) {
+  __shared__ double sa[R64_KT * R64_LD], sb[R64_KT * R64_LD];
+  const int64_t i0 = (int64_t)blockIdx.x * TILE, j0 = (int64_t)blockIdx.y * TILE;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int li = tid % TILE, lk = tid / TILE;  // this thread stages entry li of k rows lk, lk + 2, ..., lk + 14
+  const bool has_a = i0 + li < mc, has_b = j0 + li < neff;
+  const int64_t zc = has_a ? (col_index ? col_index[i0 + li] : i0 + li) : 0;
+  const int64_t uc = j0 + li;
+
+  f64x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+  double ra[R64_LOADS], rb[R64_LOADS];
+  auto fetch = [&](int64_t kb) {
+#pragma unroll
+    for (int q = 0; q < R64_LOADS; ++q) {
+      const int64_t k = kb + lk + 2 * q;
+      ra[q] = (has_a && k < n) ? z[k * ldz + zc] : 0.0;
+      rb[q] = (has_b && k < n) ? u[k * ldu + uc] : 0.0;
+    }
+  };
+  fetch(0);
+  for (int64_t kb = 0; kb < n; kb += R64_KT) {
+#pragma unroll
+    for (int q = 0; q < R64_LOADS; ++q) {
+      sa[(lk + 2 * q) * R64_LD + li] = ra[q];
+      sb[(lk + 2 * q) * R64_LD + li] = rb[q];
+    }
+    __syncthreads();
+    if (kb + R64_KT < n) fetch(kb + R64_KT);  // in flight during this stage's products
+#pragma unroll
+    for (int ks = 0; ks < R64_KT / 4; ++ks) {
+      // A[i][k] of a 16 x 4 block at lane i + 16 k, B[k][j] at lane j + 16 k
+      const int row = (ks * 4 + lane / 16) * R64_LD + lane % 16;
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = sa[row + wm * 64 + i * 16];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = sb[row + wn * 64 + j * 16];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();  // the next stage overwrites the LDS
+  }
+
+  // accumulator item v of lane l holds row l / 16 + 4 v, column l % 16 of its 16 x 16 block
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t r = i0 + wm * 64 + i * 16 + lane / 16 + 4 * v;
+        const int64_t c = j0 + wn * 64 + j * 16 + lane % 16;
+        out[r * ldo + c] = acc[i][j][v];
+      }
+}
+
+hipError_t launch_rotate_f64(const double* z, int64_t ldz, const int64_t* col_index, int64_t mc, int64_t n,
+                             const double* u, int64_t ldu, int64_t neff, double* out, int64_t rows_pad, int64_t kpad,
+                             hipStream_t stream) {
+  const dim3 grid((unsigned)(rows_pad / TILE), (unsigned)(kpad / TILE));
+  hipLaunchKernelGGL(rotate_f64_kernel, grid, dim3(NTHREADS), 0, stream, z, ldz, col_index, mc, n, u, ldu, neff, out,
+                     kpad);
+  return hipGetLastError();
+}
+
 template hipError_t launch_rotate<PREC_SPLIT_BF16>(int, const TileArgs<PREC_SPLIT_BF16, 1>&, const float*,
                                                     const float*, void*, void*, void*, void*, float*, float*,
                                                     hipStream_t);

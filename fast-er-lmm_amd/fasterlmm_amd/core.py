@@ -124,6 +124,7 @@ def eigendecompose(K: Tensor, jitter: float = 1e-6) -> tuple[Tensor, Tensor]:
 # carry ~1e-7 relative error, and rss = rWr - sWy^2 / sWs amplifies it by ~1 + F / nu, while |d log10 p| grows like
 # F x (relative F error) / (2 ln 10): below F = 30 the error stays near 1e-6, far inside the 1e-3 bar.
 RESCORE_F = 30.0
+EXACT_COLS = 4096  # leading columns whose rotation rotate_columns takes from the FP64 product by default
 
 
 class _Basis:
@@ -144,9 +145,10 @@ class _Basis:
         self.s = s.contiguous()
         self.s_range = tuple(torch.stack(torch.aminmax(self.s)).tolist()) if n_basis > 0 else (0.0, 0.0)
         self.s_ml = None if s_ml is None else s_ml.contiguous()
+        mat = mat.contiguous()
         # B^T planes: row j = basis vector j, element k = mat[k, col0 + j]
-        self.bt = engine.pack(prec, mat.contiguous(), n_basis, N, 1, L, self.kpad, self.npad, offset=col0)
-        self.mat, self.col0 = mat, col0  # the FP64 basis, for the FP64 re-scoring
+        self.bt = engine.pack(prec, mat, n_basis, N, 1, L, self.kpad, self.npad, offset=col0)
+        self.mat, self.col0 = mat, col0  # the FP64 basis: SNP rotation, exact column rows, re-scoring
         self.y_cols = None  # FP64 scan columns (at least the real ones), for the FP64 re-scoring
         self._u_pad = None
         self._uy64 = {}
@@ -155,20 +157,39 @@ class _Basis:
         self.p_pad = 0
         self.delta = None  # the DeltaResult of the last fit (its log_delta tensor identifies it)
 
-    def rotate_columns(self, Y: Tensor) -> None:
-        """UY plane of B^T Y for Y (N, P) FP64."""
+    def rotate_columns(self, Y: Tensor, n_exact: int | None = None) -> None:
+        """UY plane of B^T Y for Y (N, P) FP64; its first n_exact columns (default: all, up to EXACT_COLS) from the
+        FP64 product."""
         N, P = Y.shape
         self.P, self.p_pad = P, rup(max(P, 1), PTILE)
         ypl = engine.pack(self.prec, Y.contiguous(), P, N, 1, P, self.p_pad, self.npad)
         self.uy = engine.rotate_uy(ypl, self.bt)
         self.y_cols, self._uy64 = Y, {}
+        self._exact_rows(Y[:, : min(P, EXACT_COLS if n_exact is None else int(n_exact))])
 
     def use_column_planes(self, ypl: engine.Planes, P: int, y_real: Tensor | None = None) -> None:
         """UY plane from pre-packed column planes (shared by every chromosome of a LOCO scan); y_real: the FP64
-        real columns (N, n_real) for the re-scoring."""
+        real columns (N, n_real), whose rows come from the FP64 product and which the re-scoring reads."""
         self.P, self.p_pad = P, ypl.rows_pad
         self.uy = engine.rotate_uy(ypl, self.bt)
         self.y_cols, self._uy64 = y_real, {}
+        if y_real is not None:
+            self._exact_rows(y_real)
+
+    def _exact_rows(self, y: Tensor) -> None:
+        """Rows 0 .. n-1 of the UY plane (the leading, real columns y (N, n) FP64) from the FP64 product B^T y,
+        rounded once to FP32.  The split rotation's error is ~2^-22 of |y| in every basis direction; the scan weighs
+        direction k by 1 / (s_k + delta), so where a column's delta and some eigenvalues are small (h2 near 1,
+        delta ~ 1e-4 at the grid's lower edge) that error would reach beta and the delta fit amplified ~1e4
+        times, while the FP64 product leaves each entry's own FP32 rounding only."""
+        n = y.shape[1]
+        if n == 0 or self.neff == 0:
+            return
+        UY = torch.zeros(n, self.kpad, dtype=torch.float64, device=y.device)
+        UY[:, : self.neff] = y.T @ self.mat[:, self.col0:self.col0 + self.neff]
+        self._uy64[n] = UY  # the re-scoring's FP64 operand
+        nch = self.kpad // KCHUNK
+        self.uy.view(nch, self.p_pad, KCHUNK)[:, :n, :] = UY.float().view(n, nch, KCHUNK).permute(1, 0, 2)
 
     def _rotate_f64(self, cols: Tensor) -> Tensor:
         """FP64 B^T c for the columns c of cols (N, n): rows (rup(n, 64), kpad), row i = column i rotated."""
@@ -201,6 +222,9 @@ class _Basis:
         inv = inv.contiguous()
         US = self._rotate_f64(columns(local.index_select(0, um)))
         UY = self._uy64.get(n_real)
+        wider = [k for k in self._uy64 if k > n_real]
+        if UY is None and wider:
+            UY = self._uy64[min(wider)][:n_real]
         if UY is None:  # does not depend on the SNP block, so a library FP64 GEMM keeps results block-invariant
             UY = torch.zeros(n_real, self.kpad, dtype=torch.float64, device=self.mat.device)
             UY[:, : self.neff] = self.y_cols[:, :n_real].T @ self.mat[:, self.col0:self.col0 + self.neff]
@@ -280,10 +304,17 @@ class _Basis:
         """Scan planes S1 = B^T s, S2 = (B^T s)^2 of the SNP columns Z[:, row_index] (core.py:628-631), or
         S2 = (B^T s)^2 A for the factored weights of d (default: the last fit)."""
         d = self.delta if d is None else d
+        fac = None if d is None else d.fac
         mc = row_index.numel()
+        rot = engine.settings.snp_rotation
+        if rot == "fp64":
+            us = engine.rotate_snps_f64(Z, row_index, mc, self.mat, self.col0, self.neff, rup(mc, TILE), self.kpad)
+            return engine.snp_planes_from_f64(self.prec, us, mc, self.neff, fac)
+        if rot != "split":
+            raise ValueError(f"unknown snp_rotation setting {rot!r}")
         spl = engine.pack(self.prec, Z, mc, self.N, 1, Z.shape[1], rup(mc, TILE), self.npad, row_index=row_index)
-        if d is not None and d.fac is not None:
-            return engine.factored_snp_planes(spl, self.bt, mc, d.fac)
+        if fac is not None:
+            return engine.factored_snp_planes(spl, self.bt, mc, fac)
         return engine.rotate_us(spl, self.bt)
 
 

@@ -2,7 +2,7 @@
 
 Every array here is a torch tensor on the GPU used as device memory; all arithmetic of the hot path runs in the
 hand-written kernels of libfasterlmm_amd.so (called through ``_lib``).  torch is used for allocation, for the
-small FP64 setup products (kinship Gram, covariate projector, N x N projection) that rocBLAS runs, and for
+small FP64 setup products (covariate projector, N x N projection) that rocBLAS runs, and for
 rocSOLVER's FP64 symmetric eigendecomposition.
 
 Data layout (HBM)
@@ -41,8 +41,13 @@ class Settings:
                                    # "learned" (MLP proposal + judged Newton, grid as fallback and judge)
     weights: str = "auto"          # sWs weights 1 / (s + delta): "factored" (exponential sum, second product over
                                    # ~128 terms), "exact" (over Neff), "auto" = factored for the split precisions
-    kinship: str = "auto"          # kinship Gram Z Z^T: "split" (MFMA in the split precision, FP64 folds every
-                                   # 1,024 k), "fp64" (rocBLAS), "auto" = split for the split precisions
+    kinship: str = "auto"          # kinship Gram Z Z^T: "fp64" (FP64 MFMA kernel), "blas" (rocBLAS FP64), "split"
+                                   # (MFMA in the split precision, FP64 folds every 1,024 k; ~1e-7 of the diagonal,
+                                   # which moves beta by ~1e-4 where delta sits near 1e-4 next to eigenvalues of
+                                   # that size), "auto" = fp64
+    snp_rotation: str = "fp64"     # SNP side US = U_eff^T S: "fp64" (FP64 MFMA kernel, then packed) or "split" (one
+                                   # fused MFMA pass in the split precision: ~2^-22 of |S| in every direction, which
+                                   # the scan weighs by 1 / (s_k + delta))
 
 
 settings = Settings()
@@ -148,23 +153,33 @@ def pack(prec: int, src: torch.Tensor, n_rows: int, n_k: int, row_stride: int, k
 
 
 def gram(Z: torch.Tensor, prec: int | None = None) -> torch.Tensor:
-    """G = Z Z^T (n x n FP64) of Z (n, m) FP64 on the GPU: flmm_gram on MFMA in the split precisions (the kinship
-    setting "split", default for them), rocBLAS FP64 otherwise.  Deterministic: the same Z gives the same bits."""
+    """G = Z Z^T (n x n FP64) of Z (n, m) FP64 on the GPU, by the kinship setting: flmm_gram_f64 on the FP64
+    matrix cores (default), rocBLAS ("blas"), or flmm_gram in the split precision ("split").  Deterministic: the same
+    Z gives the same bits."""
     prec = prec_code() if prec is None else prec
     k = settings.kinship
-    if k not in ("auto", "split", "fp64"):
+    if k not in ("auto", "fp64", "blas", "split"):
         raise ValueError(f"unknown kinship setting {k!r}")
-    split = prec in (_lib.PREC_SPLIT_FP16, _lib.PREC_SPLIT_BF16)
-    if k == "fp64" or (k == "auto" and not split):
+    if k == "blas":
         return Z @ Z.T
     n, m = Z.shape
-    if m == 0:
+    if m == 0 or n == 0:
         return torch.zeros(n, n, dtype=torch.float64, device=Z.device)
     Z = Z.contiguous()
-    rows_pad, kpad = rup(n, TILE), rup(m, KCHUNK)
-    pl = pack(prec, Z, n, m, m, 1, rows_pad, kpad)
+    rows_pad = rup(n, TILE)
     t = rows_pad // TILE
     pairs = t * (t + 1) // 2
+    if k in ("auto", "fp64"):
+        n_split = max(1, min(-(-1024 // pairs), -(-m // 256)))  # >= 1,024 workgroups where K allows
+        partial = torch.empty(n_split, rows_pad, rows_pad, dtype=torch.float64, device=Z.device)
+        out = torch.empty(n, n, dtype=torch.float64, device=Z.device)
+        _lib.call("flmm_gram_f64", Z.data_ptr(), m, n, m, n_split, partial.data_ptr(), out.data_ptr(), n,
+                  _lib.stream())
+        return out
+    if prec not in (_lib.PREC_SPLIT_FP16, _lib.PREC_SPLIT_BF16):
+        raise ValueError("the split kinship needs a split precision")
+    kpad = rup(m, KCHUNK)
+    pl = pack(prec, Z, n, m, m, 1, rows_pad, kpad)
     n_split = max(1, min(-(-1024 // pairs), -(-(kpad // KCHUNK) // 32)))  # >= 1,024 workgroups where K allows
     partial = torch.empty(n_split, rows_pad, rows_pad, dtype=torch.float64, device=Z.device)
     out = torch.empty(n, n, dtype=torch.float64, device=Z.device)
@@ -271,6 +286,37 @@ def attach_factored(prec: int, d: DeltaResult, fac: Factored) -> None:
     _lib.call("flmm_factored_weight_planes", prec, P, p_pad, fac.rpad, fac.t.data_ptr(), fac.lam.data_ptr(), fac.R,
               d.log_delta.data_ptr(), q2h, q2l, q2.scale_ptr, _lib.stream())
     d.q2, d.fac = q2, fac
+
+
+def rotate_snps_f64(Z: torch.Tensor, col_index: torch.Tensor | None, mc: int, mat: torch.Tensor, col0: int,
+                    neff: int, rows_pad: int, kpad: int) -> torch.Tensor:
+    """US (rows_pad, kpad) FP64, US[i][j] = sum_n Z[n, c(i)] mat[n, col0 + j] (c = col_index, or i), zero outside
+    i < mc, j < neff: flmm_rotate_f64 on the FP64 matrix cores."""
+    if Z.stride(1) != 1:
+        Z = Z.contiguous()
+    if mat.stride(1) != 1:
+        mat = mat.contiguous()
+    if col_index is not None:
+        col_index = col_index.to(torch.int64).contiguous()
+    out = torch.empty(rows_pad, kpad, dtype=torch.float64, device=Z.device)
+    _lib.call("flmm_rotate_f64", Z.data_ptr(), Z.stride(0), _lib.ptr(col_index), mc, Z.shape[0],
+              mat.data_ptr() + 8 * col0, mat.stride(0), neff, out.data_ptr(), rows_pad, kpad, _lib.stream())
+    return out
+
+
+def snp_planes_from_f64(prec: int, us: torch.Tensor, mc: int, neff: int,
+                        fac: Factored | None) -> tuple[Planes, Planes]:
+    """Scan planes of FP64 rotated SNPs us (rows_pad, kpad): S1 = us and S2 = us o us, or S2 = (us o us) A for
+    factored weights."""
+    rows_pad, kpad = us.shape
+    s1 = pack(prec, us, mc, neff, kpad, 1, rows_pad, kpad)
+    x2 = us * us
+    if fac is None:
+        return s1, pack(prec, x2, mc, neff, kpad, 1, rows_pad, kpad)
+    c = torch.empty(rows_pad, fac.rpad, dtype=torch.float64, device=us.device)
+    _lib.call("flmm_gemm_f64", x2.data_ptr(), rows_pad, kpad, fac.A.data_ptr(), fac.rpad, c.data_ptr(), _lib.stream())
+    del x2
+    return s1, pack(prec, c, mc, fac.rpad, fac.rpad, 1, rows_pad, fac.rpad)
 
 
 def factored_snp_planes(a: Planes, bt: Planes, mc: int, fac: Factored) -> tuple[Planes, Planes]:

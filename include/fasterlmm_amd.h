@@ -193,6 +193,23 @@ int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_
 int flmm_gram(int prec, const void* z_hi, const void* z_lo, const float* z_scale, int64_t rows_pad, int64_t kpad,
               int64_t n, int n_split, double* partial, double* out, int64_t ld, void* stream);
 
+/* Kinship Gram G = Z Z^T (n x n FP64, leading dimension ld) of Z (n x m FP64, row-major, row stride ldz >= m) on
+ * the FP64 matrix cores: io.grm's K = Z Z^T / M before the division and the LOCO kernels (fasterlmm/io.py:257-265,
+ * core.py:708-716), in the reference's FP64 arithmetic.  Upper-triangular 128 x 128 tile pairs x n_split K ranges
+ * run as workgroups, each writing its FP64 tile to partial (n_split x rows_pad x rows_pad doubles, rows_pad = n
+ * rounded up to 128, caller scratch); the splits are summed in a fixed order into out and mirrored, so G is
+ * deterministic and exactly symmetric. */
+int flmm_gram_f64(const double* z, int64_t ldz, int64_t n, int64_t m, int n_split, double* partial, double* out,
+                  int64_t ld, void* stream);
+
+/* FP64 rotation of SNP columns: out[i][j] = sum_{k<n} z[k * ldz + c(i)] * u[k * ldu + j], c(i) = col_index[i] (or i
+ * when col_index is NULL), for i < mc, j < neff; the rest of the rows_pad x kpad output (row-major, multiples of
+ * 128) is zero.  z = the standardised genotypes (strains x SNPs), u = the basis U_eff (strains x neff, pointer at
+ * its first column): US = U_eff^T S_r of core.py:628-629 in the reference's FP64, on the FP64 matrix cores; the
+ * caller packs US and its square (or factored products) into the scan's SNP planes. */
+int flmm_rotate_f64(const double* z, int64_t ldz, const int64_t* col_index, int64_t mc, int64_t n, const double* u,
+                    int64_t ldu, int64_t neff, double* out, int64_t rows_pad, int64_t kpad, void* stream);
+
 /* FP64 row GEMM C = A B (A: rows x k, B: k x cols, C: rows x cols, all row-major; rows % 64 == 0, k % 32 == 0,
  * cols % 128 == 0); every element sums k in order, so results do not depend on the number of rows per call.  Used
  * for the factored weights' SNP side C = (US o US) A (with FLMM_ROTATE_US_X2) and for the FP64 operands of

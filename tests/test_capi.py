@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_error_channel():
     from fasterlmm_amd import _lib
     lib = _lib.load()
-    assert lib.flmm_abi_version() == 6
+    assert lib.flmm_abi_version() == 7
     # argument validation runs on the host and reports through flmm_last_error without touching a GPU
     rc = lib.flmm_max_reduce(None, 1, 0, 4, None, None)
     assert rc == 1
@@ -51,6 +51,14 @@ def test_bad_arguments_raise_value_error():
         _lib.call("flmm_gemm_f64", 16, 100, 32, 16, 128, 16, None)
     with pytest.raises(ValueError, match="null pointer"):
         _lib.call("flmm_gemm_f64", None, 64, 32, 16, 128, 16, None)
+    with pytest.raises(ValueError, match="neff <= kpad"):
+        _lib.call("flmm_rotate_f64", 16, 10, None, 5, 10, 16, 200, 200, 16, 128, 128, None)
+    with pytest.raises(ValueError, match="null pointer"):
+        _lib.call("flmm_rotate_f64", 16, 10, None, 5, 10, None, 200, 100, 16, 128, 128, None)
+    with pytest.raises(ValueError, match="ldz >= m"):
+        _lib.call("flmm_gram_f64", 16, 99, 10, 100, 1, 16, 16, 10, None)
+    with pytest.raises(ValueError, match="null pointer"):
+        _lib.call("flmm_gram_f64", 16, 100, 10, 100, 1, None, 16, 10, None)
     with pytest.raises(ValueError, match="bad sizes"):
         _lib.call("flmm_rescore_f64", 16, 16, 128, 16, 200, 16, 16, 16, 16, 5, 10.0, 16, 16, 16, 16, 4, None)
     with pytest.raises(ValueError, match="null pointer"):

@@ -260,18 +260,60 @@ def test_large_f_tests_are_rescored_in_fp64(core, ex, precision):
     np.testing.assert_allclose(res.max_F.cpu().numpy(), ref["max_F"], rtol=1e-4)
 
 
+def test_fp64_snp_rotation_matches_numpy():
+    """The SNP rotation on the FP64 matrix cores (engine.rotate_snps_f64, flmm_rotate_f64): U_eff^T Z[:, idx] to FP64
+    rounding for ragged shapes (fewer strains than one LDS stage, SNP and basis tiles with remainders), a gathered
+    and a contiguous column set, a basis starting at column col0 of a wider matrix, and zeros in the padding."""
+    from fasterlmm_amd import engine
+    rng = np.random.default_rng(15)
+    for n, m, mc, L, col0, neff, gather in ((150, 1500, 300, 150, 2, 148, True), (7, 40, 40, 7, 0, 7, False),
+                                           (1000, 3000, 1111, 1000, 3, 997, True), (260, 100, 33, 300, 100, 129, False)):
+        Z = rng.standard_normal((n, m))
+        U = rng.standard_normal((n, L))
+        idx = rng.permutation(m)[:mc] if gather else np.arange(mc)
+        rows_pad, kpad = -(-mc // 128) * 128, -(-neff // 128) * 128
+        us = engine.rotate_snps_f64(torch.from_numpy(Z).to(DEV), torch.from_numpy(idx).to(DEV) if gather else None,
+                                    mc, torch.from_numpy(U).to(DEV), col0, neff, rows_pad, kpad).cpu().numpy()
+        want = Z[:, idx].T @ U[:, col0:col0 + neff]
+        tol = 1e-13 * np.sqrt((Z[:, idx] ** 2).sum(0).max() * (U ** 2).sum(0).max())
+        np.testing.assert_allclose(us[:mc, :neff], want, rtol=0, atol=tol, err_msg=f"{n} {mc} {neff}")
+        assert not us[mc:].any() and not us[:, neff:].any()
+
+
+def test_fp64_gram_matches_numpy():
+    """The kinship Gram on the FP64 matrix cores (engine.gram's default, flmm_gram_f64): Z Z^T to FP64 rounding
+    (1e-13 of the largest diagonal entry), exactly symmetric, the same bits on a repeat, for ragged shapes (one
+    strain, fewer SNPs than one LDS stage, tiles and K splits with remainders) and for a strided view."""
+    from fasterlmm_amd import engine
+    rng = np.random.default_rng(14)
+    for n, m in ((1, 1), (150, 1500), (129, 33), (300, 70001), (1000, 5000), (17, 4099)):
+        Z = rng.standard_normal((n, m))
+        Zd = torch.from_numpy(Z).to(DEV)
+        G = engine.gram(Zd)
+        G2 = engine.gram(Zd)
+        want = Z @ Z.T
+        g = G.cpu().numpy()
+        np.testing.assert_allclose(g, want, rtol=0, atol=1e-13 * np.abs(np.diag(want)).max(), err_msg=f"{n}x{m}")
+        np.testing.assert_array_equal(g, g.T)
+        np.testing.assert_array_equal(g, G2.cpu().numpy())
+    Z = rng.standard_normal((200, 3000))
+    Zd = torch.from_numpy(Z).to(DEV)[:, 1000:2500]  # row stride 3000, engine.gram copies to contiguous
+    np.testing.assert_allclose(engine.gram(Zd).cpu().numpy(), Z[:, 1000:2500] @ Z[:, 1000:2500].T, rtol=0,
+                               atol=1e-13 * 1500)
+
+
 @pytest.mark.parametrize("precision", ["split_fp16", "split_bf16"])
 def test_split_gram_matches_fp64(precision):
-    """The kinship Gram on MFMA (engine.gram / flmm_gram): within the split format's rounding of the FP64 product
-    (relative to the largest diagonal entry), exactly symmetric, the same bits on a repeat, for ragged shapes
-    (one strain, 33 SNPs, several K splits)."""
+    """The split-precision kinship Gram (engine.gram with kinship="split", flmm_gram): within the split format's
+    rounding of the FP64 product (relative to the largest diagonal entry), exactly symmetric, the same bits on a
+    repeat, for ragged shapes (one strain, 33 SNPs, several K splits)."""
     from fasterlmm_amd import engine
     rng = np.random.default_rng(4)
     tol = 1e-6 if precision == "split_fp16" else 2e-5
     for n, m in ((150, 1500), (300, 70000), (1, 40), (129, 33), (1000, 5000)):
         Z = rng.standard_normal((n, m))
         Zd = torch.from_numpy(Z).to(DEV)
-        with engine.configure(precision=precision):
+        with engine.configure(precision=precision, kinship="split"):
             G = engine.gram(Zd)
             G2 = engine.gram(Zd)
         assert torch.equal(G, G2), (n, m)

@@ -169,7 +169,9 @@ def test_autotuner_choice(mods, example, tmp_path):
                 torch.testing.assert_close(a[k], b[k], rtol=1e-12, atol=0)
 
 
-def _na_dataset(example):
+def _na_dataset(example, plant: bool = False):
+    """Four missingness patterns over 18 phenotypes, missing calls in Z; plant: two phenotypes of three patterns
+    are chromosome-2 SNPs plus noise (F from ~60 to ~1.4e5, above the FP64 re-scoring threshold)."""
     from fasterlmm_amd.io import align_inputs
     n = example["Z_raw"].shape[0]
     Y = example["Y"].astype(np.float64).copy()
@@ -182,6 +184,12 @@ def _na_dataset(example):
     iid = [str(i) for i in range(n)]
     Z = example["Z_raw"].astype(np.float64)
     Z[rng.integers(0, n, 60), rng.integers(0, Z.shape[1], 60)] = np.nan  # missing calls
+    if plant:  # 13 tests above F = 60 on chromosome 2 (oracle), at several SNP positions
+        Zs, on2 = lmm.standardise_columns(Z), np.flatnonzero(example["chrom"] == 2)
+        noise = np.random.default_rng(8).standard_normal((n, 2))
+        for j in (0, 6, 12):
+            Y[:, j] = np.where(np.isnan(Y[:, j]), np.nan, Zs[:, on2[3]] + 0.005 * noise[:, 0])
+            Y[:, j + 1] = np.where(np.isnan(Y[:, j + 1]), np.nan, Zs[:, on2[40]] + 0.3 * noise[:, 1])
     return align_inputs(iid, Z, iid, Y, iid, example["X_cov2"][:, 1:], chrom=example["chrom"]), Z, Y
 
 
@@ -203,3 +211,38 @@ def test_perm_threshold_groups_vs_oracle(example, mode, loco):
         check_scan(r, ref, df, what=f"group of {len(keep)} strains")
         np.testing.assert_allclose(pmf.cpu().numpy(), ref_pmf, rtol=1e-4)
         np.testing.assert_allclose(r.max_F.cpu().numpy(), ref["max_F"], rtol=1e-4)
+
+
+@pytest.mark.parametrize("mode", ["grouped", "loop"])
+def test_grouped_large_f_tests_are_rescored(example, mode):
+    """Planted hits in three missingness patterns: every test above the re-scoring threshold equals the FP64
+    oracle's Wald statistic of the group (its strains, its chromosome-2 LOCO kernel, its real columns) to 1e-8,
+    evaluated at the driver's own log delta (recovered from nullh2: the driver's Newton refinement and the
+    oracle's golden section land ~1e-7 apart where the likelihood is flat, which F follows); the rest matches the
+    oracle's own run to the scan bars, and max_F of the hit columns follows their re-scored F."""
+    from fasterlmm_amd.core import RESCORE_F
+    from fasterlmm_amd.perms import perm_threshold_groups
+    data, Z, Y = _na_dataset(example, plant=True)
+    chrom = example["chrom"]
+    on2 = chrom == 2
+    res = perm_threshold_groups(data, list(range(18)), n_perm=3, seed=19930909, mode=mode)
+    X = data.X.cpu().numpy()
+    n_big = 0
+    for keep, cols, r, pmf in res:
+        what = f"group of {len(keep)} strains, planted hits"
+        ref, ref_pmf = operms.perm_threshold(Z[keep], X[keep], Y[keep][:, cols], chrom, list(range(len(cols))),
+                                             n_perm=3, seed=19930909, seed_idx=cols)
+        check_scan(r, ref, len(keep) - X.shape[1] - 1, what=what)
+        np.testing.assert_allclose(r.max_F.cpu().numpy(), ref["max_F"], rtol=1e-4)
+        h2 = r.nullh2.cpu().numpy()[np.flatnonzero(on2)[0]]
+        Zs = lmm.standardise_columns(Z[keep])
+        spec = lmm.compat_rotate(lmm.grm(Zs[:, ~on2]), X[keep], Y[keep][:, cols])
+        at = lmm.snp_wald_scan_compat(spec, np.log(1.0 / h2 - 1.0), Zs[:, on2])
+        big = at["f"] > 2 * RESCORE_F
+        n_big += int(big.sum())
+        for k in ("f", "beta", "se"):
+            np.testing.assert_allclose(getattr(r, k).cpu().numpy()[on2][big], at[k][big], rtol=1e-8,
+                                       err_msg=f"{k}, {what}")
+        hit = big.any(axis=0)
+        np.testing.assert_allclose(r.max_F.cpu().numpy()[: len(cols)][hit], at["f"].max(axis=0)[hit], rtol=1e-8)
+    assert n_big >= 12
