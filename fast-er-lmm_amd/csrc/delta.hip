@@ -224,6 +224,21 @@ struct ColLik {
     return wave_sum(r);
   }
 
+  // R at three deltas in one sweep (register path): each sum in the same order as R_only, so the values are
+  // bit-identical to three R_only calls
+  __device__ void R_three(const double (&delta)[3], double (&r)[3], int lane) const {
+    double a[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < (KPL > 0 ? KPL : 1); ++i) {
+      const double y = (double)yr[i];
+      const double sk = ls[lane + 64 * i];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) a[j] += y * y * rcp_pos(sk + delta[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r[j] = wave_sum(a[j]);
+  }
+
   // FP32 slope L'(t) and curvature L''(t) (register path only): a cheap first approximation of the root
   __device__ void slope32(float t, int lane, float& g, float& gp) const {
     const float delta = __expf(t);
@@ -297,8 +312,46 @@ __device__ inline double node_loss(const LK& L, int64_t p, int g, int G, double 
 }
 
 template <class LK>
+__device__ inline double node_loss_from_R(const LK& L, double R, int g, const double* __restrict__ lam64) {
+  return L.nco * (LOG2PI + log(fmax(R, FLOOR64) / L.nco) + 1.0) + lam64[g];
+}
+
+template <class LK, bool FUSED>
 __device__ inline int fp64_grid_argmin(const LK& L, int64_t p, int i, int G, double lo, double hi,
                                        const double* __restrict__ lam64, int lane) {
+  if constexpr (FUSED) {
+    // the usual outcome -- the node is the minimum of its two neighbours -- costs one sweep for all three losses;
+    // a move continues node by node below, exactly as the unfused walk
+    if (lam64 && i > 0 && i + 1 < G) {
+      const double d[3] = {exp(grid_value(i - 1, G, lo, hi)), exp(grid_value(i, G, lo, hi)),
+                           exp(grid_value(i + 1, G, lo, hi))};
+      double r[3];
+      L.R_three(d, r, lane);
+      const double lm = node_loss_from_R(L, r[0], i - 1, lam64), l0 = node_loss_from_R(L, r[1], i, lam64),
+                   lp = node_loss_from_R(L, r[2], i + 1, lam64);
+      if (lm <= l0) {
+        --i;
+        double li = lm;
+        for (int it = 0; it < G && i > 0; ++it) {
+          const double l = node_loss(L, p, i - 1, G, lo, hi, lam64, lane);
+          if (!(l <= li)) break;
+          --i;
+          li = l;
+        }
+        return i;
+      }
+      if (!(lp < l0)) return i;
+      ++i;
+      double li = lp;
+      for (int it = 0; it < G && i + 1 < G; ++it) {
+        const double l = node_loss(L, p, i + 1, G, lo, hi, lam64, lane);
+        if (!(l < li)) break;
+        ++i;
+        li = l;
+      }
+      return i;
+    }
+  }
   double li = node_loss(L, p, i, G, lo, hi, lam64, lane);
   bool moved_left = false;
   for (int it = 0; it < G && i > 0; ++it) {
@@ -387,10 +440,11 @@ __device__ inline double newton_bracket(const LK& L, int64_t p, int i, int G, do
       tn = 0.5 * (a + b);
       newton = false;
     }
-    // a Newton step below 1e-7 leaves the new iterate within ~1e-14 (quadratic convergence): accept it without
-    // another evaluation; bisection steps stop on the 1e-13 criterion
+    // a Newton step h below 1e-5 leaves the new iterate within ~K h^2 of the root (quadratic convergence, K the
+    // ratio of L's third to twice its second derivative, ~1 in log delta): far below the ~1e-7 to which the loss
+    // itself fixes the minimiser, so it is accepted without another evaluation; bisection steps stop on 1e-13
     const double step = fabs(tn - t), scale = fmax(1.0, fabs(t));
-    const bool done = step <= 1e-13 * scale || (b - a) <= 1e-13 || (newton && step <= 1e-7 * scale);
+    const bool done = step <= 1e-13 * scale || (b - a) <= 1e-13 || (newton && step <= 1e-5 * scale);
     t = tn;
     if (done) break;
   }
@@ -437,7 +491,8 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
   int i = (mode == MODE_FIXED || mode == MODE_GRID) ? 0 : idx[p];
   // every search brackets the FP64 grid argmin, as the reference does (the FP32 grid's pick differs only where
   // neighbouring grid losses tie to FP32 precision, i.e. on flat likelihoods)
-  if (mode != MODE_FIXED && mode != MODE_GRID) i = fp64_grid_argmin(L, p, i, G, lo, hi, lam64, lane);
+  if (mode != MODE_FIXED && mode != MODE_GRID)
+    i = fp64_grid_argmin<ColLik<KPL, SAME>, (KPL > 0)>(L, p, i, G, lo, hi, lam64, lane);
   bool need_bracket = mode == MODE_NEWTON;
   if (mode == MODE_FIXED) {
     t = log_delta[p];  // the caller's log delta, used as given (core.py:639)
