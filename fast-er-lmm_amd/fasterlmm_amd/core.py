@@ -289,15 +289,22 @@ class _Basis:
                                    self.s, float(self.neff), 1, 0.0, 0.0, None, log_delta_in=log_delta, fac=fac)
 
     def scan(self, Z: Tensor, row_index: Tensor, n_real: int, out: engine.ScanOutputs, d: engine.DeltaResult,
-             block: int = 8192) -> None:
-        """Scan the SNP columns Z[:, row_index] of Z (N, M_total) FP64 into out."""
+             block: int = 8192, after_launch=None) -> None:
+        """Scan the SNP columns Z[:, row_index] of Z (N, M_total) FP64 into out.  after_launch() is called once,
+        after the first block's scan is queued and before the host waits for it (the FP64 re-scoring's selection
+        reads F back)."""
         Mc = row_index.numel()
         for b0 in range(0, Mc, block):
             ri = row_index[b0:b0 + block]
             s1, s2 = self.snp_planes(Z, ri, d)
             engine.wald_scan(s1, s2, d, ri.numel(), self.P, n_real, self.nu, ri, out)
             del s1, s2
+            if after_launch is not None:
+                after_launch()
+                after_launch = None
             self.rescore(lambda idx: Z.index_select(1, idx), ri, ri, n_real, out, d)
+        if after_launch is not None:
+            after_launch()
 
     def snp_planes(self, Z: Tensor, row_index: Tensor,
                    d: engine.DeltaResult | None = None) -> tuple[engine.Planes, engine.Planes]:
@@ -443,8 +450,9 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
         del Kps
         s_mls = [torch.clamp(x, min=0.0) for x in torch.linalg.eigvalsh(torch.stack(Ks))] if Ks else None
         del Ks
-    for k, c in enumerate(chroms):
-        test = torch.from_numpy(np.flatnonzero(chrom == c)).to(dev)
+
+    def prepare(k: int):
+        """Chromosome k's basis, rotated columns and delta fit, on the current stream."""
         s = torch.clamp(ev[k, D:N] - 1.0, min=0.0)
         with stage("pack_basis"):
             st = _Basis(s, Ub[k], D, N - D, prec, nu=N - D - 1, s_ml=None if s_mls is None else s_mls[k])
@@ -452,8 +460,35 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
             st.use_column_planes(ypl, P, y_real)
         with stage("delta"):
             _, d = st.fit(n_grid, -10.0, 10.0, True, projected_loss=(not plain) or reml)
+        done = torch.cuda.Event()
+        done.record()
+        return st, d, done
+
+    # With engine.settings.pipeline, chromosome k + 1 is prepared on a second stream while chromosome k scans (the
+    # delta search and rotations beside the MFMA-bound scan); otherwise both use the current stream, in order.
+    # Each chromosome's buffers are dropped only after the scan stream has been synchronised, so the caching
+    # allocator never hands out memory a queued kernel still reads.
+    main = torch.cuda.current_stream(dev)
+    side = torch.cuda.Stream(device=dev) if engine.settings.pipeline else main
+    side.wait_stream(main)
+    with torch.cuda.stream(side):
+        nxt = prepare(0)
+    for k, c in enumerate(chroms):
+        st, d, done = nxt
+        nxt = None
+        main.wait_event(done)
+        test = torch.from_numpy(np.flatnonzero(chrom == c)).to(dev)
+
+        def launch_next(k=k):
+            nonlocal nxt
+            if k + 1 < len(chroms):
+                with torch.cuda.stream(side):
+                    nxt = prepare(k + 1)
+
         with stage("snps+scan"):
-            st.scan(Zd, test, n_real, out, d)
+            st.scan(Zd, test, n_real, out, d, after_launch=launch_next)
+        main.synchronize()
+        del st, d, done
         if on_chrom is not None:
             on_chrom(k + 1, len(chroms))
     engine.refresh_real_max(out, n_real)

@@ -48,6 +48,9 @@ class Settings:
     snp_rotation: str = "fp64"     # SNP side US = U_eff^T S: "fp64" (FP64 MFMA kernel, then packed) or "split" (one
                                    # fused MFMA pass in the split precision: ~2^-22 of |S| in every direction, which
                                    # the scan weighs by 1 / (s_k + delta))
+    pipeline: bool = False         # LOCO: prepare chromosome c + 1 (basis, rotation, delta) on a second stream
+                                   # while chromosome c scans (same bits; ~1% less step time on the N=1000,
+                                   # M=100k workload, but the co-running kernels slow the scan kernel ~1.7x)
 
 
 settings = Settings()

@@ -145,6 +145,23 @@ def test_loco_scan_compat_vs_reference_golden(core, ex, tag):
     np.testing.assert_allclose(res.max_F.cpu().numpy(), ref["max_F"], rtol=1e-4)
 
 
+@pytest.mark.parametrize("precision", ["split_fp16", "fp32"])
+def test_loco_pipeline_is_bit_identical(core, ex, precision):
+    """The LOCO driver preparing chromosome c + 1 on a second stream while c scans (engine.settings.pipeline)
+    writes the same bits as the serial order, on a panel with more SNPs and columns than one tile (16 chromosomes,
+    20 phenotypes x 13 columns)."""
+    from fasterlmm_amd import engine
+    rng = np.random.default_rng(23)
+    Y = np.concatenate([ex["Y"]] + [ex["Y"][rng.permutation(ex["Y"].shape[0])] for _ in range(12)], axis=1)
+    args = tuple(torch.from_numpy(a).to(DEV) for a in (ex["Zs"], ex["X_cov2"], Y))
+    got = {}
+    for pipe in (False, True):
+        with engine.configure(precision=precision, pipeline=pipe):
+            got[pipe] = core.loco_scan_compat(*args, ex["chrom"], n_real=20)
+    for k in ("f", "beta", "se", "sfve", "nullh2", "max_F"):
+        assert torch.equal(getattr(got[True], k), getattr(got[False], k)), k
+
+
 def test_perm_threshold_vs_reference_golden(ex):
     from fasterlmm_amd.io import align_inputs
     from fasterlmm_amd.perms import perm_threshold
