@@ -95,20 +95,57 @@ __device__ inline double rcp_pos(double x) {
   return r;
 }
 
+// Wave-wide reductions in DPP steps, register to register (no LDS round trips, which a shuffle takes): within each
+// quad (quad_perm [1,0,3,2] and [2,3,0,1]), half-row and row of 16 (row_half_mirror, row_mirror); then rows 1 and 3
+// add lane 15 of rows 0 and 2 (row_bcast:15) and rows 2 and 3 add lane 31 (row_bcast:31), so lane 63 holds the
+// total, which readlane makes wave-uniform.  Rows a step does not write keep `old` (0 for sums, the value itself
+// for maxima).  64-bit values move as two 32-bit halves.
+template <int CTRL, int ROWS>
+__device__ inline float dpp_f32(float old, float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), __builtin_bit_cast(int, v),
+                                                               CTRL, ROWS, 0xF, false));
+}
+template <int CTRL, int ROWS>
+__device__ inline double dpp_f64(double old, double v) {
+  const unsigned long long o = __builtin_bit_cast(unsigned long long, old), x = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)x, CTRL, ROWS, 0xF, false);
+  const unsigned hi =
+      (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(x >> 32), CTRL, ROWS, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ inline double lane63_f64(double v) {
+  const unsigned long long x = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), 63);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_f64<0xB1, 0xF>(0.0, v);
+  v += dpp_f64<0x4E, 0xF>(0.0, v);
+  v += dpp_f64<0x141, 0xF>(0.0, v);
+  v += dpp_f64<0x140, 0xF>(0.0, v);
+  v += dpp_f64<0x142, 0xA>(0.0, v);
+  v += dpp_f64<0x143, 0xC>(0.0, v);
+  return lane63_f64(v);
 }
 __device__ inline double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
+  v = fmax(v, dpp_f64<0xB1, 0xF>(v, v));
+  v = fmax(v, dpp_f64<0x4E, 0xF>(v, v));
+  v = fmax(v, dpp_f64<0x141, 0xF>(v, v));
+  v = fmax(v, dpp_f64<0x140, 0xF>(v, v));
+  v = fmax(v, dpp_f64<0x142, 0xA>(v, v));
+  v = fmax(v, dpp_f64<0x143, 0xC>(v, v));
+  return lane63_f64(v);
 }
 __device__ inline float wave_sum(float v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_f32<0xB1, 0xF>(0.f, v);
+  v += dpp_f32<0x4E, 0xF>(0.f, v);
+  v += dpp_f32<0x141, 0xF>(0.f, v);
+  v += dpp_f32<0x140, 0xF>(0.f, v);
+  v += dpp_f32<0x142, 0xA>(0.f, v);
+  v += dpp_f32<0x143, 0xC>(0.f, v);
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
 }
 
 // XCD-aware block remap: the hardware hands consecutive workgroup ids to the 8 XCDs round-robin.  Renumber so

@@ -485,8 +485,11 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
                 with torch.cuda.stream(side):
                     nxt = prepare(k + 1)
 
+        piped = side is not main  # serially, the next chromosome is prepared after this scan, outside its stage
         with stage("snps+scan"):
-            st.scan(Zd, test, n_real, out, d, after_launch=launch_next)
+            st.scan(Zd, test, n_real, out, d, after_launch=launch_next if piped else None)
+        if not piped:
+            launch_next()
         main.synchronize()
         del st, d, done
         if on_chrom is not None:
