@@ -481,6 +481,11 @@ def main():
         print(json.dumps({"error": "this build runs on exactly one GPU; use --gpus 1"}))
         log("bench.py: multi-GPU runs are not part of this build (one MI355X); run with --gpus 1")
         sys.exit(2)
+    if args.config not in (1, 2, 3, 5):
+        log(f"bench.py: --config {args.config} is not a bench workload (1 and 2: the LOCO compat scan, 3: uneven NA "
+            "groups, 5: the low-rank cohort); the learned delta proposal is evaluated by "
+            "`python -m fasterlmm_amd.learned --eval`")
+        sys.exit(2)
     if args.config == 3:
         run_grouped(args)
         return
