@@ -397,29 +397,59 @@ __device__ inline double golden_bracket(const LK& L, int64_t p, int i, int G, do
 //
 // On the register path an FP32 Newton phase runs first (cheap passes, root to ~1e-5); its result, when it stays
 // inside the bracket and converges, is only the starting point of the FP64 iteration, which decides the answer.
-template <class LK, bool FP32_START>
-__device__ inline double newton_bracket(const LK& L, int64_t p, int i, int G, double lo, double hi, int lane) {
-  const double a0 = grid_value(i > 0 ? i - 1 : 0, G, lo, hi);
-  const double b0 = grid_value(i + 1 < G ? i + 1 : G - 1, G, lo, hi);
-  if (!(b0 > a0)) return a0;
-  double a = a0, b = b0;
-  double t = grid_value(i, G, lo, hi);
-  if (!(t > a && t < b)) t = 0.5 * (a + b);
-  if constexpr (FP32_START) {
-    float tf = (float)t;
-    for (int it = 0; it < 12; ++it) {
-      float g, gp;
-      L.slope32(tf, lane, g, gp);
-      const float tn = tf - g / gp;
-      if (!(gp > 0.f) || !((double)tn > a0 && (double)tn < b0)) break;
-      const bool conv = fabsf(tn - tf) <= 1e-5f * fmaxf(1.f, fabsf(tf));
-      tf = tn;
-      if (conv) {
-        t = (double)tf;
-        break;
-      }
+// the grid bracket [g[i-1], g[i+1]] (clamped at the grid's ends) and the search's starting point g[i] (the
+// bracket's midpoint where g[i] is one of its ends)
+__device__ inline void grid_bracket(int i, int G, double lo, double hi, double& a0, double& b0, double& t) {
+  a0 = grid_value(i > 0 ? i - 1 : 0, G, lo, hi);
+  b0 = grid_value(i + 1 < G ? i + 1 : G - 1, G, lo, hi);
+  t = grid_value(i, G, lo, hi);
+  if (!(t > a0 && t < b0)) t = 0.5 * (a0 + b0);
+}
+
+// FP32 Newton phase (register path) from t inside (a0, b0): on convergence t becomes the FP32 root, else it stays
+template <class LK>
+__device__ inline void newton_fp32(const LK& L, double a0, double b0, double& t, int lane) {
+  float tf = (float)t;
+  for (int it = 0; it < 12; ++it) {
+    float g, gp;
+    L.slope32(tf, lane, g, gp);
+    const float tn = tf - g / gp;
+    if (!(gp > 0.f) || !((double)tn > a0 && (double)tn < b0)) return;
+    const bool conv = fabsf(tn - tf) <= 1e-5f * fmaxf(1.f, fabsf(tf));
+    tf = tn;
+    if (conv) {
+      t = (double)tf;
+      return;
     }
   }
+}
+
+template <class LK>
+__device__ inline double newton_fp64(const LK& L, int64_t p, double a0, double b0, double t, int lane);
+
+template <class LK, bool FP32_START>
+__device__ inline double newton_bracket(const LK& L, int64_t p, int i, int G, double lo, double hi, int lane) {
+  double a0, b0, t;
+  grid_bracket(i, G, lo, hi, a0, b0, t);
+  if (!(b0 > a0)) return a0;
+  if constexpr (FP32_START) newton_fp32(L, a0, b0, t, lane);
+  return newton_fp64(L, p, a0, b0, t, lane);
+}
+
+// the same search with its FP32 phase already run (delta_start32_kernel): t32 is where that phase left t
+template <class LK>
+__device__ inline double newton_bracket_from(const LK& L, int64_t p, int i, int G, double lo, double hi, double t32,
+                                             int lane) {
+  double a0, b0, t;
+  grid_bracket(i, G, lo, hi, a0, b0, t);
+  if (!(b0 > a0)) return a0;
+  return newton_fp64(L, p, a0, b0, t32, lane);
+}
+
+// safeguarded FP64 Newton on L'(t) in (a0, b0) from t
+template <class LK>
+__device__ inline double newton_fp64(const LK& L, int64_t p, double a0, double b0, double t, int lane) {
+  double a = a0, b = b0;
   bool ends_checked = false;
   for (int it = 0; it < 100; ++it) {
     double g, gp;
@@ -457,7 +487,8 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
     int64_t neff, const double* __restrict__ s_lam, int64_t nlam, double nco, int G, double lo, double hi,
     const int* __restrict__ idx, const double* __restrict__ lam64, double* __restrict__ log_delta,
     double* __restrict__ rWr, double* __restrict__ h2, void* q1a, void* q1b, void* q2a, void* q2b,
-    float* __restrict__ q1s, float* __restrict__ q2s, double radius, double judge_tol, int* __restrict__ flags) {
+    float* __restrict__ q1s, float* __restrict__ q2s, double radius, double judge_tol, int* __restrict__ flags,
+    bool pre32) {
   const int lane = threadIdx.x & 63;
   const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   // both spectra once per block in LDS (register path), padded with 1 beyond their lengths
@@ -489,6 +520,7 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
   L.load(p, lane);
   double t;
   int i = (mode == MODE_FIXED || mode == MODE_GRID) ? 0 : idx[p];
+  const int i_fp32 = i;
   // every search brackets the FP64 grid argmin, as the reference does (the FP32 grid's pick differs only where
   // neighbouring grid losses tie to FP32 precision, i.e. on flat likelihoods)
   if (mode != MODE_FIXED && mode != MODE_GRID)
@@ -544,7 +576,14 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
   } else {
     t = 0.0;
   }
-  if (need_bracket) t = newton_bracket<ColLik<KPL, SAME>, (KPL > 0)>(L, p, i, G, lo, hi, lane);
+  if (need_bracket) {
+    // pre32: log_delta[p] holds where the FP32 phase left the search from the FP32 argmin's node, valid when the
+    // FP64 walk kept that node
+    if (KPL > 0 && pre32 && i == i_fp32)
+      t = newton_bracket_from(L, p, i, G, lo, hi, log_delta[p], lane);
+    else
+      t = newton_bracket<ColLik<KPL, SAME>, (KPL > 0)>(L, p, i, G, lo, hi, lane);
+  }
   if (mode != MODE_FIXED) t = fmin(fmax(t, lo), hi);
   const double delta = exp(t);
   // pass 1: the null residual rWr and (split-FP16) the column maxima that set the planes' scales
@@ -588,6 +627,37 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
       if (q2s) q2s[p] = (float)sc2;
     }
   }
+}
+
+// The FP32 Newton phase of the Newton search, run ahead of the refinement in a kernel that holds only FP32 state
+// (a column's 16 values per lane and the FP32 spectra: 70 VGPRs against the refinement's ~210), so several times
+// more waves share each SIMD and hide the phase's latency-bound iterations.  For column p it starts where
+// newton_bracket starts from the FP32 grid argmin idx[p] and writes where the phase leaves t to t_out[p] (the
+// refinement's log_delta, which it overwrites at its end).  Compiled apart from the refinement, its last FP32 bits
+// can differ from the inline phase's; the FP64 phase then lands within ~1e-7 of the same root in log delta, the
+// resolution to which the loss itself fixes it.
+template <bool SAME>
+__global__ __launch_bounds__(256) void delta_start32_kernel(const float* __restrict__ uy, int64_t P, int64_t p_pad,
+                                                            const double* __restrict__ s, int64_t neff,
+                                                            const double* __restrict__ s_lam, int64_t nlam,
+                                                            double nco, int G, double lo, double hi,
+                                                            const int* __restrict__ idx, double* __restrict__ t_out) {
+  constexpr int KPL = 16, NS = 64 * KPL;
+  const int lane = threadIdx.x & 63;
+  const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  __shared__ float sh_sf[NS], sh_lf[SAME ? 1 : NS];
+  for (int k = threadIdx.x; k < NS; k += blockDim.x) {
+    sh_sf[k] = (float)(k < neff ? s[k] : 1.0);
+    if constexpr (!SAME) sh_lf[k] = (float)(k < nlam ? s_lam[k] : 1.0);
+  }
+  __syncthreads();
+  if (p >= P) return;
+  ColLik<KPL, SAME> L{uy, p_pad, s, neff, s_lam, nlam, nco, nullptr, nullptr, sh_sf, SAME ? sh_sf : sh_lf, {}};
+  L.load(p, lane);
+  double a0, b0, t;
+  grid_bracket(idx[p], G, lo, hi, a0, b0, t);
+  if (b0 > a0) newton_fp32(L, a0, b0, t, lane);
+  if (lane == 0) t_out[p] = t;
 }
 
 // Q1 = w o y and Q2 = w, w = 1 / (s_k + exp(log_delta[p])), written as the scan's column-side planes (zero on the
@@ -742,10 +812,22 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
   // columns that fit 16 elements per lane (neff, nlam, kpad <= 1024) keep them in registers
   const bool regs = neff <= 1024 && nlam <= 1024 && kpad <= 1024;
   const bool same = s_lam == s && nlam == neff;
+  // Newton on the register path: the FP32 phase runs ahead in its own kernel
+  const bool pre32 = mode == MODE_NEWTON && regs && idx != nullptr && P > 0;
+  if (pre32) {
+    if (same)
+      hipLaunchKernelGGL((delta_start32_kernel<true>), grid, dim3(256), 0, stream, uy, P, p_pad, s, neff, s_lam, nlam,
+                         nco, G, lo, hi, idx, log_delta);
+    else
+      hipLaunchKernelGGL((delta_start32_kernel<false>), grid, dim3(256), 0, stream, uy, P, p_pad, s, neff, s_lam,
+                         nlam, nco, G, lo, hi, idx, log_delta);
+    hipError_t e0 = hipGetLastError();
+    if (e0 != hipSuccess) return e0;
+  }
 #define FLMM_REFINE1(PR, K, SM)                                                                                     \
   hipLaunchKernelGGL((delta_refine_kernel<PR, K, SM>), grid, dim3(256), 0, stream, mode, uy, P, p_pad, kpad, s,    \
                      neff, s_lam, nlam, nco, G, lo, hi, idx, lam64, log_delta, rWr, h2, q1a, q1b, q2a, q2b, q1s,     \
-                     q2s, radius, judge_tol, flags)
+                     q2s, radius, judge_tol, flags, pre32 && K > 0)
 #define FLMM_REFINE(PR, K)                                                                                          \
   do {                                                                                                              \
     if (same) FLMM_REFINE1(PR, K, true); else FLMM_REFINE1(PR, K, false);                                           \
