@@ -72,10 +72,11 @@ struct TileArgs {
   int nchunks2;          // K / 32 of the second product where it is shorter (low-rank scan only)
 };
 
-// LDS bytes needed by one workgroup
+// LDS bytes needed by one workgroup: both sides' planes of one K chunk, twice for the double-buffered
+// single-product form (NG == 1)
 template <int PREC, int NG>
 constexpr int tile_lds_bytes() {
-  return 2 * NG * PlaneTraits<PREC>::NSUB * TILE * PlaneTraits<PREC>::ROWB;
+  return (NG == 1 ? 2 : 1) * 2 * NG * PlaneTraits<PREC>::NSUB * TILE * PlaneTraits<PREC>::ROWB;
 }
 
 template <int PREC, int NG>
@@ -132,7 +133,11 @@ __device__ inline void tile_mainloop(const TileArgs<PREC, NG>& args, int64_t a_r
         }
       }
   };
-  auto lstore = [&]() {
+  // NG == 1 double-buffers the chunks in LDS (one barrier per chunk: chunk k + 1 is stored into the other buffer
+  // while chunk k is multiplied); NG == 2 keeps one buffer (its planes fill twice the LDS)
+  constexpr bool DB = NG == 1;
+  constexpr int BUF = 2 * NPL * PLANE_LDS;
+  auto lstore = [&](int buf) {
 #pragma unroll
     for (int side = 0; side < 2; ++side)
 #pragma unroll
@@ -140,7 +145,7 @@ __device__ inline void tile_mainloop(const TileArgs<PREC, NG>& args, int64_t a_r
 #pragma unroll
         for (int ps = 0; ps < PASSES; ++ps) {
           const int r = r_in0 + ps * ROWS_PER_PASS;
-          char* dst = lds + (side * NPL + pl) * PLANE_LDS + r * T::ROWB + ((s_in ^ T::swz(r)) * 16);
+          char* dst = lds + buf * BUF + (side * NPL + pl) * PLANE_LDS + r * T::ROWB + ((s_in ^ T::swz(r)) * 16);
           uint4 v = pre[(side * NPL + pl) * PASSES + ps];
           if constexpr (SQUARE_B && PREC == PREC_FP32) {
             if (side == 1) {
@@ -154,11 +159,22 @@ __device__ inline void tile_mainloop(const TileArgs<PREC, NG>& args, int64_t a_r
   };
 
   gload(0);
+  if constexpr (DB) {
+    if (args.nchunks > 0) {
+      __syncthreads();  // the caller's previous use of the LDS is over
+      lstore(0);
+      __syncthreads();
+    }
+  }
   for (int kc = 0; kc < args.nchunks; ++kc) {
-    __syncthreads();  // previous chunk fully consumed
-    lstore();
-    __syncthreads();
+    const int cur = DB ? (kc & 1) : 0;
+    if constexpr (!DB) {
+      __syncthreads();  // previous chunk fully consumed
+      lstore(0);
+      __syncthreads();
+    }
     if (kc + 1 < args.nchunks) gload(kc + 1);
+    const char* cl = lds + cur * BUF;
 
     const int h = lane >> 5;
     if constexpr (is_split(PREC)) {
@@ -173,7 +189,7 @@ __device__ inline void tile_mainloop(const TileArgs<PREC, NG>& args, int64_t a_r
             const int r = wm * 64 + bi * 32 + (lane & 31);
 #pragma unroll
             for (int sub = 0; sub < 2; ++sub) {
-              const char* src = lds + (0 * NPL + g * 2 + sub) * PLANE_LDS + r * 64 + ((seg ^ T::swz(r)) * 16);
+              const char* src = cl + (0 * NPL + g * 2 + sub) * PLANE_LDS + r * 64 + ((seg ^ T::swz(r)) * 16);
               af[bi][sub] = *(const bf16x8*)src;
             }
           }
@@ -182,7 +198,7 @@ __device__ inline void tile_mainloop(const TileArgs<PREC, NG>& args, int64_t a_r
             const int r = wn * 64 + bj * 32 + (lane & 31);
 #pragma unroll
             for (int sub = 0; sub < 2; ++sub) {
-              const char* src = lds + (1 * NPL + g * 2 + sub) * PLANE_LDS + r * 64 + ((seg ^ T::swz(r)) * 16);
+              const char* src = cl + (1 * NPL + g * 2 + sub) * PLANE_LDS + r * 64 + ((seg ^ T::swz(r)) * 16);
               bfr[bj][sub] = *(const bf16x8*)src;
             }
           }
@@ -209,12 +225,12 @@ __device__ inline void tile_mainloop(const TileArgs<PREC, NG>& args, int64_t a_r
 #pragma unroll
           for (int bi = 0; bi < 2; ++bi) {
             const int r = wm * 64 + bi * 32 + (lane & 31);
-            af[bi] = *(const f32x4*)(lds + (0 * NPL + g) * PLANE_LDS + r * 128 + ((seg ^ T::swz(r)) * 16));
+            af[bi] = *(const f32x4*)(cl + (0 * NPL + g) * PLANE_LDS + r * 128 + ((seg ^ T::swz(r)) * 16));
           }
 #pragma unroll
           for (int bj = 0; bj < 2; ++bj) {
             const int r = wn * 64 + bj * 32 + (lane & 31);
-            bfr[bj] = *(const f32x4*)(lds + (1 * NPL + g) * PLANE_LDS + r * 128 + ((seg ^ T::swz(r)) * 16));
+            bfr[bj] = *(const f32x4*)(cl + (1 * NPL + g) * PLANE_LDS + r * 128 + ((seg ^ T::swz(r)) * 16));
           }
 #pragma unroll
           for (int e = 0; e < 4; ++e)
@@ -226,6 +242,10 @@ __device__ inline void tile_mainloop(const TileArgs<PREC, NG>& args, int64_t a_r
                     __builtin_amdgcn_mfma_f32_32x32x2f32(af[bi][e], bfr[bj][e], acc.c[g][bi][bj], 0, 0, 0);
         }
       }
+    }
+    if constexpr (DB) {
+      if (kc + 1 < args.nchunks) lstore(cur ^ 1);  // that buffer was last read before the previous barrier
+      __syncthreads();
     }
   }
 }
