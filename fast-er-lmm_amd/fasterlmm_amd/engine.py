@@ -38,7 +38,7 @@ class Settings:
     precision: str = "split_fp16"  # "split_fp16" (scaled FP16 hi+lo, 3 MFMA products), "split_bf16" (BF16 hi+lo,
                                    # 3 products) or "fp32" (FP32 MFMA)
     delta: str = "newton"          # "newton" (exact local minimiser), "golden" (the reference's 50 steps) or
-                                   # "learned" (MLP proposal + judged Newton, grid as fallback and judge)
+                                   # "learned" (MLP proposal + checked Newton, grid as fallback and arbiter)
     weights: str = "auto"          # sWs weights 1 / (s + delta): "factored" (exponential sum, second product over
                                    # ~128 terms), "exact" (over Neff), "auto" = factored for the split precisions
     kinship: str = "auto"          # kinship Gram Z Z^T: "fp64" (FP64 MFMA kernel), "blas" (rocBLAS FP64), "split"

@@ -1,5 +1,5 @@
 """Learned log-delta proposal: spectral features of a rotated column -> small FP32 MLP -> log delta, finished by
-a short Newton refinement and judged against the full grid (which stays as fallback and judge).
+a short Newton refinement and checked against the full grid (which stays as fallback and arbiter).
 
 Everything numeric runs in the gfx950 kernels of libfasterlmm_amd.so: features (flmm_mlp_features), the dense
 layers forward and backward (flmm_mlp_dense_fwd / _bwd), the MSE gradient and Adam (flmm_adam), and the judged
@@ -169,7 +169,7 @@ class LearnedFit:
 
 def refine_learned(basis, net: DeltaMLP, n_grid: int, lo: float, hi: float, *, n: int, c: int,
                    radius: float = 1.0, judge_tol: float = 1e-6, projected_loss: bool = True) -> LearnedFit:
-    """Learned delta for every column of a core._Basis: grid (judge) + MLP proposal + judged Newton."""
+    """Learned delta for every column of a core._Basis: grid (arbiter) + MLP proposal + checked Newton."""
     s_lam, nco = basis._loss(projected_loss)
     idx, lam64 = engine.delta_grid(basis.uy, basis.P, basis.p_pad, basis.kpad, basis.s, s_lam, nco, n_grid, lo, hi)
     feat = features(basis.uy, basis.P, basis.p_pad, basis.s, n, c)
@@ -306,7 +306,7 @@ def evaluate(net: DeltaMLP, *, n: int = 2000, n_pheno: int = 10_000, n_perm: int
     """The learned-delta evaluation: n_pheno held-out phenotypes (h2 ~ U(0, 0.95), a fresh Balding-Nichols panel of
     n strains) x (1 + n_perm) permutation columns on one projected kinship.  Delta is fitted by the plain grid +
     Newton (the grid's exact bracket optimum, every column), by the learned proposal + judged Newton (grid as
-    judge and fallback, every column), and by the reference's grid + golden section (the real columns).  m_test
+    arbiter and fallback, every column), and by the reference's grid + golden section (the real columns).  m_test
     SNPs are scanned against the columns with each.  Reports the fallback rate, the log-likelihood gap of the
     learned optimum, |delta log10 p| on the real columns' per-test p and on every column's genome-wide max F, and
     the delta-search times."""

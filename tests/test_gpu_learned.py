@@ -110,7 +110,7 @@ def test_training_learns_and_refinement_is_judged():
 
 
 def test_trained_network_gain_over_uninformed_proposal():
-    """What the network gains over the method without it: with the grid kept as judge, a column is finished by
+    """What the network gains over the method without it: with the grid kept as arbiter, a column is finished by
     the short judged refinement only when its proposal is good enough; the trained network's proposals get far
     more columns through than an uninformed proposal (the mean training label) does, and every accepted column
     lands on the grid + Newton optimum."""
