@@ -2,7 +2,7 @@
 a short Newton refinement and checked against the full grid (which stays as fallback and arbiter).
 
 Everything numeric runs in the gfx950 kernels of libfasterlmm_amd.so: features (flmm_mlp_features), the dense
-layers forward and backward (flmm_mlp_dense_fwd / _bwd), the MSE gradient and Adam (flmm_adam), and the judged
+layers forward and backward (flmm_mlp_dense_fwd / _bwd), the MSE gradient and Adam (flmm_adam), and the checked
 refinement (flmm_delta_refine_learned).  Training data are simulated directly in a kinship eigenbasis: for
 eigenvalues s and heritability h2, the rotated phenotype has independent entries y_k ~ N(0, sg2 s_k + se2), which
 is exactly the LMM's distribution; labels are the grid + Newton optimum of the same likelihood.
@@ -305,7 +305,7 @@ def evaluate(net: DeltaMLP, *, n: int = 2000, n_pheno: int = 10_000, n_perm: int
              m_test: int = 1_000, n_grid: int = 100, seed: int = 20250101, verbose: bool = False) -> dict:
     """The learned-delta evaluation: n_pheno held-out phenotypes (h2 ~ U(0, 0.95), a fresh Balding-Nichols panel of
     n strains) x (1 + n_perm) permutation columns on one projected kinship.  Delta is fitted by the plain grid +
-    Newton (the grid's exact bracket optimum, every column), by the learned proposal + judged Newton (grid as
+    Newton (the grid's exact bracket optimum, every column), by the learned proposal + checked Newton (grid as
     arbiter and fallback, every column), and by the reference's grid + golden section (the real columns).  m_test
     SNPs are scanned against the columns with each.  Reports the fallback rate, the log-likelihood gap of the
     learned optimum, |delta log10 p| on the real columns' per-test p and on every column's genome-wide max F, and

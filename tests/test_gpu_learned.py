@@ -1,5 +1,5 @@
 """GPU kernels of the learned delta proposal against the CPU oracle: features, forward, backward (gradient check
-against the numpy backprop, itself checked against finite differences), Adam, and the judged refinement."""
+against the numpy backprop, itself checked against finite differences), Adam, and the checked refinement."""
 from __future__ import annotations
 
 import numpy as np
@@ -80,7 +80,7 @@ def test_features_match_oracle():
 
 
 def test_training_learns_and_refinement_is_judged():
-    """A short training run on simulated eigenbasis columns lowers the loss, and the judged refinement returns
+    """A short training run on simulated eigenbasis columns lowers the loss, and the checked refinement returns
     the same optimum as the grid + Newton path within 1e-8 (or falls back to it)."""
     from fasterlmm_amd import engine
     from fasterlmm_amd.learned import (DeltaMLP, basis_from_columns, features, refine_learned,
@@ -111,7 +111,7 @@ def test_training_learns_and_refinement_is_judged():
 
 def test_trained_network_gain_over_uninformed_proposal():
     """What the network gains over the method without it: with the grid kept as arbiter, a column is finished by
-    the short judged refinement only when its proposal is good enough; the trained network's proposals get far
+    the short checked refinement only when its proposal is good enough; the trained network's proposals get far
     more columns through than an uninformed proposal (the mean training label) does, and every accepted column
     lands on the grid + Newton optimum."""
     from fasterlmm_amd import engine
