@@ -204,7 +204,7 @@ int flmm_gram_f64(const double* z, int64_t ldz, int64_t n, int64_t m, int n_spli
 
 /* FP64 rotation of SNP columns: out[i][j] = sum_{k<n} z[k * ldz + c(i)] * u[k * ldu + j], c(i) = col_index[i] (or i
  * when col_index is NULL), for i < mc, j < neff; the rest of the rows_pad x kpad output (row-major, multiples of
- * 128) is zero.  z = the standardised genotypes (strains x SNPs), u = the basis U_eff (strains x neff, pointer at
+ * 128) is zero.  col_index (device int64, mc entries) must hold values in [0, ldz); it is not checked on the host.  z = the standardised genotypes (strains x SNPs), u = the basis U_eff (strains x neff, pointer at
  * its first column): US = U_eff^T S_r of core.py:628-629 in the reference's FP64, on the FP64 matrix cores; the
  * caller packs US and its square (or factored products) into the scan's SNP planes. */
 int flmm_rotate_f64(const double* z, int64_t ldz, const int64_t* col_index, int64_t mc, int64_t n, const double* u,
