@@ -121,3 +121,18 @@ def test_scan_table_layout_and_validation():
         _lib.call("flmm_scan_table_fill", 0, _lib.SCAN_WIDE_FACTORED, ctypes.addressof(fac), 1, buf.ctypes.data)
     with pytest.raises(ValueError, match="blocks must be a positive multiple of 8"):
         _lib.call("flmm_wald_scan_grouped", 0, 1, buf.ctypes.data, 100, None)
+
+
+def test_kinship_setting_is_validated_before_any_launch():
+    """engine.gram rejects an unknown kinship setting, and the split Gram outside the split precisions, on the host
+    (no device memory is touched)."""
+    import pytest
+    import torch
+    from fasterlmm_amd import engine
+    Z = torch.zeros(3, 5, dtype=torch.float64)
+    with engine.configure(kinship="bogus"):
+        with pytest.raises(ValueError, match="kinship"):
+            engine.gram(Z)
+    with engine.configure(kinship="split", precision="fp32"):
+        with pytest.raises(ValueError, match="split precision"):
+            engine.gram(Z)
