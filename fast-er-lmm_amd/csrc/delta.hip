@@ -481,8 +481,10 @@ __device__ inline double newton_fp64(const LK& L, int64_t p, double a0, double b
   return t;
 }
 
-template <int PREC, int KPL, bool SAME>
-__global__ __launch_bounds__(256) void delta_refine_kernel(
+// NEWTON_ONLY: the launch of the usual Newton search (register path, FP32 phase run ahead), with the other
+// modes compiled out so their registers do not count against its occupancy
+template <int PREC, int KPL, bool SAME, bool NEWTON_ONLY>
+__device__ __forceinline__ void delta_refine_body(
     int mode, const float* __restrict__ uy, int64_t P, int64_t p_pad, int64_t kpad, const double* __restrict__ s,
     int64_t neff, const double* __restrict__ s_lam, int64_t nlam, double nco, int G, double lo, double hi,
     const int* __restrict__ idx, const double* __restrict__ lam64, double* __restrict__ log_delta,
@@ -525,56 +527,60 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
   // neighbouring grid losses tie to FP32 precision, i.e. on flat likelihoods)
   if (mode != MODE_FIXED && mode != MODE_GRID)
     i = fp64_grid_argmin<ColLik<KPL, SAME>, (KPL > 0)>(L, p, i, G, lo, hi, lam64, lane);
-  bool need_bracket = mode == MODE_NEWTON;
-  if (mode == MODE_FIXED) {
-    t = log_delta[p];  // the caller's log delta, used as given (core.py:639)
-  } else if (mode == MODE_GRID) {
-    // refine=False: the first argmin of the FP64 loss over the whole grid (core.py:570-573)
-    double best = INFINITY;
-    int bi = 0;
-    for (int g = 0; g < G; ++g) {
-      const double l = col_loss(L, p, grid_value(g, G, lo, hi), lane);
-      if (l < best) { best = l; bi = g; }
-    }
-    t = grid_value(bi, G, lo, hi);
-  } else if (mode == MODE_GOLDEN) {
-    t = golden_bracket(L, p, i, G, lo, hi, lane);
-  } else if (mode == MODE_LEARNED) {
-    // proposal t0 (in log_delta), undamped Newton inside [t0 - radius, t0 + radius].  Accepted only if it converges
-    // there, to a point inside the grid optimum's bracket [g[i-1], g[i+1]] (the basin the grid search refines),
-    // whose loss is not worse than the grid optimum's by more than judge_tol; otherwise the grid bracket is refined
-    const double t0 = fmin(fmax(log_delta[p], lo), hi);
-    const double a_br = grid_value(i > 0 ? i - 1 : 0, G, lo, hi);
-    const double b_br = grid_value(i + 1 < G ? i + 1 : G - 1, G, lo, hi);
-    double tt = t0;
-    bool ok = false;
-    for (int it = 0; it < 10; ++it) {
-      double g, gp;
-      col_slope(L, p, tt, lane, g, gp);
-      if (!(gp > 0.0)) break;
-      double step = -g / gp;
-      step = fmin(fmax(step, -radius), radius);
-      const double tn = fmin(fmax(tt + step, lo), hi);
-      if (fabs(tn - t0) > radius) break;
-      const bool conv = fabs(tn - tt) <= 1e-12 * fmax(1.0, fabs(tt));
-      tt = tn;
-      if (conv) { ok = true; break; }
-    }
-    ok = ok && tt >= a_br && tt <= b_br;
-    if (ok) {
-      const double lstar = col_loss(L, p, tt, lane);
-      const double lgrid = col_loss(L, p, grid_value(i, G, lo, hi), lane);
-      ok = lstar <= lgrid + judge_tol;
-    }
-    if (ok) {
-      t = tt;
+  bool need_bracket = NEWTON_ONLY || mode == MODE_NEWTON;
+  if constexpr (NEWTON_ONLY) {
+    t = 0.0;
+  } else {
+    if (mode == MODE_FIXED) {
+      t = log_delta[p];  // the caller's log delta, used as given (core.py:639)
+    } else if (mode == MODE_GRID) {
+      // refine=False: the first argmin of the FP64 loss over the whole grid (core.py:570-573)
+      double best = INFINITY;
+      int bi = 0;
+      for (int g = 0; g < G; ++g) {
+        const double l = col_loss(L, p, grid_value(g, G, lo, hi), lane);
+        if (l < best) { best = l; bi = g; }
+      }
+      t = grid_value(bi, G, lo, hi);
+    } else if (mode == MODE_GOLDEN) {
+      t = golden_bracket(L, p, i, G, lo, hi, lane);
+    } else if (mode == MODE_LEARNED) {
+      // proposal t0 (in log_delta), undamped Newton inside [t0 - radius, t0 + radius].  Accepted only if it converges
+      // there, to a point inside the grid optimum's bracket [g[i-1], g[i+1]] (the basin the grid search refines),
+      // whose loss is not worse than the grid optimum's by more than judge_tol; otherwise the grid bracket is refined
+      const double t0 = fmin(fmax(log_delta[p], lo), hi);
+      const double a_br = grid_value(i > 0 ? i - 1 : 0, G, lo, hi);
+      const double b_br = grid_value(i + 1 < G ? i + 1 : G - 1, G, lo, hi);
+      double tt = t0;
+      bool ok = false;
+      for (int it = 0; it < 10; ++it) {
+        double g, gp;
+        col_slope(L, p, tt, lane, g, gp);
+        if (!(gp > 0.0)) break;
+        double step = -g / gp;
+        step = fmin(fmax(step, -radius), radius);
+        const double tn = fmin(fmax(tt + step, lo), hi);
+        if (fabs(tn - t0) > radius) break;
+        const bool conv = fabs(tn - tt) <= 1e-12 * fmax(1.0, fabs(tt));
+        tt = tn;
+        if (conv) { ok = true; break; }
+      }
+      ok = ok && tt >= a_br && tt <= b_br;
+      if (ok) {
+        const double lstar = col_loss(L, p, tt, lane);
+        const double lgrid = col_loss(L, p, grid_value(i, G, lo, hi), lane);
+        ok = lstar <= lgrid + judge_tol;
+      }
+      if (ok) {
+        t = tt;
+      } else {
+        need_bracket = true;
+        t = 0.0;
+      }
+      if (flags && lane == 0) flags[p] = ok ? 0 : 1;
     } else {
-      need_bracket = true;
       t = 0.0;
     }
-    if (flags && lane == 0) flags[p] = ok ? 0 : 1;
-  } else {
-    t = 0.0;
   }
   if (need_bracket) {
     // pre32: log_delta[p] holds where the FP32 phase left the search from the FP32 argmin's node, valid when the
@@ -627,6 +633,29 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
       if (q2s) q2s[p] = (float)sc2;
     }
   }
+}
+
+template <int PREC, int KPL, bool SAME, bool NEWTON_ONLY = false>
+__global__ __launch_bounds__(256) void delta_refine_kernel(
+    int mode, const float* __restrict__ uy, int64_t P, int64_t p_pad, int64_t kpad, const double* __restrict__ s,
+    int64_t neff, const double* __restrict__ s_lam, int64_t nlam, double nco, int G, double lo, double hi,
+    const int* __restrict__ idx, const double* __restrict__ lam64, double* __restrict__ log_delta,
+    double* __restrict__ rWr, double* __restrict__ h2, void* q1a, void* q1b, void* q2a, void* q2b,
+    float* __restrict__ q1s, float* __restrict__ q2s, double radius, double judge_tol, int* __restrict__ flags,
+    bool pre32) {
+  delta_refine_body<PREC, KPL, SAME, NEWTON_ONLY>(mode, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco, G, lo, hi, idx, lam64, log_delta, rWr, h2, q1a, q1b, q2a, q2b, q1s, q2s, radius, judge_tol, flags, pre32);
+}
+
+// the Newton launch at three waves per SIMD (its register budget without the other modes)
+template <int PREC, int KPL, bool SAME>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void delta_refine_newton_kernel(
+    int mode, const float* __restrict__ uy, int64_t P, int64_t p_pad, int64_t kpad, const double* __restrict__ s,
+    int64_t neff, const double* __restrict__ s_lam, int64_t nlam, double nco, int G, double lo, double hi,
+    const int* __restrict__ idx, const double* __restrict__ lam64, double* __restrict__ log_delta,
+    double* __restrict__ rWr, double* __restrict__ h2, void* q1a, void* q1b, void* q2a, void* q2b,
+    float* __restrict__ q1s, float* __restrict__ q2s, double radius, double judge_tol, int* __restrict__ flags,
+    bool pre32) {
+  delta_refine_body<PREC, KPL, SAME, true>(mode, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco, G, lo, hi, idx, lam64, log_delta, rWr, h2, q1a, q1b, q2a, q2b, q1s, q2s, radius, judge_tol, flags, pre32);
 }
 
 // The FP32 Newton phase of the Newton search, run ahead of the refinement in a kernel that holds only FP32 state
@@ -825,9 +854,16 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
     if (e0 != hipSuccess) return e0;
   }
 #define FLMM_REFINE1(PR, K, SM)                                                                                     \
-  hipLaunchKernelGGL((delta_refine_kernel<PR, K, SM>), grid, dim3(256), 0, stream, mode, uy, P, p_pad, kpad, s,    \
-                     neff, s_lam, nlam, nco, G, lo, hi, idx, lam64, log_delta, rWr, h2, q1a, q1b, q2a, q2b, q1s,     \
-                     q2s, radius, judge_tol, flags, pre32 && K > 0)
+  do {                                                                                                              \
+    if (pre32 && K > 0)                                                                                             \
+      hipLaunchKernelGGL((delta_refine_newton_kernel<PR, K, SM>), grid, dim3(256), 0, stream, mode, uy, P, p_pad,    \
+                         kpad, s, neff, s_lam, nlam, nco, G, lo, hi, idx, lam64, log_delta, rWr, h2, q1a, q1b, q2a,  \
+                         q2b, q1s, q2s, radius, judge_tol, flags, true);                                            \
+    else                                                                                                            \
+      hipLaunchKernelGGL((delta_refine_kernel<PR, K, SM, false>), grid, dim3(256), 0, stream, mode, uy, P, p_pad,   \
+                         kpad, s, neff, s_lam, nlam, nco, G, lo, hi, idx, lam64, log_delta, rWr, h2, q1a, q1b, q2a,  \
+                         q2b, q1s, q2s, radius, judge_tol, flags, false);                                           \
+  } while (0)
 #define FLMM_REFINE(PR, K)                                                                                          \
   do {                                                                                                              \
     if (same) FLMM_REFINE1(PR, K, true); else FLMM_REFINE1(PR, K, false);                                           \
