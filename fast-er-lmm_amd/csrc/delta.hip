@@ -78,7 +78,7 @@ __global__ __launch_bounds__(NTHREADS, 1) void delta_grid_kernel(TileArgs<PREC_F
   const int64_t pt = xcd_remap(blockIdx.x, n_tiles);
   const int64_t p0 = pt * TILE;
   TileAcc<PREC_FP32, 1> acc;
-  tile_mainloop<PREC_FP32, 1, true>(args, 0, p0, acc, lds);
+  tile_mainloop<PREC_FP32, 1, true, false>(args, 0, p0, acc, lds);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -825,7 +825,7 @@ hipError_t launch_delta_grid(const float* wplane, const float* lam, int G, doubl
   args.b_rows_pad = p_pad;
   args.nchunks = (int)(kpad / KCHUNK);
   const int64_t n_tiles = p_pad / TILE;
-  constexpr int lds = tile_lds_bytes<PREC_FP32, 1>();
+  constexpr int lds = tile_lds_bytes<PREC_FP32, 1, false>();
   hipLaunchKernelGGL(delta_grid_kernel, dim3((unsigned)n_tiles), dim3(NTHREADS), lds, stream, args, n_tiles, lam, G,
                      (float)nco, P, idx);
   return hipGetLastError();

@@ -72,11 +72,10 @@ struct TileArgs {
   int nchunks2;          // K / 32 of the second product where it is shorter (low-rank scan only)
 };
 
-// LDS bytes needed by one workgroup: both sides' planes of one K chunk, twice for the double-buffered
-// single-product form (NG == 1)
-template <int PREC, int NG>
+// LDS bytes needed by one workgroup: both sides' planes of one K chunk, twice when double-buffered (DB)
+template <int PREC, int NG, bool DB = (NG == 1)>
 constexpr int tile_lds_bytes() {
-  return (NG == 1 ? 2 : 1) * 2 * NG * PlaneTraits<PREC>::NSUB * TILE * PlaneTraits<PREC>::ROWB;
+  return (DB ? 2 : 1) * 2 * NG * PlaneTraits<PREC>::NSUB * TILE * PlaneTraits<PREC>::ROWB;
 }
 
 template <int PREC, int NG>
@@ -85,7 +84,7 @@ struct TileAcc {
 };
 
 // Accumulate the full K extent of the 128 x 128 tile whose first rows are a_row0 (A side) and b_row0 (B side).
-template <int PREC, int NG, bool SQUARE_B = false>
+template <int PREC, int NG, bool SQUARE_B = false, bool DB = (NG == 1)>
 __device__ inline void tile_mainloop(const TileArgs<PREC, NG>& args, int64_t a_row0, int64_t b_row0,
                                      TileAcc<PREC, NG>& acc, char* lds) {
   using T = PlaneTraits<PREC>;
@@ -133,9 +132,9 @@ __device__ inline void tile_mainloop(const TileArgs<PREC, NG>& args, int64_t a_r
         }
       }
   };
-  // NG == 1 double-buffers the chunks in LDS (one barrier per chunk: chunk k + 1 is stored into the other buffer
-  // while chunk k is multiplied); NG == 2 keeps one buffer (its planes fill twice the LDS)
-  constexpr bool DB = NG == 1;
+  // DB (the default for NG == 1) double-buffers the chunks in LDS: one barrier per chunk, chunk k + 1 stored into
+  // the other buffer while chunk k is multiplied.  NG == 2 keeps one buffer (its planes fill twice the LDS), and so
+  // does the delta grid, whose small FP32 tiles run faster at the higher occupancy one buffer leaves.
   constexpr int BUF = 2 * NPL * PLANE_LDS;
   auto lstore = [&](int buf) {
 #pragma unroll
