@@ -436,16 +436,6 @@ __device__ inline double newton_bracket(const LK& L, int64_t p, int i, int G, do
   return newton_fp64(L, p, a0, b0, t, lane);
 }
 
-// the same search with its FP32 phase already run (delta_start32_kernel): t32 is where that phase left t
-template <class LK>
-__device__ inline double newton_bracket_from(const LK& L, int64_t p, int i, int G, double lo, double hi, double t32,
-                                             int lane) {
-  double a0, b0, t;
-  grid_bracket(i, G, lo, hi, a0, b0, t);
-  if (!(b0 > a0)) return a0;
-  return newton_fp64(L, p, a0, b0, t32, lane);
-}
-
 // safeguarded FP64 Newton on L'(t) in (a0, b0) from t
 template <class LK>
 __device__ inline double newton_fp64(const LK& L, int64_t p, double a0, double b0, double t, int lane) {
@@ -481,16 +471,15 @@ __device__ inline double newton_fp64(const LK& L, int64_t p, double a0, double b
   return t;
 }
 
-// NEWTON_ONLY: the launch of the usual Newton search (register path, FP32 phase run ahead), with the other
-// modes compiled out so their registers do not count against its occupancy
+// NEWTON_ONLY: the launch of the usual Newton search (register path), with the other modes compiled out so their
+// registers do not count against its occupancy
 template <int PREC, int KPL, bool SAME, bool NEWTON_ONLY>
 __device__ __forceinline__ void delta_refine_body(
     int mode, const float* __restrict__ uy, int64_t P, int64_t p_pad, int64_t kpad, const double* __restrict__ s,
     int64_t neff, const double* __restrict__ s_lam, int64_t nlam, double nco, int G, double lo, double hi,
     const int* __restrict__ idx, const double* __restrict__ lam64, double* __restrict__ log_delta,
     double* __restrict__ rWr, double* __restrict__ h2, void* q1a, void* q1b, void* q2a, void* q2b,
-    float* __restrict__ q1s, float* __restrict__ q2s, double radius, double judge_tol, int* __restrict__ flags,
-    bool pre32) {
+    float* __restrict__ q1s, float* __restrict__ q2s, double radius, double judge_tol, int* __restrict__ flags) {
   const int lane = threadIdx.x & 63;
   const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   // both spectra once per block in LDS (register path), padded with 1 beyond their lengths
@@ -522,7 +511,6 @@ __device__ __forceinline__ void delta_refine_body(
   L.load(p, lane);
   double t;
   int i = (mode == MODE_FIXED || mode == MODE_GRID) ? 0 : idx[p];
-  const int i_fp32 = i;
   // every search brackets the FP64 grid argmin, as the reference does (the FP32 grid's pick differs only where
   // neighbouring grid losses tie to FP32 precision, i.e. on flat likelihoods)
   if (mode != MODE_FIXED && mode != MODE_GRID)
@@ -582,14 +570,7 @@ __device__ __forceinline__ void delta_refine_body(
       t = 0.0;
     }
   }
-  if (need_bracket) {
-    // pre32: log_delta[p] holds where the FP32 phase left the search from the FP32 argmin's node, valid when the
-    // FP64 walk kept that node
-    if (KPL > 0 && pre32 && i == i_fp32)
-      t = newton_bracket_from(L, p, i, G, lo, hi, log_delta[p], lane);
-    else
-      t = newton_bracket<ColLik<KPL, SAME>, (KPL > 0)>(L, p, i, G, lo, hi, lane);
-  }
+  if (need_bracket) t = newton_bracket<ColLik<KPL, SAME>, (KPL > 0)>(L, p, i, G, lo, hi, lane);
   if (mode != MODE_FIXED) t = fmin(fmax(t, lo), hi);
   const double delta = exp(t);
   // pass 1: the null residual rWr and (split-FP16) the column maxima that set the planes' scales
@@ -641,9 +622,8 @@ __global__ __launch_bounds__(256) void delta_refine_kernel(
     int64_t neff, const double* __restrict__ s_lam, int64_t nlam, double nco, int G, double lo, double hi,
     const int* __restrict__ idx, const double* __restrict__ lam64, double* __restrict__ log_delta,
     double* __restrict__ rWr, double* __restrict__ h2, void* q1a, void* q1b, void* q2a, void* q2b,
-    float* __restrict__ q1s, float* __restrict__ q2s, double radius, double judge_tol, int* __restrict__ flags,
-    bool pre32) {
-  delta_refine_body<PREC, KPL, SAME, NEWTON_ONLY>(mode, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco, G, lo, hi, idx, lam64, log_delta, rWr, h2, q1a, q1b, q2a, q2b, q1s, q2s, radius, judge_tol, flags, pre32);
+    float* __restrict__ q1s, float* __restrict__ q2s, double radius, double judge_tol, int* __restrict__ flags) {
+  delta_refine_body<PREC, KPL, SAME, NEWTON_ONLY>(mode, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco, G, lo, hi, idx, lam64, log_delta, rWr, h2, q1a, q1b, q2a, q2b, q1s, q2s, radius, judge_tol, flags);
 }
 
 // the Newton launch at three waves per SIMD (its register budget without the other modes)
@@ -653,40 +633,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
     int64_t neff, const double* __restrict__ s_lam, int64_t nlam, double nco, int G, double lo, double hi,
     const int* __restrict__ idx, const double* __restrict__ lam64, double* __restrict__ log_delta,
     double* __restrict__ rWr, double* __restrict__ h2, void* q1a, void* q1b, void* q2a, void* q2b,
-    float* __restrict__ q1s, float* __restrict__ q2s, double radius, double judge_tol, int* __restrict__ flags,
-    bool pre32) {
-  delta_refine_body<PREC, KPL, SAME, true>(mode, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco, G, lo, hi, idx, lam64, log_delta, rWr, h2, q1a, q1b, q2a, q2b, q1s, q2s, radius, judge_tol, flags, pre32);
-}
-
-// The FP32 Newton phase of the Newton search, run ahead of the refinement in a kernel that holds only FP32 state
-// (a column's 16 values per lane and the FP32 spectra: 70 VGPRs against the refinement's ~210), so several times
-// more waves share each SIMD and hide the phase's latency-bound iterations.  For column p it starts where
-// newton_bracket starts from the FP32 grid argmin idx[p] and writes where the phase leaves t to t_out[p] (the
-// refinement's log_delta, which it overwrites at its end).  Compiled apart from the refinement, its last FP32 bits
-// can differ from the inline phase's; the FP64 phase then lands within ~1e-7 of the same root in log delta, the
-// resolution to which the loss itself fixes it.
-template <bool SAME>
-__global__ __launch_bounds__(256) void delta_start32_kernel(const float* __restrict__ uy, int64_t P, int64_t p_pad,
-                                                            const double* __restrict__ s, int64_t neff,
-                                                            const double* __restrict__ s_lam, int64_t nlam,
-                                                            double nco, int G, double lo, double hi,
-                                                            const int* __restrict__ idx, double* __restrict__ t_out) {
-  constexpr int KPL = 16, NS = 64 * KPL;
-  const int lane = threadIdx.x & 63;
-  const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  __shared__ float sh_sf[NS], sh_lf[SAME ? 1 : NS];
-  for (int k = threadIdx.x; k < NS; k += blockDim.x) {
-    sh_sf[k] = (float)(k < neff ? s[k] : 1.0);
-    if constexpr (!SAME) sh_lf[k] = (float)(k < nlam ? s_lam[k] : 1.0);
-  }
-  __syncthreads();
-  if (p >= P) return;
-  ColLik<KPL, SAME> L{uy, p_pad, s, neff, s_lam, nlam, nco, nullptr, nullptr, sh_sf, SAME ? sh_sf : sh_lf, {}};
-  L.load(p, lane);
-  double a0, b0, t;
-  grid_bracket(idx[p], G, lo, hi, a0, b0, t);
-  if (b0 > a0) newton_fp32(L, a0, b0, t, lane);
-  if (lane == 0) t_out[p] = t;
+    float* __restrict__ q1s, float* __restrict__ q2s, double radius, double judge_tol, int* __restrict__ flags) {
+  delta_refine_body<PREC, KPL, SAME, true>(mode, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco, G, lo, hi, idx, lam64, log_delta, rWr, h2, q1a, q1b, q2a, q2b, q1s, q2s, radius, judge_tol, flags);
 }
 
 // Q1 = w o y and Q2 = w, w = 1 / (s_k + exp(log_delta[p])), written as the scan's column-side planes (zero on the
@@ -841,28 +789,18 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
   // columns that fit 16 elements per lane (neff, nlam, kpad <= 1024) keep them in registers
   const bool regs = neff <= 1024 && nlam <= 1024 && kpad <= 1024;
   const bool same = s_lam == s && nlam == neff;
-  // Newton on the register path: the FP32 phase runs ahead in its own kernel
-  const bool pre32 = mode == MODE_NEWTON && regs && idx != nullptr && P > 0;
-  if (pre32) {
-    if (same)
-      hipLaunchKernelGGL((delta_start32_kernel<true>), grid, dim3(256), 0, stream, uy, P, p_pad, s, neff, s_lam, nlam,
-                         nco, G, lo, hi, idx, log_delta);
-    else
-      hipLaunchKernelGGL((delta_start32_kernel<false>), grid, dim3(256), 0, stream, uy, P, p_pad, s, neff, s_lam,
-                         nlam, nco, G, lo, hi, idx, log_delta);
-    hipError_t e0 = hipGetLastError();
-    if (e0 != hipSuccess) return e0;
-  }
+  // Newton on the register path runs the Newton-only kernel (three waves per SIMD)
+  const bool newton = mode == MODE_NEWTON && regs;
 #define FLMM_REFINE1(PR, K, SM)                                                                                     \
   do {                                                                                                              \
-    if (pre32 && K > 0)                                                                                             \
+    if (newton && K > 0)                                                                                            \
       hipLaunchKernelGGL((delta_refine_newton_kernel<PR, K, SM>), grid, dim3(256), 0, stream, mode, uy, P, p_pad,    \
                          kpad, s, neff, s_lam, nlam, nco, G, lo, hi, idx, lam64, log_delta, rWr, h2, q1a, q1b, q2a,  \
-                         q2b, q1s, q2s, radius, judge_tol, flags, true);                                            \
+                         q2b, q1s, q2s, radius, judge_tol, flags);                                                \
     else                                                                                                            \
       hipLaunchKernelGGL((delta_refine_kernel<PR, K, SM, false>), grid, dim3(256), 0, stream, mode, uy, P, p_pad,   \
                          kpad, s, neff, s_lam, nlam, nco, G, lo, hi, idx, lam64, log_delta, rWr, h2, q1a, q1b, q2a,  \
-                         q2b, q1s, q2s, radius, judge_tol, flags, false);                                           \
+                         q2b, q1s, q2s, radius, judge_tol, flags);                                                \
   } while (0)
 #define FLMM_REFINE(PR, K)                                                                                          \
   do {                                                                                                              \
