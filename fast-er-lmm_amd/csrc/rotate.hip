@@ -18,7 +18,7 @@ namespace flmm {
 enum RotMode : int { MODE_UY = 0, MODE_US = 1, MODE_US_X2 = 2 };
 
 template <int PREC, int MODE>
-__global__ __launch_bounds__(NTHREADS, 1) void rotate_kernel(TileArgs<PREC, 1> args, int64_t n_jtiles,
+__global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(3))) void rotate_kernel(TileArgs<PREC, 1> args, int64_t n_jtiles,
                                                              int64_t n_tiles, const float* __restrict__ a_scale,
                                                              const float* __restrict__ b_scale, void* o0, void* o1,
                                                              void* o2, void* o3, float* __restrict__ out_scale1,
@@ -30,7 +30,7 @@ __global__ __launch_bounds__(NTHREADS, 1) void rotate_kernel(TileArgs<PREC, 1> a
   const int64_t i0 = it * TILE, j0 = jt * TILE;
 
   TileAcc<PREC, 1> acc;
-  tile_mainloop<PREC, 1>(args, i0, j0, acc, lds);
+  tile_mainloop<PREC, 1, false, false>(args, i0, j0, acc, lds);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -99,7 +99,7 @@ hipError_t launch_rotate(int mode, const TileArgs<PREC, 1>& args, const float* a
   const int64_t n_itiles = args.a_rows_pad / TILE;
   const int64_t n_jtiles = args.b_rows_pad / TILE;
   const int64_t n_tiles = n_itiles * n_jtiles;
-  const int lds = tile_lds_bytes<PREC, 1>();
+  const int lds = tile_lds_bytes<PREC, 1, false>();
   if (mode == MODE_UY)
     hipLaunchKernelGGL((rotate_kernel<PREC, MODE_UY>), dim3((unsigned)n_tiles), dim3(NTHREADS), lds, stream, args,
                        n_jtiles, n_tiles, a_scale, b_scale, o0, o1, o2, o3, out_scale1, out_scale2);
