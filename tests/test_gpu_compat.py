@@ -162,6 +162,20 @@ def test_loco_pipeline_is_bit_identical(core, ex, precision):
         assert torch.equal(getattr(got[True], k), getattr(got[False], k)), k
 
 
+def test_split_kinship_and_rotation_settings_vs_reference_golden(core, ex):
+    """The opt-in split-precision kinship Gram and SNP rotation (engine.settings kinship="split",
+    snp_rotation="split") still meet the scan bars against the reference's LOCO outputs on its example panel, whose
+    columns keep delta away from the grid's lower edge (where only the FP64 defaults hold beta to 1e-5)."""
+    from fasterlmm_amd import engine
+    ref = golden("compat_loco_cov2.npz")
+    X, Y = ex["X_cov2"], ex["Y"][:, :3]
+    with engine.configure(kinship="split", snp_rotation="split"):
+        res = core.loco_scan_compat(torch.from_numpy(ex["Zs"]).to(DEV), torch.from_numpy(X).to(DEV),
+                                    torch.from_numpy(Y).to(DEV), ex["chrom"])
+    check_scan(res, ref, ex["Zs"].shape[0] - X.shape[1] - 1, what="split kinship and rotation")
+    np.testing.assert_allclose(res.max_F.cpu().numpy(), ref["max_F"], rtol=1e-4)
+
+
 def test_perm_threshold_vs_reference_golden(ex):
     from fasterlmm_amd.io import align_inputs
     from fasterlmm_amd.perms import perm_threshold
