@@ -8,9 +8,15 @@ Reference: fasterlmm/core.py (gbrach/FaST-ER-LMM v1.6.2).
   without a ROCm GPU or without the built library every entry point raises.
 * ``snp_chunk`` / ``pheno_chunk`` are accepted for signature compatibility; tiling is internal (128 x 128 MFMA
   tiles) and, as in the reference (its chunking-invariance tests), invisible in the results.
-* Precision: the rotations and the two scan products run as split-BF16 MFMA (hi + lo, three products, FP32
-  accumulation) by default, or FP32 MFMA (``engine.configure(precision="fp32")``); the Wald epilogue, the
-  delta search, the projections and the eigendecompositions are FP64.
+* Precision: the column rotation and the two scan products run as split-FP16 MFMA by default (hi + lo IEEE
+  half on power-of-two-scaled rows, ~22 significant bits, three products, FP32 accumulation); split-BF16
+  (``engine.configure(precision="split_bf16")``, ~16 bits) and FP32 MFMA (``precision="fp32"``) are selectable.
+  The kinship Gram, the SNP rotation, the Wald epilogue, the delta search, the projections and the
+  eigendecompositions are FP64.
+* The plain GLS path needs a full-column-rank covariate design X and raises ValueError otherwise: the
+  reference's Cholesky of X~^T V^-1 X~ fails on a singular Gram and it substitutes an identity factor and an
+  infinite loss (core.py:166-171, 336-340), which leaves its outputs decided by rounding.  The compat path
+  handles a rank-deficient X through X^+ as the reference does.
 * Delta refinement defaults to safeguarded Newton to the local minimiser inside the reference's golden-section
   bracket; ``engine.configure(delta="golden")`` replays the reference's 50 golden-section steps instead.
 
@@ -94,8 +100,9 @@ def _home(res: ScanResult, home: torch.device) -> ScanResult:
 
 
 def _eigh(K: Tensor) -> tuple[Tensor, Tensor]:
-    """FP64 symmetric eigendecomposition on the GPU (rocSOLVER through torch), ascending (core.py:54-77).  An
-    out-of-memory error retries on the host LAPACK as the reference's fallback does; other errors propagate."""
+    """FP64 symmetric eigendecomposition on K's device (rocSOLVER through torch on the GPU), ascending
+    (core.py:54-77).  An out-of-memory error retries on the host LAPACK as the reference's fallback does (its
+    workspace is several times K); other errors propagate."""
     try:
         return torch.linalg.eigh(K)
     except RuntimeError as e:
@@ -107,6 +114,9 @@ def _eigh(K: Tensor) -> tuple[Tensor, Tensor]:
         torch.cuda.empty_cache()
         s, U = torch.linalg.eigh(Kc)
         return s.to(dev), U.to(dev)
+
+
+_eigh_with_cpu_fallback = _eigh  # the reference's name (core.py:54), imported by its tests
 
 
 def eigendecompose(K: Tensor, jitter: float = 1e-6) -> tuple[Tensor, Tensor]:
@@ -348,8 +358,8 @@ def _spec_state(spec) -> _Basis:
 
 
 def _kinship(Z: Tensor) -> Tensor:
-    """K = Z Z^T / M of a standardised panel (io.grm's product, io.py:257-265), the Gram on MFMA in the split
-    precisions (engine.gram)."""
+    """K = Z Z^T / M of a standardised panel (io.grm's product, io.py:257-265), the Gram by the kinship setting
+    (engine.gram: the FP64 Gram kernel by default)."""
     return engine.gram(Z) / float(Z.shape[1])
 
 
@@ -498,6 +508,26 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
     return _home(ScanResult(out.f, out.beta, out.se, out.sfve, out.nullh2, out.max_F), home)
 
 
+def _profile_loss_compat(log_delta: Tensor, s: Tensor, UY: Tensor) -> Tensor:
+    """(G, P) compat -2 log-likelihood Neff (log 2 pi + log(max(sum_k UY^2 / (s_k + delta), floor) / Neff) + 1) +
+    sum_k log(s_k + delta) on a grid of log delta (core.py:525-542), FP64 on the inputs' device.  The scan's own
+    delta search evaluates this loss inside the delta kernels; this is the reference's helper for its callers."""
+    neff = s.shape[0]
+    Sd = s.unsqueeze(0) + log_delta.exp().unsqueeze(1)
+    rWr = torch.clamp((1.0 / Sd) @ (UY * UY), min=_pos_floor(s.dtype))
+    log2pi = math.log(2.0 * math.pi)
+    return neff * (log2pi + (rWr / neff).log() + 1.0) + Sd.log().sum(dim=1).unsqueeze(1)
+
+
+def _profile_loss_compat_per_pheno(log_delta: Tensor, s: Tensor, UY: Tensor) -> Tensor:
+    """(P,) compat loss with one log delta per column (core.py:545-558)."""
+    neff = s.shape[0]
+    Sd = s.unsqueeze(0) + log_delta.exp().unsqueeze(1)
+    rWr = torch.clamp(((1.0 / Sd) * UY.T * UY.T).sum(dim=1), min=_pos_floor(s.dtype))
+    log2pi = math.log(2.0 * math.pi)
+    return neff * (log2pi + (rWr / neff).log() + 1.0) + Sd.log().sum(dim=1)
+
+
 def loco_scan_compat(Z: Tensor, X: Tensor, Y: Tensor, chrom: np.ndarray, *, n_real: int | None = None,
                      on_chrom=None, Z_kin: Tensor | None = None, chrom_kin: np.ndarray | None = None,
                      n_grid: int = 64) -> ScanResult:
@@ -520,6 +550,19 @@ def single_k_scan_compat(Z: Tensor, X: Tensor, Y: Tensor, *, n_real: int | None 
 
 # PLAIN GLS PATH -------
 
+def _require_full_rank(X, what: str = "X") -> None:
+    """ValueError unless the covariate design has full column rank.  The plain GLS path's Cholesky of
+    X~^T V^-1 X~ (core.py:166-171, 203-207, 336-340) fails on a singular Gram at some deltas and not at others,
+    and the reference then substitutes an identity factor (scan) and an infinite loss (delta fit): its results
+    for such an X depend on rounding, so this build refuses them instead of reproducing them."""
+    Xt = torch.as_tensor(X).to(torch.float64)
+    C = Xt.shape[1]
+    r = int(torch.linalg.matrix_rank(Xt)) if C > 0 and Xt.shape[0] > 0 else C
+    if r < C:
+        raise ValueError(f"the plain GLS path needs a full-column-rank covariate design {what}: rank {r} < {C} "
+                         f"columns (the compat path, loco_scan_compat / single_k_scan_compat, handles it)")
+
+
 def _plain_basis(s: Tensor, X_rot: Tensor, prec: int) -> tuple[_Basis, Tensor]:
     """Projected basis inside the rotated space of a Spectrum: Q = orthonormal basis of null(X_rot^T) (complete
     QR), eigh of Q^T diag(s) Q, B = Q V.  Returns (basis state, B (N, N - C))."""
@@ -535,6 +578,7 @@ def _plain_basis(s: Tensor, X_rot: Tensor, prec: int) -> tuple[_Basis, Tensor]:
 
 def rotate(K: Tensor, X: Tensor, Y: Tensor) -> Spectrum:
     """Spectrum: s, U = eigendecompose(K) (K + 1e-6 I, clamped), X_rot = U^T X, Y_rot = U^T Y (core.py:105-116)."""
+    _require_full_rank(X)
     home, (Kd, Xd, Yd) = _dev(K, X, Y)
     s, U = eigendecompose(Kd)
     X_rot = U.T @ Xd
@@ -550,6 +594,7 @@ def rotate(K: Tensor, X: Tensor, Y: Tensor) -> Spectrum:
 def _spectrum_state(spectrum: Spectrum) -> _Basis:
     st = getattr(spectrum, "_gpu", None)
     if st is None:
+        _require_full_rank(spectrum.X_rot, "X_rot")
         _, (s, X_rot, Y_rot) = _dev(spectrum.s, spectrum.X_rot, spectrum.Y_rot)
         st, _ = _plain_basis(s, X_rot, engine.prec_code())
         st.rotate_columns(Y_rot)
@@ -653,13 +698,15 @@ def loco_scan(Z: Tensor, X: Tensor, Y: Tensor, chrom: np.ndarray, *, n_real: int
               Z_kin: Tensor | None = None, chrom_kin: np.ndarray | None = None, n_grid: int = 64,
               reml: bool = False) -> ScanResult:
     """Leave-one-chromosome-out GLS scan (core.py:381-436): per chromosome K_loco + 1e-6 I, ML (or REML) delta
-    per column and the GLS Wald test of the chromosome's SNPs."""
+    per column and the GLS Wald test of the chromosome's SNPs.  X must have full column rank (ValueError)."""
+    _require_full_rank(X)
     return _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, plain=True, reml=reml)
 
 
 def single_k_scan(Z: Tensor, X: Tensor, Y: Tensor, *, n_real: int | None = None,
                   Z_kin: Tensor | None = None) -> ScanResult:
     """One kinship, GLS rotation, delta fit and scan of every SNP (core.py:439-451)."""
+    _require_full_rank(X)
     home, (Zd, Zkd) = _dev(Z, Z_kin)
     spec = rotate(_kinship(Zd if Zkd is None else Zkd), X, Y)
     ld = fit_delta_grid(spec)

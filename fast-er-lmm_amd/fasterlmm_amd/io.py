@@ -1,18 +1,75 @@
-"""Drop-in scan-input helpers of the reference's fasterlmm/io.py on MI355X: the aligned dataset, column
-standardisation (a gfx950 kernel), the kinship Gram, NA-pattern grouping and strain subsets.
+"""Drop-in scan-input helpers of the reference's fasterlmm/io.py on MI355X: the input records (Genotypes,
+Phenotypes, Covariates) and their readers, the aligned dataset, column standardisation (a gfx950 kernel), the
+kinship Gram (the FP64 Gram kernel), NA-pattern grouping and strain subsets.
 
-File readers (PLINK / phenotype / covariate tables) are not part of this module; ``AlignedDataset`` is built
-from arrays with ``align_inputs``.
+``align_inputs(geno, pheno, covar=None, *, dtype, kin)`` takes the reference's records as its callers pass them
+(fasterlmm/cli.py:231, the reference's tests); ``align_arrays`` is the same intersection for callers that hold
+plain arrays.  PLINK .bed reading (``read_plink``) decodes the 2-bit records on the GPU (bed.py).
 """
 from __future__ import annotations
 
-from dataclasses import dataclass
+from dataclasses import dataclass, field
+from pathlib import Path
 
 import numpy as np
 import torch
 from torch import Tensor
 
 from . import _lib
+from .bed import Genotypes, read_plink  # noqa: F401  (io.py:24-35, 54-67 of the reference)
+
+
+@dataclass
+class Phenotypes:
+    """Wide phenotype table (io.py:38-51): iid = N strains, names = P labels, Y (N, P) float64 with NaN for
+    missing; codebook / auto_categorical describe text columns coded to numbers."""
+    iid: list
+    names: list
+    Y: np.ndarray
+    codebook: dict = field(default_factory=dict)
+    auto_categorical: list = field(default_factory=list)
+
+
+@dataclass
+class Covariates:
+    """PLINK-style covariate table (io.py:97-106): C (N, K) with columns named cov_0, cov_1, ..."""
+    iid: list
+    names: list
+    C: np.ndarray
+
+
+# extra missing-value tokens of a phenotype table, on top of pandas' defaults (normalize.py:74)
+_TEXT_NA = ["unknown", "Unknown", "UNKNOWN", "?", "n/a", "N/A", "none given", "-"]
+
+
+def read_phen(path, levels: dict | None = None) -> Phenotypes:
+    """Wide phenotype TSV with a 'Strain' first column (io.py:70-94).  Numeric columns become Y (a column that
+    pandas read as text only because of stray blanks is converted, as in the reference).  Text-coded phenotype
+    columns (the reference's normalize.encode_text_column: yes/no tiers, --levels, one-vs-rest expansion) are
+    phenotype preparation outside the scan and raise ValueError here; code them numerically first."""
+    import pandas as pd
+
+    df = pd.read_csv(path, sep="\t", na_values=_TEXT_NA)
+    iid = df["Strain"].astype(str).tolist()
+    names = list(df.columns[1:])
+    for n in names:
+        if pd.api.types.is_numeric_dtype(df[n]):
+            continue
+        num = pd.to_numeric(df[n], errors="coerce")
+        if num.isna().sum() != df[n].isna().sum():
+            raise ValueError(f"phenotype column {n!r} holds text values; text coding is not part of this build")
+        df[n] = num
+    Y = df[names].to_numpy(dtype=np.float64)
+    return Phenotypes(iid=iid, names=names, Y=Y)
+
+
+def read_covar(path) -> Covariates:
+    """PLINK-style covariates 'FID IID c1 c2 ...', whitespace separated, no header (io.py:109-120)."""
+    import pandas as pd
+
+    df = pd.read_csv(Path(path), sep=r"\s+", header=None)
+    cov = df.iloc[:, 2:].to_numpy(dtype=np.float64)
+    return Covariates(iid=df.iloc[:, 1].astype(str).tolist(), names=[f"cov_{i}" for i in range(cov.shape[1])], C=cov)
 
 
 @dataclass
@@ -32,9 +89,21 @@ class AlignedDataset:
     n_dropped_kin: int = 0
 
 
-def align_inputs(geno_iid, Z, pheno_iid, Y, covar_iid=None, C=None, *, chrom, pos=None, snp_id=None,
+def align_inputs(geno: Genotypes, pheno: Phenotypes, covar: Covariates | None = None, *,
+                 dtype: torch.dtype = torch.float64, kin: Genotypes | None = None) -> AlignedDataset:
+    """Strain intersection of geno, pheno, the optional covar and kinship panel, in genotype file order, with
+    X = [1 | covariates], as torch tensors of `dtype` (io.py:143-184).  ValueError when no strain is shared."""
+    return align_arrays(geno.iid, geno.Z, pheno.iid, pheno.Y,
+                        None if covar is None else covar.iid, None if covar is None else covar.C,
+                        chrom=geno.chrom, pos=geno.pos, snp_id=geno.sid, pheno_names=pheno.names, dtype=dtype,
+                        kin_iid=None if kin is None else kin.iid, Z_kin=None if kin is None else kin.Z,
+                        chrom_kin=None if kin is None else kin.chrom)
+
+
+def align_arrays(geno_iid, Z, pheno_iid, Y, covar_iid=None, C=None, *, chrom, pos=None, snp_id=None,
                  pheno_names=None, dtype=torch.float64, kin_iid=None, Z_kin=None, chrom_kin=None) -> AlignedDataset:
-    """Strain intersection in genotype order and X = [1 | covariates] (io.py:143-184), from arrays."""
+    """align_inputs on plain arrays: strain ID lists with Z (N_g, M), Y (N_p, P), covariates C (N_c, K) and an
+    optional kinship panel Z_kin; pos / snp_id / pheno_names default to 0.. and p0.."""
     common = set(geno_iid) & set(pheno_iid)
     if covar_iid is not None:
         common &= set(covar_iid)
@@ -121,5 +190,12 @@ def standardise_columns(arr, *, stats: bool = False):
 
 
 def grm(Z: Tensor) -> Tensor:
-    """K = Z Z^T / M on pre-standardised Z (io.py:257-265); FP64 on the GPU."""
-    return (Z @ Z.T) / Z.shape[1]
+    """K = Z Z^T / M on pre-standardised Z (io.py:257-265): the FP64 Gram kernel (flmm_gram_f64, FP64 matrix
+    cores, deterministic and exactly symmetric) on the GPU; the result comes back on Z's device in Z's dtype."""
+    from . import engine
+
+    t = torch.as_tensor(Z)
+    home, dt = t.device, t.dtype
+    dev = _lib.device()
+    K = engine.gram(t.to(device=dev, dtype=torch.float64), prec=engine.prec_code()) / float(t.shape[1])
+    return K.to(device=home, dtype=dt)

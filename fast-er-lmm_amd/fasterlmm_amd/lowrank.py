@@ -220,6 +220,32 @@ def _state(spec: LowRankSpectrum) -> _LowRankState:
     return st
 
 
+def _profile_loss_lowrank(log_delta: Tensor, s: Tensor, UY: Tensor, yy_off: Tensor, Neff: int) -> Tensor:
+    """(G, P) low-rank -2 log-likelihood on a grid of log delta (lowrank.py:128-150), FP64 on the inputs' device:
+    on-rank sum UY^2 / (s + delta) plus the off-rank residual sum of squares yy_off / delta, log det over the k
+    kept values plus (Neff - k) log delta.  The scan's delta search evaluates this loss inside the delta kernels
+    (on the extended spectrum [s, 0]); this is the reference's helper for its callers."""
+    k = s.shape[0]
+    delta = log_delta.exp()
+    Sd = s.unsqueeze(0) + delta.unsqueeze(1)
+    rWr = torch.clamp((1.0 / Sd) @ (UY * UY) + yy_off.unsqueeze(0) / delta.unsqueeze(1),
+                      min=1e-300 if s.dtype == torch.float64 else 1e-30)
+    logdet = Sd.log().sum(dim=1) + (Neff - k) * delta.log()
+    return Neff * (np.log(2.0 * np.pi) + (rWr / Neff).log() + 1.0) + logdet.unsqueeze(1)
+
+
+def _profile_loss_lowrank_per_pheno(log_delta: Tensor, s: Tensor, UY: Tensor, yy_off: Tensor,
+                                    Neff: int) -> Tensor:
+    """(P,) low-rank loss with one log delta per column (lowrank.py:153-170)."""
+    k = s.shape[0]
+    delta = log_delta.exp()
+    Sd = s.unsqueeze(0) + delta.unsqueeze(1)
+    rWr = torch.clamp(((1.0 / Sd) * UY.T * UY.T).sum(dim=1) + yy_off / delta,
+                      min=1e-300 if s.dtype == torch.float64 else 1e-30)
+    logdet = Sd.log().sum(dim=1) + (Neff - k) * delta.log()
+    return Neff * (np.log(2.0 * np.pi) + (rWr / Neff).log() + 1.0) + logdet
+
+
 def fit_delta_grid_lowrank(spec: LowRankSpectrum, *, n_grid: int = 64, log_delta_min: float = -10.0,
                            log_delta_max: float = 10.0, refine: bool = True) -> Tensor:
     """lowrank.py:172-208: per-column log delta on the low-rank (ML) loss, grid then refinement in

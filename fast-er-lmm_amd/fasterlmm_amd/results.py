@@ -2,7 +2,7 @@
 
 The per-phenotype table is the 14 fastlmm ``single_snp`` columns plus the permutation ``threshold`` and a
 ``significant`` flag, sorted by PValue (fasterlmm/cli.py:114-161): per-phenotype directories hold ``gwas.tsv``
-(the 14 columns, bare unquoted TSV), ``perms.tsv``, ``threshold.txt`` and ``lambda_gc.txt``; the bundle is a
+(the 14 columns, bare unquoted TSV), ``perms.tsv`` and ``threshold.txt``; the bundle is a
 directory of parquet parts with all 16 columns (fasterlmm/bundle.py:38-120).  The p-values and the threshold
 come from the GPU F(1, df2) tail (``pvalues``); this module only lays the tables out.
 """
@@ -16,7 +16,6 @@ import numpy as np
 import pyarrow as pa
 import pyarrow.csv as pacsv
 import pyarrow.parquet as pq
-from scipy.stats import chi2
 
 # fastlmm single_snp's column order (the reference's tests/conftest.py GWAS_SCHEMA_COLS)
 GWAS_COLUMNS = ["sid_index", "SNP", "Chr", "GenDist", "ChrPos", "PValue", "SnpWeight", "SnpWeightSE",
@@ -24,16 +23,6 @@ GWAS_COLUMNS = ["sid_index", "SNP", "Chr", "GenDist", "ChrPos", "PValue", "SnpWe
 BUNDLE_PARTS_DIRNAME = ".bundle_parts"
 BUNDLE_FILENAME = "gwas_bundle.parquet"
 _TSV_WRITE_OPTS = pacsv.WriteOptions(delimiter="\t", include_header=False, quoting_style="none")
-
-
-def lambda_gc(p) -> tuple[float, int]:
-    """Genomic inflation from association p-values and the number used (fasterlmm/inflation.py:19-28):
-    median(chi2_1.isf(p)) / chi2_1.ppf(0.5) over the non-NaN p."""
-    p = np.asarray(p, dtype=np.float64)
-    p = p[~np.isnan(p)]
-    if p.size == 0:
-        return float("nan"), 0
-    return float(np.median(chi2.isf(p, 1)) / chi2.ppf(0.5, 1)), int(p.size)
 
 
 def write_tsv(table: pa.Table, path) -> None:
@@ -51,14 +40,13 @@ def writer_context(snp_id, chrom, pos, g_var, n_perm: int) -> dict:
             "pos": np.asarray(pos, dtype=np.float64), "g_var": np.asarray(g_var, dtype=np.float64),
             "sid_index": np.arange(M, dtype=np.int64), "gendist": pa.nulls(M, pa.float64()),
             "mixing": np.zeros(M), "phenocount": np.full(M, 1 + int(n_perm)),
-            "pheno_idx": pa.array(np.zeros(M, dtype=np.int32)), "lambda_rows": []}
+            "pheno_idx": pa.array(np.zeros(M, dtype=np.int32))}
 
 
 def pheno_table(ctx: dict, pheno_name: str, p_col, beta_col, se_col, sfve_col, nullh2_col, p_var: float,
-                perm_min_p, perm_quantile: float) -> tuple[pa.Table, float, float, int]:
-    """One phenotype's 16-column table sorted by PValue, its threshold, and its lambda GC (cli.py:114-144)."""
+                perm_min_p, perm_quantile: float) -> tuple[pa.Table, float]:
+    """One phenotype's 16-column table sorted by PValue and its threshold (cli.py:114-144)."""
     thresh = float(np.quantile(np.asarray(perm_min_p, dtype=np.float64), perm_quantile))
-    lam, n_var = lambda_gc(p_col)
     beta_col = np.asarray(beta_col, dtype=np.float64)
     effect_size = beta_col * beta_col * ctx["g_var"] / p_var  # fastlmm single_snp.py:1454
     p_col = np.asarray(p_col, dtype=np.float64)
@@ -69,23 +57,20 @@ def pheno_table(ctx: dict, pheno_name: str, p_col, beta_col, se_col, sfve_col, n
             "Nullh2": np.asarray(nullh2_col, dtype=np.float64),
             "Pheno": pa.DictionaryArray.from_arrays(ctx["pheno_idx"], [pheno_name]),
             "PhenoCount": ctx["phenocount"], "threshold": np.full(len(p_col), thresh), "significant": p_col < thresh}
-    return pa.table(cols).sort_by([("PValue", "ascending")]), thresh, lam, n_var
+    return pa.table(cols).sort_by([("PValue", "ascending")]), thresh
 
 
 def write_pheno(ctx: dict, outdir, pheno_name: str, p_col, beta_col, se_col, sfve_col, nullh2_col, p_var: float,
                 perm_min_p, perm_quantile: float, per_pheno_dirs: bool, bundle_writer) -> None:
     """Write one phenotype's results (cli.py:114-161): per-phenotype directory and/or one bundle row group."""
-    table, thresh, lam, n_var = pheno_table(ctx, pheno_name, p_col, beta_col, se_col, sfve_col, nullh2_col, p_var,
-                                            perm_min_p, perm_quantile)
-    ctx["lambda_rows"].append((pheno_name, lam, n_var))
+    table, thresh = pheno_table(ctx, pheno_name, p_col, beta_col, se_col, sfve_col, nullh2_col, p_var,
+                                perm_min_p, perm_quantile)
     if per_pheno_dirs:
         sub = Path(outdir) / pheno_name
         sub.mkdir(parents=True, exist_ok=True)
         write_tsv(table.select(GWAS_COLUMNS), sub / "gwas.tsv")
         write_tsv(pa.table({"perm_min_p": np.asarray(perm_min_p, dtype=np.float64)}), sub / "perms.tsv")
         (sub / "threshold.txt").write_text(f"{thresh:.6e}\n")
-        if bundle_writer is None:
-            (sub / "lambda_gc.txt").write_text(f"{lam:.6f}\n")
     if bundle_writer is not None:
         bundle_writer.append(table)
 

@@ -177,11 +177,11 @@ def test_split_kinship_and_rotation_settings_vs_reference_golden(core, ex):
 
 
 def test_perm_threshold_vs_reference_golden(ex):
-    from fasterlmm_amd.io import align_inputs
+    from fasterlmm_amd.io import align_arrays
     from fasterlmm_amd.perms import perm_threshold
     n = ex["Z_raw"].shape[0]
     iid = [str(i) for i in range(n)]
-    data = align_inputs(iid, ex["Z_raw"].astype(np.float64), iid, ex["Y"], iid, ex["X_cov2"][:, 1:],
+    data = align_arrays(iid, ex["Z_raw"].astype(np.float64), iid, ex["Y"], iid, ex["X_cov2"][:, 1:],
                         chrom=ex["chrom"])
     ref = golden("perm_loco_cov2.npz")
     res, pmf = perm_threshold(data, [0, 1], n_perm=4, seed=19930909, loco=True)

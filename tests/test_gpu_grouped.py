@@ -172,7 +172,7 @@ def test_autotuner_choice(mods, example, tmp_path):
 def _na_dataset(example, plant: bool = False):
     """Four missingness patterns over 18 phenotypes, missing calls in Z; plant: two phenotypes of three patterns
     are chromosome-2 SNPs plus noise (F from ~60 to ~1.4e5, above the FP64 re-scoring threshold)."""
-    from fasterlmm_amd.io import align_inputs
+    from fasterlmm_amd.io import align_arrays
     n = example["Z_raw"].shape[0]
     Y = example["Y"].astype(np.float64).copy()
     rng = np.random.default_rng(21)
@@ -190,7 +190,7 @@ def _na_dataset(example, plant: bool = False):
         for j in (0, 6, 12):
             Y[:, j] = np.where(np.isnan(Y[:, j]), np.nan, Zs[:, on2[3]] + 0.005 * noise[:, 0])
             Y[:, j + 1] = np.where(np.isnan(Y[:, j + 1]), np.nan, Zs[:, on2[40]] + 0.3 * noise[:, 1])
-    return align_inputs(iid, Z, iid, Y, iid, example["X_cov2"][:, 1:], chrom=example["chrom"]), Z, Y
+    return align_arrays(iid, Z, iid, Y, iid, example["X_cov2"][:, 1:], chrom=example["chrom"]), Z, Y
 
 
 @pytest.mark.parametrize("mode,loco", [("grouped", True), ("loop", True), ("grouped", False)])

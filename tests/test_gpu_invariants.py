@@ -134,11 +134,11 @@ def test_kinship_panel_identity_and_poisoning(ex):
 
 
 def test_permutation_determinism_batching_and_seed_idx(ex):
-    from fasterlmm_amd.io import align_inputs
+    from fasterlmm_amd.io import align_arrays
     from fasterlmm_amd.perms import perm_orders, perm_threshold
     n = ex["Z_raw"].shape[0]
     iid = [str(i) for i in range(n)]
-    data = align_inputs(iid, ex["Z_raw"].astype(np.float64), iid, ex["Y"], iid, ex["X_cov2"][:, 1:],
+    data = align_arrays(iid, ex["Z_raw"].astype(np.float64), iid, ex["Y"], iid, ex["X_cov2"][:, 1:],
                         chrom=ex["chrom"])
     r1, p1 = perm_threshold(data, [0, 1, 2], n_perm=3, seed=7)
     r2, p2 = perm_threshold(data, [0, 1, 2], n_perm=3, seed=7, on_chrom=lambda k, n: None)

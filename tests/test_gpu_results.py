@@ -14,11 +14,11 @@ pytestmark = pytest.mark.gpu
 
 
 def test_write_batch_matches_reference_pvalues(example, tmp_path):
-    from fasterlmm_amd.io import align_inputs
+    from fasterlmm_amd.io import align_arrays
     from fasterlmm_amd.perms import perm_threshold
     n = example["Z_raw"].shape[0]
     iid = [str(i) for i in range(n)]
-    data = align_inputs(iid, example["Z_raw"].astype(np.float64), iid, example["Y"], iid, example["X_cov2"][:, 1:],
+    data = align_arrays(iid, example["Z_raw"].astype(np.float64), iid, example["Y"], iid, example["X_cov2"][:, 1:],
                         chrom=example["chrom"])
     ref = golden("perm_loco_cov2.npz")
     res, pmf = perm_threshold(data, [0, 1], n_perm=4, seed=19930909, loco=True)

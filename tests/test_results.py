@@ -1,11 +1,10 @@
 """CPU tests of the result writer: the reference's column schema, sort order, threshold, EffectSize, TSV layout
-and the parquet bundle (fasterlmm/cli.py:103-161, fasterlmm/bundle.py:38-120, fasterlmm/inflation.py:19-28)."""
+and the parquet bundle (fasterlmm/cli.py:103-161, fasterlmm/bundle.py:38-120)."""
 from __future__ import annotations
 
 import numpy as np
 import pyarrow as pa
 import pyarrow.parquet as pq
-from scipy import stats
 
 from fasterlmm_amd import results
 
@@ -20,20 +19,9 @@ def _ctx_and_cols(M=40, seed=0):
     return ctx, cols
 
 
-def test_lambda_gc_definition():
-    p = np.random.default_rng(1).uniform(size=1001)
-    p[5] = np.nan
-    lam, n = results.lambda_gc(p)
-    q = p[~np.isnan(p)]
-    assert n == 1000
-    # chi2_1 quantile of 1 - p is the squared normal quantile: median(z^2) / median of chi2_1
-    assert abs(lam - np.median(stats.norm.isf(q / 2) ** 2) / stats.chi2.ppf(0.5, 1)) < 1e-9
-    assert np.isnan(results.lambda_gc([np.nan])[0])
-
-
 def test_pheno_table_schema_sort_threshold_effect_size():
     ctx, c = _ctx_and_cols()
-    t, thr, lam, n = results.pheno_table(ctx, "YAL001C", **c)
+    t, thr = results.pheno_table(ctx, "YAL001C", **c)
     assert t.column_names == results.GWAS_COLUMNS + ["threshold", "significant"]
     p = t.column("PValue").to_numpy()
     assert np.all(np.diff(p) >= 0)
@@ -58,9 +46,7 @@ def test_per_pheno_dirs_and_bundle(tmp_path):
     assert len(lines) == 41 and '"' not in "".join(lines)
     assert (tmp_path / "A" / "threshold.txt").read_text() == f"{np.quantile(c['perm_min_p'], 0.05):.6e}\n"
     assert (tmp_path / "A" / "perms.tsv").read_text().splitlines()[0] == "perm_min_p"
-    assert not (tmp_path / "A" / "lambda_gc.txt").exists()  # with a bundle, lambda goes to the summary only
     tab = pq.read_table(out)
     assert tab.column_names == results.GWAS_COLUMNS + ["threshold", "significant"]
     assert tab.num_rows == 80 and tab.column("Pheno").type == pa.string()
     assert sorted(set(tab.column("Pheno").to_pylist())) == ["A", "B"]
-    assert [r[0] for r in ctx["lambda_rows"]] == ["A", "B"]
