@@ -25,17 +25,24 @@ def neglog10p(F, df):
 
 
 def check_scan(res, ref, df, *, log10p_tol=LOG10P_TOL, beta_tol=BETA_TOL, what=""):
+    """Per-test bars against the FP64 reference: |d log10 p| <= log10p_tol; |d beta| <= beta_tol max(|beta|, se)
+    (on tests with |beta| >= se this is the relative beta error |d beta| / |beta| <= beta_tol, asserted on its own
+    as well; where |beta| < se, beta / beta_ref is undefined near zero and se is beta's statistical scale); and
+    relative d se <= beta_tol.  Returns (max |d log10 p|, max scaled d beta, max relative d beta on |beta| >= se)."""
     f = res["f"] if isinstance(res, dict) else res.f.cpu().numpy()
     beta = res["beta"] if isinstance(res, dict) else res.beta.cpu().numpy()
     se = res["se"] if isinstance(res, dict) else res.se.cpu().numpy()
     dlp = np.abs(neglog10p(f, df) - neglog10p(ref["f"], df)).max()
     scale = np.maximum(np.abs(ref["beta"]), ref["se"])
     dbeta = (np.abs(beta - ref["beta"]) / scale).max()
+    big = np.abs(ref["beta"]) >= ref["se"]
+    drel = float((np.abs(beta - ref["beta"])[big] / np.abs(ref["beta"])[big]).max()) if big.any() else 0.0
     dse = (np.abs(se - ref["se"]) / ref["se"]).max()
     assert dlp <= log10p_tol, f"{what}: max |d log10 p| = {dlp:.3e}"
     assert dbeta <= beta_tol, f"{what}: max |d beta| / max(|beta|, se) = {dbeta:.3e}"
+    assert drel <= beta_tol, f"{what}: max |d beta| / |beta| over |beta| >= se = {drel:.3e}"
     assert dse <= beta_tol, f"{what}: max rel d se = {dse:.3e}"
-    return dlp, dbeta
+    return dlp, dbeta, drel
 
 
 @pytest.fixture(scope="module")
