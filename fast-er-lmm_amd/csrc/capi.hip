@@ -5,6 +5,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <string>
 
 #include "../../include/fasterlmm_amd.h"
@@ -39,9 +40,103 @@ static bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 static bool valid_prec(int p) { return p == FLMM_PREC_SPLIT_BF16 || p == FLMM_PREC_FP32 || p == FLMM_PREC_SPLIT_FP16; }
 static bool two_planes(int p) { return p != FLMM_PREC_FP32; }
 
+// Argument checks and launch of the two-phase wide scan; `who` names the entry point in error messages.
+static int scan_lowrank(const char* who, int prec, const void* s1_hi, const void* s1_lo, const void* s2_hi,
+                        const void* s2_lo, const float* s1_scale, const float* s2_scale, int64_t mc_pad,
+                        const void* q1_hi, const void* q1_lo, const void* q2_hi, const void* q2_lo,
+                        const float* q1_scale, const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t kpad2,
+                        int64_t mc, int64_t P, int64_t n_real, const double* rWr, const double* h2, double nu,
+                        const int64_t* row_index, double* f, double* beta, double* se, double* sfve, double* nullh2,
+                        double* pmax, void* stream) {
+  REQUIRE(valid_prec(prec), "%s: bad precision %d", who, prec);
+  REQUIRE(mc >= 0 && mc <= mc_pad && mc_pad % TILE == 0, "%s: bad SNP extent", who);
+  REQUIRE(P >= 0 && P <= p_pad && p_pad % 256 == 0, "%s: bad column extent (p_pad %% 256)", who);
+  REQUIRE(kpad2 > 0 && kpad2 % TILE == 0 && kpad % TILE == 0 && kpad2 < kpad,
+          "%s: need 0 < kpad2 < kpad, both multiples of 128", who);
+  REQUIRE(n_real >= 0 && n_real <= P, "%s: n_real must be in [0, P]", who);
+  REQUIRE(nu > 0.0, "%s: dof must be positive", who);
+  const bool two = two_planes(prec);
+  REQUIRE(s1_hi && s2_hi && q1_hi && q2_hi && (!two || (s1_lo && s2_lo && q1_lo && q2_lo)),
+          "%s: null operand plane", who);
+  REQUIRE(aligned16(s1_hi) && aligned16(s2_hi) && aligned16(q1_hi) && aligned16(q2_hi) &&
+              (!two || (aligned16(s1_lo) && aligned16(s2_lo) && aligned16(q1_lo) && aligned16(q2_lo))),
+          "%s: operand planes must be 16-byte aligned", who);
+  REQUIRE(prec != FLMM_PREC_SPLIT_FP16 || (s1_scale && s2_scale && q1_scale && q2_scale),
+          "%s: split-FP16 needs the four row-scale arrays", who);
+  REQUIRE(rWr && pmax, "%s: null rWr / pmax", who);
+  REQUIRE(n_real == 0 || (f && beta && se && sfve && nullh2 && h2 && row_index),
+          "%s: null output", who);
+  if (mc == 0 || P == 0) return FLMM_OK;
+  ScanOut out{f, beta, se, sfve, nullh2, n_real};
+  ScanScales sc{nullptr, nullptr, nullptr, nullptr};
+  if (prec == FLMM_PREC_SPLIT_FP16) sc = ScanScales{s1_scale, s2_scale, q1_scale, q2_scale};
+  const hipStream_t st = (hipStream_t)stream;
+  hipError_t e;
+#define FLMM_SCAN(PR)                                                                                                \
+  do {                                                                                                               \
+    TileArgs<PR, 2> t{};                                                                                             \
+    t.a[0][0] = s1_hi; t.a[0][1] = s1_lo; t.a[1][0] = s2_hi; t.a[1][1] = s2_lo;                                      \
+    t.b[0][0] = q1_hi; t.b[0][1] = q1_lo; t.b[1][0] = q2_hi; t.b[1][1] = q2_lo;                                      \
+    t.a_rows_pad = mc_pad; t.b_rows_pad = p_pad; t.nchunks = (int)(kpad / KCHUNK);                                  \
+    t.nchunks2 = (int)(kpad2 / KCHUNK);                                                                              \
+    e = launch_wald_scan_lowrank<PR>(t, sc, mc, P, n_real, rWr, h2, nu, row_index, out, pmax, st);                  \
+  } while (0)
+  if (prec == FLMM_PREC_SPLIT_BF16) FLMM_SCAN(PREC_SPLIT_BF16);
+  else if (prec == FLMM_PREC_SPLIT_FP16) FLMM_SCAN(PREC_SPLIT_FP16);
+  else FLMM_SCAN(PREC_FP32);
+#undef FLMM_SCAN
+  return check_launch(e, who);
+}
+
+// ---- scratch of the allocation-free scan (flmm_wald_scan_lowrank_noalloc) ----
+// The per-tile column maxima of one scan live in this file-scope device array instead of a caller buffer.  Host
+// code must not pass `scratch` itself to a kernel (in host code the name is a host-side stand-in, not a device
+// address): scratch_ptr() looks the array's device address up with hipGetSymbolAddress once per device, with that
+// device current, and every launch uses that address.  Each GPU holds its own copy until the process ends.
+static __device__ __align__(256) double scratch[FLMM_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64];  // per device ordinal
+
+// Device address of this device's scratch array (the device that `stream` belongs to; the current device for the
+// null stream), or NULL with the error set.
+static double* scratch_ptr(hipStream_t stream, const char* who) {
+  int dev = 0;
+  hipError_t e = stream ? hipStreamGetDevice(stream, &dev) : hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    fail(FLMM_ERR_HIP, "%s: device of the stream: %s", who, hipGetErrorString(e));
+    return nullptr;
+  }
+  if (dev < 0 || dev >= 64) {
+    fail(FLMM_ERR_ARG, "%s: device ordinal %d out of range", who, dev);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    if ((e = hipGetDevice(&cur)) != hipSuccess) {
+      fail(FLMM_ERR_HIP, "%s: hipGetDevice: %s", who, hipGetErrorString(e));
+      return nullptr;
+    }
+    if (cur != dev && (e = hipSetDevice(dev)) != hipSuccess) {
+      fail(FLMM_ERR_HIP, "%s: hipSetDevice(%d): %s", who, dev, hipGetErrorString(e));
+      return nullptr;
+    }
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p) {
+      fail(FLMM_ERR_HIP, "%s: hipGetSymbolAddress(scratch): %s", who, hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  return (double*)g_scratch_ptr[dev];
+}
+
 extern "C" {
 
-int flmm_abi_version(void) { return 7; }
+int flmm_abi_version(void) { return 8; }
 
 const char* flmm_last_error(void) { return g_err.c_str(); }
 
@@ -387,44 +482,42 @@ int flmm_wald_scan_lowrank(int prec, const void* s1_hi, const void* s1_lo, const
                            int64_t n_real, const double* rWr, const double* h2, double nu, const int64_t* row_index,
                            double* f, double* beta, double* se, double* sfve, double* nullh2, double* pmax,
                            void* stream) {
-  REQUIRE(valid_prec(prec), "flmm_wald_scan_lowrank: bad precision %d", prec);
-  REQUIRE(mc >= 0 && mc <= mc_pad && mc_pad % TILE == 0, "flmm_wald_scan_lowrank: bad SNP extent");
-  REQUIRE(P >= 0 && P <= p_pad && p_pad % 256 == 0, "flmm_wald_scan_lowrank: bad column extent (p_pad %% 256)");
-  REQUIRE(kpad2 > 0 && kpad2 % TILE == 0 && kpad % TILE == 0 && kpad2 < kpad,
-          "flmm_wald_scan_lowrank: need 0 < kpad2 < kpad, both multiples of 128");
-  REQUIRE(n_real >= 0 && n_real <= P, "flmm_wald_scan_lowrank: n_real must be in [0, P]");
-  REQUIRE(nu > 0.0, "flmm_wald_scan_lowrank: dof must be positive");
-  const bool two = two_planes(prec);
-  REQUIRE(s1_hi && s2_hi && q1_hi && q2_hi && (!two || (s1_lo && s2_lo && q1_lo && q2_lo)),
-          "flmm_wald_scan_lowrank: null operand plane");
-  REQUIRE(aligned16(s1_hi) && aligned16(s2_hi) && aligned16(q1_hi) && aligned16(q2_hi) &&
-              (!two || (aligned16(s1_lo) && aligned16(s2_lo) && aligned16(q1_lo) && aligned16(q2_lo))),
-          "flmm_wald_scan_lowrank: operand planes must be 16-byte aligned");
-  REQUIRE(prec != FLMM_PREC_SPLIT_FP16 || (s1_scale && s2_scale && q1_scale && q2_scale),
-          "flmm_wald_scan_lowrank: split-FP16 needs the four row-scale arrays");
-  REQUIRE(rWr && pmax, "flmm_wald_scan_lowrank: null rWr / pmax");
-  REQUIRE(n_real == 0 || (f && beta && se && sfve && nullh2 && h2 && row_index),
-          "flmm_wald_scan_lowrank: null output");
-  if (mc == 0 || P == 0) return FLMM_OK;
-  ScanOut out{f, beta, se, sfve, nullh2, n_real};
-  ScanScales sc{nullptr, nullptr, nullptr, nullptr};
-  if (prec == FLMM_PREC_SPLIT_FP16) sc = ScanScales{s1_scale, s2_scale, q1_scale, q2_scale};
+  return scan_lowrank("flmm_wald_scan_lowrank", prec, s1_hi, s1_lo, s2_hi, s2_lo, s1_scale, s2_scale, mc_pad, q1_hi,
+                      q1_lo, q2_hi, q2_lo, q1_scale, q2_scale, p_pad, kpad, kpad2, mc, P, n_real, rWr, h2, nu,
+                      row_index, f, beta, se, sfve, nullh2, pmax, stream);
+}
+
+int flmm_wald_scan_lowrank_noalloc(int prec, const void* s1_hi, const void* s1_lo, const void* s2_hi,
+                                   const void* s2_lo, const float* s1_scale, const float* s2_scale, int64_t mc_pad,
+                                   const void* q1_hi, const void* q1_lo, const void* q2_hi, const void* q2_lo,
+                                   const float* q1_scale, const float* q2_scale, int64_t p_pad, int64_t kpad,
+                                   int64_t kpad2, int64_t mc, int64_t P, int64_t n_real, const double* rWr,
+                                   const double* h2, double nu, const int64_t* row_index, double* f, double* beta,
+                                   double* se, double* sfve, double* nullh2, double* max_F, void* stream) {
+  const char* who = "flmm_wald_scan_lowrank_noalloc";
+  REQUIRE(mc >= 0 && P >= 0 && p_pad >= 0, "%s: bad sizes", who);
+  REQUIRE(max_F, "%s: null max_F", who);
+  const int64_t n_mt = (mc + TILE - 1) / TILE;
+  const int64_t need = n_mt * p_pad * (int64_t)sizeof(double);  // the kernel's per-tile maxima: n_mt x p_pad
+  REQUIRE(need <= FLMM_NOALLOC_SCRATCH_BYTES,
+          "%s: %lld SNP tiles x p_pad %lld need %lld bytes of scratch, above the maximum of %lld bytes; split the SNP "
+          "block (ceil(mc / 128) x p_pad x 8 <= %lld)",
+          who, (long long)n_mt, (long long)p_pad, (long long)need, (long long)FLMM_NOALLOC_SCRATCH_BYTES,
+          (long long)FLMM_NOALLOC_SCRATCH_BYTES);
+  if (mc == 0 || P == 0) return scan_lowrank(who, prec, s1_hi, s1_lo, s2_hi, s2_lo, s1_scale, s2_scale, mc_pad, q1_hi,
+                                             q1_lo, q2_hi, q2_lo, q1_scale, q2_scale, p_pad, kpad, kpad2, mc, P,
+                                             n_real, rWr, h2, nu, row_index, f, beta, se, sfve, nullh2, max_F,
+                                             stream);
   const hipStream_t st = (hipStream_t)stream;
-  hipError_t e;
-#define FLMM_SCAN(PR)                                                                                                \
-  do {                                                                                                               \
-    TileArgs<PR, 2> t{};                                                                                             \
-    t.a[0][0] = s1_hi; t.a[0][1] = s1_lo; t.a[1][0] = s2_hi; t.a[1][1] = s2_lo;                                      \
-    t.b[0][0] = q1_hi; t.b[0][1] = q1_lo; t.b[1][0] = q2_hi; t.b[1][1] = q2_lo;                                      \
-    t.a_rows_pad = mc_pad; t.b_rows_pad = p_pad; t.nchunks = (int)(kpad / KCHUNK);                                  \
-    t.nchunks2 = (int)(kpad2 / KCHUNK);                                                                              \
-    e = launch_wald_scan_lowrank<PR>(t, sc, mc, P, n_real, rWr, h2, nu, row_index, out, pmax, st);                  \
-  } while (0)
-  if (prec == FLMM_PREC_SPLIT_BF16) FLMM_SCAN(PREC_SPLIT_BF16);
-  else if (prec == FLMM_PREC_SPLIT_FP16) FLMM_SCAN(PREC_SPLIT_FP16);
-  else FLMM_SCAN(PREC_FP32);
-#undef FLMM_SCAN
-  return check_launch(e, "flmm_wald_scan_lowrank");
+  double* pm = scratch_ptr(st, who);
+  if (!pm) return g_err.empty() ? fail(FLMM_ERR_HIP, "%s: no scratch", who) : FLMM_ERR_HIP;
+  // every (tile, column < P) entry of the maxima is written by this call's scan before the fold reads it, so the
+  // scratch needs no clearing; calls that overlap on one GPU share the array and race
+  const int rc = scan_lowrank(who, prec, s1_hi, s1_lo, s2_hi, s2_lo, s1_scale, s2_scale, mc_pad, q1_hi, q1_lo, q2_hi,
+                              q2_lo, q1_scale, q2_scale, p_pad, kpad, kpad2, mc, P, n_real, rWr, h2, nu, row_index, f,
+                              beta, se, sfve, nullh2, pm, stream);
+  if (rc != FLMM_OK) return rc;
+  return check_launch(launch_max_reduce(pm, n_mt, p_pad, P, max_F, st), who);
 }
 
 int64_t flmm_scan_table_bytes(int n_groups) {

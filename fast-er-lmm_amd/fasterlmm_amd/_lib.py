@@ -34,6 +34,8 @@ _I64 = C.c_int64
 _I = C.c_int
 _D = C.c_double
 
+NOALLOC_SCRATCH_BYTES = 64 << 20  # FLMM_NOALLOC_SCRATCH_BYTES
+
 # name -> argtypes, in header order
 SIGNATURES: dict[str, list] = {
     "flmm_abi_version": [],
@@ -68,6 +70,8 @@ SIGNATURES: dict[str, list] = {
                        _P, _D, _P, _P, _P, _P, _P, _P, _P, _P],
     "flmm_wald_scan_lowrank": [_I, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64,
                                _I64, _I64, _P, _P, _D, _P, _P, _P, _P, _P, _P, _P, _P],
+    "flmm_wald_scan_lowrank_noalloc": [_I, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64,
+                                       _I64, _I64, _I64, _P, _P, _D, _P, _P, _P, _P, _P, _P, _P, _P],
     "flmm_max_reduce": [_P, _I64, _I64, _I64, _P, _P],
     "flmm_neglog10p_f1": [_P, _I64, _D, _P, _P],
     "flmm_effect_size": [_P, _I64, _I64, _P, _P, _P, _P],

@@ -284,6 +284,23 @@ int flmm_wald_scan_lowrank(int prec, const void* s1_hi, const void* s1_lo, const
                            double* f, double* beta, double* se, double* sfve, double* nullh2, double* pmax,
                            void* stream);
 
+/* Allocation-free variant of flmm_wald_scan_lowrank followed by flmm_max_reduce.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Arguments as flmm_wald_scan_lowrank, except that the per-tile maxima
+ * live in the library's own device scratch (a file-scope array of FLMM_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends) and are folded into max_F (P doubles, max_F[p] = max(max_F[p], block maximum)) by the same
+ * call, so the caller passes only inputs and outputs.  Calls no hipMalloc / hipFree.  Needs
+ * ceil(mc / 128) x p_pad x 8 <= FLMM_NOALLOC_SCRATCH_BYTES (FLMM_ERR_ARG otherwise: split the SNP block).  All
+ * calls on one GPU share its scratch: calls that overlap on one GPU (different streams, or threads) race. */
+#define FLMM_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int flmm_wald_scan_lowrank_noalloc(int prec, const void* s1_hi, const void* s1_lo, const void* s2_hi,
+                                   const void* s2_lo, const float* s1_scale, const float* s2_scale, int64_t mc_pad,
+                                   const void* q1_hi, const void* q1_lo, const void* q2_hi, const void* q2_lo,
+                                   const float* q1_scale, const float* q2_scale, int64_t p_pad, int64_t kpad,
+                                   int64_t kpad2, int64_t mc, int64_t P, int64_t n_real, const double* rWr,
+                                   const double* h2, double nu, const int64_t* row_index, double* f, double* beta,
+                                   double* se, double* sfve, double* nullh2, double* max_F, void* stream);
+
 /* Fused Wald scan of one SNP block (snp_wald_scan_compat's tile loop, core.py:636-667): sWy = S1.Q1, sWs = S2.Q2 on
  * MFMA, FP64 epilogue with dof nu; split-FP16 sums are divided by the planes' row scales.  Real columns
  * p < n_real write f, beta, se, sfve, nullh2 at [row_index[m] * n_real + p] of (M, n_real) row-major FP64 arrays
