@@ -13,7 +13,8 @@ permutations), C=4 -- 6.5488e10 Wald tests per step.
 
 The timed path is split-FP16 MFMA (power-of-two-scaled fp16 hi + lo, 3 products, FP32 accumulation, FP64 epilogue);
 the same step on the split-BF16 and FP32 MFMA paths and the FP64 CPU oracle (on a bounded sample) are reported
-beside it.  Prints one JSON line.
+beside it, followed (unless --no-others) by one step of the uneven-NA-group workload (--config 3) and of the
+large-cohort low-rank workload (--config 5) as extra keys.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -46,6 +47,8 @@ def parse():
     ap.add_argument("--precision", default="split_fp16", choices=["split_fp16", "split_bf16", "fp32"])
     ap.add_argument("--no-compare", action="store_true", help="skip the split-BF16 / FP32 comparison steps")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-others", action="store_true",
+                    help="config 2: skip the uneven-NA-group and low-rank cohort summaries that follow the headline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB first (reference kernel for HBM counters)")
@@ -96,15 +99,20 @@ def run_step(Z, X, Y_all, chrom, B, precision):
         return core.loco_scan_compat(Z, X, Y_all, chrom, n_real=B, n_grid=100)
 
 
-def compat_cpu_check(panel, res, B, n_top=50, frac=0.01):
-    """Full-size parity of the GPU result against the FP64 oracle (numpy, host cores): chromosome 1's LOCO
+def compat_cpu_check(panel, res, B, others=None, *, Z=None, X=None, Y_all=None, n_top=50, frac=0.01, n_perm_cols=2000):
+    """Full-size parity of the GPU results against the FP64 oracle (numpy, host cores): chromosome 1's LOCO
     kinship, projected eigh, rotation and per-column delta (100-point grid + 50 golden-section steps) for the B
     real phenotypes, then the Wald tests of a 1% SNP sample of chromosome 1 plus its n_top SNPs of largest GPU F,
     against every real phenotype.  Reports max |d log10 p| (tests whose p is below the double range in scipy's
-    tail are compared by F), max |d beta| / max(|beta|, se), and the GPU's log delta against the oracle's."""
+    tail are compared by F), max |d beta| / max(|beta|, se), the relative beta error on tests with |beta| >= se,
+    and the GPU's log delta against the oracle's, for the headline result and for each of `others` ({name: result},
+    e.g. the split-BF16 run).  With Z / X / Y_all (the GPU inputs) it also checks the permutation columns: a random
+    sample of n_perm_cols of them is scanned on the GPU over chromosome 1 with its LOCO kinship (the same kernels
+    as the headline) and their maximum F over the chromosome's SNPs is compared with the oracle's."""
     import numpy as np
     import torch
     from scipy import stats
+    from fasterlmm_amd import core
     from oracle import lmm
     t0 = time.perf_counter()
     Zs = lmm.standardise_columns(panel.G.astype(np.float64))
@@ -122,22 +130,48 @@ def compat_cpu_check(panel, res, B, n_top=50, frac=0.01):
     ref = lmm.snp_wald_scan_compat(spec, ld, Zs[:, rows], n_real=B)
     df = spec["s"].shape[0] - 1
     ri = torch.from_numpy(rows).to(res.f.device)
-    got = {k: getattr(res, k).index_select(0, ri).cpu().numpy() for k in ("f", "beta", "se", "nullh2")}
     lp = lambda F: -stats.f.logsf(F, 1, df) / np.log(10.0)  # noqa: E731
-    a, b = lp(got["f"]), lp(ref["f"])
-    fin = np.isfinite(a) & np.isfinite(b)
-    dlp = float(np.abs(a - b)[fin].max())
-    dbeta = float((np.abs(got["beta"] - ref["beta"]) / np.maximum(np.abs(ref["beta"]), ref["se"])).max())
-    ld_gpu = np.log(1.0 / got["nullh2"][0] - 1.0)
-    dld = np.abs(ld_gpu - ld)
-    worst = int(np.argmax(np.abs(a - b).max(axis=0) * fin.all(axis=0)))
-    return {"sample": f"chromosome {c}: {rows.size} SNPs ({frac:.0%} random + {n_top} of largest GPU F) x {B} real "
-                      f"phenotypes = {rows.size * B} tests; FP64 oracle LOCO pipeline, {time.perf_counter() - t0:.1f}s",
-            "max_abs_dlog10p": dlp, "max_dbeta_rel": dbeta, "max_F_oracle": float(ref["f"].max()),
-            "max_abs_dlog_delta": float(dld.max()), "columns_dlog_delta_gt_1e-3": int((dld > 1e-3).sum()),
-            "worst_column": {"col": worst, "log_delta_gpu": float(ld_gpu[worst]), "log_delta_oracle": float(ld[worst]),
-                             "dlog10p": float(np.abs(a - b)[:, worst].max())},
-            "not_finite_in_scipy": int((~fin).sum())}
+    b_ref = lp(ref["f"])
+    big = np.abs(ref["beta"]) >= ref["se"]
+
+    def compare(r):
+        got = {k: getattr(r, k).index_select(0, ri).cpu().numpy() for k in ("f", "beta", "se", "nullh2")}
+        a = lp(got["f"])
+        fin = np.isfinite(a) & np.isfinite(b_ref)
+        dlp = float(np.abs(a - b_ref)[fin].max())
+        db = np.abs(got["beta"] - ref["beta"])
+        ld_gpu = np.log(1.0 / got["nullh2"][0] - 1.0)
+        dld = np.abs(ld_gpu - ld)
+        worst = int(np.argmax(np.abs(a - b_ref).max(axis=0) * fin.all(axis=0)))
+        return {"max_abs_dlog10p": dlp, "max_dbeta_rel": float((db / np.maximum(np.abs(ref["beta"]), ref["se"])).max()),
+                "max_dbeta_over_abs_beta_where_abs_beta_ge_se": float((db[big] / np.abs(ref["beta"][big])).max()),
+                "tests_with_abs_beta_ge_se": int(big.sum()),
+                "max_abs_dlog_delta": float(dld.max()), "columns_dlog_delta_gt_1e-3": int((dld > 1e-3).sum()),
+                "worst_column": {"col": worst, "log_delta_gpu": float(ld_gpu[worst]),
+                                 "log_delta_oracle": float(ld[worst]), "dlog10p": float(np.abs(a - b_ref)[:, worst].max())},
+                "not_finite_in_scipy": int((~fin).sum())}
+
+    out = {"sample": f"chromosome {c}: {rows.size} SNPs ({frac:.0%} random + {n_top} of largest GPU F) x {B} real "
+                     f"phenotypes = {rows.size * B} tests; FP64 oracle LOCO pipeline",
+           "max_F_oracle": float(ref["f"].max()), **compare(res)}
+    for name, r in (others or {}).items():
+        out[name] = compare(r)
+    if Z is not None and Y_all is not None and Y_all.shape[1] > B:
+        P = Y_all.shape[1]
+        sel = np.sort(rng.choice(np.arange(B, P), size=min(n_perm_cols, P - B), replace=False))
+        Ysel = Y_all[:, torch.from_numpy(sel).to(Y_all.device)]
+        on_t = torch.from_numpy(test).to(Z.device)
+        rg = core.loco_scan_compat(Z.index_select(1, on_t), X, Ysel, chrom[test], n_real=0, Z_kin=Z, chrom_kin=chrom,
+                                   n_grid=100)
+        pspec = lmm.compat_rotate(K, panel.X, Ysel.cpu().numpy())
+        pld = lmm.fit_delta_grid_compat(pspec, n_grid=100)
+        pref = lmm.snp_wald_scan_compat(pspec, pld, Zs[:, test], n_real=0)
+        rel = np.abs(rg.max_F.cpu().numpy() - pref["max_F"]) / pref["max_F"]
+        out["perm_max_F"] = {"columns": int(sel.size), "snps": int(test.size), "max_rel_err": float(rel.max()),
+                             "median_rel_err": float(np.median(rel)),
+                             "min_log_delta_oracle": float(pld.min()), "max_F_oracle": float(pref["max_F"].max())}
+    out["oracle_s"] = round(time.perf_counter() - t0, 1)
+    return out
 
 
 def cpu_baseline(panel, cfg, n_perm, budget_s):
@@ -167,7 +201,10 @@ def cpu_baseline(panel, cfg, n_perm, budget_s):
     assert np.isfinite(res["max_F"]).all()
     return {"value": tests / dt, "unit": "Wald tests/s", "cores": threads, "kind": "port",
             "sample": f"oracle FP64 compat LOCO pipeline for chromosome {c} ({int(test.sum())} SNPs) x {n_cols} columns "
-                      f"({n_ph} phenotypes x {1 + n_perm}), N={Zs.shape[0]}, {dt:.1f}s wall, numpy/BLAS on {threads} threads"}
+                      f"({n_ph} phenotypes x {1 + n_perm}), N={Zs.shape[0]}, {dt:.1f}s wall, numpy/BLAS on {threads} threads; "
+                      f"this is smaller than the planned slice (all 654,884 columns over 1-2 chromosomes): one batch, "
+                      f"so the kinship, projected eigh and rotation are computed once, not once per 256-phenotype "
+                      f"batch as the reference's CLI does, which makes the rate an upper bound for the reference"}
 
 
 def traffic_per_launch(prec, scan):
@@ -204,7 +241,7 @@ def timed_run(Z, X, Y_all, chrom, B, precision, steps):
     return res, dt, scan
 
 
-def run_grouped(args):
+def run_grouped(args, emit=True):
     """Uneven batching (N=5,000 x M=200,000 x 2,000 RINT phenotypes in 64 seeded missing-at-random NA groups with
     retained N ~ U(3,000, 5,000), C=6, 100 permutations each): every group's full kinship, then the first
     chromosome's per-group LOCO bases, delta fits and SNP planes, and the scans of all 64 groups for that
@@ -323,10 +360,12 @@ def run_grouped(args):
                        "stages_ms": stages},
         "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
 
 
-def run_lowrank(args):
+def run_lowrank(args, emit=True):
     """Large-cohort low-rank path: N=50,000 x M=500,000 int8 dosages resident in HBM, scanned in 16,384-SNP blocks
     against a rank-2,000 kinship of 50,000 evenly strided markers (randomised range finder per LOCO kinship),
     P=1,000 phenotypes x (1 + 20 permutations), C=10, LOCO over 22 chromosomes.  One step = the whole LOCO scan.
@@ -420,7 +459,9 @@ def run_lowrank(args):
                      "note": "algorithmic flops per test = 4 k (on-rank products) + 2 N (off-rank cross term)"},
         "stages_ms_per_step": stages, "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
 
 
 def lowrank_cpu_check(panel, Gk, g_chrom, Y_all, B, res, rank, prec):
@@ -535,24 +576,25 @@ def main():
     peak = published / products
     kernel = (f"wald_scan_lowrank_kernel<{prec}> (factored weights)" if scan["factored"]
               else f"wald_scan_wide_kernel<{prec}>")
-    roofline = {"bound": "mfma", "achieved": round(alg, 2), "peak": round(peak, 1), "unit": "TFLOP/s",
-                "frac": round(alg / peak, 4), "traffic": traffic, "traffic_detail": traffic_detail,
+    issued = products * tflops(scan, "kernel_flops")  # MFMA flop the launches execute
+    roofline = {"bound": "mfma", "achieved": round(issued, 2), "peak": published, "unit": "TFLOP/s",
+                "frac": round(issued / published, 4), "traffic": traffic, "traffic_detail": traffic_detail,
                 "kernel": kernel, "mean_launch_ms": round(scan["mean_ms"], 4),
                 "launches_per_step": scan["launches"] // args.steps,
-                "published_peak_tflops": published,
-                "mfma_issued_tflops": round(products * tflops(scan, "kernel_flops"), 2),
-                "mfma_utilisation": round(products * tflops(scan, "kernel_flops") / published, 4),
+                "algorithmic_tflops": round(alg, 2), "algorithmic_frac_of_peak": round(alg / published, 4),
+                "algorithmic_frac_of_peak_per_split_product": round(alg / peak, 4),
                 "launched_algorithm_tflops": round(tflops(scan), 2),
                 "factored_terms": scan["R"],
-                "note": "achieved = the compat scan's algorithmic 4 (N - C) flop per Wald test x tests per launch / "
-                        "mean kernel time (HIP events on the launch stream); peak = published dense MFMA peak / 3 "
-                        "products per split product. With factored weights the kernel computes sWs over an R-term "
-                        "exponential sum instead of N - C terms: launched_algorithm_tflops counts its own "
-                        "2 (N - C) + 2 R flop per test, and mfma_utilisation = issued MFMA flop (3 products over "
-                        "the padded K) / (time x published peak), so frac exceeds mfma_utilisation by the work the "
-                        "factoring saves"}
+                "note": "achieved = MFMA flop executed per launch (3 split products over the padded K = "
+                        "kpad + kpad2 per test, factored weights) / mean kernel time (HIP events on the launch "
+                        "stream), against the published dense MFMA peak: frac is the issued MFMA utilisation. "
+                        "algorithmic_tflops counts the compat scan's 4 (N - C) flop per Wald test (the reference "
+                        "algorithm's work; the factored kernel executes 2 (N - C) + 2 R, launched_algorithm_tflops); "
+                        "algorithmic_frac_of_peak divides it by the published peak, "
+                        "algorithmic_frac_of_peak_per_split_product by a third of it"}
 
     extra = {}
+    kept = {}
     from fasterlmm_amd.pvalues import neglog10p
     lp_head = neglog10p(res.f, df)  # on the GPU: the host would need minutes for 1e9 F values
     for other in [] if args.no_compare else [p for p in ("split_bf16", "fp32") if p != prec]:
@@ -569,6 +611,8 @@ def main():
                                   "headline_scan_kernel_speedup": (s2["total_ms"] / (scan["total_ms"] / args.steps))
                                   if scan["total_ms"] else None}
         log(f"[bench] {other}: {1000 * dt2:.1f} ms/step, {rate:.4e} tests/s; {prec}/{other} = {value / rate:.2f}x")
+        if other == "split_bf16":
+            kept[other] = r2  # checked against the oracle below
         del r2
 
     cpu = None
@@ -577,8 +621,9 @@ def main():
         cpu = cpu_baseline(panel, cfg, n_perm, args.cpu_seconds)
         log(f"[bench] cpu baseline {cpu['value']:.4e} tests/s in {time.perf_counter() - t2:.1f}s")
         if args.config == 2:
-            cpu["gpu_vs_oracle"] = compat_cpu_check(panel, res, B)
+            cpu["gpu_vs_oracle"] = compat_cpu_check(panel, res, B, kept, Z=Z, X=X, Y_all=Y_all)
             log(f"[bench] full-size parity vs the FP64 oracle: {cpu['gpu_vs_oracle']}")
+    kept.clear()
 
     desc = {"split_fp16": "split-FP16 (power-of-two-scaled fp16 hi+lo, 3 MFMA products, FP32 accumulate)",
             "split_bf16": "split-BF16 (bf16 hi+lo, 3 MFMA products, FP32 accumulate)",
@@ -594,6 +639,14 @@ def main():
                    "tests_per_step": tests_per_step, "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "stages_ms_per_step": scan["stages_ms_per_step"], **extra,
     }
+    if args.config == 2 and not args.no_others:
+        # the uneven-NA-group and large-cohort workloads, measured in the same run (one step each)
+        del res, Z, X, Y_all, lp_head
+        torch.cuda.empty_cache()
+        sub = argparse.Namespace(**{**vars(args), "steps": 1, "warmup": 1, "n_perm": None, "pheno": None, "snps": None})
+        line["uneven_na_groups"] = run_grouped(sub, emit=False)
+        torch.cuda.empty_cache()
+        line["low_rank_cohort"] = run_lowrank(argparse.Namespace(**{**vars(sub), "warmup": 0}), emit=False)
     print(json.dumps(line), flush=True)
 
 

@@ -17,27 +17,44 @@ namespace flmm {
 
 constexpr int NBIN = 32;
 
-// one wave per column: bin sums of y^2 through LDS atomics, then the 34 features
+// one wave per column: bin sums of y^2 in a fixed order, then the 34 features.  y^2 is staged through LDS 1,024
+// elements at a time and lane b < 32 adds the chunk's elements of bin b in index order, so the features are
+// bit-reproducible from run to run.
 __global__ __launch_bounds__(256) void mlp_features_kernel(const float* __restrict__ uy, int64_t P, int64_t p_pad,
                                                            int64_t neff, const int* __restrict__ bin_of_k,
                                                            float logN, float C, float* __restrict__ feat) {
+  constexpr int CH = 1024;
   __shared__ float acc[4][NBIN];
   __shared__ int cnt[4][NBIN];
+  __shared__ float y2s[4][CH];
+  __shared__ int bins[CH];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t p = (int64_t)blockIdx.x * 4 + w;
-  if (lane < NBIN) { acc[w][lane] = 0.f; cnt[w][lane] = 0; }
-  __syncthreads();
-  float tot = 0.f;
-  if (p < P) {
-    for (int64_t k = lane; k < neff; k += 64) {
-      const float y = uy[((k >> 5) * p_pad + p) * 32 + (k & 31)];
-      const float y2 = y * y;
-      const int b = bin_of_k[k];
-      atomicAdd(&acc[w][b], y2);
-      atomicAdd(&cnt[w][b], 1);
-      tot += y2;
+  float tot = 0.f, my_acc = 0.f;
+  int my_cnt = 0;
+  for (int64_t k0 = 0; k0 < neff; k0 += CH) {
+    const int n = neff - k0 < CH ? (int)(neff - k0) : CH;
+    __syncthreads();  // the previous chunk has been consumed
+    for (int i = threadIdx.x; i < n; i += blockDim.x) bins[i] = bin_of_k[k0 + i];
+    if (p < P) {
+      for (int i = lane; i < n; i += 64) {
+        const int64_t k = k0 + i;
+        const float y = uy[((k >> 5) * p_pad + p) * 32 + (k & 31)];
+        y2s[w][i] = y * y;
+        tot += y * y;
+      }
+    }
+    __syncthreads();
+    if (p < P && lane < NBIN) {
+      for (int i = 0; i < n; ++i) {
+        if (bins[i] == lane) {
+          my_acc += y2s[w][i];
+          ++my_cnt;
+        }
+      }
     }
   }
+  if (lane < NBIN) { acc[w][lane] = my_acc; cnt[w][lane] = my_cnt; }
   tot = wave_sum(tot);
   __syncthreads();
   // per-bin log(mean y^2 / overall mean); empty bins interpolated linearly (in bin index) between the nearest
