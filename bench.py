@@ -480,11 +480,13 @@ def lowrank_cpu_check(panel, Gk, g_chrom, Y_all, B, res, rank, prec):
     X = panel.X.cpu().numpy()
     Nn = X.shape[0]
     df = Nn - X.shape[1] - 1
-    G32 = Gk.float()
+    factor = None
     for c in (chroms[0], chroms[-1]):
         keep = torch.from_numpy((g_chrom != c).astype(np.float64)).to(Gk.device)
         with engine.configure(precision=prec):
-            b = lowrank._basis(Gk, panel.X, keep, rank=rank, G32=G32)  # the LOCO engine's call, same bits
+            if factor is None:
+                factor = lowrank.SplitFactor(Gk, engine.prec_code())  # deterministic: the LOCO engine's planes
+            b = lowrank._basis(Gk, panel.X, keep, rank=rank, factor=factor)  # the LOCO engine's call, same bits
             UY, UUY = lowrank.rotate_phenos(b, Y_all)
             spec = lowrank.LowRankSpectrum(s=b.s, U_eff=b.U_eff, X=b.X, Xpinv=b.Xpinv, Neff=b.Neff, UY=UY, UUY=UUY)
             ld = lowrank.fit_delta_grid_lowrank(spec)

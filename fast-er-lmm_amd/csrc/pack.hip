@@ -61,34 +61,50 @@ __global__ __launch_bounds__(256) void pack_planes_kernel(const double* __restri
   }
 }
 
-// norm[r] = sqrt(sum_k src(r, k)^2); thread per row when rows are the unit-stride direction, else wave per row
+// norm[r] = sqrt(sum_k src(r, k)^2).  Rows along the unit stride (the transposed view of an (N, P) matrix):
+// a 1,024-thread block owns 64 consecutive rows, lane = row, and its 16 waves split k (wave w sums k = w, w + 16,
+// ...), so every load is 64 consecutive doubles and a few thousand rows against N ~ 10^4..10^5 strains still fill
+// the chip; the 16 partial sums are added in wave order.  Otherwise a wave per row, lanes striding k.
+__global__ __launch_bounds__(1024) void row_norms_strided_kernel(const double* __restrict__ src, int64_t n_rows,
+                                                                 int64_t n_k, int64_t k_stride,
+                                                                 const int64_t* __restrict__ row_index,
+                                                                 double* __restrict__ norm) {
+  __shared__ double part[16][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t r = (int64_t)blockIdx.x * 64 + lane;
+  double s = 0.0;
+  if (r < n_rows) {
+    const int64_t sr = row_index ? row_index[r] : r;
+    for (int64_t k = w; k < n_k; k += 16) {
+      const double x = src[sr + k * k_stride];
+      s += x * x;
+    }
+  }
+  part[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && r < n_rows) {
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) t += part[q][lane];
+    norm[r] = sqrt(t);
+  }
+}
+
 __global__ __launch_bounds__(256) void row_norms_kernel(const double* __restrict__ src, int64_t n_rows, int64_t n_k,
                                                         int64_t row_stride, int64_t k_stride,
                                                         const int64_t* __restrict__ row_index,
                                                         double* __restrict__ norm) {
-  if (row_stride == 1 && k_stride != 1) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n_rows) return;
-    const int64_t sr = row_index ? row_index[r] : r;
-    double s = 0.0;
-    for (int64_t k = 0; k < n_k; ++k) {
-      const double x = src[sr * row_stride + k * k_stride];
-      s += x * x;
-    }
-    norm[r] = sqrt(s);
-  } else {
-    const int lane = threadIdx.x & 63;
-    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (r >= n_rows) return;
-    const int64_t sr = row_index ? row_index[r] : r;
-    double s = 0.0;
-    for (int64_t k = lane; k < n_k; k += 64) {
-      const double x = src[sr * row_stride + k * k_stride];
-      s += x * x;
-    }
-    s = wave_sum(s);
-    if (lane == 0) norm[r] = sqrt(s);
+  const int lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= n_rows) return;
+  const int64_t sr = row_index ? row_index[r] : r;
+  double s = 0.0;
+  for (int64_t k = lane; k < n_k; k += 64) {
+    const double x = src[sr * row_stride + k * k_stride];
+    s += x * x;
   }
+  s = wave_sum(s);
+  if (lane == 0) norm[r] = sqrt(s);
 }
 
 hipError_t launch_pack_planes(int prec, const double* src, int64_t n_rows, int64_t n_k, int64_t row_stride,
@@ -107,10 +123,12 @@ hipError_t launch_pack_planes(int prec, const double* src, int64_t n_rows, int64
 
 hipError_t launch_row_norms(const double* src, int64_t n_rows, int64_t n_k, int64_t row_stride, int64_t k_stride,
                             const int64_t* row_index, double* norm, hipStream_t stream) {
-  const bool thread_per_row = row_stride == 1 && k_stride != 1;
-  const dim3 grid((unsigned)(thread_per_row ? (n_rows + 255) / 256 : (n_rows + 3) / 4));
-  hipLaunchKernelGGL(row_norms_kernel, grid, dim3(256), 0, stream, src, n_rows, n_k, row_stride, k_stride, row_index,
-                     norm);
+  if (row_stride == 1 && k_stride != 1)
+    hipLaunchKernelGGL(row_norms_strided_kernel, dim3((unsigned)((n_rows + 63) / 64)), dim3(1024), 0, stream, src,
+                       n_rows, n_k, k_stride, row_index, norm);
+  else
+    hipLaunchKernelGGL(row_norms_kernel, dim3((unsigned)((n_rows + 3) / 4)), dim3(256), 0, stream, src, n_rows, n_k,
+                       row_stride, k_stride, row_index, norm);
   return hipGetLastError();
 }
 

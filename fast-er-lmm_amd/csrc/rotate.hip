@@ -221,19 +221,26 @@ hipError_t launch_rescore(const double* US, const double* UY, int64_t ld, const 
 // both are small.  One 256-thread workgroup per 128 x 128 output tile, the four waves 2 x 2, each with 4 x 4 blocks
 // of 16 x 16 FP64 accumulators; both operands are strain-major (k = strain), staged 16 strains at a time in
 // k-major LDS rows 144 doubles apart (a half-wave's 8-byte reads of 16 consecutive entries of two k rows cover the
-// 64 banks once); the next stage is fetched into registers while the current one is multiplied.
+// 64 banks once); the next stage is fetched into registers while the current one is multiplied.  The 64 FP64
+// accumulators per lane live in VGPRs capped at two waves per SIMD, so a second block's products fill the first
+// one's barriers (one wave per SIMD left the matrix pipe idle through every LDS stage).
 constexpr int R64_KT = 16;
 constexpr int R64_LD = TILE + 16;
 constexpr int R64_LOADS = R64_KT / (NTHREADS / TILE);  // k rows per thread and stage (8)
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(NTHREADS) void rotate_f64_kernel(const double* __restrict__ z, int64_t ldz,
+__global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(2))) void rotate_f64_kernel(
+    const double* __restrict__ z, int64_t ldz,
                                                               const int64_t* __restrict__ col_index, int64_t mc,
                                                               int64_t n, const double* __restrict__ u, int64_t ldu,
                                                               int64_t neff, double* __restrict__ out, int64_t ldo) {
   __shared__ double sa[R64_KT * R64_LD], sb[R64_KT * R64_LD];
-  const int64_t i0 = (int64_t)blockIdx.x * TILE, j0 = (int64_t)blockIdx.y * TILE;
+  // one-dimensional grid over (row tile, column tile), column tiles fastest and XCD-contiguous, so the blocks
+  // that share a row tile of z run together on one L2
+  const int64_t n_jt = ldo / TILE;
+  const int64_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t i0 = (tile / n_jt) * TILE, j0 = (tile % n_jt) * TILE;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int li = tid % TILE, lk = tid / TILE;  // this thread stages entry li of k rows lk, lk + 2, ..., lk + 14
@@ -298,7 +305,7 @@ __global__ __launch_bounds__(NTHREADS) void rotate_f64_kernel(const double* __re
 hipError_t launch_rotate_f64(const double* z, int64_t ldz, const int64_t* col_index, int64_t mc, int64_t n,
                              const double* u, int64_t ldu, int64_t neff, double* out, int64_t rows_pad, int64_t kpad,
                              hipStream_t stream) {
-  const dim3 grid((unsigned)(rows_pad / TILE), (unsigned)(kpad / TILE));
+  const dim3 grid((unsigned)((rows_pad / TILE) * (kpad / TILE)));
   hipLaunchKernelGGL(rotate_f64_kernel, grid, dim3(NTHREADS), 0, stream, z, ldz, col_index, mc, n, u, ldu, neff, out,
                      kpad);
   return hipGetLastError();

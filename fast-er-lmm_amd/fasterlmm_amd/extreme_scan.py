@@ -16,8 +16,8 @@ from torch import Tensor
 
 from . import engine
 from .core import ScanResult, _dev, _home
-from .lowrank import (LowRankSpectrum, _basis, _ColumnPlanes, _scan_blocks, _state, fit_delta_grid_lowrank,
-                      refresh_real_max, rotate_phenos)
+from .lowrank import (LowRankSpectrum, SplitFactor, _basis, _ColumnPlanes, _scan_blocks, _state,
+                      fit_delta_grid_lowrank, refresh_real_max, rotate_phenos)
 
 
 def _loco_scan_core(chroms: list, block_source, M: int, G: Tensor, g_chrom: np.ndarray, X: Tensor, Y: Tensor, *,
@@ -30,11 +30,12 @@ def _loco_scan_core(chroms: list, block_source, M: int, G: Tensor, g_chrom: np.n
     n_real = P if n_real is None else min(int(n_real), P)
     dev = Gd.device
     out = engine.ScanOutputs.allocate(M, n_real, P, dev)
-    G32 = Gd.float() if rank is not None else None  # the range finder's iterations run on an FP32 copy
+    # the range finder's products run against G's MFMA operand planes, packed once for every chromosome
+    factor = SplitFactor(Gd, engine.prec_code()) if rank is not None else None
     for k_idx, c in enumerate(chroms):
         with engine.stage("basis"):
             keep = torch.from_numpy((g_chrom != c).astype(np.float64)).to(dev)
-            basis = _basis(Gd, Xd, keep, rank=rank, G32=G32)
+            basis = _basis(Gd, Xd, keep, rank=rank, factor=factor)
             del keep
         with engine.stage("rotate_columns"):
             UY, UUY = rotate_phenos(basis, Yd)

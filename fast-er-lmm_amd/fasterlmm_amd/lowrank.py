@@ -17,8 +17,12 @@ How the kernels carry the off-rank terms (one column p, one SNP s, w = 1/(s_j + 
     Q1 = [w o U^T y_r ; UUY / delta], because UUS^T UUY = s_r^T UUY (UUY is orthogonal to U and X);
     sWs = S2.Q2 over kpad2 only, with S2 = [(U^T s_r)^2 ; |UUS|^2 at entry k].  So the off-rank cross term is
     one K = N contraction on MFMA and the off-rank SNP sum of squares rides as one extra spectral entry.
-The basis, rotations and off-rank residuals are FP64 (rocBLAS / rocSOLVER through torch); operand planes are
-split-FP16 / split-BF16 / FP32 as ``engine.settings.precision`` says, the epilogue FP64.
+Where the work runs: the rotations U^T Y_r, U UY and U^T S_r and the range finder's Rayleigh-Ritz product
+G^T Q are FP64 on the FP64 matrix cores (flmm_rotate_f64); the range finder's subspace iterations (two K = N or
+k_all products per application) run on the MFMA rotation kernel (flmm_rotate) against G's operand planes, packed
+once per LOCO run; the small eigendecompositions, CholeskyQR Grams and the C x C covariate algebra are FP64 in
+rocSOLVER / rocBLAS through torch.  Scan operand planes are split-FP16 / split-BF16 / FP32 as
+``engine.settings.precision`` says, the epilogue FP64.
 """
 from __future__ import annotations
 
@@ -74,17 +78,19 @@ def lowrank_basis(G: Tensor, X: Tensor, *, sv_floor: float = 1e-10, rank: int | 
 
 
 def _basis(G: Tensor, X: Tensor, keep: Tensor | None, *, sv_floor: float = 1e-10, rank: int | None = None,
-           n_oversample: int = 20, n_iter: int = 2, seed: int = 19930909, G32: Tensor | None = None) -> LowRankBasis:
+           n_oversample: int = 20, n_iter: int = 2, seed: int = 19930909,
+           factor: "SplitFactor | None" = None) -> LowRankBasis:
     """lowrank_basis of the columns `keep` (a 0/1 FP64 mask over G's columns, None = all) of the FP64 GPU factor G.
     With a rank the masked factor is never copied (the mask multiplies G^T Y) and the range finder's subspace
-    iterations run on an FP32 copy of G (G32, made here if not given); the last orthonormalisation and the
-    Rayleigh-Ritz projection are FP64, so U is orthonormal and orthogonal to X to FP64 precision."""
+    iterations run on the MFMA rotation kernel against G's operand planes (``factor``, packed here if not given);
+    the last orthonormalisation and the Rayleigh-Ritz projection are FP64, so U is orthonormal and orthogonal to X
+    to FP64 precision."""
     N, k_all = G.shape
     D = X.shape[1]
     Xpinv = torch.linalg.pinv(X)
     k_in = k_all if keep is None else int(keep.sum().item())
     if rank is not None and rank < min(N, k_in):
-        s, U = _randomized_factor(G, X, Xpinv, keep, k_in, int(rank), n_oversample, n_iter, seed, G32)
+        s, U = _randomized_factor(G, X, Xpinv, keep, k_in, int(rank), n_oversample, n_iter, seed, factor)
         ok = s.sqrt() * np.sqrt(k_in) > sv_floor
         s, U = s[ok], U[:, ok]
         order = torch.argsort(s)
@@ -107,35 +113,68 @@ def _basis(G: Tensor, X: Tensor, keep: Tensor | None, *, sv_floor: float = 1e-10
     return LowRankBasis(s=s, U_eff=U, X=X, Xpinv=Xpinv, Neff=N - D)
 
 
+class SplitFactor:
+    """The kinship factor G (N, k_all) as MFMA operand planes in both orientations, packed once and shared by every
+    chromosome of a LOCO run: ``gt`` (rows = markers, K = strains) for G^T y and ``g`` (rows = strains, K =
+    markers) for G z, in the scan's precision (split-FP16 by default: hi + lo halves with power-of-two row scales,
+    three products, FP32 accumulation -- the accuracy of an FP32 GEMM)."""
+
+    def __init__(self, G: Tensor, prec: int):
+        N, k_all = G.shape
+        G = G.contiguous()
+        self.N, self.k_all, self.prec = N, k_all, prec
+        self.npad, self.mpad = rup(max(N, 1), TILE), rup(max(k_all, 1), TILE)
+        self.gt = engine.pack(prec, G, k_all, N, 1, k_all, self.mpad, self.npad)
+        self.g = engine.pack(prec, G, N, k_all, k_all, 1, self.npad, self.mpad)
+
+    def _times(self, a: engine.Planes, rows: int, v: Tensor, kpad: int) -> Tensor:
+        """(rows, b) FP64 of a . v for v (K, b) FP64: v^T is packed as the B operand and flmm_rotate forms
+        out[i][j] = sum_n a[i][n] v[n][j]."""
+        b = v.shape[1]
+        v = v.contiguous()
+        bp = rup(max(b, 1), TILE)
+        vp = engine.pack(self.prec, v, b, v.shape[0], 1, b, bp, kpad)
+        out = engine.rotate_uy(a, vp)
+        return engine.fp32_plane_to_dense(out, a.rows_pad, bp, rows, b).double()
+
+    def gt_times(self, y: Tensor) -> Tensor:
+        """G^T y for y (N, b)."""
+        return self._times(self.gt, self.k_all, y, self.npad)
+
+    def g_times(self, z: Tensor) -> Tensor:
+        """G z for z (k_all, b)."""
+        return self._times(self.g, self.N, z, self.mpad)
+
+
 def _randomized_factor(G: Tensor, X: Tensor, Xpinv: Tensor, keep: Tensor | None, k_in: int, rank: int,
-                       n_oversample: int, n_iter: int, seed: int, G32: Tensor | None):
+                       n_oversample: int, n_iter: int, seed: int, factor: SplitFactor | None):
     """Top-`rank` eigenpairs of K_ = G_ G_^T / k_in, G_ = (I - X X^+) G[:, keep], by the randomised range finder
     (Halko, Martinsson & Tropp 2011, alg. 4.4 + 5.3; extreme.py:23-45): Y = K_ Omega and n_iter subspace
-    iterations (re-orthonormalised) in FP32, then an orthonormal basis Q of (I - X X^+) Y and the eigh of
-    Q^T K_ Q in FP64."""
+    iterations (re-orthonormalised in FP64), each K_ application two MFMA products against G's planes, then an
+    orthonormal basis Q of (I - X X^+) Y and the eigh of Q^T K_ Q in FP64."""
     N = G.shape[0]
     target = min(rank + n_oversample, N)
-    G32 = G.float() if G32 is None else G32
-    X32, Xp32 = X.float(), Xpinv.float()
-    m32 = None if keep is None else keep.float().unsqueeze(1)
+    f = factor if factor is not None else SplitFactor(G, engine.prec_code())
+    mask = None if keep is None else keep.unsqueeze(1)
 
-    def apply32(Y):
-        Z = G32.T @ (Y - X32 @ (Xp32 @ Y))
-        if m32 is not None:
-            Z = Z * m32
-        W = G32 @ Z
-        return (W - X32 @ (Xp32 @ W)) / k_in
+    def apply(Y):
+        Z = f.gt_times(_regress(X, Xpinv, Y))
+        if mask is not None:
+            Z = Z * mask
+        return _regress(X, Xpinv, f.g_times(Z)) / k_in
 
     from .extreme import orthonormal_basis
     gen = torch.Generator(device=G.device).manual_seed(seed)
-    Y = apply32(torch.randn(N, target, generator=gen, device=G.device, dtype=torch.float32))
+    Y = apply(torch.randn(N, target, generator=gen, device=G.device, dtype=torch.float32).double())
     for _ in range(n_iter):
-        Y = apply32(orthonormal_basis(Y).float())
-    Q = orthonormal_basis(_regress(X, Xpinv, Y.double()))
+        Y = apply(orthonormal_basis(Y))
+    Q = orthonormal_basis(_regress(X, Xpinv, Y))
     del Y
-    Zq = G.T @ Q  # Q is orthogonal to X, so G_^T Q = G^T Q
-    if keep is not None:
-        Zq = Zq * keep.unsqueeze(1)
+    # Rayleigh-Ritz in FP64: Zq = G[:, keep]^T Q on the FP64 matrix cores (Q is orthogonal to X, so G_^T Q = G^T Q)
+    cols = None if keep is None else torch.nonzero(keep).flatten()
+    n_keep = G.shape[1] if cols is None else cols.numel()
+    t = Q.shape[1]
+    Zq = engine.rotate_snps_f64(G, cols, n_keep, Q, 0, t, rup(max(n_keep, 1), TILE), rup(t, TILE))[:n_keep, :t]
     B = (Zq.T @ Zq) / k_in
     del Zq
     s_small, U_small = torch.linalg.eigh(0.5 * (B + B.T))
@@ -145,11 +184,17 @@ def _randomized_factor(G: Tensor, X: Tensor, Xpinv: Tensor, keep: Tensor | None,
 
 
 def rotate_phenos(basis: LowRankBasis, Y: Tensor) -> tuple[Tensor, Tensor]:
-    """lowrank.py:99-111: Y_r = Y - X X^+ Y, UY = U^T Y_r (k, P), UUY = Y_r - U UY (N, P), FP64 on the GPU."""
+    """lowrank.py:99-111: Y_r = Y - X X^+ Y, UY = U^T Y_r (k, P), UUY = Y_r - U UY (N, P), FP64 on the GPU: both
+    products on the FP64 matrix cores (flmm_rotate_f64)."""
     home, (Yd, Xd, Xp, U) = _dev(Y, basis.X, basis.Xpinv, basis.U_eff)
     Y_r = _regress(Xd, Xp, Yd)
-    UY = U.T @ Y_r
-    return UY.to(home), (Y_r - U @ UY).to(home)
+    N, P = Y_r.shape
+    k = U.shape[1]
+    if k == 0 or P == 0:
+        return torch.zeros(k, P, dtype=Y_r.dtype, device=Y_r.device).to(home), Y_r.to(home)
+    UY = engine.rotate_snps_f64(Y_r, None, P, U, 0, k, rup(P, TILE), rup(k, TILE))[:P, :k].T.contiguous()
+    UUY = Y_r - engine.rotate_snps_f64(U.T.contiguous(), None, N, UY, 0, P, rup(N, TILE), rup(P, TILE))[:N, :P]
+    return UY.to(home), UUY.to(home)
 
 
 def lowrank_rotate(G: Tensor, X: Tensor, Y: Tensor, *, sv_floor: float = 1e-10, rank: int | None = None,
@@ -264,9 +309,10 @@ def _norms(parts: list[Tensor]) -> Tensor:
 
 
 def pack_concat(prec: int, parts: list[Tensor], rows_pad: int, kpads: list[int]) -> engine.Planes:
-    """Operand planes whose K axis is the concatenation of the row-major (n_k_i, n_rows) FP64 blocks `parts`,
-    block i occupying kpads[i] (a multiple of 32) entries; split-FP16 rows share one scale (from the norm of the
-    whole row), so each block is packed with the combined norm at its K offset."""
+    """Operand planes whose K axis is the concatenation of the (n_k_i, n_rows) FP64 blocks `parts` (any strides:
+    a transposed view is packed in place), block i occupying kpads[i] (a multiple of 32) entries; split-FP16 rows
+    share one scale (from the norm of the whole row), so each block is packed with the combined norm at its K
+    offset."""
     n_rows = parts[0].shape[1]
     dev = parts[0].device
     pl = engine.Planes(prec, rows_pad, sum(kpads), dev)
@@ -275,12 +321,11 @@ def pack_concat(prec: int, parts: list[Tensor], rows_pad: int, kpads: list[int])
     k0 = 0
     st = _lib.stream()
     for part, kp in zip(parts, kpads):
-        part = part.contiguous()
         off = k0 * rows_pad * esize
         hi = pl.hi.data_ptr() + off
         lo = pl.lo.data_ptr() + off if pl.lo is not None else None
-        _lib.call("flmm_pack_planes", pl.prec, part.data_ptr(), n_rows, part.shape[0], 1, n_rows, None,
-                  _lib.ptr(norm), hi, lo, pl.scale_ptr, rows_pad, kp, st)
+        _lib.call("flmm_pack_planes", pl.prec, part.data_ptr(), n_rows, part.shape[0], part.stride(1), part.stride(0),
+                  None, _lib.ptr(norm), hi, lo, pl.scale_ptr, rows_pad, kp, st)
         k0 += kp
     return pl
 
@@ -378,7 +423,7 @@ def snp_planes_lowrank(st: _LowRankState, U: Tensor, S_r: Tensor):
     (K = kpad2) for the regressed-out SNP block S_r (N, mc)."""
     mc = S_r.shape[1]
     rows_pad = rup(mc, TILE)
-    US = U.T @ S_r  # (k, mc)
+    US = engine.rotate_snps_f64(S_r, None, mc, U, 0, st.k, rows_pad, st.kpad)[:mc, :st.k].T  # (k, mc) FP64 view
     us2 = US * US
     ss_off = torch.clamp((S_r * S_r).sum(dim=0) - us2.sum(dim=0), min=0.0)
     s1 = pack_concat(st.prec, [US, S_r], rows_pad, [st.kpad, rup(st.N, TILE)])

@@ -222,7 +222,7 @@ def test_lowrank_loco_at_reduced_cohort_size():
     df = N - C - 1
     for c in (1, 2):
         keep = torch.from_numpy((g_chrom != c).astype(np.float64)).to(DEV)
-        b = lowrank._basis(Gk, T(X), keep, rank=rank, G32=Gk.float())
+        b = lowrank._basis(Gk, T(X), keep, rank=rank)  # packs the same factor planes as the LOCO run: same bits
         UY, UUY = lowrank.rotate_phenos(b, T(Y))
         spec = lowrank.LowRankSpectrum(s=b.s, U_eff=b.U_eff, X=b.X, Xpinv=b.Xpinv, Neff=b.Neff, UY=UY, UUY=UUY)
         ld = lowrank.fit_delta_grid_lowrank(spec).cpu().numpy()
