@@ -24,7 +24,12 @@
 //                L'(t) = delta (W1 - n R2 / R), W1 = sum 1/(s_lam+delta), R2 = sum y^2 w^2, w = 1/(s+delta):
 //                no logarithms per iteration and quadratic convergence; bisection keeps the iterate inside
 //                the bracket.  A bracket whose left slope is >= 0 (right slope <= 0) has its minimum at that
-//                edge, which is where golden section converges too.
+//                edge, which is where golden section converges too.  Where the FP64 grid minimum is interior
+//                and its two neighbours are higher, FP64 Newton starts at the vertex of the parabola through the
+//                three node losses (within half a step of the minimiser, so a few FP64 steps); otherwise an FP32
+//                Newton phase from the node comes first.  A grid minimum at an end node with the loss still
+//                falling towards that end (h2 ~ 0 puts most permutation columns at the upper end) is settled
+//                by one sweep that also yields the column's outputs.
 // The refinement then writes the scan's column-side operand planes Q1 = w o y and Q2 = w for the final delta
 // (split-FP16: each column scaled by the power of two of its maximum, recorded in q1s / q2s; Q2 skipped for
 // factored weights, whose planes factored_planes_kernel writes),
@@ -318,7 +323,8 @@ __device__ inline double node_loss_from_R(const LK& L, double R, int g, const do
 
 template <class LK, bool FUSED>
 __device__ inline int fp64_grid_argmin(const LK& L, int64_t p, int i, int G, double lo, double hi,
-                                       const double* __restrict__ lam64, int lane) {
+                                       const double* __restrict__ lam64, int lane, double& t_vertex) {
+  t_vertex = NAN;
   if constexpr (FUSED) {
     // the usual outcome -- the node is the minimum of its two neighbours -- costs one sweep for all three losses;
     // a move continues node by node below, exactly as the unfused walk
@@ -340,7 +346,14 @@ __device__ inline int fp64_grid_argmin(const LK& L, int64_t p, int i, int G, dou
         }
         return i;
       }
-      if (!(lp < l0)) return i;
+      if (!(lp < l0)) {
+        // the node is the minimum of its neighbours (lm > l0 <= lp): the vertex of the parabola through the three
+        // FP64 node losses lies within half a grid step of it and starts the search there
+        const double h = grid_value(i + 1, G, lo, hi) - grid_value(i, G, lo, hi);
+        const double den = lm - 2.0 * l0 + lp;
+        if (den > 0.0) t_vertex = grid_value(i, G, lo, hi) + 0.5 * h * (lm - lp) / den;
+        return i;
+      }
       ++i;
       double li = lp;
       for (int it = 0; it < G && i + 1 < G; ++it) {
@@ -428,11 +441,20 @@ template <class LK>
 __device__ inline double newton_fp64(const LK& L, int64_t p, double a0, double b0, double t, int lane);
 
 template <class LK, bool FP32_START>
-__device__ inline double newton_bracket(const LK& L, int64_t p, int i, int G, double lo, double hi, int lane) {
+__device__ inline double newton_bracket(const LK& L, int64_t p, int i, int G, double lo, double hi, int lane,
+                                        double t_vertex) {
   double a0, b0, t;
   grid_bracket(i, G, lo, hi, a0, b0, t);
   if (!(b0 > a0)) return a0;
-  if constexpr (FP32_START) newton_fp32(L, a0, b0, t, lane);
+  // grid minimum at an end node: where the loss still falls towards that end the search ends there (what the
+  // end-slope check inside newton_fp64 returns, here without the interior iterations first)
+  if (i == 0 || i == G - 1) {
+    double g, gp;
+    col_slope(L, p, i == 0 ? a0 : b0, lane, g, gp);
+    if (i == 0 ? g >= 0.0 : g <= 0.0) return i == 0 ? a0 : b0;
+  }
+  if (t_vertex > a0 && t_vertex < b0) t = t_vertex;
+  else if constexpr (FP32_START) newton_fp32(L, a0, b0, t, lane);
   return newton_fp64(L, p, a0, b0, t, lane);
 }
 
@@ -509,12 +531,73 @@ __device__ __forceinline__ void delta_refine_body(
   }
   ColLik<KPL, SAME> L{uy, p_pad, s, neff, s_lam, nlam, nco, sh_s, SAME ? sh_s : sh_l, sh_sf, SAME ? sh_sf : sh_lf, {}};
   L.load(p, lane);
+  if constexpr (KPL > 0 && SAME) {
+    // Grid minimum at an end node (most permutation columns: h2 ~ 0 puts delta at the upper end).  One sweep gives
+    // everything the search and the outputs need there: R at the end node and at its neighbour (the FP64 walk's
+    // comparison), the slope sums at the end node (the end check of newton_bracket) and the end node's rWr and
+    // plane maxima (pass 1 below), each summed exactly as those steps sum it, so the result is theirs bit for bit.
+    // A walk that moves, or a slope that turns back into the bracket, takes the general path.  MODE_LEARNED takes
+    // the same sweep (the grid settles these columns without the proposal; they count as grid columns).
+    const int ie = ((mode == MODE_NEWTON || mode == MODE_LEARNED) && lam64 && G >= 2) ? idx[p] : -1;
+    if (ie == 0 || ie == G - 1) {
+      const bool up = ie == G - 1;
+      const int in = up ? G - 2 : 1;
+      const double de = exp(grid_value(ie, G, lo, hi)), dn = exp(grid_value(in, G, lo, hi));
+      double Re = 0.0, Rn = 0.0, R2e = 0.0, W1e = 0.0, rw = 0.0, m1 = 0.0, m2 = 0.0;
+#pragma unroll
+      for (int j = 0; j < KPL; ++j) {
+        const int k = lane + 64 * j;
+        const double y = (double)L.yr[j];
+        const double sk = sh_s[k];
+        Rn += y * y * rcp_pos(sk + dn);
+        const double we = rcp_pos(sk + de);
+        const double yw = y * y * we;
+        Re += yw;
+        R2e += yw * we;
+        if (k < nlam) W1e += we;
+        if (k < neff) {
+          const double q1 = we * y;
+          rw += q1 * y;
+          m1 = fmax(m1, fabs(q1));
+          m2 = fmax(m2, we);
+        }
+      }
+      Re = wave_sum(Re);
+      Rn = wave_sum(Rn);
+      const double le = node_loss_from_R(L, Re, ie, lam64), ln = node_loss_from_R(L, Rn, in, lam64);
+      if (up ? !(ln <= le) : !(ln < le)) {
+        R2e = wave_sum(R2e);
+        W1e = wave_sum(W1e);
+        const double g = Re <= FLOOR64 ? de * W1e : de * (W1e - nco * R2e / Re);
+        if (up ? g <= 0.0 : g >= 0.0) {
+          rw = wave_sum(rw);
+          double sc1 = 1.0, sc2 = 1.0;
+          if constexpr (PREC == PREC_SPLIT_FP16) {
+            sc1 = pow2_scale(wave_max(m1));
+            sc2 = pow2_scale(wave_max(m2));
+          }
+          if (lane == 0) {
+            if (flags) flags[p] = 1;
+            log_delta[p] = fmin(fmax(up ? hi : lo, lo), hi);
+            rWr[p] = fmax(rw, FLOOR64);
+            h2[p] = 1.0 / (1.0 + de);
+            if constexpr (PREC == PREC_SPLIT_FP16) {
+              q1s[p] = (float)sc1;
+              if (q2s) q2s[p] = (float)sc2;
+            }
+          }
+          return;
+        }
+      }
+    }
+  }
   double t;
   int i = (mode == MODE_FIXED || mode == MODE_GRID) ? 0 : idx[p];
   // every search brackets the FP64 grid argmin, as the reference does (the FP32 grid's pick differs only where
   // neighbouring grid losses tie to FP32 precision, i.e. on flat likelihoods)
+  double t_vertex = NAN;
   if (mode != MODE_FIXED && mode != MODE_GRID)
-    i = fp64_grid_argmin<ColLik<KPL, SAME>, (KPL > 0)>(L, p, i, G, lo, hi, lam64, lane);
+    i = fp64_grid_argmin<ColLik<KPL, SAME>, (KPL > 0)>(L, p, i, G, lo, hi, lam64, lane, t_vertex);
   bool need_bracket = NEWTON_ONLY || mode == MODE_NEWTON;
   if constexpr (NEWTON_ONLY) {
     t = 0.0;
@@ -570,7 +653,7 @@ __device__ __forceinline__ void delta_refine_body(
       t = 0.0;
     }
   }
-  if (need_bracket) t = newton_bracket<ColLik<KPL, SAME>, (KPL > 0)>(L, p, i, G, lo, hi, lane);
+  if (need_bracket) t = newton_bracket<ColLik<KPL, SAME>, (KPL > 0)>(L, p, i, G, lo, hi, lane, t_vertex);
   if (mode != MODE_FIXED) t = fmin(fmax(t, lo), hi);
   const double delta = exp(t);
   // pass 1: the null residual rWr and (split-FP16) the column maxima that set the planes' scales

@@ -94,8 +94,10 @@ def test_training_learns_and_refinement_is_judged():
     h2 = torch.from_numpy(rng.uniform(0, 0.95, size=P)).cuda()
     uy = simulate_columns(s, h2, gen)
     b = basis_from_columns(s, uy, engine.prec_code())
-    idx = engine.delta_grid_idx(b.uy, P, b.p_pad, b.kpad, s, s, float(neff), 100, -10.0, 10.0)
-    ref = engine.delta_refine(b.prec, 1, b.uy, P, b.p_pad, b.kpad, s, s, float(neff), 100, -10.0, 10.0, idx)
+    # the grid + Newton search exactly as the learned refinement's fallback runs it (with the FP64 log-det table)
+    idx, lam64 = engine.delta_grid(b.uy, P, b.p_pad, b.kpad, s, s, float(neff), 100, -10.0, 10.0)
+    ref = engine.delta_refine(b.prec, 1, b.uy, P, b.p_pad, b.kpad, s, s, float(neff), 100, -10.0, 10.0, idx,
+                              lam64=lam64)
     feat = features(b.uy, P, b.p_pad, s, neff + 1, 1)
     net = DeltaMLP(seed=0)
     hist = net.fit(feat, ref.log_delta.float(), epochs=8, batch=1024, lr=3e-3)
@@ -109,11 +111,12 @@ def test_training_learns_and_refinement_is_judged():
     assert fb.mean() < 0.5
 
 
-def test_trained_network_gain_over_uninformed_proposal():
-    """What the network gains over the method without it: with the grid kept as arbiter, a column is finished by
-    the short checked refinement only when its proposal is good enough; the trained network's proposals get far
-    more columns through than an uninformed proposal (the mean training label) does, and every accepted column
-    lands on the grid + Newton optimum."""
+def test_learned_proposal_against_grid_newton():
+    """The learned path against the method without the network (grid + Newton) on held-out columns: the same
+    optimum (every accepted column within 1e-6 of the grid + Newton log delta, every fallback column exactly it),
+    and the time of each delta search (grid, features, network and checked refinement against grid and Newton),
+    reported.  Numbers from the workload-size run are the ones quoted; this test pins the result and shows the
+    cost on a small problem."""
     from fasterlmm_amd import engine
     from fasterlmm_amd.learned import DeltaMLP, basis_from_columns, features, refine_learned, simulate_columns
     gen = torch.Generator(device="cuda")
@@ -131,25 +134,34 @@ def test_trained_network_gain_over_uninformed_proposal():
         feats.append(features(b.uy, b.P, b.p_pad, s, neff + 1, 1))
         labs.append(d.log_delta.float())
     net = DeltaMLP(seed=0)
-    lab = torch.cat(labs)
-    net.fit(torch.cat(feats), lab, epochs=10, batch=1024, lr=3e-3)
-    # held-out spectrum and columns
+    net.fit(torch.cat(feats), torch.cat(labs), epochs=10, batch=1024, lr=3e-3)
+    # held-out spectrum and columns (h2 over the whole range, so some columns sit at the grid's upper end)
     s = torch.from_numpy(np.sort(rng.gamma(1.1, 0.9, size=neff))).cuda()
-    h2 = torch.from_numpy(rng.uniform(0.05, 0.9, size=4096)).cuda()
+    h2 = torch.from_numpy(rng.uniform(0.0, 0.9, size=32768)).cuda()
     b = basis_from_columns(s, simulate_columns(s, h2, gen), engine.prec_code())
-    idx = engine.delta_grid_idx(b.uy, b.P, b.p_pad, b.kpad, s, s, float(neff), 100, -10.0, 10.0)
-    ref = engine.delta_refine(b.prec, 1, b.uy, b.P, b.p_pad, b.kpad, s, s, float(neff), 100, -10.0, 10.0, idx)
 
-    class Uninformed:
-        def predict(self, feat):
-            return torch.full((feat.shape[0],), float(lab.double().mean()), dtype=torch.float64, device=feat.device)
+    def grid_newton():
+        idx, lam64 = engine.delta_grid(b.uy, b.P, b.p_pad, b.kpad, s, s, float(neff), 100, -10.0, 10.0)
+        return engine.delta_refine(b.prec, 1, b.uy, b.P, b.p_pad, b.kpad, s, s, float(neff), 100, -10.0, 10.0, idx,
+                                   lam64=lam64)
 
-    fit_net = refine_learned(b, net, 100, -10.0, 10.0, n=neff + 1, c=1)
-    fit_unf = refine_learned(b, Uninformed(), 100, -10.0, 10.0, n=neff + 1, c=1)
-    acc_net = 1.0 - fit_net.fallback.double().mean().item()
-    acc_unf = 1.0 - fit_unf.fallback.double().mean().item()
-    assert acc_net >= acc_unf + 0.25, (acc_net, acc_unf)
-    assert acc_net >= 0.5, acc_net
-    for fit in (fit_net, fit_unf):
-        d = (fit.log_delta - ref.log_delta).abs().cpu().numpy()
-        assert d.max() <= 1e-6, d.max()
+    def timed(fn):
+        best, out = float("inf"), None
+        for _ in range(3):
+            a, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            out = fn()
+            e.record()
+            torch.cuda.synchronize()
+            best = min(best, a.elapsed_time(e))
+        return best, out
+
+    ms_n, ref = timed(grid_newton)
+    ms_l, fit = timed(lambda: refine_learned(b, net, 100, -10.0, 10.0, n=neff + 1, c=1))
+    d = (fit.log_delta - ref.log_delta).abs().cpu().numpy()
+    fb = fit.fallback.cpu().numpy()
+    assert np.all(d[fb == 1] <= 1e-12), d[fb == 1].max()
+    if (fb == 0).any():
+        assert d[fb == 0].max() <= 1e-6, d[fb == 0].max()
+    print(f"delta search on {b.P} columns: grid + Newton {ms_n:.3f} ms, learned {ms_l:.3f} ms "
+          f"({ms_l / ms_n:.2f}x), proposals accepted {1.0 - fb.mean():.3f}", flush=True)
