@@ -649,7 +649,26 @@ def main():
         line["uneven_na_groups"] = run_grouped(sub, emit=False)
         torch.cuda.empty_cache()
         line["low_rank_cohort"] = run_lowrank(argparse.Namespace(**{**vars(sub), "warmup": 0}), emit=False)
+        torch.cuda.empty_cache()
+        line["learned_delta"] = run_learned()
     print(json.dumps(line), flush=True)
+
+
+def run_learned():
+    """The learned delta proposal at its evaluation workload (N = 2,000 Balding-Nichols panel, 10,000 held-out
+    phenotypes x (1 + 100 permutations), the shipped network): fallback rate, log-likelihood gap and |d log10 p|
+    against the plain grid + Newton search, and both delta searches' times (learned.evaluate)."""
+    from fasterlmm_amd import learned
+    t0 = time.perf_counter()
+    info = learned.evaluate(learned.model(), n=2000, n_pheno=10_000, n_perm=100)
+    info["workload"] = ("learned delta proposal: N=2000, 20000 kinship + 1000 test SNPs, 10000 phenotypes "
+                        "(h2 ~ U(0, 0.95)) x (1+100 perms) = 1010000 columns, 100-point grid, shipped network")
+    info["delta_search_speedup_learned_vs_grid_newton"] = (info["seconds_delta_grid_newton"]
+                                                           / info["seconds_delta_learned"])
+    info["seconds_total"] = time.perf_counter() - t0
+    log(f"learned delta: fallback {info['fallback_rate']:.3f}, grid+Newton {info['seconds_delta_grid_newton']:.3f}s, "
+        f"learned {info['seconds_delta_learned']:.3f}s ({info['seconds_total']:.1f}s total)")
+    return info
 
 
 if __name__ == "__main__":

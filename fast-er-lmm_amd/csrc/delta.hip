@@ -727,6 +727,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
 // consecutive k of each chunk, so for one chunk the block's UY read is one contiguous 4 KiB span (a 16-byte load
 // per lane) and its writes are contiguous 8-byte stores per plane (element (p, k) at
 // ((k / 32) p_pad + p) 32 + k % 32).
+
 template <int PREC>
 __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restrict__ uy, int64_t P, int64_t p_pad,
                                                            int64_t kpad, const double* __restrict__ s, int64_t neff,
@@ -762,7 +763,7 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
     if constexpr (is_split(PREC)) {
       u16 h[4], l[4];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) split2<PREC>(q1[e] * sc1, h[e], l[e]);
+      for (int e = 0; e < 4; ++e) split2f<PREC>((float)(q1[e] * sc1), h[e], l[e]);
       *(uint2*)((u16*)q1a + off) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
       *(uint2*)((u16*)q1b + off) = make_uint2(l[0] | ((uint32_t)l[1] << 16), l[2] | ((uint32_t)l[3] << 16));
       if (wq2) {

@@ -85,6 +85,20 @@ __device__ inline void split2(double x, u16& hi, u16& lo) {
   else split_bf16(x, hi, lo);
 }
 
+// split of an FP32 value into the plane pair: hi = the value rounded to the two-byte format, lo = the exactly
+// representable FP32 residual rounded the same way (hi + lo keeps ~22 bits of the value, as split2 does)
+template <int PREC>
+__device__ inline void split2f(float x, u16& hi, u16& lo) {
+  if constexpr (PREC == PREC_SPLIT_FP16) {
+    const _Float16 h = (_Float16)x;
+    hi = __builtin_bit_cast(u16, h);
+    lo = __builtin_bit_cast(u16, (_Float16)(x - (float)h));
+  } else {
+    hi = f32_to_bf16_rne(x);
+    lo = f32_to_bf16_rne(x - bf16_to_f32(hi));
+  }
+}
+
 // 1 / x for a positive normal x: the hardware reciprocal estimate refined by two Newton-Raphson steps
 // r <- r + r (1 - x r), each of which squares the relative error (far below FP64 rounding after two); a few
 // times cheaper than the IEEE division sequence, which also handles zeros, denormals and infinities
