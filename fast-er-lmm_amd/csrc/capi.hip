@@ -136,7 +136,7 @@ static double* scratch_ptr(hipStream_t stream, const char* who) {
 
 extern "C" {
 
-int flmm_abi_version(void) { return 8; }
+int flmm_abi_version(void) { return 9; }
 
 const char* flmm_last_error(void) { return g_err.c_str(); }
 
@@ -518,6 +518,70 @@ int flmm_wald_scan_lowrank_noalloc(int prec, const void* s1_hi, const void* s1_l
                               beta, se, sfve, nullh2, pm, stream);
   if (rc != FLMM_OK) return rc;
   return check_launch(launch_max_reduce(pm, n_mt, p_pad, P, max_F, st), who);
+}
+
+// ---- certified screen of the permutation columns (screen.hip) ----
+// Error constants of the bound stated in screen.hip.  u': fp16 rounding of an FP32-rounded value; gamma(n): FP32
+// accumulation of n terms; SUBNORMAL_EPS: the largest absolute error of an fp16 operand below the normal range
+// (2^-25: the conversions and v_mfma_f32_32x32x16_f16 keep fp16 subnormals on gfx950; 2^-14 would cover flushing);
+// EXPSUM_REL: the factored weights' relative accuracy (1e-9 target, expsum.py); ETA: the subnormal share allowed in
+// the second product's relative bound, which sets dmin; SLACK: the FP32 evaluation of x = A^2 g / D (a few ulp).
+static constexpr double SC_U = 1.0 / 2048.0 + 1.0 / 16777216.0;
+static constexpr double SUBNORMAL_EPS = 1.0 / 33554432.0;
+static constexpr double EXPSUM_REL = 1e-8;
+static constexpr double SC_ETA = 1.0 / 1048576.0;
+static constexpr double SC_SLACK = 1.0 / 524288.0;
+static double sc_gamma(double n) { return n / 16777216.0 / (1.0 - n / 16777216.0); }
+
+int flmm_wald_scan_screened(const void* s1_hi, const void* s2_hi, const float* s1_scale, const float* s2_scale,
+                            int64_t mc_pad, const void* q1_hi, const void* q2_hi, const float* q1_scale,
+                            const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t kpad2, int64_t mc,
+                            int64_t p_first, int64_t P, const double* us, int64_t ld_us, const float* uy,
+                            const double* s, int64_t neff, double s_lo, double s_hi, const double* log_delta,
+                            const double* rWr, double nu, double* max_F, void* rec, unsigned long long* n_eval,
+                            void* stream) {
+  const char* who = "flmm_wald_scan_screened";
+  REQUIRE(mc >= 0 && mc <= mc_pad && mc_pad % TILE == 0 && mc <= 64 * TILE, "%s: bad SNP extent (mc <= 8192)", who);
+  REQUIRE(P >= 0 && P <= p_pad && p_pad % 256 == 0 && p_first >= 0 && p_first % 256 == 0 && p_first <= P,
+          "%s: bad column extent (p_pad and p_first multiples of 256, p_first <= P <= p_pad)", who);
+  REQUIRE(mc_pad <= (1LL << 24) && p_pad <= (1LL << 24), "%s: rows_pad above 2^24", who);
+  REQUIRE(kpad > 0 && kpad % TILE == 0 && kpad2 > 0 && kpad2 % TILE == 0 && kpad2 <= kpad,
+          "%s: need 0 < kpad2 <= kpad, both multiples of 128", who);
+  REQUIRE(neff >= 0 && neff <= kpad && ld_us >= kpad, "%s: need neff <= kpad <= ld_us", who);
+  REQUIRE(nu > 0.0 && s_lo <= s_hi, "%s: need nu > 0 and s_lo <= s_hi", who);
+  REQUIRE(s1_hi && s2_hi && q1_hi && q2_hi && s1_scale && s2_scale && q1_scale && q2_scale,
+          "%s: null plane or scale", who);
+  REQUIRE(aligned16(s1_hi) && aligned16(s2_hi) && aligned16(q1_hi) && aligned16(q2_hi),
+          "%s: operand planes must be 16-byte aligned", who);
+  REQUIRE(us && uy && s && log_delta && rWr && max_F && rec, "%s: null pointer", who);
+  if (mc == 0 || P == p_first) return FLMM_OK;
+  const int64_t Pl = P - p_first;
+  const double u = SC_U, K = (double)kpad, K2 = (double)kpad2, e = SUBNORMAL_EPS;
+  const double kA = 2 * u + u * u + sc_gamma(2 * K) * (1 + 2 * u);
+  // subnormal part of the first product, relative to s q sqrt(d rWr) >= 2^26 / R (rows scaled to (2^13, 2^14])
+  const double kArel = (e * 16384.0 * (sqrt(K) + K) * (1 + u) * (1 + u) + K * e * e) / 67108864.0;
+  const double kD = 2 * u + u * u + sc_gamma(2 * K2) * (1 + 2 * u) + EXPSUM_REL + SC_ETA + SC_SLACK;
+  // second product: subnormal error <= e (sum Ch + sum Bh) + K2 e^2 with both rows' entries <= 2^14 (sum <= K2 2^14
+  // covers the max-scaled exact-weight planes too); a row whose scaled D is below that / ETA is a forced candidate
+  const float dmin = (float)((e * 16384.0 * 2.0 * K2 * (1 + u) + K2 * e * e) / SC_ETA);
+  const hipStream_t st = (hipStream_t)stream;
+  ScreenArgs g{};
+  g.s1 = (const u16*)s1_hi;
+  g.s2 = (const u16*)s2_hi;
+  g.q1 = (const u16*)q1_hi + p_first * 32;
+  g.q2 = (const u16*)q2_hi + p_first * 32;
+  g.s1s = s1_scale;
+  g.s2s = s2_scale;
+  g.a_rows_pad = mc_pad;
+  g.b_rows_pad = p_pad;
+  g.nchunks = (int)(kpad / KCHUNK);
+  g.nchunks2 = (int)(kpad2 / KCHUNK);
+  uint64_t* r = (uint64_t*)rec;
+  int rc = check_launch(launch_wald_screen(g, mc, Pl, dmin, r, Pl, st), who);
+  if (rc != FLMM_OK) return rc;
+  return check_launch(launch_screen_resolve(r, Pl, mc, Pl, p_first, q1_scale, q2_scale, rWr, log_delta, us, ld_us, uy,
+                                            p_pad, s, neff, s_lo, s_hi, nu, kA, kArel, kD, max_F, n_eval, st),
+                      who);
 }
 
 int64_t flmm_scan_table_bytes(int n_groups) {

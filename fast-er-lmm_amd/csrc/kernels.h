@@ -71,6 +71,26 @@ template <int PREC>
 hipError_t launch_wald_scan_lowrank(const TileArgs<PREC, 2>& args, const ScanScales& sc, int64_t Mc, int64_t P,
                                     int64_t n_real, const double* rWr, const double* h2, double nu,
                                     const int64_t* row_index, const ScanOut& out, double* pmax, hipStream_t stream);
+// certified screen of the permutation columns (screen.hip)
+struct ScreenArgs {
+  const u16* s1;       // SNP-side hi planes: S1 = US over nchunks K-chunks, S2 over nchunks2, rows a_rows_pad
+  const u16* s2;
+  const u16* q1;       // column-side hi planes from the first screened column on (row stride b_rows_pad)
+  const u16* q2;
+  const float* s1s;    // the SNP rows' power-of-two scales
+  const float* s2s;
+  int64_t a_rows_pad;
+  int64_t b_rows_pad;
+  int nchunks;         // K / 32 of the first product (sWy)
+  int nchunks2;        // K / 32 of the second (sWs): < nchunks for factored weights, == nchunks for exact ones
+};
+hipError_t launch_wald_screen(const ScreenArgs& g, int64_t Mc, int64_t P, float dmin, uint64_t* rec, int64_t rec_ld,
+                              hipStream_t stream);
+hipError_t launch_screen_resolve(const uint64_t* rec, int64_t rec_ld, int64_t Mc, int64_t P, int64_t p_first,
+                                 const float* q1s, const float* q2s, const double* rWr, const double* log_delta,
+                                 const double* US, int64_t ld_us, const float* uy, int64_t p_pad, const double* s,
+                                 int64_t neff, double s_min, double s_max, double nu, double kA, double kArel,
+                                 double kD, double* max_F, unsigned long long* n_eval, hipStream_t stream);
 hipError_t launch_max_reduce(const double* pmax, int64_t n_mtiles, int64_t ld, int64_t P, double* max_F,
                              hipStream_t stream);
 hipError_t launch_standardise(int dtype, const void* src, int64_t n, int64_t m, int64_t lds, double* dst, int64_t ldd,

@@ -1,0 +1,392 @@
+// Certified screen of the permutation columns' Wald scan, and the exact resolution of each column's maximum F.
+//
+// Permutation columns feed only max_F, each column's largest Wald F over the SNPs (perm_threshold, reference
+// fasterlmm/perms.py:60-68 over the scan of core.py:640-667).  For one column, F of a test depends on the sums
+// a = sWy and d = sWs only through the weighted cosine c = |a| / sqrt(d rWr):  F = nu c^2 / (1 - c^2), increasing in
+// c.  The screen tile runs the scan's two products on the hi halves of the split-FP16 planes only (one MFMA where the
+// split takes three) and keeps, per column and 128-row tile, the largest and second-largest x = a^2 / d (up to the
+// column constant, c^2 = x q2 / (q1^2 rWr)) and the row of the largest.
+//
+// Error bound.  Each hi value is its operand (the FP64 value rounded once to FP32) rounded to fp16: relative error at
+// most u' = 2^-11 + 2^-24 in the normal range, and at most eps absolute below it (eps = 2^-25 when subnormals are
+// kept, 2^-14 when flushed).  With w_k = 1 / (s_k + delta), Cauchy-Schwarz in the weights gives
+// sum_k |us_k| w_k |uy_k| <= sqrt(d rWr), so for the FP64 values a, d of the test
+//   |a_hi - a| <= kA sqrt(d rWr),   kA = 2u' + u'^2 + gamma(2K) (1 + 2u') + kArel R   (R = sqrt(w_max / w_min))
+//   |d_hi - d| <= kD d,             kD = 2u' + u'^2 + gamma(2K2) (1 + 2u') + e_fac + eta + slack
+// where gamma(n) = n 2^-24 / (1 - n 2^-24) bounds the FP32 accumulation, e_fac the factored weights' accuracy, the
+// kArel R term the subnormal part of the first product (its operands' rows are scaled to norms in (2^13, 2^14]), and
+// eta the subnormal part of the second product, which holds for rows whose scaled d is at least dmin (rows below are
+// forced candidates).  Hence every test's exact c lies in
+//   [cL, cU] = [c sqrt(1 - kD) - kA,  c sqrt(1 + kD) + kA sqrt((1 + kD) / (1 - kD))],
+// both increasing in the screened c.  The resolve kernel takes per column L = the largest cL over the tiles and the
+// running maximum; a test whose cU is below L cannot hold the new maximum.  It re-evaluates in FP64 (FP64 rotated
+// SNPs, the rotated column, exact weights) the best row of every tile whose best cU >= L and every row of a tile whose
+// second-best cU >= L (or that holds a forced row), and max_F is the maximum of the running value and those
+// evaluations: the maximum of FP64 evaluations of the only tests that can hold it, whatever the screen's rounding.
+#include "kernels.h"
+#include "scan_wide.h"
+
+namespace flmm {
+
+// ScreenArgs: kernels.h
+
+constexpr int SC_SLOTS = 4;                        // K-chunk products per pipeline stage
+constexpr int SC_SLOT_LDS = W_TM * 64;             // one slot: 128 rows x 32 fp16
+constexpr int SC_STAGE = SC_SLOTS * SC_SLOT_LDS;   // 32 KiB
+static_assert(2 * EP_WAVE <= SC_STAGE, "epilogue staging of two waves fits one stage buffer");
+
+struct ScASet {
+  u32x4 v[SC_SLOTS][2];      // SNP-side blocks in flight: [slot][1 KiB block]
+};
+struct ScBSet {
+  u32x4 v[SC_SLOTS][2][2];   // column-side fragments: [slot][column block][MFMA step]
+};
+
+__device__ inline int sc_swz(int row) { return (row >> 2) & 3; }
+
+// One 128 x 256 screen tile.  Stage j holds four K-chunk products: while j < nSA the first product's chunks
+// n2 + 4 j + i (i = 0..3), all into accumulator set 0; then both products' chunks 2 jb and 2 jb + 1 (jb = j - nSA)
+// as slots (S1 Q1, S2 Q2, S1 Q1, S2 Q2) into sets 0, 1, 0, 1.  A stage's operands are fetched into registers one
+// stage ahead, the SNP side then stored into the other LDS buffer.
+__device__ inline void screen_tile(const ScreenArgs& g, int64_t n_mtiles, int64_t n_ptiles, int64_t tile, int64_t Mc,
+                                   int64_t P, float dmin, uint64_t* __restrict__ rec, int64_t rec_ld) {
+  __shared__ __attribute__((aligned(16))) char lds0[SC_STAGE];
+  __shared__ __attribute__((aligned(16))) char lds1[SC_STAGE];
+  __shared__ __attribute__((aligned(16))) float row_g[W_TM];   // s2 / s1^2 of each tile row (0 past Mc)
+  __shared__ __attribute__((aligned(16))) float row_dm[W_TM];  // dmin (a negative value past Mc)
+
+  const int64_t group = tile / (W_GM * n_ptiles);
+  const int64_t first_m = group * W_GM;
+  const int64_t gm = n_mtiles - first_m < W_GM ? n_mtiles - first_m : W_GM;
+  const int64_t r_in = tile - group * (W_GM * n_ptiles);
+  const int64_t mt = first_m + r_in % gm;
+  const int64_t pt = r_in / gm;
+  const int64_t m0 = mt * W_TM, p0 = pt * W_TP;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int h = lane >> 5;
+
+  float e_g = 0.f, e_dm = -1.f;
+  if (threadIdx.x < W_TM) {
+    const int64_t m = m0 + threadIdx.x;
+    if (m < Mc) {
+      const double sa = gmem(g.s1s)[m];
+      e_g = (float)((double)gmem(g.s2s)[m] / (sa * sa));
+      e_dm = dmin;
+    }
+  }
+
+  const int n2 = g.nchunks2;
+  const int nSA = (g.nchunks - n2) / 4;
+  const int nst = nSA + n2 / 2;
+  const int64_t a_kstride = g.a_rows_pad * 64;
+  const int64_t b_kstride = g.b_rows_pad * 64;
+  // SNP side: wave w copies 1 KiB blocks 2w and 2w + 1 (16 rows each) of every slot; lane L: row 16 b + L / 4,
+  // 16-byte segment L % 4, stored at segment (L % 4) ^ swz(row)
+  // (32-bit lane offsets from uniform chunk bases, so every load takes the scalar-base + 32-bit-offset form; the
+  // host bounds rows_pad * 64 bytes below 2^31)
+  const char* const a_base = (const char*)g.s1 + m0 * 64;
+  const char* const a_base2 = (const char*)g.s2 + m0 * 64;
+  const char* const b_base = (const char*)g.q1 + p0 * 64;
+  const char* const b_base2 = (const char*)g.q2 + p0 * 64;
+  const uint32_t a_lane = wave * 2048 + lane * 16;
+  const int a_row0 = wave * 32 + lane / 4;
+  // column side: lane (column p0 + 64 wave + 32 bj + lane % 32, half h) reads bytes [32 h, 32 h + 32) of its row
+  const uint32_t b_lane = (wave * 64 + (lane & 31)) * 64 + h * 32;
+
+  // uniform chunk base of slot i of stage j, SNP side (a) and column side (b)
+  auto slot_base = [&](int j, int i, bool a_side) -> const char* {
+    const bool A = j < nSA;
+    const int ck = A ? n2 + 4 * j + i : 2 * (j - nSA) + (i >> 1);
+    const bool second = !A && (i & 1);
+    return a_side ? (second ? a_base2 : a_base) + (int64_t)ck * a_kstride
+                  : (second ? b_base2 : b_base) + (int64_t)ck * b_kstride;
+  };
+  auto load_a = [&](int j) {
+    ScASet a;
+#pragma unroll
+    for (int i = 0; i < SC_SLOTS; ++i) {
+      const char* sa = slot_base(j, i, true);
+      a.v[i][0] = *gmem((const u32x4*)(sa + a_lane));
+      a.v[i][1] = *gmem((const u32x4*)(sa + (a_lane + 1024)));
+    }
+    return a;
+  };
+  auto load_b = [&](int j) {
+    ScBSet b;
+#pragma unroll
+    for (int i = 0; i < SC_SLOTS; ++i) {
+      const char* sb = slot_base(j, i, false);
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) b.v[i][bj][q] = *gmem((const u32x4*)(sb + (b_lane + (bj * 32 * 64 + q * 16))));
+    }
+    return b;
+  };
+  auto store = [&](const ScASet& a, char* st) {
+#pragma unroll
+    for (int i = 0; i < SC_SLOTS; ++i)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int r = a_row0 + b * 16;
+        *(u32x4*)(st + i * SC_SLOT_LDS + r * 64 + (((lane & 3) ^ sc_swz(r)) * 16)) = a.v[i][b];
+      }
+  };
+
+  f32x16 acc[2][4][2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[s][i][j][v] = 0.f;
+
+  // `both` (std::true_type / std::false_type): a stage of both products (slot i into set i % 2) or of the first only
+  auto mma = [&](auto both, const char* st, const ScBSet& b) {
+    constexpr bool B2 = decltype(both)::value;
+#pragma unroll
+    for (int i = 0; i < SC_SLOTS; ++i)
+#pragma unroll
+      for (int step = 0; step < 2; ++step) {
+        const int seg = h * 2 + step;
+        bf16x8 af[4];
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi) {
+          const int r = bi * 32 + (lane & 31);
+          af[bi] = *(const bf16x8*)(st + i * SC_SLOT_LDS + r * 64 + ((seg ^ sc_swz(r)) * 16));
+        }
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+          const bf16x8 bh = __builtin_bit_cast(bf16x8, b.v[i][bj][step]);
+#pragma unroll
+          for (int bi = 0; bi < 4; ++bi)
+            acc[B2 ? (i & 1) : 0][bi][bj] = mfma_split<PREC_SPLIT_FP16>(af[bi], bh, acc[B2 ? (i & 1) : 0][bi][bj]);
+        }
+      }
+    // interleave (masks: 0x008 MFMA, 0x020 VMEM read, 0x100 DS read): the first slot-step's 4 fragment reads up
+    // front, each later slot-step's 4 reads and 3 of the stage's 24 global loads over the previous one's 8 MFMAs
+    __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+#pragma unroll
+    for (int ss = 0; ss < 2 * SC_SLOTS - 1; ++ss) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+      __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+  };
+  auto stage = [&](auto both, int j, const char* cur, char* nxt, const ScBSet& bc, ScBSet& bn) {
+    const int jn = j + 1 < nst ? j + 1 : j;  // past the last stage the prefetch repeats it (valid addresses)
+    const ScASet an = load_a(jn);
+    bn = load_b(jn);
+    mma(both, cur, bc);
+    __builtin_amdgcn_sched_barrier(0);
+    store(an, nxt);  // nxt was last read in the previous stage, which ended with a barrier
+    __syncthreads();
+  };
+  constexpr std::true_type both{};
+  constexpr std::false_type first{};
+
+  ScBSet b0 = load_b(0), b1;
+  store(load_a(0), lds0);
+  if (threadIdx.x < W_TM) {
+    row_g[threadIdx.x] = e_g;
+    row_dm[threadIdx.x] = e_dm;
+  }
+  __syncthreads();
+  // Stages in pairs so every access names a fixed buffer: phase A pairs, then (for an odd phase A) its last stage
+  // paired with the first phase-B stage, then phase-B pairs and a last single stage.  Straight-line phases: a phase
+  // branch inside one loop made the register allocator spill the accumulators.
+  int j = 0;
+  for (; j + 2 <= nSA; j += 2) {
+    stage(first, j, lds0, lds1, b0, b1);
+    stage(first, j + 1, lds1, lds0, b1, b0);
+  }
+  if (j < nSA) {  // nst > nSA: nchunks2 >= 4
+    stage(first, j, lds0, lds1, b0, b1);
+    stage(both, j + 1, lds1, lds0, b1, b0);
+    j += 2;
+  }
+  for (; j + 2 <= nst; j += 2) {
+    stage(both, j, lds0, lds1, b0, b1);
+    stage(both, j + 1, lds1, lds0, b1, b0);
+  }
+  if (j < nst) stage(both, j, lds0, lds1, b0, b1);
+
+  // ---- epilogue: each 32-row block of the wave's accumulators is staged through the free LDS column-major (the
+  // wide tile's swizzled layout); lane c then owns column p0 + 64 wave + c and walks the rows:
+  //   x = A^2 g_m / D  (A, D the raw sums; g_m = s2_m / s1_m^2, so x = q1^2 a^2 / (q2 d)),
+  //   x = +inf where the scaled D is below dmin (forced), the running largest and second-largest x and the row of the
+  //   largest.  Padding rows have A = D = 0 and g_m = 0, so x = 0.
+  float* ep = (float*)((wave < 2 ? lds0 : lds1) + (wave & 1) * EP_WAVE);
+  float m1 = 0.f, m2 = 0.f;
+  int bt = 0;
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const f32x16& c = acc[s][bi][bj];
+          const f32x4 w4 = {c[4 * q], c[4 * q + 1], c[4 * q + 2], c[4 * q + 3]};
+          *(f32x4*)(ep + ep_off(s * 64 + bj * 32 + (lane & 31), 2 * q + h)) = w4;
+        }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll 1
+    for (int t4 = 0; t4 < 8; ++t4) {  // not unrolled: unrolled, its LDS reads crowd the main loop's registers
+      const f32x4 a4 = *(const f32x4*)(ep + ep_off(lane, t4));
+      const f32x4 d4 = *(const f32x4*)(ep + ep_off(64 + lane, t4));
+      const f32x4 g4 = *(const f32x4*)&row_g[bi * 32 + 4 * t4];
+      const f32x4 n4 = *(const f32x4*)&row_dm[bi * 32 + 4 * t4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float D = d4[e];
+        float x = (a4[e] * a4[e]) * g4[e] * __builtin_amdgcn_rcpf(fmaxf(D, 1e-30f));
+        x = D < n4[e] ? __builtin_inff() : x;
+        m2 = fmaxf(m2, fminf(x, m1));
+        bt = x > m1 ? bi * 32 + 4 * t4 + e : bt;
+        m1 = fmaxf(m1, x);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  const int64_t pc = p0 + wave * 64 + lane;
+  if (pc < P) {
+    // the second-largest x rounded up to a multiple of 128 units in the last place, the row in the freed bits
+    const uint32_t x2 = ((__float_as_uint(m2) + 127u) & ~127u) | (uint32_t)bt;
+    gmem(rec)[mt * rec_ld + pc] = ((uint64_t)x2 << 32) | __float_as_uint(m1);
+  }
+}
+
+__global__ __launch_bounds__(256, 1) void wald_screen_kernel(ScreenArgs g, int64_t n_mtiles, int64_t n_ptiles,
+                                                             int64_t n_tiles, int64_t Mc, int64_t P, float dmin,
+                                                             uint64_t* __restrict__ rec, int64_t rec_ld) {
+  screen_tile(g, n_mtiles, n_ptiles, xcd_remap(blockIdx.x, n_tiles), Mc, P, dmin, rec, rec_ld);
+}
+
+__device__ inline uint64_t readlane64(uint64_t v, int j) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, j);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), j);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Resolution.  Thread i (screened column i, global column p_first + i) reads the column's n_mt tile records and
+// marks the tiles to re-evaluate (bit t of mask1: the tile's best row; of mask2: all its rows); then each wave walks
+// its lanes that marked any, evaluating the marked tests with all 64 lanes (k strided) in FP64 as the re-scoring
+// kernel does, and writes the column's max_F.
+__global__ __launch_bounds__(256) void screen_resolve_kernel(
+    const uint64_t* __restrict__ rec, int64_t rec_ld, int64_t n_mt, int64_t Mc, int64_t P, int64_t p_first,
+    const float* __restrict__ q1s, const float* __restrict__ q2s, const double* __restrict__ rWr,
+    const double* __restrict__ log_delta, const double* __restrict__ US, int64_t ld_us, const float* __restrict__ uy,
+    int64_t p_pad, const double* __restrict__ s, int64_t neff, double s_min, double s_max, double nu, double kA,
+    double kArel, double kD, double* __restrict__ max_F, unsigned long long* __restrict__ n_eval) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  uint64_t mask1 = 0, mask2 = 0;
+  if (i < P) {
+    const int64_t p = p_first + i;
+    const double q1 = gmem(q1s)[p], q2 = gmem(q2s)[p], r = gmem(rWr)[p];
+    const double cc = q2 / (q1 * q1 * r);
+    const double delta = exp(gmem(log_delta)[p]);
+    const double ka = kA + kArel * sqrt((s_max + delta) / (fmax(s_min, 0.0) + delta));
+    const double up = sqrt(1.0 + kD), dn = sqrt(1.0 - kD), ur = ka * sqrt((1.0 + kD) / (1.0 - kD));
+    const double F0 = gmem(max_F)[p];
+    double L = F0 > 0.0 ? sqrt(F0 / (nu + F0)) : 0.0;  // c of the running maximum (F = nu c^2 / (1 - c^2))
+    for (int64_t t = 0; t < n_mt; ++t) {
+      const float x1 = __uint_as_float((uint32_t)gmem(rec)[t * rec_ld + i]);
+      if (x1 < __builtin_inff()) L = fmax(L, sqrt((double)x1 * cc) * dn - ka);
+    }
+    for (int64_t t = 0; t < n_mt; ++t) {
+      const uint64_t v = gmem(rec)[t * rec_ld + i];
+      const float x1 = __uint_as_float((uint32_t)v), x2 = __uint_as_float((uint32_t)(v >> 32) & ~127u);
+      if (!(x1 < __builtin_inff()) || sqrt((double)x2 * cc) * up + ur >= L)
+        mask2 |= 1ull << t;
+      else if (sqrt((double)x1 * cc) * up + ur >= L)
+        mask1 |= 1ull << t;
+    }
+  }
+  uint64_t pend = __ballot((mask1 | mask2) != 0);
+  const int64_t wave_col0 = i - lane;
+  while (pend) {
+    const int j = __builtin_ctzll(pend);
+    pend &= pend - 1;
+    const int64_t ij = wave_col0 + j;
+    const int64_t p = p_first + ij;
+    const uint64_t k1 = readlane64(mask1, j), k2 = readlane64(mask2, j);
+    const double delta = exp(gmem(log_delta)[p]);
+    double fmx = -__builtin_inf();
+    unsigned long long cnt = 0;
+    auto eval = [&](int64_t m) {
+      const double* u = US + m * ld_us;
+      double a = 0.0, d = 0.0, rr = 0.0;
+      for (int64_t k = lane; k < neff; k += 64) {
+        const double w = 1.0 / (gmem(s)[k] + delta);
+        const double uk = gmem(u)[k];
+        const double yk = (double)gmem(uy)[((k >> 5) * p_pad + p) * 32 + (k & 31)];
+        a += uk * w * yk;
+        d += uk * uk * w;
+        rr += w * yk * yk;
+      }
+      a = wave_sum(a);
+      d = fmax(wave_sum(d), FLOOR64);
+      rr = fmax(wave_sum(rr), FLOOR64);
+      const double b = a / d;
+      const double rss = fmax(rr - a * a / d, FLOOR64);
+      fmx = fmax(fmx, b * b / (rss / nu / d));
+      ++cnt;
+    };
+    for (uint64_t b = k1; b; b &= b - 1) {
+      const int64_t t = __builtin_ctzll(b);
+      eval(t * W_TM + (int64_t)((gmem(rec)[t * rec_ld + ij] >> 32) & 127u));
+    }
+    for (uint64_t b = k2; b; b &= b - 1) {
+      const int64_t t = __builtin_ctzll(b);
+      const int64_t hi = (t + 1) * W_TM < Mc ? (t + 1) * W_TM : Mc;
+      for (int64_t m = t * W_TM; m < hi; ++m) eval(m);
+    }
+    if (lane == 0) {
+      gmem(max_F)[p] = fmax(gmem(max_F)[p], fmx);
+      if (n_eval) atomicAdd(n_eval, cnt);
+    }
+  }
+}
+
+hipError_t launch_wald_screen(const ScreenArgs& g, int64_t Mc, int64_t P, float dmin, uint64_t* rec, int64_t rec_ld,
+                              hipStream_t stream) {
+  const int64_t n_mtiles = (Mc + W_TM - 1) / W_TM;
+  const int64_t n_ptiles = (P + W_TP - 1) / W_TP;
+  const int64_t n_tiles = n_mtiles * n_ptiles;
+  hipLaunchKernelGGL(wald_screen_kernel, dim3((unsigned)n_tiles), dim3(256), 0, stream, g, n_mtiles, n_ptiles,
+                     n_tiles, Mc, P, dmin, rec, rec_ld);
+  return hipGetLastError();
+}
+
+hipError_t launch_screen_resolve(const uint64_t* rec, int64_t rec_ld, int64_t Mc, int64_t P, int64_t p_first,
+                                 const float* q1s, const float* q2s, const double* rWr, const double* log_delta,
+                                 const double* US, int64_t ld_us, const float* uy, int64_t p_pad, const double* s,
+                                 int64_t neff, double s_min, double s_max, double nu, double kA, double kArel,
+                                 double kD, double* max_F, unsigned long long* n_eval, hipStream_t stream) {
+  const int64_t n_mt = (Mc + W_TM - 1) / W_TM;
+  hipLaunchKernelGGL(screen_resolve_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, stream, rec, rec_ld,
+                     n_mt, Mc, P, p_first, q1s, q2s, rWr, log_delta, US, ld_us, uy, p_pad, s, neff, s_min, s_max, nu,
+                     kA, kArel, kD, max_F, n_eval);
+  return hipGetLastError();
+}
+
+}  // namespace flmm

@@ -66,6 +66,8 @@ SIGNATURES: dict[str, list] = {
     "flmm_mlp_dense_bwd": [_P, _P, _P, _P, _I64, _I, _I, _I, _P, _P, _P, _P, _P],
     "flmm_mlp_mse_grad": [_P, _P, _I64, _P, _P],
     "flmm_adam": [_P, _P, _P, _P, _I64, _D, _D, _D, _D, _I, _P],
+    "flmm_wald_scan_screened": [_P, _P, _P, _P, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _I64,
+                                _P, _P, _I64, _D, _D, _P, _P, _D, _P, _P, _P, _P],
     "flmm_wald_scan": [_I, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P,
                        _P, _D, _P, _P, _P, _P, _P, _P, _P, _P],
     "flmm_wald_scan_lowrank": [_I, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64,

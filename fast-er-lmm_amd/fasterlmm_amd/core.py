@@ -304,11 +304,17 @@ class _Basis:
         after the first block's scan is queued and before the host waits for it (the FP64 re-scoring's selection
         reads F back)."""
         Mc = row_index.numel()
+        screen = engine.screen_applies(self.prec) and self.P > n_real and self.neff > 0
         for b0 in range(0, Mc, block):
             ri = row_index[b0:b0 + block]
-            s1, s2 = self.snp_planes(Z, ri, d)
-            engine.wald_scan(s1, s2, d, ri.numel(), self.P, n_real, self.nu, ri, out)
-            del s1, s2
+            scr = None
+            if screen:
+                s1, s2, us = self.snp_planes(Z, ri, d, keep_us=True)
+                scr = engine.ScreenOperands(us, self.uy, self.s, self.s_range[0], self.s_range[1], self.neff)
+            else:
+                s1, s2 = self.snp_planes(Z, ri, d)
+            engine.wald_scan(s1, s2, d, ri.numel(), self.P, n_real, self.nu, ri, out, screen=scr)
+            del s1, s2, scr
             if after_launch is not None:
                 after_launch()
                 after_launch = None
@@ -316,17 +322,21 @@ class _Basis:
         if after_launch is not None:
             after_launch()
 
-    def snp_planes(self, Z: Tensor, row_index: Tensor,
-                   d: engine.DeltaResult | None = None) -> tuple[engine.Planes, engine.Planes]:
+    def snp_planes(self, Z: Tensor, row_index: Tensor, d: engine.DeltaResult | None = None, *,
+                   keep_us: bool = False):
         """Scan planes S1 = B^T s, S2 = (B^T s)^2 of the SNP columns Z[:, row_index] (core.py:628-631), or
-        S2 = (B^T s)^2 A for the factored weights of d (default: the last fit)."""
+        S2 = (B^T s)^2 A for the factored weights of d (default: the last fit); keep_us (FP64 SNP rotation only)
+        also returns the FP64 rotated SNPs US (rows_pad, kpad)."""
         d = self.delta if d is None else d
         fac = None if d is None else d.fac
         mc = row_index.numel()
         rot = engine.settings.snp_rotation
         if rot == "fp64":
             us = engine.rotate_snps_f64(Z, row_index, mc, self.mat, self.col0, self.neff, rup(mc, TILE), self.kpad)
-            return engine.snp_planes_from_f64(self.prec, us, mc, self.neff, fac)
+            s1, s2 = engine.snp_planes_from_f64(self.prec, us, mc, self.neff, fac)
+            return (s1, s2, us) if keep_us else (s1, s2)
+        if keep_us:
+            raise ValueError("keep_us needs the FP64 SNP rotation")
         if rot != "split":
             raise ValueError(f"unknown snp_rotation setting {rot!r}")
         spl = engine.pack(self.prec, Z, mc, self.N, 1, Z.shape[1], rup(mc, TILE), self.npad, row_index=row_index)

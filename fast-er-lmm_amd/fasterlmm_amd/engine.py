@@ -48,6 +48,9 @@ class Settings:
     snp_rotation: str = "fp64"     # SNP side US = U_eff^T S: "fp64" (FP64 MFMA kernel, then packed) or "split" (one
                                    # fused MFMA pass in the split precision: ~2^-22 of |S| in every direction, which
                                    # the scan weighs by 1 / (s_k + delta))
+    screen: str = "auto"           # permutation columns' max_F: "auto" = the certified hi-plane screen with FP64
+                                   # re-evaluation of the tests that can hold each maximum (split_fp16 with the
+                                   # FP64 SNP rotation), "off" = the full split scan for every column
     pipeline: bool = False         # LOCO: prepare chromosome c + 1 (basis, rotation, delta) on a second stream
                                    # while chromosome c scans (same bits; ~1% less step time on the N=1000,
                                    # M=100k workload, but the co-running kernels slow the scan kernel ~1.7x)
@@ -395,23 +398,27 @@ class _ScanTimer:
         self.enabled = enabled
         self.launches = []
 
-    def record(self, ev, tests: int, ref_pt: float, alg_pt: float, kernel_pt: float, R: int = 0) -> None:
+    def record(self, ev, tests: int, ref_pt: float, alg_pt: float, kernel_pt: float, R: int = 0,
+               kind: str = "full") -> None:
         """One launch: its events, Wald tests, and flops per test of the reference algorithm (ref_pt), of the
-        launched algorithm (alg_pt) and over the padded K the kernel multiplies (kernel_pt); R > 0 for factored
-        weights (the exponential sum's term count)."""
-        self.launches.append((ev[0], ev[1], tests, ref_pt, alg_pt, kernel_pt, R))
+        launched algorithm (alg_pt) and over the padded K the kernel multiplies (kernel_pt: per product term; the
+        split scan issues three such products, the screen one);
+        R > 0 for factored weights (the exponential sum's term count); kind: "full" (flmm_wald_scan*) or
+        "screen" (flmm_wald_scan_screened: screen + FP64 resolution)."""
+        self.launches.append((ev[0], ev[1], tests, ref_pt, alg_pt, kernel_pt, R, kind))
 
-    def summary(self) -> dict:
+    def summary(self, kind: str | None = None) -> dict:
         """Launch count, total / mean kernel ms, Wald tests and the three flop totals (ref_flops: the reference
         algorithm, 4 Neff per compat test; alg_flops: what the launches compute, 2 Neff + 2 R per test with
-        factored weights; kernel_flops: the same over the padded K)."""
+        factored weights; kernel_flops: the same over the padded K), over the launches of one kind or all."""
         torch.cuda.synchronize()
-        ms = [a.elapsed_time(b) for a, b, *_ in self.launches]
-        tot = lambda i: float(sum(r[2] * r[i] for r in self.launches))  # noqa: E731
+        rows = [r for r in self.launches if kind is None or r[7] == kind]
+        ms = [a.elapsed_time(b) for a, b, *_ in rows]
+        tot = lambda i: float(sum(r[2] * r[i] for r in rows))  # noqa: E731
         return {"launches": len(ms), "total_ms": float(sum(ms)), "mean_ms": float(np.mean(ms)) if ms else 0.0,
-                "tests": int(sum(r[2] for r in self.launches)), "ref_flops": tot(3), "alg_flops": tot(4),
-                "kernel_flops": tot(5), "factored": any(r[6] for r in self.launches),
-                "R": max((r[6] for r in self.launches), default=0)}
+                "tests": int(sum(r[2] for r in rows)), "ref_flops": tot(3), "alg_flops": tot(4),
+                "kernel_flops": tot(5), "factored": any(r[6] for r in rows),
+                "R": max((r[6] for r in rows), default=0)}
 
 
 scan_timer = _ScanTimer()
@@ -475,16 +482,67 @@ def refresh_real_max(out: ScanOutputs, n_real: int) -> None:
         out.max_F[:n_real] = out.f.max(dim=0).values
 
 
+@dataclass
+class ScreenOperands:
+    """What the certified screen's FP64 re-evaluation reads: the block's rotated SNPs us (>= mc rows, kpad) FP64,
+    the rotated columns' fp32 plane uy, the basis spectrum s (neff) and bounds s_lo <= s <= s_hi."""
+    us: torch.Tensor
+    uy: torch.Tensor
+    s: torch.Tensor
+    s_lo: float
+    s_hi: float
+    neff: int
+
+
+class _ScreenStats:
+    """Optional count of the screen's FP64 re-evaluations (a device counter, read by summary())."""
+
+    def __init__(self):
+        self.counter = None
+        self.tests = 0
+
+    def reset(self, enabled: bool = True, device=None):
+        self.counter = torch.zeros(1, dtype=torch.int64, device=device or "cuda") if enabled else None
+        self.tests = 0
+
+    def summary(self) -> dict:
+        n = int(self.counter.item()) if self.counter is not None else 0
+        return {"screened_tests": self.tests, "fp64_evaluations": n,
+                "fraction": n / self.tests if self.tests else 0.0}
+
+
+screen_stats = _ScreenStats()
+
+
+def screen_applies(prec: int) -> bool:
+    """Whether scans of this precision screen their permutation columns (settings.screen)."""
+    if settings.screen not in ("auto", "off"):
+        raise ValueError(f"unknown screen setting {settings.screen!r}")
+    return settings.screen == "auto" and prec == _lib.PREC_SPLIT_FP16 and settings.snp_rotation == "fp64"
+
+
 def wald_scan(s1: Planes, s2: Planes, d: DeltaResult, mc: int, P: int, n_real: int, nu: float,
-              row_index: torch.Tensor, out: ScanOutputs, pmax: torch.Tensor | None = None) -> None:
-    """Scan one SNP block against every column, update out (real-column rows and the running max_F)."""
+              row_index: torch.Tensor, out: ScanOutputs, pmax: torch.Tensor | None = None,
+              screen: ScreenOperands | None = None) -> None:
+    """Scan one SNP block against every column, update out (real-column rows and the running max_F).  With screen
+    operands (split-FP16), the column tiles past the real columns run the certified screen instead
+    (flmm_wald_scan_screened)."""
     if mc == 0 or P == 0:
         return
     mc_pad = s1.rows_pad
     p_pad = d.q1.rows_pad
     n_mt = (mc + TILE - 1) // TILE
-    if pmax is None or pmax.numel() < n_mt * p_pad:
-        pmax = torch.empty(n_mt * p_pad, dtype=torch.float64, device=s1.hi.device)
+    p_full = P
+    if screen is not None and s1.prec == _lib.PREC_SPLIT_FP16 and mc <= 64 * TILE:
+        p_full = min(P, rup(n_real, PTILE))
+    need = n_mt * p_pad
+    if pmax is None or pmax.numel() < need:
+        pmax = torch.empty(need, dtype=torch.float64, device=s1.hi.device)
+    if p_full < P:
+        _screened(s1, s2, d, mc, p_full, P, nu, out, pmax, screen)
+    if p_full == 0:
+        return
+    P = p_full
     s1h, s1l = s1.ptrs
     s2h, s2l = s2.ptrs
     q1h, q1l = d.q1.ptrs
@@ -512,3 +570,29 @@ def wald_scan(s1: Planes, s2: Planes, d: DeltaResult, mc: int, P: int, n_real: i
         k2, r2 = (d.fac.rpad, d.fac.R) if d.fac else (s1.kpad, d.neff)
         scan_timer.record(ev, mc * P, 4.0 * d.neff, 2.0 * (d.neff + r2), 2.0 * (s1.kpad + k2), d.fac.R if d.fac else 0)
     _lib.call("flmm_max_reduce", pmax.data_ptr(), n_mt, p_pad, P, out.max_F.data_ptr(), st)
+
+
+def _screened(s1: Planes, s2: Planes, d: DeltaResult, mc: int, p_first: int, P: int, nu: float, out: ScanOutputs,
+              rec: torch.Tensor, screen: ScreenOperands) -> None:
+    """flmm_wald_scan_screened over the columns p_first <= p < P of one SNP block (rec: >= ceil(mc/128) x
+    (P - p_first) 8-byte words of workspace)."""
+    kpad2 = d.fac.rpad if d.fac is not None else s1.kpad
+    st = _lib.stream()
+    ev = None
+    if scan_timer.enabled:
+        ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        ev[0].record()
+    us = screen.us
+    cnt = screen_stats.counter
+    _lib.call("flmm_wald_scan_screened", s1.hi.data_ptr(), s2.hi.data_ptr(), s1.scale_ptr, s2.scale_ptr, s1.rows_pad,
+              d.q1.hi.data_ptr(), d.q2.hi.data_ptr(), d.q1.scale_ptr, d.q2.scale_ptr, d.q1.rows_pad, s1.kpad, kpad2,
+              mc, p_first, P, us.data_ptr(), us.stride(0), screen.uy.data_ptr(), screen.s.data_ptr(), screen.neff,
+              float(screen.s_lo), float(screen.s_hi), d.log_delta.data_ptr(), d.rWr.data_ptr(), float(nu),
+              out.max_F.data_ptr(), rec.data_ptr(), _lib.ptr(cnt), st)
+    if cnt is not None:
+        screen_stats.tests += mc * (P - p_first)
+    if ev is not None:
+        ev[1].record()
+        k2, r2 = (d.fac.rpad, d.fac.R) if d.fac else (s1.kpad, d.neff)
+        scan_timer.record(ev, mc * (P - p_first), 4.0 * d.neff, 2.0 * (d.neff + r2), 2.0 * (s1.kpad + k2),
+                          d.fac.R if d.fac else 0, kind="screen")

@@ -301,6 +301,26 @@ int flmm_wald_scan_lowrank_noalloc(int prec, const void* s1_hi, const void* s1_l
                                    const double* h2, double nu, const int64_t* row_index, double* f, double* beta,
                                    double* se, double* sfve, double* nullh2, double* max_F, void* stream);
 
+/* Certified screen of the permutation columns' maximum F (perm_threshold's max_F, fasterlmm/perms.py:60-68, over
+ * the scan of core.py:640-667), split-FP16 planes only.  Replaces flmm_wald_scan_lowrank + flmm_max_reduce for the
+ * columns p_first <= p < P of one SNP block (p_first a multiple of 256, no real column among them): the two scan
+ * products run on the hi planes alone (one fp16 MFMA instead of the split's three), each test's sums carry a proven
+ * error bound, and only the tests whose bound can reach a column's maximum are re-evaluated in FP64 from
+ *   us     the block's rotated SNPs in FP64, (>= mc rows, ld_us >= kpad) row-major, row m = plane row m,
+ *   uy     the rotated columns' FP32 plane (p_pad x kpad, plane layout), s the basis spectrum (neff values, all
+ *          within [s_lo, s_hi]), log_delta and rWr per column,
+ * with the exact weights 1 / (s_k + delta); max_F[p] = max(max_F[p], those values).  The result is the maximum of
+ * FP64 evaluations of every test that can hold it.  S1 / Q1 hold kpad, S2 / Q2 kpad2 (== kpad: exact weights).
+ * rec: caller workspace of ceil(mc / 128) x (P - p_first) uint64; n_eval (may be NULL) accumulates the number of
+ * FP64 evaluations.  Needs mc <= 8192 and mc_pad, p_pad <= 2^24. */
+int flmm_wald_scan_screened(const void* s1_hi, const void* s2_hi, const float* s1_scale, const float* s2_scale,
+                            int64_t mc_pad, const void* q1_hi, const void* q2_hi, const float* q1_scale,
+                            const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t kpad2, int64_t mc,
+                            int64_t p_first, int64_t P, const double* us, int64_t ld_us, const float* uy,
+                            const double* s, int64_t neff, double s_lo, double s_hi, const double* log_delta,
+                            const double* rWr, double nu, double* max_F, void* rec, unsigned long long* n_eval,
+                            void* stream);
+
 /* Fused Wald scan of one SNP block (snp_wald_scan_compat's tile loop, core.py:636-667): sWy = S1.Q1, sWs = S2.Q2 on
  * MFMA, FP64 epilogue with dof nu; split-FP16 sums are divided by the planes' row scales.  Real columns
  * p < n_real write f, beta, se, sfve, nullh2 at [row_index[m] * n_real + p] of (M, n_real) row-major FP64 arrays
