@@ -136,7 +136,7 @@ static double* scratch_ptr(hipStream_t stream, const char* who) {
 
 extern "C" {
 
-int flmm_abi_version(void) { return 9; }
+int flmm_abi_version(void) { return 10; }
 
 const char* flmm_last_error(void) { return g_err.c_str(); }
 
@@ -268,7 +268,8 @@ int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_
                       int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo, double hi,
                       const int32_t* idx, const double* lam64, double* log_delta, double* rWr, double* h2,
                       void* q1_hi, void* q1_lo, void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale,
-                      void* stream) {
+                      int64_t hi_from, void* stream) {
+  REQUIRE(hi_from >= 0, "flmm_delta_refine: hi_from must be >= 0");
   REQUIRE(mode == FLMM_DELTA_GOLDEN || mode == FLMM_DELTA_NEWTON || mode == FLMM_DELTA_FIXED ||
               mode == FLMM_DELTA_GRID,
           "flmm_delta_refine: bad mode %d", mode);
@@ -279,7 +280,7 @@ int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_
   if (p_pad == 0) return FLMM_OK;
   return check_launch(launch_delta_refine(prec, mode, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco, n_grid, lo, hi,
                                           idx, lam64, log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, q1_scale,
-                                          q2_scale, 0.0, 0.0, nullptr, (hipStream_t)stream),
+                                          q2_scale, 0.0, 0.0, nullptr, hi_from, (hipStream_t)stream),
                       "flmm_delta_refine");
 }
 
@@ -352,8 +353,9 @@ int flmm_rescore_f64(const double* US, const double* UY, int64_t ld, const doubl
 
 int flmm_factored_weight_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
                                 int64_t R, const double* log_delta, void* q2_hi, void* q2_lo, float* q2_scale,
-                                void* stream) {
+                                int64_t hi_from, void* stream) {
   REQUIRE(valid_prec(prec), "flmm_factored_weight_planes: bad precision %d", prec);
+  REQUIRE(hi_from >= 0, "flmm_factored_weight_planes: hi_from must be >= 0");
   REQUIRE(P >= 0 && P <= p_pad && p_pad % TILE == 0, "flmm_factored_weight_planes: bad column extent");
   REQUIRE(rpad > 0 && rpad % TILE == 0 && R >= 1 && R <= rpad,
           "flmm_factored_weight_planes: need 1 <= R <= rpad, rpad a multiple of 128");
@@ -363,7 +365,7 @@ int flmm_factored_weight_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad
   REQUIRE(prec != FLMM_PREC_SPLIT_FP16 || q2_scale, "flmm_factored_weight_planes: split-FP16 needs q2_scale");
   if (p_pad == 0) return FLMM_OK;
   return check_launch(launch_factored_planes(prec, P, p_pad, rpad, t, lam, R, log_delta, q2_hi, q2_lo, q2_scale,
-                                             (hipStream_t)stream),
+                                             hi_from, (hipStream_t)stream),
                       "flmm_factored_weight_planes");
 }
 
@@ -371,15 +373,17 @@ int flmm_delta_refine_learned(int prec, const float* uy, int64_t P, int64_t p_pa
                               int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo,
                               double hi, const int32_t* idx, const double* lam64, double radius, double judge_tol,
                               double* log_delta, int32_t* fallback, double* rWr, double* h2, void* q1_hi,
-                              void* q1_lo, void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale, void* stream) {
+                              void* q1_lo, void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale,
+                              int64_t hi_from, void* stream) {
   REQUIRE(radius > 0.0 && judge_tol >= 0.0 && idx, "flmm_delta_refine_learned: bad radius / tol / idx");
+  REQUIRE(hi_from >= 0, "flmm_delta_refine_learned: hi_from must be >= 0");
   const int rc = check_refine("flmm_delta_refine_learned", prec, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco,
                               n_grid, log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, q1_scale, q2_scale);
   if (rc) return rc;
   if (p_pad == 0) return FLMM_OK;
   return check_launch(launch_delta_refine(prec, 3, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco, n_grid, lo, hi, idx,
                                           lam64, log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, q1_scale, q2_scale,
-                                          radius, judge_tol, fallback, (hipStream_t)stream),
+                                          radius, judge_tol, fallback, hi_from, (hipStream_t)stream),
                       "flmm_delta_refine_learned");
 }
 
@@ -521,12 +525,14 @@ int flmm_wald_scan_lowrank_noalloc(int prec, const void* s1_hi, const void* s1_l
 }
 
 // ---- certified screen of the permutation columns (screen.hip) ----
-// Error constants of the bound stated in screen.hip.  u': fp16 rounding of an FP32-rounded value; gamma(n): FP32
+// Error constants of the bound stated in screen.hip.  u': fp16 rounding of an FP32 value within 2^-21 (relative) of
+// the FP64 one (the SNP planes round once; the screened columns' hi planes come from FP32 arithmetic, ~5 rounding
+// units); gamma(n): FP32
 // accumulation of n terms; SUBNORMAL_EPS: the largest absolute error of an fp16 operand below the normal range
 // (2^-25: the conversions and v_mfma_f32_32x32x16_f16 keep fp16 subnormals on gfx950; 2^-14 would cover flushing);
 // EXPSUM_REL: the factored weights' relative accuracy (1e-9 target, expsum.py); ETA: the subnormal share allowed in
 // the second product's relative bound, which sets dmin; SLACK: the FP32 evaluation of x = A^2 g / D (a few ulp).
-static constexpr double SC_U = 1.0 / 2048.0 + 1.0 / 16777216.0;
+static constexpr double SC_U = 1.0 / 2048.0 + 1.0 / 2097152.0;
 static constexpr double SUBNORMAL_EPS = 1.0 / 33554432.0;
 static constexpr double EXPSUM_REL = 1e-8;
 static constexpr double SC_ETA = 1.0 / 1048576.0;

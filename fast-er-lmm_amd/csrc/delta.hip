@@ -723,6 +723,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
 // Q1 = w o y and Q2 = w, w = 1 / (s_k + exp(log_delta[p])), written as the scan's column-side planes (zero on the
 // padding k in [neff, kpad) and on the padding columns p in [P, p_pad)); split-FP16 rows are multiplied by the
 // scales the refinement chose.  q2a == NULL: Q2 is not written (factored weights, factored_planes_kernel).
+// Columns hi_from <= p < P (the certified screen's, split precisions) get the hi planes only, from FP32 arithmetic:
+// each value is then within ~5 FP32 rounding units of the FP64 one before its fp16 rounding, which the screen's
+// error bound covers.
 // A block owns 32 adjacent columns and walks the K chunks; thread (column tid / 8, quad tid % 8) handles 4
 // consecutive k of each chunk, so for one chunk the block's UY read is one contiguous 4 KiB span (a 16-byte load
 // per lane) and its writes are contiguous 8-byte stores per plane (element (p, k) at
@@ -734,7 +737,7 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
                                                            const double* __restrict__ log_delta,
                                                            const float* __restrict__ q1s,
                                                            const float* __restrict__ q2s, void* q1a, void* q1b,
-                                                           void* q2a, void* q2b) {
+                                                           void* q2a, void* q2b, int64_t hi_from) {
   const int64_t p = (int64_t)blockIdx.x * 32 + (threadIdx.x >> 3);  // p < p_pad (p_pad is a multiple of 32)
   const int j4 = (threadIdx.x & 7) * 4;
   const bool live = p < P;
@@ -745,6 +748,29 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
   }
   const bool wq2 = q2a != nullptr;
   const int64_t nch = kpad / 32;  // a multiple of 4 (kpad % 128 == 0)
+  if constexpr (is_split(PREC)) {
+    if (live && p >= hi_from) {
+      const float d32 = (float)delta, s1f = (float)sc1, s2f = (float)sc2;
+#pragma unroll 2
+      for (int64_t c = 0; c < nch; ++c) {
+        const int64_t k0 = c * 32 + j4;
+        const int64_t off = (c * p_pad + p) * 32 + j4;
+        const f32x4 y4 = *(const f32x4*)(uy + off);
+        u16 h[4], g[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          float w = 0.f;
+          if (k0 + e < neff) w = __builtin_amdgcn_rcpf((float)s[k0 + e] + d32);
+          h[e] = hi_of<PREC>(w * y4[e] * s1f);
+          g[e] = hi_of<PREC>(w * s2f);
+        }
+        *(uint2*)((u16*)q1a + off) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
+        if (wq2)
+          *(uint2*)((u16*)q2a + off) = make_uint2(g[0] | ((uint32_t)g[1] << 16), g[2] | ((uint32_t)g[3] << 16));
+      }
+      return;
+    }
+  }
 #pragma unroll 2
   for (int64_t c = 0; c < nch; ++c) {
     const int64_t k0 = c * 32 + j4;
@@ -787,13 +813,14 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
 // (rows p_pad, K = rpad; zero for r >= R and on the padding columns p >= P); split-FP16 rows are multiplied by the
 // power of two of their L2 norm (q2_scale[p]), which bounds every entry by 2^14.  A block owns 8 columns; thread
 // (column tid / 32, j = tid % 32) handles r = 32 c + j of every chunk c, so each chunk's 256 elements are
-// contiguous in the planes.  The norm is a fixed-order sum over the column's 32 lanes.
+// contiguous in the planes.  The norm is a fixed-order sum over the column's 32 lanes.  Columns hi_from <= p < P
+// (the certified screen's) get the hi plane only.
 template <int PREC>
 __global__ __launch_bounds__(256) void factored_planes_kernel(int64_t P, int64_t p_pad, int64_t rpad,
                                                               const double* __restrict__ t,
                                                               const double* __restrict__ lam, int64_t R,
                                                               const double* __restrict__ log_delta, void* qa,
-                                                              void* qb, float* __restrict__ qs) {
+                                                              void* qb, float* __restrict__ qs, int64_t hi_from) {
   const int64_t p = (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);  // p < p_pad (p_pad is a multiple of 8)
   const int j = threadIdx.x & 31;
   const bool live = p < P;
@@ -820,7 +847,7 @@ __global__ __launch_bounds__(256) void factored_planes_kernel(int64_t P, int64_t
       u16 hh, ll;
       split2<PREC>(v * sc, hh, ll);
       ((u16*)qa)[off] = hh;
-      ((u16*)qb)[off] = ll;
+      if (!live || p < hi_from) ((u16*)qb)[off] = ll;
     } else {
       ((float*)qa)[off] = (float)v;
     }
@@ -828,12 +855,12 @@ __global__ __launch_bounds__(256) void factored_planes_kernel(int64_t P, int64_t
 }
 
 hipError_t launch_factored_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
-                                  int64_t R, const double* log_delta, void* qa, void* qb, float* qs,
+                                  int64_t R, const double* log_delta, void* qa, void* qb, float* qs, int64_t hi_from,
                                   hipStream_t stream) {
   const dim3 grid((unsigned)(p_pad / 8));
 #define FLMM_FPL(PR)                                                                                                 \
   hipLaunchKernelGGL((factored_planes_kernel<PR>), grid, dim3(256), 0, stream, P, p_pad, rpad, t, lam, R, log_delta, \
-                     qa, qb, qs)
+                     qa, qb, qs, hi_from)
   if (prec == PREC_SPLIT_BF16) FLMM_FPL(PREC_SPLIT_BF16);
   else if (prec == PREC_SPLIT_FP16) FLMM_FPL(PREC_SPLIT_FP16);
   else FLMM_FPL(PREC_FP32);
@@ -868,7 +895,7 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
                                double lo, double hi, const int* idx, const double* lam64,
                                double* log_delta, double* rWr, double* h2, void* q1a, void* q1b, void* q2a,
                                void* q2b, float* q1s, float* q2s, double radius, double judge_tol, int* flags,
-                               hipStream_t stream) {
+                               int64_t hi_from, hipStream_t stream) {
   const dim3 grid((unsigned)((p_pad + 3) / 4));
   // columns that fit 16 elements per lane (neff, nlam, kpad <= 1024) keep them in registers
   const bool regs = neff <= 1024 && nlam <= 1024 && kpad <= 1024;
@@ -904,7 +931,7 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
   const dim3 pgrid((unsigned)(p_pad / 32));
 #define FLMM_PLANES(PR)                                                                                              \
   hipLaunchKernelGGL((delta_planes_kernel<PR>), pgrid, dim3(256), 0, stream, uy, P, p_pad, kpad, s, neff, log_delta, \
-                     q1s, q2s, q1a, q1b, q2a, q2b)
+                     q1s, q2s, q1a, q1b, q2a, q2b, hi_from)
   if (prec == PREC_SPLIT_BF16) FLMM_PLANES(PREC_SPLIT_BF16);
   else if (prec == PREC_SPLIT_FP16) FLMM_PLANES(PREC_SPLIT_FP16);
   else FLMM_PLANES(PREC_FP32);

@@ -85,6 +85,13 @@ __device__ inline void split2(double x, u16& hi, u16& lo) {
   else split_bf16(x, hi, lo);
 }
 
+// the hi half alone: an FP32 value rounded to the two-byte format
+template <int PREC>
+__device__ inline u16 hi_of(float x) {
+  if constexpr (PREC == PREC_SPLIT_FP16) return __builtin_bit_cast(u16, (_Float16)x);
+  else return f32_to_bf16_rne(x);
+}
+
 // split of an FP32 value into the plane pair: hi = the value rounded to the two-byte format, lo = the exactly
 // representable FP32 residual rounded the same way (hi + lo keeps ~22 bits of the value, as split2 does)
 template <int PREC>

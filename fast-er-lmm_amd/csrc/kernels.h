@@ -40,7 +40,7 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
                                double lo, double hi, const int* idx, const double* lam64, double* log_delta,
                                double* rWr, double* h2,
                                void* q1a, void* q1b, void* q2a, void* q2b, float* q1s, float* q2s, double radius,
-                               double judge_tol, int* flags, hipStream_t stream);
+                               double judge_tol, int* flags, int64_t hi_from, hipStream_t stream);
 hipError_t launch_gemm_f64(const double* x2, int64_t rows, int64_t kpad, const double* A, int64_t rpad, double* C,
                            hipStream_t stream);
 hipError_t launch_rescore(const double* US, const double* UY, int64_t ld, const double* s, int64_t neff,
@@ -57,7 +57,7 @@ hipError_t launch_rotate_f64(const double* z, int64_t ldz, const int64_t* col_in
 hipError_t launch_gram_f64(const double* Z, int64_t ldz, int64_t n, int64_t m, int n_split, double* partial,
                            double* out, int64_t ld, hipStream_t stream);
 hipError_t launch_factored_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
-                                  int64_t R, const double* log_delta, void* qa, void* qb, float* qs,
+                                  int64_t R, const double* log_delta, void* qa, void* qb, float* qs, int64_t hi_from,
                                   hipStream_t stream);
 template <int PREC>
 hipError_t launch_wald_scan(const TileArgs<PREC, 2>& args, const ScanScales& sc, int64_t Mc, int64_t P,

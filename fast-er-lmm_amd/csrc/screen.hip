@@ -7,9 +7,10 @@
 // split takes three) and keeps, per column and 128-row tile, the largest and second-largest x = a^2 / d (up to the
 // column constant, c^2 = x q2 / (q1^2 rWr)) and the row of the largest.
 //
-// Error bound.  Each hi value is its operand (the FP64 value rounded once to FP32) rounded to fp16: relative error at
-// most u' = 2^-11 + 2^-24 in the normal range, and at most eps absolute below it (eps = 2^-25 when subnormals are
-// kept, 2^-14 when flushed).  With w_k = 1 / (s_k + delta), Cauchy-Schwarz in the weights gives
+// Error bound.  Each hi value is its operand rounded to fp16 from an FP32 value within 2^-21 (relative) of the FP64
+// one (the SNP planes round the FP64 value once; the screened columns' hi planes come from FP32 arithmetic): relative
+// error at most u' = 2^-11 + 2^-21 in the normal range, and at most eps absolute below it (eps = 2^-25: gfx950 keeps
+// fp16 subnormals in the conversion and in v_mfma_f32_32x32x16_f16; 2^-14 would cover flushing).  With w_k = 1 / (s_k + delta), Cauchy-Schwarz in the weights gives
 // sum_k |us_k| w_k |uy_k| <= sqrt(d rWr), so for the FP64 values a, d of the test
 //   |a_hi - a| <= kA sqrt(d rWr),   kA = 2u' + u'^2 + gamma(2K) (1 + 2u') + kArel R   (R = sqrt(w_max / w_min))
 //   |d_hi - d| <= kD d,             kD = 2u' + u'^2 + gamma(2K2) (1 + 2u') + e_fac + eta + slack
@@ -287,9 +288,14 @@ __device__ inline uint64_t readlane64(uint64_t v, int j) {
 }
 
 // Resolution.  Thread i (screened column i, global column p_first + i) reads the column's n_mt tile records and
-// marks the tiles to re-evaluate (bit t of mask1: the tile's best row; of mask2: all its rows); then each wave walks
-// its lanes that marked any, evaluating the marked tests with all 64 lanes (k strided) in FP64 as the re-scoring
-// kernel does, and writes the column's max_F.
+// marks the tiles to re-evaluate (bit t of mask1: the tile's best row; of mask2: all its rows).  cL and cU are
+// increasing in x, so L comes from the largest best x and the marks from one threshold X on x (cU(x) >= L iff
+// x >= X), with no square root per tile.  Then each wave walks its lanes that marked any: the column's w_k y_k and
+// w_k are formed once (held in registers when neff <= 1,024), and each marked test is evaluated with all 64 lanes
+// (k strided) in FP64 as the re-scoring kernel does; the column's max_F is the maximum of the running value and
+// those evaluations.
+constexpr int RS_KREG = 16;  // k per lane held in registers: neff <= 64 * RS_KREG
+
 __global__ __launch_bounds__(256) void screen_resolve_kernel(
     const uint64_t* __restrict__ rec, int64_t rec_ld, int64_t n_mt, int64_t Mc, int64_t P, int64_t p_first,
     const float* __restrict__ q1s, const float* __restrict__ q2s, const double* __restrict__ rWr,
@@ -302,27 +308,32 @@ __global__ __launch_bounds__(256) void screen_resolve_kernel(
   if (i < P) {
     const int64_t p = p_first + i;
     const double q1 = gmem(q1s)[p], q2 = gmem(q2s)[p], r = gmem(rWr)[p];
-    const double cc = q2 / (q1 * q1 * r);
+    const double cc = q2 / (q1 * q1 * r);  // c^2 = x cc
     const double delta = exp(gmem(log_delta)[p]);
     const double ka = kA + kArel * sqrt((s_max + delta) / (fmax(s_min, 0.0) + delta));
     const double up = sqrt(1.0 + kD), dn = sqrt(1.0 - kD), ur = ka * sqrt((1.0 + kD) / (1.0 - kD));
     const double F0 = gmem(max_F)[p];
-    double L = F0 > 0.0 ? sqrt(F0 / (nu + F0)) : 0.0;  // c of the running maximum (F = nu c^2 / (1 - c^2))
+    float xm = 0.f;
     for (int64_t t = 0; t < n_mt; ++t) {
       const float x1 = __uint_as_float((uint32_t)gmem(rec)[t * rec_ld + i]);
-      if (x1 < __builtin_inff()) L = fmax(L, sqrt((double)x1 * cc) * dn - ka);
+      if (x1 < __builtin_inff()) xm = fmaxf(xm, x1);
     }
+    // L: the largest lower bound, c of the running maximum (F = nu c^2 / (1 - c^2)) or the best tile's cL
+    const double L = fmax(F0 > 0.0 ? sqrt(F0 / (nu + F0)) : 0.0, sqrt((double)xm * cc) * dn - ka);
+    const double th = (L - ur) / up;  // cU(x) >= L  <=>  sqrt(x cc) >= th
+    const double X = th > 0.0 ? th * th / cc : -1.0;
     for (int64_t t = 0; t < n_mt; ++t) {
       const uint64_t v = gmem(rec)[t * rec_ld + i];
       const float x1 = __uint_as_float((uint32_t)v), x2 = __uint_as_float((uint32_t)(v >> 32) & ~127u);
-      if (!(x1 < __builtin_inff()) || sqrt((double)x2 * cc) * up + ur >= L)
+      if (!(x1 < __builtin_inff()) || (double)x2 >= X)
         mask2 |= 1ull << t;
-      else if (sqrt((double)x1 * cc) * up + ur >= L)
+      else if ((double)x1 >= X)
         mask1 |= 1ull << t;
     }
   }
   uint64_t pend = __ballot((mask1 | mask2) != 0);
   const int64_t wave_col0 = i - lane;
+  const bool regs = neff <= 64 * RS_KREG;
   while (pend) {
     const int j = __builtin_ctzll(pend);
     pend &= pend - 1;
@@ -330,22 +341,50 @@ __global__ __launch_bounds__(256) void screen_resolve_kernel(
     const int64_t p = p_first + ij;
     const uint64_t k1 = readlane64(mask1, j), k2 = readlane64(mask2, j);
     const double delta = exp(gmem(log_delta)[p]);
+    auto y_at = [&](int64_t k) { return (double)gmem(uy)[((k >> 5) * p_pad + p) * 32 + (k & 31)]; };
+    // the column's factors: w y and w per k (registers), and rWr = sum w y^2
+    double wy[RS_KREG], ww[RS_KREG], rr = 0.0;
+#pragma unroll
+    for (int e = 0; e < RS_KREG; ++e) {
+      const int64_t k = lane + 64 * e;
+      wy[e] = 0.0;
+      ww[e] = 0.0;
+      if (regs && k < neff) {
+        const double w = rcp_pos(gmem(s)[k] + delta), y = y_at(k);
+        wy[e] = w * y;
+        ww[e] = w;
+        rr += wy[e] * y;
+      }
+    }
+    if (!regs) {
+      for (int64_t k = lane; k < neff; k += 64) {
+        const double w = rcp_pos(gmem(s)[k] + delta), y = y_at(k);
+        rr += w * y * y;
+      }
+    }
+    rr = fmax(wave_sum(rr), FLOOR64);
     double fmx = -__builtin_inf();
     unsigned long long cnt = 0;
     auto eval = [&](int64_t m) {
       const double* u = US + m * ld_us;
-      double a = 0.0, d = 0.0, rr = 0.0;
-      for (int64_t k = lane; k < neff; k += 64) {
-        const double w = 1.0 / (gmem(s)[k] + delta);
-        const double uk = gmem(u)[k];
-        const double yk = (double)gmem(uy)[((k >> 5) * p_pad + p) * 32 + (k & 31)];
-        a += uk * w * yk;
-        d += uk * uk * w;
-        rr += w * yk * yk;
+      double a = 0.0, d = 0.0;
+      if (regs) {
+#pragma unroll
+        for (int e = 0; e < RS_KREG; ++e) {
+          const int64_t k = lane + 64 * e;
+          const double uk = k < neff ? gmem(u)[k] : 0.0;
+          a += uk * wy[e];
+          d += uk * uk * ww[e];
+        }
+      } else {
+        for (int64_t k = lane; k < neff; k += 64) {
+          const double w = rcp_pos(gmem(s)[k] + delta), uk = gmem(u)[k];
+          a += uk * w * y_at(k);
+          d += uk * uk * w;
+        }
       }
       a = wave_sum(a);
       d = fmax(wave_sum(d), FLOOR64);
-      rr = fmax(wave_sum(rr), FLOOR64);
       const double b = a / d;
       const double rss = fmax(rr - a * a / d, FLOOR64);
       fmx = fmax(fmx, b * b / (rss / nu / d));

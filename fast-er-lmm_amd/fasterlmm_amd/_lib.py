@@ -52,15 +52,15 @@ SIGNATURES: dict[str, list] = {
     "flmm_delta_weights": [_P, _I64, _P, _I64, _I64, _I, _D, _D, _P, _P, _P, _P],
     "flmm_delta_grid": [_P, _P, _I, _D, _P, _I64, _I64, _I64, _P, _P],
     "flmm_delta_refine": [_I, _I, _P, _I64, _I64, _I64, _P, _I64, _P, _I64, _D, _I, _D, _D, _P, _P, _P, _P, _P,
-                          _P, _P, _P, _P, _P, _P, _P],
+                          _P, _P, _P, _P, _P, _P, _I64, _P],
     "flmm_gram": [_I, _P, _P, _P, _I64, _I64, _I64, _I, _P, _P, _I64, _P],
     "flmm_gram_f64": [_P, _I64, _I64, _I64, _I, _P, _P, _I64, _P],
     "flmm_rotate_f64": [_P, _I64, _P, _I64, _I64, _P, _I64, _I64, _P, _I64, _I64, _P],
     "flmm_gemm_f64": [_P, _I64, _I64, _P, _I64, _P, _P],
     "flmm_rescore_f64": [_P, _P, _I64, _P, _I64, _P, _P, _P, _P, _I64, _D, _P, _P, _P, _P, _I64, _P],
-    "flmm_factored_weight_planes": [_I, _I64, _I64, _I64, _P, _P, _I64, _P, _P, _P, _P, _P],
+    "flmm_factored_weight_planes": [_I, _I64, _I64, _I64, _P, _P, _I64, _P, _P, _P, _P, _I64, _P],
     "flmm_delta_refine_learned": [_I, _P, _I64, _I64, _I64, _P, _I64, _P, _I64, _D, _I, _D, _D, _P, _P, _D, _D, _P,
-                                  _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+                                  _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P],
     "flmm_mlp_features": [_P, _I64, _I64, _I64, _P, _D, _D, _P, _P],
     "flmm_mlp_dense_fwd": [_P, _I64, _I, _P, _P, _I, _I, _P, _P],
     "flmm_mlp_dense_bwd": [_P, _P, _P, _P, _I64, _I, _I, _I, _P, _P, _P, _P, _P],

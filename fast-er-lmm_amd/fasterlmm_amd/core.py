@@ -265,29 +265,34 @@ class _Basis:
             return None
         return engine.factored_weights(self.s, self.s_range, self.kpad, lo, hi)
 
-    def fit(self, n_grid: int, lo: float, hi: float, refine: bool, *, projected_loss: bool = True):
+    def fit(self, n_grid: int, lo: float, hi: float, refine: bool, *, projected_loss: bool = True,
+            n_real: int | None = None):
+        """Delta search of every column; n_real (the leading real columns), when given, lets the columns of the
+        certified screen get their hi planes only."""
         s_lam, nco = self._loss(projected_loss)
         fac = self.factored(lo, hi)  # every refined or grid log delta lies in [lo, hi]
+        hf = engine.screen_start(self.prec, n_real, self.P) if self.neff > 0 else None
         if refine and engine.settings.delta == "learned":
             from . import learned
             fit = learned.refine_learned(self, learned.model(), n_grid, lo, hi, n=self.N, c=self.N - self.neff,
-                                         projected_loss=projected_loss)
+                                         projected_loss=projected_loss, hi_from=hf)
             if fac is not None:
                 engine.attach_factored(self.prec, fit.delta, fac)
             self.last_fallback = fit.fallback
             return fit.log_delta, fit.delta
         if not refine:
             d = engine.delta_refine(self.prec, _lib.DELTA_GRID, self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam,
-                                    nco, n_grid, lo, hi, None, fac=fac)
+                                    nco, n_grid, lo, hi, None, fac=fac, hi_from=hf)
             self.delta = d
             return d.log_delta, d
         idx, lam64 = engine.delta_grid(self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam, nco, n_grid, lo, hi)
         d = engine.delta_refine(self.prec, engine.delta_code(), self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam,
-                                nco, n_grid, lo, hi, idx, fac=fac, lam64=lam64)
+                                nco, n_grid, lo, hi, idx, fac=fac, lam64=lam64, hi_from=hf)
         self.delta = d
         return d.log_delta, d
 
-    def delta_for(self, log_delta: Tensor) -> engine.DeltaResult:
+    def delta_for(self, log_delta: Tensor, n_real: int | None = None) -> engine.DeltaResult:
+        """The column planes for given log delta values (n_real: as in fit)."""
         d = self.delta
         if d is not None and d.log_delta.data_ptr() == log_delta.data_ptr() and d.q1.prec == self.prec:
             return d
@@ -295,8 +300,10 @@ class _Basis:
         if log_delta.numel() > 0:
             lo, hi = (float(v) for v in torch.aminmax(log_delta.double()))
             fac = self.factored(lo, hi)
+        hf = engine.screen_start(self.prec, n_real, self.P) if self.neff > 0 else None
         return engine.delta_refine(self.prec, _lib.DELTA_FIXED, self.uy, self.P, self.p_pad, self.kpad, self.s,
-                                   self.s, float(self.neff), 1, 0.0, 0.0, None, log_delta_in=log_delta, fac=fac)
+                                   self.s, float(self.neff), 1, 0.0, 0.0, None, log_delta_in=log_delta, fac=fac,
+                                   hi_from=hf)
 
     def scan(self, Z: Tensor, row_index: Tensor, n_real: int, out: engine.ScanOutputs, d: engine.DeltaResult,
              block: int = 8192, after_launch=None) -> None:
@@ -414,7 +421,7 @@ def snp_wald_scan_compat(spec: CompatSpectrum, log_delta: Tensor, S: Tensor, *, 
     P = st.P
     M = Sd.shape[1]
     n_real = P if n_real is None else min(int(n_real), P)
-    d = st.delta_for(ldd)
+    d = st.delta_for(ldd, n_real)
     out = _new_outputs(M, n_real, P, Sd.device)
     st.scan(Sd.contiguous(), torch.arange(M, device=Sd.device, dtype=torch.int64), n_real, out, d)
     engine.refresh_real_max(out, n_real)
@@ -479,7 +486,7 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
         with stage("rotate_columns"):
             st.use_column_planes(ypl, P, y_real)
         with stage("delta"):
-            _, d = st.fit(n_grid, -10.0, 10.0, True, projected_loss=(not plain) or reml)
+            _, d = st.fit(n_grid, -10.0, 10.0, True, projected_loss=(not plain) or reml, n_real=n_real)
         done = torch.cuda.Event()
         done.record()
         return st, d, done
@@ -697,7 +704,7 @@ def snp_wald_scan(spectrum: Spectrum, log_delta: Tensor, S_rot: Tensor, *, snp_c
     P = st.P
     M = Sd.shape[1]
     n_real = P if n_real is None else min(int(n_real), P)
-    d = st.delta_for(ldd)
+    d = st.delta_for(ldd, n_real)
     out = _new_outputs(M, n_real, P, Sd.device)
     st.scan(Sd.contiguous(), torch.arange(M, device=Sd.device, dtype=torch.int64), n_real, out, d)
     engine.refresh_real_max(out, n_real)

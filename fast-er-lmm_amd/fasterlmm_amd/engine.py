@@ -281,6 +281,7 @@ class DeltaResult:
     idx: torch.Tensor | None = None
     neff: int = 0
     fac: Factored | None = None
+    hi_from: int | None = None  # columns from here on have the hi planes only (the screen's); None: all have both
 
 
 def attach_factored(prec: int, d: DeltaResult, fac: Factored) -> None:
@@ -290,7 +291,7 @@ def attach_factored(prec: int, d: DeltaResult, fac: Factored) -> None:
     q2 = Planes(prec, p_pad, fac.rpad, d.q1.hi.device)
     q2h, q2l = q2.ptrs
     _lib.call("flmm_factored_weight_planes", prec, P, p_pad, fac.rpad, fac.t.data_ptr(), fac.lam.data_ptr(), fac.R,
-              d.log_delta.data_ptr(), q2h, q2l, q2.scale_ptr, _lib.stream())
+              d.log_delta.data_ptr(), q2h, q2l, q2.scale_ptr, _hi_from_arg(d.hi_from, p_pad), _lib.stream())
     d.q2, d.fac = q2, fac
 
 
@@ -360,12 +361,25 @@ def delta_grid_idx(uy: torch.Tensor, P: int, p_pad: int, kpad: int, s: torch.Ten
     return delta_grid(uy, P, p_pad, kpad, s, s_lam, nco, n_grid, lo, hi)[0]
 
 
+def _hi_from_arg(hi_from: int | None, p_pad: int) -> int:
+    return p_pad if hi_from is None else int(hi_from)
+
+
+def screen_start(prec: int, n_real: int | None, P: int) -> int | None:
+    """First column of the certified screen for a scan of P columns with n_real real ones (None: no screen)."""
+    if n_real is None or not screen_applies(prec):
+        return None
+    p0 = rup(n_real, PTILE)
+    return p0 if p0 < P else None
+
+
 def delta_refine(prec: int, mode: int, uy: torch.Tensor, P: int, p_pad: int, kpad: int, s: torch.Tensor,
                  s_lam: torch.Tensor, nco: float, n_grid: int, lo: float, hi: float, idx: torch.Tensor | None,
                  log_delta_in: torch.Tensor | None = None, fac: Factored | None = None,
-                 lam64: torch.Tensor | None = None) -> DeltaResult:
+                 lam64: torch.Tensor | None = None, hi_from: int | None = None) -> DeltaResult:
     """Refine (or, for DELTA_FIXED, take log_delta_in) and build the column-side scan planes (with fac, the
-    factored Q2 instead of the exact one); lam64: delta_grid's FP64 log-det table for the same grid."""
+    factored Q2 instead of the exact one); lam64: delta_grid's FP64 log-det table for the same grid; hi_from: the
+    screen's first column (screen_start), from which only the hi planes are written."""
     dev = uy.device
     log_delta = torch.empty(max(P, 1), dtype=torch.float64, device=dev)
     if log_delta_in is not None:
@@ -380,8 +394,8 @@ def delta_refine(prec: int, mode: int, uy: torch.Tensor, P: int, p_pad: int, kpa
               s_lam.data_ptr(), s_lam.numel(), float(nco), n_grid, lo, hi, _lib.ptr(idx), _lib.ptr(lam64),
               log_delta.data_ptr(),
               rWr.data_ptr(), h2.data_ptr(), q1h, q1l, q2h, q2l, q1.scale_ptr, q2.scale_ptr if q2 else None,
-              _lib.stream())
-    d = DeltaResult(log_delta[:P], rWr[:P], h2[:P], q1, q2, idx, int(s.numel()))
+              _hi_from_arg(hi_from, p_pad), _lib.stream())
+    d = DeltaResult(log_delta[:P], rWr[:P], h2[:P], q1, q2, idx, int(s.numel()), hi_from=hi_from)
     if fac is not None:
         attach_factored(prec, d, fac)
     return d
@@ -535,6 +549,9 @@ def wald_scan(s1: Planes, s2: Planes, d: DeltaResult, mc: int, P: int, n_real: i
     p_full = P
     if screen is not None and s1.prec == _lib.PREC_SPLIT_FP16 and mc <= 64 * TILE:
         p_full = min(P, rup(n_real, PTILE))
+    if d.hi_from is not None and d.hi_from < p_full:
+        raise ValueError(f"wald_scan: the column planes past column {d.hi_from} hold the hi halves only (the screen's), "
+                         f"but columns up to {p_full} need the full split")
     need = n_mt * p_pad
     if pmax is None or pmax.numel() < need:
         pmax = torch.empty(need, dtype=torch.float64, device=s1.hi.device)

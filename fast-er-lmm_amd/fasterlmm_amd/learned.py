@@ -168,8 +168,10 @@ class LearnedFit:
 
 
 def refine_learned(basis, net: DeltaMLP, n_grid: int, lo: float, hi: float, *, n: int, c: int,
-                   radius: float = 1.0, judge_tol: float = 1e-6, projected_loss: bool = True) -> LearnedFit:
-    """Learned delta for every column of a core._Basis: grid (arbiter) + MLP proposal + checked Newton."""
+                   radius: float = 1.0, judge_tol: float = 1e-6, projected_loss: bool = True,
+                   hi_from: int | None = None) -> LearnedFit:
+    """Learned delta for every column of a core._Basis: grid (arbiter) + MLP proposal + checked Newton; hi_from:
+    the certified screen's first column (engine.screen_start), whose planes get the hi halves only."""
     s_lam, nco = basis._loss(projected_loss)
     idx, lam64 = engine.delta_grid(basis.uy, basis.P, basis.p_pad, basis.kpad, basis.s, s_lam, nco, n_grid, lo, hi)
     feat = features(basis.uy, basis.P, basis.p_pad, basis.s, n, c)
@@ -189,8 +191,9 @@ def refine_learned(basis, net: DeltaMLP, n_grid: int, lo: float, hi: float, *, n
               basis.s.numel(), s_lam.data_ptr(), s_lam.numel(), float(nco), n_grid, float(lo), float(hi),
               idx.data_ptr(), lam64.data_ptr(), float(radius), float(judge_tol), log_delta.data_ptr(), fb.data_ptr(),
               rWr.data_ptr(),
-              h2.data_ptr(), q1h, q1l, q2h, q2l, q1.scale_ptr, q2.scale_ptr, _lib.stream())
-    d = engine.DeltaResult(log_delta[:P], rWr[:P], h2[:P], q1, q2, idx, int(basis.s.numel()))
+              h2.data_ptr(), q1h, q1l, q2h, q2l, q1.scale_ptr, q2.scale_ptr, p_pad if hi_from is None else hi_from,
+              _lib.stream())
+    d = engine.DeltaResult(log_delta[:P], rWr[:P], h2[:P], q1, q2, idx, int(basis.s.numel()), hi_from=hi_from)
     basis.delta = d
     return LearnedFit(log_delta[:P], fb[:P], d)
 

@@ -166,13 +166,15 @@ int flmm_delta_grid(const float* wplane, const float* lam, int n_grid, double nc
  * the first minimum of the FP64 loss, as refine=False does (core.py:570-573); idx may be NULL for both), then
  * write log_delta[p], rWr[p] = max(sum w y^2, 1e-300) (core.py:641-643), h2[p] = 1/(1+delta) (core.py:668-669)
  * and the scan's column-side planes Q1 = w o y (q1_hi, q1_lo) and Q2 = w (q2_hi, q2_lo), rows p_pad, with row
- * scales q1_scale / q2_scale (split-FP16); q2_hi == NULL skips Q2 (factored weights).  The loss is nco * (log 2pi + log(R / nco) + 1) + sum_j
- * log(s_lam_j + delta). */
+ * scales q1_scale / q2_scale (split-FP16); q2_hi == NULL skips Q2 (factored weights).  Columns hi_from <= p < P
+ * (those flmm_wald_scan_screened reads: a multiple of 256 at or past the real columns) get the hi planes only,
+ * evaluated in FP32; hi_from >= P writes both planes of every column.  The loss is nco * (log 2pi + log(R / nco) + 1)
+ * + sum_j log(s_lam_j + delta). */
 int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_pad, int64_t kpad, const double* s,
                       int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo, double hi,
                       const int32_t* idx, const double* lam64, double* log_delta, double* rWr, double* h2,
                       void* q1_hi, void* q1_lo, void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale,
-                      void* stream);
+                      int64_t hi_from, void* stream);
 
 /* Factored weights (no counterpart in the reference, whose sWs = w^T (US o US) runs over all Neff eigenvalues,
  * core.py:640-651).  With a positive exponential sum 1/z = sum_{r<R} lam_r^2 exp(-t_r z) (relative accuracy
@@ -182,7 +184,12 @@ int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_
  * so the scan's second product runs over rpad ~ 128 instead of Neff (flmm_wald_scan_lowrank with kpad2 = rpad;
  * S2 = planes of C, e.g. from FLMM_ROTATE_US_X2 and flmm_pack_planes).  This call writes B (delta_p =
  * exp(log_delta[p]), p < P) as the Q2 planes (rows p_pad, K = rpad, zero for r >= R and p >= P), split-FP16 rows
- * scaled by the power of two of their L2 norm (q2_scale). */
+ * scaled by the power of two of their L2 norm (q2_scale); columns hi_from <= p < P get the hi plane only (as
+ * flmm_delta_refine). */
+int flmm_factored_weight_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
+                                int64_t R, const double* log_delta, void* q2_hi, void* q2_lo, float* q2_scale,
+                                int64_t hi_from, void* stream);
+
 /* Kinship Gram G = Z Z^T (n x n FP64, leading dimension ld) for Z (n x m) packed as operand planes with the strains
  * as rows (rows_pad x kpad, flmm_pack_planes; K = the SNPs): the product of io.grm's K = Z Z^T / M and of the LOCO
  * kernels (fasterlmm/io.py:257-265, core.py:708-716) on MFMA in the library's split precisions.  Upper-triangular
@@ -228,10 +235,6 @@ int flmm_rescore_f64(const double* US, const double* UY, int64_t ld, const doubl
                      int64_t n, double nu, double* f, double* beta, double* se, double* sfve, int64_t out_ld,
                      void* stream);
 
-int flmm_factored_weight_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
-                                int64_t R, const double* log_delta, void* q2_hi, void* q2_lo, float* q2_scale,
-                                void* stream);
-
 /* Learned-proposal refinement (no counterpart in the reference: the learned delta proposal).  log_delta[p]
  * holds the proposal t0 on entry.  Newton on the loss slope inside [t0 - radius, t0 + radius]; the result is
  * accepted when it converges there, lies in the grid optimum's bracket [grid[idx[p] - 1], grid[idx[p] + 1]],
@@ -242,7 +245,8 @@ int flmm_delta_refine_learned(int prec, const float* uy, int64_t P, int64_t p_pa
                               int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo,
                               double hi, const int32_t* idx, const double* lam64, double radius, double judge_tol,
                               double* log_delta, int32_t* fallback, double* rWr, double* h2, void* q1_hi,
-                              void* q1_lo, void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale, void* stream);
+                              void* q1_lo, void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale,
+                              int64_t hi_from, void* stream);
 
 /* Spectral features of P rotated columns for the learned delta proposal (34 fp32 per column, row-major):
  * feature b < 32 = log(mean y^2 in eigenvalue bin b / mean y^2), bin_of_k[k] in [0, 32) for k < neff;
