@@ -50,6 +50,10 @@ def parse():
     ap.add_argument("--no-others", action="store_true",
                     help="config 2: skip the uneven-NA-group and low-rank cohort summaries that follow the headline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--pipeline", action="store_true",
+                    help="LOCO: prepare chromosome c + 1 on a second stream while chromosome c scans")
+    ap.add_argument("--screen", default="auto", choices=["auto", "off"],
+                    help="permutation columns: the certified screen (auto) or the full split scan (off)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB first (reference kernel for HBM counters)")
     return ap.parse_args()
@@ -93,9 +97,12 @@ def build_workload(cfg_id, n_perm, pheno, snps):
     return cfg, panel, Z, X, Y_all, B, n_perm
 
 
+STEP_SETTINGS: dict = {}  # engine settings of the timed steps (--pipeline, --screen)
+
+
 def run_step(Z, X, Y_all, chrom, B, precision):
     from fasterlmm_amd import core, engine
-    with engine.configure(precision=precision):
+    with engine.configure(precision=precision, **STEP_SETTINGS):
         return core.loco_scan_compat(Z, X, Y_all, chrom, n_real=B, n_grid=100)
 
 
@@ -548,6 +555,7 @@ def main():
         log(f"[bench] copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
         del src, dst
 
+    STEP_SETTINGS.update(pipeline=args.pipeline, screen=args.screen)
     cfg, panel, Z, X, Y_all, B, n_perm = build_workload(args.config, args.n_perm, args.pheno, args.snps)
     N, M = Z.shape
     P = Y_all.shape[1]

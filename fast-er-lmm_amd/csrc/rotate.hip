@@ -11,6 +11,9 @@
 // Split-FP16 operands arrive with per-row power-of-two scales (a_scale, b_scale) that the epilogue divides out
 // exactly.  Because B's rows are unit vectors, |x[i][j]| <= ||A_i|| < 2^14 / a_scale[i], so the S1 / S2 planes are
 // written with scales a_scale[i] and a_scale[i]^2 / 2^14 (out_scale1 / out_scale2).
+#include <stdlib.h>
+#include <string.h>
+
 #include "kernels.h"
 
 namespace flmm {
@@ -92,6 +95,132 @@ __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(3))) v
     }
 }
 
+// MODE_UY in the split precisions on a 256 x 256 tile: 512 threads = 8 waves, two per SIMD; wave (wi, wj) owns rows
+// 128 wi .. + 128 and columns 64 wj .. + 64 (4 x 2 blocks of 32 x 32, 128 accumulators), so a second wave per SIMD
+// fills the first one's waits.  Each 32-k chunk of both operands (4 planes x 256 rows x 64 B = 64 KiB) is staged in
+// LDS, double-buffered, one barrier per chunk; per MFMA this moves half the bytes of the 128 x 128 tile.  The products
+// of an output element are summed in the same order as in rotate_kernel (per chunk and step: hi hi, hi lo, lo hi),
+// so the results are bit-identical.
+constexpr int RW_T = 256;                          // tile rows and columns
+constexpr int RW_PLANE = RW_T * 64;                // bytes of one plane's chunk (256 rows x 32 two-byte values)
+constexpr int RW_STAGE = 4 * RW_PLANE;             // A hi, A lo, B hi, B lo: 64 KiB
+
+template <int PREC>
+__global__ __launch_bounds__(512, 2) void rotate_uy_wide_kernel(TileArgs<PREC, 1> args, int64_t n_jtiles,
+                                                                int64_t n_tiles, const float* __restrict__ a_scale,
+                                                                const float* __restrict__ b_scale,
+                                                                float* __restrict__ out) {
+  __shared__ __attribute__((aligned(16))) char lds[2 * RW_STAGE];
+  const int64_t tile = xcd_remap(blockIdx.x, n_tiles);
+  const int64_t jt = tile % n_jtiles;  // j fastest: the tiles sharing one A block run together on one XCD
+  const int64_t it = tile / n_jtiles;
+  const int64_t i0 = it * RW_T, j0 = jt * RW_T;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wi = wave >> 2, wj = wave & 3;
+  const int h = lane >> 5;
+  const int nch = args.nchunks;
+
+  // chunk loads: piece q = tid + 512 ps (ps = 0, 1) of a plane's chunk is row q / 4, 16-byte segment q % 4
+  const char* const pl_base[4] = {(const char*)args.a[0][0] + i0 * 64, (const char*)args.a[0][1] + i0 * 64,
+                                  (const char*)args.b[0][0] + j0 * 64, (const char*)args.b[0][1] + j0 * 64};
+  const int64_t kstride[4] = {args.a_rows_pad * 64, args.a_rows_pad * 64, args.b_rows_pad * 64,
+                              args.b_rows_pad * 64};
+  auto gload = [&](int kc, u32x4 (&f)[8]) {
+#pragma unroll
+    for (int pl = 0; pl < 4; ++pl)
+#pragma unroll
+      for (int ps = 0; ps < 2; ++ps)
+        f[pl * 2 + ps] = *gmem((const u32x4*)(pl_base[pl] + kc * kstride[pl] + (uint32_t)(tid + 512 * ps) * 16));
+  };
+  auto lstore = [&](const u32x4 (&f)[8], char* st) {
+#pragma unroll
+    for (int pl = 0; pl < 4; ++pl)
+#pragma unroll
+      for (int ps = 0; ps < 2; ++ps) {
+        const int q = tid + 512 * ps, r = q >> 2, sg = q & 3;
+        *(u32x4*)(st + pl * RW_PLANE + r * 64 + ((sg ^ ((r >> 2) & 3)) * 16)) = f[pl * 2 + ps];
+      }
+  };
+
+  f32x16 acc[4][2];
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[bi][bj][v] = 0.f;
+
+  auto mma = [&](const char* st) {
+#pragma unroll
+    for (int step = 0; step < 2; ++step) {
+      const int seg = h * 2 + step;
+      bf16x8 af[4][2], bf[2][2];
+#pragma unroll
+      for (int bi = 0; bi < 4; ++bi) {
+        const int r = wi * 128 + bi * 32 + (lane & 31);
+#pragma unroll
+        for (int sub = 0; sub < 2; ++sub)
+          af[bi][sub] = *(const bf16x8*)(st + sub * RW_PLANE + r * 64 + ((seg ^ ((r >> 2) & 3)) * 16));
+      }
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) {
+        const int r = wj * 64 + bj * 32 + (lane & 31);
+#pragma unroll
+        for (int sub = 0; sub < 2; ++sub)
+          bf[bj][sub] = *(const bf16x8*)(st + (2 + sub) * RW_PLANE + r * 64 + ((seg ^ ((r >> 2) & 3)) * 16));
+      }
+#pragma unroll
+      for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+          f32x16 c = acc[bi][bj];
+          c = mfma_split<PREC>(af[bi][0], bf[bj][0], c);
+          c = mfma_split<PREC>(af[bi][0], bf[bj][1], c);
+          c = mfma_split<PREC>(af[bi][1], bf[bj][0], c);
+          acc[bi][bj] = c;
+        }
+    }
+  };
+
+  u32x4 pre[8];
+  gload(0, pre);
+  lstore(pre, lds);
+  __syncthreads();
+  for (int kc = 0; kc < nch; ++kc) {
+    char* cur = lds + (kc & 1) * RW_STAGE;
+    char* nxt = lds + ((kc & 1) ^ 1) * RW_STAGE;
+    const bool more = kc + 1 < nch;
+    if (more) gload(kc + 1, pre);
+    mma(cur);
+    if (more) lstore(pre, nxt);  // nxt was last read in the previous chunk, which ended with a barrier
+    __syncthreads();
+  }
+
+  // epilogue: divide out the power-of-two row scales (split-FP16) and write the fp32 plane; lanes 0..31 of one store
+  // write 32 consecutive j of row i (128 contiguous bytes)
+  const int64_t rows_pad = args.a_rows_pad;
+  double inv_b[2] = {1.0, 1.0};
+  if constexpr (PREC == PREC_SPLIT_FP16) {
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) inv_b[bj] = 1.0 / (double)b_scale[j0 + wj * 64 + bj * 32 + (lane & 31)];
+  }
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int64_t i = i0 + wi * 128 + bi * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+      const double inv_a = PREC == PREC_SPLIT_FP16 ? 1.0 / (double)a_scale[i] : 1.0;
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) {
+        const int64_t j = j0 + wj * 64 + bj * 32 + (lane & 31);
+        double x = (double)acc[bi][bj][v];
+        if constexpr (PREC == PREC_SPLIT_FP16) x *= inv_a * inv_b[bj];
+        out[((j >> 5) * rows_pad + i) * 32 + (j & 31)] = (float)x;
+      }
+    }
+}
+
 template <int PREC>
 hipError_t launch_rotate(int mode, const TileArgs<PREC, 1>& args, const float* a_scale, const float* b_scale,
                          void* o0, void* o1, void* o2, void* o3, float* out_scale1, float* out_scale2,
@@ -100,6 +229,18 @@ hipError_t launch_rotate(int mode, const TileArgs<PREC, 1>& args, const float* a
   const int64_t n_jtiles = args.b_rows_pad / TILE;
   const int64_t n_tiles = n_itiles * n_jtiles;
   const int lds = tile_lds_bytes<PREC, 1, false>();
+  if constexpr (is_split(PREC)) {
+    // the 256 x 256 tile where both row counts allow it (FLMM_ROTATE_VARIANT=square keeps the 128 x 128 one)
+    const char* var = getenv("FLMM_ROTATE_VARIANT");
+    const bool square = var && strcmp(var, "square") == 0;
+    if (mode == MODE_UY && !square && args.a_rows_pad % RW_T == 0 && args.b_rows_pad % RW_T == 0) {
+      const int64_t nj = args.b_rows_pad / RW_T;
+      const int64_t nt = (args.a_rows_pad / RW_T) * nj;
+      hipLaunchKernelGGL((rotate_uy_wide_kernel<PREC>), dim3((unsigned)nt), dim3(512), 0, stream, args, nj, nt,
+                         a_scale, b_scale, (float*)o0);
+      return hipGetLastError();
+    }
+  }
   if (mode == MODE_UY)
     hipLaunchKernelGGL((rotate_kernel<PREC, MODE_UY>), dim3((unsigned)n_tiles), dim3(NTHREADS), lds, stream, args,
                        n_jtiles, n_tiles, a_scale, b_scale, o0, o1, o2, o3, out_scale1, out_scale2);

@@ -359,3 +359,30 @@ def test_split_gram_matches_fp64(precision):
         want = Z @ Z.T
         err = float(np.abs(G.cpu().numpy() - want).max() / np.abs(np.diag(want)).max())
         assert err <= tol, (n, m, err)
+
+
+@pytest.mark.parametrize("precision", ["split_fp16", "split_bf16"])
+def test_wide_uy_rotation_is_bit_identical(precision, monkeypatch):
+    """The 256 x 256, 8-wave column rotation (flmm_rotate UY mode where both row counts are multiples of 256) sums
+    each output's products in the 128 x 128 kernel's order: the same bits (FLMM_ROTATE_VARIANT=square selects the
+    latter), and the FP32 plane of the rotated columns matches numpy's FP64 product to the split precision."""
+    from fasterlmm_amd import engine
+    rng = np.random.default_rng(21)
+    prec = engine.prec_code(precision)
+    for P, N, K in ((512, 300, 256), (768, 1000, 512)):
+        Y = rng.standard_normal((N, P))
+        U = np.linalg.qr(rng.standard_normal((N, K)))[0]  # orthonormal basis columns
+        Yd, Ud = torch.from_numpy(Y).to(DEV), torch.from_numpy(np.ascontiguousarray(U)).to(DEV)
+        npad = -(-N // 32) * 32
+        a = engine.pack(prec, Yd, P, N, 1, P, P, npad)
+        b = engine.pack(prec, Ud, K, N, 1, K, K, npad)
+        monkeypatch.delenv("FLMM_ROTATE_VARIANT", raising=False)
+        wide = engine.rotate_uy(a, b)
+        monkeypatch.setenv("FLMM_ROTATE_VARIANT", "square")
+        square = engine.rotate_uy(a, b)
+        torch.cuda.synchronize()
+        assert torch.equal(wide.view(torch.int32), square.view(torch.int32)), (P, N, K)
+        got = engine.fp32_plane_to_dense(wide, P, K, P, K).cpu().numpy()
+        want = Y.T @ U
+        tol = (3e-5 if precision == "split_bf16" else 2e-6) * np.sqrt((Y ** 2).sum(0).max())
+        np.testing.assert_allclose(got, want, rtol=0, atol=tol)
