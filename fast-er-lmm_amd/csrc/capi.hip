@@ -136,7 +136,7 @@ static double* scratch_ptr(hipStream_t stream, const char* who) {
 
 extern "C" {
 
-int flmm_abi_version(void) { return 10; }
+int flmm_abi_version(void) { return 11; }
 
 const char* flmm_last_error(void) { return g_err.c_str(); }
 
@@ -539,14 +539,13 @@ static constexpr double SC_ETA = 1.0 / 1048576.0;
 static constexpr double SC_SLACK = 1.0 / 524288.0;
 static double sc_gamma(double n) { return n / 16777216.0 / (1.0 - n / 16777216.0); }
 
-int flmm_wald_scan_screened(const void* s1_hi, const void* s2_hi, const float* s1_scale, const float* s2_scale,
-                            int64_t mc_pad, const void* q1_hi, const void* q2_hi, const float* q1_scale,
-                            const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t kpad2, int64_t mc,
-                            int64_t p_first, int64_t P, const double* us, int64_t ld_us, const float* uy,
-                            const double* s, int64_t neff, double s_lo, double s_hi, const double* log_delta,
-                            const double* rWr, double nu, double* max_F, void* rec, unsigned long long* n_eval,
-                            void* stream) {
-  const char* who = "flmm_wald_scan_screened";
+// phases: 1 = the screen kernel, 2 = the resolution kernel
+static int screened(const char* who, int phases, const void* s1_hi, const void* s2_hi, const float* s1_scale,
+                    const float* s2_scale, int64_t mc_pad, const void* q1_hi, const void* q2_hi, const float* q1_scale,
+                    const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t kpad2, int64_t mc, int64_t p_first,
+                    int64_t P, const double* us, int64_t ld_us, const float* uy, const double* s, int64_t neff,
+                    double s_lo, double s_hi, const double* log_delta, const double* rWr, double nu, double* max_F,
+                    void* rec, unsigned long long* n_eval, void* stream) {
   REQUIRE(mc >= 0 && mc <= mc_pad && mc_pad % TILE == 0 && mc <= 64 * TILE, "%s: bad SNP extent (mc <= 8192)", who);
   REQUIRE(P >= 0 && P <= p_pad && p_pad % 256 == 0 && p_first >= 0 && p_first % 256 == 0 && p_first <= P,
           "%s: bad column extent (p_pad and p_first multiples of 256, p_first <= P <= p_pad)", who);
@@ -583,12 +582,34 @@ int flmm_wald_scan_screened(const void* s1_hi, const void* s2_hi, const float* s
   g.nchunks = (int)(kpad / KCHUNK);
   g.nchunks2 = (int)(kpad2 / KCHUNK);
   uint64_t* r = (uint64_t*)rec;
-  int rc = check_launch(launch_wald_screen(g, mc, Pl, dmin, r, Pl, st), who);
-  if (rc != FLMM_OK) return rc;
-  return check_launch(launch_screen_resolve(r, Pl, mc, Pl, p_first, q1_scale, q2_scale, rWr, log_delta, us, ld_us, uy,
-                                            p_pad, s, neff, s_lo, s_hi, nu, kA, kArel, kD, max_F, n_eval, st),
-                      who);
+  if (phases & 1) {
+    const int rc = check_launch(launch_wald_screen(g, mc, Pl, dmin, r, Pl, st), who);
+    if (rc != FLMM_OK) return rc;
+  }
+  if (phases & 2)
+    return check_launch(launch_screen_resolve(r, Pl, mc, Pl, p_first, q1_scale, q2_scale, rWr, log_delta, us, ld_us,
+                                              uy, p_pad, s, neff, s_lo, s_hi, nu, kA, kArel, kD, max_F, n_eval, st),
+                        who);
+  return FLMM_OK;
 }
+
+#define FLMM_SCREENED_ARGS                                                                                           \
+  s1_hi, s2_hi, s1_scale, s2_scale, mc_pad, q1_hi, q2_hi, q1_scale, q2_scale, p_pad, kpad, kpad2, mc, p_first, P, us, \
+      ld_us, uy, s, neff, s_lo, s_hi, log_delta, rWr, nu, max_F, rec, n_eval, stream
+#define FLMM_SCREENED_PARAMS                                                                                         \
+  const void *s1_hi, const void *s2_hi, const float *s1_scale, const float *s2_scale, int64_t mc_pad,                \
+      const void *q1_hi, const void *q2_hi, const float *q1_scale, const float *q2_scale, int64_t p_pad,             \
+      int64_t kpad, int64_t kpad2, int64_t mc, int64_t p_first, int64_t P, const double *us, int64_t ld_us,          \
+      const float *uy, const double *s, int64_t neff, double s_lo, double s_hi, const double *log_delta,             \
+      const double *rWr, double nu, double *max_F, void *rec, unsigned long long *n_eval, void *stream
+
+int flmm_wald_scan_screened(FLMM_SCREENED_PARAMS) {
+  return screened("flmm_wald_scan_screened", 3, FLMM_SCREENED_ARGS);
+}
+int flmm_wald_screen(FLMM_SCREENED_PARAMS) { return screened("flmm_wald_screen", 1, FLMM_SCREENED_ARGS); }
+int flmm_screen_resolve(FLMM_SCREENED_PARAMS) { return screened("flmm_screen_resolve", 2, FLMM_SCREENED_ARGS); }
+#undef FLMM_SCREENED_ARGS
+#undef FLMM_SCREENED_PARAMS
 
 int64_t flmm_scan_table_bytes(int n_groups) {
   if (n_groups < 0) return -1;

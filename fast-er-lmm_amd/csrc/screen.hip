@@ -54,7 +54,7 @@ __device__ inline void screen_tile(const ScreenArgs& g, int64_t n_mtiles, int64_
   __shared__ __attribute__((aligned(16))) char lds0[SC_STAGE];
   __shared__ __attribute__((aligned(16))) char lds1[SC_STAGE];
   __shared__ __attribute__((aligned(16))) float row_g[W_TM];   // s2 / s1^2 of each tile row (0 past Mc)
-  __shared__ __attribute__((aligned(16))) float row_dm[W_TM];  // dmin (a negative value past Mc)
+  __shared__ __attribute__((aligned(16))) uint32_t row_dm[W_TM];  // 0, or +inf bits past Mc (padding)
 
   const int64_t group = tile / (W_GM * n_ptiles);
   const int64_t first_m = group * W_GM;
@@ -67,13 +67,14 @@ __device__ inline void screen_tile(const ScreenArgs& g, int64_t n_mtiles, int64_
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int h = lane >> 5;
 
-  float e_g = 0.f, e_dm = -1.f;
+  float e_g = 0.f;
+  uint32_t e_dm = 0x7f800000u;
   if (threadIdx.x < W_TM) {
     const int64_t m = m0 + threadIdx.x;
     if (m < Mc) {
       const double sa = gmem(g.s1s)[m];
       e_g = (float)((double)gmem(g.s2s)[m] / (sa * sa));
-      e_dm = dmin;
+      e_dm = 0u;
     }
   }
 
@@ -230,12 +231,15 @@ __device__ inline void screen_tile(const ScreenArgs& g, int64_t n_mtiles, int64_
 
   // ---- epilogue: each 32-row block of the wave's accumulators is staged through the free LDS column-major (the
   // wide tile's swizzled layout); lane c then owns column p0 + 64 wave + c and walks the rows:
-  //   x = A^2 g_m / D  (A, D the raw sums; g_m = s2_m / s1_m^2, so x = q1^2 a^2 / (q2 d)),
-  //   x = +inf where the scaled D is below dmin (forced), the running largest and second-largest x and the row of the
-  //   largest.  Padding rows have A = D = 0 and g_m = 0, so x = 0.
+  //   x = A^2 g_m / max(D, dmin)  (A, D the raw sums; g_m = s2_m / s1_m^2, so x = q1^2 a^2 / (q2 d));
+  // x and D are non-negative, so their float bit patterns order as unsigned integers, and the running maxima are
+  // integer maxima (no NaN canonicalisation on the LDS values): m1 = the largest of (bits(x) & ~127) | row (the value
+  // rounded down to 128 units in the last place, the row in the freed bits), m2 = the second largest, and dlo =
+  // the smallest D of the tile's rows (padding rows count as +inf); a row with D < dmin makes the tile a forced
+  // one.  Padding rows have A = D = 0 and g_m = 0, so x = 0.
   float* ep = (float*)((wave < 2 ? lds0 : lds1) + (wave & 1) * EP_WAVE);
-  float m1 = 0.f, m2 = 0.f;
-  int bt = 0;
+  uint32_t m1 = 0u, m2 = 0u, dlo = 0x7f800000u;
+  const uint32_t dmin_b = __float_as_uint(dmin);
 #pragma unroll
   for (int bi = 0; bi < 4; ++bi) {
 #pragma unroll
@@ -252,26 +256,29 @@ __device__ inline void screen_tile(const ScreenArgs& g, int64_t n_mtiles, int64_
 #pragma unroll 1
     for (int t4 = 0; t4 < 8; ++t4) {  // not unrolled: unrolled, its LDS reads crowd the main loop's registers
       const f32x4 a4 = *(const f32x4*)(ep + ep_off(lane, t4));
-      const f32x4 d4 = *(const f32x4*)(ep + ep_off(64 + lane, t4));
+      const u32x4 d4 = *(const u32x4*)(ep + ep_off(64 + lane, t4));
       const f32x4 g4 = *(const f32x4*)&row_g[bi * 32 + 4 * t4];
-      const f32x4 n4 = *(const f32x4*)&row_dm[bi * 32 + 4 * t4];
+      const u32x4 p4 = *(const u32x4*)&row_dm[bi * 32 + 4 * t4];
+      const uint32_t row0 = bi * 32 + 4 * t4;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const float D = d4[e];
-        float x = (a4[e] * a4[e]) * g4[e] * __builtin_amdgcn_rcpf(fmaxf(D, 1e-30f));
-        x = D < n4[e] ? __builtin_inff() : x;
-        m2 = fmaxf(m2, fminf(x, m1));
-        bt = x > m1 ? bi * 32 + 4 * t4 + e : bt;
-        m1 = fmaxf(m1, x);
+        const uint32_t db = d4[e] > dmin_b ? d4[e] : dmin_b;
+        const float x = (a4[e] * a4[e]) * g4[e] * __builtin_amdgcn_rcpf(__uint_as_float(db));
+        const uint32_t xp = (__float_as_uint(x) & ~127u) | (row0 + e);
+        const uint32_t lo = xp < m1 ? xp : m1;
+        m2 = lo > m2 ? lo : m2;
+        m1 = xp > m1 ? xp : m1;
+        const uint32_t dp = d4[e] | p4[e];
+        dlo = dp < dlo ? dp : dlo;
       }
     }
     __builtin_amdgcn_wave_barrier();
   }
   const int64_t pc = p0 + wave * 64 + lane;
   if (pc < P) {
-    // the second-largest x rounded up to a multiple of 128 units in the last place, the row in the freed bits
-    const uint32_t x2 = ((__float_as_uint(m2) + 127u) & ~127u) | (uint32_t)bt;
-    gmem(rec)[mt * rec_ld + pc] = ((uint64_t)x2 << 32) | __float_as_uint(m1);
+    // high word: m2 rounded down to 128 units in the last place, bit 0 = forced
+    const uint32_t x2 = (m2 & ~127u) | (dlo < dmin_b ? 1u : 0u);
+    gmem(rec)[mt * rec_ld + pc] = ((uint64_t)x2 << 32) | m1;
   }
 }
 
@@ -313,10 +320,13 @@ __global__ __launch_bounds__(256) void screen_resolve_kernel(
     const double ka = kA + kArel * sqrt((s_max + delta) / (fmax(s_min, 0.0) + delta));
     const double up = sqrt(1.0 + kD), dn = sqrt(1.0 - kD), ur = ka * sqrt((1.0 + kD) / (1.0 - kD));
     const double F0 = gmem(max_F)[p];
+    // record: low word m1 = (bits(x) & ~127) | row of the tile's best row; high word (m2 & ~127) | forced
+    // (an x that overflowed FP32 counts as forced)
+    auto forced = [](uint64_t v) { return ((v >> 32) & 1u) || ((uint32_t)v & ~127u) >= 0x7f800000u; };
     float xm = 0.f;
     for (int64_t t = 0; t < n_mt; ++t) {
-      const float x1 = __uint_as_float((uint32_t)gmem(rec)[t * rec_ld + i]);
-      if (x1 < __builtin_inff()) xm = fmaxf(xm, x1);
+      const uint64_t v = gmem(rec)[t * rec_ld + i];
+      if (!forced(v)) xm = fmaxf(xm, __uint_as_float((uint32_t)v & ~127u));  // a lower bound of its x
     }
     // L: the largest lower bound, c of the running maximum (F = nu c^2 / (1 - c^2)) or the best tile's cL
     const double L = fmax(F0 > 0.0 ? sqrt(F0 / (nu + F0)) : 0.0, sqrt((double)xm * cc) * dn - ka);
@@ -324,8 +334,11 @@ __global__ __launch_bounds__(256) void screen_resolve_kernel(
     const double X = th > 0.0 ? th * th / cc : -1.0;
     for (int64_t t = 0; t < n_mt; ++t) {
       const uint64_t v = gmem(rec)[t * rec_ld + i];
-      const float x1 = __uint_as_float((uint32_t)v), x2 = __uint_as_float((uint32_t)(v >> 32) & ~127u);
-      if (!(x1 < __builtin_inff()) || (double)x2 >= X)
+      const uint32_t hw = (uint32_t)(v >> 32);
+      // upper bounds: the best row's x and every other row's x (values rounded down by < 128 units in the last
+      // place before packing)
+      const float x1 = __uint_as_float((uint32_t)v | 127u), x2 = __uint_as_float(hw | 127u);
+      if (forced(v) || (double)x2 >= X)
         mask2 |= 1ull << t;
       else if ((double)x1 >= X)
         mask1 |= 1ull << t;
@@ -334,6 +347,7 @@ __global__ __launch_bounds__(256) void screen_resolve_kernel(
   uint64_t pend = __ballot((mask1 | mask2) != 0);
   const int64_t wave_col0 = i - lane;
   const bool regs = neff <= 64 * RS_KREG;
+  unsigned long long cnt = 0;  // this wave's evaluations (one counter update per wave)
   while (pend) {
     const int j = __builtin_ctzll(pend);
     pend &= pend - 1;
@@ -364,7 +378,6 @@ __global__ __launch_bounds__(256) void screen_resolve_kernel(
     }
     rr = fmax(wave_sum(rr), FLOOR64);
     double fmx = -__builtin_inf();
-    unsigned long long cnt = 0;
     auto eval = [&](int64_t m) {
       const double* u = US + m * ld_us;
       double a = 0.0, d = 0.0;
@@ -392,18 +405,16 @@ __global__ __launch_bounds__(256) void screen_resolve_kernel(
     };
     for (uint64_t b = k1; b; b &= b - 1) {
       const int64_t t = __builtin_ctzll(b);
-      eval(t * W_TM + (int64_t)((gmem(rec)[t * rec_ld + ij] >> 32) & 127u));
+      eval(t * W_TM + (int64_t)(gmem(rec)[t * rec_ld + ij] & 127u));
     }
     for (uint64_t b = k2; b; b &= b - 1) {
       const int64_t t = __builtin_ctzll(b);
       const int64_t hi = (t + 1) * W_TM < Mc ? (t + 1) * W_TM : Mc;
       for (int64_t m = t * W_TM; m < hi; ++m) eval(m);
     }
-    if (lane == 0) {
-      gmem(max_F)[p] = fmax(gmem(max_F)[p], fmx);
-      if (n_eval) atomicAdd(n_eval, cnt);
-    }
+    if (lane == 0) gmem(max_F)[p] = fmax(gmem(max_F)[p], fmx);
   }
+  if (n_eval && lane == 0 && cnt) atomicAdd(n_eval, cnt);
 }
 
 hipError_t launch_wald_screen(const ScreenArgs& g, int64_t Mc, int64_t P, float dmin, uint64_t* rec, int64_t rec_ld,

@@ -68,6 +68,10 @@ SIGNATURES: dict[str, list] = {
     "flmm_adam": [_P, _P, _P, _P, _I64, _D, _D, _D, _D, _I, _P],
     "flmm_wald_scan_screened": [_P, _P, _P, _P, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _I64,
                                 _P, _P, _I64, _D, _D, _P, _P, _D, _P, _P, _P, _P],
+    "flmm_wald_screen": [_P, _P, _P, _P, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _I64, _P, _P,
+                         _I64, _D, _D, _P, _P, _D, _P, _P, _P, _P],
+    "flmm_screen_resolve": [_P, _P, _P, _P, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _I64, _P,
+                            _P, _I64, _D, _D, _P, _P, _D, _P, _P, _P, _P],
     "flmm_wald_scan": [_I, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P,
                        _P, _D, _P, _P, _P, _P, _P, _P, _P, _P],
     "flmm_wald_scan_lowrank": [_I, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64,

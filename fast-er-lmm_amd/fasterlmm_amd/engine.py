@@ -417,8 +417,8 @@ class _ScanTimer:
         """One launch: its events, Wald tests, and flops per test of the reference algorithm (ref_pt), of the
         launched algorithm (alg_pt) and over the padded K the kernel multiplies (kernel_pt: per product term; the
         split scan issues three such products, the screen one);
-        R > 0 for factored weights (the exponential sum's term count); kind: "full" (flmm_wald_scan*) or
-        "screen" (flmm_wald_scan_screened: screen + FP64 resolution)."""
+        R > 0 for factored weights (the exponential sum's term count); kind: "full" (flmm_wald_scan*), "screen"
+        (flmm_wald_screen) or "resolve" (flmm_screen_resolve, recorded with 0 tests)."""
         self.launches.append((ev[0], ev[1], tests, ref_pt, alg_pt, kernel_pt, R, kind))
 
     def summary(self, kind: str | None = None) -> dict:
@@ -594,22 +594,25 @@ def _screened(s1: Planes, s2: Planes, d: DeltaResult, mc: int, p_first: int, P: 
     """flmm_wald_scan_screened over the columns p_first <= p < P of one SNP block (rec: >= ceil(mc/128) x
     (P - p_first) 8-byte words of workspace)."""
     kpad2 = d.fac.rpad if d.fac is not None else s1.kpad
-    st = _lib.stream()
-    ev = None
-    if scan_timer.enabled:
-        ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-        ev[0].record()
     us = screen.us
     cnt = screen_stats.counter
-    _lib.call("flmm_wald_scan_screened", s1.hi.data_ptr(), s2.hi.data_ptr(), s1.scale_ptr, s2.scale_ptr, s1.rows_pad,
-              d.q1.hi.data_ptr(), d.q2.hi.data_ptr(), d.q1.scale_ptr, d.q2.scale_ptr, d.q1.rows_pad, s1.kpad, kpad2,
-              mc, p_first, P, us.data_ptr(), us.stride(0), screen.uy.data_ptr(), screen.s.data_ptr(), screen.neff,
-              float(screen.s_lo), float(screen.s_hi), d.log_delta.data_ptr(), d.rWr.data_ptr(), float(nu),
-              out.max_F.data_ptr(), rec.data_ptr(), _lib.ptr(cnt), st)
+    args = (s1.hi.data_ptr(), s2.hi.data_ptr(), s1.scale_ptr, s2.scale_ptr, s1.rows_pad, d.q1.hi.data_ptr(),
+            d.q2.hi.data_ptr(), d.q1.scale_ptr, d.q2.scale_ptr, d.q1.rows_pad, s1.kpad, kpad2, mc, p_first, P,
+            us.data_ptr(), us.stride(0), screen.uy.data_ptr(), screen.s.data_ptr(), screen.neff, float(screen.s_lo),
+            float(screen.s_hi), d.log_delta.data_ptr(), d.rWr.data_ptr(), float(nu), out.max_F.data_ptr(),
+            rec.data_ptr(), _lib.ptr(cnt), _lib.stream())
     if cnt is not None:
         screen_stats.tests += mc * (P - p_first)
-    if ev is not None:
+    if not scan_timer.enabled:
+        _lib.call("flmm_wald_scan_screened", *args)
+        return
+    # timed: the screen and the resolution as two calls, each between HIP events on the launch stream
+    k2, r2 = (d.fac.rpad, d.fac.R) if d.fac else (s1.kpad, d.neff)
+    for name, kind in (("flmm_wald_screen", "screen"), ("flmm_screen_resolve", "resolve")):
+        ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        ev[0].record()
+        _lib.call(name, *args)
         ev[1].record()
-        k2, r2 = (d.fac.rpad, d.fac.R) if d.fac else (s1.kpad, d.neff)
-        scan_timer.record(ev, mc * (P - p_first), 4.0 * d.neff, 2.0 * (d.neff + r2), 2.0 * (s1.kpad + k2),
-                          d.fac.R if d.fac else 0, kind="screen")
+        # the resolution adds time, not tests: the tests count once, with the screen
+        scan_timer.record(ev, mc * (P - p_first) if kind == "screen" else 0, 4.0 * d.neff, 2.0 * (d.neff + r2),
+                          2.0 * (s1.kpad + k2), d.fac.R if d.fac else 0, kind=kind)

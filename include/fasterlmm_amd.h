@@ -324,6 +324,21 @@ int flmm_wald_scan_screened(const void* s1_hi, const void* s2_hi, const float* s
                             const double* s, int64_t neff, double s_lo, double s_hi, const double* log_delta,
                             const double* rWr, double nu, double* max_F, void* rec, unsigned long long* n_eval,
                             void* stream);
+/* The two halves of flmm_wald_scan_screened, same arguments: flmm_wald_screen runs the screen alone (writes rec),
+ * flmm_screen_resolve the FP64 re-evaluation from a rec the screen wrote (reads rec, updates max_F); in that order
+ * on one stream they equal one flmm_wald_scan_screened call (a caller times them apart). */
+int flmm_wald_screen(const void* s1_hi, const void* s2_hi, const float* s1_scale, const float* s2_scale,
+                     int64_t mc_pad, const void* q1_hi, const void* q2_hi, const float* q1_scale,
+                     const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t kpad2, int64_t mc, int64_t p_first,
+                     int64_t P, const double* us, int64_t ld_us, const float* uy, const double* s, int64_t neff,
+                     double s_lo, double s_hi, const double* log_delta, const double* rWr, double nu, double* max_F,
+                     void* rec, unsigned long long* n_eval, void* stream);
+int flmm_screen_resolve(const void* s1_hi, const void* s2_hi, const float* s1_scale, const float* s2_scale,
+                        int64_t mc_pad, const void* q1_hi, const void* q2_hi, const float* q1_scale,
+                        const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t kpad2, int64_t mc,
+                        int64_t p_first, int64_t P, const double* us, int64_t ld_us, const float* uy, const double* s,
+                        int64_t neff, double s_lo, double s_hi, const double* log_delta, const double* rWr, double nu,
+                        double* max_F, void* rec, unsigned long long* n_eval, void* stream);
 
 /* Fused Wald scan of one SNP block (snp_wald_scan_compat's tile loop, core.py:636-667): sWy = S1.Q1, sWs = S2.Q2 on
  * MFMA, FP64 epilogue with dof nu; split-FP16 sums are divided by the planes' row scales.  Real columns
