@@ -122,6 +122,163 @@ __global__ __launch_bounds__(NTHREADS, 1) void delta_grid_kernel(TileArgs<PREC_F
   }
 }
 
+// The same grid argmin on split-FP16 MFMA.  Both operands arrive as fp32 planes and are scaled and split into fp16
+// (hi, lo) planes as each 32-k chunk is stored into LDS: grid row g (weights w_gk = 1 / (s_k + delta_g) <=
+// 1 / (s_min + delta_g)) by the power of two sa_g of that bound, column p (y_k^2 <= |y_p|^2) by
+// sb_p = col_scale[p]^2 / 2^14, col_scale[p] a power of two with col_scale[p] |y_p| <= 2^14 (the rotated columns'
+// own plane scale: the basis is orthonormal).  Three fp16 products per step (hi hi, hi lo, lo hi) with FP32
+// accumulation give the weighted sums to ~2^-21 relative, as FP32 products with FP32 accumulation do; sb_p adds
+// the same constant to every grid loss of column p (no change to its argmin) and sa_g is taken out through lam.
+// 128 grid rows x 128 columns per workgroup, 4 waves of 64 x 64, chunks double-buffered in LDS.
+__global__ __launch_bounds__(NTHREADS, 2) void delta_grid_split_kernel(
+    const float* __restrict__ wplane, const float* __restrict__ uy, const float* __restrict__ col_scale, double s_min,
+    int G, double lo, double hi, const float* __restrict__ lam, float nco, int64_t P, int64_t p_pad, int64_t kpad,
+    int64_t n_tiles, int* __restrict__ idx_out) {
+  constexpr int PLANE = TILE * 64;  // one fp16 plane of a chunk: 128 rows x 32 values
+  constexpr int STAGE = 4 * PLANE;  // A hi, A lo, B hi, B lo
+  __shared__ __attribute__((aligned(16))) char lds[2 * STAGE];
+  __shared__ float lam_s[TILE];
+  const int64_t pt = xcd_remap(blockIdx.x, n_tiles);
+  const int64_t p0 = pt * TILE;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1, h = lane >> 5;
+  const int nch = (int)(kpad / KCHUNK);
+  // staging: thread t covers (row r = t / 4 + 64 ps, 8-value segment sg = t % 4), ps = 0, 1, of each side
+  const int sg = tid & 3;
+  float sc[2][2];  // [side][ps]
+#pragma unroll
+  for (int ps = 0; ps < 2; ++ps) {
+    const int r = (tid >> 2) + 64 * ps;
+    sc[0][ps] = (float)pow2_scale(1.0 / (s_min + exp(grid_value(r, G, lo, hi))));
+    const int64_t p = p0 + r;
+    const float cs = p < P ? gmem(col_scale)[p] : 0.f;
+    sc[1][ps] = cs * cs * (1.0f / 16384.0f);
+  }
+  if (tid < TILE) {
+    const double sa = pow2_scale(1.0 / (s_min + exp(grid_value(tid, G, lo, hi))));
+    lam_s[tid] = tid < G ? (float)((double)gmem(lam)[tid] - (double)nco * log(sa)) : 0.f;
+  }
+  auto gload = [&](int kc, f32x4 (&f)[2][2][2]) {
+#pragma unroll
+    for (int ps = 0; ps < 2; ++ps) {
+      const int r = (tid >> 2) + 64 * ps;
+      const float* a = wplane + ((int64_t)kc * TILE + r) * 32 + sg * 8;
+      const float* b = uy + ((int64_t)kc * p_pad + p0 + r) * 32 + sg * 8;
+      f[0][ps][0] = *gmem((const f32x4*)a);
+      f[0][ps][1] = *gmem((const f32x4*)(a + 4));
+      f[1][ps][0] = *gmem((const f32x4*)b);
+      f[1][ps][1] = *gmem((const f32x4*)(b + 4));
+    }
+  };
+  auto lstore = [&](const f32x4 (&f)[2][2][2], char* st) {
+#pragma unroll
+    for (int side = 0; side < 2; ++side)
+#pragma unroll
+      for (int ps = 0; ps < 2; ++ps) {
+        const int r = (tid >> 2) + 64 * ps;
+        u16 hv[8], lv[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const float v = f[side][ps][e >> 2][e & 3];
+          split2f<PREC_SPLIT_FP16>(side ? v * v * sc[1][ps] : v * sc[0][ps], hv[e], lv[e]);
+        }
+        const int off = r * 64 + ((sg ^ ((r >> 2) & 3)) * 16);
+        u32x4 ph, pl;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          ph[q] = hv[2 * q] | ((uint32_t)hv[2 * q + 1] << 16);
+          pl[q] = lv[2 * q] | ((uint32_t)lv[2 * q + 1] << 16);
+        }
+        *(u32x4*)(st + (side * 2) * PLANE + off) = ph;
+        *(u32x4*)(st + (side * 2 + 1) * PLANE + off) = pl;
+      }
+  };
+
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+
+  f32x4 pre[2][2][2];
+  gload(0, pre);
+  lstore(pre, lds);
+  __syncthreads();
+  for (int kc = 0; kc < nch; ++kc) {
+    const char* cur = lds + (kc & 1) * STAGE;
+    char* nxt = lds + ((kc & 1) ^ 1) * STAGE;
+    const bool more = kc + 1 < nch;
+    if (more) gload(kc + 1, pre);
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      const int seg = h * 2 + ks;
+      bf16x8 af[2][2], bfr[2][2];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi) {
+        const int r = wm * 64 + bi * 32 + (lane & 31);
+#pragma unroll
+        for (int sub = 0; sub < 2; ++sub)
+          af[bi][sub] = *(const bf16x8*)(cur + sub * PLANE + r * 64 + ((seg ^ ((r >> 2) & 3)) * 16));
+      }
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) {
+        const int r = wn * 64 + bj * 32 + (lane & 31);
+#pragma unroll
+        for (int sub = 0; sub < 2; ++sub)
+          bfr[bj][sub] = *(const bf16x8*)(cur + (2 + sub) * PLANE + r * 64 + ((seg ^ ((r >> 2) & 3)) * 16));
+      }
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+          f32x16 c = acc[bi][bj];
+          c = mfma_split<PREC_SPLIT_FP16>(af[bi][0], bfr[bj][0], c);
+          c = mfma_split<PREC_SPLIT_FP16>(af[bi][0], bfr[bj][1], c);
+          c = mfma_split<PREC_SPLIT_FP16>(af[bi][1], bfr[bj][0], c);
+          acc[bi][bj] = c;
+        }
+    }
+    if (more) lstore(pre, nxt);  // nxt was last read in the previous chunk, which ended with a barrier
+    __syncthreads();
+  }
+
+  // argmin over the grid rows, as delta_grid_kernel (first index of the minimum)
+  float* xl = (float*)lds;
+  int* xi = (int*)(xl + 128);
+#pragma unroll
+  for (int bj = 0; bj < 2; ++bj) {
+    float best = INFINITY;
+    int bidx = 0x7fffffff;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int g = acc_row(wm, bi, lane, v);
+        if (g < G) {
+          const float loss = nco * __logf(fmaxf(acc[bi][bj][v], 1e-30f)) + lam_s[g];
+          if (loss < best || (loss == best && g < bidx)) { best = loss; bidx = g; }
+        }
+      }
+    {
+      const float ob = __shfl_xor(best, 32, 64);
+      const int oi = __shfl_xor(bidx, 32, 64);
+      if (ob < best || (ob == best && oi < bidx)) { best = ob; bidx = oi; }
+    }
+    if (wm == 1 && lane < 32) { xl[(wn * 2 + bj) * 32 + lane] = best; xi[(wn * 2 + bj) * 32 + lane] = bidx; }
+    __syncthreads();
+    if (wm == 0 && lane < 32) {
+      const float ob = xl[(wn * 2 + bj) * 32 + lane];
+      const int oi = xi[(wn * 2 + bj) * 32 + lane];
+      if (ob < best || (ob == best && oi < bidx)) { best = ob; bidx = oi; }
+      const int64_t p = p0 + acc_col(wn, bj, lane);
+      if (p < P) idx_out[p] = (bidx == 0x7fffffff) ? 0 : bidx;
+    }
+    __syncthreads();
+  }
+}
+
 struct ColSums {
   double R, R2, R3, W1, W2, LS;
 };
@@ -887,6 +1044,15 @@ hipError_t launch_delta_grid(const float* wplane, const float* lam, int G, doubl
   constexpr int lds = tile_lds_bytes<PREC_FP32, 1, false>();
   hipLaunchKernelGGL(delta_grid_kernel, dim3((unsigned)n_tiles), dim3(NTHREADS), lds, stream, args, n_tiles, lam, G,
                      (float)nco, P, idx);
+  return hipGetLastError();
+}
+
+hipError_t launch_delta_grid_split(const float* wplane, const float* lam, int G, double nco, const float* uy,
+                                   const float* col_scale, double s_min, double lo, double hi, int64_t P, int64_t p_pad,
+                                   int64_t kpad, int* idx, hipStream_t stream) {
+  const int64_t n_tiles = p_pad / TILE;
+  hipLaunchKernelGGL(delta_grid_split_kernel, dim3((unsigned)n_tiles), dim3(NTHREADS), 0, stream, wplane, uy,
+                     col_scale, s_min, G, lo, hi, lam, (float)nco, P, p_pad, kpad, n_tiles, idx);
   return hipGetLastError();
 }
 

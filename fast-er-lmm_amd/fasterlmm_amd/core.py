@@ -163,25 +163,31 @@ class _Basis:
         self._u_pad = None
         self._uy64 = {}
         self.uy = None  # fp32 plane of B^T Y
+        self.col_scale = None  # split-FP16 columns inside span(B): the Y planes' row scales also bound B^T y
         self.P = 0
         self.p_pad = 0
         self.delta = None  # the DeltaResult of the last fit (its log_delta tensor identifies it)
 
-    def rotate_columns(self, Y: Tensor, n_exact: int | None = None) -> None:
+    def rotate_columns(self, Y: Tensor, n_exact: int | None = None, *, in_span: bool = False) -> None:
         """UY plane of B^T Y for Y (N, P) FP64; its first n_exact columns (default: all, up to EXACT_COLS) from the
-        FP64 product."""
+        FP64 product.  in_span: the columns lie in span(B) (regressed on X, B a complete basis of null(X^T)), so
+        |B^T y| = |y| and the planes' row scales serve the split delta grid."""
         N, P = Y.shape
         self.P, self.p_pad = P, rup(max(P, 1), PTILE)
         ypl = engine.pack(self.prec, Y.contiguous(), P, N, 1, P, self.p_pad, self.npad)
         self.uy = engine.rotate_uy(ypl, self.bt)
+        self.col_scale = ypl.scale if in_span else None
         self.y_cols, self._uy64 = Y, {}
         self._exact_rows(Y[:, : min(P, EXACT_COLS if n_exact is None else int(n_exact))])
 
-    def use_column_planes(self, ypl: engine.Planes, P: int, y_real: Tensor | None = None) -> None:
+    def use_column_planes(self, ypl: engine.Planes, P: int, y_real: Tensor | None = None, *,
+                          in_span: bool = False) -> None:
         """UY plane from pre-packed column planes (shared by every chromosome of a LOCO scan); y_real: the FP64
-        real columns (N, n_real), whose rows come from the FP64 product and which the re-scoring reads."""
+        real columns (N, n_real), whose rows come from the FP64 product and which the re-scoring reads; in_span as
+        rotate_columns."""
         self.P, self.p_pad = P, ypl.rows_pad
         self.uy = engine.rotate_uy(ypl, self.bt)
+        self.col_scale = ypl.scale if in_span else None
         self.y_cols, self._uy64 = y_real, {}
         if y_real is not None:
             self._exact_rows(y_real)
@@ -285,7 +291,8 @@ class _Basis:
                                     nco, n_grid, lo, hi, None, fac=fac, hi_from=hf)
             self.delta = d
             return d.log_delta, d
-        idx, lam64 = engine.delta_grid(self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam, nco, n_grid, lo, hi)
+        idx, lam64 = engine.delta_grid(self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam, nco, n_grid, lo, hi,
+                                       col_scale=self.col_scale, s_min=self.s_range[0])
         d = engine.delta_refine(self.prec, engine.delta_code(), self.uy, self.P, self.p_pad, self.kpad, self.s, s_lam,
                                 nco, n_grid, lo, hi, idx, fac=fac, lam64=lam64, hi_from=hf)
         self.delta = d
@@ -395,7 +402,7 @@ def fastlmm_compat_rotate(K: Tensor, X: Tensor, Y: Tensor) -> CompatSpectrum:
     Xpinv, Iperp = _projector(Xd)
     s, U = _projected_eigh(Kd, Iperp, D)
     st = _Basis(s, U, D, N - D, engine.prec_code(), nu=N - D - 1)
-    st.rotate_columns(Yd - Xd @ (Xpinv @ Yd))
+    st.rotate_columns(Yd - Xd @ (Xpinv @ Yd), in_span=True)
     spec = CompatSpectrum(s=s.to(home), U_eff=U[:, D:N].contiguous().to(home), X=Xd.to(home), Xpinv=Xpinv.to(home),
                           UY=st.uy_dense().to(home))
     spec._gpu = st
@@ -484,7 +491,7 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
         with stage("pack_basis"):
             st = _Basis(s, Ub[k], D, N - D, prec, nu=N - D - 1, s_ml=None if s_mls is None else s_mls[k])
         with stage("rotate_columns"):
-            st.use_column_planes(ypl, P, y_real)
+            st.use_column_planes(ypl, P, y_real, in_span=True)
         with stage("delta"):
             _, d = st.fit(n_grid, -10.0, 10.0, True, projected_loss=(not plain) or reml, n_real=n_real)
         done = torch.cuda.Event()

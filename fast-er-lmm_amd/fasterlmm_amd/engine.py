@@ -48,6 +48,8 @@ class Settings:
     snp_rotation: str = "fp64"     # SNP side US = U_eff^T S: "fp64" (FP64 MFMA kernel, then packed) or "split" (one
                                    # fused MFMA pass in the split precision: ~2^-22 of |S| in every direction, which
                                    # the scan weighs by 1 / (s_k + delta))
+    grid: str = "auto"             # delta grid GEMM: "auto" = split-FP16 MFMA where the rotated columns carry plane
+                                   # scales (split_fp16), "fp32" = FP32 MFMA
     screen: str = "auto"           # permutation columns' max_F: "auto" = the certified hi-plane screen with FP64
                                    # re-evaluation of the tests that can hold each maximum (split_fp16 with the
                                    # FP64 SNP rotation), "off" = the full split scan for every column
@@ -337,9 +339,12 @@ def factored_snp_planes(a: Planes, bt: Planes, mc: int, fac: Factored) -> tuple[
 
 
 def delta_grid(uy: torch.Tensor, P: int, p_pad: int, kpad: int, s: torch.Tensor, s_lam: torch.Tensor,
-               nco: float, n_grid: int, lo: float, hi: float) -> tuple[torch.Tensor, torch.Tensor]:
-    """Per-column FP32 grid argmin (int32, (P,)) and the FP64 log-det term at every grid node (the refinement's
-    FP64 walk to the grid minimum uses it)."""
+               nco: float, n_grid: int, lo: float, hi: float, col_scale: torch.Tensor | None = None,
+               s_min: float = 0.0) -> tuple[torch.Tensor, torch.Tensor]:
+    """Per-column grid argmin (int32, (P,)) and the FP64 log-det term at every grid node (the refinement's FP64
+    walk to the grid minimum uses it).  With col_scale (power-of-two scales with col_scale[p] |column p| <= 2^14)
+    and settings.grid "auto" the grid GEMM runs on split-FP16 MFMA (flmm_delta_grid_split; 0 <= s_min <= every
+    s_k), else on FP32 MFMA."""
     if n_grid > TILE:
         raise ValueError(f"n_grid <= {TILE} is supported, got {n_grid}")
     dev = uy.device
@@ -350,8 +355,14 @@ def delta_grid(uy: torch.Tensor, P: int, p_pad: int, kpad: int, s: torch.Tensor,
     st = _lib.stream()
     _lib.call("flmm_delta_weights", s.data_ptr(), s.numel(), s_lam.data_ptr(), s_lam.numel(), kpad, n_grid, lo, hi,
               wplane.data_ptr(), lam.data_ptr(), lam64.data_ptr(), st)
-    _lib.call("flmm_delta_grid", wplane.data_ptr(), lam.data_ptr(), n_grid, float(nco), uy.data_ptr(), P, p_pad, kpad,
-              idx.data_ptr(), st)
+    if settings.grid not in ("auto", "fp32"):
+        raise ValueError(f"unknown grid setting {settings.grid!r}")
+    if col_scale is not None and settings.grid == "auto" and s_min >= 0.0:
+        _lib.call("flmm_delta_grid_split", wplane.data_ptr(), lam.data_ptr(), n_grid, float(nco), uy.data_ptr(),
+                  col_scale.data_ptr(), float(s_min), lo, hi, P, p_pad, kpad, idx.data_ptr(), st)
+    else:
+        _lib.call("flmm_delta_grid", wplane.data_ptr(), lam.data_ptr(), n_grid, float(nco), uy.data_ptr(), P, p_pad,
+                  kpad, idx.data_ptr(), st)
     return idx[:P], lam64
 
 

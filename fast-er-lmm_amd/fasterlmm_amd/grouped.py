@@ -406,7 +406,7 @@ class GroupedLoco:
                 with engine.stage("pack_basis"):
                     st = core._Basis(s, U[i, :N, :], lo, N - D, self.prec, nu=N - D - 1)
                 with engine.stage("rotate_columns"):
-                    st.use_column_planes(self.ypls[g], self.Ps[g], self.y_reals[g])
+                    st.use_column_planes(self.ypls[g], self.Ps[g], self.y_reals[g], in_span=True)
                 with engine.stage("delta"):
                     _, d = st.fit(self.n_grid, -10.0, 10.0, True, projected_loss=True)
                 out[g] = (st, d)

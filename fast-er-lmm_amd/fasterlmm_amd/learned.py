@@ -173,7 +173,8 @@ def refine_learned(basis, net: DeltaMLP, n_grid: int, lo: float, hi: float, *, n
     """Learned delta for every column of a core._Basis: grid (arbiter) + MLP proposal + checked Newton; hi_from:
     the certified screen's first column (engine.screen_start), whose planes get the hi halves only."""
     s_lam, nco = basis._loss(projected_loss)
-    idx, lam64 = engine.delta_grid(basis.uy, basis.P, basis.p_pad, basis.kpad, basis.s, s_lam, nco, n_grid, lo, hi)
+    idx, lam64 = engine.delta_grid(basis.uy, basis.P, basis.p_pad, basis.kpad, basis.s, s_lam, nco, n_grid, lo, hi,
+                                   col_scale=basis.col_scale, s_min=basis.s_range[0])
     feat = features(basis.uy, basis.P, basis.p_pad, basis.s, n, c)
     t0 = net.predict(feat)
     dev = basis.uy.device
@@ -247,6 +248,7 @@ def basis_from_columns(s: torch.Tensor, uy: torch.Tensor, prec: int):
     pl = engine.pack(_lib.PREC_FP32, uy.contiguous(), P, neff, 1, P, b.p_pad, b.kpad)
     b.uy = pl.hi
     b.delta = None
+    b.col_scale, b.s_range = None, (0.0, 0.0)  # simulated columns: the FP32 grid
     b._loss = lambda projected=True: (b.s, float(neff))
     return b
 
@@ -357,9 +359,9 @@ def evaluate(net: DeltaMLP, *, n: int = 2000, n_pheno: int = 10_000, n_perm: int
     prec = engine.prec_code()
     Y -= X @ (Xpinv @ Y)
     st = _Basis(s, U, 1, n - 1, prec, nu=n - 2)
-    st.rotate_columns(Y)
+    st.rotate_columns(Y, in_span=True)
     st_real = _Basis(s, U, 1, n - 1, prec, nu=n - 2)
-    st_real.rotate_columns(Y[:, :n_pheno].contiguous())
+    st_real.rotate_columns(Y[:, :n_pheno].contiguous(), in_span=True)
     del Y
     neff = n - 1
     say("rotated; fitting delta")

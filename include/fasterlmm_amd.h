@@ -157,6 +157,15 @@ int flmm_delta_weights(const double* s, int64_t neff, const double* s_lam, int64
 int flmm_delta_grid(const float* wplane, const float* lam, int n_grid, double nco, const float* uy, int64_t P,
                     int64_t p_pad, int64_t kpad, int32_t* idx, void* stream);
 
+/* flmm_delta_grid on split-FP16 MFMA: the same argmin from the same fp32 wplane / lam (flmm_delta_weights) and UY
+ * plane, both operands scaled and split into fp16 (hi, lo) planes as they are staged (three products, FP32
+ * accumulation, ~2^-21 relative: the accuracy of the FP32 product).  Needs a power-of-two col_scale[p] with
+ * col_scale[p] * |column p| <= 2^14 (the rotated columns' plane scale: an orthonormal basis keeps the norm) and
+ * 0 <= s_min <= every s_k (grid row g is scaled by the power of two of 1 / (s_min + delta_g)). */
+int flmm_delta_grid_split(const float* wplane, const float* lam, int n_grid, double nco, const float* uy,
+                          const float* col_scale, double s_min, double lo, double hi, int64_t P, int64_t p_pad,
+                          int64_t kpad, int32_t* idx, void* stream);
+
 /* Refine log delta in [grid[i-1], grid[i+1]] (core.py:575-595), i = the FP64 grid minimum reached from the FP32 grid
  * argmin idx (the reference brackets the FP64 argmin; FP32 ties on flat likelihoods are resolved in FP64, each
  * step using lam64 -- the FP64 log-det table of flmm_delta_weights -- or, if NULL, per-column sums;
