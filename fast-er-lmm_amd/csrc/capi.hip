@@ -136,7 +136,7 @@ static double* scratch_ptr(hipStream_t stream, const char* who) {
 
 extern "C" {
 
-int flmm_abi_version(void) { return 12; }
+int flmm_abi_version(void) { return 13; }
 
 const char* flmm_last_error(void) { return g_err.c_str(); }
 
@@ -284,8 +284,9 @@ int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_
                       int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo, double hi,
                       const int32_t* idx, const double* lam64, double* log_delta, double* rWr, double* h2,
                       void* q1_hi, void* q1_lo, void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale,
-                      int64_t hi_from, void* stream) {
+                      int64_t hi_from, double ld_uniform, void* order, void* stream) {
   REQUIRE(hi_from >= 0, "flmm_delta_refine: hi_from must be >= 0");
+  REQUIRE(!order || isfinite(ld_uniform), "flmm_delta_refine: a layout needs a finite ld_uniform");
   REQUIRE(mode == FLMM_DELTA_GOLDEN || mode == FLMM_DELTA_NEWTON || mode == FLMM_DELTA_FIXED ||
               mode == FLMM_DELTA_GRID,
           "flmm_delta_refine: bad mode %d", mode);
@@ -296,7 +297,8 @@ int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_
   if (p_pad == 0) return FLMM_OK;
   return check_launch(launch_delta_refine(prec, mode, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco, n_grid, lo, hi,
                                           idx, lam64, log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, q1_scale,
-                                          q2_scale, 0.0, 0.0, nullptr, hi_from, (hipStream_t)stream),
+                                          q2_scale, 0.0, 0.0, nullptr, hi_from, ld_uniform, order,
+                                          (hipStream_t)stream),
                       "flmm_delta_refine");
 }
 
@@ -369,7 +371,7 @@ int flmm_rescore_f64(const double* US, const double* UY, int64_t ld, const doubl
 
 int flmm_factored_weight_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
                                 int64_t R, const double* log_delta, void* q2_hi, void* q2_lo, float* q2_scale,
-                                int64_t hi_from, void* stream) {
+                                int64_t hi_from, const void* order, void* stream) {
   REQUIRE(valid_prec(prec), "flmm_factored_weight_planes: bad precision %d", prec);
   REQUIRE(hi_from >= 0, "flmm_factored_weight_planes: hi_from must be >= 0");
   REQUIRE(P >= 0 && P <= p_pad && p_pad % TILE == 0, "flmm_factored_weight_planes: bad column extent");
@@ -380,8 +382,9 @@ int flmm_factored_weight_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad
   REQUIRE(aligned16(q2_hi) && (!two || aligned16(q2_lo)), "flmm_factored_weight_planes: planes must be 16-byte aligned");
   REQUIRE(prec != FLMM_PREC_SPLIT_FP16 || q2_scale, "flmm_factored_weight_planes: split-FP16 needs q2_scale");
   if (p_pad == 0) return FLMM_OK;
+  const int* pos = (order && hi_from < P) ? screen_order((void*)order, P - hi_from).pos : nullptr;
   return check_launch(launch_factored_planes(prec, P, p_pad, rpad, t, lam, R, log_delta, q2_hi, q2_lo, q2_scale,
-                                             hi_from, (hipStream_t)stream),
+                                             hi_from, pos, (hipStream_t)stream),
                       "flmm_factored_weight_planes");
 }
 
@@ -390,16 +393,18 @@ int flmm_delta_refine_learned(int prec, const float* uy, int64_t P, int64_t p_pa
                               double hi, const int32_t* idx, const double* lam64, double radius, double judge_tol,
                               double* log_delta, int32_t* fallback, double* rWr, double* h2, void* q1_hi,
                               void* q1_lo, void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale,
-                              int64_t hi_from, void* stream) {
+                              int64_t hi_from, double ld_uniform, void* order, void* stream) {
   REQUIRE(radius > 0.0 && judge_tol >= 0.0 && idx, "flmm_delta_refine_learned: bad radius / tol / idx");
   REQUIRE(hi_from >= 0, "flmm_delta_refine_learned: hi_from must be >= 0");
+  REQUIRE(!order || isfinite(ld_uniform), "flmm_delta_refine_learned: a layout needs a finite ld_uniform");
   const int rc = check_refine("flmm_delta_refine_learned", prec, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco,
                               n_grid, log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, q1_scale, q2_scale);
   if (rc) return rc;
   if (p_pad == 0) return FLMM_OK;
   return check_launch(launch_delta_refine(prec, 3, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco, n_grid, lo, hi, idx,
                                           lam64, log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, q1_scale, q2_scale,
-                                          radius, judge_tol, fallback, hi_from, (hipStream_t)stream),
+                                          radius, judge_tol, fallback, hi_from, ld_uniform, order,
+                                          (hipStream_t)stream),
                       "flmm_delta_refine_learned");
 }
 
@@ -556,12 +561,23 @@ static constexpr double SC_SLACK = 1.0 / 524288.0;
 static double sc_gamma(double n) { return n / 16777216.0 / (1.0 - n / 16777216.0); }
 
 // phases: 1 = the screen kernel, 2 = the resolution kernel
+int64_t flmm_screen_order_bytes(int64_t n_cols) {
+  if (n_cols < 0) return -1;
+  return 4 * screen_order_ints(n_cols);
+}
+
+int64_t flmm_screen_work_bytes(int64_t mc_pad) {
+  if (mc_pad < 0) return -1;
+  return 4 * mc_pad;
+}
+
 static int screened(const char* who, int phases, const void* s1_hi, const void* s2_hi, const float* s1_scale,
                     const float* s2_scale, int64_t mc_pad, const void* q1_hi, const void* q2_hi, const float* q1_scale,
                     const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t kpad2, int64_t mc, int64_t p_first,
                     int64_t P, const double* us, int64_t ld_us, const float* uy, const double* s, int64_t neff,
                     double s_lo, double s_hi, const double* log_delta, const double* rWr, double nu, double* max_F,
-                    void* rec, unsigned long long* n_eval, void* stream) {
+                    void* rec, double ld_uniform, const void* order, void* work, unsigned long long* n_eval,
+                    void* stream) {
   REQUIRE(mc >= 0 && mc <= mc_pad && mc_pad % TILE == 0 && mc <= 64 * TILE, "%s: bad SNP extent (mc <= 8192)", who);
   REQUIRE(P >= 0 && P <= p_pad && p_pad % 256 == 0 && p_first >= 0 && p_first % 256 == 0 && p_first <= P,
           "%s: bad column extent (p_pad and p_first multiples of 256, p_first <= P <= p_pad)", who);
@@ -575,6 +591,9 @@ static int screened(const char* who, int phases, const void* s1_hi, const void* 
   REQUIRE(aligned16(s1_hi) && aligned16(s2_hi) && aligned16(q1_hi) && aligned16(q2_hi),
           "%s: operand planes must be 16-byte aligned", who);
   REQUIRE(us && uy && s && log_delta && rWr && max_F && rec, "%s: null pointer", who);
+  const bool uniform = order != nullptr;
+  REQUIRE(!uniform || (isfinite(ld_uniform) && work),
+          "%s: the uniform-delta split needs a finite ld_uniform and the workspace (flmm_screen_work_bytes)", who);
   if (mc == 0 || P == p_first) return FLMM_OK;
   const int64_t Pl = P - p_first;
   const double u = SC_U, K = (double)kpad, K2 = (double)kpad2, e = SUBNORMAL_EPS;
@@ -599,6 +618,28 @@ static int screened(const char* who, int phases, const void* s1_hi, const void* 
   g.nchunks2 = (int)(kpad2 / KCHUNK);
   uint64_t* r = (uint64_t*)rec;
   if (phases & 1) {
+    if (uniform) {
+      // the columns at log delta == ld_uniform (plane slots 0 .. counts[0] - 1): the sWy-only tile; the rest (the
+      // slots after): the general tile
+      const ScreenOrder o = screen_order((void*)order, Pl);
+      float* row_f = (float*)work;
+      int rc = check_launch(launch_screen_rows(us, ld_us, mc, mc_pad, s, neff, ld_uniform, s1_scale, row_f, st), who);
+      if (rc != FLMM_OK) return rc;
+      UniformArgs ua{};
+      ua.s1 = g.s1;
+      ua.q1 = g.q1;
+      ua.q2s = q2_scale + p_first;
+      ua.row_f = row_f;
+      ua.order = o.order;
+      ua.counts = o.counts;
+      ua.a_rows_pad = mc_pad;
+      ua.b_rows_pad = p_pad;
+      ua.nchunks = g.nchunks;
+      rc = check_launch(launch_screen_uniform(ua, mc, Pl, r, Pl, st), who);
+      if (rc != FLMM_OK) return rc;
+      g.order = o.order;
+      g.counts = o.counts;
+    }
     const int rc = check_launch(launch_wald_screen(g, mc, Pl, dmin, r, Pl, st), who);
     if (rc != FLMM_OK) return rc;
   }
@@ -611,13 +652,14 @@ static int screened(const char* who, int phases, const void* s1_hi, const void* 
 
 #define FLMM_SCREENED_ARGS                                                                                           \
   s1_hi, s2_hi, s1_scale, s2_scale, mc_pad, q1_hi, q2_hi, q1_scale, q2_scale, p_pad, kpad, kpad2, mc, p_first, P, us, \
-      ld_us, uy, s, neff, s_lo, s_hi, log_delta, rWr, nu, max_F, rec, n_eval, stream
+      ld_us, uy, s, neff, s_lo, s_hi, log_delta, rWr, nu, max_F, rec, ld_uniform, order, work, n_eval, stream
 #define FLMM_SCREENED_PARAMS                                                                                         \
   const void *s1_hi, const void *s2_hi, const float *s1_scale, const float *s2_scale, int64_t mc_pad,                \
       const void *q1_hi, const void *q2_hi, const float *q1_scale, const float *q2_scale, int64_t p_pad,             \
       int64_t kpad, int64_t kpad2, int64_t mc, int64_t p_first, int64_t P, const double *us, int64_t ld_us,          \
       const float *uy, const double *s, int64_t neff, double s_lo, double s_hi, const double *log_delta,             \
-      const double *rWr, double nu, double *max_F, void *rec, unsigned long long *n_eval, void *stream
+      const double *rWr, double nu, double *max_F, void *rec, double ld_uniform, const void *order, void *work,       \
+      unsigned long long *n_eval, void *stream
 
 int flmm_wald_scan_screened(FLMM_SCREENED_PARAMS) {
   return screened("flmm_wald_scan_screened", 3, FLMM_SCREENED_ARGS);

@@ -882,7 +882,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
 // scales the refinement chose.  q2a == NULL: Q2 is not written (factored weights, factored_planes_kernel).
 // Columns hi_from <= p < P (the certified screen's, split precisions) get the hi planes only, from FP32 arithmetic:
 // each value is then within ~5 FP32 rounding units of the FP64 one before its fp16 rounding, which the screen's
-// error bound covers.
+// error bound covers; with a layout (pos), column p's rows go to plane slot hi_from + pos[p - hi_from].
 // A block owns 32 adjacent columns and walks the K chunks; thread (column tid / 8, quad tid % 8) handles 4
 // consecutive k of each chunk, so for one chunk the block's UY read is one contiguous 4 KiB span (a 16-byte load
 // per lane) and its writes are contiguous 8-byte stores per plane (element (p, k) at
@@ -894,7 +894,8 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
                                                            const double* __restrict__ log_delta,
                                                            const float* __restrict__ q1s,
                                                            const float* __restrict__ q2s, void* q1a, void* q1b,
-                                                           void* q2a, void* q2b, int64_t hi_from) {
+                                                           void* q2a, void* q2b, int64_t hi_from,
+                                                           const int* __restrict__ pos) {
   const int64_t p = (int64_t)blockIdx.x * 32 + (threadIdx.x >> 3);  // p < p_pad (p_pad is a multiple of 32)
   const int j4 = (threadIdx.x & 7) * 4;
   const bool live = p < P;
@@ -908,10 +909,12 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
   if constexpr (is_split(PREC)) {
     if (live && p >= hi_from) {
       const float d32 = (float)delta, s1f = (float)sc1, s2f = (float)sc2;
+      const int64_t dst = pos ? hi_from + gmem(pos)[p - hi_from] : p;  // the screened columns' plane slot
 #pragma unroll 2
       for (int64_t c = 0; c < nch; ++c) {
         const int64_t k0 = c * 32 + j4;
         const int64_t off = (c * p_pad + p) * 32 + j4;
+        const int64_t woff = (c * p_pad + dst) * 32 + j4;
         const f32x4 y4 = *(const f32x4*)(uy + off);
         u16 h[4], g[4];
 #pragma unroll
@@ -921,9 +924,9 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
           h[e] = hi_of<PREC>(w * y4[e] * s1f);
           g[e] = hi_of<PREC>(w * s2f);
         }
-        *(uint2*)((u16*)q1a + off) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
+        *(uint2*)((u16*)q1a + woff) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
         if (wq2)
-          *(uint2*)((u16*)q2a + off) = make_uint2(g[0] | ((uint32_t)g[1] << 16), g[2] | ((uint32_t)g[3] << 16));
+          *(uint2*)((u16*)q2a + woff) = make_uint2(g[0] | ((uint32_t)g[1] << 16), g[2] | ((uint32_t)g[3] << 16));
       }
       return;
     }
@@ -977,7 +980,8 @@ __global__ __launch_bounds__(256) void factored_planes_kernel(int64_t P, int64_t
                                                               const double* __restrict__ t,
                                                               const double* __restrict__ lam, int64_t R,
                                                               const double* __restrict__ log_delta, void* qa,
-                                                              void* qb, float* __restrict__ qs, int64_t hi_from) {
+                                                              void* qb, float* __restrict__ qs, int64_t hi_from,
+                                                              const int* __restrict__ pos) {
   const int64_t p = (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);  // p < p_pad (p_pad is a multiple of 8)
   const int j = threadIdx.x & 31;
   const bool live = p < P;
@@ -996,9 +1000,11 @@ __global__ __launch_bounds__(256) void factored_planes_kernel(int64_t P, int64_t
     if (j == 0) qs[p] = (float)sc;
   }
   const int64_t nch = rpad / 32;
+  // the screened columns' rows at their plane slots (pos), hi halves only
+  const int64_t dst = (live && p >= hi_from && pos) ? hi_from + gmem(pos)[p - hi_from] : p;
   for (int64_t c = 0; c < nch; ++c) {
     const int64_t r = c * 32 + j;
-    const int64_t off = (c * p_pad + p) * 32 + j;
+    const int64_t off = (c * p_pad + dst) * 32 + j;
     const double v = val(r);
     if constexpr (is_split(PREC)) {
       u16 hh, ll;
@@ -1013,11 +1019,11 @@ __global__ __launch_bounds__(256) void factored_planes_kernel(int64_t P, int64_t
 
 hipError_t launch_factored_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
                                   int64_t R, const double* log_delta, void* qa, void* qb, float* qs, int64_t hi_from,
-                                  hipStream_t stream) {
+                                  const int* pos, hipStream_t stream) {
   const dim3 grid((unsigned)(p_pad / 8));
 #define FLMM_FPL(PR)                                                                                                 \
   hipLaunchKernelGGL((factored_planes_kernel<PR>), grid, dim3(256), 0, stream, P, p_pad, rpad, t, lam, R, log_delta, \
-                     qa, qb, qs, hi_from)
+                     qa, qb, qs, hi_from, pos)
   if (prec == PREC_SPLIT_BF16) FLMM_FPL(PREC_SPLIT_BF16);
   else if (prec == PREC_SPLIT_FP16) FLMM_FPL(PREC_SPLIT_FP16);
   else FLMM_FPL(PREC_FP32);
@@ -1061,7 +1067,7 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
                                double lo, double hi, const int* idx, const double* lam64,
                                double* log_delta, double* rWr, double* h2, void* q1a, void* q1b, void* q2a,
                                void* q2b, float* q1s, float* q2s, double radius, double judge_tol, int* flags,
-                               int64_t hi_from, hipStream_t stream) {
+                               int64_t hi_from, double ld_uniform, void* order_ws, hipStream_t stream) {
   const dim3 grid((unsigned)((p_pad + 3) / 4));
   // columns that fit 16 elements per lane (neff, nlam, kpad <= 1024) keep them in registers
   const bool regs = neff <= 1024 && nlam <= 1024 && kpad <= 1024;
@@ -1094,10 +1100,17 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
 #undef FLMM_REFINE1
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  // the screened columns' layout from the fitted log delta: those at ld_uniform first
+  const int* pos = nullptr;
+  if (order_ws && hi_from < P) {
+    e = launch_screen_classify(log_delta + hi_from, P - hi_from, ld_uniform, order_ws, stream);
+    if (e != hipSuccess) return e;
+    pos = screen_order(order_ws, P - hi_from).pos;
+  }
   const dim3 pgrid((unsigned)(p_pad / 32));
 #define FLMM_PLANES(PR)                                                                                              \
   hipLaunchKernelGGL((delta_planes_kernel<PR>), pgrid, dim3(256), 0, stream, uy, P, p_pad, kpad, s, neff, log_delta, \
-                     q1s, q2s, q1a, q1b, q2a, q2b, hi_from)
+                     q1s, q2s, q1a, q1b, q2a, q2b, hi_from, pos)
   if (prec == PREC_SPLIT_BF16) FLMM_PLANES(PREC_SPLIT_BF16);
   else if (prec == PREC_SPLIT_FP16) FLMM_PLANES(PREC_SPLIT_FP16);
   else FLMM_PLANES(PREC_FP32);

@@ -43,7 +43,8 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
                                double lo, double hi, const int* idx, const double* lam64, double* log_delta,
                                double* rWr, double* h2,
                                void* q1a, void* q1b, void* q2a, void* q2b, float* q1s, float* q2s, double radius,
-                               double judge_tol, int* flags, int64_t hi_from, hipStream_t stream);
+                               double judge_tol, int* flags, int64_t hi_from, double ld_uniform, void* order_ws,
+                               hipStream_t stream);
 hipError_t launch_gemm_f64(const double* x2, int64_t rows, int64_t kpad, const double* A, int64_t rpad, double* C,
                            hipStream_t stream);
 hipError_t launch_rescore(const double* US, const double* UY, int64_t ld, const double* s, int64_t neff,
@@ -61,7 +62,7 @@ hipError_t launch_gram_f64(const double* Z, int64_t ldz, int64_t n, int64_t m, i
                            double* out, int64_t ld, hipStream_t stream);
 hipError_t launch_factored_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
                                   int64_t R, const double* log_delta, void* qa, void* qb, float* qs, int64_t hi_from,
-                                  hipStream_t stream);
+                                  const int* pos, hipStream_t stream);
 template <int PREC>
 hipError_t launch_wald_scan(const TileArgs<PREC, 2>& args, const ScanScales& sc, int64_t Mc, int64_t P,
                             int64_t n_real, const double* rWr, const double* h2, double nu, const int64_t* row_index,
@@ -86,7 +87,51 @@ struct ScreenArgs {
   int64_t b_rows_pad;
   int nchunks;         // K / 32 of the first product (sWy)
   int nchunks2;        // K / 32 of the second (sWs): < nchunks for factored weights, == nchunks for exact ones
+  const int* order;    // NULL: columns in plane order; else the screened columns' layout (ScreenOrder): the tile
+  const int* counts;   // runs plane slots counts[0] .. counts[0] + counts[1] - 1, column order[slot]
 };
+// The screened columns' layout when the uniform-delta split is on (int32 workspace of screen_order_ints(n) ints, n
+// the screened columns): counts[0] columns at log delta == the uniform value take plane slots 0 .. counts[0] - 1,
+// the counts[1] others the slots after, each group in column order; order[slot] = column, pos[column] = slot
+// (columns and slots counted from the first screened column).
+struct ScreenOrder {
+  int* counts;
+  int* order;
+  int* pos;
+  int* blk;
+};
+inline int64_t screen_order_up64(int64_t n) { return (n + 63) / 64 * 64; }
+inline int64_t screen_order_ints(int64_t n) {
+  return 64 + 2 * screen_order_up64(n) + screen_order_up64((n + 1023) / 1024);
+}
+inline ScreenOrder screen_order(void* ws, int64_t n) {
+  int* p = (int*)ws;
+  ScreenOrder o;
+  o.counts = p;
+  o.order = p + 64;
+  o.pos = o.order + screen_order_up64(n);
+  o.blk = o.pos + screen_order_up64(n);
+  return o;
+}
+// The uniform-delta screen: columns whose log delta equals one value share the sWs of every SNP row (d_m, exact in
+// FP64), so only the sWy product runs on MFMA.  row_f[m] = 1 / (s1_m^2 d_m) (< 0: the row is a forced candidate).
+struct UniformArgs {
+  const u16* s1;       // SNP-side hi planes of the first product, rows a_rows_pad
+  const u16* q1;       // column-side hi planes from the first screened column on (row stride b_rows_pad)
+  const float* q2s;    // the columns' second-product scales (from the first screened column on)
+  const float* row_f;  // per SNP row (a_rows_pad entries)
+  const int* order;    // the screened columns' layout (ScreenOrder): the tile runs slots 0 .. counts[0] - 1
+  const int* counts;
+  int64_t a_rows_pad;
+  int64_t b_rows_pad;
+  int nchunks;
+};
+hipError_t launch_screen_classify(const double* log_delta, int64_t n, double ld_uniform, void* order_ws,
+                                  hipStream_t stream);
+hipError_t launch_screen_rows(const double* US, int64_t ld_us, int64_t mc, int64_t mc_pad, const double* s,
+                              int64_t neff, double ld_uniform, const float* s1s, float* row_f, hipStream_t stream);
+hipError_t launch_screen_uniform(const UniformArgs& g, int64_t Mc, int64_t P, uint64_t* rec, int64_t rec_ld,
+                                 hipStream_t stream);
 hipError_t launch_wald_screen(const ScreenArgs& g, int64_t Mc, int64_t P, float dmin, uint64_t* rec, int64_t rec_ld,
                               hipStream_t stream);
 hipError_t launch_screen_resolve(const uint64_t* rec, int64_t rec_ld, int64_t Mc, int64_t P, int64_t p_first,

@@ -63,6 +63,9 @@ __device__ inline void screen_tile(const ScreenArgs& g, int64_t n_mtiles, int64_
   const int64_t mt = first_m + r_in % gm;
   const int64_t pt = r_in / gm;
   const int64_t m0 = mt * W_TM, p0 = pt * W_TP;
+  // with a layout, the tile's columns are plane slots slot0 + j (n_ptiles covers the n_cols of them)
+  const int64_t n_cols = g.order ? (int64_t)gmem(g.counts)[1] : P;
+  const int64_t slot0 = g.order ? (int64_t)gmem(g.counts)[0] : 0;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int h = lane >> 5;
@@ -89,12 +92,18 @@ __device__ inline void screen_tile(const ScreenArgs& g, int64_t n_mtiles, int64_
   // host bounds rows_pad * 64 bytes below 2^31)
   const char* const a_base = (const char*)g.s1 + m0 * 64;
   const char* const a_base2 = (const char*)g.s2 + m0 * 64;
-  const char* const b_base = (const char*)g.q1 + p0 * 64;
-  const char* const b_base2 = (const char*)g.q2 + p0 * 64;
+  const char* const b_base = (const char*)g.q1;
+  const char* const b_base2 = (const char*)g.q2;
   const uint32_t a_lane = wave * 2048 + lane * 16;
   const int a_row0 = wave * 32 + lane / 4;
   // column side: lane (column p0 + 64 wave + 32 bj + lane % 32, half h) reads bytes [32 h, 32 h + 32) of its row
-  const uint32_t b_lane = (wave * 64 + (lane & 31)) * 64 + h * 32;
+  // (columns past n_cols read the last one; their records are not written)
+  uint32_t b_off[2];
+#pragma unroll
+  for (int bj = 0; bj < 2; ++bj) {
+    const int64_t j = p0 + wave * 64 + bj * 32 + (lane & 31);
+    b_off[bj] = (uint32_t)((slot0 + (j < n_cols ? j : n_cols - 1)) * 64 + h * 32);
+  }
 
   // uniform chunk base of slot i of stage j, SNP side (a) and column side (b)
   auto slot_base = [&](int j, int i, bool a_side) -> const char* {
@@ -122,7 +131,7 @@ __device__ inline void screen_tile(const ScreenArgs& g, int64_t n_mtiles, int64_
 #pragma unroll
       for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-        for (int q = 0; q < 2; ++q) b.v[i][bj][q] = *gmem((const u32x4*)(sb + (b_lane + (bj * 32 * 64 + q * 16))));
+        for (int q = 0; q < 2; ++q) b.v[i][bj][q] = *gmem((const u32x4*)(sb + (b_off[bj] + q * 16)));
     }
     return b;
   };
@@ -275,17 +284,363 @@ __device__ inline void screen_tile(const ScreenArgs& g, int64_t n_mtiles, int64_
     __builtin_amdgcn_wave_barrier();
   }
   const int64_t pc = p0 + wave * 64 + lane;
-  if (pc < P) {
+  if (pc < n_cols) {
     // high word: m2 rounded down to 128 units in the last place, bit 0 = forced
     const uint32_t x2 = (m2 & ~127u) | (dlo < dmin_b ? 1u : 0u);
-    gmem(rec)[mt * rec_ld + pc] = ((uint64_t)x2 << 32) | m1;
+    const int64_t col = g.order ? (int64_t)gmem(g.order)[slot0 + pc] : pc;
+    gmem(rec)[mt * rec_ld + col] = ((uint64_t)x2 << 32) | m1;
   }
 }
 
-__global__ __launch_bounds__(256, 1) void wald_screen_kernel(ScreenArgs g, int64_t n_mtiles, int64_t n_ptiles,
-                                                             int64_t n_tiles, int64_t Mc, int64_t P, float dmin,
-                                                             uint64_t* __restrict__ rec, int64_t rec_ld) {
-  screen_tile(g, n_mtiles, n_ptiles, xcd_remap(blockIdx.x, n_tiles), Mc, P, dmin, rec, rec_ld);
+// The grid covers every column; with a layout only the first n_mtiles x ceil(count / 256) blocks have a tile
+// (the rest exit), so the live tiles spread evenly over the XCDs (blocks go to XCDs round robin) and xcd_remap over
+// the live count keeps each XCD's tiles adjacent.
+__device__ inline int64_t live_ptiles(const int* count, int64_t P) {
+  const int64_t n = count ? (int64_t)*gmem(count) : P;
+  return (n + W_TP - 1) / W_TP;
+}
+
+__global__ __launch_bounds__(256, 1) void wald_screen_kernel(ScreenArgs g, int64_t n_mtiles, int64_t Mc, int64_t P,
+                                                             float dmin, uint64_t* __restrict__ rec, int64_t rec_ld) {
+  const int64_t n_pt = live_ptiles(g.order ? g.counts + 1 : nullptr, P);
+  const int64_t n_live = n_mtiles * n_pt;
+  if ((int64_t)blockIdx.x >= n_live) return;
+  screen_tile(g, n_mtiles, n_pt, xcd_remap(blockIdx.x, n_live), Mc, P, dmin, rec, rec_ld);
+}
+
+// ---- uniform-delta screen ----
+// Permutation columns with h2 ~ 0 mostly settle at the grid's upper end, all at the same log delta.  For such columns
+// the weights w_k = 1 / (s_k + delta) are shared, so sWs of SNP row m is one number d_m (computed in FP64 by
+// screen_rows_kernel), and the tile runs the sWy product alone: 128 accumulators per lane, so two workgroups share
+// each CU and one's epilogue and load waits overlap the other's MFMAs.  With A the raw hi-plane sum (s1_m q1_p a),
+//   x = A^2 f_m / q2_p,   f_m = 1 / (s1_m^2 d_m)
+// is the general tile's x = q1^2 a^2 / (q2 d) with d exact (rounded once to FP32 in f_m, well inside kD), so the
+// records mean the same and one resolution serves both.  1 / q2_p is a power of two applied to the packed maxima
+// at the end (exact; a result outside the normal range marks the record forced).
+constexpr int SU_SLOTS = 2;                        // K-chunks per stage
+constexpr int SU_STAGE = SU_SLOTS * SC_SLOT_LDS;   // 16 KiB
+
+__global__ __launch_bounds__(256, 2) void screen_uniform_kernel(UniformArgs g, int64_t n_mtiles,
+                                                                uint64_t* __restrict__ rec, int64_t rec_ld) {
+  __shared__ __attribute__((aligned(16))) char lds0[SU_STAGE];
+  __shared__ __attribute__((aligned(16))) char lds1[SU_STAGE];
+  __shared__ __attribute__((aligned(16))) float row_f[W_TM];
+  __shared__ int tile_forced;
+
+  const int64_t n_cols = gmem(g.counts)[0];
+  const int64_t n_ptiles = (n_cols + W_TP - 1) / W_TP;
+  const int64_t n_live = n_mtiles * n_ptiles;
+  if ((int64_t)blockIdx.x >= n_live) return;
+  const int64_t tile = xcd_remap(blockIdx.x, n_live);
+  const int64_t group = tile / (W_GM * n_ptiles);
+  const int64_t first_m = group * W_GM;
+  const int64_t gm = n_mtiles - first_m < W_GM ? n_mtiles - first_m : W_GM;
+  const int64_t r_in = tile - group * (W_GM * n_ptiles);
+  const int64_t mt = first_m + r_in % gm;
+  const int64_t pt = r_in / gm;
+  const int64_t m0 = mt * W_TM, p0 = pt * W_TP;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int h = lane >> 5;
+
+  if (threadIdx.x == 0) tile_forced = 0;
+  __syncthreads();
+  if (threadIdx.x < W_TM) {
+    const float f = gmem(g.row_f)[m0 + threadIdx.x];
+    row_f[threadIdx.x] = f > 0.f ? f : 0.f;
+    if (f < 0.f) tile_forced = 1;
+  }
+
+  const int64_t a_kstride = g.a_rows_pad * 64;
+  const int64_t b_kstride = g.b_rows_pad * 64;
+  const char* const a_base = (const char*)g.s1 + m0 * 64;
+  const char* const b_base = (const char*)g.q1;
+  // SNP side: thread t copies 16-byte segment t % 4 of rows t / 4 and 64 + t / 4 of every chunk
+  const uint32_t a_lane = threadIdx.x * 16;
+  const int a_row0 = threadIdx.x >> 2;
+  uint32_t b_off[2];  // the uniform columns hold plane slots 0 .. n_cols - 1
+#pragma unroll
+  for (int bj = 0; bj < 2; ++bj) {
+    const int64_t j = p0 + wave * 64 + bj * 32 + (lane & 31);
+    b_off[bj] = (uint32_t)((j < n_cols ? j : n_cols - 1) * 64 + h * 32);
+  }
+
+  struct ASet { u32x4 v[SU_SLOTS][2]; };
+  struct BSet { u32x4 v[SU_SLOTS][2][2]; };
+  auto load_a = [&](int j) {
+    ASet a;
+#pragma unroll
+    for (int i = 0; i < SU_SLOTS; ++i) {
+      const char* sa = a_base + (int64_t)(j * SU_SLOTS + i) * a_kstride;
+      a.v[i][0] = *gmem((const u32x4*)(sa + a_lane));
+      a.v[i][1] = *gmem((const u32x4*)(sa + (a_lane + 4096)));
+    }
+    return a;
+  };
+  auto load_b = [&](int j, BSet& b) {
+#pragma unroll
+    for (int i = 0; i < SU_SLOTS; ++i) {
+      const char* sb = b_base + (int64_t)(j * SU_SLOTS + i) * b_kstride;
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) b.v[i][bj][q] = *gmem((const u32x4*)(sb + (b_off[bj] + q * 16)));
+    }
+  };
+  auto store = [&](const ASet& a, char* st) {
+#pragma unroll
+    for (int i = 0; i < SU_SLOTS; ++i)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int r = a_row0 + b * 64;
+        *(u32x4*)(st + i * SC_SLOT_LDS + r * 64 + (((threadIdx.x & 3) ^ sc_swz(r)) * 16)) = a.v[i][b];
+      }
+  };
+
+  f32x16 acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+
+  auto mma = [&](const char* st, const BSet& b) {
+#pragma unroll
+    for (int i = 0; i < SU_SLOTS; ++i)
+#pragma unroll
+      for (int step = 0; step < 2; ++step) {
+        const int seg = h * 2 + step;
+        bf16x8 af[4];
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi) {
+          const int r = bi * 32 + (lane & 31);
+          af[bi] = *(const bf16x8*)(st + i * SC_SLOT_LDS + r * 64 + ((seg ^ sc_swz(r)) * 16));
+        }
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+          const bf16x8 bh = __builtin_bit_cast(bf16x8, b.v[i][bj][step]);
+#pragma unroll
+          for (int bi = 0; bi < 4; ++bi) acc[bi][bj] = mfma_split<PREC_SPLIT_FP16>(af[bi], bh, acc[bi][bj]);
+        }
+      }
+    // interleave as the general tile: the first slot-step's 4 fragment reads up front, then each later slot-step's
+    // 4 reads and 4 of the stage's global loads over the previous one's 8 MFMAs
+    constexpr int NLOAD = SU_SLOTS * 6;  // A: 2 per slot, B: 4 per slot
+    constexpr int PER = NLOAD / (2 * SU_SLOTS - 1) > 0 ? NLOAD / (2 * SU_SLOTS - 1) : 1;
+    __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+#pragma unroll
+    for (int ss = 0; ss < 2 * SU_SLOTS - 1; ++ss) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (r < PER) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+      }
+    }
+    __builtin_amdgcn_sched_group_barrier(0x020, NLOAD - PER * (2 * SU_SLOTS - 1), 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
+  };
+  const int nst = g.nchunks / SU_SLOTS;
+  auto stage = [&](int j, const char* cur, char* nxt, const BSet& bc, BSet& bn) {
+    const int jn = j + 1 < nst ? j + 1 : j;  // past the last stage the prefetch repeats it (valid addresses)
+    const ASet an = load_a(jn);
+    load_b(jn, bn);
+    mma(cur, bc);
+    __builtin_amdgcn_sched_barrier(0);
+    store(an, nxt);  // nxt was last read in the previous stage, which ended with a barrier
+    __syncthreads();
+  };
+
+  BSet b0, b1;
+  load_b(0, b0);
+  store(load_a(0), lds0);
+  __syncthreads();
+  for (int j = 0; j < nst; j += 2) {  // nchunks is a multiple of 4: stages pair up
+    stage(j, lds0, lds1, b0, b1);
+    stage(j + 1, lds1, lds0, b1, b0);
+  }
+
+  // epilogue from the accumulators: lane (h, c) holds column 32 bj + c of its wave and rows 32 bi + 8 q + 4 h + i;
+  // x = A^2 f_m, packed as in the general tile ((bits(x) & ~127) | row), top two per column over the lane's 64
+  // rows, then merged with lane c + 32 (the column's other 64 rows)
+  uint32_t m1[2] = {0u, 0u}, m2[2] = {0u, 0u};
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r0 = bi * 32 + 8 * q + 4 * h;
+      const f32x4 f4 = *(const f32x4*)&row_f[r0];
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float a = acc[bi][bj][4 * q + e];
+          const float x = (a * a) * f4[e];
+          const uint32_t xp = (__float_as_uint(x) & ~127u) | (uint32_t)(r0 + e);
+          const uint32_t lo = xp < m1[bj] ? xp : m1[bj];
+          m2[bj] = lo > m2[bj] ? lo : m2[bj];
+          m1[bj] = xp > m1[bj] ? xp : m1[bj];
+        }
+    }
+#pragma unroll
+  for (int bj = 0; bj < 2; ++bj) {
+    const uint32_t o1 = (uint32_t)__shfl_xor((int)m1[bj], 32, 64), o2 = (uint32_t)__shfl_xor((int)m2[bj], 32, 64);
+    const uint32_t lo = m1[bj] < o1 ? m1[bj] : o1;
+    const uint32_t hi2 = m2[bj] > o2 ? m2[bj] : o2;
+    m2[bj] = lo > hi2 ? lo : hi2;
+    m1[bj] = m1[bj] > o1 ? m1[bj] : o1;
+  }
+  // lane (h, c) writes column 32 h + c of its wave
+  const uint32_t w1 = h ? m1[1] : m1[0], w2 = h ? m2[1] : m2[0];
+  const int64_t pc = p0 + wave * 64 + lane;
+  if (pc < n_cols) {
+    const int64_t col = gmem(g.order)[pc];
+    // 1 / q2 = 2^e: add e to the exponent of both packed values (the low 7 bits carry the row / flag)
+    const int e = 127 - (int)((__float_as_uint(gmem(g.q2s)[col]) >> 23) & 255u);
+    bool bad = tile_forced != 0;
+    auto scale = [&](uint32_t v) -> uint32_t {
+      if ((v & ~127u) == 0u) return v & 127u;
+      const int ex = (int)((v >> 23) & 255u);
+      if (ex == 0 || ex == 255 || ex + e <= 0 || ex + e >= 255) {
+        bad = true;
+        return v;
+      }
+      return v + ((uint32_t)e << 23);
+    };
+    const uint32_t s1v = scale(w1);
+    const uint32_t s2v = scale(w2 & ~127u);
+    const uint32_t x2 = (s2v & ~127u) | (bad ? 1u : 0u);
+    gmem(rec)[mt * rec_ld + col] = ((uint64_t)x2 << 32) | s1v;
+  }
+}
+
+// d_m = sum_k US[m][k]^2 / (s_k + delta) in FP64 at delta = exp(ld_uniform) (the resolution's arithmetic), one wave
+// per row; row_f[m] = 1 / (s1_m^2 d_m) rounded to FP32, -1 where d_m is not positive or f overflows (a forced
+// row), 0 on the padding rows mc <= m < mc_pad.
+__global__ __launch_bounds__(256) void screen_rows_kernel(const double* __restrict__ US, int64_t ld_us, int64_t mc,
+                                                          int64_t mc_pad, const double* __restrict__ s, int64_t neff,
+                                                          double ld_uniform, const float* __restrict__ s1s,
+                                                          float* __restrict__ row_f) {
+  const int64_t m = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (m >= mc_pad) return;
+  if (m >= mc) {
+    if (lane == 0) row_f[m] = 0.f;
+    return;
+  }
+  const double delta = exp(ld_uniform);
+  const double* u = US + m * ld_us;
+  double d = 0.0;
+  for (int64_t k = lane; k < neff; k += 64) {
+    const double uk = gmem(u)[k];
+    d += uk * uk * rcp_pos(gmem(s)[k] + delta);
+  }
+  d = wave_sum(d);
+  if (lane == 0) {
+    const double sc = (double)gmem(s1s)[m];
+    const double f = 1.0 / (sc * sc * d);
+    row_f[m] = (d > 0.0 && f < 0x1p100) ? (float)f : -1.f;
+  }
+}
+
+// The screened columns' layout (ScreenOrder): a stable split into those at log delta == ld_uniform and the rest,
+// by per-block counts, one block's exclusive scan of them, then each block's ranks by wave ballots.
+constexpr int CL_BLOCK = 1024;
+
+__global__ __launch_bounds__(CL_BLOCK) void screen_count_kernel(const double* __restrict__ ld, int64_t n, double lu,
+                                                               int* __restrict__ blk) {
+  __shared__ int wc[CL_BLOCK / 64];
+  const int64_t i = (int64_t)blockIdx.x * CL_BLOCK + threadIdx.x;
+  const bool u = i < n && gmem(ld)[i] == lu;
+  const uint64_t b = __ballot(u);
+  if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(b);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < CL_BLOCK / 64; ++w) t += wc[w];
+    blk[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(CL_BLOCK) void screen_offsets_kernel(int* __restrict__ blk, int64_t nb, int64_t n,
+                                                                 int* __restrict__ counts) {
+  __shared__ int part[CL_BLOCK];
+  const int64_t per = (nb + CL_BLOCK - 1) / CL_BLOCK;
+  const int64_t b0 = threadIdx.x * per, b1 = b0 + per < nb ? b0 + per : nb;
+  int t = 0;
+  for (int64_t b = b0; b < b1; ++b) t += blk[b];
+  part[threadIdx.x] = t;
+  __syncthreads();
+  for (int off = 1; off < CL_BLOCK; off <<= 1) {  // inclusive scan (Hillis-Steele)
+    const int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+  for (int64_t b = b0; b < b1; ++b) {
+    const int c = blk[b];
+    blk[b] = run;
+    run += c;
+  }
+  if (threadIdx.x == CL_BLOCK - 1) {
+    counts[0] = part[CL_BLOCK - 1];
+    counts[1] = (int)(n - part[CL_BLOCK - 1]);
+  }
+}
+
+__global__ __launch_bounds__(CL_BLOCK) void screen_scatter_kernel(const double* __restrict__ ld, int64_t n, double lu,
+                                                                 const int* __restrict__ blk,
+                                                                 const int* __restrict__ counts,
+                                                                 int* __restrict__ order, int* __restrict__ pos) {
+  __shared__ int wc[CL_BLOCK / 64];
+  const int64_t i = (int64_t)blockIdx.x * CL_BLOCK + threadIdx.x;
+  const bool u = i < n && gmem(ld)[i] == lu;
+  const uint64_t b = __ballot(u);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) wc[w] = __popcll(b);
+  __syncthreads();
+  int before = 0;  // uniform columns in this block's earlier waves
+  for (int v = 0; v < w; ++v) before += wc[v];
+  const uint64_t below = lane ? (b & ((1ull << lane) - 1ull)) : 0ull;
+  const int ur = before + __popcll(below);  // rank among the block's uniform columns
+  const int gr = w * 64 + lane - ur;        // rank among its other columns
+  if (i < n) {
+    const int ub = blk[blockIdx.x];  // uniform columns in earlier blocks
+    const int slot = u ? ub + ur : gmem(counts)[0] + (int)((int64_t)blockIdx.x * CL_BLOCK - ub) + gr;
+    order[slot] = (int)i;
+    pos[i] = slot;
+  }
+}
+
+hipError_t launch_screen_classify(const double* log_delta, int64_t n, double ld_uniform, void* order_ws,
+                                  hipStream_t stream) {
+  const ScreenOrder o = screen_order(order_ws, n);
+  const int64_t nb = (n + CL_BLOCK - 1) / CL_BLOCK;
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL(screen_count_kernel, dim3((unsigned)nb), dim3(CL_BLOCK), 0, stream, log_delta, n, ld_uniform,
+                     o.blk);
+  hipLaunchKernelGGL(screen_offsets_kernel, dim3(1), dim3(CL_BLOCK), 0, stream, o.blk, nb, n, o.counts);
+  hipLaunchKernelGGL(screen_scatter_kernel, dim3((unsigned)nb), dim3(CL_BLOCK), 0, stream, log_delta, n, ld_uniform,
+                     o.blk, o.counts, o.order, o.pos);
+  return hipGetLastError();
+}
+
+hipError_t launch_screen_rows(const double* US, int64_t ld_us, int64_t mc, int64_t mc_pad, const double* s,
+                              int64_t neff, double ld_uniform, const float* s1s, float* row_f, hipStream_t stream) {
+  hipLaunchKernelGGL(screen_rows_kernel, dim3((unsigned)((mc_pad + 3) / 4)), dim3(256), 0, stream, US, ld_us, mc,
+                     mc_pad, s, neff, ld_uniform, s1s, row_f);
+  return hipGetLastError();
+}
+
+hipError_t launch_screen_uniform(const UniformArgs& g, int64_t Mc, int64_t P, uint64_t* rec, int64_t rec_ld,
+                                 hipStream_t stream) {
+  const int64_t n_mtiles = (Mc + W_TM - 1) / W_TM;
+  const int64_t n_tiles = n_mtiles * ((P + W_TP - 1) / W_TP);
+  hipLaunchKernelGGL(screen_uniform_kernel, dim3((unsigned)n_tiles), dim3(256), 0, stream, g, n_mtiles, rec, rec_ld);
+  return hipGetLastError();
 }
 
 __device__ inline uint64_t readlane64(uint64_t v, int j) {
@@ -405,7 +760,9 @@ __global__ __launch_bounds__(256) void screen_resolve_kernel(
     };
     for (uint64_t b = k1; b; b &= b - 1) {
       const int64_t t = __builtin_ctzll(b);
-      eval(t * W_TM + (int64_t)(gmem(rec)[t * rec_ld + ij] & 127u));
+      // a padding row can hold the packed maximum only where every row of the tile has x = 0; it is no test
+      const int64_t m = t * W_TM + (int64_t)(gmem(rec)[t * rec_ld + ij] & 127u);
+      if (m < Mc) eval(m);
     }
     for (uint64_t b = k2; b; b &= b - 1) {
       const int64_t t = __builtin_ctzll(b);
@@ -420,10 +777,9 @@ __global__ __launch_bounds__(256) void screen_resolve_kernel(
 hipError_t launch_wald_screen(const ScreenArgs& g, int64_t Mc, int64_t P, float dmin, uint64_t* rec, int64_t rec_ld,
                               hipStream_t stream) {
   const int64_t n_mtiles = (Mc + W_TM - 1) / W_TM;
-  const int64_t n_ptiles = (P + W_TP - 1) / W_TP;
-  const int64_t n_tiles = n_mtiles * n_ptiles;
-  hipLaunchKernelGGL(wald_screen_kernel, dim3((unsigned)n_tiles), dim3(256), 0, stream, g, n_mtiles, n_ptiles,
-                     n_tiles, Mc, P, dmin, rec, rec_ld);
+  const int64_t n_tiles = n_mtiles * ((P + W_TP - 1) / W_TP);
+  hipLaunchKernelGGL(wald_screen_kernel, dim3((unsigned)n_tiles), dim3(256), 0, stream, g, n_mtiles, Mc, P, dmin,
+                     rec, rec_ld);
   return hipGetLastError();
 }
 

@@ -41,6 +41,8 @@ SIGNATURES: dict[str, list] = {
     "flmm_abi_version": [],
     "flmm_last_error": [],
     "flmm_scan_table_bytes": [_I],
+    "flmm_screen_work_bytes": [_I64],
+    "flmm_screen_order_bytes": [_I64],
     "flmm_scan_table_fill": [_I, _I, _P, _I, _P],
     "flmm_wald_scan_grouped": [_I, _I, _P, _I, _P],
     "flmm_max_reduce_grouped": [_P, _I, _I64, _P],
@@ -53,26 +55,26 @@ SIGNATURES: dict[str, list] = {
     "flmm_delta_grid": [_P, _P, _I, _D, _P, _I64, _I64, _I64, _P, _P],
     "flmm_delta_grid_split": [_P, _P, _I, _D, _P, _P, _D, _D, _D, _I64, _I64, _I64, _P, _P],
     "flmm_delta_refine": [_I, _I, _P, _I64, _I64, _I64, _P, _I64, _P, _I64, _D, _I, _D, _D, _P, _P, _P, _P, _P,
-                          _P, _P, _P, _P, _P, _P, _I64, _P],
+                          _P, _P, _P, _P, _P, _P, _I64, _D, _P, _P],
     "flmm_gram": [_I, _P, _P, _P, _I64, _I64, _I64, _I, _P, _P, _I64, _P],
     "flmm_gram_f64": [_P, _I64, _I64, _I64, _I, _P, _P, _I64, _P],
     "flmm_rotate_f64": [_P, _I64, _P, _I64, _I64, _P, _I64, _I64, _P, _I64, _I64, _P],
     "flmm_gemm_f64": [_P, _I64, _I64, _P, _I64, _P, _P],
     "flmm_rescore_f64": [_P, _P, _I64, _P, _I64, _P, _P, _P, _P, _I64, _D, _P, _P, _P, _P, _I64, _P],
-    "flmm_factored_weight_planes": [_I, _I64, _I64, _I64, _P, _P, _I64, _P, _P, _P, _P, _I64, _P],
+    "flmm_factored_weight_planes": [_I, _I64, _I64, _I64, _P, _P, _I64, _P, _P, _P, _P, _I64, _P, _P],
     "flmm_delta_refine_learned": [_I, _P, _I64, _I64, _I64, _P, _I64, _P, _I64, _D, _I, _D, _D, _P, _P, _D, _D, _P,
-                                  _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P],
+                                  _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _D, _P, _P],
     "flmm_mlp_features": [_P, _I64, _I64, _I64, _P, _D, _D, _P, _P],
     "flmm_mlp_dense_fwd": [_P, _I64, _I, _P, _P, _I, _I, _P, _P],
     "flmm_mlp_dense_bwd": [_P, _P, _P, _P, _I64, _I, _I, _I, _P, _P, _P, _P, _P],
     "flmm_mlp_mse_grad": [_P, _P, _I64, _P, _P],
     "flmm_adam": [_P, _P, _P, _P, _I64, _D, _D, _D, _D, _I, _P],
     "flmm_wald_scan_screened": [_P, _P, _P, _P, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _I64,
-                                _P, _P, _I64, _D, _D, _P, _P, _D, _P, _P, _P, _P],
+                                _P, _P, _I64, _D, _D, _P, _P, _D, _P, _P, _D, _P, _P, _P, _P],
     "flmm_wald_screen": [_P, _P, _P, _P, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _I64, _P, _P,
-                         _I64, _D, _D, _P, _P, _D, _P, _P, _P, _P],
+                         _I64, _D, _D, _P, _P, _D, _P, _P, _D, _P, _P, _P, _P],
     "flmm_screen_resolve": [_P, _P, _P, _P, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _I64, _P,
-                            _P, _I64, _D, _D, _P, _P, _D, _P, _P, _P, _P],
+                            _P, _I64, _D, _D, _P, _P, _D, _P, _P, _D, _P, _P, _P, _P],
     "flmm_wald_scan": [_I, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P,
                        _P, _D, _P, _P, _P, _P, _P, _P, _P, _P],
     "flmm_wald_scan_lowrank": [_I, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64,
@@ -85,7 +87,8 @@ SIGNATURES: dict[str, list] = {
 }
 
 # entry points that do not return an int status
-RESTYPES = {"flmm_last_error": C.c_char_p, "flmm_scan_table_bytes": C.c_int64}
+RESTYPES = {"flmm_last_error": C.c_char_p, "flmm_scan_table_bytes": C.c_int64, "flmm_screen_work_bytes": C.c_int64,
+            "flmm_screen_order_bytes": C.c_int64}
 
 
 class ScanGroup(C.Structure):

@@ -17,6 +17,7 @@ with N_pad = roundup(N, 32), K_pad = roundup(Neff, 128), P_pad = roundup(P, 128)
 from __future__ import annotations
 
 import contextlib
+import math
 from dataclasses import dataclass
 
 import numpy as np
@@ -52,7 +53,9 @@ class Settings:
                                    # scales (split_fp16), "fp32" = FP32 MFMA
     screen: str = "auto"           # permutation columns' max_F: "auto" = the certified hi-plane screen with FP64
                                    # re-evaluation of the tests that can hold each maximum (split_fp16 with the
-                                   # FP64 SNP rotation), "off" = the full split scan for every column
+                                   # FP64 SNP rotation), the columns at the grid's upper log delta on the sWy-only
+                                   # tile; "general" = the same screen without that split; "off" = the full split
+                                   # scan for every column
     pipeline: bool = False         # LOCO: prepare chromosome c + 1 (basis, rotation, delta) on a second stream
                                    # while chromosome c scans (same bits; ~1% less step time on the N=1000,
                                    # M=100k workload, but the co-running kernels slow the scan kernel ~1.7x)
@@ -284,6 +287,8 @@ class DeltaResult:
     neff: int = 0
     fac: Factored | None = None
     hi_from: int | None = None  # columns from here on have the hi planes only (the screen's); None: all have both
+    ld_uniform: float | None = None  # a log delta many columns hold exactly (the fit grid's upper end), or None
+    order: torch.Tensor | None = None  # the screened columns' plane layout (uniform-delta split), or None
 
 
 def attach_factored(prec: int, d: DeltaResult, fac: Factored) -> None:
@@ -293,7 +298,8 @@ def attach_factored(prec: int, d: DeltaResult, fac: Factored) -> None:
     q2 = Planes(prec, p_pad, fac.rpad, d.q1.hi.device)
     q2h, q2l = q2.ptrs
     _lib.call("flmm_factored_weight_planes", prec, P, p_pad, fac.rpad, fac.t.data_ptr(), fac.lam.data_ptr(), fac.R,
-              d.log_delta.data_ptr(), q2h, q2l, q2.scale_ptr, _hi_from_arg(d.hi_from, p_pad), _lib.stream())
+              d.log_delta.data_ptr(), q2h, q2l, q2.scale_ptr, _hi_from_arg(d.hi_from, p_pad), _lib.ptr(d.order),
+              _lib.stream())
     d.q2, d.fac = q2, fac
 
 
@@ -384,13 +390,26 @@ def screen_start(prec: int, n_real: int | None, P: int) -> int | None:
     return p0 if p0 < P else None
 
 
+def screen_order(hi_from: int | None, P: int, ld_uniform: float | None, dev) -> torch.Tensor | None:
+    """Workspace for the screened columns' plane layout (the screen's uniform-delta split, settings.screen "auto"),
+    or None where no column is screened or no uniform log delta is given."""
+    if hi_from is None or hi_from >= P or ld_uniform is None or not math.isfinite(ld_uniform):
+        return None
+    if settings.screen != "auto":
+        return None
+    nb = _lib.load().flmm_screen_order_bytes(P - hi_from)
+    return torch.empty(nb, dtype=torch.uint8, device=dev)
+
+
 def delta_refine(prec: int, mode: int, uy: torch.Tensor, P: int, p_pad: int, kpad: int, s: torch.Tensor,
                  s_lam: torch.Tensor, nco: float, n_grid: int, lo: float, hi: float, idx: torch.Tensor | None,
                  log_delta_in: torch.Tensor | None = None, fac: Factored | None = None,
-                 lam64: torch.Tensor | None = None, hi_from: int | None = None) -> DeltaResult:
+                 lam64: torch.Tensor | None = None, hi_from: int | None = None,
+                 ld_uniform: float | None = None) -> DeltaResult:
     """Refine (or, for DELTA_FIXED, take log_delta_in) and build the column-side scan planes (with fac, the
     factored Q2 instead of the exact one); lam64: delta_grid's FP64 log-det table for the same grid; hi_from: the
-    screen's first column (screen_start), from which only the hi planes are written."""
+    screen's first column (screen_start), from which only the hi planes are written; ld_uniform (default: hi for
+    the searching modes): the log delta whose columns the screen's uniform-delta split groups."""
     dev = uy.device
     log_delta = torch.empty(max(P, 1), dtype=torch.float64, device=dev)
     if log_delta_in is not None:
@@ -401,12 +420,17 @@ def delta_refine(prec: int, mode: int, uy: torch.Tensor, P: int, p_pad: int, kpa
     q2 = Planes(prec, p_pad, kpad, dev) if fac is None else None
     q1h, q1l = q1.ptrs
     q2h, q2l = q2.ptrs if q2 is not None else (None, None)
+    if ld_uniform is None and mode != _lib.DELTA_FIXED:
+        ld_uniform = float(hi)
+    order = screen_order(hi_from, P, ld_uniform, dev)
     _lib.call("flmm_delta_refine", prec, mode, uy.data_ptr(), P, p_pad, kpad, s.data_ptr(), s.numel(),
               s_lam.data_ptr(), s_lam.numel(), float(nco), n_grid, lo, hi, _lib.ptr(idx), _lib.ptr(lam64),
               log_delta.data_ptr(),
               rWr.data_ptr(), h2.data_ptr(), q1h, q1l, q2h, q2l, q1.scale_ptr, q2.scale_ptr if q2 else None,
-              _hi_from_arg(hi_from, p_pad), _lib.stream())
-    d = DeltaResult(log_delta[:P], rWr[:P], h2[:P], q1, q2, idx, int(s.numel()), hi_from=hi_from)
+              _hi_from_arg(hi_from, p_pad), float(ld_uniform) if order is not None else 0.0, _lib.ptr(order),
+              _lib.stream())
+    d = DeltaResult(log_delta[:P], rWr[:P], h2[:P], q1, q2, idx, int(s.numel()), hi_from=hi_from,
+                    ld_uniform=ld_uniform if order is not None else None, order=order)
     if fac is not None:
         attach_factored(prec, d, fac)
     return d
@@ -541,9 +565,9 @@ screen_stats = _ScreenStats()
 
 def screen_applies(prec: int) -> bool:
     """Whether scans of this precision screen their permutation columns (settings.screen)."""
-    if settings.screen not in ("auto", "off"):
+    if settings.screen not in ("auto", "general", "off"):
         raise ValueError(f"unknown screen setting {settings.screen!r}")
-    return settings.screen == "auto" and prec == _lib.PREC_SPLIT_FP16 and settings.snp_rotation == "fp64"
+    return settings.screen != "off" and prec == _lib.PREC_SPLIT_FP16 and settings.snp_rotation == "fp64"
 
 
 def wald_scan(s1: Planes, s2: Planes, d: DeltaResult, mc: int, P: int, n_real: int, nu: float,
@@ -607,11 +631,17 @@ def _screened(s1: Planes, s2: Planes, d: DeltaResult, mc: int, p_first: int, P: 
     kpad2 = d.fac.rpad if d.fac is not None else s1.kpad
     us = screen.us
     cnt = screen_stats.counter
+    # the uniform-delta split: the layout the refinement wrote for exactly these columns
+    order = d.order if (d.order is not None and d.hi_from == p_first) else None
+    lu = float(d.ld_uniform) if order is not None else float("nan")
+    work = None
+    if order is not None:
+        work = torch.empty(_lib.load().flmm_screen_work_bytes(s1.rows_pad), dtype=torch.uint8, device=rec.device)
     args = (s1.hi.data_ptr(), s2.hi.data_ptr(), s1.scale_ptr, s2.scale_ptr, s1.rows_pad, d.q1.hi.data_ptr(),
             d.q2.hi.data_ptr(), d.q1.scale_ptr, d.q2.scale_ptr, d.q1.rows_pad, s1.kpad, kpad2, mc, p_first, P,
             us.data_ptr(), us.stride(0), screen.uy.data_ptr(), screen.s.data_ptr(), screen.neff, float(screen.s_lo),
             float(screen.s_hi), d.log_delta.data_ptr(), d.rWr.data_ptr(), float(nu), out.max_F.data_ptr(),
-            rec.data_ptr(), _lib.ptr(cnt), _lib.stream())
+            rec.data_ptr(), lu, _lib.ptr(order), _lib.ptr(work), _lib.ptr(cnt), _lib.stream())
     if cnt is not None:
         screen_stats.tests += mc * (P - p_first)
     if not scan_timer.enabled:

@@ -178,12 +178,18 @@ int flmm_delta_grid_split(const float* wplane, const float* lam, int n_grid, dou
  * scales q1_scale / q2_scale (split-FP16); q2_hi == NULL skips Q2 (factored weights).  Columns hi_from <= p < P
  * (those flmm_wald_scan_screened reads: a multiple of 256 at or past the real columns) get the hi planes only,
  * evaluated in FP32; hi_from >= P writes both planes of every column.  The loss is nco * (log 2pi + log(R / nco) + 1)
- * + sum_j log(s_lam_j + delta). */
+ * + sum_j log(s_lam_j + delta).
+ * order (may be NULL; flmm_screen_order_bytes(P - hi_from) bytes of device workspace) lays the columns
+ * hi_from <= p < P out for the screen's uniform-delta split: those whose fitted log_delta equals ld_uniform exactly
+ * take plane rows hi_from, hi_from + 1, ... in column order, the others the rows after them; order records the
+ * layout (counts, slot -> column, column -> slot) for flmm_factored_weight_planes and flmm_wald_scan_screened. */
 int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_pad, int64_t kpad, const double* s,
                       int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo, double hi,
                       const int32_t* idx, const double* lam64, double* log_delta, double* rWr, double* h2,
                       void* q1_hi, void* q1_lo, void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale,
-                      int64_t hi_from, void* stream);
+                      int64_t hi_from, double ld_uniform, void* order, void* stream);
+/* Bytes of the layout workspace for n_cols screened columns (-1 on a bad size).  No counterpart in the reference. */
+int64_t flmm_screen_order_bytes(int64_t n_cols);
 
 /* Factored weights (no counterpart in the reference, whose sWs = w^T (US o US) runs over all Neff eigenvalues,
  * core.py:640-651).  With a positive exponential sum 1/z = sum_{r<R} lam_r^2 exp(-t_r z) (relative accuracy
@@ -194,10 +200,10 @@ int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_
  * S2 = planes of C, e.g. from FLMM_ROTATE_US_X2 and flmm_pack_planes).  This call writes B (delta_p =
  * exp(log_delta[p]), p < P) as the Q2 planes (rows p_pad, K = rpad, zero for r >= R and p >= P), split-FP16 rows
  * scaled by the power of two of their L2 norm (q2_scale); columns hi_from <= p < P get the hi plane only (as
- * flmm_delta_refine). */
+ * flmm_delta_refine), at the plane rows of the layout order (the one flmm_delta_refine wrote; NULL: row p). */
 int flmm_factored_weight_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
                                 int64_t R, const double* log_delta, void* q2_hi, void* q2_lo, float* q2_scale,
-                                int64_t hi_from, void* stream);
+                                int64_t hi_from, const void* order, void* stream);
 
 /* Kinship Gram G = Z Z^T (n x n FP64, leading dimension ld) for Z (n x m) packed as operand planes with the strains
  * as rows (rows_pad x kpad, flmm_pack_planes; K = the SNPs): the product of io.grm's K = Z Z^T / M and of the LOCO
@@ -255,7 +261,7 @@ int flmm_delta_refine_learned(int prec, const float* uy, int64_t P, int64_t p_pa
                               double hi, const int32_t* idx, const double* lam64, double radius, double judge_tol,
                               double* log_delta, int32_t* fallback, double* rWr, double* h2, void* q1_hi,
                               void* q1_lo, void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale,
-                              int64_t hi_from, void* stream);
+                              int64_t hi_from, double ld_uniform, void* order, void* stream);
 
 /* Spectral features of P rotated columns for the learned delta proposal (34 fp32 per column, row-major):
  * feature b < 32 = log(mean y^2 in eigenvalue bin b / mean y^2), bin_of_k[k] in [0, 32) for k < neff;
@@ -325,14 +331,18 @@ int flmm_wald_scan_lowrank_noalloc(int prec, const void* s1_hi, const void* s1_l
  * with the exact weights 1 / (s_k + delta); max_F[p] = max(max_F[p], those values).  The result is the maximum of
  * FP64 evaluations of every test that can hold it.  S1 / Q1 hold kpad, S2 / Q2 kpad2 (== kpad: exact weights).
  * rec: caller workspace of ceil(mc / 128) x (P - p_first) uint64; n_eval (may be NULL) accumulates the number of
- * FP64 evaluations.  Needs mc <= 8192 and mc_pad, p_pad <= 2^24. */
+ * FP64 evaluations.  Needs mc <= 8192 and mc_pad, p_pad <= 2^24.
+ * order (may be NULL): the layout flmm_delta_refine wrote for these columns (its hi_from == p_first, the same
+ * ld_uniform): the columns at log_delta == ld_uniform share one sWs per SNP row, computed in FP64 from us, and run
+ * the sWy product alone (their planes are adjacent); the rest run both products.  Same bounds, same result
+ * semantics.  work: device workspace of flmm_screen_work_bytes(mc_pad) bytes, with order. */
 int flmm_wald_scan_screened(const void* s1_hi, const void* s2_hi, const float* s1_scale, const float* s2_scale,
                             int64_t mc_pad, const void* q1_hi, const void* q2_hi, const float* q1_scale,
                             const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t kpad2, int64_t mc,
                             int64_t p_first, int64_t P, const double* us, int64_t ld_us, const float* uy,
                             const double* s, int64_t neff, double s_lo, double s_hi, const double* log_delta,
-                            const double* rWr, double nu, double* max_F, void* rec, unsigned long long* n_eval,
-                            void* stream);
+                            const double* rWr, double nu, double* max_F, void* rec, double ld_uniform,
+                            const void* order, void* work, unsigned long long* n_eval, void* stream);
 /* The two halves of flmm_wald_scan_screened, same arguments: flmm_wald_screen runs the screen alone (writes rec),
  * flmm_screen_resolve the FP64 re-evaluation from a rec the screen wrote (reads rec, updates max_F); in that order
  * on one stream they equal one flmm_wald_scan_screened call (a caller times them apart). */
@@ -341,13 +351,18 @@ int flmm_wald_screen(const void* s1_hi, const void* s2_hi, const float* s1_scale
                      const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t kpad2, int64_t mc, int64_t p_first,
                      int64_t P, const double* us, int64_t ld_us, const float* uy, const double* s, int64_t neff,
                      double s_lo, double s_hi, const double* log_delta, const double* rWr, double nu, double* max_F,
-                     void* rec, unsigned long long* n_eval, void* stream);
+                     void* rec, double ld_uniform, const void* order, void* work, unsigned long long* n_eval,
+                     void* stream);
 int flmm_screen_resolve(const void* s1_hi, const void* s2_hi, const float* s1_scale, const float* s2_scale,
                         int64_t mc_pad, const void* q1_hi, const void* q2_hi, const float* q1_scale,
                         const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t kpad2, int64_t mc,
                         int64_t p_first, int64_t P, const double* us, int64_t ld_us, const float* uy, const double* s,
                         int64_t neff, double s_lo, double s_hi, const double* log_delta, const double* rWr, double nu,
-                        double* max_F, void* rec, unsigned long long* n_eval, void* stream);
+                        double* max_F, void* rec, double ld_uniform, const void* order, void* work,
+                        unsigned long long* n_eval, void* stream);
+/* Bytes of the uniform-delta split's workspace for mc_pad SNP rows (-1 on a bad size).  No counterpart in the
+ * reference (the workspace of flmm_wald_scan_screened). */
+int64_t flmm_screen_work_bytes(int64_t mc_pad);
 
 /* Fused Wald scan of one SNP block (snp_wald_scan_compat's tile loop, core.py:636-667): sWy = S1.Q1, sWs = S2.Q2 on
  * MFMA, FP64 epilogue with dof nu; split-FP16 sums are divided by the planes' row scales.  Real columns

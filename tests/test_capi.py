@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_error_channel():
     from fasterlmm_amd import _lib
     lib = _lib.load()
-    assert lib.flmm_abi_version() == 12
+    assert lib.flmm_abi_version() == 13
     # argument validation runs on the host and reports through flmm_last_error without touching a GPU
     rc = lib.flmm_max_reduce(None, 1, 0, 4, None, None)
     assert rc == 1
@@ -64,9 +64,9 @@ def test_bad_arguments_raise_value_error():
     with pytest.raises(ValueError, match="null pointer"):
         _lib.call("flmm_rescore_f64", 16, None, 128, 16, 100, 16, 16, 16, 16, 5, 10.0, 16, 16, 16, 16, 4, None)
     with pytest.raises(ValueError, match="R <= rpad"):
-        _lib.call("flmm_factored_weight_planes", 2, 10, 256, 128, 16, 16, 129, 16, 16, 16, 16, 256, None)
+        _lib.call("flmm_factored_weight_planes", 2, 10, 256, 128, 16, 16, 129, 16, 16, 16, 16, 256, None, None)
     with pytest.raises(ValueError, match="q2_scale"):
-        _lib.call("flmm_factored_weight_planes", 2, 10, 256, 128, 16, 16, 100, 16, 16, 16, None, 256, None)
+        _lib.call("flmm_factored_weight_planes", 2, 10, 256, 128, 16, 16, 100, 16, 16, 16, None, 256, None, None)
 
 
 def _fake_group(_lib, mc=300, P=200, p_pad=256, n_real=3, base=0x10000):

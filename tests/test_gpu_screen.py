@@ -81,15 +81,49 @@ def test_screen_max_f_at_fixed_delta(panel, layout, weights):
     assert rel[:256].max() <= 1e-5
 
 
+@pytest.mark.parametrize("layout", ["block", "interleaved"])
+def test_uniform_delta_split_at_fixed_delta(panel, layout):
+    """Half the screened columns at exactly the largest log delta (10, the grid's upper end): those run the
+    sWy-only tile with one FP64 sWs per SNP row, the rest the general tile, each over its column list.  max_F equals
+    the unsplit screen's bit for bit (both are the FP64 value of the same maximising test) and the oracle's."""
+    from fasterlmm_amd import core, engine
+    pan, Zs, Y = panel
+    on = pan.chrom == 1
+    K = lmm.grm(Zs[:, ~on])
+    S0 = Zs[:, on][:, :1500]
+    S = np.concatenate([S0, _twins(S0, np.random.default_rng(9))], axis=1)
+    want = lmm.compat_rotate(K, pan.X, Y)
+    ld = lmm.fit_delta_grid_compat(want, n_grid=64).copy()
+    cols = np.arange(256, Y.shape[1])
+    sel = cols[::2] if layout == "interleaved" else cols[: cols.size // 2]
+    ld[sel] = 10.0
+    ref = lmm.snp_wald_scan_compat(want, ld, S, n_real=4)
+    got, ev = {}, {}
+    for mode in ("auto", "general"):
+        engine.screen_stats.reset(True)
+        with engine.configure(screen=mode):
+            spec = core.fastlmm_compat_rotate(T(K), T(pan.X), T(Y))
+            got[mode] = core.snp_wald_scan_compat(spec, T(ld), T(S), n_real=4).max_F.cpu().numpy()
+        ev[mode] = engine.screen_stats.summary()["fp64_evaluations"]
+        engine.screen_stats.reset(False)
+    rel = np.abs(got["auto"] - ref["max_F"]) / ref["max_F"]
+    say(f"uniform-delta split ({layout}): max_F rel err {rel[256:].max():.2e}; FP64 evaluations {ev['auto']} "
+        f"(split) vs {ev['general']} (general tile only)")
+    np.testing.assert_array_equal(got["auto"], got["general"])
+    assert rel[256:].max() <= 1e-5, (rel[256:].max(), 256 + int(np.argmax(rel[256:])))
+
+
 def test_screen_loco_matches_full_scan_and_oracle(panel):
     from fasterlmm_amd import core, engine
     pan, Zs, Y = panel
     args = (T(Zs), T(pan.X), T(Y), pan.chrom)
     res = {}
-    for mode in ("auto", "off"):
+    for mode in ("auto", "general", "off"):
         with engine.configure(screen=mode):
             res[mode] = core.loco_scan_compat(*args, n_real=4)
     torch.cuda.synchronize()
+    # the uniform-delta split changes which tests are re-evaluated, not the maximum
+    np.testing.assert_array_equal(res["auto"].max_F.cpu().numpy(), res["general"].max_F.cpu().numpy())
     # the real columns run the full scan either way: same bits
     for k in ("f", "beta", "se", "sfve", "nullh2"):
         np.testing.assert_array_equal(getattr(res["auto"], k).cpu().numpy(), getattr(res["off"], k).cpu().numpy())
