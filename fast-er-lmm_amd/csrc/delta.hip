@@ -882,7 +882,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
 // scales the refinement chose.  q2a == NULL: Q2 is not written (factored weights, factored_planes_kernel).
 // Columns hi_from <= p < P (the certified screen's, split precisions) get the hi planes only, from FP32 arithmetic:
 // each value is then within ~5 FP32 rounding units of the FP64 one before its fp16 rounding, which the screen's
-// error bound covers; with a layout (pos), column p's rows go to plane slot hi_from + pos[p - hi_from].
+// error bound covers; with the uniform-delta layout (order), they go to their plane slots in the fragment-native
+// layout (frag_off).
 // A block owns 32 adjacent columns and walks the K chunks; thread (column tid / 8, quad tid % 8) handles 4
 // consecutive k of each chunk, so for one chunk the block's UY read is one contiguous 4 KiB span (a 16-byte load
 // per lane) and its writes are contiguous 8-byte stores per plane (element (p, k) at
@@ -895,9 +896,41 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
                                                            const float* __restrict__ q1s,
                                                            const float* __restrict__ q2s, void* q1a, void* q1b,
                                                            void* q2a, void* q2b, int64_t hi_from,
-                                                           const int* __restrict__ pos) {
-  const int64_t p = (int64_t)blockIdx.x * 32 + (threadIdx.x >> 3);  // p < p_pad (p_pad is a multiple of 32)
+                                                           const int* __restrict__ order) {
   const int j4 = (threadIdx.x & 7) * 4;
+  const int64_t nch = kpad / 32;  // a multiple of 4 (kpad % 128 == 0)
+  if constexpr (is_split(PREC)) {
+    if (order && (int64_t)blockIdx.x * 32 >= hi_from) {
+      // screened columns with the uniform-delta layout: this block writes plane slots b .. b + 31 (slot t: column
+      // hi_from + order[t]) in the fragment-native layout, hi halves only (hi_from % 256 == 0: blocks do not straddle)
+      const int64_t slot = (int64_t)blockIdx.x * 32 - hi_from + (threadIdx.x >> 3);
+      if (slot >= P - hi_from) return;
+      const int64_t p = hi_from + gmem(order)[slot];
+      const float d32 = (float)exp(log_delta[p]);
+      const float s1f = PREC == PREC_SPLIT_FP16 ? q1s[p] : 1.f;
+      const float s2f = (PREC == PREC_SPLIT_FP16 && q2s) ? q2s[p] : 1.f;
+      const bool wq2 = q2a != nullptr;
+#pragma unroll 2
+      for (int64_t c = 0; c < nch; ++c) {
+        const int64_t k0 = c * 32 + j4;
+        const f32x4 y4 = *(const f32x4*)(uy + (c * p_pad + p) * 32 + j4);
+        u16 h[4], g[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          float w = 0.f;
+          if (k0 + e < neff) w = __builtin_amdgcn_rcpf((float)s[k0 + e] + d32);
+          h[e] = hi_of<PREC>(w * y4[e] * s1f);
+          g[e] = hi_of<PREC>(w * s2f);
+        }
+        const int64_t woff = frag_off(c, p_pad, hi_from, slot, j4) / 2;
+        *(uint2*)((u16*)q1a + woff) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
+        if (wq2)
+          *(uint2*)((u16*)q2a + woff) = make_uint2(g[0] | ((uint32_t)g[1] << 16), g[2] | ((uint32_t)g[3] << 16));
+      }
+      return;
+    }
+  }
+  const int64_t p = (int64_t)blockIdx.x * 32 + (threadIdx.x >> 3);  // p < p_pad (p_pad is a multiple of 32)
   const bool live = p < P;
   const double delta = live ? exp(log_delta[p]) : 1.0;
   double sc1 = 1.0, sc2 = 1.0;
@@ -905,16 +938,13 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
     if (live) { sc1 = (double)q1s[p]; sc2 = q2s ? (double)q2s[p] : 1.0; }
   }
   const bool wq2 = q2a != nullptr;
-  const int64_t nch = kpad / 32;  // a multiple of 4 (kpad % 128 == 0)
   if constexpr (is_split(PREC)) {
     if (live && p >= hi_from) {
       const float d32 = (float)delta, s1f = (float)sc1, s2f = (float)sc2;
-      const int64_t dst = pos ? hi_from + gmem(pos)[p - hi_from] : p;  // the screened columns' plane slot
 #pragma unroll 2
       for (int64_t c = 0; c < nch; ++c) {
         const int64_t k0 = c * 32 + j4;
         const int64_t off = (c * p_pad + p) * 32 + j4;
-        const int64_t woff = (c * p_pad + dst) * 32 + j4;
         const f32x4 y4 = *(const f32x4*)(uy + off);
         u16 h[4], g[4];
 #pragma unroll
@@ -924,9 +954,9 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
           h[e] = hi_of<PREC>(w * y4[e] * s1f);
           g[e] = hi_of<PREC>(w * s2f);
         }
-        *(uint2*)((u16*)q1a + woff) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
+        *(uint2*)((u16*)q1a + off) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
         if (wq2)
-          *(uint2*)((u16*)q2a + woff) = make_uint2(g[0] | ((uint32_t)g[1] << 16), g[2] | ((uint32_t)g[3] << 16));
+          *(uint2*)((u16*)q2a + off) = make_uint2(g[0] | ((uint32_t)g[1] << 16), g[2] | ((uint32_t)g[3] << 16));
       }
       return;
     }
@@ -1000,16 +1030,17 @@ __global__ __launch_bounds__(256) void factored_planes_kernel(int64_t P, int64_t
     if (j == 0) qs[p] = (float)sc;
   }
   const int64_t nch = rpad / 32;
-  // the screened columns' rows at their plane slots (pos), hi halves only
-  const int64_t dst = (live && p >= hi_from && pos) ? hi_from + gmem(pos)[p - hi_from] : p;
+  // with the uniform-delta layout the screened columns' hi plane goes to its slot in the fragment-native layout
+  const bool frag = is_split(PREC) && live && p >= hi_from && pos;
+  const int64_t slot = frag ? (int64_t)gmem(pos)[p - hi_from] : 0;
   for (int64_t c = 0; c < nch; ++c) {
     const int64_t r = c * 32 + j;
-    const int64_t off = (c * p_pad + dst) * 32 + j;
+    const int64_t off = (c * p_pad + p) * 32 + j;
     const double v = val(r);
     if constexpr (is_split(PREC)) {
       u16 hh, ll;
       split2<PREC>(v * sc, hh, ll);
-      ((u16*)qa)[off] = hh;
+      ((u16*)qa)[frag ? frag_off(c, p_pad, hi_from, slot, j) / 2 : off] = hh;
       if (!live || p < hi_from) ((u16*)qb)[off] = ll;
     } else {
       ((float*)qa)[off] = (float)v;
@@ -1101,16 +1132,16 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   // the screened columns' layout from the fitted log delta: those at ld_uniform first
-  const int* pos = nullptr;
+  const int* order = nullptr;
   if (order_ws && hi_from < P) {
     e = launch_screen_classify(log_delta + hi_from, P - hi_from, ld_uniform, order_ws, stream);
     if (e != hipSuccess) return e;
-    pos = screen_order(order_ws, P - hi_from).pos;
+    order = screen_order(order_ws, P - hi_from).order;
   }
   const dim3 pgrid((unsigned)(p_pad / 32));
 #define FLMM_PLANES(PR)                                                                                              \
   hipLaunchKernelGGL((delta_planes_kernel<PR>), pgrid, dim3(256), 0, stream, uy, P, p_pad, kpad, s, neff, log_delta, \
-                     q1s, q2s, q1a, q1b, q2a, q2b, hi_from, pos)
+                     q1s, q2s, q1a, q1b, q2a, q2b, hi_from, order)
   if (prec == PREC_SPLIT_BF16) FLMM_PLANES(PREC_SPLIT_BF16);
   else if (prec == PREC_SPLIT_FP16) FLMM_PLANES(PREC_SPLIT_FP16);
   else FLMM_PLANES(PREC_FP32);

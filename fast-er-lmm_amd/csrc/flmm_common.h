@@ -70,6 +70,17 @@ __device__ inline void split_f16(double x, u16& hi, u16& lo) {
   lo = __builtin_bit_cast(u16, l);
 }
 
+// Fragment-native layout of the screened columns' hi planes (with the uniform-delta layout): within each 32-k chunk
+// kc, the region of plane rows hi_from .. p_pad - 1 (p_pad - hi_from rows of 64 bytes in the row layout) holds, for
+// each block of 32 plane slots and each 16-k MFMA step s, the 1 KiB B operand of v_mfma_f32_32x32x16_f16 in lane
+// order: lane 32 h + c = slot 32 b + c, k segment 2 h + s (8 fp16 each).  One 16-byte load per lane then reads one
+// contiguous KiB per wave.  Byte offset of element (slot, k = 32 kc + kk) from the plane's start:
+__device__ inline int64_t frag_off(int64_t kc, int64_t p_pad, int64_t hi_from, int64_t slot, int kk) {
+  const int seg = kk >> 3;
+  return (kc * p_pad + hi_from) * 64 + ((slot >> 5) * 2 + (seg & 1)) * 1024 + ((seg >> 1) * 32 + (slot & 31)) * 16 +
+         (kk & 7) * 2;
+}
+
 // power-of-two scale that maps magnitudes up to `bound` to at most 2^14 (1 for a zero bound)
 __device__ inline double pow2_scale(double bound) {
   if (!(bound > 0.0) || isinf(bound)) return 1.0;

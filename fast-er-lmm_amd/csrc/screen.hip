@@ -49,6 +49,7 @@ __device__ inline int sc_swz(int row) { return (row >> 2) & 3; }
 // n2 + 4 j + i (i = 0..3), all into accumulator set 0; then both products' chunks 2 jb and 2 jb + 1 (jb = j - nSA)
 // as slots (S1 Q1, S2 Q2, S1 Q1, S2 Q2) into sets 0, 1, 0, 1.  A stage's operands are fetched into registers one
 // stage ahead, the SNP side then stored into the other LDS buffer.
+template <bool FRAG>
 __device__ inline void screen_tile(const ScreenArgs& g, int64_t n_mtiles, int64_t n_ptiles, int64_t tile, int64_t Mc,
                                    int64_t P, float dmin, uint64_t* __restrict__ rec, int64_t rec_ld) {
   __shared__ __attribute__((aligned(16))) char lds0[SC_STAGE];
@@ -97,12 +98,15 @@ __device__ inline void screen_tile(const ScreenArgs& g, int64_t n_mtiles, int64_
   const uint32_t a_lane = wave * 2048 + lane * 16;
   const int a_row0 = wave * 32 + lane / 4;
   // column side: lane (column p0 + 64 wave + 32 bj + lane % 32, half h) reads bytes [32 h, 32 h + 32) of its row
-  // (columns past n_cols read the last one; their records are not written)
+  // (row layout; FRAG: its 16 bytes of the step's 1 KiB fragment, frag_off); columns past n_cols read the last one,
+  // their records are not written
+  constexpr uint32_t QSTEP = FRAG ? 1024u : 16u;
   uint32_t b_off[2];
 #pragma unroll
   for (int bj = 0; bj < 2; ++bj) {
     const int64_t j = p0 + wave * 64 + bj * 32 + (lane & 31);
-    b_off[bj] = (uint32_t)((slot0 + (j < n_cols ? j : n_cols - 1)) * 64 + h * 32);
+    const int64_t sl = slot0 + (j < n_cols ? j : n_cols - 1);
+    b_off[bj] = FRAG ? (uint32_t)((sl >> 5) * 2048 + (h * 32 + (sl & 31)) * 16) : (uint32_t)(sl * 64 + h * 32);
   }
 
   // uniform chunk base of slot i of stage j, SNP side (a) and column side (b)
@@ -131,7 +135,7 @@ __device__ inline void screen_tile(const ScreenArgs& g, int64_t n_mtiles, int64_
 #pragma unroll
       for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-        for (int q = 0; q < 2; ++q) b.v[i][bj][q] = *gmem((const u32x4*)(sb + (b_off[bj] + q * 16)));
+        for (int q = 0; q < 2; ++q) b.v[i][bj][q] = *gmem((const u32x4*)(sb + (b_off[bj] + q * QSTEP)));
     }
     return b;
   };
@@ -300,12 +304,14 @@ __device__ inline int64_t live_ptiles(const int* count, int64_t P) {
   return (n + W_TP - 1) / W_TP;
 }
 
+// FRAG: the column planes in the fragment-native layout (with the uniform-delta layout, g.order set)
+template <bool FRAG>
 __global__ __launch_bounds__(256, 1) void wald_screen_kernel(ScreenArgs g, int64_t n_mtiles, int64_t Mc, int64_t P,
                                                              float dmin, uint64_t* __restrict__ rec, int64_t rec_ld) {
   const int64_t n_pt = live_ptiles(g.order ? g.counts + 1 : nullptr, P);
   const int64_t n_live = n_mtiles * n_pt;
   if ((int64_t)blockIdx.x >= n_live) return;
-  screen_tile(g, n_mtiles, n_pt, xcd_remap(blockIdx.x, n_live), Mc, P, dmin, rec, rec_ld);
+  screen_tile<FRAG>(g, n_mtiles, n_pt, xcd_remap(blockIdx.x, n_live), Mc, P, dmin, rec, rec_ld);
 }
 
 // ---- uniform-delta screen ----
@@ -358,11 +364,14 @@ __global__ __launch_bounds__(256, 2) void screen_uniform_kernel(UniformArgs g, i
   // SNP side: thread t copies 16-byte segment t % 4 of rows t / 4 and 64 + t / 4 of every chunk
   const uint32_t a_lane = threadIdx.x * 16;
   const int a_row0 = threadIdx.x >> 2;
-  uint32_t b_off[2];  // the uniform columns hold plane slots 0 .. n_cols - 1
+  // the uniform columns hold plane slots 0 .. n_cols - 1 in the fragment-native layout (frag_off): lane (c, h) of
+  // column block bj reads its 16 bytes of the step's 1 KiB fragment
+  uint32_t b_off[2];
 #pragma unroll
   for (int bj = 0; bj < 2; ++bj) {
     const int64_t j = p0 + wave * 64 + bj * 32 + (lane & 31);
-    b_off[bj] = (uint32_t)((j < n_cols ? j : n_cols - 1) * 64 + h * 32);
+    const int64_t sl = j < n_cols ? j : n_cols - 1;
+    b_off[bj] = (uint32_t)((sl >> 5) * 2048 + (h * 32 + (sl & 31)) * 16);
   }
 
   struct ASet { u32x4 v[SU_SLOTS][2]; };
@@ -384,7 +393,7 @@ __global__ __launch_bounds__(256, 2) void screen_uniform_kernel(UniformArgs g, i
 #pragma unroll
       for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-        for (int q = 0; q < 2; ++q) b.v[i][bj][q] = *gmem((const u32x4*)(sb + (b_off[bj] + q * 16)));
+        for (int q = 0; q < 2; ++q) b.v[i][bj][q] = *gmem((const u32x4*)(sb + (b_off[bj] + q * 1024)));
     }
   };
   auto store = [&](const ASet& a, char* st) {
@@ -778,8 +787,12 @@ hipError_t launch_wald_screen(const ScreenArgs& g, int64_t Mc, int64_t P, float 
                               hipStream_t stream) {
   const int64_t n_mtiles = (Mc + W_TM - 1) / W_TM;
   const int64_t n_tiles = n_mtiles * ((P + W_TP - 1) / W_TP);
-  hipLaunchKernelGGL(wald_screen_kernel, dim3((unsigned)n_tiles), dim3(256), 0, stream, g, n_mtiles, Mc, P, dmin,
-                     rec, rec_ld);
+  if (g.order)
+    hipLaunchKernelGGL(wald_screen_kernel<true>, dim3((unsigned)n_tiles), dim3(256), 0, stream, g, n_mtiles, Mc, P,
+                       dmin, rec, rec_ld);
+  else
+    hipLaunchKernelGGL(wald_screen_kernel<false>, dim3((unsigned)n_tiles), dim3(256), 0, stream, g, n_mtiles, Mc, P,
+                       dmin, rec, rec_ld);
   return hipGetLastError();
 }
 

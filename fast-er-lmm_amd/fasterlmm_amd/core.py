@@ -222,21 +222,29 @@ class _Basis:
         return c
 
     def rescore(self, columns, local: Tensor, row_index: Tensor, n_real: int, out: engine.ScanOutputs,
-                d: engine.DeltaResult) -> int:
+                d: engine.DeltaResult, us: Tensor | None = None, row_range: tuple[int, int] | None = None) -> int:
         """FP64 re-evaluation (flmm_rescore_f64) of one scanned SNP block's real-column tests with F > RESCORE_F.
         Block position i holds SNP column local[i] of columns(idx) -> (N, len(idx)) FP64 and output row
-        row_index[i].  Returns the number of re-scored tests."""
+        row_index[i]; us, if given, the block's FP64 rotated SNPs (row i = block position i), which then need no
+        second rotation; row_range (start, stop), if given, says row_index is start, start + 1, ..., stop - 1.
+        Returns the number of re-scored tests."""
         if n_real == 0 or self.y_cols is None or row_index.numel() == 0:
             return 0
-        F = out.f.index_select(0, row_index)
+        if row_range is not None:  # consecutive output rows: a view instead of a gathered copy
+            F = out.f[row_range[0]:row_range[1]]
+        else:
+            F = out.f.index_select(0, row_index)
         nz = torch.nonzero(F > RESCORE_F)  # (n, 2): block position, column
         n = int(nz.shape[0])
         if n == 0:
             return 0
         mi, pi = nz[:, 0].contiguous(), nz[:, 1].contiguous()  # the kernel reads plain int64 arrays
-        um, inv = torch.unique(mi, return_inverse=True)
-        inv = inv.contiguous()
-        US = self._rotate_f64(columns(local.index_select(0, um)))
+        if us is not None:
+            US, inv = us, mi
+        else:
+            um, inv = torch.unique(mi, return_inverse=True)
+            inv = inv.contiguous()
+            US = self._rotate_f64(columns(local.index_select(0, um)))
         UY = self._uy64.get(n_real)
         wider = [k for k in self._uy64 if k > n_real]
         if UY is None and wider:
@@ -315,26 +323,30 @@ class _Basis:
                                    hi_from=hf, ld_uniform=hi)
 
     def scan(self, Z: Tensor, row_index: Tensor, n_real: int, out: engine.ScanOutputs, d: engine.DeltaResult,
-             block: int = 8192, after_launch=None) -> None:
+             block: int = 8192, after_launch=None, first_row: int | None = None) -> None:
         """Scan the SNP columns Z[:, row_index] of Z (N, M_total) FP64 into out.  after_launch() is called once,
         after the first block's scan is queued and before the host waits for it (the FP64 re-scoring's selection
-        reads F back)."""
+        reads F back).  first_row, if given, says row_index is first_row, first_row + 1, ..."""
         Mc = row_index.numel()
         screen = engine.screen_applies(self.prec) and self.P > n_real and self.neff > 0
+        f64 = engine.settings.snp_rotation == "fp64"
         for b0 in range(0, Mc, block):
             ri = row_index[b0:b0 + block]
-            scr = None
-            if screen:
+            scr = us = None
+            if f64:  # the block's FP64 rotated SNPs also serve the screen and the re-scoring
                 s1, s2, us = self.snp_planes(Z, ri, d, keep_us=True)
-                scr = engine.ScreenOperands(us, self.uy, self.s, self.s_range[0], self.s_range[1], self.neff)
             else:
                 s1, s2 = self.snp_planes(Z, ri, d)
+            if screen:
+                scr = engine.ScreenOperands(us, self.uy, self.s, self.s_range[0], self.s_range[1], self.neff)
             engine.wald_scan(s1, s2, d, ri.numel(), self.P, n_real, self.nu, ri, out, screen=scr)
             del s1, s2, scr
             if after_launch is not None:
                 after_launch()
                 after_launch = None
-            self.rescore(lambda idx: Z.index_select(1, idx), ri, ri, n_real, out, d)
+            rr = None if first_row is None else (first_row + b0, first_row + b0 + ri.numel())
+            self.rescore(lambda idx: Z.index_select(1, idx), ri, ri, n_real, out, d, us=us, row_range=rr)
+            del us
         if after_launch is not None:
             after_launch()
 
@@ -368,6 +380,16 @@ def _projector(X: Tensor) -> tuple[Tensor, Tensor]:
     return Xpinv, torch.eye(N, dtype=X.dtype, device=X.device) - X @ Xpinv
 
 
+def _project(K: Tensor, X: Tensor, Xpinv: Tensor, Iperp: Tensor) -> Tensor:
+    """Iperp (K + I) Iperp (core.py:503-510) for symmetric K, with H = X X^+ = I - Iperp as rank-C updates:
+    K - H K - K H + H K H + Iperp, H K = X (X^+ K), H K H = X (X^+ K X) X^+ (O(N^2 C) instead of two N^3
+    products; equal to FP64 rounding)."""
+    G = Xpinv @ K                 # C x N
+    T = X @ G                     # H K
+    M = (G @ X) @ Xpinv           # C x N: X^+ K X X^+
+    return K - T - T.T + X @ M + Iperp
+
+
 def _projected_eigh(K: Tensor, Iperp: Tensor, D: int, shift: float = 1.0) -> tuple[Tensor, Tensor]:
     """eigh of Iperp (K + shift I) Iperp (core.py:503-510); returns (s = clamp(lambda[D:] - shift, 0), U)."""
     N = K.shape[0]
@@ -389,8 +411,8 @@ def _kinship(Z: Tensor) -> Tensor:
     return engine.gram(Z) / float(Z.shape[1])
 
 
-def _new_outputs(M, n_real, P, dev):
-    return engine.ScanOutputs.allocate(M, n_real, P, dev)
+def _new_outputs(M, n_real, P, dev, *, zero: bool = True):
+    return engine.ScanOutputs.allocate(M, n_real, P, dev, zero=zero)
 
 
 # COMPAT PATH -------
@@ -451,7 +473,7 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
     n_real = P if n_real is None else min(int(n_real), P)
     dev = Zd.device
     prec = engine.prec_code()
-    out = _new_outputs(M, n_real, P, dev)
+    out = _new_outputs(M, n_real, P, dev, zero=False)  # every SNP belongs to one chromosome: every row is written
     Zk = Zd if Zkd is None else Zkd
     ck = chrom if chrom_kin is None else np.asarray(chrom_kin)
     stage = engine.stage
@@ -478,7 +500,7 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
             del Zc
             if jitter:
                 K = K + jitter * eye
-            Kps.append((Iperp @ (K + eye)) @ Iperp)
+            Kps.append(_project(K, Xd, Xpinv, Iperp))
             if plain and not reml:
                 Ks.append(K)
     with stage("eigh"):
@@ -513,7 +535,9 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
         st, d, done = nxt
         nxt = None
         main.wait_event(done)
-        test = torch.from_numpy(np.flatnonzero(chrom == c)).to(dev)
+        rows = np.flatnonzero(chrom == c)
+        test = torch.from_numpy(rows).to(dev)
+        first = int(rows[0]) if rows.size and rows[-1] - rows[0] + 1 == rows.size else None  # consecutive rows
 
         def launch_next(k=k):
             nonlocal nxt
@@ -523,7 +547,7 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
 
         piped = side is not main  # serially, the next chromosome is prepared after this scan, outside its stage
         with stage("snps+scan"):
-            st.scan(Zd, test, n_real, out, d, after_launch=launch_next if piped else None)
+            st.scan(Zd, test, n_real, out, d, after_launch=launch_next if piped else None, first_row=first)
         if not piped:
             launch_next()
         main.synchronize()

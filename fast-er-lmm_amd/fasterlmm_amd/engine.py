@@ -520,8 +520,9 @@ class ScanOutputs:
     max_F: torch.Tensor
 
     @staticmethod
-    def allocate(M: int, n_real: int, P: int, dev) -> "ScanOutputs":
-        z = lambda: torch.zeros(M, n_real, dtype=torch.float64, device=dev)  # noqa: E731
+    def allocate(M: int, n_real: int, P: int, dev, *, zero: bool = True) -> "ScanOutputs":
+        """zero=False leaves the per-test outputs unset: for callers whose scans write every row."""
+        z = lambda: (torch.zeros if zero else torch.empty)(M, n_real, dtype=torch.float64, device=dev)  # noqa: E731
         return ScanOutputs(z(), z(), z(), z(), z(), torch.full((P,), float("-inf"), dtype=torch.float64, device=dev))
 
 
