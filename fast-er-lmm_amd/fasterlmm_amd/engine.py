@@ -545,20 +545,27 @@ class ScreenOperands:
 
 
 class _ScreenStats:
-    """Optional count of the screen's FP64 re-evaluations (a device counter, read by summary())."""
+    """Optional counts of the screen's FP64 re-evaluations and of its tests on the sWy-only (uniform-delta) tile
+    (device counters, read by summary())."""
 
     def __init__(self):
         self.counter = None
+        self.uniform = None
         self.tests = 0
+        self.kpad2 = 0
 
     def reset(self, enabled: bool = True, device=None):
-        self.counter = torch.zeros(1, dtype=torch.int64, device=device or "cuda") if enabled else None
+        z = lambda: torch.zeros(1, dtype=torch.int64, device=device or "cuda") if enabled else None  # noqa: E731
+        self.counter, self.uniform = z(), z()
         self.tests = 0
+        self.kpad2 = 0
 
     def summary(self) -> dict:
         n = int(self.counter.item()) if self.counter is not None else 0
+        u = int(self.uniform.item()) if self.uniform is not None else 0
         return {"screened_tests": self.tests, "fp64_evaluations": n,
-                "fraction": n / self.tests if self.tests else 0.0}
+                "fraction": n / self.tests if self.tests else 0.0, "uniform_tests": u,
+                "uniform_fraction": u / self.tests if self.tests else 0.0, "kpad2": self.kpad2}
 
 
 screen_stats = _ScreenStats()
@@ -645,6 +652,9 @@ def _screened(s1: Planes, s2: Planes, d: DeltaResult, mc: int, p_first: int, P: 
             rec.data_ptr(), lu, _lib.ptr(order), _lib.ptr(work), _lib.ptr(cnt), _lib.stream())
     if cnt is not None:
         screen_stats.tests += mc * (P - p_first)
+        screen_stats.kpad2 = kpad2
+        if order is not None:  # counts[0]: this layout's columns on the sWy-only tile (no host wait)
+            screen_stats.uniform += order[:4].view(torch.int32).to(torch.int64) * mc
     if not scan_timer.enabled:
         _lib.call("flmm_wald_scan_screened", *args)
         return
