@@ -136,7 +136,7 @@ static double* scratch_ptr(hipStream_t stream, const char* who) {
 
 extern "C" {
 
-int flmm_abi_version(void) { return 13; }
+int flmm_abi_version(void) { return 14; }
 
 const char* flmm_last_error(void) { return g_err.c_str(); }
 
@@ -367,6 +367,15 @@ int flmm_rescore_f64(const double* US, const double* UY, int64_t ld, const doubl
   return check_launch(launch_rescore(US, UY, ld, s, neff, log_delta, us_row, col, out_row, n, nu, f, beta, se, sfve,
                                      out_ld, (hipStream_t)stream),
                       "flmm_rescore_f64");
+}
+
+int flmm_select_above(const double* f, int64_t rows, int64_t cols, int64_t ld, double thr, int64_t* out_row,
+                      int64_t* out_col, int64_t cap, unsigned long long* count, void* stream) {
+  REQUIRE(rows >= 0 && cols >= 0 && ld >= cols && cap >= 0, "flmm_select_above: bad sizes");
+  REQUIRE(count && (rows * cols == 0 || f) && (cap == 0 || (out_row && out_col)), "flmm_select_above: null pointer");
+  if (rows * cols == 0) return FLMM_OK;
+  return check_launch(launch_select_above(f, rows, cols, ld, thr, out_row, out_col, cap, count, (hipStream_t)stream),
+                      "flmm_select_above");
 }
 
 int flmm_factored_weight_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,

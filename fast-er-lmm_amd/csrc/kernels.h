@@ -47,6 +47,8 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
                                hipStream_t stream);
 hipError_t launch_gemm_f64(const double* x2, int64_t rows, int64_t kpad, const double* A, int64_t rpad, double* C,
                            hipStream_t stream);
+hipError_t launch_select_above(const double* f, int64_t rows, int64_t cols, int64_t ld, double thr, int64_t* out_row,
+                               int64_t* out_col, int64_t cap, unsigned long long* count, hipStream_t stream);
 hipError_t launch_rescore(const double* US, const double* UY, int64_t ld, const double* s, int64_t neff,
                           const double* log_delta, const int64_t* us_row, const int64_t* col, const int64_t* out_row,
                           int64_t n, double nu, double* f, double* beta, double* se, double* sfve, int64_t out_ld,

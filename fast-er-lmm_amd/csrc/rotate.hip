@@ -346,6 +346,43 @@ __global__ __launch_bounds__(256) void rescore_kernel(const double* __restrict__
   }
 }
 
+// The tests the re-scoring takes: every (row, column) of f (rows x cols, leading dimension ld) with f > thr, appended
+// at an atomic counter (one add per wave: ballot, then each lane's rank below it); entries past cap are counted, not
+// written.  The order of the list is arbitrary; the re-scoring of each test does not depend on it.
+__global__ __launch_bounds__(256) void select_above_kernel(const double* __restrict__ f, int64_t rows, int64_t cols,
+                                                           int64_t ld, double thr, int64_t* __restrict__ out_row,
+                                                           int64_t* __restrict__ out_col, int64_t cap,
+                                                           unsigned long long* __restrict__ count) {
+  const int lane = threadIdx.x & 63;
+  const int64_t n = rows * cols;
+  for (int64_t base = (int64_t)blockIdx.x * 256 + (threadIdx.x & ~63); base < n; base += (int64_t)gridDim.x * 256) {
+    const int64_t i = base + lane;
+    const int64_t r = i / cols, c = i - r * cols;
+    const bool hit = i < n && gmem(f)[r * ld + c] > thr;
+    const uint64_t b = __ballot(hit);
+    if (!b) continue;
+    unsigned long long first = 0;
+    if (lane == __builtin_ctzll(b)) first = atomicAdd(count, (unsigned long long)__popcll(b));
+    first = __shfl(first, __builtin_ctzll(b), 64);
+    if (hit) {
+      const int64_t slot = (int64_t)first + __popcll(b & ((1ull << lane) - 1ull));
+      if (slot < cap) {
+        out_row[slot] = r;
+        out_col[slot] = c;
+      }
+    }
+  }
+}
+
+hipError_t launch_select_above(const double* f, int64_t rows, int64_t cols, int64_t ld, double thr, int64_t* out_row,
+                               int64_t* out_col, int64_t cap, unsigned long long* count, hipStream_t stream) {
+  const int64_t n = rows * cols;
+  const int64_t blocks = n / 256 + 1 < 4096 ? n / 256 + 1 : 4096;
+  hipLaunchKernelGGL(select_above_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, f, rows, cols, ld, thr,
+                     out_row, out_col, cap, count);
+  return hipGetLastError();
+}
+
 hipError_t launch_rescore(const double* US, const double* UY, int64_t ld, const double* s, int64_t neff,
                           const double* log_delta, const int64_t* us_row, const int64_t* col, const int64_t* out_row,
                           int64_t n, double nu, double* f, double* beta, double* se, double* sfve, int64_t out_ld,

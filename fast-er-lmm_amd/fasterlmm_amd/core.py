@@ -201,8 +201,8 @@ class _Basis:
         n = y.shape[1]
         if n == 0 or self.neff == 0:
             return
-        UY = torch.zeros(n, self.kpad, dtype=torch.float64, device=y.device)
-        UY[:, : self.neff] = y.T @ self.mat[:, self.col0:self.col0 + self.neff]
+        # B^T y on the FP64 matrix cores (flmm_rotate_f64): rows i < n, entries j < neff, zero beyond
+        UY = engine.rotate_snps_f64(y, None, n, self.mat, self.col0, self.neff, rup(n, TILE), self.kpad)[:n]
         self._uy64[n] = UY  # the re-scoring's FP64 operand
         nch = self.kpad // KCHUNK
         self.uy.view(nch, self.p_pad, KCHUNK)[:, :n, :] = UY.float().view(n, nch, KCHUNK).permute(1, 0, 2)
@@ -234,11 +234,10 @@ class _Basis:
             F = out.f[row_range[0]:row_range[1]]
         else:
             F = out.f.index_select(0, row_index)
-        nz = torch.nonzero(F > RESCORE_F)  # (n, 2): block position, column
-        n = int(nz.shape[0])
+        mi, pi = _select_above(F, RESCORE_F)  # block positions and columns of the tests with F above the bar
+        n = int(mi.numel())
         if n == 0:
             return 0
-        mi, pi = nz[:, 0].contiguous(), nz[:, 1].contiguous()  # the kernel reads plain int64 arrays
         if us is not None:
             US, inv = us, mi
         else:
@@ -249,9 +248,10 @@ class _Basis:
         wider = [k for k in self._uy64 if k > n_real]
         if UY is None and wider:
             UY = self._uy64[min(wider)][:n_real]
-        if UY is None:  # does not depend on the SNP block, so a library FP64 GEMM keeps results block-invariant
-            UY = torch.zeros(n_real, self.kpad, dtype=torch.float64, device=self.mat.device)
-            UY[:, : self.neff] = self.y_cols[:, :n_real].T @ self.mat[:, self.col0:self.col0 + self.neff]
+        if UY is None:  # does not depend on the SNP block, so one FP64 product keeps results block-invariant
+            yc = self.y_cols[:, :n_real].contiguous()
+            UY = engine.rotate_snps_f64(yc, None, n_real, self.mat, self.col0, self.neff, rup(n_real, TILE),
+                                        self.kpad)[:n_real]
             self._uy64[n_real] = UY
         _lib.call("flmm_rescore_f64", US.data_ptr(), UY.data_ptr(), self.kpad, self.s.data_ptr(), self.neff,
                   d.log_delta.data_ptr(), inv.data_ptr(), pi.data_ptr(), row_index.index_select(0, mi).data_ptr(), n,
@@ -380,6 +380,27 @@ def _projector(X: Tensor) -> tuple[Tensor, Tensor]:
     return Xpinv, torch.eye(N, dtype=X.dtype, device=X.device) - X @ Xpinv
 
 
+_SELECT_CAP = [1 << 16]  # list length of the last selection (grown on demand)
+
+
+def _select_above(F: Tensor, thr: float) -> tuple[Tensor, Tensor]:
+    """(row, column) int64 pairs of the entries of F (rows x cols FP64, unit column stride) above thr, in an
+    arbitrary order (flmm_select_above; one host read of the count)."""
+    rows, cols = F.shape
+    cnt = torch.zeros(1, dtype=torch.int64, device=F.device)
+    while True:
+        cap = _SELECT_CAP[0]
+        mi = torch.empty(cap, dtype=torch.int64, device=F.device)
+        pi = torch.empty(cap, dtype=torch.int64, device=F.device)
+        _lib.call("flmm_select_above", F.data_ptr(), rows, cols, F.stride(0), float(thr), mi.data_ptr(),
+                  pi.data_ptr(), cap, cnt.data_ptr(), _lib.stream())
+        n = int(cnt.item())
+        if n <= cap:
+            return mi[:n], pi[:n]
+        _SELECT_CAP[0] = rup(n, 1 << 16)
+        cnt.zero_()
+
+
 def _project(K: Tensor, X: Tensor, Xpinv: Tensor, Iperp: Tensor) -> Tensor:
     """Iperp (K + I) Iperp (core.py:503-510) for symmetric K, with H = X X^+ = I - Iperp as rank-C updates:
     K - H K - K H + H K H + Iperp, H K = X (X^+ K), H K H = X (X^+ K X) X^+ (O(N^2 C) instead of two N^3
@@ -483,7 +504,7 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
     # the basis is orthogonal to X, so B^T (Y - X X^+ Y) = B^T Y; the OLS regress-out keeps the compat
     # path's arithmetic (core.py:517) and costs one pass
     with stage("pack_columns"):
-        Y_r = (Yd - Xd @ (Xpinv @ Yd)).contiguous()
+        Y_r = torch.addmm(Yd, Xd, Xpinv @ Yd, alpha=-1.0)  # Y - X (X^+ Y), the subtraction in the product's pass
         ypl = engine.pack(prec, Y_r, P, N, 1, P, rup(max(P, 1), PTILE), rup(N, KCHUNK))
         y_real = Y_r[:, :n_real].contiguous()  # kept for the FP64 re-scoring of large-F real tests
         del Y_r
@@ -494,9 +515,12 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
     Kps, Ks = [], []
     with stage("kinship"):
         for c in chroms:
-            kin_c = torch.from_numpy(np.flatnonzero(ck == c)).to(dev)
-            Zc = Zk.index_select(1, kin_c)
-            K = (K_all - engine.gram(Zc, prec)) / float(Zk.shape[1] - kin_c.numel())
+            cols = np.flatnonzero(ck == c)
+            if cols.size and cols[-1] - cols[0] + 1 == cols.size:  # consecutive columns: a view, no gather
+                Zc = Zk[:, int(cols[0]):int(cols[-1]) + 1]
+            else:
+                Zc = Zk.index_select(1, torch.from_numpy(cols).to(dev))
+            K = (K_all - engine.gram(Zc, prec)) / float(Zk.shape[1] - cols.size)
             del Zc
             if jitter:
                 K = K + jitter * eye

@@ -176,7 +176,9 @@ def gram(Z: torch.Tensor, prec: int | None = None) -> torch.Tensor:
     n, m = Z.shape
     if m == 0 or n == 0:
         return torch.zeros(n, n, dtype=torch.float64, device=Z.device)
-    Z = Z.contiguous()
+    if Z.stride(1) != 1:
+        Z = Z.contiguous()
+    ldz = Z.stride(0)  # a column range of a wider panel is read in place
     rows_pad = rup(n, TILE)
     t = rows_pad // TILE
     pairs = t * (t + 1) // 2
@@ -184,13 +186,13 @@ def gram(Z: torch.Tensor, prec: int | None = None) -> torch.Tensor:
         n_split = max(1, min(-(-1024 // pairs), -(-m // 256)))  # >= 1,024 workgroups where K allows
         partial = torch.empty(n_split, rows_pad, rows_pad, dtype=torch.float64, device=Z.device)
         out = torch.empty(n, n, dtype=torch.float64, device=Z.device)
-        _lib.call("flmm_gram_f64", Z.data_ptr(), m, n, m, n_split, partial.data_ptr(), out.data_ptr(), n,
+        _lib.call("flmm_gram_f64", Z.data_ptr(), ldz, n, m, n_split, partial.data_ptr(), out.data_ptr(), n,
                   _lib.stream())
         return out
     if prec not in (_lib.PREC_SPLIT_FP16, _lib.PREC_SPLIT_BF16):
         raise ValueError("the split kinship needs a split precision")
     kpad = rup(m, KCHUNK)
-    pl = pack(prec, Z, n, m, m, 1, rows_pad, kpad)
+    pl = pack(prec, Z, n, m, ldz, 1, rows_pad, kpad)
     n_split = max(1, min(-(-1024 // pairs), -(-(kpad // KCHUNK) // 32)))  # >= 1,024 workgroups where K allows
     partial = torch.empty(n_split, rows_pad, rows_pad, dtype=torch.float64, device=Z.device)
     out = torch.empty(n, n, dtype=torch.float64, device=Z.device)

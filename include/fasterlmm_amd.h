@@ -250,6 +250,14 @@ int flmm_rescore_f64(const double* US, const double* UY, int64_t ld, const doubl
                      int64_t n, double nu, double* f, double* beta, double* se, double* sfve, int64_t out_ld,
                      void* stream);
 
+/* The tests the FP64 re-scoring takes (core.py:640-667's F of one SNP block, the selection before
+ * flmm_rescore_f64): every (row, column) of f (rows x cols, leading dimension ld, FP64) with f > thr, written to
+ * out_row / out_col in an arbitrary order; *count (zeroed by the caller) receives the number of such tests, of which
+ * the first cap are written (a caller seeing *count > cap selects again with a larger list).  No counterpart in the
+ * reference, which re-scores nothing (its scan is FP64 throughout). */
+int flmm_select_above(const double* f, int64_t rows, int64_t cols, int64_t ld, double thr, int64_t* out_row,
+                      int64_t* out_col, int64_t cap, unsigned long long* count, void* stream);
+
 /* Learned-proposal refinement (no counterpart in the reference: the learned delta proposal).  log_delta[p]
  * holds the proposal t0 on entry.  Newton on the loss slope inside [t0 - radius, t0 + radius]; the result is
  * accepted when it converges there, lies in the grid optimum's bracket [grid[idx[p] - 1], grid[idx[p] + 1]],
