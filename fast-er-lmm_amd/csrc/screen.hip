@@ -31,6 +31,7 @@ namespace flmm {
 
 // ScreenArgs: kernels.h
 
+constexpr int64_t SC_GM = 16;                      // SNP tiles per scheduling group of the screen tiles
 constexpr int SC_SLOTS = 4;                        // K-chunk products per pipeline stage
 constexpr int SC_SLOT_LDS = W_TM * 64;             // one slot: 128 rows x 32 fp16
 constexpr int SC_STAGE = SC_SLOTS * SC_SLOT_LDS;   // 32 KiB
@@ -57,10 +58,10 @@ __device__ inline void screen_tile(const ScreenArgs& g, int64_t n_mtiles, int64_
   __shared__ __attribute__((aligned(16))) float row_g[W_TM];   // s2 / s1^2 of each tile row (0 past Mc)
   __shared__ __attribute__((aligned(16))) uint32_t row_dm[W_TM];  // 0, or +inf bits past Mc (padding)
 
-  const int64_t group = tile / (W_GM * n_ptiles);
-  const int64_t first_m = group * W_GM;
-  const int64_t gm = n_mtiles - first_m < W_GM ? n_mtiles - first_m : W_GM;
-  const int64_t r_in = tile - group * (W_GM * n_ptiles);
+  const int64_t group = tile / (SC_GM * n_ptiles);
+  const int64_t first_m = group * SC_GM;
+  const int64_t gm = n_mtiles - first_m < SC_GM ? n_mtiles - first_m : SC_GM;
+  const int64_t r_in = tile - group * (SC_GM * n_ptiles);
   const int64_t mt = first_m + r_in % gm;
   const int64_t pt = r_in / gm;
   const int64_t m0 = mt * W_TM, p0 = pt * W_TP;
@@ -338,10 +339,10 @@ __global__ __launch_bounds__(256, 2) void screen_uniform_kernel(UniformArgs g, i
   const int64_t n_live = n_mtiles * n_ptiles;
   if ((int64_t)blockIdx.x >= n_live) return;
   const int64_t tile = xcd_remap(blockIdx.x, n_live);
-  const int64_t group = tile / (W_GM * n_ptiles);
-  const int64_t first_m = group * W_GM;
-  const int64_t gm = n_mtiles - first_m < W_GM ? n_mtiles - first_m : W_GM;
-  const int64_t r_in = tile - group * (W_GM * n_ptiles);
+  const int64_t group = tile / (SC_GM * n_ptiles);
+  const int64_t first_m = group * SC_GM;
+  const int64_t gm = n_mtiles - first_m < SC_GM ? n_mtiles - first_m : SC_GM;
+  const int64_t r_in = tile - group * (SC_GM * n_ptiles);
   const int64_t mt = first_m + r_in % gm;
   const int64_t pt = r_in / gm;
   const int64_t m0 = mt * W_TM, p0 = pt * W_TP;
