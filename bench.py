@@ -584,10 +584,10 @@ def main():
         return sc[key] / (sc["total_ms"] * 1e-3) / 1e12 if sc["total_ms"] else 0.0
 
     def screen_kernel_flops(sc):
-        """The screen's executed MFMA flop: the recorded 2 (kpad + kpad2) per test, less 2 kpad2 for each test on
-        the uniform-delta tile (no second product)."""
+        """The screen's MFMA flop: the recorded 2 (kpad + kpad2) per test, less 2 kpad2 for each test on the
+        bucketed tile (sWy only; its sWs comes from FP64 node sums)."""
         ev = scan["screen_evaluations"]
-        return sc["kernel_flops"] - 2.0 * ev.get("kpad2", 0) * ev.get("uniform_tests", 0)
+        return sc["kernel_flops"] - 2.0 * ev.get("kpad2", 0) * ev.get("swy_only_tests", 0)
 
     split = prec != "fp32"
     alg = tflops(scan, "ref_flops")  # the compat scan's 4 (N - C) flop per Wald test, over every scan kernel's time
@@ -599,21 +599,20 @@ def main():
                                    "issued_tflops": round(products * tflops(full, "kernel_flops"), 2)},
                "screen": {"ms_per_step": per_step(scr), "launches_per_step": scr["launches"] // args.steps,
                           "issued_tflops": round(screen_kernel_flops(scr) / max(scr["total_ms"], 1e-9) / 1e9, 2),
-                          "uniform_delta_fraction": round(scan["screen_evaluations"].get("uniform_fraction", 0.0), 4)},
+                          "swy_only_fraction": round(scan["screen_evaluations"].get("swy_only_fraction", 0.0), 4)},
                "resolve": {"ms_per_step": per_step(rsv), "launches_per_step": rsv["launches"] // args.steps},
                "fp64_evaluations": scan["screen_evaluations"]}
     if scr["launches"]:
         # the dominant kernel: the hi-plane screen of the permutation columns, one fp16 product per K element
         dom, issued = scr, screen_kernel_flops(scr) / max(scr["total_ms"], 1e-9) / 1e9
         kernel = ("the certified screen of the permutation columns (one flmm_wald_screen call per chromosome): "
-                  "wald_screen_kernel (sWy and factored sWs, both on hi fp16 planes) for the columns at an interior "
-                  "log delta, screen_uniform_kernel (sWy only, FP64 sWs per SNP row) for those at the grid's upper "
-                  "end")
+                  "screen_bucketed_kernel (sWy on hi fp16 planes; sWs interpolated per log-delta bucket from FP64 "
+                  "node sums, screen_nodes_kernel)")
         traffic, traffic_detail = traffic_per_launch("screen", scr)
-        how = ("achieved = MFMA flop the screen's two tiles execute per call (one fp16 product over the padded K: "
-               "kpad + kpad2 per test on the general tile, kpad on the uniform-delta tile) / the call's mean time (HIP "
-               "events on the launch stream, around both kernels and the small row-factor kernel), against the "
-               "published dense fp16 peak: frac is its issued MFMA utilisation.")
+        how = ("achieved = MFMA flop the screen executes per call (one fp16 product over the padded K: kpad per "
+               "test on the bucketed tile, kpad + kpad2 on the general one where it runs) / the call's mean time "
+               "(HIP events on the launch stream, around the node-sum and screen kernels), against the published "
+               "dense fp16 peak: frac is its issued MFMA utilisation.")
     else:
         dom, issued = full, products * tflops(full, "kernel_flops")
         kernel = (f"wald_scan_lowrank_kernel<{prec}> (factored weights)" if full["factored"]

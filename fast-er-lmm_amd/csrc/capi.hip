@@ -136,7 +136,7 @@ static double* scratch_ptr(hipStream_t stream, const char* who) {
 
 extern "C" {
 
-int flmm_abi_version(void) { return 14; }
+int flmm_abi_version(void) { return 15; }
 
 const char* flmm_last_error(void) { return g_err.c_str(); }
 
@@ -280,13 +280,32 @@ static int check_refine(const char* who, int prec, const float* uy, int64_t P, i
   return FLMM_OK;
 }
 
+// the bucketed layout's arguments: q_rows Q-plane rows (p_pad without a layout, at least
+// flmm_screen_q_rows(p_pad, hi_from, P) with one), buckets of width t_h <= 1/5 (the interpolation bound) from t_lo
+static int check_layout(const char* who, int64_t P, int64_t p_pad, int64_t hi_from, int64_t q_rows, double t_lo,
+                        double t_h, int nb, const void* order) {
+  if (!order || hi_from >= P) {
+    REQUIRE(q_rows >= p_pad && q_rows % TILE == 0, "%s: q_rows must be >= p_pad and a multiple of 128", who);
+    return FLMM_OK;
+  }
+  REQUIRE(hi_from % 256 == 0, "%s: a layout needs hi_from a multiple of 256", who);
+  REQUIRE(q_rows % TILE == 0 && q_rows >= hi_from + screen_order_slots(P - hi_from) && q_rows >= p_pad,
+          "%s: q_rows below flmm_screen_q_rows", who);
+  REQUIRE(nb >= 1 && nb <= SO_NB_MAX && isfinite(t_lo) && t_h > 0.0 && t_h <= 0.2,
+          "%s: the layout needs 1 <= nb <= %d buckets of width 0 < t_h <= 1/5", who, SO_NB_MAX);
+  REQUIRE(P - hi_from < (1LL << 31) - (int64_t)SO_PAD * SO_NB_MAX, "%s: too many screened columns", who);
+  return FLMM_OK;
+}
+
 int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_pad, int64_t kpad, const double* s,
                       int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo, double hi,
                       const int32_t* idx, const double* lam64, double* log_delta, double* rWr, double* h2,
                       void* q1_hi, void* q1_lo, void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale,
-                      int64_t hi_from, double ld_uniform, void* order, void* stream) {
+                      int64_t hi_from, int64_t q_rows, double t_lo, double t_h, int nb, void* order,
+                      void* stream) {
   REQUIRE(hi_from >= 0, "flmm_delta_refine: hi_from must be >= 0");
-  REQUIRE(!order || isfinite(ld_uniform), "flmm_delta_refine: a layout needs a finite ld_uniform");
+  const int rl = check_layout("flmm_delta_refine", P, p_pad, hi_from, q_rows, t_lo, t_h, nb, order);
+  if (rl) return rl;
   REQUIRE(mode == FLMM_DELTA_GOLDEN || mode == FLMM_DELTA_NEWTON || mode == FLMM_DELTA_FIXED ||
               mode == FLMM_DELTA_GRID,
           "flmm_delta_refine: bad mode %d", mode);
@@ -297,8 +316,8 @@ int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_
   if (p_pad == 0) return FLMM_OK;
   return check_launch(launch_delta_refine(prec, mode, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco, n_grid, lo, hi,
                                           idx, lam64, log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, q1_scale,
-                                          q2_scale, 0.0, 0.0, nullptr, hi_from, ld_uniform, order,
-                                          (hipStream_t)stream),
+                                          q2_scale, 0.0, 0.0, nullptr, hi_from, q_rows, t_lo, t_h, nb,
+                                          hi_from < P ? order : nullptr, (hipStream_t)stream),
                       "flmm_delta_refine");
 }
 
@@ -391,9 +410,8 @@ int flmm_factored_weight_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad
   REQUIRE(aligned16(q2_hi) && (!two || aligned16(q2_lo)), "flmm_factored_weight_planes: planes must be 16-byte aligned");
   REQUIRE(prec != FLMM_PREC_SPLIT_FP16 || q2_scale, "flmm_factored_weight_planes: split-FP16 needs q2_scale");
   if (p_pad == 0) return FLMM_OK;
-  const int* pos = (order && hi_from < P) ? screen_order((void*)order, P - hi_from).pos : nullptr;
   return check_launch(launch_factored_planes(prec, P, p_pad, rpad, t, lam, R, log_delta, q2_hi, q2_lo, q2_scale,
-                                             hi_from, pos, (hipStream_t)stream),
+                                             hi_from, order != nullptr && hi_from < P, (hipStream_t)stream),
                       "flmm_factored_weight_planes");
 }
 
@@ -402,18 +420,20 @@ int flmm_delta_refine_learned(int prec, const float* uy, int64_t P, int64_t p_pa
                               double hi, const int32_t* idx, const double* lam64, double radius, double judge_tol,
                               double* log_delta, int32_t* fallback, double* rWr, double* h2, void* q1_hi,
                               void* q1_lo, void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale,
-                              int64_t hi_from, double ld_uniform, void* order, void* stream) {
+                              int64_t hi_from, int64_t q_rows, double t_lo, double t_h, int nb, void* order,
+                              void* stream) {
   REQUIRE(radius > 0.0 && judge_tol >= 0.0 && idx, "flmm_delta_refine_learned: bad radius / tol / idx");
   REQUIRE(hi_from >= 0, "flmm_delta_refine_learned: hi_from must be >= 0");
-  REQUIRE(!order || isfinite(ld_uniform), "flmm_delta_refine_learned: a layout needs a finite ld_uniform");
+  const int rl = check_layout("flmm_delta_refine_learned", P, p_pad, hi_from, q_rows, t_lo, t_h, nb, order);
+  if (rl) return rl;
   const int rc = check_refine("flmm_delta_refine_learned", prec, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco,
                               n_grid, log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, q1_scale, q2_scale);
   if (rc) return rc;
   if (p_pad == 0) return FLMM_OK;
   return check_launch(launch_delta_refine(prec, 3, uy, P, p_pad, kpad, s, neff, s_lam, nlam, nco, n_grid, lo, hi, idx,
                                           lam64, log_delta, rWr, h2, q1_hi, q1_lo, q2_hi, q2_lo, q1_scale, q2_scale,
-                                          radius, judge_tol, fallback, hi_from, ld_uniform, order,
-                                          (hipStream_t)stream),
+                                          radius, judge_tol, fallback, hi_from, q_rows, t_lo, t_h, nb,
+                                          hi_from < P ? order : nullptr, (hipStream_t)stream),
                       "flmm_delta_refine_learned");
 }
 
@@ -575,22 +595,30 @@ int64_t flmm_screen_order_bytes(int64_t n_cols) {
   return 4 * screen_order_ints(n_cols);
 }
 
-int64_t flmm_screen_work_bytes(int64_t mc_pad) {
-  if (mc_pad < 0) return -1;
-  return 4 * mc_pad;
+int64_t flmm_screen_q_rows(int64_t p_pad, int64_t hi_from, int64_t P) {
+  if (p_pad < 0 || hi_from < 0 || P < 0 || P > p_pad) return -1;
+  if (hi_from >= P) return p_pad;
+  const int64_t need = (hi_from + screen_order_slots(P - hi_from) + 255) / 256 * 256;
+  return need > p_pad ? need : p_pad;
+}
+
+int64_t flmm_screen_work_bytes(int64_t mc_pad, int64_t kpad) {
+  if (mc_pad < 0 || kpad < 0) return -1;
+  return 8 * (int64_t)SO_NODES * ((kpad + 31) / 32 * 32 + (mc_pad + 31) / 32 * 32);
 }
 
 static int screened(const char* who, int phases, const void* s1_hi, const void* s2_hi, const float* s1_scale,
                     const float* s2_scale, int64_t mc_pad, const void* q1_hi, const void* q2_hi, const float* q1_scale,
-                    const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t kpad2, int64_t mc, int64_t p_first,
-                    int64_t P, const double* us, int64_t ld_us, const float* uy, const double* s, int64_t neff,
-                    double s_lo, double s_hi, const double* log_delta, const double* rWr, double nu, double* max_F,
-                    void* rec, double ld_uniform, const void* order, void* work, unsigned long long* n_eval,
+                    const float* q2_scale, int64_t p_pad, int64_t q_rows, int64_t kpad, int64_t kpad2, int64_t mc,
+                    int64_t p_first, int64_t P, const double* us, int64_t ld_us, const float* uy, const double* s,
+                    int64_t neff, double s_lo, double s_hi, const double* log_delta, const double* rWr, double nu,
+                    double* max_F, void* rec, const void* order, void* work, unsigned long long* n_eval,
                     void* stream) {
   REQUIRE(mc >= 0 && mc <= mc_pad && mc_pad % TILE == 0 && mc <= 64 * TILE, "%s: bad SNP extent (mc <= 8192)", who);
   REQUIRE(P >= 0 && P <= p_pad && p_pad % 256 == 0 && p_first >= 0 && p_first % 256 == 0 && p_first <= P,
           "%s: bad column extent (p_pad and p_first multiples of 256, p_first <= P <= p_pad)", who);
-  REQUIRE(mc_pad <= (1LL << 24) && p_pad <= (1LL << 24), "%s: rows_pad above 2^24", who);
+  REQUIRE(q_rows >= p_pad && q_rows % 256 == 0, "%s: q_rows must be >= p_pad and a multiple of 256", who);
+  REQUIRE(mc_pad <= (1LL << 24) && q_rows <= (1LL << 24), "%s: rows_pad above 2^24", who);
   REQUIRE(kpad > 0 && kpad % TILE == 0 && kpad2 > 0 && kpad2 % TILE == 0 && kpad2 <= kpad,
           "%s: need 0 < kpad2 <= kpad, both multiples of 128", who);
   REQUIRE(neff >= 0 && neff <= kpad && ld_us >= kpad, "%s: need neff <= kpad <= ld_us", who);
@@ -600,9 +628,10 @@ static int screened(const char* who, int phases, const void* s1_hi, const void* 
   REQUIRE(aligned16(s1_hi) && aligned16(s2_hi) && aligned16(q1_hi) && aligned16(q2_hi),
           "%s: operand planes must be 16-byte aligned", who);
   REQUIRE(us && uy && s && log_delta && rWr && max_F && rec, "%s: null pointer", who);
-  const bool uniform = order != nullptr;
-  REQUIRE(!uniform || (isfinite(ld_uniform) && work),
-          "%s: the uniform-delta split needs a finite ld_uniform and the workspace (flmm_screen_work_bytes)", who);
+  const bool bucketed = order != nullptr;
+  REQUIRE(!bucketed || work, "%s: the bucketed layout needs the workspace (flmm_screen_work_bytes)", who);
+  REQUIRE(!bucketed || q_rows >= p_first + screen_order_slots(P - p_first),
+          "%s: q_rows below flmm_screen_q_rows for the bucketed layout", who);
   if (mc == 0 || P == p_first) return FLMM_OK;
   const int64_t Pl = P - p_first;
   const double u = SC_U, K = (double)kpad, K2 = (double)kpad2, e = SUBNORMAL_EPS;
@@ -610,47 +639,54 @@ static int screened(const char* who, int phases, const void* s1_hi, const void* 
   // subnormal part of the first product, relative to s q sqrt(d rWr) >= 2^26 / R (rows scaled to (2^13, 2^14])
   const double kArel = (e * 16384.0 * (sqrt(K) + K) * (1 + u) * (1 + u) + K * e * e) / 67108864.0;
   const double kD = 2 * u + u * u + sc_gamma(2 * K2) * (1 + 2 * u) + EXPSUM_REL + SC_ETA + SC_SLACK;
+  // the bucketed tile's sWs: 1 / sWs interpolated within 3.1 e^h 0.0481 h^3 / 6 (h <= 1/5), d~ within that over
+  // (1 - it), plus the FP32 evaluation (SC_SLACK): inside kD
+  static_assert(3.1 * 1.2215 * 0.0481 * 0.008 / 6.0 * 1.001 + SC_SLACK < 2 * SC_U, "bucketed sWs error within kD");
   // second product: subnormal error <= e (sum Ch + sum Bh) + K2 e^2 with both rows' entries <= 2^14 (sum <= K2 2^14
   // covers the max-scaled exact-weight planes too); a row whose scaled D is below that / ETA is a forced candidate
   const float dmin = (float)((e * 16384.0 * 2.0 * K2 * (1 + u) + K2 * e * e) / SC_ETA);
   const hipStream_t st = (hipStream_t)stream;
-  ScreenArgs g{};
-  g.s1 = (const u16*)s1_hi;
-  g.s2 = (const u16*)s2_hi;
-  g.q1 = (const u16*)q1_hi + p_first * 32;
-  g.q2 = (const u16*)q2_hi + p_first * 32;
-  g.s1s = s1_scale;
-  g.s2s = s2_scale;
-  g.a_rows_pad = mc_pad;
-  g.b_rows_pad = p_pad;
-  g.nchunks = (int)(kpad / KCHUNK);
-  g.nchunks2 = (int)(kpad2 / KCHUNK);
   uint64_t* r = (uint64_t*)rec;
   if (phases & 1) {
-    if (uniform) {
-      // the columns at log delta == ld_uniform (plane slots 0 .. counts[0] - 1): the sWy-only tile; the rest (the
-      // slots after): the general tile
+    if (bucketed) {
+      // every screened column on the sWy-only tile, sWs from the node sums of its log-delta bucket
       const ScreenOrder o = screen_order((void*)order, Pl);
-      float* row_f = (float*)work;
-      int rc = check_launch(launch_screen_rows(us, ld_us, mc, mc_pad, s, neff, ld_uniform, s1_scale, row_f, st), who);
+      double* wnode = (double*)work;
+      double* dnode = wnode + (kpad + 31) / 32 * 32 * SO_NODES;
+      int rc = check_launch(launch_screen_nodes(us, ld_us, mc, s, neff, kpad, order, wnode, dnode, st), who);
       if (rc != FLMM_OK) return rc;
-      UniformArgs ua{};
-      ua.s1 = g.s1;
-      ua.q1 = g.q1;
-      ua.q2s = q2_scale + p_first;
-      ua.row_f = row_f;
-      ua.order = o.order;
-      ua.counts = o.counts;
-      ua.a_rows_pad = mc_pad;
-      ua.b_rows_pad = p_pad;
-      ua.nchunks = g.nchunks;
-      rc = check_launch(launch_screen_uniform(ua, mc, Pl, r, Pl, st), who);
-      if (rc != FLMM_OK) return rc;
+      BucketArgs g{};
+      g.s1 = (const u16*)s1_hi;
+      g.q1 = (const u16*)q1_hi + p_first * 32;
+      g.s1s = s1_scale;
+      g.q2s = q2_scale + p_first;
+      g.log_delta = log_delta + p_first;
+      g.dnode = dnode;
+      g.mc = mc;
+      g.head = o.head;
+      g.tb = o.tb;
       g.order = o.order;
-      g.counts = o.counts;
+      g.gbucket = o.gbucket;
+      g.a_rows_pad = mc_pad;
+      g.b_rows_pad = q_rows;
+      g.nchunks = (int)(kpad / KCHUNK);
+      rc = check_launch(launch_screen_bucketed(g, mc, Pl, r, Pl, st), who);
+      if (rc != FLMM_OK) return rc;
+    } else {
+      ScreenArgs g{};
+      g.s1 = (const u16*)s1_hi;
+      g.s2 = (const u16*)s2_hi;
+      g.q1 = (const u16*)q1_hi + p_first * 32;
+      g.q2 = (const u16*)q2_hi + p_first * 32;
+      g.s1s = s1_scale;
+      g.s2s = s2_scale;
+      g.a_rows_pad = mc_pad;
+      g.b_rows_pad = q_rows;
+      g.nchunks = (int)(kpad / KCHUNK);
+      g.nchunks2 = (int)(kpad2 / KCHUNK);
+      const int rc = check_launch(launch_wald_screen(g, mc, Pl, dmin, r, Pl, st), who);
+      if (rc != FLMM_OK) return rc;
     }
-    const int rc = check_launch(launch_wald_screen(g, mc, Pl, dmin, r, Pl, st), who);
-    if (rc != FLMM_OK) return rc;
   }
   if (phases & 2)
     return check_launch(launch_screen_resolve(r, Pl, mc, Pl, p_first, q1_scale, q2_scale, rWr, log_delta, us, ld_us,
@@ -660,15 +696,15 @@ static int screened(const char* who, int phases, const void* s1_hi, const void* 
 }
 
 #define FLMM_SCREENED_ARGS                                                                                           \
-  s1_hi, s2_hi, s1_scale, s2_scale, mc_pad, q1_hi, q2_hi, q1_scale, q2_scale, p_pad, kpad, kpad2, mc, p_first, P, us, \
-      ld_us, uy, s, neff, s_lo, s_hi, log_delta, rWr, nu, max_F, rec, ld_uniform, order, work, n_eval, stream
+  s1_hi, s2_hi, s1_scale, s2_scale, mc_pad, q1_hi, q2_hi, q1_scale, q2_scale, p_pad, q_rows, kpad, kpad2, mc,     \
+      p_first, P, us, ld_us, uy, s, neff, s_lo, s_hi, log_delta, rWr, nu, max_F, rec, order, work, n_eval, stream
 #define FLMM_SCREENED_PARAMS                                                                                         \
   const void *s1_hi, const void *s2_hi, const float *s1_scale, const float *s2_scale, int64_t mc_pad,                \
       const void *q1_hi, const void *q2_hi, const float *q1_scale, const float *q2_scale, int64_t p_pad,             \
-      int64_t kpad, int64_t kpad2, int64_t mc, int64_t p_first, int64_t P, const double *us, int64_t ld_us,          \
-      const float *uy, const double *s, int64_t neff, double s_lo, double s_hi, const double *log_delta,             \
-      const double *rWr, double nu, double *max_F, void *rec, double ld_uniform, const void *order, void *work,       \
-      unsigned long long *n_eval, void *stream
+      int64_t q_rows, int64_t kpad, int64_t kpad2, int64_t mc, int64_t p_first, int64_t P, const double *us,         \
+      int64_t ld_us, const float *uy, const double *s, int64_t neff, double s_lo, double s_hi,                       \
+      const double *log_delta, const double *rWr, double nu, double *max_F, void *rec, const void *order,            \
+      void *work, unsigned long long *n_eval, void *stream
 
 int flmm_wald_scan_screened(FLMM_SCREENED_PARAMS) {
   return screened("flmm_wald_scan_screened", 3, FLMM_SCREENED_ARGS);

@@ -882,8 +882,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
 // scales the refinement chose.  q2a == NULL: Q2 is not written (factored weights, factored_planes_kernel).
 // Columns hi_from <= p < P (the certified screen's, split precisions) get the hi planes only, from FP32 arithmetic:
 // each value is then within ~5 FP32 rounding units of the FP64 one before its fp16 rounding, which the screen's
-// error bound covers; with the uniform-delta layout (order), they go to their plane slots in the fragment-native
-// layout (frag_off).
+// error bound covers; with the bucketed layout (ows, a ScreenOrder workspace), their Q1 hi plane goes to their plane
+// slots in the fragment-native layout (frag_off) and Q2 is not written.  Q planes have q_rows rows per K-chunk
+// (p_pad, or more for the bucketed layout's padded slots).
 // A block owns 32 adjacent columns and walks the K chunks; thread (column tid / 8, quad tid % 8) handles 4
 // consecutive k of each chunk, so for one chunk the block's UY read is one contiguous 4 KiB span (a 16-byte load
 // per lane) and its writes are contiguous 8-byte stores per plane (element (p, k) at
@@ -895,42 +896,43 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
                                                            const double* __restrict__ log_delta,
                                                            const float* __restrict__ q1s,
                                                            const float* __restrict__ q2s, void* q1a, void* q1b,
-                                                           void* q2a, void* q2b, int64_t hi_from,
-                                                           const int* __restrict__ order) {
+                                                           void* q2a, void* q2b, int64_t hi_from, int64_t q_rows,
+                                                           const int* __restrict__ ows) {
   const int j4 = (threadIdx.x & 7) * 4;
   const int64_t nch = kpad / 32;  // a multiple of 4 (kpad % 128 == 0)
   if constexpr (is_split(PREC)) {
-    if (order && (int64_t)blockIdx.x * 32 >= hi_from) {
-      // screened columns with the uniform-delta layout: this block writes plane slots b .. b + 31 (slot t: column
-      // hi_from + order[t]) in the fragment-native layout, hi halves only (hi_from % 256 == 0: blocks do not straddle)
+    if (ows && (int64_t)blockIdx.x * 32 >= hi_from) {
+      // screened columns with the bucketed layout: this block writes plane slots b .. b + 31 (slot t: column
+      // hi_from + order[t]; padding slots are skipped: only their own columns of the product read them) in the
+      // fragment-native layout, the Q1 hi half only (the bucketed screen takes sWs from FP64 node sums; hi_from %
+      // 256 == 0: blocks do not straddle it)
+      const ScreenOrder o = screen_order((void*)ows, P - hi_from);
       const int64_t slot = (int64_t)blockIdx.x * 32 - hi_from + (threadIdx.x >> 3);
-      if (slot >= P - hi_from) return;
-      const int64_t p = hi_from + gmem(order)[slot];
+      if (slot >= gmem(o.head)[0]) return;
+      const int col = gmem(o.order)[slot];
+      if (col < 0) return;
+      const int64_t p = hi_from + col;
       const float d32 = (float)exp(log_delta[p]);
       const float s1f = PREC == PREC_SPLIT_FP16 ? q1s[p] : 1.f;
-      const float s2f = (PREC == PREC_SPLIT_FP16 && q2s) ? q2s[p] : 1.f;
-      const bool wq2 = q2a != nullptr;
 #pragma unroll 2
       for (int64_t c = 0; c < nch; ++c) {
         const int64_t k0 = c * 32 + j4;
         const f32x4 y4 = *(const f32x4*)(uy + (c * p_pad + p) * 32 + j4);
-        u16 h[4], g[4];
+        u16 h[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           float w = 0.f;
           if (k0 + e < neff) w = __builtin_amdgcn_rcpf((float)s[k0 + e] + d32);
           h[e] = hi_of<PREC>(w * y4[e] * s1f);
-          g[e] = hi_of<PREC>(w * s2f);
         }
-        const int64_t woff = frag_off(c, p_pad, hi_from, slot, j4) / 2;
+        const int64_t woff = frag_off(c, q_rows, hi_from, slot, j4) / 2;
         *(uint2*)((u16*)q1a + woff) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
-        if (wq2)
-          *(uint2*)((u16*)q2a + woff) = make_uint2(g[0] | ((uint32_t)g[1] << 16), g[2] | ((uint32_t)g[3] << 16));
       }
       return;
     }
   }
-  const int64_t p = (int64_t)blockIdx.x * 32 + (threadIdx.x >> 3);  // p < p_pad (p_pad is a multiple of 32)
+  const int64_t p = (int64_t)blockIdx.x * 32 + (threadIdx.x >> 3);
+  if (p >= p_pad) return;  // rows past the columns' padding (taller Q planes) are only the layout's slots
   const bool live = p < P;
   const double delta = live ? exp(log_delta[p]) : 1.0;
   double sc1 = 1.0, sc2 = 1.0;
@@ -944,7 +946,7 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
 #pragma unroll 2
       for (int64_t c = 0; c < nch; ++c) {
         const int64_t k0 = c * 32 + j4;
-        const int64_t off = (c * p_pad + p) * 32 + j4;
+        const int64_t off = (c * p_pad + p) * 32 + j4, qoff = (c * q_rows + p) * 32 + j4;
         const f32x4 y4 = *(const f32x4*)(uy + off);
         u16 h[4], g[4];
 #pragma unroll
@@ -954,9 +956,9 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
           h[e] = hi_of<PREC>(w * y4[e] * s1f);
           g[e] = hi_of<PREC>(w * s2f);
         }
-        *(uint2*)((u16*)q1a + off) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
+        *(uint2*)((u16*)q1a + qoff) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
         if (wq2)
-          *(uint2*)((u16*)q2a + off) = make_uint2(g[0] | ((uint32_t)g[1] << 16), g[2] | ((uint32_t)g[3] << 16));
+          *(uint2*)((u16*)q2a + qoff) = make_uint2(g[0] | ((uint32_t)g[1] << 16), g[2] | ((uint32_t)g[3] << 16));
       }
       return;
     }
@@ -964,7 +966,7 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
 #pragma unroll 2
   for (int64_t c = 0; c < nch; ++c) {
     const int64_t k0 = c * 32 + j4;
-    const int64_t off = (c * p_pad + p) * 32 + j4;
+    const int64_t off = (c * p_pad + p) * 32 + j4, qoff = (c * q_rows + p) * 32 + j4;
     const f32x4 y4 = *(const f32x4*)(uy + off);
     double w[4], q1[4];
 #pragma unroll
@@ -980,17 +982,17 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
       u16 h[4], l[4];
 #pragma unroll
       for (int e = 0; e < 4; ++e) split2f<PREC>((float)(q1[e] * sc1), h[e], l[e]);
-      *(uint2*)((u16*)q1a + off) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
-      *(uint2*)((u16*)q1b + off) = make_uint2(l[0] | ((uint32_t)l[1] << 16), l[2] | ((uint32_t)l[3] << 16));
+      *(uint2*)((u16*)q1a + qoff) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
+      *(uint2*)((u16*)q1b + qoff) = make_uint2(l[0] | ((uint32_t)l[1] << 16), l[2] | ((uint32_t)l[3] << 16));
       if (wq2) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) split2<PREC>(w[e] * sc2, h[e], l[e]);
-        *(uint2*)((u16*)q2a + off) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
-        *(uint2*)((u16*)q2b + off) = make_uint2(l[0] | ((uint32_t)l[1] << 16), l[2] | ((uint32_t)l[3] << 16));
+        *(uint2*)((u16*)q2a + qoff) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
+        *(uint2*)((u16*)q2b + qoff) = make_uint2(l[0] | ((uint32_t)l[1] << 16), l[2] | ((uint32_t)l[3] << 16));
       }
     } else {
-      *(f32x4*)((float*)q1a + off) = f32x4{(float)q1[0], (float)q1[1], (float)q1[2], (float)q1[3]};
-      if (wq2) *(f32x4*)((float*)q2a + off) = f32x4{(float)w[0], (float)w[1], (float)w[2], (float)w[3]};
+      *(f32x4*)((float*)q1a + qoff) = f32x4{(float)q1[0], (float)q1[1], (float)q1[2], (float)q1[3]};
+      if (wq2) *(f32x4*)((float*)q2a + qoff) = f32x4{(float)w[0], (float)w[1], (float)w[2], (float)w[3]};
     }
   }
 }
@@ -1011,7 +1013,7 @@ __global__ __launch_bounds__(256) void factored_planes_kernel(int64_t P, int64_t
                                                               const double* __restrict__ lam, int64_t R,
                                                               const double* __restrict__ log_delta, void* qa,
                                                               void* qb, float* __restrict__ qs, int64_t hi_from,
-                                                              const int* __restrict__ pos) {
+                                                              bool layout) {
   const int64_t p = (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);  // p < p_pad (p_pad is a multiple of 8)
   const int j = threadIdx.x & 31;
   const bool live = p < P;
@@ -1030,9 +1032,9 @@ __global__ __launch_bounds__(256) void factored_planes_kernel(int64_t P, int64_t
     if (j == 0) qs[p] = (float)sc;
   }
   const int64_t nch = rpad / 32;
-  // with the uniform-delta layout the screened columns' hi plane goes to its slot in the fragment-native layout
-  const bool frag = is_split(PREC) && live && p >= hi_from && pos;
-  const int64_t slot = frag ? (int64_t)gmem(pos)[p - hi_from] : 0;
+  // with the bucketed layout the screened columns' sWs comes from the FP64 node sums: their planes are not written
+  // (their scales are)
+  if (is_split(PREC) && live && p >= hi_from && layout) return;
   for (int64_t c = 0; c < nch; ++c) {
     const int64_t r = c * 32 + j;
     const int64_t off = (c * p_pad + p) * 32 + j;
@@ -1040,7 +1042,7 @@ __global__ __launch_bounds__(256) void factored_planes_kernel(int64_t P, int64_t
     if constexpr (is_split(PREC)) {
       u16 hh, ll;
       split2<PREC>(v * sc, hh, ll);
-      ((u16*)qa)[frag ? frag_off(c, p_pad, hi_from, slot, j) / 2 : off] = hh;
+      ((u16*)qa)[off] = hh;
       if (!live || p < hi_from) ((u16*)qb)[off] = ll;
     } else {
       ((float*)qa)[off] = (float)v;
@@ -1050,11 +1052,11 @@ __global__ __launch_bounds__(256) void factored_planes_kernel(int64_t P, int64_t
 
 hipError_t launch_factored_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
                                   int64_t R, const double* log_delta, void* qa, void* qb, float* qs, int64_t hi_from,
-                                  const int* pos, hipStream_t stream) {
+                                  bool layout, hipStream_t stream) {
   const dim3 grid((unsigned)(p_pad / 8));
 #define FLMM_FPL(PR)                                                                                                 \
   hipLaunchKernelGGL((factored_planes_kernel<PR>), grid, dim3(256), 0, stream, P, p_pad, rpad, t, lam, R, log_delta, \
-                     qa, qb, qs, hi_from, pos)
+                     qa, qb, qs, hi_from, layout)
   if (prec == PREC_SPLIT_BF16) FLMM_FPL(PREC_SPLIT_BF16);
   else if (prec == PREC_SPLIT_FP16) FLMM_FPL(PREC_SPLIT_FP16);
   else FLMM_FPL(PREC_FP32);
@@ -1098,7 +1100,8 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
                                double lo, double hi, const int* idx, const double* lam64,
                                double* log_delta, double* rWr, double* h2, void* q1a, void* q1b, void* q2a,
                                void* q2b, float* q1s, float* q2s, double radius, double judge_tol, int* flags,
-                               int64_t hi_from, double ld_uniform, void* order_ws, hipStream_t stream) {
+                               int64_t hi_from, int64_t q_rows, double t_lo, double t_h, int nb, void* order_ws,
+                               hipStream_t stream) {
   const dim3 grid((unsigned)((p_pad + 3) / 4));
   // columns that fit 16 elements per lane (neff, nlam, kpad <= 1024) keep them in registers
   const bool regs = neff <= 1024 && nlam <= 1024 && kpad <= 1024;
@@ -1131,17 +1134,17 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
 #undef FLMM_REFINE1
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  // the screened columns' layout from the fitted log delta: those at ld_uniform first
-  const int* order = nullptr;
+  // the screened columns' layout from the fitted log delta: bucketed by log delta
+  const int* ows = nullptr;
   if (order_ws && hi_from < P) {
-    e = launch_screen_classify(log_delta + hi_from, P - hi_from, ld_uniform, order_ws, stream);
+    e = launch_screen_classify(log_delta + hi_from, P - hi_from, t_lo, t_h, nb, order_ws, stream);
     if (e != hipSuccess) return e;
-    order = screen_order(order_ws, P - hi_from).order;
+    ows = (const int*)order_ws;
   }
-  const dim3 pgrid((unsigned)(p_pad / 32));
+  const dim3 pgrid((unsigned)(q_rows / 32));
 #define FLMM_PLANES(PR)                                                                                              \
   hipLaunchKernelGGL((delta_planes_kernel<PR>), pgrid, dim3(256), 0, stream, uy, P, p_pad, kpad, s, neff, log_delta, \
-                     q1s, q2s, q1a, q1b, q2a, q2b, hi_from, order)
+                     q1s, q2s, q1a, q1b, q2a, q2b, hi_from, q_rows, ows)
   if (prec == PREC_SPLIT_BF16) FLMM_PLANES(PREC_SPLIT_BF16);
   else if (prec == PREC_SPLIT_FP16) FLMM_PLANES(PREC_SPLIT_FP16);
   else FLMM_PLANES(PREC_FP32);

@@ -43,8 +43,8 @@ hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, i
                                double lo, double hi, const int* idx, const double* lam64, double* log_delta,
                                double* rWr, double* h2,
                                void* q1a, void* q1b, void* q2a, void* q2b, float* q1s, float* q2s, double radius,
-                               double judge_tol, int* flags, int64_t hi_from, double ld_uniform, void* order_ws,
-                               hipStream_t stream);
+                               double judge_tol, int* flags, int64_t hi_from, int64_t q_rows, double t_lo,
+                               double t_h, int nb, void* order_ws, hipStream_t stream);
 hipError_t launch_gemm_f64(const double* x2, int64_t rows, int64_t kpad, const double* A, int64_t rpad, double* C,
                            hipStream_t stream);
 hipError_t launch_select_above(const double* f, int64_t rows, int64_t cols, int64_t ld, double thr, int64_t* out_row,
@@ -64,7 +64,7 @@ hipError_t launch_gram_f64(const double* Z, int64_t ldz, int64_t n, int64_t m, i
                            double* out, int64_t ld, hipStream_t stream);
 hipError_t launch_factored_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
                                   int64_t R, const double* log_delta, void* qa, void* qb, float* qs, int64_t hi_from,
-                                  const int* pos, hipStream_t stream);
+                                  bool layout, hipStream_t stream);
 template <int PREC>
 hipError_t launch_wald_scan(const TileArgs<PREC, 2>& args, const ScanScales& sc, int64_t Mc, int64_t P,
                             int64_t n_real, const double* rWr, const double* h2, double nu, const int64_t* row_index,
@@ -89,51 +89,75 @@ struct ScreenArgs {
   int64_t b_rows_pad;
   int nchunks;         // K / 32 of the first product (sWy)
   int nchunks2;        // K / 32 of the second (sWs): < nchunks for factored weights, == nchunks for exact ones
-  const int* order;    // NULL: columns in plane order; else the screened columns' layout (ScreenOrder): the tile
-  const int* counts;   // runs plane slots counts[0] .. counts[0] + counts[1] - 1, column order[slot]
 };
-// The screened columns' layout when the uniform-delta split is on (int32 workspace of screen_order_ints(n) ints, n
-// the screened columns): counts[0] columns at log delta == the uniform value take plane slots 0 .. counts[0] - 1,
-// the counts[1] others the slots after, each group in column order; order[slot] = column, pos[column] = slot
-// (columns and slots counted from the first screened column).
+// The screened columns' layout for the bucketed screen (int32 workspace of screen_order_ints(n) ints, n the
+// screened columns): column i's log delta t falls in bucket floor((t - t_lo) / t_h) (clamped to [0, nb)); bucket b's
+// columns hold plane slots bstart[b] .. bstart[b] + bcount[b] - 1, bstart[b] a multiple of SO_PAD, the slots up to
+// bstart[b + 1] are padding (order = -1); head[0] = the slot count S = bstart[nb], head[1] = nb, tb = {t_lo, t_h};
+// order[slot] = column, pos[column] = slot, gbucket[g] = the bucket of slots SO_PAD g .. SO_PAD g + SO_PAD - 1
+// (columns and slots counted from the first screened column).  The order within a bucket is not fixed; nothing
+// depends on it.
+constexpr int SO_NB_MAX = 120;  // buckets (2 SO_NB_MAX + 1 nodes fit SO_NODES)
+constexpr int SO_PAD = 32;     // slot alignment of a bucket: one MFMA column block
+__host__ __device__ inline int64_t screen_order_up64(int64_t n) { return (n + 63) / 64 * 64; }
+__host__ __device__ inline int64_t screen_order_slots(int64_t n) { return n + (int64_t)SO_PAD * SO_NB_MAX; }  // the most slots S
+__host__ __device__ inline int64_t screen_order_nblk(int64_t n) { return (n + 1023) / 1024; }
 struct ScreenOrder {
-  int* counts;
-  int* order;
-  int* pos;
-  int* blk;
+  int* head;     // [0] S, [1] nb
+  double* tb;    // [0] t_lo, [1] t_h
+  int* bcount;   // SO_NB_MAX
+  int* bstart;   // SO_NB_MAX + 1
+  int* order;    // screen_order_slots(n)
+  int* pos;      // n
+  int* gbucket;  // screen_order_slots(n) / SO_PAD
+  int* blk;      // per 1,024-column block and bucket: counts, then offsets (nblk x SO_NB_MAX)
 };
-inline int64_t screen_order_up64(int64_t n) { return (n + 63) / 64 * 64; }
-inline int64_t screen_order_ints(int64_t n) {
-  return 64 + 2 * screen_order_up64(n) + screen_order_up64((n + 1023) / 1024);
+__host__ __device__ inline int64_t screen_order_ints(int64_t n) {
+  return 64 + 128 + 128 + screen_order_up64(screen_order_slots(n)) + screen_order_up64(n) +
+         screen_order_up64(screen_order_slots(n) / SO_PAD) + screen_order_up64(screen_order_nblk(n) * SO_NB_MAX);
 }
-inline ScreenOrder screen_order(void* ws, int64_t n) {
+__host__ __device__ inline ScreenOrder screen_order(void* ws, int64_t n) {
   int* p = (int*)ws;
   ScreenOrder o;
-  o.counts = p;
-  o.order = p + 64;
-  o.pos = o.order + screen_order_up64(n);
-  o.blk = o.pos + screen_order_up64(n);
+  o.head = p;
+  o.tb = (double*)(p + 8);
+  o.bcount = p + 64;
+  o.bstart = o.bcount + 128;
+  o.order = o.bstart + 128;
+  o.pos = o.order + screen_order_up64(screen_order_slots(n));
+  o.gbucket = o.pos + screen_order_up64(n);
+  o.blk = o.gbucket + screen_order_up64(screen_order_slots(n) / SO_PAD);
   return o;
 }
-// The uniform-delta screen: columns whose log delta equals one value share the sWs of every SNP row (d_m, exact in
-// FP64), so only the sWy product runs on MFMA.  row_f[m] = 1 / (s1_m^2 d_m) (< 0: the row is a forced candidate).
-struct UniformArgs {
-  const u16* s1;       // SNP-side hi planes of the first product, rows a_rows_pad
-  const u16* q1;       // column-side hi planes from the first screened column on (row stride b_rows_pad)
-  const float* q2s;    // the columns' second-product scales (from the first screened column on)
-  const float* row_f;  // per SNP row (a_rows_pad entries)
-  const int* order;    // the screened columns' layout (ScreenOrder): the tile runs slots 0 .. counts[0] - 1
-  const int* counts;
+// The bucketed screen tile: every screened column of bucket b has its sWs of SNP row m interpolated from the FP64
+// sums dnode[m][2b + j] = sum_k US[m][k]^2 / (s_k + exp(t_lo + (2b + j) t_h / 2)), j = 0, 1, 2 (1 / sWs quadratic
+// in log delta), so only the sWy product runs on MFMA.
+constexpr int SO_NODES = 256;  // node columns of dnode (2 SO_NB_MAX + 1 used at most)
+static_assert(2 * SO_NB_MAX + 1 <= SO_NODES, "the nodes fit dnode's columns");
+struct BucketArgs {
+  const u16* s1;         // SNP-side hi planes of the first product, rows a_rows_pad
+  const u16* q1;         // column-side hi planes, fragment-native layout by slot (frag_off), row stride b_rows_pad
+  const float* s1s;      // the SNP rows' power-of-two scales
+  const float* q2s;      // the columns' second-product scales (from the first screened column on)
+  const double* log_delta;  // the columns' log delta (from the first screened column on)
+  const double* dnode;   // (rows, SO_NODES) FP64
+  int64_t mc;            // SNP rows
+  const int* head;       // ScreenOrder
+  const double* tb;
+  const int* order;
+  const int* gbucket;
   int64_t a_rows_pad;
   int64_t b_rows_pad;
   int nchunks;
 };
-hipError_t launch_screen_classify(const double* log_delta, int64_t n, double ld_uniform, void* order_ws,
+hipError_t launch_screen_classify(const double* log_delta, int64_t n, double t_lo, double t_h, int nb,
+                                  void* order_ws, hipStream_t stream);
+// dnode of the bucketed screen for SNP rows m < mc of us (row stride ld_us, k < neff): node weights into
+// wnode (kpad x SO_NODES FP64), then dnode (rup(mc, 32) x SO_NODES FP64) from the layout's t_lo, t_h, nb
+hipError_t launch_screen_nodes(const double* us, int64_t ld_us, int64_t mc, const double* s, int64_t neff,
+                               int64_t kpad, const void* order_ws, double* wnode, double* dnode, hipStream_t stream);
+hipError_t launch_screen_bucketed(const BucketArgs& g, int64_t Mc, int64_t P, uint64_t* rec, int64_t rec_ld,
                                   hipStream_t stream);
-hipError_t launch_screen_rows(const double* US, int64_t ld_us, int64_t mc, int64_t mc_pad, const double* s,
-                              int64_t neff, double ld_uniform, const float* s1s, float* row_f, hipStream_t stream);
-hipError_t launch_screen_uniform(const UniformArgs& g, int64_t Mc, int64_t P, uint64_t* rec, int64_t rec_ld,
-                                 hipStream_t stream);
 hipError_t launch_wald_screen(const ScreenArgs& g, int64_t Mc, int64_t P, float dmin, uint64_t* rec, int64_t rec_ld,
                               hipStream_t stream);
 hipError_t launch_screen_resolve(const uint64_t* rec, int64_t rec_ld, int64_t Mc, int64_t P, int64_t p_first,

@@ -50,7 +50,6 @@ __device__ inline int sc_swz(int row) { return (row >> 2) & 3; }
 // n2 + 4 j + i (i = 0..3), all into accumulator set 0; then both products' chunks 2 jb and 2 jb + 1 (jb = j - nSA)
 // as slots (S1 Q1, S2 Q2, S1 Q1, S2 Q2) into sets 0, 1, 0, 1.  A stage's operands are fetched into registers one
 // stage ahead, the SNP side then stored into the other LDS buffer.
-template <bool FRAG>
 __device__ inline void screen_tile(const ScreenArgs& g, int64_t n_mtiles, int64_t n_ptiles, int64_t tile, int64_t Mc,
                                    int64_t P, float dmin, uint64_t* __restrict__ rec, int64_t rec_ld) {
   __shared__ __attribute__((aligned(16))) char lds0[SC_STAGE];
@@ -65,9 +64,7 @@ __device__ inline void screen_tile(const ScreenArgs& g, int64_t n_mtiles, int64_
   const int64_t mt = first_m + r_in % gm;
   const int64_t pt = r_in / gm;
   const int64_t m0 = mt * W_TM, p0 = pt * W_TP;
-  // with a layout, the tile's columns are plane slots slot0 + j (n_ptiles covers the n_cols of them)
-  const int64_t n_cols = g.order ? (int64_t)gmem(g.counts)[1] : P;
-  const int64_t slot0 = g.order ? (int64_t)gmem(g.counts)[0] : 0;
+  const int64_t n_cols = P;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int h = lane >> 5;
@@ -99,15 +96,12 @@ __device__ inline void screen_tile(const ScreenArgs& g, int64_t n_mtiles, int64_
   const uint32_t a_lane = wave * 2048 + lane * 16;
   const int a_row0 = wave * 32 + lane / 4;
   // column side: lane (column p0 + 64 wave + 32 bj + lane % 32, half h) reads bytes [32 h, 32 h + 32) of its row
-  // (row layout; FRAG: its 16 bytes of the step's 1 KiB fragment, frag_off); columns past n_cols read the last one,
-  // their records are not written
-  constexpr uint32_t QSTEP = FRAG ? 1024u : 16u;
+  // (columns past n_cols read the last one; their records are not written)
   uint32_t b_off[2];
 #pragma unroll
   for (int bj = 0; bj < 2; ++bj) {
     const int64_t j = p0 + wave * 64 + bj * 32 + (lane & 31);
-    const int64_t sl = slot0 + (j < n_cols ? j : n_cols - 1);
-    b_off[bj] = FRAG ? (uint32_t)((sl >> 5) * 2048 + (h * 32 + (sl & 31)) * 16) : (uint32_t)(sl * 64 + h * 32);
+    b_off[bj] = (uint32_t)((j < n_cols ? j : n_cols - 1) * 64 + h * 32);
   }
 
   // uniform chunk base of slot i of stage j, SNP side (a) and column side (b)
@@ -136,7 +130,7 @@ __device__ inline void screen_tile(const ScreenArgs& g, int64_t n_mtiles, int64_
 #pragma unroll
       for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-        for (int q = 0; q < 2; ++q) b.v[i][bj][q] = *gmem((const u32x4*)(sb + (b_off[bj] + q * QSTEP)));
+        for (int q = 0; q < 2; ++q) b.v[i][bj][q] = *gmem((const u32x4*)(sb + (b_off[bj] + q * 16)));
     }
     return b;
   };
@@ -292,49 +286,54 @@ __device__ inline void screen_tile(const ScreenArgs& g, int64_t n_mtiles, int64_
   if (pc < n_cols) {
     // high word: m2 rounded down to 128 units in the last place, bit 0 = forced
     const uint32_t x2 = (m2 & ~127u) | (dlo < dmin_b ? 1u : 0u);
-    const int64_t col = g.order ? (int64_t)gmem(g.order)[slot0 + pc] : pc;
-    gmem(rec)[mt * rec_ld + col] = ((uint64_t)x2 << 32) | m1;
+    gmem(rec)[mt * rec_ld + pc] = ((uint64_t)x2 << 32) | m1;
   }
 }
 
-// The grid covers every column; with a layout only the first n_mtiles x ceil(count / 256) blocks have a tile
-// (the rest exit), so the live tiles spread evenly over the XCDs (blocks go to XCDs round robin) and xcd_remap over
-// the live count keeps each XCD's tiles adjacent.
-__device__ inline int64_t live_ptiles(const int* count, int64_t P) {
-  const int64_t n = count ? (int64_t)*gmem(count) : P;
-  return (n + W_TP - 1) / W_TP;
+__global__ __launch_bounds__(256, 1) void wald_screen_kernel(ScreenArgs g, int64_t n_mtiles, int64_t n_ptiles,
+                                                             int64_t n_tiles, int64_t Mc, int64_t P, float dmin,
+                                                             uint64_t* __restrict__ rec, int64_t rec_ld) {
+  screen_tile(g, n_mtiles, n_ptiles, xcd_remap(blockIdx.x, n_tiles), Mc, P, dmin, rec, rec_ld);
 }
 
-// FRAG: the column planes in the fragment-native layout (with the uniform-delta layout, g.order set)
-template <bool FRAG>
-__global__ __launch_bounds__(256, 1) void wald_screen_kernel(ScreenArgs g, int64_t n_mtiles, int64_t Mc, int64_t P,
-                                                             float dmin, uint64_t* __restrict__ rec, int64_t rec_ld) {
-  const int64_t n_pt = live_ptiles(g.order ? g.counts + 1 : nullptr, P);
-  const int64_t n_live = n_mtiles * n_pt;
-  if ((int64_t)blockIdx.x >= n_live) return;
-  screen_tile<FRAG>(g, n_mtiles, n_pt, xcd_remap(blockIdx.x, n_live), Mc, P, dmin, rec, rec_ld);
-}
-
-// ---- uniform-delta screen ----
-// Permutation columns with h2 ~ 0 mostly settle at the grid's upper end, all at the same log delta.  For such columns
-// the weights w_k = 1 / (s_k + delta) are shared, so sWs of SNP row m is one number d_m (computed in FP64 by
-// screen_rows_kernel), and the tile runs the sWy product alone: 128 accumulators per lane, so two workgroups share
-// each CU and one's epilogue and load waits overlap the other's MFMAs.  With A the raw hi-plane sum (s1_m q1_p a),
-//   x = A^2 f_m / q2_p,   f_m = 1 / (s1_m^2 d_m)
-// is the general tile's x = q1^2 a^2 / (q2 d) with d exact (rounded once to FP32 in f_m, well inside kD), so the
-// records mean the same and one resolution serves both.  1 / q2_p is a power of two applied to the packed maxima
-// at the end (exact; a result outside the normal range marks the record forced).
+// ---- bucketed screen ----
+// sWs of SNP row m for a column at log delta t is d_m(t) = sum_k c_k f_k(t), c_k = US_mk^2, f_k = 1 / (s_k + e^t):
+// positive, decreasing and smooth in t.  The screened columns are bucketed by t (ScreenOrder: buckets of width
+// h <= 1/5 from t_lo, their plane slots adjacent); for bucket b the FP64 sums D_j at the three nodes t_b + j h / 2
+// (dnode, one FP64 GEMM of US^2 with the node weights per SNP block) give g = 1 / d by quadratic interpolation,
+// g~ = G0 + u (C1 + u C2) (G_j = 1 / D_j, C1 = -3 G0 + 4 G1 - G2, C2 = 2 (G0 - 2 G1 + G2), u = (t - t_b) / h), so
+// the tile's epilogue takes two FMAs and two products per test.  Error: let q_k = e^t / (s_k + e^t) in [0, 1] and m_i = E[q^i] under the weights c_k f_k / d, and write
+// L1, L2, L3 for the first three derivatives of log d.  Then L1 = -m1, L2 = -m1' with m1' = E[q (1 - q)] - Var q
+// (modulus <= 1/4), L3 = -m1'' with m1'' = m1' (1 + 2 m1) - 2 (E[2 q^2 (1 - q)] - Cov(q, q^2)) (modulus at most
+// 3/4 + 16/27), and the third derivative of g = exp(-log d) is (m1^3 + 3 m1 L2 - L3) g, of modulus at most 3.1 g.
+// g grows by at most e^h over a bucket, so |g~ - g| <= 3.1 e^h 0.0481 h^3 g / 6 (0.0481 h^3 = the nodes' largest
+// |(t - t0)(t - t1)(t - t2)|): 2.4e-4 g at h = 1/5, and d~ = 1 / g~ is within 2.5e-4 of d, inside kD together with
+// the FP32 evaluation (~1e-6).  The tile runs the sWy product alone: 128 accumulators per lane, so two workgroups
+// share each CU and one's epilogue and load waits overlap the other's MFMAs.  With A the raw hi-plane sum
+// (s1_m q1_p a),
+//   x = A^2 g~ / (s1_m^2 q2_p)     (the tile keeps G0, C1, C2 over s1_m^2 per row, from FP64)
+// is the general tile's x = q1^2 a^2 / (q2 d) with d~ for the exact d, so the records mean the same and one
+// resolution serves both; 1 / q2_p is a power of two applied to the packed maxima at the end (exact; a result outside
+// the normal range marks the record forced).  Rows whose node sums are not positive, or whose scaled reciprocals are
+// not finite FP32 numbers, get G = 0 (x = 0) and make their tile forced; padding rows get G = 0.
 constexpr int SU_SLOTS = 2;                        // K-chunks per stage
+
+// the median of three unsigned values (one VALU instruction)
+__device__ inline uint32_t umed3(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
 constexpr int SU_STAGE = SU_SLOTS * SC_SLOT_LDS;   // 16 KiB
 
-__global__ __launch_bounds__(256, 2) void screen_uniform_kernel(UniformArgs g, int64_t n_mtiles,
-                                                                uint64_t* __restrict__ rec, int64_t rec_ld) {
+__global__ __launch_bounds__(256, 2) void screen_bucketed_kernel(BucketArgs g, int64_t n_mtiles,
+                                                                 uint64_t* __restrict__ rec, int64_t rec_ld) {
   __shared__ __attribute__((aligned(16))) char lds0[SU_STAGE];
   __shared__ __attribute__((aligned(16))) char lds1[SU_STAGE];
-  __shared__ __attribute__((aligned(16))) float row_f[W_TM];
-  __shared__ int tile_forced;
+  __shared__ __attribute__((aligned(16))) float fac[4][2][3][W_TM];  // [wave][column block][coefficient][row]
+  __shared__ int blk_forced[4][2];
 
-  const int64_t n_cols = gmem(g.counts)[0];
+  const int64_t n_cols = gmem(g.head)[0];  // plane slots, padding included
   const int64_t n_ptiles = (n_cols + W_TP - 1) / W_TP;
   const int64_t n_live = n_mtiles * n_ptiles;
   if ((int64_t)blockIdx.x >= n_live) return;
@@ -350,12 +349,42 @@ __global__ __launch_bounds__(256, 2) void screen_uniform_kernel(UniformArgs g, i
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int h = lane >> 5;
 
-  if (threadIdx.x == 0) tile_forced = 0;
+  // the buckets of the tile's eight 32-slot column blocks (every block lies in one bucket; -1 past the slots)
+  auto block_bucket = [&](int w, int bj) -> int {
+    const int64_t s0 = p0 + w * 64 + bj * 32;
+    return s0 < n_cols ? gmem(g.gbucket)[s0 / SO_PAD] : -1;
+  };
+  if (threadIdx.x < 8) blk_forced[threadIdx.x >> 1][threadIdx.x & 1] = 0;
   __syncthreads();
-  if (threadIdx.x < W_TM) {
-    const float f = gmem(g.row_f)[m0 + threadIdx.x];
-    row_f[threadIdx.x] = f > 0.f ? f : 0.f;
-    if (f < 0.f) tile_forced = 1;
+  // the interpolant of 1 / (s1_m^2 sWs) of the tile's 128 rows for each column block's bucket (G0, C1, C2 from the
+  // reciprocal node sums in FP64, rounded to FP32; 0 on padding and bad rows)
+  for (int q = threadIdx.x; q < 8 * W_TM; q += 256) {
+    const int wb = q / W_TM, r = q % W_TM, w = wb >> 1, bj = wb & 1;
+    const int bk = block_bucket(w, bj);
+    const int64_t m = m0 + r;
+    float f[3] = {0.f, 0.f, 0.f};
+    if (bk >= 0 && m < g.mc) {
+      const double sc = (double)gmem(g.s1s)[m];
+      bool bad = false;
+      double gj[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double d = gmem(g.dnode)[m * SO_NODES + 2 * bk + j];
+        gj[j] = 1.0 / (d * sc * sc);
+        bad |= !(d > 0.0);
+      }
+      f[0] = (float)gj[0];
+      f[1] = (float)(-3.0 * gj[0] + 4.0 * gj[1] - gj[2]);
+      f[2] = (float)(2.0 * (gj[0] - 2.0 * gj[1] + gj[2]));
+#pragma unroll
+      for (int j = 0; j < 3; ++j) bad |= !(fabsf(f[j]) < INFINITY);
+      if (bad) {
+        blk_forced[w][bj] = 1;
+        f[0] = f[1] = f[2] = 0.f;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) fac[w][bj][j][r] = f[j];
   }
 
   const int64_t a_kstride = g.a_rows_pad * 64;
@@ -365,8 +394,8 @@ __global__ __launch_bounds__(256, 2) void screen_uniform_kernel(UniformArgs g, i
   // SNP side: thread t copies 16-byte segment t % 4 of rows t / 4 and 64 + t / 4 of every chunk
   const uint32_t a_lane = threadIdx.x * 16;
   const int a_row0 = threadIdx.x >> 2;
-  // the uniform columns hold plane slots 0 .. n_cols - 1 in the fragment-native layout (frag_off): lane (c, h) of
-  // column block bj reads its 16 bytes of the step's 1 KiB fragment
+  // column side (fragment-native layout): lane (c, h) of column block bj reads its 16 bytes of the step's 1 KiB
+  // fragment; slots past n_cols read the last one (their records are not written)
   uint32_t b_off[2];
 #pragma unroll
   for (int bj = 0; bj < 2; ++bj) {
@@ -472,8 +501,25 @@ __global__ __launch_bounds__(256, 2) void screen_uniform_kernel(UniformArgs g, i
     stage(j + 1, lds1, lds0, b1, b0);
   }
 
+  // each lane's column: u = (t - t_b) / h in its bucket, t its log delta; a column outside its bucket (log delta past
+  // the buckets' range, clamped into the first or last) is forced
+  const double t_lo = gmem(g.tb)[0], t_h = gmem(g.tb)[1];
+  int col[2];
+  bool out_b[2];
+  float uu[2];
+#pragma unroll
+  for (int bj = 0; bj < 2; ++bj) {
+    const int64_t j = p0 + wave * 64 + bj * 32 + (lane & 31);
+    const int bk = block_bucket(wave, bj);
+    col[bj] = (j < n_cols && bk >= 0) ? gmem(g.order)[j] : -1;
+    double u = 0.0;
+    if (col[bj] >= 0) u = (gmem(g.log_delta)[col[bj]] - (t_lo + bk * t_h)) / t_h;
+    out_b[bj] = !(u >= -1e-9 && u <= 1.0 + 1e-9);
+    uu[bj] = (float)fmin(fmax(u, 0.0), 1.0);
+  }
+
   // epilogue from the accumulators: lane (h, c) holds column 32 bj + c of its wave and rows 32 bi + 8 q + 4 h + i;
-  // x = A^2 f_m, packed as in the general tile ((bits(x) & ~127) | row), top two per column over the lane's 64
+  // x = A^2 g~ / s1^2, packed as in the general tile ((bits(x) & ~127) | row), top two per column over the lane's 64
   // rows, then merged with lane c + 32 (the column's other 64 rows)
   uint32_t m1[2] = {0u, 0u}, m2[2] = {0u, 0u};
 #pragma unroll
@@ -481,18 +527,23 @@ __global__ __launch_bounds__(256, 2) void screen_uniform_kernel(UniformArgs g, i
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int r0 = bi * 32 + 8 * q + 4 * h;
-      const f32x4 f4 = *(const f32x4*)&row_f[r0];
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
+      for (int bj = 0; bj < 2; ++bj) {
+        const f32x4 d0 = *(const f32x4*)&fac[wave][bj][0][r0];
+        const f32x4 d1 = *(const f32x4*)&fac[wave][bj][1][r0];
+        const f32x4 d2 = *(const f32x4*)&fac[wave][bj][2][r0];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const float a = acc[bi][bj][4 * q + e];
-          const float x = (a * a) * f4[e];
-          const uint32_t xp = (__float_as_uint(x) & ~127u) | (uint32_t)(r0 + e);
-          const uint32_t lo = xp < m1[bj] ? xp : m1[bj];
-          m2[bj] = lo > m2[bj] ? lo : m2[bj];
+          const float gg = fmaf(uu[bj], fmaf(uu[bj], d2[e], d1[e]), d0[e]);
+          // gg >= 0 up to rounding (g~ is within 2.5e-4 of g > 0, or the row's coefficients are 0); the sign bit is
+          // dropped with the low bits, so -0 packs as 0
+          const float x = (a * a) * gg;
+          const uint32_t xp = (__float_as_uint(x) & 0x7fffff80u) | (uint32_t)(r0 + e);
+          m2[bj] = umed3(xp, m1[bj], m2[bj]);  // = max(m2, min(xp, m1)) as m2 <= m1
           m1[bj] = xp > m1[bj] ? xp : m1[bj];
         }
+      }
     }
 #pragma unroll
   for (int bj = 0; bj < 2; ++bj) {
@@ -502,14 +553,13 @@ __global__ __launch_bounds__(256, 2) void screen_uniform_kernel(UniformArgs g, i
     m2[bj] = lo > hi2 ? lo : hi2;
     m1[bj] = m1[bj] > o1 ? m1[bj] : o1;
   }
-  // lane (h, c) writes column 32 h + c of its wave
+  // lane (h, c) writes column block h's column c
   const uint32_t w1 = h ? m1[1] : m1[0], w2 = h ? m2[1] : m2[0];
-  const int64_t pc = p0 + wave * 64 + lane;
-  if (pc < n_cols) {
-    const int64_t col = gmem(g.order)[pc];
+  const int cw = h ? col[1] : col[0];
+  if (cw >= 0) {
     // 1 / q2 = 2^e: add e to the exponent of both packed values (the low 7 bits carry the row / flag)
-    const int e = 127 - (int)((__float_as_uint(gmem(g.q2s)[col]) >> 23) & 255u);
-    bool bad = tile_forced != 0;
+    const int e = 127 - (int)((__float_as_uint(gmem(g.q2s)[cw]) >> 23) & 255u);
+    bool bad = blk_forced[wave][h] != 0 || (h ? out_b[1] : out_b[0]);
     auto scale = [&](uint32_t v) -> uint32_t {
       if ((v & ~127u) == 0u) return v & 127u;
       const int ex = (int)((v >> 23) & 255u);
@@ -522,134 +572,217 @@ __global__ __launch_bounds__(256, 2) void screen_uniform_kernel(UniformArgs g, i
     const uint32_t s1v = scale(w1);
     const uint32_t s2v = scale(w2 & ~127u);
     const uint32_t x2 = (s2v & ~127u) | (bad ? 1u : 0u);
-    gmem(rec)[mt * rec_ld + col] = ((uint64_t)x2 << 32) | s1v;
+    gmem(rec)[mt * rec_ld + cw] = ((uint64_t)x2 << 32) | s1v;
   }
 }
 
-// d_m = sum_k US[m][k]^2 / (s_k + delta) in FP64 at delta = exp(ld_uniform) (the resolution's arithmetic), one wave
-// per row; row_f[m] = 1 / (s1_m^2 d_m) rounded to FP32, -1 where d_m is not positive or f overflows (a forced
-// row), 0 on the padding rows mc <= m < mc_pad.
-__global__ __launch_bounds__(256) void screen_rows_kernel(const double* __restrict__ US, int64_t ld_us, int64_t mc,
-                                                          int64_t mc_pad, const double* __restrict__ s, int64_t neff,
-                                                          double ld_uniform, const float* __restrict__ s1s,
-                                                          float* __restrict__ row_f) {
-  const int64_t m = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (m >= mc_pad) return;
-  if (m >= mc) {
-    if (lane == 0) row_f[m] = 0.f;
-    return;
-  }
-  const double delta = exp(ld_uniform);
-  const double* u = US + m * ld_us;
-  double d = 0.0;
-  for (int64_t k = lane; k < neff; k += 64) {
-    const double uk = gmem(u)[k];
-    d += uk * uk * rcp_pos(gmem(s)[k] + delta);
-  }
-  d = wave_sum(d);
-  if (lane == 0) {
-    const double sc = (double)gmem(s1s)[m];
-    const double f = 1.0 / (sc * sc * d);
-    row_f[m] = (d > 0.0 && f < 0x1p100) ? (float)f : -1.f;
-  }
-}
-
-// The screened columns' layout (ScreenOrder): a stable split into those at log delta == ld_uniform and the rest,
-// by per-block counts, one block's exclusive scan of them, then each block's ranks by wave ballots.
+// The screened columns' layout (ScreenOrder): order[] cleared to -1, per-1,024-column block counts of each bucket,
+// one block's scan of them (bucket starts padded to SO_PAD, the slot groups' buckets), then each column's slot from
+// its block's offset and a rank by LDS atomics.
 constexpr int CL_BLOCK = 1024;
 
-__global__ __launch_bounds__(CL_BLOCK) void screen_count_kernel(const double* __restrict__ ld, int64_t n, double lu,
-                                                               int* __restrict__ blk) {
-  __shared__ int wc[CL_BLOCK / 64];
-  const int64_t i = (int64_t)blockIdx.x * CL_BLOCK + threadIdx.x;
-  const bool u = i < n && gmem(ld)[i] == lu;
-  const uint64_t b = __ballot(u);
-  if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(b);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int t = 0;
-    for (int w = 0; w < CL_BLOCK / 64; ++w) t += wc[w];
-    blk[blockIdx.x] = t;
-  }
+__device__ inline int bucket_of(double t, double t_lo, double t_h, int nb) {
+  const double v = (t - t_lo) / t_h;
+  const int b = v >= 0.0 ? (v < (double)nb ? (int)v : nb - 1) : 0;  // floor, clamped to [0, nb); NaN to 0
+  return b;
 }
 
-__global__ __launch_bounds__(CL_BLOCK) void screen_offsets_kernel(int* __restrict__ blk, int64_t nb, int64_t n,
-                                                                 int* __restrict__ counts) {
-  __shared__ int part[CL_BLOCK];
-  const int64_t per = (nb + CL_BLOCK - 1) / CL_BLOCK;
-  const int64_t b0 = threadIdx.x * per, b1 = b0 + per < nb ? b0 + per : nb;
-  int t = 0;
-  for (int64_t b = b0; b < b1; ++b) t += blk[b];
-  part[threadIdx.x] = t;
-  __syncthreads();
-  for (int off = 1; off < CL_BLOCK; off <<= 1) {  // inclusive scan (Hillis-Steele)
-    const int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
-  }
-  int run = threadIdx.x ? part[threadIdx.x - 1] : 0;
-  for (int64_t b = b0; b < b1; ++b) {
-    const int c = blk[b];
-    blk[b] = run;
-    run += c;
-  }
-  if (threadIdx.x == CL_BLOCK - 1) {
-    counts[0] = part[CL_BLOCK - 1];
-    counts[1] = (int)(n - part[CL_BLOCK - 1]);
-  }
+__global__ __launch_bounds__(256) void screen_fill_kernel(int* __restrict__ order, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) order[i] = -1;
 }
 
-__global__ __launch_bounds__(CL_BLOCK) void screen_scatter_kernel(const double* __restrict__ ld, int64_t n, double lu,
-                                                                 const int* __restrict__ blk,
-                                                                 const int* __restrict__ counts,
-                                                                 int* __restrict__ order, int* __restrict__ pos) {
-  __shared__ int wc[CL_BLOCK / 64];
+__global__ __launch_bounds__(CL_BLOCK) void screen_hist_kernel(const double* __restrict__ ld, int64_t n, double t_lo,
+                                                              double t_h, int nb, int* __restrict__ blk) {
+  __shared__ int hist[SO_NB_MAX];
+  for (int b = threadIdx.x; b < SO_NB_MAX; b += CL_BLOCK) hist[b] = 0;
+  __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * CL_BLOCK + threadIdx.x;
-  const bool u = i < n && gmem(ld)[i] == lu;
-  const uint64_t b = __ballot(u);
+  if (i < n) atomicAdd(&hist[bucket_of(gmem(ld)[i], t_lo, t_h, nb)], 1);
+  __syncthreads();
+  for (int b = threadIdx.x; b < SO_NB_MAX; b += CL_BLOCK) blk[(int64_t)blockIdx.x * SO_NB_MAX + b] = hist[b];
+}
+
+// bucket b's blocks: exclusive prefix of their counts over the blocks (in place), and the bucket's count; one
+// 256-thread workgroup per bucket, 256 blocks per pass (wave scans by lane shuffles, then the four wave totals)
+__global__ __launch_bounds__(256) void screen_scan_kernel(int* __restrict__ blk, int64_t nblk,
+                                                          int* __restrict__ bcount) {
+  __shared__ int wsum[4];
+  const int b = blockIdx.x;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) wc[w] = __popcll(b);
+  int carry = 0;
+  for (int64_t k0 = 0; k0 < nblk; k0 += 256) {
+    const int64_t k = k0 + threadIdx.x;
+    const int c = k < nblk ? blk[k * SO_NB_MAX + b] : 0;
+    int x = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(x, d, 64);
+      if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int before = carry;
+    for (int v = 0; v < w; ++v) before += wsum[v];
+    if (k < nblk) blk[k * SO_NB_MAX + b] = before + x - c;
+    carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();  // wsum is rewritten by the next pass
+  }
+  if (threadIdx.x == 0) bcount[b] = carry;
+}
+
+// the buckets' starts (each padded to SO_PAD slots), the slot count and the bucket parameters
+__global__ __launch_bounds__(128) void screen_starts_kernel(int nb, double t_lo, double t_h,
+                                                            const int* __restrict__ bcount, int* __restrict__ head,
+                                                            double* __restrict__ tb, int* __restrict__ bstart) {
+  __shared__ int padded[SO_NB_MAX];
+  const int b = threadIdx.x;
+  if (b < SO_NB_MAX) padded[b] = b < nb ? (bcount[b] + SO_PAD - 1) / SO_PAD * SO_PAD : 0;
   __syncthreads();
-  int before = 0;  // uniform columns in this block's earlier waves
-  for (int v = 0; v < w; ++v) before += wc[v];
-  const uint64_t below = lane ? (b & ((1ull << lane) - 1ull)) : 0ull;
-  const int ur = before + __popcll(below);  // rank among the block's uniform columns
-  const int gr = w * 64 + lane - ur;        // rank among its other columns
+  if (b == 0) {
+    int st = 0;
+    for (int k = 0; k < SO_NB_MAX; ++k) {
+      bstart[k] = st;
+      st += padded[k];
+    }
+    bstart[SO_NB_MAX] = st;
+    head[0] = st;
+    head[1] = nb;
+    tb[0] = t_lo;
+    tb[1] = t_h;
+  }
+}
+
+// each column's slot: its bucket's start + its block's offset in the bucket + its rank in the block (LDS atomics);
+// and the bucket of every SO_PAD-slot group (-1 past the slots)
+__global__ __launch_bounds__(CL_BLOCK) void screen_scatter_kernel(const double* __restrict__ ld, int64_t n,
+                                                                 double t_lo, double t_h, int nb,
+                                                                 const int* __restrict__ blk,
+                                                                 const int* __restrict__ bstart,
+                                                                 int* __restrict__ order, int* __restrict__ pos,
+                                                                 int* __restrict__ gbucket, int64_t n_groups) {
+  __shared__ int rank[SO_NB_MAX];
+  __shared__ int st[SO_NB_MAX + 1];
+  for (int b = threadIdx.x; b <= SO_NB_MAX; b += CL_BLOCK) {
+    if (b < SO_NB_MAX) rank[b] = 0;
+    st[b] = gmem(bstart)[b];
+  }
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * CL_BLOCK + threadIdx.x;
   if (i < n) {
-    const int ub = blk[blockIdx.x];  // uniform columns in earlier blocks
-    const int slot = u ? ub + ur : gmem(counts)[0] + (int)((int64_t)blockIdx.x * CL_BLOCK - ub) + gr;
+    const int b = bucket_of(gmem(ld)[i], t_lo, t_h, nb);
+    const int slot = st[b] + gmem(blk)[(int64_t)blockIdx.x * SO_NB_MAX + b] + atomicAdd(&rank[b], 1);
     order[slot] = (int)i;
     pos[i] = slot;
   }
+  if (i < n_groups) {  // the launch covers the n columns and the n_groups groups
+    const int s0 = (int)(i * SO_PAD);
+    int g = -1;
+    if (s0 < st[nb]) {
+      int lo = 0, hi = nb - 1;  // the last bucket b with st[b] <= s0 (buckets may be empty: st repeats)
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (st[mid] <= s0) lo = mid; else hi = mid - 1;
+      }
+      g = lo;
+    }
+    gbucket[i] = g;
+  }
 }
 
-hipError_t launch_screen_classify(const double* log_delta, int64_t n, double ld_uniform, void* order_ws,
-                                  hipStream_t stream) {
+hipError_t launch_screen_classify(const double* log_delta, int64_t n, double t_lo, double t_h, int nb,
+                                  void* order_ws, hipStream_t stream) {
   const ScreenOrder o = screen_order(order_ws, n);
-  const int64_t nb = (n + CL_BLOCK - 1) / CL_BLOCK;
-  if (nb == 0) return hipSuccess;
-  hipLaunchKernelGGL(screen_count_kernel, dim3((unsigned)nb), dim3(CL_BLOCK), 0, stream, log_delta, n, ld_uniform,
+  const int64_t nblk = screen_order_nblk(n);
+  if (nblk == 0) return hipSuccess;
+  const int64_t ns = screen_order_slots(n);
+  const int64_t fb = (ns + 255) / 256 < 4096 ? (ns + 255) / 256 : 4096;
+  hipLaunchKernelGGL(screen_fill_kernel, dim3((unsigned)fb), dim3(256), 0, stream, o.order, ns);
+  hipLaunchKernelGGL(screen_hist_kernel, dim3((unsigned)nblk), dim3(CL_BLOCK), 0, stream, log_delta, n, t_lo, t_h, nb,
                      o.blk);
-  hipLaunchKernelGGL(screen_offsets_kernel, dim3(1), dim3(CL_BLOCK), 0, stream, o.blk, nb, n, o.counts);
-  hipLaunchKernelGGL(screen_scatter_kernel, dim3((unsigned)nb), dim3(CL_BLOCK), 0, stream, log_delta, n, ld_uniform,
-                     o.blk, o.counts, o.order, o.pos);
+  hipLaunchKernelGGL(screen_scan_kernel, dim3((unsigned)nb), dim3(256), 0, stream, o.blk, nblk, o.bcount);
+  hipLaunchKernelGGL(screen_starts_kernel, dim3(1), dim3(128), 0, stream, nb, t_lo, t_h, o.bcount, o.head, o.tb,
+                     o.bstart);
+  const int64_t n_groups = ns / SO_PAD;
+  const int64_t sblk = (n_groups + CL_BLOCK - 1) / CL_BLOCK > nblk ? (n_groups + CL_BLOCK - 1) / CL_BLOCK : nblk;
+  hipLaunchKernelGGL(screen_scatter_kernel, dim3((unsigned)sblk), dim3(CL_BLOCK), 0, stream, log_delta, n, t_lo, t_h,
+                     nb, o.blk, o.bstart, o.order, o.pos, o.gbucket, n_groups);
   return hipGetLastError();
 }
 
-hipError_t launch_screen_rows(const double* US, int64_t ld_us, int64_t mc, int64_t mc_pad, const double* s,
-                              int64_t neff, double ld_uniform, const float* s1s, float* row_f, hipStream_t stream) {
-  hipLaunchKernelGGL(screen_rows_kernel, dim3((unsigned)((mc_pad + 3) / 4)), dim3(256), 0, stream, US, ld_us, mc,
-                     mc_pad, s, neff, ld_uniform, s1s, row_f);
+// The node sums: wnode[k][j] = 1 / (s_k + exp(t_lo + j t_h / 2)) for k < neff and j <= 2 nb (0 elsewhere), then
+// dnode[i][j] = sum_k us[i][k]^2 wnode[k][j] in FP64 (rows past mc give 0): a 256-thread block owns 32 rows x 64
+// nodes, thread (g = tid / 32, c = tid % 32) accumulates rows 4 g .. 4 g + 3 at nodes c and c + 32, from K chunks of
+// 32 staged in LDS (the squares of us in a padded 32 x 33 tile).  Node blocks past 2 nb exit.
+__global__ __launch_bounds__(256) void screen_node_weights_kernel(const double* __restrict__ s, int64_t neff,
+                                                                  const int* __restrict__ head,
+                                                                  const double* __restrict__ tb,
+                                                                  double* __restrict__ wnode) {
+  const int64_t k = blockIdx.x;
+  const int j = threadIdx.x;
+  const int nb = gmem(head)[1];
+  double w = 0.0;
+  if (k < neff && j <= 2 * nb) w = 1.0 / (gmem(s)[k] + exp(gmem(tb)[0] + j * 0.5 * gmem(tb)[1]));
+  wnode[k * SO_NODES + j] = w;
+}
+
+__global__ __launch_bounds__(256) void screen_nodes_kernel(const double* __restrict__ us, int64_t ld_us, int64_t mc,
+                                                           int64_t kpad, const int* __restrict__ head,
+                                                           const double* __restrict__ wnode,
+                                                           double* __restrict__ dnode) {
+  __shared__ double ws[32][64];
+  __shared__ double xs[32][33];
+  const int tid = threadIdx.x, g = tid >> 5, c = tid & 31;
+  const int64_t i0 = (int64_t)blockIdx.x * 32;
+  const int r0 = blockIdx.y * 64;
+  if (r0 > 2 * gmem(head)[1]) return;  // block-uniform
+  double acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) acc[i][0] = acc[i][1] = 0.0;
+  for (int64_t k0 = 0; k0 < kpad; k0 += 32) {
+#pragma unroll
+    for (int e = tid; e < 32 * 64; e += 256) ws[e >> 6][e & 63] = wnode[(k0 + (e >> 6)) * SO_NODES + r0 + (e & 63)];
+#pragma unroll
+    for (int e = tid; e < 32 * 32; e += 256) {
+      const int64_t i = i0 + (e >> 5);
+      const double x = i < mc ? gmem(us)[i * ld_us + k0 + (e & 31)] : 0.0;
+      xs[e >> 5][e & 31] = x * x;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int k = 0; k < 32; ++k) {
+      const double a0 = ws[k][c], a1 = ws[k][c + 32];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const double x = xs[4 * g + i][k];
+        acc[i][0] = fma(x, a0, acc[i][0]);
+        acc[i][1] = fma(x, a1, acc[i][1]);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    dnode[(i0 + 4 * g + i) * SO_NODES + r0 + c] = acc[i][0];
+    dnode[(i0 + 4 * g + i) * SO_NODES + r0 + c + 32] = acc[i][1];
+  }
+}
+
+hipError_t launch_screen_nodes(const double* us, int64_t ld_us, int64_t mc, const double* s, int64_t neff,
+                               int64_t kpad, const void* order_ws, double* wnode, double* dnode, hipStream_t stream) {
+  if (mc == 0) return hipSuccess;
+  const ScreenOrder o = screen_order((void*)order_ws, 0);  // head and tb do not depend on the column count
+  hipLaunchKernelGGL(screen_node_weights_kernel, dim3((unsigned)kpad), dim3(SO_NODES), 0, stream, s, neff, o.head,
+                     o.tb, wnode);
+  const dim3 grid((unsigned)((mc + 31) / 32), SO_NODES / 64);
+  hipLaunchKernelGGL(screen_nodes_kernel, grid, dim3(256), 0, stream, us, ld_us, mc, kpad, o.head, wnode, dnode);
   return hipGetLastError();
 }
 
-hipError_t launch_screen_uniform(const UniformArgs& g, int64_t Mc, int64_t P, uint64_t* rec, int64_t rec_ld,
-                                 hipStream_t stream) {
+hipError_t launch_screen_bucketed(const BucketArgs& g, int64_t Mc, int64_t P, uint64_t* rec, int64_t rec_ld,
+                                  hipStream_t stream) {
   const int64_t n_mtiles = (Mc + W_TM - 1) / W_TM;
-  const int64_t n_tiles = n_mtiles * ((P + W_TP - 1) / W_TP);
-  hipLaunchKernelGGL(screen_uniform_kernel, dim3((unsigned)n_tiles), dim3(256), 0, stream, g, n_mtiles, rec, rec_ld);
+  const int64_t n_tiles = n_mtiles * ((screen_order_slots(P) + W_TP - 1) / W_TP);  // every possible slot tile
+  hipLaunchKernelGGL(screen_bucketed_kernel, dim3((unsigned)n_tiles), dim3(256), 0, stream, g, n_mtiles, rec, rec_ld);
   return hipGetLastError();
 }
 
@@ -787,13 +920,10 @@ __global__ __launch_bounds__(256) void screen_resolve_kernel(
 hipError_t launch_wald_screen(const ScreenArgs& g, int64_t Mc, int64_t P, float dmin, uint64_t* rec, int64_t rec_ld,
                               hipStream_t stream) {
   const int64_t n_mtiles = (Mc + W_TM - 1) / W_TM;
-  const int64_t n_tiles = n_mtiles * ((P + W_TP - 1) / W_TP);
-  if (g.order)
-    hipLaunchKernelGGL(wald_screen_kernel<true>, dim3((unsigned)n_tiles), dim3(256), 0, stream, g, n_mtiles, Mc, P,
-                       dmin, rec, rec_ld);
-  else
-    hipLaunchKernelGGL(wald_screen_kernel<false>, dim3((unsigned)n_tiles), dim3(256), 0, stream, g, n_mtiles, Mc, P,
-                       dmin, rec, rec_ld);
+  const int64_t n_ptiles = (P + W_TP - 1) / W_TP;
+  const int64_t n_tiles = n_mtiles * n_ptiles;
+  hipLaunchKernelGGL(wald_screen_kernel, dim3((unsigned)n_tiles), dim3(256), 0, stream, g, n_mtiles, n_ptiles,
+                     n_tiles, Mc, P, dmin, rec, rec_ld);
   return hipGetLastError();
 }
 

@@ -36,13 +36,18 @@ _D = C.c_double
 
 NOALLOC_SCRATCH_BYTES = 64 << 20  # FLMM_NOALLOC_SCRATCH_BYTES
 
+# flmm_wald_scan_screened and its two halves
+_SCREENED = [_P, _P, _P, _P, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _P, _I64, _P, _P, _I64,
+             _D, _D, _P, _P, _D, _P, _P, _P, _P, _P, _P]
+
 # name -> argtypes, in header order
 SIGNATURES: dict[str, list] = {
     "flmm_abi_version": [],
     "flmm_last_error": [],
     "flmm_scan_table_bytes": [_I],
-    "flmm_screen_work_bytes": [_I64],
+    "flmm_screen_work_bytes": [_I64, _I64],
     "flmm_screen_order_bytes": [_I64],
+    "flmm_screen_q_rows": [_I64, _I64, _I64],
     "flmm_scan_table_fill": [_I, _I, _P, _I, _P],
     "flmm_wald_scan_grouped": [_I, _I, _P, _I, _P],
     "flmm_max_reduce_grouped": [_P, _I, _I64, _P],
@@ -55,7 +60,7 @@ SIGNATURES: dict[str, list] = {
     "flmm_delta_grid": [_P, _P, _I, _D, _P, _I64, _I64, _I64, _P, _P],
     "flmm_delta_grid_split": [_P, _P, _I, _D, _P, _P, _D, _D, _D, _I64, _I64, _I64, _P, _P],
     "flmm_delta_refine": [_I, _I, _P, _I64, _I64, _I64, _P, _I64, _P, _I64, _D, _I, _D, _D, _P, _P, _P, _P, _P,
-                          _P, _P, _P, _P, _P, _P, _I64, _D, _P, _P],
+                          _P, _P, _P, _P, _P, _P, _I64, _I64, _D, _D, _I, _P, _P],
     "flmm_gram": [_I, _P, _P, _P, _I64, _I64, _I64, _I, _P, _P, _I64, _P],
     "flmm_gram_f64": [_P, _I64, _I64, _I64, _I, _P, _P, _I64, _P],
     "flmm_rotate_f64": [_P, _I64, _P, _I64, _I64, _P, _I64, _I64, _P, _I64, _I64, _P],
@@ -64,18 +69,15 @@ SIGNATURES: dict[str, list] = {
     "flmm_select_above": [_P, _I64, _I64, _I64, _D, _P, _P, _I64, _P, _P],
     "flmm_factored_weight_planes": [_I, _I64, _I64, _I64, _P, _P, _I64, _P, _P, _P, _P, _I64, _P, _P],
     "flmm_delta_refine_learned": [_I, _P, _I64, _I64, _I64, _P, _I64, _P, _I64, _D, _I, _D, _D, _P, _P, _D, _D, _P,
-                                  _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _D, _P, _P],
+                                  _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I64, _D, _D, _I, _P, _P],
     "flmm_mlp_features": [_P, _I64, _I64, _I64, _P, _D, _D, _P, _P],
     "flmm_mlp_dense_fwd": [_P, _I64, _I, _P, _P, _I, _I, _P, _P],
     "flmm_mlp_dense_bwd": [_P, _P, _P, _P, _I64, _I, _I, _I, _P, _P, _P, _P, _P],
     "flmm_mlp_mse_grad": [_P, _P, _I64, _P, _P],
     "flmm_adam": [_P, _P, _P, _P, _I64, _D, _D, _D, _D, _I, _P],
-    "flmm_wald_scan_screened": [_P, _P, _P, _P, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _I64,
-                                _P, _P, _I64, _D, _D, _P, _P, _D, _P, _P, _D, _P, _P, _P, _P],
-    "flmm_wald_screen": [_P, _P, _P, _P, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _I64, _P, _P,
-                         _I64, _D, _D, _P, _P, _D, _P, _P, _D, _P, _P, _P, _P],
-    "flmm_screen_resolve": [_P, _P, _P, _P, _I64, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _I64, _P,
-                            _P, _I64, _D, _D, _P, _P, _D, _P, _P, _D, _P, _P, _P, _P],
+    "flmm_wald_scan_screened": _SCREENED,
+    "flmm_wald_screen": _SCREENED,
+    "flmm_screen_resolve": _SCREENED,
     "flmm_wald_scan": [_I, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P,
                        _P, _D, _P, _P, _P, _P, _P, _P, _P, _P],
     "flmm_wald_scan_lowrank": [_I, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64,
@@ -89,7 +91,7 @@ SIGNATURES: dict[str, list] = {
 
 # entry points that do not return an int status
 RESTYPES = {"flmm_last_error": C.c_char_p, "flmm_scan_table_bytes": C.c_int64, "flmm_screen_work_bytes": C.c_int64,
-            "flmm_screen_order_bytes": C.c_int64}
+            "flmm_screen_order_bytes": C.c_int64, "flmm_screen_q_rows": C.c_int64}
 
 
 class ScanGroup(C.Structure):

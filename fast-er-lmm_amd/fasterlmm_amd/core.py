@@ -307,20 +307,20 @@ class _Basis:
         return d.log_delta, d
 
     def delta_for(self, log_delta: Tensor, n_real: int | None = None) -> engine.DeltaResult:
-        """The column planes for given log delta values (n_real: as in fit).  The columns at the largest value share
-        their weights (the screen's uniform-delta split)."""
+        """The column planes for given log delta values (n_real: as in fit); the screened columns' bucketed layout
+        covers the values' range."""
         d = self.delta
         if d is not None and d.log_delta.data_ptr() == log_delta.data_ptr() and d.q1.prec == self.prec:
             return d
         fac = None
-        hi = None
+        t_range = None
         if log_delta.numel() > 0:
-            lo, hi = (float(v) for v in torch.aminmax(log_delta.double()))
-            fac = self.factored(lo, hi)
+            t_range = tuple(float(v) for v in torch.aminmax(log_delta.double()))
+            fac = self.factored(*t_range)
         hf = engine.screen_start(self.prec, n_real, self.P) if self.neff > 0 else None
         return engine.delta_refine(self.prec, _lib.DELTA_FIXED, self.uy, self.P, self.p_pad, self.kpad, self.s,
                                    self.s, float(self.neff), 1, 0.0, 0.0, None, log_delta_in=log_delta, fac=fac,
-                                   hi_from=hf, ld_uniform=hi)
+                                   hi_from=hf, t_range=t_range)
 
     def scan(self, Z: Tensor, row_index: Tensor, n_real: int, out: engine.ScanOutputs, d: engine.DeltaResult,
              block: int = 8192, after_launch=None, first_row: int | None = None) -> None:

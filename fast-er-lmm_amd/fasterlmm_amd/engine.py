@@ -289,19 +289,19 @@ class DeltaResult:
     neff: int = 0
     fac: Factored | None = None
     hi_from: int | None = None  # columns from here on have the hi planes only (the screen's); None: all have both
-    ld_uniform: float | None = None  # a log delta many columns hold exactly (the fit grid's upper end), or None
-    order: torch.Tensor | None = None  # the screened columns' plane layout (uniform-delta split), or None
+    layout: ScreenLayout | None = None  # the screened columns' bucketed plane layout, or None
+    p_pad: int = 0           # rows of the rotated columns' plane (the Q planes have q1.rows_pad >= p_pad)
 
 
 def attach_factored(prec: int, d: DeltaResult, fac: Factored) -> None:
     """Write the factored Q2 planes B[p][r] = lam_r exp(-t_r delta_p) for d's columns and switch d to them."""
     P = d.log_delta.numel()
-    p_pad = d.q1.rows_pad
-    q2 = Planes(prec, p_pad, fac.rpad, d.q1.hi.device)
+    q_rows = d.q1.rows_pad
+    q2 = Planes(prec, q_rows, fac.rpad, d.q1.hi.device)
     q2h, q2l = q2.ptrs
-    _lib.call("flmm_factored_weight_planes", prec, P, p_pad, fac.rpad, fac.t.data_ptr(), fac.lam.data_ptr(), fac.R,
-              d.log_delta.data_ptr(), q2h, q2l, q2.scale_ptr, _hi_from_arg(d.hi_from, p_pad), _lib.ptr(d.order),
-              _lib.stream())
+    _lib.call("flmm_factored_weight_planes", prec, P, q_rows, fac.rpad, fac.t.data_ptr(), fac.lam.data_ptr(), fac.R,
+              d.log_delta.data_ptr(), q2h, q2l, q2.scale_ptr, _hi_from_arg(d.hi_from, q_rows),
+              _lib.ptr(d.layout.ws if d.layout else None), _lib.stream())
     d.q2, d.fac = q2, fac
 
 
@@ -392,47 +392,79 @@ def screen_start(prec: int, n_real: int | None, P: int) -> int | None:
     return p0 if p0 < P else None
 
 
-def screen_order(hi_from: int | None, P: int, ld_uniform: float | None, dev) -> torch.Tensor | None:
-    """Workspace for the screened columns' plane layout (the screen's uniform-delta split, settings.screen "auto"),
-    or None where no column is screened or no uniform log delta is given."""
-    if hi_from is None or hi_from >= P or ld_uniform is None or not math.isfinite(ld_uniform):
+SO_NB_MAX = 120   # the bucketed screen's most log-delta buckets
+BUCKET_H = 0.2    # its widest bucket (the interpolation's error bound holds up to 1/5)
+
+
+@dataclass
+class ScreenLayout:
+    """The bucketed screen's layout of the screened columns: the workspace flmm_delta_refine fills (order), nb
+    buckets of width t_h from t_lo, and the Q planes' rows it needs."""
+    ws: torch.Tensor
+    t_lo: float
+    t_h: float
+    nb: int
+    q_rows: int
+
+
+def screen_layout(hi_from: int | None, P: int, p_pad: int, t_range: tuple[float, float] | None,
+                  dev) -> ScreenLayout | None:
+    """The bucketed layout for the screened columns hi_from <= p < P whose log delta lies in t_range (settings.screen
+    "auto"), or None where no column is screened, no range is given or it needs more than SO_NB_MAX buckets."""
+    if hi_from is None or hi_from >= P or t_range is None or settings.screen != "auto":
         return None
-    if settings.screen != "auto":
+    lo, hi = (float(v) for v in t_range)
+    if not (math.isfinite(lo) and math.isfinite(hi)) or hi < lo:
         return None
-    nb = _lib.load().flmm_screen_order_bytes(P - hi_from)
-    return torch.empty(nb, dtype=torch.uint8, device=dev)
+    nb = max(1, math.ceil((hi - lo) / BUCKET_H))
+    while nb <= SO_NB_MAX and (hi - lo) / nb > BUCKET_H:
+        nb += 1
+    if nb > SO_NB_MAX:
+        return None
+    t_h = (hi - lo) / nb if hi > lo else BUCKET_H
+    lib = _lib.load()
+    ws = torch.empty(lib.flmm_screen_order_bytes(P - hi_from), dtype=torch.uint8, device=dev)
+    return ScreenLayout(ws, lo, t_h, nb, int(lib.flmm_screen_q_rows(p_pad, hi_from, P)))
+
+
+def layout_args(layout: ScreenLayout | None, p_pad: int) -> tuple:
+    """flmm_delta_refine's (q_rows, t_lo, t_h, nb, order) for a layout (or none)."""
+    if layout is None:
+        return p_pad, 0.0, 0.0, 0, None
+    return layout.q_rows, layout.t_lo, layout.t_h, layout.nb, layout.ws.data_ptr()
 
 
 def delta_refine(prec: int, mode: int, uy: torch.Tensor, P: int, p_pad: int, kpad: int, s: torch.Tensor,
                  s_lam: torch.Tensor, nco: float, n_grid: int, lo: float, hi: float, idx: torch.Tensor | None,
                  log_delta_in: torch.Tensor | None = None, fac: Factored | None = None,
                  lam64: torch.Tensor | None = None, hi_from: int | None = None,
-                 ld_uniform: float | None = None) -> DeltaResult:
+                 t_range: tuple[float, float] | None = None) -> DeltaResult:
     """Refine (or, for DELTA_FIXED, take log_delta_in) and build the column-side scan planes (with fac, the
     factored Q2 instead of the exact one); lam64: delta_grid's FP64 log-det table for the same grid; hi_from: the
-    screen's first column (screen_start), from which only the hi planes are written; ld_uniform (default: hi for
-    the searching modes): the log delta whose columns the screen's uniform-delta split groups."""
+    screen's first column (screen_start), from which only the hi planes are written; t_range (default: (lo, hi)
+    for the searching modes, whose fits stay in the grid's range): the range of the columns' log delta, which the
+    bucketed screen's layout covers."""
     dev = uy.device
     log_delta = torch.empty(max(P, 1), dtype=torch.float64, device=dev)
     if log_delta_in is not None:
         log_delta[:P].copy_(log_delta_in.to(device=dev, dtype=torch.float64))
     rWr = torch.empty(max(P, 1), dtype=torch.float64, device=dev)
     h2 = torch.empty(max(P, 1), dtype=torch.float64, device=dev)
-    q1 = Planes(prec, p_pad, kpad, dev)
-    q2 = Planes(prec, p_pad, kpad, dev) if fac is None else None
+    if t_range is None and mode != _lib.DELTA_FIXED:
+        t_range = (float(lo), float(hi))
+    layout = screen_layout(hi_from, P, p_pad, t_range, dev)
+    la = layout_args(layout, p_pad)
+    q1 = Planes(prec, la[0], kpad, dev)
+    q2 = Planes(prec, la[0], kpad, dev) if fac is None else None
     q1h, q1l = q1.ptrs
     q2h, q2l = q2.ptrs if q2 is not None else (None, None)
-    if ld_uniform is None and mode != _lib.DELTA_FIXED:
-        ld_uniform = float(hi)
-    order = screen_order(hi_from, P, ld_uniform, dev)
     _lib.call("flmm_delta_refine", prec, mode, uy.data_ptr(), P, p_pad, kpad, s.data_ptr(), s.numel(),
               s_lam.data_ptr(), s_lam.numel(), float(nco), n_grid, lo, hi, _lib.ptr(idx), _lib.ptr(lam64),
               log_delta.data_ptr(),
               rWr.data_ptr(), h2.data_ptr(), q1h, q1l, q2h, q2l, q1.scale_ptr, q2.scale_ptr if q2 else None,
-              _hi_from_arg(hi_from, p_pad), float(ld_uniform) if order is not None else 0.0, _lib.ptr(order),
-              _lib.stream())
-    d = DeltaResult(log_delta[:P], rWr[:P], h2[:P], q1, q2, idx, int(s.numel()), hi_from=hi_from,
-                    ld_uniform=ld_uniform if order is not None else None, order=order)
+              _hi_from_arg(hi_from, p_pad), *la, _lib.stream())
+    d = DeltaResult(log_delta[:P], rWr[:P], h2[:P], q1, q2, idx, int(s.numel()), hi_from=hi_from, layout=layout,
+                    p_pad=p_pad)
     if fac is not None:
         attach_factored(prec, d, fac)
     return d
@@ -547,27 +579,27 @@ class ScreenOperands:
 
 
 class _ScreenStats:
-    """Optional counts of the screen's FP64 re-evaluations and of its tests on the sWy-only (uniform-delta) tile
-    (device counters, read by summary())."""
+    """Optional counts of the screen's FP64 re-evaluations (a device counter, read by summary()) and of its tests,
+    those on the sWy-only (bucketed) tile among them."""
 
     def __init__(self):
         self.counter = None
-        self.uniform = None
+        self.swy_only = 0
         self.tests = 0
         self.kpad2 = 0
 
     def reset(self, enabled: bool = True, device=None):
-        z = lambda: torch.zeros(1, dtype=torch.int64, device=device or "cuda") if enabled else None  # noqa: E731
-        self.counter, self.uniform = z(), z()
+        self.counter = torch.zeros(1, dtype=torch.int64, device=device or "cuda") if enabled else None
+        self.swy_only = 0
         self.tests = 0
         self.kpad2 = 0
 
     def summary(self) -> dict:
         n = int(self.counter.item()) if self.counter is not None else 0
-        u = int(self.uniform.item()) if self.uniform is not None else 0
+        u = self.swy_only
         return {"screened_tests": self.tests, "fp64_evaluations": n,
-                "fraction": n / self.tests if self.tests else 0.0, "uniform_tests": u,
-                "uniform_fraction": u / self.tests if self.tests else 0.0, "kpad2": self.kpad2}
+                "fraction": n / self.tests if self.tests else 0.0, "swy_only_tests": u,
+                "swy_only_fraction": u / self.tests if self.tests else 0.0, "kpad2": self.kpad2}
 
 
 screen_stats = _ScreenStats()
@@ -641,22 +673,22 @@ def _screened(s1: Planes, s2: Planes, d: DeltaResult, mc: int, p_first: int, P: 
     kpad2 = d.fac.rpad if d.fac is not None else s1.kpad
     us = screen.us
     cnt = screen_stats.counter
-    # the uniform-delta split: the layout the refinement wrote for exactly these columns
-    order = d.order if (d.order is not None and d.hi_from == p_first) else None
-    lu = float(d.ld_uniform) if order is not None else float("nan")
+    # the bucketed screen: the layout the refinement wrote for exactly these columns
+    order = d.layout.ws if (d.layout is not None and d.hi_from == p_first) else None
     work = None
     if order is not None:
-        work = torch.empty(_lib.load().flmm_screen_work_bytes(s1.rows_pad), dtype=torch.uint8, device=rec.device)
+        work = torch.empty(_lib.load().flmm_screen_work_bytes(s1.rows_pad, s1.kpad), dtype=torch.uint8,
+                           device=rec.device)
     args = (s1.hi.data_ptr(), s2.hi.data_ptr(), s1.scale_ptr, s2.scale_ptr, s1.rows_pad, d.q1.hi.data_ptr(),
-            d.q2.hi.data_ptr(), d.q1.scale_ptr, d.q2.scale_ptr, d.q1.rows_pad, s1.kpad, kpad2, mc, p_first, P,
-            us.data_ptr(), us.stride(0), screen.uy.data_ptr(), screen.s.data_ptr(), screen.neff, float(screen.s_lo),
-            float(screen.s_hi), d.log_delta.data_ptr(), d.rWr.data_ptr(), float(nu), out.max_F.data_ptr(),
-            rec.data_ptr(), lu, _lib.ptr(order), _lib.ptr(work), _lib.ptr(cnt), _lib.stream())
+            d.q2.hi.data_ptr(), d.q1.scale_ptr, d.q2.scale_ptr, d.p_pad or d.q1.rows_pad, d.q1.rows_pad, s1.kpad,
+            kpad2, mc, p_first, P, us.data_ptr(), us.stride(0), screen.uy.data_ptr(), screen.s.data_ptr(),
+            screen.neff, float(screen.s_lo), float(screen.s_hi), d.log_delta.data_ptr(), d.rWr.data_ptr(), float(nu),
+            out.max_F.data_ptr(), rec.data_ptr(), _lib.ptr(order), _lib.ptr(work), _lib.ptr(cnt), _lib.stream())
     if cnt is not None:
         screen_stats.tests += mc * (P - p_first)
         screen_stats.kpad2 = kpad2
-        if order is not None:  # counts[0]: this layout's columns on the sWy-only tile (no host wait)
-            screen_stats.uniform += order[:4].view(torch.int32).to(torch.int64) * mc
+        if order is not None:  # every test of the bucketed layout runs on the sWy-only tile
+            screen_stats.swy_only += mc * (P - p_first)
     if not scan_timer.enabled:
         _lib.call("flmm_wald_scan_screened", *args)
         return

@@ -184,19 +184,20 @@ def refine_learned(basis, net: DeltaMLP, n_grid: int, lo: float, hi: float, *, n
     fb = torch.zeros(max(P, 1), dtype=torch.int32, device=dev)
     rWr = torch.empty(max(P, 1), dtype=torch.float64, device=dev)
     h2 = torch.empty(max(P, 1), dtype=torch.float64, device=dev)
-    q1 = engine.Planes(basis.prec, p_pad, kpad, dev)
-    q2 = engine.Planes(basis.prec, p_pad, kpad, dev)
+    layout = engine.screen_layout(hi_from, P, p_pad, (float(lo), float(hi)), dev)  # the fits stay in [lo, hi]
+    la = engine.layout_args(layout, p_pad)
+    q1 = engine.Planes(basis.prec, la[0], kpad, dev)
+    q2 = engine.Planes(basis.prec, la[0], kpad, dev)
     q1h, q1l = q1.ptrs
     q2h, q2l = q2.ptrs
-    order = engine.screen_order(hi_from, P, float(hi), dev)
     _lib.call("flmm_delta_refine_learned", basis.prec, basis.uy.data_ptr(), P, p_pad, kpad, basis.s.data_ptr(),
               basis.s.numel(), s_lam.data_ptr(), s_lam.numel(), float(nco), n_grid, float(lo), float(hi),
               idx.data_ptr(), lam64.data_ptr(), float(radius), float(judge_tol), log_delta.data_ptr(), fb.data_ptr(),
               rWr.data_ptr(),
               h2.data_ptr(), q1h, q1l, q2h, q2l, q1.scale_ptr, q2.scale_ptr, p_pad if hi_from is None else hi_from,
-              float(hi), _lib.ptr(order), _lib.stream())
+              *la, _lib.stream())
     d = engine.DeltaResult(log_delta[:P], rWr[:P], h2[:P], q1, q2, idx, int(basis.s.numel()), hi_from=hi_from,
-                           ld_uniform=float(hi) if order is not None else None, order=order)
+                           layout=layout, p_pad=p_pad)
     basis.delta = d
     return LearnedFit(log_delta[:P], fb[:P], d)
 

@@ -179,17 +179,24 @@ int flmm_delta_grid_split(const float* wplane, const float* lam, int n_grid, dou
  * (those flmm_wald_scan_screened reads: a multiple of 256 at or past the real columns) get the hi planes only,
  * evaluated in FP32; hi_from >= P writes both planes of every column.  The loss is nco * (log 2pi + log(R / nco) + 1)
  * + sum_j log(s_lam_j + delta).
+ * The Q planes hold q_rows rows per K-chunk (q_rows >= p_pad, a multiple of 128; uy keeps p_pad).
  * order (may be NULL; flmm_screen_order_bytes(P - hi_from) bytes of device workspace) lays the columns
- * hi_from <= p < P out for the screen's uniform-delta split: those whose fitted log_delta equals ld_uniform exactly
- * take plane rows hi_from, hi_from + 1, ... in column order, the others the rows after them; order records the
- * layout (counts, slot -> column, column -> slot) for flmm_factored_weight_planes and flmm_wald_scan_screened. */
+ * hi_from <= p < P out for the bucketed screen: bucket b of nb holds the columns with
+ * t_lo + b t_h <= log_delta < t_lo + (b + 1) t_h (those outside [t_lo, t_lo + nb t_h] join the first or last
+ * bucket and are screened as forced), each bucket's plane rows adjacent from hi_from on and padded to 32 rows; only
+ * their Q1 hi plane is written (in the layout the screen tile loads), and order records the layout for
+ * flmm_factored_weight_planes and flmm_wald_scan_screened.  With order: hi_from a multiple of 256,
+ * q_rows >= flmm_screen_q_rows(p_pad, hi_from, P), 1 <= nb <= 120 and 0 < t_h <= 1/5. */
 int flmm_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_pad, int64_t kpad, const double* s,
                       int64_t neff, const double* s_lam, int64_t nlam, double nco, int n_grid, double lo, double hi,
                       const int32_t* idx, const double* lam64, double* log_delta, double* rWr, double* h2,
                       void* q1_hi, void* q1_lo, void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale,
-                      int64_t hi_from, double ld_uniform, void* order, void* stream);
+                      int64_t hi_from, int64_t q_rows, double t_lo, double t_h, int nb, void* order, void* stream);
 /* Bytes of the layout workspace for n_cols screened columns (-1 on a bad size).  No counterpart in the reference. */
 int64_t flmm_screen_order_bytes(int64_t n_cols);
+/* Q-plane rows a bucketed layout of the columns hi_from <= p < P needs (p_pad when hi_from >= P; -1 on bad sizes).
+ * No counterpart in the reference. */
+int64_t flmm_screen_q_rows(int64_t p_pad, int64_t hi_from, int64_t P);
 
 /* Factored weights (no counterpart in the reference, whose sWs = w^T (US o US) runs over all Neff eigenvalues,
  * core.py:640-651).  With a positive exponential sum 1/z = sum_{r<R} lam_r^2 exp(-t_r z) (relative accuracy
@@ -200,7 +207,8 @@ int64_t flmm_screen_order_bytes(int64_t n_cols);
  * S2 = planes of C, e.g. from FLMM_ROTATE_US_X2 and flmm_pack_planes).  This call writes B (delta_p =
  * exp(log_delta[p]), p < P) as the Q2 planes (rows p_pad, K = rpad, zero for r >= R and p >= P), split-FP16 rows
  * scaled by the power of two of their L2 norm (q2_scale); columns hi_from <= p < P get the hi plane only (as
- * flmm_delta_refine), at the plane rows of the layout order (the one flmm_delta_refine wrote; NULL: row p). */
+ * flmm_delta_refine), or, with a bucketed layout order (the one flmm_delta_refine wrote; may be NULL), their scale
+ * alone (the bucketed screen reads no Q2 plane).  p_pad: the Q planes' rows (flmm_delta_refine's q_rows). */
 int flmm_factored_weight_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
                                 int64_t R, const double* log_delta, void* q2_hi, void* q2_lo, float* q2_scale,
                                 int64_t hi_from, const void* order, void* stream);
@@ -269,7 +277,8 @@ int flmm_delta_refine_learned(int prec, const float* uy, int64_t P, int64_t p_pa
                               double hi, const int32_t* idx, const double* lam64, double radius, double judge_tol,
                               double* log_delta, int32_t* fallback, double* rWr, double* h2, void* q1_hi,
                               void* q1_lo, void* q2_hi, void* q2_lo, float* q1_scale, float* q2_scale,
-                              int64_t hi_from, double ld_uniform, void* order, void* stream);
+                              int64_t hi_from, int64_t q_rows, double t_lo, double t_h, int nb, void* order,
+                              void* stream);
 
 /* Spectral features of P rotated columns for the learned delta proposal (34 fp32 per column, row-major):
  * feature b < 32 = log(mean y^2 in eigenvalue bin b / mean y^2), bin_of_k[k] in [0, 32) for k < neff;
@@ -337,40 +346,41 @@ int flmm_wald_scan_lowrank_noalloc(int prec, const void* s1_hi, const void* s1_l
  *   uy     the rotated columns' FP32 plane (p_pad x kpad, plane layout), s the basis spectrum (neff values, all
  *          within [s_lo, s_hi]), log_delta and rWr per column,
  * with the exact weights 1 / (s_k + delta); max_F[p] = max(max_F[p], those values).  The result is the maximum of
- * FP64 evaluations of every test that can hold it.  S1 / Q1 hold kpad, S2 / Q2 kpad2 (== kpad: exact weights).
+ * FP64 evaluations of every test that can hold it.  S1 / Q1 hold kpad, S2 / Q2 kpad2 (== kpad: exact weights);
+ * the Q planes have q_rows rows per K-chunk, uy p_pad.
  * rec: caller workspace of ceil(mc / 128) x (P - p_first) uint64; n_eval (may be NULL) accumulates the number of
- * FP64 evaluations.  Needs mc <= 8192 and mc_pad, p_pad <= 2^24.
- * order (may be NULL): the layout flmm_delta_refine wrote for these columns (its hi_from == p_first, the same
- * ld_uniform): the columns at log_delta == ld_uniform share one sWs per SNP row, computed in FP64 from us, and run
- * the sWy product alone (their planes are adjacent); the rest run both products.  Same bounds, same result
- * semantics.  work: device workspace of flmm_screen_work_bytes(mc_pad) bytes, with order. */
+ * FP64 evaluations.  Needs mc <= 8192 and mc_pad, q_rows <= 2^24.
+ * order (may be NULL): the bucketed layout flmm_delta_refine wrote for these columns (its hi_from == p_first):
+ * every column then runs the sWy product alone, its 1 / sWs interpolated (quadratic in log delta, sWs within 2.5e-4
+ * relative, inside the bound's kD) from FP64 sums of us^2 with the weights at three nodes of its bucket; S2 and Q2
+ * are not read.  Same bounds, same result semantics.  work: device workspace of flmm_screen_work_bytes(mc_pad,
+ * kpad) bytes, with order. */
 int flmm_wald_scan_screened(const void* s1_hi, const void* s2_hi, const float* s1_scale, const float* s2_scale,
                             int64_t mc_pad, const void* q1_hi, const void* q2_hi, const float* q1_scale,
-                            const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t kpad2, int64_t mc,
-                            int64_t p_first, int64_t P, const double* us, int64_t ld_us, const float* uy,
+                            const float* q2_scale, int64_t p_pad, int64_t q_rows, int64_t kpad, int64_t kpad2,
+                            int64_t mc, int64_t p_first, int64_t P, const double* us, int64_t ld_us, const float* uy,
                             const double* s, int64_t neff, double s_lo, double s_hi, const double* log_delta,
-                            const double* rWr, double nu, double* max_F, void* rec, double ld_uniform,
-                            const void* order, void* work, unsigned long long* n_eval, void* stream);
+                            const double* rWr, double nu, double* max_F, void* rec, const void* order, void* work,
+                            unsigned long long* n_eval, void* stream);
 /* The two halves of flmm_wald_scan_screened, same arguments: flmm_wald_screen runs the screen alone (writes rec),
  * flmm_screen_resolve the FP64 re-evaluation from a rec the screen wrote (reads rec, updates max_F); in that order
  * on one stream they equal one flmm_wald_scan_screened call (a caller times them apart). */
-int flmm_wald_screen(const void* s1_hi, const void* s2_hi, const float* s1_scale, const float* s2_scale,
-                     int64_t mc_pad, const void* q1_hi, const void* q2_hi, const float* q1_scale,
-                     const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t kpad2, int64_t mc, int64_t p_first,
-                     int64_t P, const double* us, int64_t ld_us, const float* uy, const double* s, int64_t neff,
-                     double s_lo, double s_hi, const double* log_delta, const double* rWr, double nu, double* max_F,
-                     void* rec, double ld_uniform, const void* order, void* work, unsigned long long* n_eval,
-                     void* stream);
+int flmm_wald_screen(const void* s1_hi, const void* s2_hi, const float* s1_scale, const float* s2_scale, int64_t mc_pad,
+                     const void* q1_hi, const void* q2_hi, const float* q1_scale, const float* q2_scale, int64_t p_pad,
+                     int64_t q_rows, int64_t kpad, int64_t kpad2, int64_t mc, int64_t p_first, int64_t P,
+                     const double* us, int64_t ld_us, const float* uy, const double* s, int64_t neff, double s_lo,
+                     double s_hi, const double* log_delta, const double* rWr, double nu, double* max_F, void* rec,
+                     const void* order, void* work, unsigned long long* n_eval, void* stream);
 int flmm_screen_resolve(const void* s1_hi, const void* s2_hi, const float* s1_scale, const float* s2_scale,
                         int64_t mc_pad, const void* q1_hi, const void* q2_hi, const float* q1_scale,
-                        const float* q2_scale, int64_t p_pad, int64_t kpad, int64_t kpad2, int64_t mc,
+                        const float* q2_scale, int64_t p_pad, int64_t q_rows, int64_t kpad, int64_t kpad2, int64_t mc,
                         int64_t p_first, int64_t P, const double* us, int64_t ld_us, const float* uy, const double* s,
                         int64_t neff, double s_lo, double s_hi, const double* log_delta, const double* rWr, double nu,
-                        double* max_F, void* rec, double ld_uniform, const void* order, void* work,
-                        unsigned long long* n_eval, void* stream);
-/* Bytes of the uniform-delta split's workspace for mc_pad SNP rows (-1 on a bad size).  No counterpart in the
+                        double* max_F, void* rec, const void* order, void* work, unsigned long long* n_eval,
+                        void* stream);
+/* Bytes of the bucketed screen's workspace for mc_pad SNP rows and kpad (-1 on a bad size).  No counterpart in the
  * reference (the workspace of flmm_wald_scan_screened). */
-int64_t flmm_screen_work_bytes(int64_t mc_pad);
+int64_t flmm_screen_work_bytes(int64_t mc_pad, int64_t kpad);
 
 /* Fused Wald scan of one SNP block (snp_wald_scan_compat's tile loop, core.py:636-667): sWy = S1.Q1, sWs = S2.Q2 on
  * MFMA, FP64 epilogue with dof nu; split-FP16 sums are divided by the planes' row scales.  Real columns

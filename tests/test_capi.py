@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_error_channel():
     from fasterlmm_amd import _lib
     lib = _lib.load()
-    assert lib.flmm_abi_version() == 14
+    assert lib.flmm_abi_version() == 15
     # argument validation runs on the host and reports through flmm_last_error without touching a GPU
     rc = lib.flmm_max_reduce(None, 1, 0, 4, None, None)
     assert rc == 1
@@ -67,6 +67,42 @@ def test_bad_arguments_raise_value_error():
         _lib.call("flmm_factored_weight_planes", 2, 10, 256, 128, 16, 16, 129, 16, 16, 16, 16, 256, None, None)
     with pytest.raises(ValueError, match="q2_scale"):
         _lib.call("flmm_factored_weight_planes", 2, 10, 256, 128, 16, 16, 100, 16, 16, 16, None, 256, None, None)
+
+
+def test_screen_layout_buckets():
+    """The bucketed screen's layout on the host: buckets of width <= 1/5 over the log-delta range (at most 120, else
+    no layout), Q-plane rows covering every padded slot, and the checks flmm_delta_refine makes on them."""
+    import pytest
+    from fasterlmm_amd import _lib, engine
+    lib = _lib.load()
+    P, p_pad, hf = 1000, 1024, 256
+    lay = engine.screen_layout(hf, P, p_pad, (-10.0, 10.0), "cpu")
+    assert (lay.nb, lay.t_lo, lay.t_h) == (100, -10.0, 0.2)
+    assert lay.q_rows == lib.flmm_screen_q_rows(p_pad, hf, P) == (hf + (P - hf) + 32 * 120 + 255) // 256 * 256
+    assert lay.ws.numel() == lib.flmm_screen_order_bytes(P - hf)
+    lay = engine.screen_layout(hf, P, p_pad, (0.1, 0.7), "cpu")
+    assert lay.nb == 3 and abs(lay.t_h - 0.2) < 1e-15 and lay.t_h <= 0.2
+    assert engine.screen_layout(hf, P, p_pad, (3.0, 3.0), "cpu").nb == 1
+    assert engine.screen_layout(hf, P, p_pad, (-12.0, 12.0), "cpu").nb == 120
+    assert engine.screen_layout(hf, P, p_pad, (-12.0, 12.01), "cpu") is None  # 121 buckets
+    assert engine.screen_layout(None, P, p_pad, (-10.0, 10.0), "cpu") is None
+    assert engine.screen_layout(P, P, p_pad, (-10.0, 10.0), "cpu") is None
+    with engine.configure(screen="general"):
+        assert engine.screen_layout(hf, P, p_pad, (-10.0, 10.0), "cpu") is None
+    assert lib.flmm_screen_q_rows(p_pad, P, P) == p_pad and lib.flmm_screen_q_rows(p_pad, 0, p_pad + 1) == -1
+    assert lib.flmm_screen_work_bytes(256, 1024) == 8 * 256 * (1024 + 256)
+    # flmm_delta_refine's layout checks (host only: they fail before any launch)
+    args = [2, 0, 16, P, p_pad, 128, 16, 100, 16, 100, 900.0, 64, -10.0, 10.0, 16, 16, 16, 16, 16, 16, 16, 16, 16,
+            16, 16, hf]
+    with pytest.raises(ValueError, match="q_rows below"):
+        _lib.call("flmm_delta_refine", *args, p_pad, -10.0, 0.2, 100, 16, None)
+    q = lib.flmm_screen_q_rows(p_pad, hf, P)
+    with pytest.raises(ValueError, match="t_h <= 1/5"):
+        _lib.call("flmm_delta_refine", *args, q, -10.0, 0.25, 80, 16, None)
+    with pytest.raises(ValueError, match="t_h <= 1/5"):
+        _lib.call("flmm_delta_refine", *args, q, -10.0, 0.2, 121, 16, None)
+    with pytest.raises(ValueError, match="multiple of 256"):
+        _lib.call("flmm_delta_refine", *(args[:-1] + [128]), q, -10.0, 0.2, 100, 16, None)
 
 
 def _fake_group(_lib, mc=300, P=200, p_pad=256, n_real=3, base=0x10000):

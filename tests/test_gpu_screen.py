@@ -81,11 +81,14 @@ def test_screen_max_f_at_fixed_delta(panel, layout, weights):
     assert rel[:256].max() <= 1e-5
 
 
-@pytest.mark.parametrize("layout", ["block", "interleaved"])
-def test_uniform_delta_split_at_fixed_delta(panel, layout):
-    """Half the screened columns at exactly the largest log delta (10, the grid's upper end): those run the
-    sWy-only tile with one FP64 sWs per SNP row, the rest the general tile, each over its column list.  max_F equals
-    the unsplit screen's bit for bit (both are the FP64 value of the same maximising test) and the oracle's."""
+@pytest.mark.parametrize("layout", ["spread", "edges", "clustered"])
+def test_bucketed_screen_at_fixed_delta(panel, layout):
+    """The bucketed screen (every screened column on the sWy-only tile, sWs interpolated within its log-delta
+    bucket from FP64 node sums) against the general tile (both products on hi planes) and the oracle, with the
+    screened columns' log delta spread over [-10, 10] (100 buckets of 1/5), placed on the buckets' nodes and edges
+    (u = 0, 1/2, 1 and just inside), or half at exactly 10 (the grid's upper end, where null columns settle).
+    max_F equals the general tile's bit for bit (both are the FP64 value of the same maximising test) and the
+    oracle's."""
     from fasterlmm_amd import core, engine
     pan, Zs, Y = panel
     on = pan.chrom == 1
@@ -95,20 +98,30 @@ def test_uniform_delta_split_at_fixed_delta(panel, layout):
     want = lmm.compat_rotate(K, pan.X, Y)
     ld = lmm.fit_delta_grid_compat(want, n_grid=64).copy()
     cols = np.arange(256, Y.shape[1])
-    sel = cols[::2] if layout == "interleaved" else cols[: cols.size // 2]
-    ld[sel] = 10.0
+    rng = np.random.default_rng(17)
+    if layout == "spread":
+        ld[cols] = rng.uniform(-10.0, 10.0, cols.size)
+    elif layout == "edges":
+        b = rng.integers(0, 100, cols.size)
+        u = rng.choice([0.0, 0.5, 1.0, 1e-12, 1.0 - 1e-12], cols.size)
+        ld[cols] = -10.0 + (b + u) * 0.2
+    else:
+        ld[cols[: cols.size // 2]] = 10.0
+    ld[256], ld[257] = -10.0, 10.0  # the range [-10, 10] whatever the draw
     ref = lmm.snp_wald_scan_compat(want, ld, S, n_real=4)
-    got, ev = {}, {}
+    got, ev, st = {}, {}, {}
     for mode in ("auto", "general"):
         engine.screen_stats.reset(True)
         with engine.configure(screen=mode):
             spec = core.fastlmm_compat_rotate(T(K), T(pan.X), T(Y))
             got[mode] = core.snp_wald_scan_compat(spec, T(ld), T(S), n_real=4).max_F.cpu().numpy()
-        ev[mode] = engine.screen_stats.summary()["fp64_evaluations"]
+        st[mode] = engine.screen_stats.summary()
         engine.screen_stats.reset(False)
     rel = np.abs(got["auto"] - ref["max_F"]) / ref["max_F"]
-    say(f"uniform-delta split ({layout}): max_F rel err {rel[256:].max():.2e}; FP64 evaluations {ev['auto']} "
-        f"(split) vs {ev['general']} (general tile only)")
+    say(f"bucketed screen ({layout}): max_F rel err {rel[256:].max():.2e}; FP64 evaluations "
+        f"{st['auto']['fp64_evaluations']} (bucketed) vs {st['general']['fp64_evaluations']} (general tile)")
+    assert st["auto"]["swy_only_tests"] == st["auto"]["screened_tests"] == S.shape[1] * cols.size
+    assert st["general"]["swy_only_tests"] == 0
     np.testing.assert_array_equal(got["auto"], got["general"])
     assert rel[256:].max() <= 1e-5, (rel[256:].max(), 256 + int(np.argmax(rel[256:])))
 
@@ -122,7 +135,7 @@ def test_screen_loco_matches_full_scan_and_oracle(panel):
         with engine.configure(screen=mode):
             res[mode] = core.loco_scan_compat(*args, n_real=4)
     torch.cuda.synchronize()
-    # the uniform-delta split changes which tests are re-evaluated, not the maximum
+    # the bucketed screen changes which tests are re-evaluated, not the maximum
     np.testing.assert_array_equal(res["auto"].max_F.cpu().numpy(), res["general"].max_F.cpu().numpy())
     # the real columns run the full scan either way: same bits
     for k in ("f", "beta", "se", "sfve", "nullh2"):
