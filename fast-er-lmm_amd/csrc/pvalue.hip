@@ -8,6 +8,8 @@
 // with CF the continued fraction of I_x (modified Lentz evaluation, as published in Press et al., Numerical
 // Recipes, sect. 6.4), used when x < (a + 1) / (a + b + 2); otherwise p = 1 - I_{1-x}(b, a) through the same
 // continued fraction.
+#include <cmath>
+
 #include "kernels.h"
 
 namespace flmm {
@@ -42,31 +44,31 @@ __device__ inline double betacf(double a, double b, double x) {
   return h;
 }
 
-// natural log of I_x(a, b)
-__device__ inline double log_ibeta(double a, double b, double x) {
+// natural log of I_x(a, b); lnorm_a = log a + log B(a, b), lnorm_b = log b + log B(a, b) (constants of a call)
+__device__ inline double log_ibeta(double a, double b, double x, double lnorm_a, double lnorm_b) {
   if (x <= 0.0) return -INFINITY;
   if (x >= 1.0) return 0.0;
-  const double lbeta = lgamma(a) + lgamma(b) - lgamma(a + b);
   if (x < (a + 1.0) / (a + b + 2.0)) {
-    return a * log(x) + b * log1p(-x) - log(a) - lbeta + log(betacf(a, b, x));
+    return a * log(x) + b * log1p(-x) - lnorm_a + log(betacf(a, b, x));
   }
   const double y = 1.0 - x;
-  const double comp = exp(b * log(y) + a * log1p(-y) - log(b) - lbeta) * betacf(b, a, y);
+  const double comp = exp(b * log(y) + a * log1p(-y) - lnorm_b) * betacf(b, a, y);
   return log1p(-comp);
 }
 
-__device__ inline double neglog10_f1_sf(double F, double df2) {
+__device__ inline double neglog10_f1_sf(double F, double df2, double lnorm_a, double lnorm_b) {
   if (isnan(F)) return NAN;
   if (F <= 0.0) return 0.0;
   if (isinf(F)) return INFINITY;
   const double x = df2 / (df2 + F);
-  return -log_ibeta(0.5 * df2, 0.5, x) / 2.302585092994045684;
+  return -log_ibeta(0.5 * df2, 0.5, x, lnorm_a, lnorm_b) / 2.302585092994045684;
 }
 
 // out[i] = -log10 F(1, df2).sf(F[i])
-__global__ void neglog10p_kernel(const double* __restrict__ F, int64_t n, double df2, double* __restrict__ out) {
+__global__ void neglog10p_kernel(const double* __restrict__ F, int64_t n, double df2, double lnorm_a,
+                                 double lnorm_b, double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = neglog10_f1_sf(F[i], df2);
+  if (i < n) out[i] = neglog10_f1_sf(F[i], df2, lnorm_a, lnorm_b);
 }
 
 // (M, n_real) row-major beta -> effect[m][j] = beta^2 * g_var[m] / p_var[j]
@@ -81,7 +83,11 @@ __global__ void effect_size_kernel(const double* __restrict__ beta, int64_t M, i
 }
 
 hipError_t launch_neglog10p(const double* F, int64_t n, double df2, double* out, hipStream_t stream) {
-  hipLaunchKernelGGL(neglog10p_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, F, n, df2, out);
+  // the call's constants on the host: log B(a, b) = lgamma(a) + lgamma(b) - lgamma(a + b), a = df2 / 2, b = 1/2
+  const double a = 0.5 * df2, b = 0.5;
+  const double lbeta = std::lgamma(a) + std::lgamma(b) - std::lgamma(a + b);
+  hipLaunchKernelGGL(neglog10p_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, F, n, df2,
+                     std::log(a) + lbeta, std::log(b) + lbeta, out);
   return hipGetLastError();
 }
 

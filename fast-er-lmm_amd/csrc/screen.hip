@@ -317,6 +317,11 @@ __global__ __launch_bounds__(256, 1) void wald_screen_kernel(ScreenArgs g, int64
 // the normal range marks the record forced).  Rows whose node sums are not positive, or whose scaled reciprocals are
 // not finite FP32 numbers, get G = 0 (x = 0) and make their tile forced; padding rows get G = 0.
 constexpr int SU_SLOTS = 2;                        // K-chunks per stage
+constexpr int64_t SB_ROWS_PER_GROUP = 16 * W_TM;   // SNP rows per scheduling group of the bucketed tiles
+// 128-row halves per bucketed workgroup: 1 (two workgroups per CU, whose stages drift apart so one's epilogue and
+// load waits overlap the other's MFMAs) measured 6.6 ms per chromosome at N = 1,000 x 648k screened columns against
+// 7.4 ms for 2 (one 512-thread workgroup per CU, half the SNP-side bytes per flop, waves in lockstep at each stage)
+constexpr int SB_WM = 1;
 
 // the median of three unsigned values (one VALU instruction)
 __device__ inline uint32_t umed3(uint32_t a, uint32_t b, uint32_t c) {
@@ -326,42 +331,52 @@ __device__ inline uint32_t umed3(uint32_t a, uint32_t b, uint32_t c) {
 }
 constexpr int SU_STAGE = SU_SLOTS * SC_SLOT_LDS;   // 16 KiB
 
-__global__ __launch_bounds__(256, 2) void screen_bucketed_kernel(BucketArgs g, int64_t n_mtiles,
-                                                                 uint64_t* __restrict__ rec, int64_t rec_ld) {
-  __shared__ __attribute__((aligned(16))) char lds0[SU_STAGE];
-  __shared__ __attribute__((aligned(16))) char lds1[SU_STAGE];
-  __shared__ __attribute__((aligned(16))) float fac[4][2][3][W_TM];  // [wave][column block][coefficient][row]
-  __shared__ int blk_forced[4][2];
+// WM row halves: a workgroup of 256 WM threads owns 128 WM SNP rows x 256 slots; wave w computes rows
+// 128 (w / 4) .. + 127 of the columns 64 (w % 4) .. + 63, as one 128-row tile of its own (records per 128 rows)
+template <int WM>
+__global__ __launch_bounds__(256 * WM, 2 / WM) void screen_bucketed_kernel(BucketArgs g, int64_t n_mtiles,
+                                                                           uint64_t* __restrict__ rec,
+                                                                           int64_t rec_ld) {
+  constexpr int NT = 256 * WM;
+  constexpr int64_t GM = SB_ROWS_PER_GROUP / (W_TM * WM);  // workgroup row tiles per scheduling group
+  __shared__ __attribute__((aligned(16))) char lds0[WM * SU_STAGE];
+  __shared__ __attribute__((aligned(16))) char lds1[WM * SU_STAGE];
+  __shared__ __attribute__((aligned(16))) float fac[4 * WM][2][3][W_TM];  // [wave][column block][coefficient][row]
+  __shared__ int blk_forced[4 * WM][2];
 
   const int64_t n_cols = gmem(g.head)[0];  // plane slots, padding included
   const int64_t n_ptiles = (n_cols + W_TP - 1) / W_TP;
-  const int64_t n_live = n_mtiles * n_ptiles;
+  const int64_t n_wtiles = (n_mtiles + WM - 1) / WM;
+  const int64_t n_live = n_wtiles * n_ptiles;
   if ((int64_t)blockIdx.x >= n_live) return;
   const int64_t tile = xcd_remap(blockIdx.x, n_live);
-  const int64_t group = tile / (SC_GM * n_ptiles);
-  const int64_t first_m = group * SC_GM;
-  const int64_t gm = n_mtiles - first_m < SC_GM ? n_mtiles - first_m : SC_GM;
-  const int64_t r_in = tile - group * (SC_GM * n_ptiles);
-  const int64_t mt = first_m + r_in % gm;
+  const int64_t group = tile / (GM * n_ptiles);
+  const int64_t first_m = group * GM;
+  const int64_t gm = n_wtiles - first_m < GM ? n_wtiles - first_m : GM;
+  const int64_t r_in = tile - group * (GM * n_ptiles);
+  const int64_t mtw = first_m + r_in % gm;
   const int64_t pt = r_in / gm;
-  const int64_t m0 = mt * W_TM, p0 = pt * W_TP;
+  const int64_t p0 = pt * W_TP;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int cwave = wave & 3;                // the wave's 64 columns
+  const int64_t mt = mtw * WM + (wave >> 2);  // the wave's 128-row tile
+  const int64_t m0 = mtw * WM * W_TM;         // the workgroup's first row
   const int h = lane >> 5;
 
   // the buckets of the tile's eight 32-slot column blocks (every block lies in one bucket; -1 past the slots)
   auto block_bucket = [&](int w, int bj) -> int {
-    const int64_t s0 = p0 + w * 64 + bj * 32;
+    const int64_t s0 = p0 + (w & 3) * 64 + bj * 32;
     return s0 < n_cols ? gmem(g.gbucket)[s0 / SO_PAD] : -1;
   };
-  if (threadIdx.x < 8) blk_forced[threadIdx.x >> 1][threadIdx.x & 1] = 0;
+  if (threadIdx.x < 8 * WM) blk_forced[threadIdx.x >> 1][threadIdx.x & 1] = 0;
   __syncthreads();
-  // the interpolant of 1 / (s1_m^2 sWs) of the tile's 128 rows for each column block's bucket (G0, C1, C2 from the
-  // reciprocal node sums in FP64, rounded to FP32; 0 on padding and bad rows)
-  for (int q = threadIdx.x; q < 8 * W_TM; q += 256) {
+  // the interpolant of 1 / (s1_m^2 sWs) of each wave's 128 rows for each of its column blocks' buckets (G0, C1, C2
+  // from the reciprocal node sums in FP64, rounded to FP32; 0 on padding and bad rows)
+  for (int q = threadIdx.x; q < 8 * WM * W_TM; q += NT) {
     const int wb = q / W_TM, r = q % W_TM, w = wb >> 1, bj = wb & 1;
     const int bk = block_bucket(w, bj);
-    const int64_t m = m0 + r;
+    const int64_t m = m0 + (w >> 2) * W_TM + r;
     float f[3] = {0.f, 0.f, 0.f};
     if (bk >= 0 && m < g.mc) {
       const double sc = (double)gmem(g.s1s)[m];
@@ -391,15 +406,18 @@ __global__ __launch_bounds__(256, 2) void screen_bucketed_kernel(BucketArgs g, i
   const int64_t b_kstride = g.b_rows_pad * 64;
   const char* const a_base = (const char*)g.s1 + m0 * 64;
   const char* const b_base = (const char*)g.q1;
-  // SNP side: thread t copies 16-byte segment t % 4 of rows t / 4 and 64 + t / 4 of every chunk
+  // SNP side: thread t copies 16-byte segment t % 4 of rows t / 4 and 64 WM + t / 4 of every chunk; rows past the
+  // planes (the second half of a workgroup past the last 128-row tile) read the first half's (their records are not
+  // written)
   const uint32_t a_lane = threadIdx.x * 16;
+  const uint32_t a_lane2 = m0 + 64 * WM + (threadIdx.x >> 2) < g.a_rows_pad ? a_lane + 4096 * WM : a_lane;
   const int a_row0 = threadIdx.x >> 2;
   // column side (fragment-native layout): lane (c, h) of column block bj reads its 16 bytes of the step's 1 KiB
   // fragment; slots past n_cols read the last one (their records are not written)
   uint32_t b_off[2];
 #pragma unroll
   for (int bj = 0; bj < 2; ++bj) {
-    const int64_t j = p0 + wave * 64 + bj * 32 + (lane & 31);
+    const int64_t j = p0 + cwave * 64 + bj * 32 + (lane & 31);
     const int64_t sl = j < n_cols ? j : n_cols - 1;
     b_off[bj] = (uint32_t)((sl >> 5) * 2048 + (h * 32 + (sl & 31)) * 16);
   }
@@ -412,7 +430,7 @@ __global__ __launch_bounds__(256, 2) void screen_bucketed_kernel(BucketArgs g, i
     for (int i = 0; i < SU_SLOTS; ++i) {
       const char* sa = a_base + (int64_t)(j * SU_SLOTS + i) * a_kstride;
       a.v[i][0] = *gmem((const u32x4*)(sa + a_lane));
-      a.v[i][1] = *gmem((const u32x4*)(sa + (a_lane + 4096)));
+      a.v[i][1] = *gmem((const u32x4*)(sa + a_lane2));
     }
     return a;
   };
@@ -431,8 +449,8 @@ __global__ __launch_bounds__(256, 2) void screen_bucketed_kernel(BucketArgs g, i
     for (int i = 0; i < SU_SLOTS; ++i)
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
-        const int r = a_row0 + b * 64;
-        *(u32x4*)(st + i * SC_SLOT_LDS + r * 64 + (((threadIdx.x & 3) ^ sc_swz(r)) * 16)) = a.v[i][b];
+        const int r = a_row0 + b * 64 * WM;
+        *(u32x4*)(st + i * WM * SC_SLOT_LDS + r * 64 + (((threadIdx.x & 3) ^ sc_swz(r)) * 16)) = a.v[i][b];
       }
   };
 
@@ -453,8 +471,8 @@ __global__ __launch_bounds__(256, 2) void screen_bucketed_kernel(BucketArgs g, i
         bf16x8 af[4];
 #pragma unroll
         for (int bi = 0; bi < 4; ++bi) {
-          const int r = bi * 32 + (lane & 31);
-          af[bi] = *(const bf16x8*)(st + i * SC_SLOT_LDS + r * 64 + ((seg ^ sc_swz(r)) * 16));
+          const int r = (wave >> 2) * W_TM + bi * 32 + (lane & 31);
+          af[bi] = *(const bf16x8*)(st + i * WM * SC_SLOT_LDS + r * 64 + ((seg ^ sc_swz(r)) * 16));
         }
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj) {
@@ -509,9 +527,9 @@ __global__ __launch_bounds__(256, 2) void screen_bucketed_kernel(BucketArgs g, i
   float uu[2];
 #pragma unroll
   for (int bj = 0; bj < 2; ++bj) {
-    const int64_t j = p0 + wave * 64 + bj * 32 + (lane & 31);
+    const int64_t j = p0 + cwave * 64 + bj * 32 + (lane & 31);
     const int bk = block_bucket(wave, bj);
-    col[bj] = (j < n_cols && bk >= 0) ? gmem(g.order)[j] : -1;
+    col[bj] = (j < n_cols && bk >= 0 && mt < n_mtiles) ? gmem(g.order)[j] : -1;
     double u = 0.0;
     if (col[bj] >= 0) u = (gmem(g.log_delta)[col[bj]] - (t_lo + bk * t_h)) / t_h;
     out_b[bj] = !(u >= -1e-9 && u <= 1.0 + 1e-9);
@@ -781,8 +799,10 @@ hipError_t launch_screen_nodes(const double* us, int64_t ld_us, int64_t mc, cons
 hipError_t launch_screen_bucketed(const BucketArgs& g, int64_t Mc, int64_t P, uint64_t* rec, int64_t rec_ld,
                                   hipStream_t stream) {
   const int64_t n_mtiles = (Mc + W_TM - 1) / W_TM;
-  const int64_t n_tiles = n_mtiles * ((screen_order_slots(P) + W_TP - 1) / W_TP);  // every possible slot tile
-  hipLaunchKernelGGL(screen_bucketed_kernel, dim3((unsigned)n_tiles), dim3(256), 0, stream, g, n_mtiles, rec, rec_ld);
+  // every possible slot tile
+  const int64_t n_tiles = (n_mtiles + SB_WM - 1) / SB_WM * ((screen_order_slots(P) + W_TP - 1) / W_TP);
+  hipLaunchKernelGGL(screen_bucketed_kernel<SB_WM>, dim3((unsigned)n_tiles), dim3(256 * SB_WM), 0, stream, g,
+                     n_mtiles, rec, rec_ld);
   return hipGetLastError();
 }
 
