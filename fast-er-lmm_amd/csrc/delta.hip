@@ -890,6 +890,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
 // per lane) and its writes are contiguous 8-byte stores per plane (element (p, k) at
 // ((k / 32) p_pad + p) 32 + k % 32).
 
+constexpr int64_t PL_SF_MAX = 4096;  // spectra up to this kpad are staged in LDS by the bucketed-layout blocks
+
 template <int PREC>
 __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restrict__ uy, int64_t P, int64_t p_pad,
                                                            int64_t kpad, const double* __restrict__ s, int64_t neff,
@@ -898,14 +900,21 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
                                                            const float* __restrict__ q2s, void* q1a, void* q1b,
                                                            void* q2a, void* q2b, int64_t hi_from, int64_t q_rows,
                                                            const int* __restrict__ ows) {
+  __shared__ __attribute__((aligned(16))) float sf_lds[PL_SF_MAX];
   const int j4 = (threadIdx.x & 7) * 4;
   const int64_t nch = kpad / 32;  // a multiple of 4 (kpad % 128 == 0)
   if constexpr (is_split(PREC)) {
-    if (ows && (int64_t)blockIdx.x * 32 >= hi_from) {
+    if (ows && (int64_t)blockIdx.x * 32 >= hi_from) {  // block-uniform
       // screened columns with the bucketed layout: this block writes plane slots b .. b + 31 (slot t: column
       // hi_from + order[t]; padding slots are skipped: only their own columns of the product read them) in the
       // fragment-native layout, the Q1 hi half only (the bucketed screen takes sWs from FP64 node sums; hi_from %
       // 256 == 0: blocks do not straddle it)
+      // the spectrum in FP32 once per block (+inf past neff, where w = 1 / (inf + delta) = 0)
+      const bool staged = kpad <= PL_SF_MAX;
+      if (staged) {
+        for (int64_t k = threadIdx.x; k < kpad; k += 256) sf_lds[k] = k < neff ? (float)s[k] : INFINITY;
+        __syncthreads();
+      }
       const ScreenOrder o = screen_order((void*)ows, P - hi_from);
       const int64_t slot = (int64_t)blockIdx.x * 32 - hi_from + (threadIdx.x >> 3);
       if (slot >= gmem(o.head)[0]) return;
@@ -914,17 +923,20 @@ __global__ __launch_bounds__(256) void delta_planes_kernel(const float* __restri
       const int64_t p = hi_from + col;
       const float d32 = (float)exp(log_delta[p]);
       const float s1f = PREC == PREC_SPLIT_FP16 ? q1s[p] : 1.f;
-#pragma unroll 2
+#pragma unroll 4
       for (int64_t c = 0; c < nch; ++c) {
         const int64_t k0 = c * 32 + j4;
         const f32x4 y4 = *(const f32x4*)(uy + (c * p_pad + p) * 32 + j4);
+        f32x4 s4;
+        if (staged) {
+          s4 = *(const f32x4*)&sf_lds[k0];
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) s4[e] = k0 + e < neff ? (float)s[k0 + e] : INFINITY;
+        }
         u16 h[4];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          float w = 0.f;
-          if (k0 + e < neff) w = __builtin_amdgcn_rcpf((float)s[k0 + e] + d32);
-          h[e] = hi_of<PREC>(w * y4[e] * s1f);
-        }
+        for (int e = 0; e < 4; ++e) h[e] = hi_of<PREC>(__builtin_amdgcn_rcpf(s4[e] + d32) * y4[e] * s1f);
         const int64_t woff = frag_off(c, q_rows, hi_from, slot, j4) / 2;
         *(uint2*)((u16*)q1a + woff) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
       }

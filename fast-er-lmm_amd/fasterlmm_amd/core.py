@@ -547,9 +547,10 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
         return st, d, done
 
     # With engine.settings.pipeline, chromosome k + 1 is prepared on a second stream while chromosome k scans (the
-    # delta search and rotations beside the MFMA-bound scan); otherwise both use the current stream, in order.
-    # Each chromosome's buffers are dropped only after the scan stream has been synchronised, so the caching
-    # allocator never hands out memory a queued kernel still reads.
+    # delta search and rotations beside the MFMA-bound scan); each chromosome's buffers are then dropped only after
+    # the scan stream has been synchronised, so the caching allocator never hands out memory a queued kernel on the
+    # other stream still reads.  Otherwise everything runs on the current stream, in order, where freed memory is
+    # only reused by later work on that stream, and the host does not wait between chromosomes.
     main = torch.cuda.current_stream(dev)
     side = torch.cuda.Stream(device=dev) if engine.settings.pipeline else main
     side.wait_stream(main)
@@ -574,7 +575,8 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
             st.scan(Zd, test, n_real, out, d, after_launch=launch_next if piped else None, first_row=first)
         if not piped:
             launch_next()
-        main.synchronize()
+        if piped or on_chrom is not None:
+            main.synchronize()
         del st, d, done
         if on_chrom is not None:
             on_chrom(k + 1, len(chroms))

@@ -53,9 +53,9 @@ class Settings:
                                    # scales (split_fp16), "fp32" = FP32 MFMA
     screen: str = "auto"           # permutation columns' max_F: "auto" = the certified hi-plane screen with FP64
                                    # re-evaluation of the tests that can hold each maximum (split_fp16 with the
-                                   # FP64 SNP rotation), the columns at the grid's upper log delta on the sWy-only
-                                   # tile; "general" = the same screen without that split; "off" = the full split
-                                   # scan for every column
+                                   # FP64 SNP rotation), every column on the sWy-only tile with 1 / sWs
+                                   # interpolated in its log-delta bucket; "general" = the same screen with both
+                                   # products on hi planes; "off" = the full split scan for every column
     pipeline: bool = False         # LOCO: prepare chromosome c + 1 (basis, rotation, delta) on a second stream
                                    # while chromosome c scans (same bits; ~1% less step time on the N=1000,
                                    # M=100k workload, but the co-running kernels slow the scan kernel ~1.7x)
