@@ -317,7 +317,10 @@ __global__ __launch_bounds__(256, 1) void wald_screen_kernel(ScreenArgs g, int64
 // the normal range marks the record forced).  Rows whose node sums are not positive, or whose scaled reciprocals are
 // not finite FP32 numbers, get G = 0 (x = 0) and make their tile forced; padding rows get G = 0.
 constexpr int SU_SLOTS = 2;                        // K-chunks per stage
-constexpr int64_t SB_ROWS_PER_GROUP = 16 * W_TM;   // SNP rows per scheduling group of the bucketed tiles
+// SNP rows per scheduling group of the bucketed tiles: 12 tiles of 128 rows (at N = 1,000 x 648k screened columns
+// x 4,546 SNPs, 36 row tiles: 6.47 ms and 13.6 GB fetched per call, against 6.53 ms / 15.9 GB for 16 and
+// 6.60 ms / 11.2 GB for 8; fewer rows keep the group's SNP planes in L2, more share each column chunk more ways)
+constexpr int64_t SB_ROWS_PER_GROUP = 12 * W_TM;
 // 128-row halves per bucketed workgroup: 1 (two workgroups per CU, whose stages drift apart so one's epilogue and
 // load waits overlap the other's MFMAs) measured 6.6 ms per chromosome at N = 1,000 x 648k screened columns against
 // 7.4 ms for 2 (one 512-thread workgroup per CU, half the SNP-side bytes per flop, waves in lockstep at each stage)
