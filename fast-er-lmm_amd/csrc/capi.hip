@@ -136,7 +136,7 @@ static double* scratch_ptr(hipStream_t stream, const char* who) {
 
 extern "C" {
 
-int flmm_abi_version(void) { return 15; }
+int flmm_abi_version(void) { return 16; }
 
 const char* flmm_last_error(void) { return g_err.c_str(); }
 
@@ -177,6 +177,24 @@ int flmm_pack_planes(int prec, const double* src, int64_t n_rows, int64_t n_k, i
   return check_launch(launch_pack_planes(prec, src, n_rows, n_k, row_stride, k_stride, row_index, row_norm, plane_hi,
                                          plane_lo, row_scale, rows_pad, kpad, (hipStream_t)stream),
                       "flmm_pack_planes");
+}
+
+int flmm_pack_columns_projected(int prec, const double* y, int64_t N, int64_t P, const double* x, int C,
+                                const double* b, double* norm, void* plane_hi, void* plane_lo, float* row_scale,
+                                int64_t rows_pad, int64_t kpad, void* stream) {
+  REQUIRE(valid_prec(prec), "flmm_pack_columns_projected: bad precision %d", prec);
+  REQUIRE(N >= 0 && P >= 0 && C >= 0 && C <= 8 && P <= rows_pad && N <= kpad,
+          "flmm_pack_columns_projected: bad sizes (C <= 8)");
+  REQUIRE(rows_pad % TILE == 0 && kpad % KCHUNK == 0,
+          "flmm_pack_columns_projected: rows_pad %% 128 and kpad %% 32 required");
+  REQUIRE(plane_hi && (!two_planes(prec) || plane_lo), "flmm_pack_columns_projected: null plane");
+  REQUIRE(prec != FLMM_PREC_SPLIT_FP16 || (row_scale && (norm || P == 0)),
+          "flmm_pack_columns_projected: split-FP16 needs norm and row_scale");
+  REQUIRE((y && (C == 0 || (x && b))) || P == 0 || N == 0, "flmm_pack_columns_projected: null pointer");
+  if (rows_pad == 0 || kpad == 0) return FLMM_OK;
+  return check_launch(launch_pack_columns_projected(prec, y, N, P, x, C, b, norm, plane_hi, plane_lo, row_scale,
+                                                    rows_pad, kpad, (hipStream_t)stream),
+                      "flmm_pack_columns_projected");
 }
 
 int flmm_rotate(int prec, int mode, const void* a_hi, const void* a_lo, const float* a_scale, int64_t a_rows_pad,

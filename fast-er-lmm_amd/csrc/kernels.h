@@ -25,6 +25,9 @@ struct ScanScales {
 hipError_t launch_pack_planes(int prec, const double* src, int64_t n_rows, int64_t n_k, int64_t row_stride,
                               int64_t k_stride, const int64_t* row_index, const double* row_norm, void* out_a,
                               void* out_b, float* row_scale, int64_t rows_pad, int64_t kpad, hipStream_t stream);
+hipError_t launch_pack_columns_projected(int prec, const double* y, int64_t N, int64_t P, const double* x, int C,
+                                         const double* b, double* norm, void* out_a, void* out_b, float* row_scale,
+                                         int64_t rows_pad, int64_t kpad, hipStream_t stream);
 hipError_t launch_row_norms(const double* src, int64_t n_rows, int64_t n_k, int64_t row_stride, int64_t k_stride,
                             const int64_t* row_index, double* norm, hipStream_t stream);
 template <int PREC>

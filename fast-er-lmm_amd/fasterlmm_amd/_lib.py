@@ -55,6 +55,7 @@ SIGNATURES: dict[str, list] = {
     "flmm_bed_decode": [_P, _I64, _P, _I64, _P, _I64, _I, _I, _P, _I64, _P, _P, _P],
     "flmm_row_norms": [_P, _I64, _I64, _I64, _I64, _P, _P, _P],
     "flmm_pack_planes": [_I, _P, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _I64, _I64, _P],
+    "flmm_pack_columns_projected": [_I, _P, _I64, _I64, _P, _I, _P, _P, _P, _P, _P, _I64, _I64, _P],
     "flmm_rotate": [_I, _I, _P, _P, _P, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P],
     "flmm_delta_weights": [_P, _I64, _P, _I64, _I64, _I, _D, _D, _P, _P, _P, _P],
     "flmm_delta_grid": [_P, _P, _I, _D, _P, _I64, _I64, _I64, _P, _P],

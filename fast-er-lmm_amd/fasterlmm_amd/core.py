@@ -504,10 +504,16 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
     # the basis is orthogonal to X, so B^T (Y - X X^+ Y) = B^T Y; the OLS regress-out keeps the compat
     # path's arithmetic (core.py:517) and costs one pass
     with stage("pack_columns"):
-        Y_r = torch.addmm(Yd, Xd, Xpinv @ Yd, alpha=-1.0)  # Y - X (X^+ Y), the subtraction in the product's pass
-        ypl = engine.pack(prec, Y_r, P, N, 1, P, rup(max(P, 1), PTILE), rup(N, KCHUNK))
-        y_real = Y_r[:, :n_real].contiguous()  # kept for the FP64 re-scoring of large-F real tests
-        del Y_r
+        Bc = Xpinv @ Yd
+        if D <= 8:  # Y_r = Y - X Bc is packed in one pass, never formed
+            ypl = engine.pack_columns_projected(prec, Yd, Xd, Bc, rup(max(P, 1), PTILE), rup(N, KCHUNK))
+            y_real = torch.addmm(Yd[:, :n_real], Xd, Bc[:, :n_real], alpha=-1.0)  # kept for the FP64 re-scoring
+        else:
+            Y_r = torch.addmm(Yd, Xd, Bc, alpha=-1.0)
+            ypl = engine.pack(prec, Y_r, P, N, 1, P, rup(max(P, 1), PTILE), rup(N, KCHUNK))
+            y_real = Y_r[:, :n_real].contiguous()
+            del Y_r
+        del Bc
     jitter = 1e-6 if plain else 0.0
     eye = torch.eye(N, dtype=torch.float64, device=dev)
     chroms = sorted(np.unique(chrom).tolist())

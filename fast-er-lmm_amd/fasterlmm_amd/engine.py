@@ -163,6 +163,20 @@ def pack(prec: int, src: torch.Tensor, n_rows: int, n_k: int, row_stride: int, k
     return pl
 
 
+def pack_columns_projected(prec: int, Y: torch.Tensor, X: torch.Tensor, B: torch.Tensor, rows_pad: int,
+                           kpad: int) -> Planes:
+    """Planes of the transposed residual Y - X B (row p = column p, k = strain) without forming it
+    (flmm_pack_columns_projected): Y (N, P), X (N, C), B (C, P) FP64 on the GPU."""
+    Y, X, B = (t.contiguous() for t in (Y, X, B))
+    N, P = Y.shape
+    pl = Planes(prec, rows_pad, kpad, Y.device)
+    hi, lo = pl.ptrs
+    norm = torch.empty(max(P, 1), dtype=torch.float64, device=Y.device) if prec == _lib.PREC_SPLIT_FP16 else None
+    _lib.call("flmm_pack_columns_projected", prec, Y.data_ptr(), N, P, X.data_ptr(), X.shape[1], B.data_ptr(),
+              _lib.ptr(norm), hi, lo, pl.scale_ptr, rows_pad, kpad, _lib.stream())
+    return pl
+
+
 def gram(Z: torch.Tensor, prec: int | None = None) -> torch.Tensor:
     """G = Z Z^T (n x n FP64) of Z (n, m) FP64 on the GPU, by the kinship setting: flmm_gram_f64 on the FP64
     matrix cores (default), rocBLAS ("blas"), or flmm_gram in the split precision ("split").  Deterministic: the same

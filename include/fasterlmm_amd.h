@@ -130,6 +130,16 @@ int flmm_pack_planes(int prec, const double* src, int64_t n_rows, int64_t n_k, i
                      const int64_t* row_index, const double* row_norm, void* plane_hi, void* plane_lo,
                      float* row_scale, int64_t rows_pad, int64_t kpad, void* stream);
 
+/* The regression of the columns on the covariates and their packing in one pass (fastlmm_compat_rotate's
+ * Y_r = Y - X (X^+ Y), fasterlmm/core.py:517, and the packing of Y_r for U_eff^T Y_r, core.py:518): planes of the
+ * transposed view of Y_r (row p = column p, k = strain n < N, zeros up to (rows_pad, kpad)) from Y (N x P row-major
+ * FP64), X (N x C row-major) and B = X^+ Y (C x P row-major), each element formed as y - x_0 b_0 - x_1 b_1 - ...
+ * without writing Y_r; C <= 8.  Split-FP16 first writes the rows' L2 norms to norm (P doubles) and scales as
+ * flmm_pack_planes does. */
+int flmm_pack_columns_projected(int prec, const double* y, int64_t N, int64_t P, const double* x, int C,
+                                const double* b, double* norm, void* plane_hi, void* plane_lo, float* row_scale,
+                                int64_t rows_pad, int64_t kpad, void* stream);
+
 /* Rotation product x[i][j] = sum_n A[i][n] B[j][n] (A: a_rows_pad x n_pad planes, B: b_rows_pad x n_pad planes
  * whose rows are orthonormal basis vectors).  Replaces U_eff.T @ Y_r (core.py:518) and U_eff.T @ S_r
  * (core.py:629).  a_scale / b_scale are the operands' row scales (split-FP16 only).

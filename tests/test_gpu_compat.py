@@ -169,6 +169,34 @@ def test_loco_pipeline_is_bit_identical(core, ex, precision):
         assert torch.equal(getattr(got[True], k), getattr(got[False], k)), k
 
 
+@pytest.mark.parametrize("precision", ["split_fp16", "split_bf16", "fp32"])
+def test_pack_columns_projected_matches_packed_residual(precision):
+    """flmm_pack_columns_projected (the residual Y - X B formed inside the norm and pack passes) against packing the
+    materialised residual: the same row scales and values to the planes' precision, and the FP64 residual."""
+    from fasterlmm_amd import engine
+    prec = engine.prec_code(precision)
+    rng = np.random.default_rng(31)
+    N, P, C = 300, 700, 3
+    Y = rng.standard_normal((N, P)) * 10.0 ** rng.uniform(-3, 3, P)
+    X = np.concatenate([np.ones((N, 1)), rng.standard_normal((N, C - 1))], axis=1)
+    B = np.linalg.pinv(X) @ Y
+    Yd, Xd, Bd = (torch.from_numpy(np.ascontiguousarray(a)).to(DEV) for a in (Y, X, B))
+    rows_pad, kpad = 768, 320
+    got = engine.pack_columns_projected(prec, Yd, Xd, Bd, rows_pad, kpad)
+    ref = engine.pack(prec, torch.addmm(Yd, Xd, Bd, alpha=-1.0), P, N, 1, P, rows_pad, kpad)
+    torch.cuda.synchronize()
+    if precision == "split_fp16":
+        assert torch.equal(got.scale, ref.scale)
+    g, r = got.to_dense(P, N).cpu().numpy(), ref.to_dense(P, N).cpu().numpy()
+    want = (Y - X @ B).T
+    scale = np.abs(want).max(axis=1, keepdims=True)  # each row against its largest entry
+    tol = {"split_fp16": 1e-6, "split_bf16": 3e-5, "fp32": 3e-7}[precision]
+    assert np.all(np.abs(g - r) <= tol * scale)
+    assert np.all(np.abs(g - want) <= tol * scale)
+    dense_pad = got.to_dense(rows_pad, kpad).cpu().numpy()  # padding rows and k are zero
+    assert not dense_pad[P:].any() and not dense_pad[:, N:].any()
+
+
 def test_split_kinship_and_rotation_settings_vs_reference_golden(core, ex):
     """The opt-in split-precision kinship Gram and SNP rotation (engine.settings kinship="split",
     snp_rotation="split") still meet the scan bars against the reference's LOCO outputs on its example panel, whose
