@@ -136,7 +136,7 @@ static double* scratch_ptr(hipStream_t stream, const char* who) {
 
 extern "C" {
 
-int flmm_abi_version(void) { return 16; }
+int flmm_abi_version(void) { return 17; }
 
 const char* flmm_last_error(void) { return g_err.c_str(); }
 
@@ -395,14 +395,15 @@ int flmm_gemm_f64(const double* a, int64_t rows, int64_t k, const double* b, int
 
 int flmm_rescore_f64(const double* US, const double* UY, int64_t ld, const double* s, int64_t neff,
                      const double* log_delta, const int64_t* us_row, const int64_t* col, const int64_t* out_row,
-                     int64_t n, double nu, double* f, double* beta, double* se, double* sfve, int64_t out_ld,
-                     void* stream) {
-  REQUIRE(n >= 0 && neff >= 1 && neff <= ld && out_ld >= 1 && nu > 0.0, "flmm_rescore_f64: bad sizes");
-  REQUIRE(n == 0 || (US && UY && s && log_delta && us_row && col && out_row && f && beta && se && sfve),
+                     int64_t out_row_base, int64_t n, const unsigned long long* n_dev, double nu, double* f,
+                     double* beta, double* se, double* sfve, int64_t out_ld, void* stream) {
+  REQUIRE(n >= 0 && neff >= 1 && neff <= ld && out_ld >= 1 && nu > 0.0 && out_row_base >= 0,
+          "flmm_rescore_f64: bad sizes");
+  REQUIRE(n == 0 || (US && UY && s && log_delta && us_row && col && f && beta && se && sfve),
           "flmm_rescore_f64: null pointer");
   if (n == 0) return FLMM_OK;
-  return check_launch(launch_rescore(US, UY, ld, s, neff, log_delta, us_row, col, out_row, n, nu, f, beta, se, sfve,
-                                     out_ld, (hipStream_t)stream),
+  return check_launch(launch_rescore(US, UY, ld, s, neff, log_delta, us_row, col, out_row, out_row_base, n, n_dev, nu,
+                                     f, beta, se, sfve, out_ld, (hipStream_t)stream),
                       "flmm_rescore_f64");
 }
 

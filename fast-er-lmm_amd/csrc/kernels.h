@@ -54,8 +54,8 @@ hipError_t launch_select_above(const double* f, int64_t rows, int64_t cols, int6
                                int64_t* out_col, int64_t cap, unsigned long long* count, hipStream_t stream);
 hipError_t launch_rescore(const double* US, const double* UY, int64_t ld, const double* s, int64_t neff,
                           const double* log_delta, const int64_t* us_row, const int64_t* col, const int64_t* out_row,
-                          int64_t n, double nu, double* f, double* beta, double* se, double* sfve, int64_t out_ld,
-                          hipStream_t stream);
+                          int64_t out_row_base, int64_t n, const unsigned long long* n_dev, double nu, double* f,
+                          double* beta, double* se, double* sfve, int64_t out_ld, hipStream_t stream);
 template <int PREC>
 hipError_t launch_gram(const TileArgs<PREC, 1>& args, const float* row_scale, int n_split, double* partial, int64_t n,
                        double* out, int64_t ld, hipStream_t stream);

@@ -302,24 +302,11 @@ hipError_t launch_gemm_f64(const double* x2, int64_t rows, int64_t kpad, const d
   return hipGetLastError();
 }
 
-// FP64 re-evaluation of selected tests i < n, (SNP row us_row[i] of US, column col[i]): with w_k = 1 / (s_k + delta),
-// delta = exp(log_delta[col]), the exact sums a = sum_k US w UY, d = sum_k US^2 w and rWr = sum_k w UY^2 over the
-// FP64 operands (US: SNP rows, UY: column rows, both with leading dimension ld), then the scan epilogue's Wald
-// arithmetic (core.py:649-667) in FP64, written at [out_row[i] * out_ld + col[i]].  One wave per test, lanes
-// striding k, fixed-order wave sums.
-__global__ __launch_bounds__(256) void rescore_kernel(const double* __restrict__ US, const double* __restrict__ UY,
-                                                      int64_t ld, const double* __restrict__ s, int64_t neff,
-                                                      const double* __restrict__ log_delta,
-                                                      const int64_t* __restrict__ us_row,
-                                                      const int64_t* __restrict__ col,
-                                                      const int64_t* __restrict__ out_row, int64_t n, double nu,
-                                                      double* __restrict__ f, double* __restrict__ beta,
-                                                      double* __restrict__ se, double* __restrict__ sfve,
-                                                      int64_t out_ld) {
-  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (i >= n) return;  // wave-uniform
-  const int64_t m = us_row[i], p = col[i];
+__device__ inline void rescore_one(const double* __restrict__ US, const double* __restrict__ UY, int64_t ld,
+                                   const double* __restrict__ s, int64_t neff, const double* __restrict__ log_delta,
+                                   int64_t m, int64_t p, int64_t orow, double nu, double* __restrict__ f,
+                                   double* __restrict__ beta, double* __restrict__ se, double* __restrict__ sfve,
+                                   int64_t out_ld, int lane) {
   const double delta = exp(log_delta[p]);
   const double* u = US + m * ld;
   const double* y = UY + p * ld;
@@ -338,12 +325,39 @@ __global__ __launch_bounds__(256) void rescore_kernel(const double* __restrict__
     const double b = a / d;
     const double rss = fmax(r - a * a / d, FLOOR64);
     const double var = rss / nu / d;
-    const int64_t o = out_row[i] * out_ld + p;
+    const int64_t o = orow * out_ld + p;
     f[o] = b * b / var;
     beta[o] = b;
     se[o] = sqrt(var);
     sfve[o] = sqrt(fmax(a * a / (d * r), 0.0));
   }
+}
+
+// FP64 re-evaluation of selected tests i < n, (SNP row us_row[i] of US, column col[i]): with w_k = 1 / (s_k + delta),
+// delta = exp(log_delta[col]), the exact sums a = sum_k US w UY, d = sum_k US^2 w and rWr = sum_k w UY^2 over the
+// FP64 operands (US: SNP rows, UY: column rows, both with leading dimension ld), then the scan epilogue's Wald
+// arithmetic (core.py:649-667) in FP64, written at [row * out_ld + col[i]], row = out_row[i] (or out_row_base +
+// us_row[i] without out_row).  One wave per test, lanes striding k, fixed-order wave sums.  With n_dev the number of
+// tests is min(*n_dev, n), read on the device, and the waves stride over them (a fixed grid).
+__global__ __launch_bounds__(256) void rescore_kernel(const double* __restrict__ US, const double* __restrict__ UY,
+                                                      int64_t ld, const double* __restrict__ s, int64_t neff,
+                                                      const double* __restrict__ log_delta,
+                                                      const int64_t* __restrict__ us_row,
+                                                      const int64_t* __restrict__ col,
+                                                      const int64_t* __restrict__ out_row, int64_t out_row_base,
+                                                      int64_t n, const unsigned long long* __restrict__ n_dev,
+                                                      double nu, double* __restrict__ f, double* __restrict__ beta,
+                                                      double* __restrict__ se, double* __restrict__ sfve,
+                                                      int64_t out_ld) {
+  const int lane = threadIdx.x & 63;
+  int64_t nt = n;
+  if (n_dev) {
+    const unsigned long long c = *n_dev;
+    nt = c < (unsigned long long)n ? (int64_t)c : n;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); i < nt; i += (int64_t)gridDim.x * 4)  // wave-uniform
+    rescore_one(US, UY, ld, s, neff, log_delta, us_row[i], col[i], out_row ? out_row[i] : out_row_base + us_row[i],
+                nu, f, beta, se, sfve, out_ld, lane);
 }
 
 // The tests the re-scoring takes: every (row, column) of f (rows x cols, leading dimension ld) with f > thr, appended
@@ -385,11 +399,12 @@ hipError_t launch_select_above(const double* f, int64_t rows, int64_t cols, int6
 
 hipError_t launch_rescore(const double* US, const double* UY, int64_t ld, const double* s, int64_t neff,
                           const double* log_delta, const int64_t* us_row, const int64_t* col, const int64_t* out_row,
-                          int64_t n, double nu, double* f, double* beta, double* se, double* sfve, int64_t out_ld,
-                          hipStream_t stream) {
-  const dim3 grid((unsigned)((n + 3) / 4));
-  hipLaunchKernelGGL(rescore_kernel, grid, dim3(256), 0, stream, US, UY, ld, s, neff, log_delta, us_row, col, out_row,
-                     n, nu, f, beta, se, sfve, out_ld);
+                          int64_t out_row_base, int64_t n, const unsigned long long* n_dev, double nu, double* f,
+                          double* beta, double* se, double* sfve, int64_t out_ld, hipStream_t stream) {
+  // one wave per test, or with a device count a fixed grid of 4,096 waves striding over the tests
+  const int64_t blocks = n_dev ? (n + 3) / 4 < 1024 ? (n + 3) / 4 : 1024 : (n + 3) / 4;
+  hipLaunchKernelGGL(rescore_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, US, UY, ld, s, neff, log_delta,
+                     us_row, col, out_row, out_row_base, n, n_dev, nu, f, beta, se, sfve, out_ld);
   return hipGetLastError();
 }
 

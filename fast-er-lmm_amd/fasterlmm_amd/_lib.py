@@ -66,7 +66,7 @@ SIGNATURES: dict[str, list] = {
     "flmm_gram_f64": [_P, _I64, _I64, _I64, _I, _P, _P, _I64, _P],
     "flmm_rotate_f64": [_P, _I64, _P, _I64, _I64, _P, _I64, _I64, _P, _I64, _I64, _P],
     "flmm_gemm_f64": [_P, _I64, _I64, _P, _I64, _P, _P],
-    "flmm_rescore_f64": [_P, _P, _I64, _P, _I64, _P, _P, _P, _P, _I64, _D, _P, _P, _P, _P, _I64, _P],
+    "flmm_rescore_f64": [_P, _P, _I64, _P, _I64, _P, _P, _P, _P, _I64, _I64, _P, _D, _P, _P, _P, _P, _I64, _P],
     "flmm_select_above": [_P, _I64, _I64, _I64, _D, _P, _P, _I64, _P, _P],
     "flmm_factored_weight_planes": [_I, _I64, _I64, _I64, _P, _P, _I64, _P, _P, _P, _P, _I64, _P, _P],
     "flmm_delta_refine_learned": [_I, _P, _I64, _I64, _I64, _P, _I64, _P, _I64, _D, _I, _D, _D, _P, _P, _D, _D, _P,

@@ -227,9 +227,27 @@ class _Basis:
         Block position i holds SNP column local[i] of columns(idx) -> (N, len(idx)) FP64 and output row
         row_index[i]; us, if given, the block's FP64 rotated SNPs (row i = block position i), which then need no
         second rotation; row_range (start, stop), if given, says row_index is start, start + 1, ..., stop - 1.
-        Returns the number of re-scored tests."""
+        Returns the number of re-scored tests, or -1 where both us and row_range are given (the count then stays on
+        the device: the host does not wait for the scan)."""
         if n_real == 0 or self.y_cols is None or row_index.numel() == 0:
             return 0
+        UY = self._uy64_for(n_real)
+        if us is not None and row_range is not None:
+            # consecutive output rows and the block's FP64 rotated SNPs at hand: select into lists that hold every
+            # test of the block and re-score on the device's count, with no host read (-1: count not read)
+            F = out.f[row_range[0]:row_range[1]]
+            cap = F.shape[0] * n_real
+            mi = torch.empty(cap, dtype=torch.int64, device=F.device)
+            pi = torch.empty(cap, dtype=torch.int64, device=F.device)
+            cnt = torch.zeros(1, dtype=torch.int64, device=F.device)
+            st = _lib.stream()
+            _lib.call("flmm_select_above", F.data_ptr(), F.shape[0], n_real, F.stride(0), float(RESCORE_F),
+                      mi.data_ptr(), pi.data_ptr(), cap, cnt.data_ptr(), st)
+            _lib.call("flmm_rescore_f64", us.data_ptr(), UY.data_ptr(), self.kpad, self.s.data_ptr(), self.neff,
+                      d.log_delta.data_ptr(), mi.data_ptr(), pi.data_ptr(), None, int(row_range[0]), cap,
+                      cnt.data_ptr(), float(self.nu), out.f.data_ptr(), out.beta.data_ptr(), out.se.data_ptr(),
+                      out.sfve.data_ptr(), out.f.shape[1], st)
+            return -1
         if row_range is not None:  # consecutive output rows: a view instead of a gathered copy
             F = out.f[row_range[0]:row_range[1]]
         else:
@@ -244,20 +262,25 @@ class _Basis:
             um, inv = torch.unique(mi, return_inverse=True)
             inv = inv.contiguous()
             US = self._rotate_f64(columns(local.index_select(0, um)))
+        _lib.call("flmm_rescore_f64", US.data_ptr(), UY.data_ptr(), self.kpad, self.s.data_ptr(), self.neff,
+                  d.log_delta.data_ptr(), inv.data_ptr(), pi.data_ptr(), row_index.index_select(0, mi).data_ptr(), 0,
+                  n, None, float(self.nu), out.f.data_ptr(), out.beta.data_ptr(), out.se.data_ptr(),
+                  out.sfve.data_ptr(), out.f.shape[1], _lib.stream())
+        return n
+
+    def _uy64_for(self, n_real: int) -> Tensor:
+        """The first n_real columns' FP64 rotation (cached: it does not depend on the SNP block, so one FP64
+        product keeps results block-invariant)."""
         UY = self._uy64.get(n_real)
         wider = [k for k in self._uy64 if k > n_real]
         if UY is None and wider:
             UY = self._uy64[min(wider)][:n_real]
-        if UY is None:  # does not depend on the SNP block, so one FP64 product keeps results block-invariant
+        if UY is None:
             yc = self.y_cols[:, :n_real].contiguous()
             UY = engine.rotate_snps_f64(yc, None, n_real, self.mat, self.col0, self.neff, rup(n_real, TILE),
                                         self.kpad)[:n_real]
             self._uy64[n_real] = UY
-        _lib.call("flmm_rescore_f64", US.data_ptr(), UY.data_ptr(), self.kpad, self.s.data_ptr(), self.neff,
-                  d.log_delta.data_ptr(), inv.data_ptr(), pi.data_ptr(), row_index.index_select(0, mi).data_ptr(), n,
-                  float(self.nu), out.f.data_ptr(), out.beta.data_ptr(), out.se.data_ptr(), out.sfve.data_ptr(),
-                  out.f.shape[1], _lib.stream())
-        return n
+        return UY
 
     def uy_dense(self) -> Tensor:
         return engine.fp32_plane_to_dense(self.uy, self.p_pad, self.kpad, self.P, self.neff).T.double()
@@ -498,6 +521,12 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
     Zk = Zd if Zkd is None else Zkd
     ck = chrom if chrom_kin is None else np.asarray(chrom_kin)
     stage = engine.stage
+    chroms = sorted(np.unique(chrom).tolist())
+    # every chromosome's SNP rows in one upload before any kernel is queued (a copy from host memory waits for the
+    # stream), then device slices
+    rows_of = [np.flatnonzero(chrom == c) for c in chroms]
+    all_rows = torch.from_numpy(np.concatenate(rows_of) if rows_of else np.zeros(0, np.int64)).to(dev)
+    row_start = np.concatenate([[0], np.cumsum([r.size for r in rows_of])])
     with stage("kinship"):
         K_all = engine.gram(Zk, prec)
     Xpinv, Iperp = _projector(Xd)
@@ -516,7 +545,6 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
         del Bc
     jitter = 1e-6 if plain else 0.0
     eye = torch.eye(N, dtype=torch.float64, device=dev)
-    chroms = sorted(np.unique(chrom).tolist())
     # every chromosome's projected kinship first, then one batched FP64 eigendecomposition
     Kps, Ks = [], []
     with stage("kinship"):
@@ -566,8 +594,8 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
         st, d, done = nxt
         nxt = None
         main.wait_event(done)
-        rows = np.flatnonzero(chrom == c)
-        test = torch.from_numpy(rows).to(dev)
+        rows = rows_of[k]
+        test = all_rows[int(row_start[k]):int(row_start[k + 1])]
         first = int(rows[0]) if rows.size and rows[-1] - rows[0] + 1 == rows.size else None  # consecutive rows
 
         def launch_next(k=k):

@@ -260,13 +260,16 @@ int flmm_gemm_f64(const double* a, int64_t rows, int64_t k, const double* b, int
  * row us_row[i] of US (rotated SNPs, leading dimension ld) with column col[i] (row col[i] of UY, leading
  * dimension ld; delta = exp(log_delta[col[i]])): a = sum_k US w UY, d = sum_k US^2 w, rWr = sum_k w UY^2,
  * w_k = 1 / (s_k + delta), k < neff; then f, beta, se, sfve as the scan writes them, at
- * [out_row[i] * out_ld + col[i]].  The split-precision scan's FP32 sums carry ~1e-7 relative error, which
+ * [row * out_ld + col[i]], row = out_row[i], or out_row_base + us_row[i] where out_row is NULL.  n_dev (may be
+ * NULL): a device count, read by the kernel; the tests are then i < min(*n_dev, n) (n the lists' capacity), so a
+ * caller can launch on flmm_select_above's output without reading its count.  The split-precision scan's FP32
+ * sums carry ~1e-7 relative error, which
  * rss = rWr - sWy^2 / sWs amplifies by ~F / nu: re-scoring the large-F tests of the real columns keeps
  * |d log10 p| at FP64 level where p is smallest. */
 int flmm_rescore_f64(const double* US, const double* UY, int64_t ld, const double* s, int64_t neff,
                      const double* log_delta, const int64_t* us_row, const int64_t* col, const int64_t* out_row,
-                     int64_t n, double nu, double* f, double* beta, double* se, double* sfve, int64_t out_ld,
-                     void* stream);
+                     int64_t out_row_base, int64_t n, const unsigned long long* n_dev, double nu, double* f,
+                     double* beta, double* se, double* sfve, int64_t out_ld, void* stream);
 
 /* The tests the FP64 re-scoring takes (core.py:640-667's F of one SNP block, the selection before
  * flmm_rescore_f64): every (row, column) of f (rows x cols, leading dimension ld, FP64) with f > thr, written to

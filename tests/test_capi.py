@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_error_channel():
     from fasterlmm_amd import _lib
     lib = _lib.load()
-    assert lib.flmm_abi_version() == 16
+    assert lib.flmm_abi_version() == 17
     # argument validation runs on the host and reports through flmm_last_error without touching a GPU
     rc = lib.flmm_max_reduce(None, 1, 0, 4, None, None)
     assert rc == 1
@@ -60,9 +60,9 @@ def test_bad_arguments_raise_value_error():
     with pytest.raises(ValueError, match="null pointer"):
         _lib.call("flmm_gram_f64", 16, 100, 10, 100, 1, None, 16, 10, None)
     with pytest.raises(ValueError, match="bad sizes"):
-        _lib.call("flmm_rescore_f64", 16, 16, 128, 16, 200, 16, 16, 16, 16, 5, 10.0, 16, 16, 16, 16, 4, None)
+        _lib.call("flmm_rescore_f64", 16, 16, 128, 16, 200, 16, 16, 16, 16, 0, 5, None, 10.0, 16, 16, 16, 16, 4, None)
     with pytest.raises(ValueError, match="null pointer"):
-        _lib.call("flmm_rescore_f64", 16, None, 128, 16, 100, 16, 16, 16, 16, 5, 10.0, 16, 16, 16, 16, 4, None)
+        _lib.call("flmm_rescore_f64", 16, None, 128, 16, 100, 16, 16, 16, 16, 0, 5, None, 10.0, 16, 16, 16, 16, 4, None)
     with pytest.raises(ValueError, match="R <= rpad"):
         _lib.call("flmm_factored_weight_planes", 2, 10, 256, 128, 16, 16, 129, 16, 16, 16, 16, 256, None, None)
     with pytest.raises(ValueError, match="q2_scale"):
