@@ -128,6 +128,42 @@ def test_bucketed_screen_at_fixed_delta(panel, layout):
     assert rel[256:].max() <= 1e-5, (rel[256:].max(), 256 + int(np.argmax(rel[256:])))
 
 
+def test_bucketed_screen_forces_columns_outside_its_buckets(panel):
+    """Columns whose log delta lies outside the bucketed layout's range (here a layout over [-10, -2] for columns
+    spread over [-10, 10]) are screened as forced: every row of theirs is re-evaluated in FP64, so max_F still equals
+    the general tile's bit for bit, with more FP64 evaluations."""
+    from fasterlmm_amd import _lib, core, engine
+    pan, Zs, Y = panel
+    on = pan.chrom == 1
+    K = lmm.grm(Zs[:, ~on])
+    S = Zs[:, on][:, :1000]
+    want = lmm.compat_rotate(K, pan.X, Y)
+    ld = lmm.fit_delta_grid_compat(want, n_grid=64).copy()
+    cols = np.arange(256, Y.shape[1])
+    ld[cols] = np.random.default_rng(3).uniform(-10.0, 10.0, cols.size)
+    spec = core.fastlmm_compat_rotate(T(K), T(pan.X), T(Y))
+    st = spec._gpu
+    hf = engine.screen_start(st.prec, 4, st.P)
+    fac = st.factored(float(ld.min()), float(ld.max()))
+    got, ev = {}, {}
+    for mode, t_range in (("general", None), ("auto", (-10.0, -2.0))):
+        with engine.configure(screen=mode):
+            d = engine.delta_refine(st.prec, _lib.DELTA_FIXED, st.uy, st.P, st.p_pad, st.kpad, st.s, st.s,
+                                    float(st.neff), 1, 0.0, 0.0, None, log_delta_in=T(ld), fac=fac, hi_from=hf,
+                                    t_range=t_range)
+            assert (d.layout is not None) == (mode == "auto")
+            out = engine.ScanOutputs.allocate(S.shape[1], 4, st.P, T(S).device)
+            engine.screen_stats.reset(True)
+            st.scan(T(S), torch.arange(S.shape[1], device=DEV), 4, out, d)
+            engine.refresh_real_max(out, 4)
+            got[mode] = out.max_F.cpu().numpy()
+            ev[mode] = engine.screen_stats.summary()["fp64_evaluations"]
+            engine.screen_stats.reset(False)
+    say(f"forced columns: FP64 evaluations {ev['auto']} (layout over [-10, -2]) vs {ev['general']} (general tile)")
+    np.testing.assert_array_equal(got["auto"], got["general"])
+    assert ev["auto"] > ev["general"]
+
+
 def test_screen_loco_matches_full_scan_and_oracle(panel):
     from fasterlmm_amd import core, engine
     pan, Zs, Y = panel
