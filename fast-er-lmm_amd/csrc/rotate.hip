@@ -221,6 +221,153 @@ __global__ __launch_bounds__(512, 2) void rotate_uy_wide_kernel(TileArgs<PREC, 1
     }
 }
 
+// MODE_UY in the split precisions on a 128 x 256 tile per 256-thread workgroup, two workgroups per CU: wave w owns
+// the tile's 128 A rows (columns p) and B rows 64 w .. + 64 (basis directions k), 4 x 2 blocks of 32 x 32, 128
+// accumulators.  Each 32-k chunk of the A planes (2 x 128 rows x 64 B = 16 KiB) is staged in LDS, double-buffered
+// (32 KiB per workgroup); the B planes are small (the basis, L2-resident) and each wave loads its own fragments
+// straight into registers one chunk ahead (lane (c, h) of block bj: 16 bytes at row 32 bj + c, segment 2 h + step).
+// The products of an output element are summed in the same order as in rotate_kernel (per chunk and step: hi hi,
+// hi lo, lo hi), so the results are bit-identical.  Two independent workgroups per CU drift apart, so one's load
+// waits and epilogue overlap the other's MFMAs.
+constexpr int RP_TA = 128;  // A rows (columns p) per tile
+constexpr int RP_TB = 256;  // B rows (basis directions k) per tile
+
+template <int PREC>
+__global__ __launch_bounds__(256, 2) void rotate_uy_pair_kernel(TileArgs<PREC, 1> args, int64_t n_jtiles,
+                                                                int64_t n_tiles, const float* __restrict__ a_scale,
+                                                                const float* __restrict__ b_scale,
+                                                                float* __restrict__ out) {
+  __shared__ __attribute__((aligned(16))) char lds[2][2 * RP_TA * 64];  // [buffer][plane][row][64 B]
+  const int64_t tile = xcd_remap(blockIdx.x, n_tiles);
+  const int64_t jt = tile % n_jtiles;  // j fastest: the tiles sharing one A block run together on one XCD
+  const int64_t it = tile / n_jtiles;
+  const int64_t i0 = it * RP_TA, j0 = jt * RP_TB;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int h = lane >> 5;
+  const int nch = args.nchunks;
+
+  // A side: thread t copies 16-byte segment t % 4 of rows t / 4 and 64 + t / 4 of both planes of every chunk
+  const char* const a_base[2] = {(const char*)args.a[0][0] + i0 * 64, (const char*)args.a[0][1] + i0 * 64};
+  const int64_t a_kstride = args.a_rows_pad * 64;
+  const uint32_t a_lane = tid * 16;
+  // B side: lane (c, h) of block bj reads row j0 + 64 wave + 32 bj + c, bytes [32 h, 32 h + 32) of each chunk
+  const char* const b_base[2] = {(const char*)args.b[0][0], (const char*)args.b[0][1]};
+  const int64_t b_kstride = args.b_rows_pad * 64;
+  uint32_t b_off[2];
+#pragma unroll
+  for (int bj = 0; bj < 2; ++bj) b_off[bj] = (uint32_t)((j0 + wave * 64 + bj * 32 + (lane & 31)) * 64 + h * 32);
+
+  struct ASet { u32x4 v[2][2]; };     // [plane][row half]
+  struct BSet { u32x4 v[2][2][2]; };  // [block][step][plane]
+  auto load_a = [&](int kc) {
+    ASet a;
+#pragma unroll
+    for (int pl = 0; pl < 2; ++pl) {
+      const char* sa = a_base[pl] + (int64_t)kc * a_kstride;
+      a.v[pl][0] = *gmem((const u32x4*)(sa + a_lane));
+      a.v[pl][1] = *gmem((const u32x4*)(sa + (a_lane + 4096)));
+    }
+    return a;
+  };
+  auto load_b = [&](int kc, BSet& b) {
+#pragma unroll
+    for (int pl = 0; pl < 2; ++pl) {
+      const char* sb = b_base[pl] + (int64_t)kc * b_kstride;
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int step = 0; step < 2; ++step) b.v[bj][step][pl] = *gmem((const u32x4*)(sb + (b_off[bj] + step * 16)));
+    }
+  };
+  auto store = [&](const ASet& a, char* st) {
+#pragma unroll
+    for (int pl = 0; pl < 2; ++pl)
+#pragma unroll
+      for (int hf = 0; hf < 2; ++hf) {
+        const int r = (tid >> 2) + hf * 64;
+        *(u32x4*)(st + pl * (RP_TA * 64) + r * 64 + (((tid & 3) ^ ((r >> 2) & 3)) * 16)) = a.v[pl][hf];
+      }
+  };
+
+  f32x16 acc[4][2];
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[bi][bj][v] = 0.f;
+
+  auto mma = [&](const char* st, const BSet& b) {
+#pragma unroll
+    for (int step = 0; step < 2; ++step) {
+      const int seg = h * 2 + step;
+      bf16x8 af[4][2];
+#pragma unroll
+      for (int bi = 0; bi < 4; ++bi) {
+        const int r = bi * 32 + (lane & 31);
+#pragma unroll
+        for (int sub = 0; sub < 2; ++sub)
+          af[bi][sub] = *(const bf16x8*)(st + sub * (RP_TA * 64) + r * 64 + ((seg ^ ((r >> 2) & 3)) * 16));
+      }
+#pragma unroll
+      for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+          const bf16x8 bh = __builtin_bit_cast(bf16x8, b.v[bj][step][0]);
+          const bf16x8 bl = __builtin_bit_cast(bf16x8, b.v[bj][step][1]);
+          f32x16 c = acc[bi][bj];
+          c = mfma_split<PREC>(af[bi][0], bh, c);
+          c = mfma_split<PREC>(af[bi][0], bl, c);
+          c = mfma_split<PREC>(af[bi][1], bh, c);
+          acc[bi][bj] = c;
+        }
+    }
+  };
+  auto chunk = [&](int kc, const char* cur, char* nxt, const BSet& bc, BSet& bn) {
+    const int kn = kc + 1 < nch ? kc + 1 : kc;  // past the last chunk the prefetch repeats it (valid addresses)
+    const ASet an = load_a(kn);
+    load_b(kn, bn);
+    mma(cur, bc);
+    store(an, nxt);  // nxt was last read in the previous chunk, which ended with a barrier
+    __syncthreads();
+  };
+
+  BSet b0, b1;
+  load_b(0, b0);
+  store(load_a(0), lds[0]);
+  __syncthreads();
+  int kc = 0;
+  for (; kc + 2 <= nch; kc += 2) {
+    chunk(kc, lds[0], lds[1], b0, b1);
+    chunk(kc + 1, lds[1], lds[0], b1, b0);
+  }
+  if (kc < nch) chunk(kc, lds[0], lds[1], b0, b1);
+
+  // epilogue: divide out the power-of-two row scales (split-FP16) and write the fp32 plane; lanes 0..31 of one store
+  // write 32 consecutive j of row i (128 contiguous bytes)
+  const int64_t rows_pad = args.a_rows_pad;
+  double inv_b[2] = {1.0, 1.0};
+  if constexpr (PREC == PREC_SPLIT_FP16) {
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) inv_b[bj] = 1.0 / (double)b_scale[j0 + wave * 64 + bj * 32 + (lane & 31)];
+  }
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int64_t i = i0 + bi * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+      const double inv_a = PREC == PREC_SPLIT_FP16 ? 1.0 / (double)a_scale[i] : 1.0;
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) {
+        const int64_t j = j0 + wave * 64 + bj * 32 + (lane & 31);
+        double x = (double)acc[bi][bj][v];
+        if constexpr (PREC == PREC_SPLIT_FP16) x *= inv_a * inv_b[bj];
+        out[((j >> 5) * rows_pad + i) * 32 + (j & 31)] = (float)x;
+      }
+    }
+}
+
 template <int PREC>
 hipError_t launch_rotate(int mode, const TileArgs<PREC, 1>& args, const float* a_scale, const float* b_scale,
                          void* o0, void* o1, void* o2, void* o3, float* out_scale1, float* out_scale2,
@@ -230,9 +377,19 @@ hipError_t launch_rotate(int mode, const TileArgs<PREC, 1>& args, const float* a
   const int64_t n_tiles = n_itiles * n_jtiles;
   const int lds = tile_lds_bytes<PREC, 1, false>();
   if constexpr (is_split(PREC)) {
-    // the 256 x 256 tile where both row counts allow it (FLMM_ROTATE_VARIANT=square keeps the 128 x 128 one)
+    // UY mode: the 128 x 256 tile at two workgroups per CU where the row counts allow it (a measured ~2.5% faster
+    // than the 256 x 256 one at N = 1,000 x 655k columns); FLMM_ROTATE_VARIANT=wide keeps the 256 x 256 tile,
+    // =square the 128 x 128 one (all three give the same bits)
     const char* var = getenv("FLMM_ROTATE_VARIANT");
     const bool square = var && strcmp(var, "square") == 0;
+    const bool wide = var && strcmp(var, "wide") == 0;
+    if (mode == MODE_UY && !square && !wide && args.a_rows_pad % RP_TA == 0 && args.b_rows_pad % RP_TB == 0) {
+      const int64_t nj = args.b_rows_pad / RP_TB;
+      const int64_t nt = (args.a_rows_pad / RP_TA) * nj;
+      hipLaunchKernelGGL((rotate_uy_pair_kernel<PREC>), dim3((unsigned)nt), dim3(256), 0, stream, args, nj, nt,
+                         a_scale, b_scale, (float*)o0);
+      return hipGetLastError();
+    }
     if (mode == MODE_UY && !square && args.a_rows_pad % RW_T == 0 && args.b_rows_pad % RW_T == 0) {
       const int64_t nj = args.b_rows_pad / RW_T;
       const int64_t nt = (args.a_rows_pad / RW_T) * nj;

@@ -391,9 +391,10 @@ def test_split_gram_matches_fp64(precision):
 
 @pytest.mark.parametrize("precision", ["split_fp16", "split_bf16"])
 def test_wide_uy_rotation_is_bit_identical(precision, monkeypatch):
-    """The 256 x 256, 8-wave column rotation (flmm_rotate UY mode where both row counts are multiples of 256) sums
-    each output's products in the 128 x 128 kernel's order: the same bits (FLMM_ROTATE_VARIANT=square selects the
-    latter), and the FP32 plane of the rotated columns matches numpy's FP64 product to the split precision."""
+    """The column rotation's 128 x 256 tile at two workgroups per CU (the default where the row counts allow) and the
+    256 x 256, 8-wave one (FLMM_ROTATE_VARIANT=wide) sum each output's products in the 128 x 128 kernel's order: the
+    same bits (FLMM_ROTATE_VARIANT=square selects the latter), and the FP32 plane of the rotated columns matches
+    numpy's FP64 product to the split precision."""
     from fasterlmm_amd import engine
     rng = np.random.default_rng(21)
     prec = engine.prec_code(precision)
@@ -404,12 +405,15 @@ def test_wide_uy_rotation_is_bit_identical(precision, monkeypatch):
         npad = -(-N // 32) * 32
         a = engine.pack(prec, Yd, P, N, 1, P, P, npad)
         b = engine.pack(prec, Ud, K, N, 1, K, K, npad)
-        monkeypatch.delenv("FLMM_ROTATE_VARIANT", raising=False)
+        monkeypatch.setenv("FLMM_ROTATE_VARIANT", "wide")
         wide = engine.rotate_uy(a, b)
         monkeypatch.setenv("FLMM_ROTATE_VARIANT", "square")
         square = engine.rotate_uy(a, b)
+        monkeypatch.delenv("FLMM_ROTATE_VARIANT", raising=False)
+        pair = engine.rotate_uy(a, b)  # the default
         torch.cuda.synchronize()
         assert torch.equal(wide.view(torch.int32), square.view(torch.int32)), (P, N, K)
+        assert torch.equal(pair.view(torch.int32), square.view(torch.int32)), (P, N, K)
         got = engine.fp32_plane_to_dense(wide, P, K, P, K).cpu().numpy()
         want = Y.T @ U
         tol = (3e-5 if precision == "split_bf16" else 2e-6) * np.sqrt((Y ** 2).sum(0).max())
