@@ -136,7 +136,7 @@ static double* scratch_ptr(hipStream_t stream, const char* who) {
 
 extern "C" {
 
-int flmm_abi_version(void) { return 17; }
+int flmm_abi_version(void) { return 18; }
 
 const char* flmm_last_error(void) { return g_err.c_str(); }
 
@@ -266,17 +266,18 @@ int flmm_delta_grid(const float* wplane, const float* lam, int n_grid, double nc
 
 int flmm_delta_grid_split(const float* wplane, const float* lam, int n_grid, double nco, const float* uy,
                           const float* col_scale, double s_min, double lo, double hi, int64_t P, int64_t p_pad,
-                          int64_t kpad, int32_t* idx, void* stream) {
+                          int64_t kpad, int32_t* idx, void* wsplit, void* stream) {
   REQUIRE(n_grid >= 1 && n_grid <= TILE, "flmm_delta_grid_split: n_grid must be in [1, 128]");
   REQUIRE(P >= 0 && P <= p_pad && p_pad % TILE == 0 && kpad % TILE == 0 && kpad > 0,
           "flmm_delta_grid_split: bad sizes");
   REQUIRE(nco > 0.0, "flmm_delta_grid_split: loss coefficient must be positive");
   REQUIRE(s_min >= 0.0, "flmm_delta_grid_split: s_min must be >= 0 (the FP32 grid takes other spectra)");
   REQUIRE(wplane && lam && uy && col_scale && idx, "flmm_delta_grid_split: null pointer");
-  REQUIRE(aligned16(wplane) && aligned16(uy), "flmm_delta_grid_split: planes must be 16-byte aligned");
+  REQUIRE(aligned16(wplane) && aligned16(uy) && aligned16(wsplit),
+          "flmm_delta_grid_split: planes and workspace must be 16-byte aligned");
   if (p_pad == 0) return FLMM_OK;
   return check_launch(launch_delta_grid_split(wplane, lam, n_grid, nco, uy, col_scale, s_min, lo, hi, P, p_pad, kpad,
-                                              idx, (hipStream_t)stream),
+                                              idx, wsplit, (hipStream_t)stream),
                       "flmm_delta_grid_split");
 }
 

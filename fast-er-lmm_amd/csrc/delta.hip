@@ -130,10 +130,25 @@ __global__ __launch_bounds__(NTHREADS, 1) void delta_grid_kernel(TileArgs<PREC_F
 // accumulation give the weighted sums to ~2^-21 relative, as FP32 products with FP32 accumulation do; sb_p adds
 // the same constant to every grid loss of column p (no change to its argmin) and sa_g is taken out through lam.
 // 128 grid rows x 128 columns per workgroup, 4 waves of 64 x 64, chunks double-buffered in LDS.
+// The grid weights' scaled split (hi, lo) planes, once per call (PRE): element (kc, r, kk) of the FP32 W plane at
+// ((kc 128 + r) 32 + kk) -> the same index of the hi plane and of the lo plane (128 kpad values after it).
+__global__ __launch_bounds__(256) void delta_wsplit_kernel(const float* __restrict__ wplane, double s_min, int G,
+                                                           double lo, double hi, int64_t kpad, u16* __restrict__ wsp) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= TILE * kpad) return;
+  const int r = (int)((e >> 5) % TILE);
+  const float sc = (float)pow2_scale(1.0 / (s_min + exp(grid_value(r, G, lo, hi))));
+  u16 h, l;
+  split2f<PREC_SPLIT_FP16>(wplane[e] * sc, h, l);
+  wsp[e] = h;
+  wsp[TILE * kpad + e] = l;
+}
+
+template <bool PRE>
 __global__ __launch_bounds__(NTHREADS, 2) void delta_grid_split_kernel(
-    const float* __restrict__ wplane, const float* __restrict__ uy, const float* __restrict__ col_scale, double s_min,
-    int G, double lo, double hi, const float* __restrict__ lam, float nco, int64_t P, int64_t p_pad, int64_t kpad,
-    int64_t n_tiles, int* __restrict__ idx_out) {
+    const float* __restrict__ wplane, const u16* __restrict__ wsp, const float* __restrict__ uy,
+    const float* __restrict__ col_scale, double s_min, int G, double lo, double hi, const float* __restrict__ lam,
+    float nco, int64_t P, int64_t p_pad, int64_t kpad, int64_t n_tiles, int* __restrict__ idx_out) {
   constexpr int PLANE = TILE * 64;  // one fp16 plane of a chunk: 128 rows x 32 values
   constexpr int STAGE = 4 * PLANE;  // A hi, A lo, B hi, B lo
   __shared__ __attribute__((aligned(16))) char lds[2 * STAGE];
@@ -162,10 +177,16 @@ __global__ __launch_bounds__(NTHREADS, 2) void delta_grid_split_kernel(
 #pragma unroll
     for (int ps = 0; ps < 2; ++ps) {
       const int r = (tid >> 2) + 64 * ps;
-      const float* a = wplane + ((int64_t)kc * TILE + r) * 32 + sg * 8;
       const float* b = uy + ((int64_t)kc * p_pad + p0 + r) * 32 + sg * 8;
-      f[0][ps][0] = *gmem((const f32x4*)a);
-      f[0][ps][1] = *gmem((const f32x4*)(a + 4));
+      if constexpr (PRE) {  // the split planes: 8 hi values, then the same 8 lo values
+        const u16* a = wsp + ((int64_t)kc * TILE + r) * 32 + sg * 8;
+        f[0][ps][0] = __builtin_bit_cast(f32x4, *gmem((const u32x4*)a));
+        f[0][ps][1] = __builtin_bit_cast(f32x4, *gmem((const u32x4*)(a + TILE * kpad)));
+      } else {
+        const float* a = wplane + ((int64_t)kc * TILE + r) * 32 + sg * 8;
+        f[0][ps][0] = *gmem((const f32x4*)a);
+        f[0][ps][1] = *gmem((const f32x4*)(a + 4));
+      }
       f[1][ps][0] = *gmem((const f32x4*)b);
       f[1][ps][1] = *gmem((const f32x4*)(b + 4));
     }
@@ -176,6 +197,12 @@ __global__ __launch_bounds__(NTHREADS, 2) void delta_grid_split_kernel(
 #pragma unroll
       for (int ps = 0; ps < 2; ++ps) {
         const int r = (tid >> 2) + 64 * ps;
+        if (PRE && side == 0) {  // already scaled and split
+          const int off = r * 64 + ((sg ^ ((r >> 2) & 3)) * 16);
+          *(u32x4*)(st + off) = __builtin_bit_cast(u32x4, f[0][ps][0]);
+          *(u32x4*)(st + PLANE + off) = __builtin_bit_cast(u32x4, f[0][ps][1]);
+          continue;
+        }
         u16 hv[8], lv[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -1100,10 +1127,19 @@ hipError_t launch_delta_grid(const float* wplane, const float* lam, int G, doubl
 
 hipError_t launch_delta_grid_split(const float* wplane, const float* lam, int G, double nco, const float* uy,
                                    const float* col_scale, double s_min, double lo, double hi, int64_t P, int64_t p_pad,
-                                   int64_t kpad, int* idx, hipStream_t stream) {
+                                   int64_t kpad, int* idx, void* wsplit, hipStream_t stream) {
   const int64_t n_tiles = p_pad / TILE;
-  hipLaunchKernelGGL(delta_grid_split_kernel, dim3((unsigned)n_tiles), dim3(NTHREADS), 0, stream, wplane, uy,
-                     col_scale, s_min, G, lo, hi, lam, (float)nco, P, p_pad, kpad, n_tiles, idx);
+  if (wsplit) {
+    hipLaunchKernelGGL(delta_wsplit_kernel, dim3((unsigned)((TILE * kpad + 255) / 256)), dim3(256), 0, stream, wplane,
+                       s_min, G, lo, hi, kpad, (u16*)wsplit);
+    hipLaunchKernelGGL(delta_grid_split_kernel<true>, dim3((unsigned)n_tiles), dim3(NTHREADS), 0, stream, wplane,
+                       (const u16*)wsplit, uy, col_scale, s_min, G, lo, hi, lam, (float)nco, P, p_pad, kpad, n_tiles,
+                       idx);
+  } else {
+    hipLaunchKernelGGL(delta_grid_split_kernel<false>, dim3((unsigned)n_tiles), dim3(NTHREADS), 0, stream, wplane,
+                       (const u16*)nullptr, uy, col_scale, s_min, G, lo, hi, lam, (float)nco, P, p_pad, kpad, n_tiles,
+                       idx);
+  }
   return hipGetLastError();
 }
 

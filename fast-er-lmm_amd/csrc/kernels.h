@@ -38,7 +38,7 @@ hipError_t launch_delta_weights(const double* s, int64_t neff, const double* s_l
                                 double lo, double hi, float* wplane, float* lam, double* lam64, hipStream_t stream);
 hipError_t launch_delta_grid_split(const float* wplane, const float* lam, int G, double nco, const float* uy,
                                    const float* col_scale, double s_min, double lo, double hi, int64_t P, int64_t p_pad,
-                                   int64_t kpad, int* idx, hipStream_t stream);
+                                   int64_t kpad, int* idx, void* wsplit, hipStream_t stream);
 hipError_t launch_delta_grid(const float* wplane, const float* lam, int G, double nco, const float* uy, int64_t P,
                              int64_t p_pad, int64_t kpad, int* idx, hipStream_t stream);
 hipError_t launch_delta_refine(int prec, int mode, const float* uy, int64_t P, int64_t p_pad, int64_t kpad,

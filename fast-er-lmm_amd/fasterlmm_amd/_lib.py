@@ -59,7 +59,7 @@ SIGNATURES: dict[str, list] = {
     "flmm_rotate": [_I, _I, _P, _P, _P, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P],
     "flmm_delta_weights": [_P, _I64, _P, _I64, _I64, _I, _D, _D, _P, _P, _P, _P],
     "flmm_delta_grid": [_P, _P, _I, _D, _P, _I64, _I64, _I64, _P, _P],
-    "flmm_delta_grid_split": [_P, _P, _I, _D, _P, _P, _D, _D, _D, _I64, _I64, _I64, _P, _P],
+    "flmm_delta_grid_split": [_P, _P, _I, _D, _P, _P, _D, _D, _D, _I64, _I64, _I64, _P, _P, _P],
     "flmm_delta_refine": [_I, _I, _P, _I64, _I64, _I64, _P, _I64, _P, _I64, _D, _I, _D, _D, _P, _P, _P, _P, _P,
                           _P, _P, _P, _P, _P, _P, _I64, _I64, _D, _D, _I, _P, _P],
     "flmm_gram": [_I, _P, _P, _P, _I64, _I64, _I64, _I, _P, _P, _I64, _P],

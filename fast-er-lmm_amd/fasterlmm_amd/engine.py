@@ -380,8 +380,9 @@ def delta_grid(uy: torch.Tensor, P: int, p_pad: int, kpad: int, s: torch.Tensor,
     if settings.grid not in ("auto", "fp32"):
         raise ValueError(f"unknown grid setting {settings.grid!r}")
     if col_scale is not None and settings.grid == "auto" and s_min >= 0.0:
+        wsplit = torch.empty(4 * TILE * kpad, dtype=torch.uint8, device=dev)  # the grid rows' split planes
         _lib.call("flmm_delta_grid_split", wplane.data_ptr(), lam.data_ptr(), n_grid, float(nco), uy.data_ptr(),
-                  col_scale.data_ptr(), float(s_min), lo, hi, P, p_pad, kpad, idx.data_ptr(), st)
+                  col_scale.data_ptr(), float(s_min), lo, hi, P, p_pad, kpad, idx.data_ptr(), wsplit.data_ptr(), st)
     else:
         _lib.call("flmm_delta_grid", wplane.data_ptr(), lam.data_ptr(), n_grid, float(nco), uy.data_ptr(), P, p_pad,
                   kpad, idx.data_ptr(), st)

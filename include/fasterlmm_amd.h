@@ -171,10 +171,12 @@ int flmm_delta_grid(const float* wplane, const float* lam, int n_grid, double nc
  * plane, both operands scaled and split into fp16 (hi, lo) planes as they are staged (three products, FP32
  * accumulation, ~2^-21 relative: the accuracy of the FP32 product).  Needs a power-of-two col_scale[p] with
  * col_scale[p] * |column p| <= 2^14 (the rotated columns' plane scale: an orthonormal basis keeps the norm) and
- * 0 <= s_min <= every s_k (grid row g is scaled by the power of two of 1 / (s_min + delta_g)). */
+ * 0 <= s_min <= every s_k (grid row g is scaled by the power of two of 1 / (s_min + delta_g)).  wsplit (may be
+ * NULL): 4 * 128 * kpad bytes of device workspace, where the grid rows' split planes are formed once per call
+ * instead of in every column tile (the same values, less VALU work per tile). */
 int flmm_delta_grid_split(const float* wplane, const float* lam, int n_grid, double nco, const float* uy,
                           const float* col_scale, double s_min, double lo, double hi, int64_t P, int64_t p_pad,
-                          int64_t kpad, int32_t* idx, void* stream);
+                          int64_t kpad, int32_t* idx, void* wsplit, void* stream);
 
 /* Refine log delta in [grid[i-1], grid[i+1]] (core.py:575-595), i = the FP64 grid minimum reached from the FP32 grid
  * argmin idx (the reference brackets the FP64 argmin; FP32 ties on flat likelihoods are resolved in FP64, each

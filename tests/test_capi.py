@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_error_channel():
     from fasterlmm_amd import _lib
     lib = _lib.load()
-    assert lib.flmm_abi_version() == 17
+    assert lib.flmm_abi_version() == 18
     # argument validation runs on the host and reports through flmm_last_error without touching a GPU
     rc = lib.flmm_max_reduce(None, 1, 0, 4, None, None)
     assert rc == 1

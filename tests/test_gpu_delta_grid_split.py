@@ -75,6 +75,31 @@ def test_split_grid_picks_the_fp64_minimum(panel):
     assert gap_f.max() <= 1.0
 
 
+def test_split_grid_pre_split_weights_same_picks(panel):
+    """flmm_delta_grid_split with the grid rows' split planes formed once in a workspace (the engine's call) and
+    formed in every column tile (no workspace): the same picks, bit for bit."""
+    from fasterlmm_amd import _lib, core, engine
+    pan, Zs, Y = panel
+    on = pan.chrom == 1
+    spec = core.fastlmm_compat_rotate(T(lmm.grm(Zs[:, ~on])), T(pan.X), T(Y))
+    st = spec._gpu
+    s_lam, nco = st._loss(True)
+    idx_ws = engine.delta_grid(st.uy, st.P, st.p_pad, st.kpad, st.s, s_lam, nco, 64, -10.0, 10.0,
+                               col_scale=st.col_scale, s_min=st.s_range[0])[0]
+    wplane = torch.empty(128 * st.kpad, dtype=torch.float32, device=DEV)
+    lam = torch.empty(128, dtype=torch.float32, device=DEV)
+    lam64 = torch.empty(128, dtype=torch.float64, device=DEV)
+    idx = torch.empty(st.P, dtype=torch.int32, device=DEV)
+    stream = _lib.stream()
+    _lib.call("flmm_delta_weights", st.s.data_ptr(), st.s.numel(), s_lam.data_ptr(), s_lam.numel(), st.kpad, 64,
+              -10.0, 10.0, wplane.data_ptr(), lam.data_ptr(), lam64.data_ptr(), stream)
+    _lib.call("flmm_delta_grid_split", wplane.data_ptr(), lam.data_ptr(), 64, float(nco), st.uy.data_ptr(),
+              st.col_scale.data_ptr(), float(st.s_range[0]), -10.0, 10.0, st.P, st.p_pad, st.kpad, idx.data_ptr(),
+              None, stream)
+    torch.cuda.synchronize()
+    assert torch.equal(idx, idx_ws)
+
+
 def test_split_grid_loco_matches_fp32_grid_and_oracle(panel):
     from fasterlmm_amd import core, engine
     pan, Zs, Y = panel
