@@ -528,7 +528,20 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
     all_rows = torch.from_numpy(np.concatenate(rows_of) if rows_of else np.zeros(0, np.int64)).to(dev)
     row_start = np.concatenate([[0], np.cumsum([r.size for r in rows_of])])
     with stage("kinship"):
-        K_all = engine.gram(Zk, prec)
+        # each kinship chromosome's Gram once; K_all = their sum, in chromosome order (the Gram of all markers to
+        # FP64 rounding), and K_loco(c) = (K_all - G_c) / (M - m_c) below
+        grams = {}
+        for c in sorted(np.unique(ck).tolist()):
+            cols = np.flatnonzero(ck == c)
+            if cols[-1] - cols[0] + 1 == cols.size:  # consecutive columns: a view, no gather
+                Zc = Zk[:, int(cols[0]):int(cols[-1]) + 1]
+            else:
+                Zc = Zk.index_select(1, torch.from_numpy(cols).to(dev))
+            grams[c] = (engine.gram(Zc, prec), cols.size)
+            del Zc
+        K_all = None
+        for G, _ in grams.values():
+            K_all = G.clone() if K_all is None else K_all.add_(G)
     Xpinv, Iperp = _projector(Xd)
     # the basis is orthogonal to X, so B^T (Y - X X^+ Y) = B^T Y; the OLS regress-out keeps the compat
     # path's arithmetic (core.py:517) and costs one pass
@@ -549,18 +562,15 @@ def _loco(Z, X, Y, chrom, n_real, on_chrom, Z_kin, chrom_kin, n_grid, *, plain: 
     Kps, Ks = [], []
     with stage("kinship"):
         for c in chroms:
-            cols = np.flatnonzero(ck == c)
-            if cols.size and cols[-1] - cols[0] + 1 == cols.size:  # consecutive columns: a view, no gather
-                Zc = Zk[:, int(cols[0]):int(cols[-1]) + 1]
-            else:
-                Zc = Zk.index_select(1, torch.from_numpy(cols).to(dev))
-            K = (K_all - engine.gram(Zc, prec)) / float(Zk.shape[1] - cols.size)
-            del Zc
+            G, m_c = grams.pop(c, (None, 0))
+            K = ((K_all - G) if G is not None else K_all) / float(Zk.shape[1] - m_c)
+            del G
             if jitter:
                 K = K + jitter * eye
             Kps.append(_project(K, Xd, Xpinv, Iperp))
             if plain and not reml:
                 Ks.append(K)
+        del grams, K_all
     with stage("eigh"):
         ev, Ub = _eigh(torch.stack(Kps))
         del Kps
