@@ -845,7 +845,8 @@ __global__ __launch_bounds__(256) void screen_resolve_kernel(
     // (an x that overflowed FP32 counts as forced)
     auto forced = [](uint64_t v) { return ((v >> 32) & 1u) || ((uint32_t)v & ~127u) >= 0x7f800000u; };
     float xm = 0.f;
-    for (int64_t t = 0; t < n_mt; ++t) {
+#pragma unroll 8
+    for (int64_t t = 0; t < n_mt; ++t) {  // (unrolled: eight record loads in flight per thread)
       const uint64_t v = gmem(rec)[t * rec_ld + i];
       if (!forced(v)) xm = fmaxf(xm, __uint_as_float((uint32_t)v & ~127u));  // a lower bound of its x
     }
@@ -853,6 +854,7 @@ __global__ __launch_bounds__(256) void screen_resolve_kernel(
     const double L = fmax(F0 > 0.0 ? sqrt(F0 / (nu + F0)) : 0.0, sqrt((double)xm * cc) * dn - ka);
     const double th = (L - ur) / up;  // cU(x) >= L  <=>  sqrt(x cc) >= th
     const double X = th > 0.0 ? th * th / cc : -1.0;
+#pragma unroll 8
     for (int64_t t = 0; t < n_mt; ++t) {
       const uint64_t v = gmem(rec)[t * rec_ld + i];
       const uint32_t hw = (uint32_t)(v >> 32);
