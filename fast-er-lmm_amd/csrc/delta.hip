@@ -1058,6 +1058,12 @@ __global__ __launch_bounds__(256) void factored_planes_kernel(int64_t P, int64_t
   const bool live = p < P;
   const double delta = live ? exp(log_delta[p]) : 0.0;
   auto val = [&](int64_t r) { return (live && r < R) ? lam[r] * exp(-t[r] * delta) : 0.0; };
+  // with the bucketed layout the screened columns' sWs comes from the FP64 node sums: no planes are written for them
+  // and their scale is 1 (the screen's records and the resolution both read it, so any power of two agrees)
+  if (is_split(PREC) && live && p >= hi_from && layout) {
+    if (PREC == PREC_SPLIT_FP16 && j == 0) qs[p] = 1.f;
+    return;
+  }
   double sc = 1.0;
   if constexpr (PREC == PREC_SPLIT_FP16) {
     double ss = 0.0;
@@ -1071,9 +1077,6 @@ __global__ __launch_bounds__(256) void factored_planes_kernel(int64_t P, int64_t
     if (j == 0) qs[p] = (float)sc;
   }
   const int64_t nch = rpad / 32;
-  // with the bucketed layout the screened columns' sWs comes from the FP64 node sums: their planes are not written
-  // (their scales are)
-  if (is_split(PREC) && live && p >= hi_from && layout) return;
   for (int64_t c = 0; c < nch; ++c) {
     const int64_t r = c * 32 + j;
     const int64_t off = (c * p_pad + p) * 32 + j;

@@ -220,7 +220,8 @@ int64_t flmm_screen_q_rows(int64_t p_pad, int64_t hi_from, int64_t P);
  * exp(log_delta[p]), p < P) as the Q2 planes (rows p_pad, K = rpad, zero for r >= R and p >= P), split-FP16 rows
  * scaled by the power of two of their L2 norm (q2_scale); columns hi_from <= p < P get the hi plane only (as
  * flmm_delta_refine), or, with a bucketed layout order (the one flmm_delta_refine wrote; may be NULL), their scale
- * alone (the bucketed screen reads no Q2 plane).  p_pad: the Q planes' rows (flmm_delta_refine's q_rows). */
+ * alone, set to 1 (the bucketed screen reads no Q2 plane; its records and flmm_wald_scan_screened's resolution read
+ * the same scale, so any power of two agrees).  p_pad: the Q planes' rows (flmm_delta_refine's q_rows). */
 int flmm_factored_weight_planes(int prec, int64_t P, int64_t p_pad, int64_t rpad, const double* t, const double* lam,
                                 int64_t R, const double* log_delta, void* q2_hi, void* q2_lo, float* q2_scale,
                                 int64_t hi_from, const void* order, void* stream);
